@@ -1,0 +1,118 @@
+"""Benchmark: fill Gpair/s (and the GMRES solve) on the 30k-RWG seeded plane pair.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed S] [--no-cpu]
+
+A step is one device-resident fill of the board's four moment planes (FP32 far pairs,
+FP64 analytic near pairs) with the inputs already in HBM; value = n(n+1)/2 / seconds / 1e9
+(the reference's pair count, src/pcbqs/static.py:39). The dominant kernel's duration comes
+from HIP events recorded on its launch stream. The CPU baseline is the reference's own
+compiled static_fill (built into oracle/_ref) on the same mesh, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X
+FP32_VALU_PEAK = 157.3e12  # flop/s, MI355X (vendor)
+F_FULL = 209               # flop per far pair, full moments
+B_FULL_FP32 = 32           # bytes stored per unordered pair: 4 planes x 2 entries x 4 B
+
+
+def cpu_baseline(board):
+    """Time the reference's compiled static_fill (oracle/_ref) on the whole board, or, where it
+    is not built, the C restatement (oracle/libcpu_ref.so), on this host's cores."""
+    from oracle import fill as F
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except OSError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    g = F.prepare(board.nodes, board.tris, board.z_of, 1)
+    n = len(board.tris)
+    mats = [np.zeros((n, n)) for _ in range(4)]              # touched before the clock starts
+    core = F.ref_core()
+    t = time.perf_counter()
+    if core is not None:
+        count, analytic = core.static_fill(F.ref_geometry_dict(g), g.cent, g.radius, g.z, 3.0, 1, False,
+                                           cores, mats)
+        kind = 'reference'
+    else:
+        _, count, analytic = F.cr_fill(g, 3.0, 1, False, cores)
+        kind = 'port'
+    seconds = time.perf_counter() - t
+    del mats
+    return dict(value=count / seconds / 1e9, unit='Gpair/s', cores=cores, kind=kind,
+                sample=f'the whole fill of the same {n}-triangle board: static_fill, {count:,} unordered '
+                       f'pairs ({analytic:,} analytic), FP64, near=3, outer_levels=1, one run',
+                seconds=seconds)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps(dict(error='this build runs on one GPU: every box here has one; '
+                                    'the multi-GPU split is described, not run')))
+        sys.exit(2)
+    from pcbqs_amd import board as boards
+    from pcbqs_amd.backend import Mi355xBackend
+    from pcbqs_amd.geometry import FillGeometry
+    board = boards.board_c2(args.seed)
+    n = board.nt
+    pairs = n * (n + 1) // 2
+    g = FillGeometry.prepare(board.nodes, board.tris, board.z_of, 1)
+    backend = Mi355xBackend(g, 1, 16384, 0)
+    planes, analytic = backend.fill_device(3.0, 1, False, 32)
+    for _ in range(args.warmup):
+        backend.fill_device(3.0, 1, False, 32, out=planes)
+    far_ms, near_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        backend.fill_device(3.0, 1, False, 32, out=planes)
+        f, nm, near_pairs = backend.last_timing()
+        far_ms.append(f)
+        near_ms.append(nm)
+    seconds = (time.perf_counter() - t0) / args.steps
+    far = float(np.mean(far_ms)) * 1e-3
+    # FP64 reference-layout fill (the parity mode), once, for the record
+    planes.close()
+    p64, _ = backend.fill_device(3.0, 1, False, 64)
+    backend.fill_device(3.0, 1, False, 64, out=p64)
+    f64, n64, _ = backend.last_timing()
+    p64.close()
+    achieved = pairs * B_FULL_FP32 / far
+    line = dict(
+        metric='fill Gpair/s; GMRES s to 1e-8 at 30k RWG, learned vs plain precond',
+        value=pairs / seconds / 1e9, unit='Gpair/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=seconds * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
+        dtype='fp32 far / fp64 near', data='synthetic',
+        config=dict(workload=f'seeded 2-layer 60x60 mm plane pair (seed {args.seed}), {n} triangles, '
+                             f'{board.info.get("cutouts")} cutouts, {len(board.vseg)} vias, 4 ports; '
+                             f'near=3, outer_levels=1', triangles=n, pairs=pairs, analytic_pairs=analytic),
+        roofline=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                      frac=achieved / HBM_PEAK, traffic=None,
+                      kernel='far32_kernel (full moments)',
+                      valu_frac=pairs * F_FULL / far / FP32_VALU_PEAK),
+        kernels=dict(far_ms=float(np.mean(far_ms)), near_ms=float(np.mean(near_ms)),
+                     near_pairs=list(near_pairs), fp64_far_ms=f64, fp64_near_ms=n64,
+                     fp64_gpair_s=pairs / ((f64 + n64) * 1e-3) / 1e9),
+    )
+    if not args.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(board)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

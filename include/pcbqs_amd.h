@@ -1,0 +1,132 @@
+/*
+ * pcbqs_amd.h -- C ABI of libpcbqs_hip.so, the MI355X (gfx950) static-fill and
+ * loop-solve library behind the `mi355x` pcbqs.static_backends plugin.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero pq_status otherwise;
+ *     nothing throws across the ABI. pq_last_error() gives the message of the
+ *     calling thread's last failure.
+ *   - Host pointers are plain C arrays, row-major. Device pointers (named *_dev)
+ *     are addresses returned by pq_malloc (or any HIP allocation on the same
+ *     device). `stream` is a hipStream_t (NULL: the library's own stream).
+ *   - A handle is not thread-safe; calls on one handle must not overlap.
+ *
+ * The reference's boundary for this path is a Python protocol (a factory
+ * returning an object with evaluate / fill_symmetric / parallel_tiles); each
+ * entry point below cites the reference interface it replaces.
+ */
+#ifndef PCBQS_AMD_H
+#define PCBQS_AMD_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pq_status {
+    PQ_OK = 0,
+    PQ_EINVAL = 1,      /* invalid argument (ValueError in Python)                  */
+    PQ_ENONFINITE = 2,  /* non-finite moments (FloatingPointError)                  */
+    PQ_EDEVICE = 3,     /* HIP runtime / device failure (RuntimeError)              */
+    PQ_ENOMEM = 4,      /* device allocation failed (MemoryError)                   */
+    PQ_ECONVERGE = 5    /* iterative solve did not converge (ConvergenceError)      */
+};
+
+/* Read-only triangle geometry (float64, C order), the arrays of a prepared
+ * FillGeometry: src/pcbqs/static_backends.py:105-130 (prepare), the level
+ * tuples of src/pcbqs/_numerics/sing.py:56-69, and the dict the reference core
+ * reads in src/pcbqs/core/static_kernels.hpp:16-43 (static_geometry). */
+typedef struct pq_geometry {
+    int32_t nt;            /* triangles                                         */
+    const double *V;       /* (nt,3,2) vertices, centred                        */
+    const double *L;       /* (nt,3)   edge lengths                             */
+    const double *tau;     /* (nt,3,2) unit edge tangents                       */
+    const double *un;      /* (nt,3,2) outward edge normals                     */
+    const double *area;    /* (nt)                                              */
+    const double *P3;      /* (nt,3,2) far-rule points                          */
+    const double *qp[3];   /* (nt,nq[k],2) observation points of level k        */
+    const double *qw[3];   /* (nq[k]) observation weights of level k            */
+    int32_t nq[3];         /* 6 * 4^k (a level not prepared repeats level 0)    */
+    const double *z;       /* (nt) sheet height of each triangle                */
+    const double *cent;    /* (nt,2) centroids                                  */
+    const double *radius;  /* (nt) bounding radii                               */
+} pq_geometry;
+
+typedef struct pq_fill pq_fill;   /* device copy of one geometry + scratch      */
+
+/* ---- runtime ------------------------------------------------------------ */
+const char *pq_last_error(void);
+int pq_device_count(int *count);
+int pq_malloc(int device, size_t bytes, void **ptr_dev);
+int pq_free(void *ptr_dev);
+int pq_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int pq_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int pq_memset(void *dst_dev, int value, size_t bytes);
+int pq_synchronize(int device);
+
+/* ---- fill --------------------------------------------------------------- */
+
+/* Upload one geometry to `device` once (geometry stays read-only).
+ * Replaces CupyBackend.__init__ (src/pcbqs/static_cupy.py:73-107) / the
+ * factory make_backend(geometry, workers, chunk, device)
+ * (docs/static-backends.md:157-193). */
+int pq_fill_create(const pq_geometry *g, int device, pq_fill **out);
+int pq_fill_destroy(pq_fill *f);
+
+/* Whole reciprocal FP64 fill into caller-owned host (nt,nt) matrices
+ * planes[0..3] = M0, Mrx, Mry, Mrr (planes[0] only when scalar_only), with
+ * near = INFINITY for the all-analytic rule; writes the unordered analytic
+ * pair count. Replaces CupyBackend.fill_symmetric
+ * (src/pcbqs/static_cupy.py:127-197), called from static_moments
+ * (src/pcbqs/static.py:35-42); same result as static_fill
+ * (src/pcbqs/core/static_kernels.hpp:184-252). PQ_ENONFINITE when any moment
+ * is not finite. */
+int pq_fill_symmetric(pq_fill *f, double near, int outer_levels, int scalar_only,
+                      double *const planes[4], int64_t *analytic);
+
+/* Explicit pair list: p, q int32, d float64, kind int32 in {-1, 0, 1, 2}
+ * (levels must be prepared); out float64 (n,6) [M0, Mro_x, Mro_y, Mrs_x,
+ * Mrs_y, Mrr] or (n,1) when scalar_only, host memory, synchronized.
+ * Replaces CupyBackend.evaluate (src/pcbqs/static_cupy.py:108-125), the
+ * backend side of FillEngine.evaluate (src/pcbqs/static_backends.py:300-336);
+ * same result as static_pairs (src/pcbqs/core/static_kernels.hpp:147-172). */
+int pq_pairs(pq_fill *f, const int32_t *p, const int32_t *q, const double *d,
+             const int32_t *kind, int64_t n, int scalar_only, double *out);
+
+/* Device-resident fill: planes_dev holds 1 or 4 planes of nt*nt elements,
+ * plane k at planes_dev + k*nt*nt, row-major. precision 64: reference layout
+ * (moments about the centred origin, FP64 throughout). precision 32: FP32
+ * planes, far pairs in FP32, near pairs in FP64, first moments taken about each
+ * row/column triangle's own centroid ("local frame": Mrx[r,c] = sum g (x_r -
+ * c_r), Mrr[r,c] = sum g (x_r - c_r).(x_c - c_c)) so that the RWG combination
+ * does not cancel. Same rule and pair set as static_fill
+ * (src/pcbqs/core/static_kernels.hpp:184-252); asynchronous on `stream`
+ * except that it returns the analytic count (one small copy). */
+int pq_fill_device(pq_fill *f, double near, int outer_levels, int scalar_only, int precision,
+                   void *planes_dev, int64_t *analytic, void *stream);
+
+/* Number of pair-kernel launches and timings of the last pq_fill_device call:
+ * far kernel milliseconds, near kernel milliseconds (HIP events on the launch
+ * stream), near pairs per level. Not in the reference (measurement only). */
+int pq_fill_last_timing(pq_fill *f, double *far_ms, double *near_ms, int64_t near_pairs[2]);
+
+/* ---- RWG combination ---------------------------------------------------- */
+
+/* L[m,n] = factor s_m s_n sum_{i,j} a_mi a_nj [Mrr(p,q) - v_nj.Mr(p,q)
+ *          - v_mi.Mr(q,p) + v_mi.v_nj M0(p,q)], p = tri[i][m], q = tri[j][n],
+ * written symmetrically into L_dev (ne,ne). planes_dev as produced by
+ * pq_fill_device with the same precision (32: local frame; cent_dev (nt,2)
+ * float64 centroids are then required). tri_dev (2,ne) int32, alpha_dev (2,ne),
+ * vertex_dev (2,ne,2), scale_dev (ne) float64. out_precision 32 or 64.
+ * Replaces static_combine (src/pcbqs/core/static_kernels.hpp:257-310) as
+ * called by edge_inductance (src/pcbqs/static.py:103-128). */
+int pq_combine_device(int precision, const void *planes_dev, int64_t nt, const double *cent_dev,
+                      const int32_t *tri_dev, const double *alpha_dev, const double *vertex_dev,
+                      const double *scale_dev, int64_t ne, double factor, int out_precision,
+                      void *L_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,15 @@
+"""TEST INFRASTRUCTURE ONLY: the CPU oracle for the static fill and the loop/port solve.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+package, and only as the checker (never as the thing measured or shipped).
+
+  oracle.fill   FillGeometry.prepare / kinds / far and analytic pair rules in numpy,
+                plus ctypes access to libcpu_ref.so (cpu_ref.c, the C restatement of
+                the reference's static_fill / static_pairs / static_combine) and to the
+                reference's own compiled core built into oracle/_ref (when present).
+  oracle.mqs    RWG basis, sheet resistance, via R/L, loop basis, dense port solve.
+  oracle.biot   Biot-Savart of RWG sheet currents and of straight filaments.
+  oracle.gmres  restarted and lockstep GMRES, and the graph-Laplacian preconditioner.
+
+Parity status of each function is stated in its docstring (what pins it).
+"""

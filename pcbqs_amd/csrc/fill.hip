@@ -1,0 +1,739 @@
+// Triangle-pair 1/R moment fill for gfx950.
+//
+// Semantics follow src/pcbqs/core/static_kernels.hpp:53-252 (static_pair, pair_kind,
+// static_fill) and the device row-strip contract of src/pcbqs/static_cupy.py:127-197.
+//
+// Work split:
+//   far kernel   one wave per matrix row, the 64 lanes over 64 consecutive columns, so every
+//                plane store is one coalesced 256 B (FP32) / 512 B (FP64) row segment; the row
+//                triangle's data is wave-uniform (scalar loads), the column triangle's is per
+//                lane. Each entry is computed from its canonical pair (min, max), which makes
+//                reciprocal entries bitwise equal. Pairs the near rule keeps analytic are not
+//                integrated here: they are appended (one atomic per wave) to per-level lists.
+//   near kernel  FP64 closed-form source integrals, one thread per (pair, observation point);
+//                each pair's observation sum is then taken sequentially in point order, the
+//                reference's order.
+// Classification is the reference rule evaluated in FP64 without contraction; an FP32
+// prefilter with a 1e-5 relative margin settles the clearly-far pairs without it.
+#include "pq_common.h"
+#include <cmath>
+#include <vector>
+#include <algorithm>
+
+namespace pq {
+
+constexpr int WAVE = 64;
+constexpr int FAR_ROWS_PER_WAVE = 4;
+constexpr int FAR_ROWS_PER_BLOCK = 4 * FAR_ROWS_PER_WAVE;  // 4 waves per 256-thread block
+
+// Device view of one prepared geometry.
+struct GeomDev {
+    int nt;
+    const double *V, *L, *tau, *un, *area, *P3, *z, *cent, *radius;
+    const double *qp[3];
+    const double *qw[3];
+    int nq[3];
+    // FP32 far records, 2 float4 per triangle: (x0,y0,x1,y1), (x2,y2,area,radius) with the far
+    // points taken relative to the triangle's own centroid.
+    const float4 *far32;
+};
+
+struct NearLists {
+    unsigned long long *list[2];   // packed (row << 32) | col entries
+    unsigned long long *count;     // [0],[1]: list lengths; [2]: analytic pairs with row <= col
+    unsigned long long cap;
+};
+
+// ---- classification ----------------------------------------------------------------------
+
+// The reference rule (static_kernels.hpp:130-136): every product rounded, then summed.
+__device__ __noinline__ int exact_kind(const double *cent, const double *radius, const double *z, int a,
+                                       int b, double near, int all_near, int outer) {
+#pragma clang fp contract(off)
+    const double dx = cent[2 * a] - cent[2 * b], dy = cent[2 * a + 1] - cent[2 * b + 1];
+    const double d = z[a] - z[b];
+    const double dist = __builtin_sqrt((dx * dx + dy * dy) + d * d);
+    const double size = radius[a] + radius[b];
+    if (!all_near && !(dist <= near * size)) return -1;
+    return dist <= size ? outer : 0;
+}
+
+// ---- per-pair moment functions -----------------------------------------------------------
+
+// FP64 far 3x3 product rule, reference arithmetic order (static_kernels.hpp:57-72).
+template <bool SCALAR>
+__device__ __forceinline__ void far64(const double *pa, const double *pb, double w, double d, double *m) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) m[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double x = pa[2 * i], y = pa[2 * i + 1];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double sx = pb[2 * j], sy = pb[2 * j + 1];
+            const double dx = x - sx, dy = y - sy;
+            const double gg = w / __builtin_sqrt(dx * dx + dy * dy + d * d);
+            m[0] += gg;
+            if (!SCALAR) {
+                m[1] += gg * x; m[2] += gg * y; m[3] += gg * sx; m[4] += gg * sy;
+                m[5] += gg * (x * sx + y * sy);
+            }
+        }
+    }
+}
+
+// The closed-form inner integrals (Wilton, Rao et al., IEEE TAP 1984) of 1/R and (r'-rho)/R over
+// source triangle q at observation point (x, y), height offset d (static_kernels.hpp:78-97).
+template <bool SCALAR>
+__device__ __forceinline__ void inner_integrals(const GeomDev &g, int q, double x, double y, double d,
+                                                double &I1, double &IRx, double &IRy) {
+    const double ad = fabs(d);
+    I1 = 0; IRx = 0; IRy = 0;
+    const double *Vq = g.V + 6 * (size_t)q, *Tq = g.tau + 6 * (size_t)q, *Nq = g.un + 6 * (size_t)q;
+    const double *Lq = g.L + 3 * (size_t)q;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double vx = Vq[2 * e] - x, vy = Vq[2 * e + 1] - y;
+        const double sm = vx * Tq[2 * e] + vy * Tq[2 * e + 1];
+        const double sp = sm + Lq[e];
+        const double t0 = vx * Nq[2 * e] + vy * Nq[2 * e + 1];
+        const double r02 = t0 * t0 + d * d, r0 = sqrt(r02);
+        const double rm = sqrt(r02 + sm * sm), rp = sqrt(r02 + sp * sp);
+        const double lg = r0 > 1e-14 * Lq[e] ? asinh(sp / r0) - asinh(sm / r0) : 0.0;
+        double term = t0 * lg;
+        if (ad > 0) term -= ad * (atan2(t0 * sp, r02 + ad * rp) - atan2(t0 * sm, r02 + ad * rm));
+        I1 += term;
+        if (!SCALAR) {
+            const double er = .5 * (r02 * lg + sp * rp - sm * rm);
+            IRx += er * Nq[2 * e]; IRy += er * Nq[2 * e + 1];
+        }
+    }
+}
+
+// One observation point's contribution to the six moments of canonical pair (a, b).
+// LOCAL: first moments about each triangle's own centroid (FP32 planes' frame).
+template <bool SCALAR, bool LOCAL>
+__device__ __forceinline__ void near_point(const GeomDev &g, int a, int b, int level, int j, double *m) {
+    const int nq = g.nq[level];
+    const double *pt = g.qp[level] + 2 * ((size_t)a * nq + j);
+    const double x = pt[0], y = pt[1];
+    const double d = g.z[a] - g.z[b];
+    double I1, IRx, IRy;
+    inner_integrals<SCALAR>(g, b, x, y, d, I1, IRx, IRy);
+    const double w = g.area[a] * g.qw[level][j];
+    m[0] = w * I1;
+    if (SCALAR) return;
+    double xo = x, yo = y, xs = x, ys = y;
+    if (LOCAL) {
+        xo = x - g.cent[2 * a]; yo = y - g.cent[2 * a + 1];
+        xs = x - g.cent[2 * b]; ys = y - g.cent[2 * b + 1];
+    }
+    const double sx = IRx + xs * I1, sy = IRy + ys * I1;
+    m[1] = w * xo * I1; m[2] = w * yo * I1;
+    m[3] = w * sx; m[4] = w * sy; m[5] = w * (xo * sx + yo * sy);
+}
+
+// Full FP64 moments of one ordered pair (reference static_pair), for explicit pair lists.
+template <bool SCALAR>
+__device__ void pair64(const GeomDev &g, int p, int q, double d, int k, double *m) {
+    if (k < 0) {
+        far64<SCALAR>(g.P3 + 6 * (size_t)p, g.P3 + 6 * (size_t)q, g.area[p] * g.area[q] / 9, d, m);
+        return;
+    }
+    for (int j = 0; j < 6; ++j) m[j] = 0;
+    const int nq = g.nq[k];
+    for (int j = 0; j < nq; ++j) {
+        const double *pt = g.qp[k] + 2 * ((size_t)p * nq + j);
+        const double x = pt[0], y = pt[1];
+        double I1, IRx, IRy;
+        inner_integrals<SCALAR>(g, q, x, y, d, I1, IRx, IRy);
+        const double w = g.area[p] * g.qw[k][j];
+        m[0] += w * I1;
+        if (!SCALAR) {
+            const double sx = IRx + x * I1, sy = IRy + y * I1;
+            m[1] += w * x * I1; m[2] += w * y * I1;
+            m[3] += w * sx; m[4] += w * sy; m[5] += w * (x * sx + y * sy);
+        }
+    }
+}
+
+// Entry (r, c) of the planes from the canonical pair's moments (static_cupy_kernels.py:106-115
+// contract; diagonal: the mean of the two first moments, static_kernels.hpp:230-236).
+template <class T, bool SCALAR>
+__device__ __forceinline__ void store_entry(T *out, size_t plane, size_t i, int r, int c, const double *m,
+                                            unsigned *flag) {
+    bool ok = isfinite(m[0]);
+    out[i] = (T)m[0];
+    if (!SCALAR) {
+        double mx, my;
+        if (r == c) { mx = (m[1] + m[3]) / 2; my = (m[2] + m[4]) / 2; }
+        else if (r < c) { mx = m[1]; my = m[2]; }
+        else { mx = m[3]; my = m[4]; }
+        out[plane + i] = (T)mx; out[2 * plane + i] = (T)my; out[3 * plane + i] = (T)m[5];
+        ok = ok && isfinite(mx) && isfinite(my) && isfinite(m[5]);
+    }
+    if (!ok) atomicOr(flag, 1u);
+}
+
+// ---- near-list append, aggregated per wave ------------------------------------------------
+
+__device__ __forceinline__ void wave_append(bool want, unsigned long long *list, unsigned long long *count,
+                                            unsigned long long cap, unsigned long long entry) {
+    const unsigned long long mask = __ballot(want);
+    if (!mask) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long rank = __popcll(mask & ((1ull << lane) - 1ull));
+    if (want && base + rank < cap) list[base + rank] = entry;
+}
+
+__device__ __forceinline__ void wave_count(bool want, unsigned long long *count) {
+    const unsigned long long mask = __ballot(want);
+    if (!mask) return;
+    if (__lane_id() == __ffsll((long long)mask) - 1) atomicAdd(count, (unsigned long long)__popcll(mask));
+}
+
+// ---- far kernels ---------------------------------------------------------------------------
+
+struct FarArgs {
+    int r0, rows, n;        // rows [r0, r0+rows) of the n x n matrix; out row 0 is row r0
+    double near;
+    float near32;
+    int all_near, outer, mirror;
+    size_t plane;           // elements between planes
+    NearLists lists;
+    unsigned *flag;
+};
+
+// Shared classification of the canonical pair (a, b): -1 far, else the analytic level.
+__device__ __forceinline__ int classify(const GeomDev &g, int a, int b, const FarArgs &A, float ra32,
+                                        float rb32) {
+    const double dxd = g.cent[2 * a] - g.cent[2 * b], dyd = g.cent[2 * a + 1] - g.cent[2 * b + 1];
+    const double dzd = g.z[a] - g.z[b];
+    const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float thr = A.near32 * (ra32 + rb32);
+    const bool certain_far = !A.all_near && d2 > thr * thr * 1.00001f;
+    if (certain_far) return -1;
+    return exact_kind(g.cent, g.radius, g.z, a, b, A.near, A.all_near, A.outer);
+}
+
+__device__ __forceinline__ void handle_near(const FarArgs &A, bool valid, int kind, int r, int c) {
+    const bool near_entry = valid && kind >= 0 && (r <= c || !A.mirror);
+    const unsigned long long e = ((unsigned long long)(unsigned)r << 32) | (unsigned)c;
+    const bool l1 = kind > 0;
+    wave_append(near_entry && !l1, A.lists.list[0], A.lists.count + 0, A.lists.cap, e);
+    wave_append(near_entry && l1, A.lists.list[1], A.lists.count + 1, A.lists.cap, e);
+    if (!A.mirror) wave_count(valid && kind >= 0 && r <= c, A.lists.count + 2);
+    else wave_count(near_entry, A.lists.count + 2);
+}
+
+// FP32 local-frame far kernel.
+template <bool SCALAR>
+__global__ __launch_bounds__(256) void far32_kernel(GeomDev g, FarArgs A, float *out) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * WAVE + lane;
+    const bool cvalid = c < A.n;
+    const int cc = cvalid ? c : A.n - 1;
+    const float4 c0 = g.far32[2 * cc], c1 = g.far32[2 * cc + 1];
+    const int rbase = blockIdx.y * FAR_ROWS_PER_BLOCK + wave * FAR_ROWS_PER_WAVE;
+    for (int k = 0; k < FAR_ROWS_PER_WAVE; ++k) {
+        const int rl = rbase + k;
+        if (rl >= A.rows) break;
+        const int r = A.r0 + rl;
+        const float4 q0 = g.far32[2 * r], q1 = g.far32[2 * r + 1];
+        const bool rfirst = r <= c;
+        const int a = rfirst ? r : cc, b = rfirst ? cc : r;
+        const float4 a0 = rfirst ? q0 : c0, a1 = rfirst ? q1 : c1;
+        const float4 b0 = rfirst ? c0 : q0, b1 = rfirst ? c1 : q1;
+        const int kind = classify(g, a, b, A, a1.w, b1.w);
+        handle_near(A, cvalid, kind, r, c);
+        // far moments in the pair's local frames
+        const float dcx = (float)(g.cent[2 * a] - g.cent[2 * b]);
+        const float dcy = (float)(g.cent[2 * a + 1] - g.cent[2 * b + 1]);
+        const float dz = (float)(g.z[a] - g.z[b]);
+        const float dz2 = dz * dz;
+        const float ax[3] = {a0.x, a0.z, a1.x}, ay[3] = {a0.y, a0.w, a1.y};
+        const float bx[3] = {b0.x, b0.z, b1.x}, by[3] = {b0.y, b0.w, b1.y};
+        const float w = a1.z * b1.z * (1.0f / 9.0f);
+        float G[3] = {0, 0, 0}, H[3] = {0, 0, 0}, Tx[3] = {0, 0, 0}, Ty[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float px = ax[i] + dcx, py = ay[i] + dcy;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float dx = px - bx[j], dy = py - by[j];
+                const float r2 = dx * dx + dy * dy + dz2;
+                const float gv = w * __builtin_amdgcn_rsqf(r2);
+                G[i] += gv;
+                if (!SCALAR) {
+                    H[j] += gv;
+                    Tx[i] += gv * bx[j]; Ty[i] += gv * by[j];
+                }
+            }
+        }
+        if (!cvalid || kind >= 0) continue;
+        const size_t idx = (size_t)rl * A.n + c;
+        const float m0 = G[0] + G[1] + G[2];
+        bool ok = isfinite(m0);
+        out[idx] = m0;
+        if (!SCALAR) {
+            const float rox = G[0] * ax[0] + G[1] * ax[1] + G[2] * ax[2];
+            const float roy = G[0] * ay[0] + G[1] * ay[1] + G[2] * ay[2];
+            const float rsx = H[0] * bx[0] + H[1] * bx[1] + H[2] * bx[2];
+            const float rsy = H[0] * by[0] + H[1] * by[1] + H[2] * by[2];
+            const float rr = ax[0] * Tx[0] + ay[0] * Ty[0] + ax[1] * Tx[1] + ay[1] * Ty[1] + ax[2] * Tx[2] +
+                             ay[2] * Ty[2];
+            const float mx = rfirst ? rox : rsx, my = rfirst ? roy : rsy;
+            out[A.plane + idx] = mx;
+            out[2 * A.plane + idx] = my;
+            out[3 * A.plane + idx] = rr;
+            ok = ok && isfinite(mx) && isfinite(my) && isfinite(rr);
+        }
+        if (!ok) atomicOr(A.flag, 1u);
+    }
+}
+
+// FP64 reference-layout far kernel.
+template <bool SCALAR>
+__global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double *out) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * WAVE + lane;
+    const bool cvalid = c < A.n;
+    const int cc = cvalid ? c : A.n - 1;
+    double pc[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) pc[i] = g.P3[6 * (size_t)cc + i];
+    const double areac = g.area[cc];
+    const float radc = (float)g.radius[cc];
+    const int rbase = blockIdx.y * FAR_ROWS_PER_BLOCK + wave * FAR_ROWS_PER_WAVE;
+    for (int k = 0; k < FAR_ROWS_PER_WAVE; ++k) {
+        const int rl = rbase + k;
+        if (rl >= A.rows) break;
+        const int r = A.r0 + rl;
+        double pr[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) pr[i] = g.P3[6 * (size_t)r + i];
+        const bool rfirst = r <= c;
+        const int a = rfirst ? r : cc, b = rfirst ? cc : r;
+        const float rada = rfirst ? (float)g.radius[r] : radc, radb = rfirst ? radc : (float)g.radius[r];
+        const int kind = classify(g, a, b, A, rada, radb);
+        handle_near(A, cvalid, kind, r, c);
+        double pa[6], pb[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) { pa[i] = rfirst ? pr[i] : pc[i]; pb[i] = rfirst ? pc[i] : pr[i]; }
+        const double w = (rfirst ? g.area[r] : areac) * (rfirst ? areac : g.area[r]) / 9;
+        double m[6];
+        far64<SCALAR>(pa, pb, w, g.z[a] - g.z[b], m);
+        if (!cvalid || kind >= 0) continue;
+        store_entry<double, SCALAR>(out, A.plane, (size_t)rl * A.n + c, r, c, m, A.flag);
+    }
+}
+
+// ---- near kernel -----------------------------------------------------------------------------
+
+template <class T, bool SCALAR, bool LOCAL>
+__global__ __launch_bounds__(256) void near_kernel(GeomDev g, const unsigned long long *list,
+                                                  const unsigned long long *count, unsigned long long cap,
+                                                  int level, int r0, int n, int mirror, T *out, size_t plane,
+                                                  unsigned *flag) {
+    __shared__ double buf[256 * 6];
+    const int nq = g.nq[level];
+    const int ppb = 256 / nq;
+    const unsigned long long total = min(*count, cap);
+    const int lp = threadIdx.x / nq, j = threadIdx.x - lp * nq;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * ppb; base < total;
+         base += (unsigned long long)gridDim.x * ppb) {
+        const unsigned long long idx = base + lp;
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        if (lp < ppb && idx < total) {
+            const unsigned long long e = list[idx];
+            const int r = (int)(e >> 32), c = (int)(e & 0xffffffffu);
+            near_point<SCALAR, LOCAL>(g, min(r, c), max(r, c), level, j, m);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) buf[threadIdx.x * 6 + k] = m[k];
+        __syncthreads();
+        if (threadIdx.x < ppb && base + threadIdx.x < total) {
+            const unsigned long long e = list[base + threadIdx.x];
+            const int r = (int)(e >> 32), c = (int)(e & 0xffffffffu);
+            double s[6] = {0, 0, 0, 0, 0, 0};
+            for (int jj = 0; jj < nq; ++jj) {
+                const double *v = buf + (threadIdx.x * nq + jj) * 6;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s[k] += v[k];
+            }
+            store_entry<T, SCALAR>(out, plane, (size_t)(r - r0) * n + c, r, c, s, flag);
+            if (mirror && r != c) store_entry<T, SCALAR>(out, plane, (size_t)(c - r0) * n + r, c, r, s, flag);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- explicit pair list ---------------------------------------------------------------------
+
+template <bool SCALAR>
+__global__ __launch_bounds__(128) void pairs_kernel(GeomDev g, const int *P, const int *Q, const double *D,
+                                                   const int *K, const int *order, long long n, double *out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long i = order[t];
+    double m[6];
+    pair64<SCALAR>(g, P[i], Q[i], D[i], K[i], m);
+    const int width = SCALAR ? 1 : 6;
+    for (int k = 0; k < width; ++k) out[i * width + k] = m[k];
+}
+
+// FP32 far records: far points relative to the centroid, area, radius.
+__global__ void far32_records(GeomDev g, float4 *rec) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nt) return;
+    const double cx = g.cent[2 * t], cy = g.cent[2 * t + 1];
+    const double *p = g.P3 + 6 * (size_t)t;
+    rec[2 * t] = make_float4((float)(p[0] - cx), (float)(p[1] - cy), (float)(p[2] - cx), (float)(p[3] - cy));
+    rec[2 * t + 1] = make_float4((float)(p[4] - cx), (float)(p[5] - cy), (float)g.area[t], (float)g.radius[t]);
+}
+
+}  // namespace pq
+
+// ---- handle ---------------------------------------------------------------------------------
+
+using namespace pq;
+
+struct pq_fill {
+    int device = 0;
+    int nt = 0;
+    GeomDev g{};
+    DevBuf geom, far32, lists[2], counts, flag;
+    unsigned long long cap = 0;
+    hipStream_t stream = nullptr, copy_stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double far_ms = 0, near_ms = 0;
+    long long near_pairs[2] = {0, 0};
+    int cus = 256;
+};
+
+namespace {
+
+size_t arr_bytes(long long n) { return (size_t)n * sizeof(double); }
+
+void launch_far(pq_fill *f, int precision, bool scalar, const FarArgs &A, void *out, hipStream_t s) {
+    dim3 grid((A.n + WAVE - 1) / WAVE, (A.rows + FAR_ROWS_PER_BLOCK - 1) / FAR_ROWS_PER_BLOCK);
+    if (precision == 32) {
+        if (scalar) hipLaunchKernelGGL(far32_kernel<true>, grid, dim3(256), 0, s, f->g, A, (float *)out);
+        else hipLaunchKernelGGL(far32_kernel<false>, grid, dim3(256), 0, s, f->g, A, (float *)out);
+    } else {
+        if (scalar) hipLaunchKernelGGL(far64_kernel<true>, grid, dim3(256), 0, s, f->g, A, (double *)out);
+        else hipLaunchKernelGGL(far64_kernel<false>, grid, dim3(256), 0, s, f->g, A, (double *)out);
+    }
+    PQ_HIP(hipGetLastError());
+}
+
+void launch_near(pq_fill *f, int precision, bool scalar, int which, int level, const FarArgs &A, void *out,
+                 hipStream_t s) {
+    const int nq = f->g.nq[level];
+    const int ppb = 256 / nq;
+    (void)ppb;
+    dim3 grid(f->cus * 8), block(256);
+    const unsigned long long *list = f->lists[which].as<unsigned long long>();
+    const unsigned long long *count = f->counts.as<unsigned long long>() + which;
+#define PQ_NEAR(T, S, LOC)                                                                           \
+    hipLaunchKernelGGL((near_kernel<T, S, LOC>), grid, block, 0, s, f->g, list, count, A.lists.cap, level, \
+                       A.r0, A.n, A.mirror, (T *)out, A.plane, A.flag)
+    if (precision == 32) {
+        if (scalar) PQ_NEAR(float, true, true); else PQ_NEAR(float, false, true);
+    } else {
+        if (scalar) PQ_NEAR(double, true, false); else PQ_NEAR(double, false, false);
+    }
+#undef PQ_NEAR
+    PQ_HIP(hipGetLastError());
+}
+
+void check_rule(double near, int outer, pq_fill *f) {
+    PQ_REQUIRE(!std::isnan(near) && near >= 0, "near must be nonnegative (or infinity)");
+    PQ_REQUIRE(outer >= 0 && outer <= 2, "outer_levels must be 0, 1 or 2");
+    PQ_REQUIRE(f->g.nq[outer] == 6 * (1 << (2 * outer)), "outer level was not prepared");
+}
+
+FarArgs far_args(pq_fill *f, double near, int outer, int r0, int rows, int mirror, size_t plane) {
+    FarArgs A{};
+    A.r0 = r0; A.rows = rows; A.n = f->nt;
+    A.all_near = std::isinf(near) ? 1 : 0;
+    A.near = A.all_near ? 0.0 : near;
+    A.near32 = (float)A.near;
+    A.outer = outer; A.mirror = mirror; A.plane = plane;
+    A.lists.list[0] = f->lists[0].as<unsigned long long>();
+    A.lists.list[1] = f->lists[1].as<unsigned long long>();
+    A.lists.count = f->counts.as<unsigned long long>();
+    A.lists.cap = f->cap;
+    A.flag = f->flag.as<unsigned>();
+    return A;
+}
+
+void reserve_lists(pq_fill *f, unsigned long long cap) {
+    if (cap <= f->cap) return;
+    f->lists[0].reserve(cap * 8);
+    f->lists[1].reserve(cap * 8);
+    f->cap = cap;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pq_fill_create(const pq_geometry *g, int device, pq_fill **out) {
+    return guard([&] {
+        PQ_REQUIRE(g && out, "pq_fill_create: null argument");
+        PQ_REQUIRE(g->nt > 0, "pq_fill_create: empty geometry");
+        for (int k = 0; k < 3; ++k)
+            PQ_REQUIRE(g->qp[k] && g->qw[k] && g->nq[k] > 0 && g->nq[k] <= 256,
+                       "pq_fill_create: invalid observation quadrature");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            throw Error(PQ_EDEVICE, "no HIP device available");
+        PQ_REQUIRE(device >= 0 && device < ndev, "pq_fill_create: invalid device ordinal");
+        DeviceScope scope(device);
+        auto *f = new pq_fill();
+        try {
+            f->device = device;
+            f->nt = g->nt;
+            const long long nt = g->nt;
+            hipDeviceProp_t prop;
+            PQ_HIP(hipGetDeviceProperties(&prop, device));
+            f->cus = prop.multiProcessorCount;
+            // one buffer for every FP64 array
+            const long long sizes[] = {nt * 6, nt * 3, nt * 6, nt * 6, nt, nt * 6, nt, nt * 2, nt,
+                                       nt * 2LL * g->nq[0], g->nq[0], nt * 2LL * g->nq[1], g->nq[1],
+                                       nt * 2LL * g->nq[2], g->nq[2]};
+            const double *src[] = {g->V, g->L, g->tau, g->un, g->area, g->P3, g->z, g->cent, g->radius,
+                                   g->qp[0], g->qw[0], g->qp[1], g->qw[1], g->qp[2], g->qw[2]};
+            size_t total = 0;
+            for (long long s : sizes) total += (arr_bytes(s) + 255) / 256 * 256;
+            f->geom.reserve(total);
+            const double *dst[15];
+            size_t off = 0;
+            for (int i = 0; i < 15; ++i) {
+                PQ_REQUIRE(src[i], "pq_fill_create: missing geometry array");
+                double *d = (double *)((char *)f->geom.p + off);
+                PQ_HIP(hipMemcpy(d, src[i], arr_bytes(sizes[i]), hipMemcpyHostToDevice));
+                dst[i] = d;
+                off += (arr_bytes(sizes[i]) + 255) / 256 * 256;
+            }
+            GeomDev &G = f->g;
+            G.nt = g->nt;
+            G.V = dst[0]; G.L = dst[1]; G.tau = dst[2]; G.un = dst[3]; G.area = dst[4]; G.P3 = dst[5];
+            G.z = dst[6]; G.cent = dst[7]; G.radius = dst[8];
+            for (int k = 0; k < 3; ++k) { G.qp[k] = dst[9 + 2 * k]; G.qw[k] = dst[10 + 2 * k]; G.nq[k] = g->nq[k]; }
+            f->far32.reserve((size_t)nt * 2 * sizeof(float4));
+            G.far32 = f->far32.as<float4>();
+            hipLaunchKernelGGL(far32_records, dim3((nt + 255) / 256), dim3(256), 0, 0, G, f->far32.as<float4>());
+            PQ_HIP(hipGetLastError());
+            f->counts.reserve(4 * sizeof(unsigned long long));
+            f->flag.reserve(sizeof(unsigned));
+            PQ_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+            PQ_HIP(hipStreamCreateWithFlags(&f->copy_stream, hipStreamNonBlocking));
+            for (auto &e : f->ev) PQ_HIP(hipEventCreate(&e));
+            PQ_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            pq_fill_destroy(f);
+            throw;
+        }
+        *out = f;
+    });
+}
+
+int pq_fill_destroy(pq_fill *f) {
+    if (!f) return PQ_OK;
+    return guard([&] {
+        DeviceScope scope(f->device);
+        if (f->stream) (void)hipStreamSynchronize(f->stream);
+        if (f->copy_stream) (void)hipStreamSynchronize(f->copy_stream);
+        for (auto e : f->ev) if (e) (void)hipEventDestroy(e);
+        if (f->stream) (void)hipStreamDestroy(f->stream);
+        if (f->copy_stream) (void)hipStreamDestroy(f->copy_stream);
+        f->geom.release(); f->far32.release(); f->lists[0].release(); f->lists[1].release();
+        f->counts.release(); f->flag.release();
+        delete f;
+    });
+}
+
+int pq_fill_device(pq_fill *f, double near, int outer, int scalar_only, int precision, void *planes,
+                   int64_t *analytic, void *stream_) {
+    return guard([&] {
+        PQ_REQUIRE(f && planes, "pq_fill_device: null argument");
+        PQ_REQUIRE(precision == 32 || precision == 64, "pq_fill_device: precision must be 32 or 64");
+        check_rule(near, outer, f);
+        DeviceScope scope(f->device);
+        hipStream_t s = stream_ ? (hipStream_t)stream_ : f->stream;
+        const long long n = f->nt;
+        const size_t plane = (size_t)n * n;
+        reserve_lists(f, std::max<unsigned long long>(1ull << 20, 64ull * n));
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            FarArgs A = far_args(f, near, outer, 0, (int)n, 1, plane);
+            PQ_HIP(hipMemsetAsync(f->counts.p, 0, 4 * sizeof(unsigned long long), s));
+            PQ_HIP(hipMemsetAsync(f->flag.p, 0, sizeof(unsigned), s));
+            PQ_HIP(hipEventRecord(f->ev[0], s));
+            launch_far(f, precision, scalar_only, A, planes, s);
+            PQ_HIP(hipEventRecord(f->ev[1], s));
+            launch_near(f, precision, scalar_only, 0, 0, A, planes, s);
+            if (outer > 0) launch_near(f, precision, scalar_only, 1, outer, A, planes, s);
+            PQ_HIP(hipEventRecord(f->ev[2], s));
+            unsigned long long counts[4];
+            unsigned flag = 0;
+            PQ_HIP(hipMemcpyAsync(counts, f->counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
+            PQ_HIP(hipMemcpyAsync(&flag, f->flag.p, sizeof(flag), hipMemcpyDeviceToHost, s));
+            PQ_HIP(hipStreamSynchronize(s));
+            const unsigned long long need = std::max(counts[0], counts[1]);
+            if (need > f->cap) {             // lists overflowed: grow them and fill again
+                PQ_REQUIRE(attempt == 0, "pq_fill_device: near lists overflowed twice");
+                reserve_lists(f, need);
+                continue;
+            }
+            float t0 = 0, t1 = 0;
+            PQ_HIP(hipEventElapsedTime(&t0, f->ev[0], f->ev[1]));
+            PQ_HIP(hipEventElapsedTime(&t1, f->ev[1], f->ev[2]));
+            f->far_ms = t0; f->near_ms = t1;
+            f->near_pairs[0] = (long long)counts[0]; f->near_pairs[1] = (long long)counts[1];
+            if (analytic) *analytic = (int64_t)counts[2];
+            if (flag) throw Error(PQ_ENONFINITE, "non-finite triangle-pair moments from fill backend");
+            break;
+        }
+    });
+}
+
+int pq_fill_last_timing(pq_fill *f, double *far_ms, double *near_ms, int64_t near_pairs[2]) {
+    return guard([&] {
+        PQ_REQUIRE(f, "pq_fill_last_timing: null handle");
+        if (far_ms) *far_ms = f->far_ms;
+        if (near_ms) *near_ms = f->near_ms;
+        if (near_pairs) { near_pairs[0] = f->near_pairs[0]; near_pairs[1] = f->near_pairs[1]; }
+    });
+}
+
+// Row strips on the device, each copied into the caller's host rows (static_cupy.py:127-197).
+int pq_fill_symmetric(pq_fill *f, double near, int outer, int scalar_only, double *const planes[4],
+                      int64_t *analytic) {
+    return guard([&] {
+        PQ_REQUIRE(f && planes && planes[0], "pq_fill_symmetric: null argument");
+        const int width = scalar_only ? 1 : 4;
+        for (int k = 0; k < width; ++k) PQ_REQUIRE(planes[k], "pq_fill_symmetric: missing output plane");
+        check_rule(near, outer, f);
+        DeviceScope scope(f->device);
+        const long long n = f->nt;
+        const long long rows = std::max(1LL, std::min(n, (1LL << 22) / n));
+        const size_t plane = (size_t)rows * n;
+        DevBuf strip[2];
+        strip[0].reserve(plane * width * sizeof(double));
+        strip[1].reserve(plane * width * sizeof(double));
+        reserve_lists(f, std::max<unsigned long long>(plane, 1024));
+        hipEvent_t done[2], copied[2];
+        for (int i = 0; i < 2; ++i) { PQ_HIP(hipEventCreate(&done[i])); PQ_HIP(hipEventCreate(&copied[i])); }
+        DevBuf counts;
+        const long long nstrips = (n + rows - 1) / rows;
+        counts.reserve(4 * sizeof(unsigned long long) * nstrips);
+        PQ_HIP(hipMemsetAsync(counts.p, 0, 4 * sizeof(unsigned long long) * nstrips, f->stream));
+        PQ_HIP(hipMemsetAsync(f->flag.p, 0, sizeof(unsigned), f->stream));
+        bool used[2] = {false, false};
+        unsigned long long analytic_total = 0;
+        // The strip near lists are shared by both buffers: serialize strips on one stream and
+        // copy each finished strip on the copy stream while the next one is integrated.
+        for (long long s = 0; s < nstrips; ++s) {
+            const int b = (int)(s & 1);
+            const long long r0 = s * rows, m = std::min(rows, n - r0);
+            if (used[b]) PQ_HIP(hipStreamWaitEvent(f->stream, copied[b], 0));
+            FarArgs A = far_args(f, near, outer, (int)r0, (int)m, 0, plane);
+            A.lists.count = counts.as<unsigned long long>() + 4 * s;
+            launch_far(f, 64, scalar_only, A, strip[b].p, f->stream);
+            const unsigned long long *cnt = A.lists.count;
+            // near kernels read this strip's counters
+            {
+                const int nq0 = f->g.nq[0];
+                (void)nq0;
+                dim3 grid(f->cus * 8), block(256);
+                for (int which = 0; which < (outer > 0 ? 2 : 1); ++which) {
+                    const int level = which ? outer : 0;
+#define PQ_NEAR_STRIP(S)                                                                                    \
+    hipLaunchKernelGGL((near_kernel<double, S, false>), grid, block, 0, f->stream, f->g,                      \
+                       f->lists[which].as<unsigned long long>(), cnt + which, A.lists.cap, level, (int)r0,     \
+                       (int)n, 0, strip[b].as<double>(), plane, A.flag)
+                    if (scalar_only) PQ_NEAR_STRIP(true); else PQ_NEAR_STRIP(false);
+#undef PQ_NEAR_STRIP
+                    PQ_HIP(hipGetLastError());
+                }
+            }
+            PQ_HIP(hipEventRecord(done[b], f->stream));
+            PQ_HIP(hipStreamWaitEvent(f->copy_stream, done[b], 0));
+            for (int k = 0; k < width; ++k)
+                PQ_HIP(hipMemcpyAsync(planes[k] + (size_t)r0 * n, strip[b].as<double>() + (size_t)k * plane,
+                                      (size_t)m * n * sizeof(double), hipMemcpyDeviceToHost, f->copy_stream));
+            PQ_HIP(hipEventRecord(copied[b], f->copy_stream));
+            used[b] = true;
+        }
+        PQ_HIP(hipStreamSynchronize(f->stream));
+        PQ_HIP(hipStreamSynchronize(f->copy_stream));
+        std::vector<unsigned long long> host(4 * nstrips);
+        PQ_HIP(hipMemcpy(host.data(), counts.p, host.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        unsigned flag = 0;
+        PQ_HIP(hipMemcpy(&flag, f->flag.p, sizeof(flag), hipMemcpyDeviceToHost));
+        for (long long s = 0; s < nstrips; ++s) analytic_total += host[4 * s + 2];
+        for (int i = 0; i < 2; ++i) { (void)hipEventDestroy(done[i]); (void)hipEventDestroy(copied[i]); }
+        strip[0].release(); strip[1].release(); counts.release();
+        if (analytic) *analytic = (int64_t)analytic_total;
+        if (flag) throw Error(PQ_ENONFINITE, "non-finite triangle-pair moments from fill backend");
+    });
+}
+
+int pq_pairs(pq_fill *f, const int32_t *p, const int32_t *q, const double *d, const int32_t *kind, int64_t n,
+             int scalar_only, double *out) {
+    return guard([&] {
+        PQ_REQUIRE(f && (n == 0 || (p && q && d && kind && out)), "pq_pairs: null argument");
+        PQ_REQUIRE(n >= 0, "pq_pairs: negative pair count");
+        if (n == 0) return;
+        // validate on the host (static_kernels.hpp:156-159) and sort by kind so far and analytic
+        // pairs do not share wavefronts (static_cupy.py:117-120)
+        std::vector<int> order((size_t)n);
+        std::vector<long long> bucket(4, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            PQ_REQUIRE(p[i] >= 0 && p[i] < f->nt && q[i] >= 0 && q[i] < f->nt && kind[i] >= -1 && kind[i] <= 2 &&
+                           std::isfinite(d[i]),
+                       "pq_pairs: invalid triangle index, kind or offset");
+            if (kind[i] >= 0)
+                PQ_REQUIRE(f->g.nq[kind[i]] == 6 * (1 << (2 * kind[i])),
+                           "pair kinds must select far (-1) or a prepared analytic level");
+            ++bucket[kind[i] + 1];
+        }
+        std::vector<long long> start(4, 0);
+        for (int k = 1; k < 4; ++k) start[k] = start[k - 1] + bucket[k - 1];
+        for (int64_t i = 0; i < n; ++i) order[(size_t)start[kind[i] + 1]++] = (int)i;
+        DeviceScope scope(f->device);
+        const int width = scalar_only ? 1 : 6;
+        DevBuf buf;
+        const size_t bytes = (size_t)n * (4 * 4 + 8) + (size_t)n * width * 8;
+        buf.reserve(bytes + 5 * 256);
+        char *base = (char *)buf.p;
+        auto carve = [&](size_t sz) { char *r = base; base += (sz + 255) / 256 * 256; return r; };
+        int *dp = (int *)carve(n * 4), *dq = (int *)carve(n * 4), *dk = (int *)carve(n * 4),
+            *dord = (int *)carve(n * 4);
+        double *dd = (double *)carve(n * 8), *dout = (double *)carve(n * width * 8);
+        hipStream_t s = f->stream;
+        PQ_HIP(hipMemcpyAsync(dp, p, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dq, q, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dk, kind, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dord, order.data(), n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dd, d, n * 8, hipMemcpyHostToDevice, s));
+        dim3 grid((unsigned)((n + 127) / 128));
+        if (scalar_only) hipLaunchKernelGGL(pairs_kernel<true>, grid, dim3(128), 0, s, f->g, dp, dq, dd, dk, dord, (long long)n, dout);
+        else hipLaunchKernelGGL(pairs_kernel<false>, grid, dim3(128), 0, s, f->g, dp, dq, dd, dk, dord, (long long)n, dout);
+        PQ_HIP(hipGetLastError());
+        PQ_HIP(hipMemcpyAsync(out, dout, (size_t)n * width * 8, hipMemcpyDeviceToHost, s));
+        PQ_HIP(hipStreamSynchronize(s));
+        buf.release();
+    });
+}
+
+}  // extern "C"

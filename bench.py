@@ -61,6 +61,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
+                    help='first copy BYTES device-to-device (the scale of a counter run)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error='this build runs on one GPU: every box here has one; '
@@ -69,6 +71,12 @@ def main():
     from pcbqs_amd import board as boards
     from pcbqs_amd.backend import Mi355xBackend
     from pcbqs_amd.geometry import FillGeometry
+    if args.copy_probe:
+        from pcbqs_amd._lib import DeviceArray, call
+        src, dst = DeviceArray(args.copy_probe, np.uint8), DeviceArray(args.copy_probe, np.uint8)
+        call('pq_memcpy_d2d', dst.ptr, src.ptr, args.copy_probe)
+        src.close()
+        dst.close()
     board = boards.board_c2(args.seed)
     n = board.nt
     pairs = n * (n + 1) // 2

@@ -62,6 +62,7 @@ int pq_malloc(int device, size_t bytes, void **ptr_dev);
 int pq_free(void *ptr_dev);
 int pq_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int pq_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int pq_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int pq_memset(void *dst_dev, int value, size_t bytes);
 int pq_synchronize(int device);
 
@@ -104,7 +105,7 @@ int pq_pairs(pq_fill *f, const int32_t *p, const int32_t *q, const double *d,
  * (src/pcbqs/core/static_kernels.hpp:184-252); asynchronous on `stream`
  * except that it returns the analytic count (one small copy). */
 int pq_fill_device(pq_fill *f, double near, int outer_levels, int scalar_only, int precision,
-                   void *planes_dev, int64_t *analytic, void *stream);
+                   void *planes_dev, int64_t ld, int64_t *analytic, void *stream);
 
 /* Number of pair-kernel launches and timings of the last pq_fill_device call:
  * far kernel milliseconds, near kernel milliseconds (HIP events on the launch
@@ -115,16 +116,17 @@ int pq_fill_last_timing(pq_fill *f, double *far_ms, double *near_ms, int64_t nea
 
 /* L[m,n] = factor s_m s_n sum_{i,j} a_mi a_nj [Mrr(p,q) - v_nj.Mr(p,q)
  *          - v_mi.Mr(q,p) + v_mi.v_nj M0(p,q)], p = tri[i][m], q = tri[j][n],
- * written symmetrically into L_dev (ne,ne). planes_dev as produced by
+ * written symmetrically into L_dev (ne rows, leading dimension ldl >= ne; 0:
+ * ne). planes_dev (leading dimension ld, 0: nt) as produced by
  * pq_fill_device with the same precision (32: local frame; cent_dev (nt,2)
  * float64 centroids are then required). tri_dev (2,ne) int32, alpha_dev (2,ne),
  * vertex_dev (2,ne,2), scale_dev (ne) float64. out_precision 32 or 64.
  * Replaces static_combine (src/pcbqs/core/static_kernels.hpp:257-310) as
  * called by edge_inductance (src/pcbqs/static.py:103-128). */
-int pq_combine_device(int precision, const void *planes_dev, int64_t nt, const double *cent_dev,
+int pq_combine_device(int precision, const void *planes_dev, int64_t nt, int64_t ld, const double *cent_dev,
                       const int32_t *tri_dev, const double *alpha_dev, const double *vertex_dev,
                       const double *scale_dev, int64_t ne, double factor, int out_precision,
-                      void *L_dev, void *stream);
+                      void *L_dev, int64_t ldl, void *stream);
 
 #ifdef __cplusplus
 }

@@ -44,15 +44,16 @@ _SIGNATURES = {
     'pq_free': ([P], ctypes.c_int),
     'pq_memcpy_h2d': ([P, P, SIZE], ctypes.c_int),
     'pq_memcpy_d2h': ([P, P, SIZE], ctypes.c_int),
+    'pq_memcpy_d2d': ([P, P, SIZE], ctypes.c_int),
     'pq_memset': ([P, ctypes.c_int, SIZE], ctypes.c_int),
     'pq_synchronize': ([ctypes.c_int], ctypes.c_int),
     'pq_fill_create': ([P, ctypes.c_int, P], ctypes.c_int),
     'pq_fill_destroy': ([P], ctypes.c_int),
     'pq_fill_symmetric': ([P, DBL, ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),
     'pq_pairs': ([P, P, P, P, P, I64, ctypes.c_int, P], ctypes.c_int),
-    'pq_fill_device': ([P, DBL, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P], ctypes.c_int),
+    'pq_fill_device': ([P, DBL, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, I64, P, P], ctypes.c_int),
     'pq_fill_last_timing': ([P, P, P, P], ctypes.c_int),
-    'pq_combine_device': ([ctypes.c_int, P, I64, P, P, P, P, P, I64, DBL, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_combine_device': ([ctypes.c_int, P, I64, I64, P, P, P, P, P, I64, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
 }
 
 

@@ -91,18 +91,20 @@ class Mi355xBackend:
 
     # -- device-resident fill -----------------------------------------------------------------
     def fill_device(self, near=3.0, outer_levels=1, scalar_only=False, precision=64, out=None, stream=None):
-        """Fill the moment planes into HBM: a DeviceArray (1 or 4, nt, nt) of float64 (reference
-        layout) or float32 (local frame). Returns (planes, analytic pairs)."""
+        """Fill the moment planes into HBM: a DeviceArray (1 or 4, nt, ld) of float64 (reference
+        layout) or float32 (local frame), ld = nt rounded up to 64 (whole cache lines per row
+        segment); columns nt..ld-1 are padding. Returns (planes, analytic pairs)."""
         n = self.nt
         width = 1 if scalar_only else 4
         dtype = np.float64 if precision == 64 else np.float32
+        ld = padded(n)
         if out is None:
-            out = DeviceArray((width, n, n), dtype, self.device_id)
-        elif out.shape != (width, n, n) or out.dtype != dtype:
+            out = DeviceArray((width, n, ld), dtype, self.device_id)
+        elif out.shape != (width, n, ld) or out.dtype != dtype:
             raise ValueError('fill_device: output planes of the wrong shape or dtype')
         analytic = ctypes.c_int64()
         call('pq_fill_device', self._handle, float(near), int(outer_levels), int(bool(scalar_only)),
-             int(precision), out.ptr, ctypes.byref(analytic), stream)
+             int(precision), out.ptr, ld, ctypes.byref(analytic), stream)
         return out, int(analytic.value)
 
     def last_timing(self):
@@ -124,6 +126,11 @@ class Mi355xBackend:
             pass
 
 
+def padded(n, multiple=64):
+    """Leading dimension of device matrices: n rounded up to a whole number of 64-element rows."""
+    return (int(n) + multiple - 1) // multiple * multiple
+
+
 def make_backend(geometry, workers, chunk, device):
     """Entry point `mi355x` of the pcbqs.static_backends group (pyproject.toml)."""
     return Mi355xBackend(geometry, workers, chunk, device)
@@ -131,8 +138,9 @@ def make_backend(geometry, workers, chunk, device):
 
 def combine_device(planes, precision, rwg, cent, factor=1.0, scale=None, out_precision=64, device=0,
                    stream=None):
-    """L (Ne, Ne) on the device from device moment planes: static_combine's formula."""
-    nt = planes.shape[-1]
+    """L (Ne, ldl) on the device (ldl = Ne rounded up to 64; padding columns unused) from device
+    moment planes (1 or 4, nt, ld): static_combine's formula."""
+    nt, ld = planes.shape[-2], planes.shape[-1]
     tri, alpha, vertex = rwg.combine_arrays()
     ne = tri.shape[1]
     scale = np.ones(ne) if scale is None else np.ascontiguousarray(scale, float)
@@ -141,9 +149,10 @@ def combine_device(planes, precision, rwg, cent, factor=1.0, scale=None, out_pre
     dv = DeviceArray.from_host(vertex, device)
     ds = DeviceArray.from_host(scale, device)
     dc = DeviceArray.from_host(np.ascontiguousarray(cent, np.float64), device)
-    L = DeviceArray((ne, ne), np.float64 if out_precision == 64 else np.float32, device)
-    call('pq_combine_device', int(precision), planes.ptr, nt, dc.ptr, dt.ptr, da.ptr, dv.ptr, ds.ptr, ne,
-         float(factor), int(out_precision), L.ptr, stream)
+    ldl = padded(ne)
+    L = DeviceArray((ne, ldl), np.float64 if out_precision == 64 else np.float32, device)
+    call('pq_combine_device', int(precision), planes.ptr, nt, ld, dc.ptr, dt.ptr, da.ptr, dv.ptr, ds.ptr, ne,
+         float(factor), int(out_precision), L.ptr, ldl, stream)
     call('pq_synchronize', device)
     for a in (dt, da, dv, ds, dc):
         a.close()

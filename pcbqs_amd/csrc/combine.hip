@@ -19,10 +19,10 @@ namespace pq {
 
 template <class PT, bool LOCAL, class OT>
 __global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx, const PT *My, const PT *Mrr,
-                                                     long long nt, const double *cent, const int *tri,
+                                                     long long ld, const double *cent, const int *tri,
                                                      const double *alpha, const double *vertex,
                                                      const double *scale, long long ne, double factor,
-                                                     OT *L) {
+                                                     OT *L, long long ldl) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long c = (long long)blockIdx.x * 64 + lane;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx
                 for (int j = 0; j < 2; ++j) {
                     const size_t q = (size_t)tb[j];
                     const float unx = (float)vb[j][0], uny = (float)vb[j][1];
-                    const size_t pq = p * nt + q, qp = q * nt + p;
+                    const size_t pq = p * ld + q, qp = q * ld + p;
                     const float t = (float)Mrr[pq] + unx * (float)Mx[pq] + uny * (float)My[pq] +
                                     umx * (float)Mx[qp] + umy * (float)My[qp] + (umx * unx + umy * uny) * (float)M0[pq];
                     s32 += (float)(al[i] * bl[j]) * t;
@@ -73,14 +73,14 @@ __global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx
                 for (int j = 0; j < 2; ++j) {
                     const size_t q = (size_t)tb[j];
                     const double vnx = vb[j][0], vny = vb[j][1];
-                    const size_t pq = p * nt + q, qp = q * nt + p;
+                    const size_t pq = p * ld + q, qp = q * ld + p;
                     sum += al[i] * bl[j] * ((double)Mrr[pq] - vnx * (double)Mx[pq] - vny * (double)My[pq] -
                                             vmx * (double)Mx[qp] - vmy * (double)My[qp] +
                                             (vmx * vnx + vmy * vny) * (double)M0[pq]);
                 }
             }
         }
-        if (cvalid) L[r * ne + c] = (OT)(factor * scale[a] * scale[b] * sum);
+        if (cvalid) L[r * ldl + c] = (OT)(factor * scale[a] * scale[b] * sum);
     }
 }
 
@@ -88,40 +88,44 @@ __global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx
 
 using namespace pq;
 
-extern "C" int pq_combine_device(int precision, const void *planes, int64_t nt, const double *cent,
+extern "C" int pq_combine_device(int precision, const void *planes, int64_t nt, int64_t ld, const double *cent,
                                  const int32_t *tri, const double *alpha, const double *vertex,
                                  const double *scale, int64_t ne, double factor, int out_precision, void *L,
-                                 void *stream) {
+                                 int64_t ldl, void *stream) {
     return guard([&] {
         PQ_REQUIRE(planes && tri && alpha && vertex && scale && L, "pq_combine_device: null argument");
         PQ_REQUIRE(precision == 32 || precision == 64, "pq_combine_device: precision must be 32 or 64");
         PQ_REQUIRE(out_precision == 32 || out_precision == 64, "pq_combine_device: out_precision 32 or 64");
         PQ_REQUIRE(precision == 64 || cent, "pq_combine_device: FP32 (local-frame) planes need centroids");
         PQ_REQUIRE(nt > 0 && ne >= 0, "pq_combine_device: invalid sizes");
+        if (ld == 0) ld = nt;
+        PQ_REQUIRE(ld >= nt, "pq_combine_device: leading dimension below the triangle count");
+        if (ldl == 0) ldl = ne;
+        PQ_REQUIRE(ldl >= ne, "pq_combine_device: L leading dimension below the edge count");
         if (ne == 0) return;
         hipStream_t s = (hipStream_t)stream;
-        const size_t plane = (size_t)nt * nt;
+        const size_t plane = (size_t)nt * ld;
         dim3 grid((unsigned)((ne + 63) / 64), (unsigned)((ne + 15) / 16)), block(256);
         if (precision == 32) {
             const float *P = (const float *)planes;
             if (out_precision == 32)
                 hipLaunchKernelGGL((combine_kernel<float, true, float>), grid, block, 0, s, P, P + plane, P + 2 * plane,
-                                   P + 3 * plane, (long long)nt, cent, tri, alpha, vertex, scale, (long long)ne, factor,
-                                   (float *)L);
+                                   P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale, (long long)ne, factor,
+                                   (float *)L, (long long)ldl);
             else
                 hipLaunchKernelGGL((combine_kernel<float, true, double>), grid, block, 0, s, P, P + plane,
-                                   P + 2 * plane, P + 3 * plane, (long long)nt, cent, tri, alpha, vertex, scale,
-                                   (long long)ne, factor, (double *)L);
+                                   P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
+                                   (long long)ne, factor, (double *)L, (long long)ldl);
         } else {
             const double *P = (const double *)planes;
             if (out_precision == 32)
                 hipLaunchKernelGGL((combine_kernel<double, false, float>), grid, block, 0, s, P, P + plane,
-                                   P + 2 * plane, P + 3 * plane, (long long)nt, cent, tri, alpha, vertex, scale,
-                                   (long long)ne, factor, (float *)L);
+                                   P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
+                                   (long long)ne, factor, (float *)L, (long long)ldl);
             else
                 hipLaunchKernelGGL((combine_kernel<double, false, double>), grid, block, 0, s, P, P + plane,
-                                   P + 2 * plane, P + 3 * plane, (long long)nt, cent, tri, alpha, vertex, scale,
-                                   (long long)ne, factor, (double *)L);
+                                   P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
+                                   (long long)ne, factor, (double *)L, (long long)ldl);
         }
         PQ_HIP(hipGetLastError());
     });

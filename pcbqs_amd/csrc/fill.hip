@@ -46,14 +46,16 @@ struct NearLists {
 
 // ---- classification ----------------------------------------------------------------------
 
-// The reference rule (static_kernels.hpp:130-136): every product rounded, then summed.
-__device__ __noinline__ int exact_kind(const double *cent, const double *radius, const double *z, int a,
-                                       int b, double near, int all_near, int outer) {
+// The reference rule (static_kernels.hpp:130-136): every product rounded, then summed. It is
+// exactly symmetric in its two triangles (differences are squared, sums commute), so entry (r,c)
+// and (c,r) classify alike without choosing the canonical order.
+__device__ __noinline__ int exact_kind(double ax, double ay, double az, double arad, double bx, double by,
+                                       double bz, double brad, double near, int all_near, int outer) {
 #pragma clang fp contract(off)
-    const double dx = cent[2 * a] - cent[2 * b], dy = cent[2 * a + 1] - cent[2 * b + 1];
-    const double d = z[a] - z[b];
+    const double dx = ax - bx, dy = ay - by;
+    const double d = az - bz;
     const double dist = __builtin_sqrt((dx * dx + dy * dy) + d * d);
-    const double size = radius[a] + radius[b];
+    const double size = arad + brad;
     if (!all_near && !(dist <= near * size)) return -1;
     return dist <= size ? outer : 0;
 }
@@ -200,6 +202,7 @@ __device__ __forceinline__ void wave_count(bool want, unsigned long long *count)
 
 struct FarArgs {
     int r0, rows, n;        // rows [r0, r0+rows) of the n x n matrix; out row 0 is row r0
+    long long ld;           // elements between rows of a plane (>= n)
     double near;
     float near32;
     int all_near, outer, mirror;
@@ -208,17 +211,82 @@ struct FarArgs {
     unsigned *flag;
 };
 
-// Shared classification of the canonical pair (a, b): -1 far, else the analytic level.
-__device__ __forceinline__ int classify(const GeomDev &g, int a, int b, const FarArgs &A, float ra32,
-                                        float rb32) {
-    const double dxd = g.cent[2 * a] - g.cent[2 * b], dyd = g.cent[2 * a + 1] - g.cent[2 * b + 1];
-    const double dzd = g.z[a] - g.z[b];
-    const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+// Classification of entry (r, c): -1 far, else the analytic level. An FP32 prefilter (the
+// differences taken in FP64, then rounded) settles the clearly-far pairs; its relative error is
+// below 1e-6, far inside the 1e-5 margin, so it never decides a pair the exact rule would not.
+struct TriVals {
+    double x, y, z, rad;
+};
+
+__device__ __forceinline__ int classify(const TriVals &r, const TriVals &c, const FarArgs &A) {
+    const float dx = (float)(r.x - c.x), dy = (float)(r.y - c.y), dz = (float)(r.z - c.z);
     const float d2 = dx * dx + dy * dy + dz * dz;
-    const float thr = A.near32 * (ra32 + rb32);
+    const float thr = A.near32 * ((float)r.rad + (float)c.rad);
     const bool certain_far = !A.all_near && d2 > thr * thr * 1.00001f;
     if (certain_far) return -1;
-    return exact_kind(g.cent, g.radius, g.z, a, b, A.near, A.all_near, A.outer);
+    return exact_kind(r.x, r.y, r.z, r.rad, c.x, c.y, c.z, c.rad, A.near, A.all_near, A.outer);
+}
+
+__device__ __forceinline__ TriVals tri_vals(const GeomDev &g, int t) {
+    return TriVals{g.cent[2 * t], g.cent[2 * t + 1], g.z[t], g.radius[t]};
+}
+
+// Near entries are collected per block in LDS (cheap LDS atomics, one per wave and list) and
+// reserved in the global lists once per block and list: device-scope atomics on three shared
+// counters, issued per wave from every XCD, had serialized the whole fill.
+constexpr int NEAR_CAP = 512;   // LDS entries per list and block; overflow goes straight to global
+
+struct BlockNear {
+    unsigned long long *list[2];  // LDS lists
+    unsigned *count;              // LDS: [0],[1] list lengths (may exceed NEAR_CAP), [2] analytic, [3],[4] bases
+    unsigned long long *gbase;    // LDS: reserved global offsets of the two lists
+};
+
+__device__ __forceinline__ void block_near_init(const BlockNear &B) {
+    if (threadIdx.x < 5) B.count[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void block_append(const BlockNear &B, const FarArgs &A, int which, bool want,
+                                             unsigned long long entry) {
+    const unsigned long long mask = __ballot(want);
+    if (!mask) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(B.count + which, (unsigned)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    if (want && slot < NEAR_CAP) B.list[which][slot] = entry;
+    // overflow of the LDS list (dense near rules): straight to the global list
+    wave_append(want && slot >= NEAR_CAP, A.lists.list[which], A.lists.count + which, A.lists.cap, entry);
+}
+
+__device__ __forceinline__ void block_near(const BlockNear &B, const FarArgs &A, bool valid, int kind, int r, int c) {
+    const bool near_entry = valid && kind >= 0 && (r <= c || !A.mirror);
+    const unsigned long long e = ((unsigned long long)(unsigned)r << 32) | (unsigned)c;
+    const bool l1 = kind > 0;
+    block_append(B, A, 0, near_entry && !l1, e);
+    block_append(B, A, 1, near_entry && l1, e);
+    const unsigned long long am = __ballot(A.mirror ? near_entry : (valid && kind >= 0 && r <= c));
+    if (am && __lane_id() == __ffsll((long long)am) - 1) atomicAdd(B.count + 2, (unsigned)__popcll(am));
+}
+
+// After a __syncthreads(): reserve global space once per list, then copy the LDS lists out.
+__device__ __forceinline__ void block_near_flush(const BlockNear &B, const FarArgs &A) {
+    if (threadIdx.x < 2) {
+        const unsigned n = min(B.count[threadIdx.x], (unsigned)NEAR_CAP);
+        B.gbase[threadIdx.x] = n ? atomicAdd(A.lists.count + threadIdx.x, (unsigned long long)n) : 0ull;
+    } else if (threadIdx.x == 2 && B.count[2]) {
+        atomicAdd(A.lists.count + 2, (unsigned long long)B.count[2]);
+    }
+    __syncthreads();
+    for (int which = 0; which < 2; ++which) {
+        const unsigned n = min(B.count[which], (unsigned)NEAR_CAP);
+        for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+            const unsigned long long at = B.gbase[which] + i;
+            if (at < A.lists.cap) A.lists.list[which][at] = B.list[which][i];
+        }
+    }
 }
 
 __device__ __forceinline__ void handle_near(const FarArgs &A, bool valid, int kind, int r, int c) {
@@ -234,29 +302,37 @@ __device__ __forceinline__ void handle_near(const FarArgs &A, bool valid, int ki
 // FP32 local-frame far kernel.
 template <bool SCALAR>
 __global__ __launch_bounds__(256) void far32_kernel(GeomDev g, FarArgs A, float *out) {
+    __shared__ unsigned long long nlist[2][NEAR_CAP], nbase[2];
+    __shared__ unsigned ncount[5];
+    const BlockNear B{{nlist[0], nlist[1]}, ncount, nbase};
+    block_near_init(B);
+    __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * WAVE + lane;
     const bool cvalid = c < A.n;
     const int cc = cvalid ? c : A.n - 1;
     const float4 c0 = g.far32[2 * cc], c1 = g.far32[2 * cc + 1];
+    const TriVals cv = tri_vals(g, cc);
+    __builtin_amdgcn_s_waitcnt(0);
     const int rbase = blockIdx.y * FAR_ROWS_PER_BLOCK + wave * FAR_ROWS_PER_WAVE;
     for (int k = 0; k < FAR_ROWS_PER_WAVE; ++k) {
         const int rl = rbase + k;
-        if (rl >= A.rows) break;
-        const int r = A.r0 + rl;
+        if (rl >= A.rows) break;                       // wave-uniform: the wave leaves together
+        const bool rvalid = true;
+        const int r = A.r0 + rl;                       // wave-uniform: scalar loads below
         const float4 q0 = g.far32[2 * r], q1 = g.far32[2 * r + 1];
+        const TriVals rv = tri_vals(g, r);
         const bool rfirst = r <= c;
-        const int a = rfirst ? r : cc, b = rfirst ? cc : r;
+        const int kind = classify(rv, cv, A);
+        block_near(B, A, cvalid && rvalid, kind, r, c);
+        // canonical pair (a, b) = (min, max): frame offset c_a - c_b, rounded once
+        const float ox = (float)(rv.x - cv.x), oy = (float)(rv.y - cv.y);
+        const float dcx = rfirst ? ox : -ox, dcy = rfirst ? oy : -oy;
+        const float dz = (float)(rv.z - cv.z);
+        const float dz2 = dz * dz;
         const float4 a0 = rfirst ? q0 : c0, a1 = rfirst ? q1 : c1;
         const float4 b0 = rfirst ? c0 : q0, b1 = rfirst ? c1 : q1;
-        const int kind = classify(g, a, b, A, a1.w, b1.w);
-        handle_near(A, cvalid, kind, r, c);
-        // far moments in the pair's local frames
-        const float dcx = (float)(g.cent[2 * a] - g.cent[2 * b]);
-        const float dcy = (float)(g.cent[2 * a + 1] - g.cent[2 * b + 1]);
-        const float dz = (float)(g.z[a] - g.z[b]);
-        const float dz2 = dz * dz;
         const float ax[3] = {a0.x, a0.z, a1.x}, ay[3] = {a0.y, a0.w, a1.y};
         const float bx[3] = {b0.x, b0.z, b1.x}, by[3] = {b0.y, b0.w, b1.y};
         const float w = a1.z * b1.z * (1.0f / 9.0f);
@@ -276,8 +352,8 @@ __global__ __launch_bounds__(256) void far32_kernel(GeomDev g, FarArgs A, float 
                 }
             }
         }
-        if (!cvalid || kind >= 0) continue;
-        const size_t idx = (size_t)rl * A.n + c;
+        if (!cvalid || !rvalid || kind >= 0) continue;
+        const size_t idx = (size_t)rl * A.ld + c;
         const float m0 = G[0] + G[1] + G[2];
         bool ok = isfinite(m0);
         out[idx] = m0;
@@ -296,11 +372,154 @@ __global__ __launch_bounds__(256) void far32_kernel(GeomDev g, FarArgs A, float 
         }
         if (!ok) atomicOr(A.flag, 1u);
     }
+    __syncthreads();
+    block_near_flush(B, A);
+}
+
+// FP32 local-frame far kernel over upper-triangular 64 x 64 tiles: each unordered pair is
+// integrated once, its row r < column c order is canonical without any selection, the forward
+// entries (r, c) are stored straight from registers and the reverse entries (c, r) are staged in
+// LDS (4 planes x 64 x 65 floats; the odd row pitch keeps the column-wise writes conflict-free)
+// and stored as coalesced rows once the tile is done. An entry that must not be stored from here
+// (an analytic pair, the lower half of a diagonal tile, beyond the matrix) is marked with a NaN
+// in the M0 stage; a computed non-finite moment still sets the error flag.
+constexpr int TILE = 64, TILE_PITCH = 65;
+
+__device__ __forceinline__ void tile_coords(unsigned t, int nb, int &br, int &bc) {
+    // row-major enumeration of the upper-triangular tiles: row br holds nb - br tiles
+    const double N = 2.0 * nb + 1.0;
+    int b = (int)floor((N - sqrt(N * N - 8.0 * (double)t)) * 0.5);
+    b = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+    auto first = [nb](int row) { return (long long)row * nb - (long long)row * (row - 1) / 2; };
+    while (b > 0 && first(b) > (long long)t) --b;
+    while (b + 1 < nb && first(b + 1) <= (long long)t) ++b;
+    br = b;
+    bc = b + (int)((long long)t - first(b));
+}
+
+template <bool SCALAR>
+__global__ __launch_bounds__(512) void far32_tile_kernel(GeomDev g, FarArgs A, float *out, int nb) {
+    constexpr int NP = SCALAR ? 1 : 4;
+    constexpr int WAVES = 8, HALF = TILE / 2, ROWS = HALF / WAVES, PITCH = HALF + 1;
+    __shared__ float stage[NP * TILE * PITCH];
+    __shared__ float4 rowf[2 * TILE], colf[2 * TILE];
+    __shared__ TriVals rowd[TILE], cold[TILE];
+    __shared__ unsigned long long nlist[2][NEAR_CAP], nbase[2];
+    __shared__ unsigned ncount[5];
+    const BlockNear B{{nlist[0], nlist[1]}, ncount, nbase};
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int br, bc;
+    tile_coords(blockIdx.x, nb, br, bc);
+    const int R0 = br * TILE, C0 = bc * TILE;
+    // The tile's 64 row and 64 column triangles go through LDS: the row loop below then holds no
+    // global load, so no wait on the (in-order) vector memory counter ever covers the stores of
+    // earlier rows.
+    if (threadIdx.x < 2 * TILE) {
+        const int i = threadIdx.x & (TILE - 1);
+        const bool row = threadIdx.x < TILE;
+        const int t = min((row ? R0 : C0) + i, A.n - 1);
+        float4 *f = row ? rowf : colf;
+        f[2 * i] = g.far32[2 * t];
+        f[2 * i + 1] = g.far32[2 * t + 1];
+        (row ? rowd : cold)[i] = tri_vals(g, t);
+    }
+    block_near_init(B);
+    __syncthreads();
+    const int c = C0 + lane;
+    const bool cvalid = c < A.n;
+    const float4 c0 = colf[2 * lane], c1 = colf[2 * lane + 1];
+    const float bx[3] = {c0.x, c0.z, c1.x}, by[3] = {c0.y, c0.w, c1.y};
+    const float barea = c1.z;
+    const TriVals cv = cold[lane];
+    for (int half = 0; half < 2; ++half) {
+#pragma unroll 2
+        for (int k = 0; k < ROWS; ++k) {
+            const int hl = wave * ROWS + k;            // row within the half
+            const int rl = half * HALF + hl;
+            const int r = R0 + rl;                     // wave-uniform
+            const bool valid = cvalid && r < A.n && r <= c;
+            const float4 q0 = rowf[2 * rl], q1 = rowf[2 * rl + 1];
+            const TriVals rv = rowd[rl];
+            const int kind = valid ? classify(rv, cv, A) : -1;
+            block_near(B, A, valid, kind, r, c);
+            const float dcx = (float)(rv.x - cv.x), dcy = (float)(rv.y - cv.y), dz = (float)(rv.z - cv.z);
+            const float dz2 = dz * dz;
+            const float ax[3] = {q0.x, q0.z, q1.x}, ay[3] = {q0.y, q0.w, q1.y};
+            const float w = q1.z * barea * (1.0f / 9.0f);
+            float G[3] = {0, 0, 0}, H[3] = {0, 0, 0}, Tx[3] = {0, 0, 0}, Ty[3] = {0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float px = ax[i] + dcx, py = ay[i] + dcy;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const float dx = px - bx[j], dy = py - by[j];
+                    const float r2 = dx * dx + dy * dy + dz2;
+                    const float gv = w * __builtin_amdgcn_rsqf(r2);
+                    G[i] += gv;
+                    if (!SCALAR) {
+                        H[j] += gv;
+                        Tx[i] += gv * bx[j]; Ty[i] += gv * by[j];
+                    }
+                }
+            }
+            const bool store = valid && kind < 0;
+            const float m0 = G[0] + G[1] + G[2];
+            float rox = 0, roy = 0, rsx = 0, rsy = 0, rr = 0;
+            if (!SCALAR) {
+                rox = G[0] * ax[0] + G[1] * ax[1] + G[2] * ax[2];
+                roy = G[0] * ay[0] + G[1] * ay[1] + G[2] * ay[2];
+                rsx = H[0] * bx[0] + H[1] * bx[1] + H[2] * bx[2];
+                rsy = H[0] * by[0] + H[1] * by[1] + H[2] * by[2];
+                rr = ax[0] * Tx[0] + ay[0] * Ty[0] + ax[1] * Tx[1] + ay[1] * Ty[1] + ax[2] * Tx[2] + ay[2] * Ty[2];
+            }
+            if (store) {
+                const size_t idx = (size_t)r * A.ld + c;
+                out[idx] = m0;
+                if (!SCALAR) { out[A.plane + idx] = rox; out[2 * A.plane + idx] = roy; out[3 * A.plane + idx] = rr; }
+                const bool ok = isfinite(m0) && (SCALAR || (isfinite(rox) && isfinite(roy) && isfinite(rsx) &&
+                                                            isfinite(rsy) && isfinite(rr)));
+                if (!ok) atomicOr(A.flag, 1u);
+            }
+            // reverse entry (c, r): staged at [c - C0][r - R0 - half * 32]
+            const int sidx = lane * PITCH + hl;
+            stage[sidx] = (store && r != c) ? m0 : __builtin_nanf("");
+            if (!SCALAR) {
+                stage[TILE * PITCH + sidx] = rsx;
+                stage[2 * TILE * PITCH + sidx] = rsy;
+                stage[3 * TILE * PITCH + sidx] = rr;
+            }
+        }
+        __syncthreads();
+        // the staged half: 64 matrix rows C0.. of 32 columns, two rows per wave instruction
+        const int sub = lane >> 5, col = lane & 31;
+        for (int k = 0; k < TILE / (2 * WAVES); ++k) {
+            const int cl = (wave * (TILE / (2 * WAVES)) + k) * 2 + sub;
+            const int sidx = cl * PITCH + col;
+            const float m0 = stage[sidx];
+            if (!isnan(m0)) {
+                const size_t idx = (size_t)(C0 + cl) * A.ld + R0 + half * HALF + col;
+                out[idx] = m0;
+                if (!SCALAR) {
+                    out[A.plane + idx] = stage[TILE * PITCH + sidx];
+                    out[2 * A.plane + idx] = stage[2 * TILE * PITCH + sidx];
+                    out[3 * A.plane + idx] = stage[3 * TILE * PITCH + sidx];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    block_near_flush(B, A);
 }
 
 // FP64 reference-layout far kernel.
 template <bool SCALAR>
 __global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double *out) {
+    __shared__ unsigned long long nlist[2][NEAR_CAP], nbase[2];
+    __shared__ unsigned ncount[5];
+    const BlockNear B{{nlist[0], nlist[1]}, ncount, nbase};
+    block_near_init(B);
+    __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * WAVE + lane;
@@ -310,29 +529,36 @@ __global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double
 #pragma unroll
     for (int i = 0; i < 6; ++i) pc[i] = g.P3[6 * (size_t)cc + i];
     const double areac = g.area[cc];
-    const float radc = (float)g.radius[cc];
+    const TriVals cv = tri_vals(g, cc);
+    // The row loop stores; settle the column loads here so that no wait inside the loop (the
+    // vector memory counter is in order) has to cover the stores of earlier rows.
+    __builtin_amdgcn_s_waitcnt(0);
     const int rbase = blockIdx.y * FAR_ROWS_PER_BLOCK + wave * FAR_ROWS_PER_WAVE;
     for (int k = 0; k < FAR_ROWS_PER_WAVE; ++k) {
         const int rl = rbase + k;
-        if (rl >= A.rows) break;
+        if (rl >= A.rows) break;                       // wave-uniform: the wave leaves together
+        const bool rvalid = true;
         const int r = A.r0 + rl;
         double pr[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) pr[i] = g.P3[6 * (size_t)r + i];
+        const double arear = g.area[r];
+        const TriVals rv = tri_vals(g, r);
         const bool rfirst = r <= c;
-        const int a = rfirst ? r : cc, b = rfirst ? cc : r;
-        const float rada = rfirst ? (float)g.radius[r] : radc, radb = rfirst ? radc : (float)g.radius[r];
-        const int kind = classify(g, a, b, A, rada, radb);
-        handle_near(A, cvalid, kind, r, c);
+        const int kind = classify(rv, cv, A);
+        block_near(B, A, cvalid && rvalid, kind, r, c);
         double pa[6], pb[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) { pa[i] = rfirst ? pr[i] : pc[i]; pb[i] = rfirst ? pc[i] : pr[i]; }
-        const double w = (rfirst ? g.area[r] : areac) * (rfirst ? areac : g.area[r]) / 9;
+        const double w = (rfirst ? arear : areac) * (rfirst ? areac : arear) / 9;
+        const double d = rfirst ? rv.z - cv.z : cv.z - rv.z;
         double m[6];
-        far64<SCALAR>(pa, pb, w, g.z[a] - g.z[b], m);
-        if (!cvalid || kind >= 0) continue;
-        store_entry<double, SCALAR>(out, A.plane, (size_t)rl * A.n + c, r, c, m, A.flag);
+        far64<SCALAR>(pa, pb, w, d, m);
+        if (!cvalid || !rvalid || kind >= 0) continue;
+        store_entry<double, SCALAR>(out, A.plane, (size_t)rl * A.ld + c, r, c, m, A.flag);
     }
+    __syncthreads();
+    block_near_flush(B, A);
 }
 
 // ---- near kernel -----------------------------------------------------------------------------
@@ -340,8 +566,8 @@ __global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double
 template <class T, bool SCALAR, bool LOCAL>
 __global__ __launch_bounds__(256) void near_kernel(GeomDev g, const unsigned long long *list,
                                                   const unsigned long long *count, unsigned long long cap,
-                                                  int level, int r0, int n, int mirror, T *out, size_t plane,
-                                                  unsigned *flag) {
+                                                  int level, int r0, long long ld, int mirror, T *out,
+                                                  size_t plane, unsigned *flag) {
     __shared__ double buf[256 * 6];
     const int nq = g.nq[level];
     const int ppb = 256 / nq;
@@ -368,8 +594,8 @@ __global__ __launch_bounds__(256) void near_kernel(GeomDev g, const unsigned lon
 #pragma unroll
                 for (int k = 0; k < 6; ++k) s[k] += v[k];
             }
-            store_entry<T, SCALAR>(out, plane, (size_t)(r - r0) * n + c, r, c, s, flag);
-            if (mirror && r != c) store_entry<T, SCALAR>(out, plane, (size_t)(c - r0) * n + r, c, r, s, flag);
+            store_entry<T, SCALAR>(out, plane, (size_t)(r - r0) * ld + c, r, c, s, flag);
+            if (mirror && r != c) store_entry<T, SCALAR>(out, plane, (size_t)(c - r0) * ld + r, c, r, s, flag);
         }
         __syncthreads();
     }
@@ -424,7 +650,12 @@ size_t arr_bytes(long long n) { return (size_t)n * sizeof(double); }
 
 void launch_far(pq_fill *f, int precision, bool scalar, const FarArgs &A, void *out, hipStream_t s) {
     dim3 grid((A.n + WAVE - 1) / WAVE, (A.rows + FAR_ROWS_PER_BLOCK - 1) / FAR_ROWS_PER_BLOCK);
-    if (precision == 32) {
+    if (precision == 32 && A.mirror && A.r0 == 0 && A.rows == A.n) {
+        const int nb = (A.n + TILE - 1) / TILE;
+        dim3 tiles((unsigned)((long long)nb * (nb + 1) / 2));
+        if (scalar) hipLaunchKernelGGL(far32_tile_kernel<true>, tiles, dim3(512), 0, s, f->g, A, (float *)out, nb);
+        else hipLaunchKernelGGL(far32_tile_kernel<false>, tiles, dim3(512), 0, s, f->g, A, (float *)out, nb);
+    } else if (precision == 32) {
         if (scalar) hipLaunchKernelGGL(far32_kernel<true>, grid, dim3(256), 0, s, f->g, A, (float *)out);
         else hipLaunchKernelGGL(far32_kernel<false>, grid, dim3(256), 0, s, f->g, A, (float *)out);
     } else {
@@ -444,7 +675,7 @@ void launch_near(pq_fill *f, int precision, bool scalar, int which, int level, c
     const unsigned long long *count = f->counts.as<unsigned long long>() + which;
 #define PQ_NEAR(T, S, LOC)                                                                           \
     hipLaunchKernelGGL((near_kernel<T, S, LOC>), grid, block, 0, s, f->g, list, count, A.lists.cap, level, \
-                       A.r0, A.n, A.mirror, (T *)out, A.plane, A.flag)
+                       A.r0, A.ld, A.mirror, (T *)out, A.plane, A.flag)
     if (precision == 32) {
         if (scalar) PQ_NEAR(float, true, true); else PQ_NEAR(float, false, true);
     } else {
@@ -460,9 +691,9 @@ void check_rule(double near, int outer, pq_fill *f) {
     PQ_REQUIRE(f->g.nq[outer] == 6 * (1 << (2 * outer)), "outer level was not prepared");
 }
 
-FarArgs far_args(pq_fill *f, double near, int outer, int r0, int rows, int mirror, size_t plane) {
+FarArgs far_args(pq_fill *f, double near, int outer, int r0, int rows, int mirror, size_t plane, long long ld) {
     FarArgs A{};
-    A.r0 = r0; A.rows = rows; A.n = f->nt;
+    A.r0 = r0; A.rows = rows; A.n = f->nt; A.ld = ld;
     A.all_near = std::isinf(near) ? 1 : 0;
     A.near = A.all_near ? 0.0 : near;
     A.near32 = (float)A.near;
@@ -563,7 +794,7 @@ int pq_fill_destroy(pq_fill *f) {
 }
 
 int pq_fill_device(pq_fill *f, double near, int outer, int scalar_only, int precision, void *planes,
-                   int64_t *analytic, void *stream_) {
+                   int64_t ld, int64_t *analytic, void *stream_) {
     return guard([&] {
         PQ_REQUIRE(f && planes, "pq_fill_device: null argument");
         PQ_REQUIRE(precision == 32 || precision == 64, "pq_fill_device: precision must be 32 or 64");
@@ -571,10 +802,12 @@ int pq_fill_device(pq_fill *f, double near, int outer, int scalar_only, int prec
         DeviceScope scope(f->device);
         hipStream_t s = stream_ ? (hipStream_t)stream_ : f->stream;
         const long long n = f->nt;
-        const size_t plane = (size_t)n * n;
+        if (ld == 0) ld = n;
+        PQ_REQUIRE(ld >= n, "pq_fill_device: leading dimension below the triangle count");
+        const size_t plane = (size_t)n * ld;
         reserve_lists(f, std::max<unsigned long long>(1ull << 20, 64ull * n));
         for (int attempt = 0; attempt < 2; ++attempt) {
-            FarArgs A = far_args(f, near, outer, 0, (int)n, 1, plane);
+            FarArgs A = far_args(f, near, outer, 0, (int)n, 1, plane, ld);
             PQ_HIP(hipMemsetAsync(f->counts.p, 0, 4 * sizeof(unsigned long long), s));
             PQ_HIP(hipMemsetAsync(f->flag.p, 0, sizeof(unsigned), s));
             PQ_HIP(hipEventRecord(f->ev[0], s));
@@ -646,7 +879,7 @@ int pq_fill_symmetric(pq_fill *f, double near, int outer, int scalar_only, doubl
             const int b = (int)(s & 1);
             const long long r0 = s * rows, m = std::min(rows, n - r0);
             if (used[b]) PQ_HIP(hipStreamWaitEvent(f->stream, copied[b], 0));
-            FarArgs A = far_args(f, near, outer, (int)r0, (int)m, 0, plane);
+            FarArgs A = far_args(f, near, outer, (int)r0, (int)m, 0, plane, n);
             A.lists.count = counts.as<unsigned long long>() + 4 * s;
             launch_far(f, 64, scalar_only, A, strip[b].p, f->stream);
             const unsigned long long *cnt = A.lists.count;
@@ -660,7 +893,7 @@ int pq_fill_symmetric(pq_fill *f, double near, int outer, int scalar_only, doubl
 #define PQ_NEAR_STRIP(S)                                                                                    \
     hipLaunchKernelGGL((near_kernel<double, S, false>), grid, block, 0, f->stream, f->g,                      \
                        f->lists[which].as<unsigned long long>(), cnt + which, A.lists.cap, level, (int)r0,     \
-                       (int)n, 0, strip[b].as<double>(), plane, A.flag)
+                       (long long)n, 0, strip[b].as<double>(), plane, A.flag)
                     if (scalar_only) PQ_NEAR_STRIP(true); else PQ_NEAR_STRIP(false);
 #undef PQ_NEAR_STRIP
                     PQ_HIP(hipGetLastError());

@@ -49,6 +49,12 @@ int pq_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     });
 }
 
+int pq_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if (bytes) PQ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    });
+}
+
 int pq_memset(void *dst, int value, size_t bytes) {
     return guard([&] {
         if (bytes) PQ_HIP(hipMemset(dst, value, bytes));

@@ -190,7 +190,7 @@ def edge_inductance(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None
         call('pq_memcpy_h2d', planes.offset(k * nt * nt), ptr(m), m.nbytes)
     cent = rwg.nodes[rwg.tris].mean(1)
     L = combine_device(planes, 64, rwg, cent, factor, scale, 64, device)
-    out = L.host()
+    out = np.ascontiguousarray(L.host()[:, :L.shape[0]])
     L.close()
     planes.close()
     return out, dict(backend='mi355x', workers=1, seconds=time.perf_counter() - started)

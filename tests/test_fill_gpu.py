@@ -75,7 +75,7 @@ def test_device_fill_matches_oracle(meshes, precision):
     g = F.prepare(nodes, tris, z, 1)
     backend = Mi355xBackend(g, 1, 16384, 0)
     planes, analytic = backend.fill_device(3.0, 1, False, precision)
-    host = planes.host()
+    host = planes.host()[:, :, :len(tris)]
     ref, pairs, ranalytic = F.cr_fill(g, 3.0, 1, False)
     assert analytic == ranalytic
     if precision == 64:
@@ -89,7 +89,7 @@ def test_device_fill_matches_oracle(meshes, precision):
         assert np.array_equal(host[0], host[0].T) and np.array_equal(host[3], host[3].T)
     scalar, analytic_s = backend.fill_device(3.0, 1, True, precision)
     assert analytic_s == ranalytic
-    assert rel(scalar.host()[0].astype(np.float64), ref[0]) < (1e-10 if precision == 64 else 2e-6)
+    assert rel(scalar.host()[0, :, :len(tris)].astype(np.float64), ref[0]) < (1e-10 if precision == 64 else 2e-6)
     far_ms, near_ms, near_pairs = backend.last_timing()
     assert far_ms > 0 and sum(near_pairs) == ranalytic
     backend.close()
@@ -126,7 +126,7 @@ def test_device_combine_matches_oracle(meshes, precision):
     planes, _ = backend.fill_device(3.0, 1, False, precision)
     r = RWG(g.nodes, tris)
     scale = 1 / r.l
-    L = combine_device(planes, precision, r, g.cent, 1e-7, scale, 64).host()
+    L = combine_device(planes, precision, r, g.cent, 1e-7, scale, 64).host()[:, :r.Ne]
     ref_planes, _, _ = F.cr_fill(g, 3.0, 1, False)
     m0, mx, my, rr = ref_planes
     Lref = F.cr_combine(M.RWG(g.nodes, tris), rr, mx, my, m0, 1e-7, scale)

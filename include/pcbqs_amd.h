@@ -128,6 +128,45 @@ int pq_combine_device(int precision, const void *planes_dev, int64_t nt, int64_t
                       const double *scale_dev, int64_t ne, double factor, int out_precision,
                       void *L_dev, int64_t ldl, void *stream);
 
+/* ---- loop-space port solve --------------------------------------------- */
+
+typedef struct pq_loops pq_loops;   /* a board's loop operator on one device      */
+
+/* Project the branch inductance onto the loop basis on the device and keep the
+ * loop operator: Ll = C^T Lb C (FP32 storage, FP64 accumulation) and Rl (CSR,
+ * FP64). C is given in CSC form over loops (c_colptr (n_loops+1), c_rows,
+ * c_sign, c_nnz entries; branch rows >= n_branch_rows carry no inductance);
+ * Lb_dev is the (n_branch_rows, ldb) FP32 branch inductance (RWG edges then via
+ * segments). Loops are ordered free loops first (n_free), then the port loops.
+ * Replaces the dense loop matrices of MQS._assemble / _project_inductance
+ * (src/pcbqs/mqs.py:639-665). */
+int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr, const int32_t *c_rows,
+                    const float *c_sign, int64_t c_nnz, const float *Lb_dev, int64_t ldb, int n_branch_rows,
+                    const int32_t *r_ptr, const int32_t *r_col, const double *r_val, pq_loops **out);
+int pq_loops_destroy(pq_loops *loops);
+
+/* The device loop inductance matrix (FP32, n_loops rows, leading dimension ldl). */
+int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
+
+/* Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
+ * frequency omega: the free loops eliminated, X = -Z_ff^-1 Z_fs, by restarted
+ * lockstep GMRES(restart) with all P columns sharing every operator product
+ * (precond 0: none, 1: block-Jacobi on 64-loop blocks of the free-loop order),
+ * then Zp = Z_ss + Z_sf X. X_host (n_free, 2P) (optional) receives the free
+ * loop currents, real parts first. PQ_ECONVERGE when the true relative
+ * residual exceeds max(5 tol, 1e-13). Replaces direct_ports
+ * (src/pcbqs/mqs_ports.py:52-69) solved as port_matrix's iterative route
+ * (src/pcbqs/mqs_iterative.py:298-367, _lockstep_gmres :109-170) with the
+ * acceptance rule of peec_iterative._gmres (src/pcbqs/peec_iterative.py:100-112). */
+int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *port_cols, int precond, double tol,
+                         int restart, int maxiter, double *Zp, double *X_host, int64_t *iterations,
+                         double *residual);
+
+/* Statistics of the last solve: iterations, operator products, relative
+ * residual, wall seconds. Not in the reference (solve_info's role,
+ * src/pcbqs/mqs_iterative.py:229-230). */
+int pq_loops_last_solve(pq_loops *loops, int64_t *iterations, int64_t *matvecs, double *residual, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256) void far32_kernel(GeomDev g, FarArgs A, float 
 // and stored as coalesced rows once the tile is done. An entry that must not be stored from here
 // (an analytic pair, the lower half of a diagonal tile, beyond the matrix) is marked with a NaN
 // in the M0 stage; a computed non-finite moment still sets the error flag.
-constexpr int TILE = 64, TILE_PITCH = 65;
+constexpr int TILE = 64;
 
 __device__ __forceinline__ void tile_coords(unsigned t, int nb, int &br, int &bc) {
     // row-major enumeration of the upper-triangular tiles: row br holds nb - br tiles

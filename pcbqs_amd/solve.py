@@ -1,0 +1,125 @@
+"""Device-resident port solve of one board: fill -> RWG combine -> loop projection -> GMRES.
+
+The pipeline keeps everything in HBM: the moment planes (FP32 far field, FP64 analytic near
+pairs, local frames), the branch inductance Lb (RWG edges and via segments, FP32), the loop
+operator Ll = C^T Lb C and Rl = C^T R C, and the Krylov basis. Port impedances follow the
+reference's copper-port reduction (src/pcbqs/mqs_ports.py:35-86): Zp = Z_ss + Z_sf X with
+X = -Z_ff^-1 Z_fs, here by lockstep GMRES (src/pcbqs/mqs_iterative.py:109-170, 298-367);
+R = Re Zp and L = Im Zp / w (rl_equivalent, src/pcbqs/mqs_ports.py:72-86).
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import DeviceArray, call, ptr
+from .backend import Mi355xBackend, padded
+from .geometry import MU0, RWG, FillGeometry
+from .loops import LoopModel
+
+PRECONDITIONERS = {'none': 0, 'block-jacobi': 1}
+
+
+class PortSolver:
+    """One board on one device. `ports(freqs)` returns Zp (F, P, P) complex and per-frequency info."""
+
+    def __init__(self, board, device=0, near=3.0, outer_levels=1, precision=32, ideal_terminals=True):
+        t0 = time.perf_counter()
+        self.device = device
+        self.model = LoopModel(board, ideal_terminals=ideal_terminals)
+        m = self.model
+        t1 = time.perf_counter()
+        g = FillGeometry.prepare(board.nodes, board.tris, board.z_of, outer_levels)
+        self.backend = Mi355xBackend(g, 1, 16384, device)
+        planes, self.analytic = self.backend.fill_device(near, outer_levels, False, precision)
+        rwg = RWG(g.nodes, board.tris)                        # the fill's centred frame, same edges
+        ne, nv = m.ne, m.nv
+        nbr = ne + nv
+        ldb = padded(nbr)
+        Lb = DeviceArray((nbr, ldb), np.float32, device).zero()
+        tri, alpha, vertex = rwg.combine_arrays()
+        scale = 1.0 / rwg.l
+        dev = [DeviceArray.from_host(a, device) for a in (tri, alpha, vertex, scale,
+                                                          np.ascontiguousarray(g.cent, np.float64))]
+        call('pq_combine_device', int(precision), planes.ptr, len(g.tris), planes.shape[-1], dev[4].ptr, dev[0].ptr,
+             dev[1].ptr, dev[2].ptr, dev[3].ptr, ne, MU0 / (4 * np.pi), 32, Lb.ptr, ldb, None)
+        call('pq_synchronize', device)
+        for a in dev:
+            a.close()
+        planes.close()
+        for k in range(nv):                                   # the via block
+            row = np.ascontiguousarray(m.Lv[k], np.float32)
+            call('pq_memcpy_h2d', Lb.offset((ne + k) * ldb + ne), ptr(row), row.nbytes)
+        C = m.basis.C.tocsc()
+        colptr = np.ascontiguousarray(C.indptr, np.int32)
+        rows = np.ascontiguousarray(C.indices, np.int32)
+        sign = np.ascontiguousarray(C.data, np.float32)
+        Rl = m.Rl.tocsr()
+        Rl.sort_indices()
+        rptr, rcol, rval = (np.ascontiguousarray(Rl.indptr, np.int32), np.ascontiguousarray(Rl.indices, np.int32),
+                            np.ascontiguousarray(Rl.data, np.float64))
+        handle = ctypes.c_void_p()
+        call('pq_loops_create', device, m.n_loops, m.n_free, ptr(colptr), ptr(rows), ptr(sign), len(rows), Lb.ptr,
+             ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
+        call('pq_synchronize', device)
+        Lb.close()
+        self._handle = handle
+        self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
+        self.setup_seconds = dict(topology=t1 - t0, device=time.perf_counter() - t1)
+
+    @property
+    def n_ports(self):
+        return len(self.port_cols)
+
+    def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False):
+        """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds)."""
+        P = self.n_ports
+        f = np.atleast_1d(np.asarray(freqs, float))
+        if f.ndim != 1 or not np.isfinite(f).all() or (f < 0).any():
+            raise ValueError('port frequencies must be finite and nonnegative')
+        Z = np.zeros((len(f), P, P), complex)
+        X = np.zeros((len(f), self.model.n_free, 2 * P)) if currents else None
+        info = []
+        kind = PRECONDITIONERS[precond]
+        for i, fi in enumerate(f):
+            zp = np.zeros((P, P, 2))
+            its, res = ctypes.c_int64(), ctypes.c_double()
+            xh = X[i] if currents else None
+            call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, P, ptr(self.port_cols), kind, float(tol),
+                 int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None, ctypes.byref(its),
+                 ctypes.byref(res))
+            Z[i] = zp[..., 0] + 1j * zp[..., 1]
+            seconds, mv = ctypes.c_double(), ctypes.c_int64()
+            call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))
+            info.append(dict(frequency_Hz=float(fi), iterations=int(its.value), matvecs=int(mv.value),
+                             relative_residual=float(res.value), seconds=float(seconds.value), preconditioner=precond))
+        return (Z, info, X) if currents else (Z, info)
+
+    def close(self):
+        if getattr(self, '_handle', None) is not None and _lib._lib is not None:
+            _lib._lib.pq_loops_destroy(self._handle)
+        self._handle = None
+        if getattr(self, 'backend', None) is not None:
+            self.backend.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:                                     # noqa: BLE001 (interpreter shutdown)
+            pass
+
+
+def rl_equivalent(frequency, Z, tolerance=1e-7):
+    """(R, L) of one port matrix, each made PSD (src/pcbqs/mqs_ports.py:72-86)."""
+    if frequency <= 0:
+        raise ValueError('RL extraction needs a positive reference frequency')
+    mats = []
+    for A in (Z.real, Z.imag / (2 * np.pi * frequency)):
+        A = (A + A.T) / 2
+        e, V = np.linalg.eigh(A)
+        scale = max(abs(e).max(), 1e-30)
+        if e.min() < -tolerance * scale:
+            raise ValueError('non-passive copper reduction; refine/check topology')
+        mats.append((V * np.maximum(e, scale * 1e-12)) @ V.T)
+    return mats[0], mats[1]

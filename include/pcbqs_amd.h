@@ -151,7 +151,8 @@ int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
 /* Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
  * frequency omega: the free loops eliminated, X = -Z_ff^-1 Z_fs, by restarted
  * lockstep GMRES(restart) with all P columns sharing every operator product
- * (precond 0: none, 1: block-Jacobi on 64-loop blocks of the free-loop order),
+ * (precond 0: none, 1: block-Jacobi on 64-loop blocks of the free-loop order,
+ * 2: two-level: the coarse space of pq_loops_set_coarse, then block-Jacobi),
  * then Zp = Z_ss + Z_sf X. X_host (n_free, 2P) (optional) receives the free
  * loop currents, real parts first. PQ_ECONVERGE when the true relative
  * residual exceeds max(5 tol, 1e-13). Replaces direct_ports
@@ -161,6 +162,14 @@ int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
 int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *port_cols, int precond, double tol,
                          int restart, int maxiter, double *Zp, double *X_host, int64_t *iterations,
                          double *residual);
+
+/* The coarse space of the two-level preconditioner (precond 2 of
+ * pq_loops_solve_ports): Phi (n_free, k) FP64 host, row-major. Each solve then
+ * applies y = Phi c + BJ(x - A Phi c), c = (Phi^T A Phi)^-1 Phi^T x (coarse
+ * correction first, block-Jacobi on the remainder). k = 0 removes it. No
+ * counterpart in the reference (its iterative path uses a graph-Laplacian
+ * preconditioner, src/pcbqs/mqs_precond.py:27-59). */
+int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi);
 
 /* Statistics of the last solve: iterations, operator products, relative
  * residual, wall seconds. Not in the reference (solve_info's role,

@@ -21,6 +21,8 @@
 
 namespace pq {
 
+using cplx = std::complex<double>;
+
 // ---- projection: Ll = C^T Lb C ------------------------------------------------------------
 
 // U[k, j] = sum_t s_t Lb[k, row_t] over the branches t of loop j (C in CSC: colptr, rows, signs).
@@ -136,6 +138,71 @@ __global__ __launch_bounds__(256) void gemv_kernel(const float *Ll, long long ld
             }
         }
     }
+}
+
+// Ll := (Ll + Ll^T) / 2 on the upper triangle and its mirror (the loop matrices' symmetric part,
+// as the reference takes it: src/pcbqs/mqs_iterative.py:244-257). Entry (i, j) and (j, i) then
+// agree bitwise, which lets the product read columns of Ll as rows.
+__global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = e / n, j = e % n;
+    if (i >= n || j <= i) return;
+    const float v = 0.5f * (Ll[i * ldl + j] + Ll[j * ldl + i]);
+    Ll[i * ldl + j] = v;
+    Ll[j * ldl + i] = v;
+}
+
+// Partial products of the symmetric Ll: part[s][row][r] = sum over columns c of chunk s of
+// Ll[c][row] X[c][r]. One thread per row, so each column read is 256 consecutive floats of row c
+// (coalesced); the chunk's X rows are staged in LDS and read as broadcasts.
+constexpr int GEMV_SPLIT = 16, GEMV_ROWS = 256, GEMV_CHUNK = 640;
+template <int NR>
+__global__ __launch_bounds__(256) void gemv_sym_kernel(const float *Ll, long long ldl, int n, const double *X,
+                                                      int chunk, double *part) {
+    __shared__ double xs[GEMV_CHUNK * NR];
+    const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
+    const int c0 = blockIdx.y * chunk, c1 = min(n, c0 + chunk);
+    for (int e = threadIdx.x; e < (c1 - c0) * NR; e += 256) xs[e] = X[(size_t)c0 * NR + e];
+    __syncthreads();
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    if (row < n) {
+        const float *col = Ll + row;
+        int c = c0;
+        for (; c + 4 <= c1; c += 4) {
+            const double m0 = col[(size_t)c * ldl], m1 = col[(size_t)(c + 1) * ldl];
+            const double m2 = col[(size_t)(c + 2) * ldl], m3 = col[(size_t)(c + 3) * ldl];
+            const double *x = xs + (c - c0) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += m0 * x[r] + m1 * x[NR + r] + m2 * x[2 * NR + r] + m3 * x[3 * NR + r];
+        }
+        for (; c < c1; ++c) {
+            const double m = col[(size_t)c * ldl];
+            const double *x = xs + (c - c0) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += m * x[r];
+        }
+        double *o = part + ((size_t)blockIdx.y * n + row) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) o[r] = acc[r];
+    }
+}
+
+// W_re -= w sum_s part_im, W_im += w sum_s part_re (chunks summed in order).
+template <int NR>
+__global__ void gemv_epilogue_kernel(const double *part, int nsplit, int n, double omega, double *W) {
+    constexpr int P = NR / 2;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * P) return;
+    const int row = e / P, p = e % P;
+    double yr = 0, yi = 0;
+    for (int q = 0; q < nsplit; ++q) {
+        yr += part[((size_t)q * n + row) * NR + p];
+        yi += part[((size_t)q * n + row) * NR + P + p];
+    }
+    W[(size_t)row * NR + p] -= omega * yi;
+    W[(size_t)row * NR + P + p] += omega * yr;
 }
 
 // ---- block-Jacobi --------------------------------------------------------------------------
@@ -376,6 +443,232 @@ __global__ __launch_bounds__(256) void port_reduce_kernel(const float *Ll, long 
     if (threadIdx.x == 0) { out[2 * blockIdx.x] = sr[0]; out[2 * blockIdx.x + 1] = si[0]; }
 }
 
+
+// ---- two-level (coarse space) preconditioner -------------------------------------------------
+
+// G[a][b] = sum_i A[i][a] B[i][b] (A (n, ka), B (n, kb), row-major with leading dims), FP64.
+__global__ __launch_bounds__(256) void gram_kernel(const double *A, int lda, int ka, const double *B, int ldb, int kb,
+                                                  int n, double *G) {
+    __shared__ double sred[256];
+    const int a = blockIdx.x / kb, b = blockIdx.x % kb;
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) s += A[(size_t)i * lda + a] * B[(size_t)i * ldb + b];
+    sred[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sred[threadIdx.x] += sred[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) G[blockIdx.x] = sred[0];
+}
+
+// t[c][p] = sum_i Phi[i][c] v[i][p] (complex v, real Phi): partial sums per row block.
+__global__ __launch_bounds__(256) void coarse_restrict_kernel(const double *Phi, int k, const double *V, int n, int P,
+                                                             double *partial) {
+    __shared__ double sr[256], si[256];
+    const int NR = 2 * P;
+    const int c = blockIdx.y;
+    for (int p = 0; p < P; ++p) {
+        double re = 0, im = 0;
+        for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+            const double f = Phi[(size_t)i * k + c];
+            re += f * V[(size_t)i * NR + p];
+            im += f * V[(size_t)i * NR + P + p];
+        }
+        sr[threadIdx.x] = re; si[threadIdx.x] = im;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st) { sr[threadIdx.x] += sr[threadIdx.x + st]; si[threadIdx.x] += si[threadIdx.x + st]; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double *o = partial + (((size_t)blockIdx.x * k + c) * P + p) * 2;
+            o[0] = sr[0]; o[1] = si[0];
+        }
+        __syncthreads();
+    }
+}
+
+// c = Ainv t: t (k, P) complex summed from nblk partials, Ainv (k, k) complex; one thread per (row, p).
+__global__ void coarse_solve_kernel(const double *partial, int nblk, const double *Ainv, int k, int P, double *cvec) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= k * P) return;
+    const int a = e / P, p = e % P;
+    double yr = 0, yi = 0;
+    for (int b = 0; b < k; ++b) {
+        double tr = 0, ti = 0;
+        for (int q = 0; q < nblk; ++q) {
+            tr += partial[(((size_t)q * k + b) * P + p) * 2];
+            ti += partial[(((size_t)q * k + b) * P + p) * 2 + 1];
+        }
+        const double ar = Ainv[2 * ((size_t)a * k + b)], ai = Ainv[2 * ((size_t)a * k + b) + 1];
+        yr += ar * tr - ai * ti;
+        yi += ar * ti + ai * tr;
+    }
+    cvec[2 * e] = yr; cvec[2 * e + 1] = yi;
+}
+
+// R = V - (RPhi + j w LPhi) c   and   Y = Phi c  (complex c (k, P)).
+__global__ __launch_bounds__(256) void coarse_prolong_kernel(const double *Phi, const double *RPhi, const double *LPhi,
+                                                            int k, const double *cvec, double omega, const double *V,
+                                                            int n, int P, double *R, double *Y) {
+    const int NR = 2 * P;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (size_t)n * P) return;
+    const int i = (int)(e / P), p = (int)(e % P);
+    double ar = 0, ai = 0, yr = 0, yi = 0;
+    for (int c = 0; c < k; ++c) {
+        const double cr = cvec[2 * (c * P + p)], ci = cvec[2 * (c * P + p) + 1];
+        const double r = RPhi[(size_t)i * k + c], l = omega * LPhi[(size_t)i * k + c], f = Phi[(size_t)i * k + c];
+        ar += r * cr - l * ci;
+        ai += r * ci + l * cr;
+        yr += f * cr;
+        yi += f * ci;
+    }
+    R[(size_t)i * NR + p] = V[(size_t)i * NR + p] - ar;
+    R[(size_t)i * NR + P + p] = V[(size_t)i * NR + P + p] - ai;
+    Y[(size_t)i * NR + p] = yr;
+    Y[(size_t)i * NR + P + p] = yi;
+}
+
+// Y[n,p] = X[n,p] / sqrt(nrm[p]) with nrm[p] = (|X_p|^2, 0) as dots_reduce leaves it (0 -> 0).
+__global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n, int P, double *Y) {
+    const int NR = 2 * P;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * NR) return;
+    const int c = (int)(e % NR) % P;
+    const double s = nrm[2 * c];
+    Y[e] = s > 0 ? X[e] / sqrt(s) : 0.0;
+}
+
+// Restriction t = Phi^T v (Phi real FP32 (n, k), v complex (n, 2P)): per row block, each thread
+// owns outputs (c, r) = e / NR, e % NR for e = tid, tid + 256, ... and sums over the block's rows.
+constexpr int CR_MAX = 16;  // outputs per thread: k * NR <= 256 * CR_MAX
+__global__ __launch_bounds__(256) void coarse_restrict_kernel2(const float *Phi, int k, const double *V, int n, int NR,
+                                                              int rows_per_block, double *partial) {
+    const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
+    const int total = k * NR;
+    double acc[CR_MAX];
+#pragma unroll
+    for (int q = 0; q < CR_MAX; ++q) acc[q] = 0;
+    for (int i = i0; i < i1; ++i) {
+        const float *prow = Phi + (size_t)i * k;
+        const double *vrow = V + (size_t)i * NR;
+#pragma unroll
+        for (int q = 0; q < CR_MAX; ++q) {
+            const int e = threadIdx.x + q * 256;
+            if (e < total) acc[q] += (double)prow[e / NR] * vrow[e % NR];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CR_MAX; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < total) partial[(size_t)blockIdx.x * total + e] = acc[q];
+    }
+}
+
+// Restriction as a tiled product: partial[chunk][c][r] = sum_{i in chunk} Phit[c][i] V[i][r] for
+// 32 coarse columns per block and 512 rows per chunk (V chunk staged in LDS, Phit = Phi^T FP32).
+constexpr int CR_COLS = 32, CR_ROWS = 512;
+template <int NR>
+__global__ __launch_bounds__(256) void coarse_restrict_kernel3(const float *Phit, int k, const double *V, int n,
+                                                              double *partial) {
+    __shared__ double vs[CR_ROWS * NR];
+    const int c0 = blockIdx.x * CR_COLS, i0 = blockIdx.y * CR_ROWS, i1 = min(n, i0 + CR_ROWS);
+    for (int e = threadIdx.x; e < (i1 - i0) * NR; e += 256) vs[e] = V[(size_t)i0 * NR + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < CR_COLS * NR; e += 256) {
+        const int cl = e / NR, r = e % NR, c = c0 + cl;
+        if (c >= k) continue;
+        const float *prow = Phit + (size_t)c * n;
+        double acc = 0;
+        for (int i = i0; i < i1; ++i) acc += (double)prow[i] * vs[(i - i0) * NR + r];
+        partial[((size_t)blockIdx.y * k + c) * NR + r] = acc;
+    }
+}
+
+// c = Ainv t, one wave per output (a, p), lanes over b.
+__global__ __launch_bounds__(256) void coarse_solve_kernel3(const double *t, const double *Ainv, int k, int P,
+                                                           double *cvec) {
+    const int NR = 2 * P;
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= k * P) return;
+    const int a = e / P, p = e % P;
+    double yr = 0, yi = 0;
+    for (int b = lane; b < k; b += 64) {
+        const double tr = t[b * NR + p], ti = t[b * NR + P + p];
+        const double ar = Ainv[2 * ((size_t)a * k + b)], ai = Ainv[2 * ((size_t)a * k + b) + 1];
+        yr += ar * tr - ai * ti;
+        yi += ar * ti + ai * tr;
+    }
+    for (int off = 32; off > 0; off >>= 1) { yr += __shfl_down(yr, off); yi += __shfl_down(yi, off); }
+    if (lane == 0) { cvec[2 * e] = yr; cvec[2 * e + 1] = yi; }
+}
+
+// t[e] = sum over the row-block partials, in block order (e = c * NR + r).
+__global__ void coarse_sum_kernel(const double *partial, int nblk, int total, double *t) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    double s = 0;
+    for (int q = 0; q < nblk; ++q) s += partial[(size_t)q * total + e];
+    t[e] = s;
+}
+
+// c = Ainv t (complex; t (k, NR) real/imaginary columns), cvec (k, P) complex interleaved.
+__global__ void coarse_solve_kernel2(const double *t, const double *Ainv, int k, int P, double *cvec) {
+    const int NR = 2 * P;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= k * P) return;
+    const int a = e / P, p = e % P;
+    double yr = 0, yi = 0;
+    for (int b = 0; b < k; ++b) {
+        const double tr = t[b * NR + p], ti = t[b * NR + P + p];
+        const double ar = Ainv[2 * ((size_t)a * k + b)], ai = Ainv[2 * ((size_t)a * k + b) + 1];
+        yr += ar * tr - ai * ti;
+        yi += ar * ti + ai * tr;
+    }
+    cvec[2 * e] = yr; cvec[2 * e + 1] = yi;
+}
+
+// R = V - (RPhi + j w LPhi) c and Y = Phi c, one wave per row, lanes over the k coarse columns.
+__global__ __launch_bounds__(256) void coarse_prolong_kernel2(const float *Phi, const float *RPhi, const float *LPhi,
+                                                             int k, const double *cvec, double omega,
+                                                             const double *V, int n, int P, double *R, double *Y) {
+    const int NR = 2 * P;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    for (int p = 0; p < P; ++p) {
+        double ar = 0, ai = 0, yr = 0, yi = 0;
+        for (int c = lane; c < k; c += 64) {
+            const double cr = cvec[2 * (c * P + p)], ci = cvec[2 * (c * P + p) + 1];
+            const double r = RPhi[(size_t)i * k + c], l = omega * (double)LPhi[(size_t)i * k + c];
+            const double f = Phi[(size_t)i * k + c];
+            ar += r * cr - l * ci;
+            ai += r * ci + l * cr;
+            yr += f * cr;
+            yi += f * ci;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            ar += __shfl_down(ar, off); ai += __shfl_down(ai, off);
+            yr += __shfl_down(yr, off); yi += __shfl_down(yi, off);
+        }
+        if (lane == 0) {
+            R[(size_t)i * NR + p] = V[(size_t)i * NR + p] - ar;
+            R[(size_t)i * NR + P + p] = V[(size_t)i * NR + P + p] - ai;
+            Y[(size_t)i * NR + p] = yr;
+            Y[(size_t)i * NR + P + p] = yi;
+        }
+    }
+}
+
+// Y += X
+__global__ void vec_add_kernel(const double *X, size_t count, double *Y) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) Y[e] += X[e];
+}
+
 }  // namespace pq
 
 using namespace pq;
@@ -391,6 +684,14 @@ struct pq_loops {
     hipStream_t stream = nullptr;
     int nblk = 0;
     double omega_bj = -1;
+    // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
+    int kc = 0;
+    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart;
+    std::vector<double> Rc, Lc;
+    double omega_c = -1;
+    DevBuf Hd;                    // Arnoldi coefficients per step (device)
+    double *Hh = nullptr;         // their pinned host copy
+    size_t Hh_bytes = 0;
     // the last solve
     int iterations = 0, restarts = 0;
     double residual = 0, seconds = 0, matvec_ms = 0;
@@ -414,29 +715,125 @@ template <class F> void dispatch_nr(int NR, F &&f) {
 // W = (Rl + j w Ll) X on the free block.
 void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P) {
     const int NR = 2 * P, n = L->nf;
+    const int chunk = std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT));
+    const int nsplit = (n + chunk - 1) / chunk;
+    L->gpart.reserve((size_t)nsplit * n * NR * 8);
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
         hipLaunchKernelGGL(spmv_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
                            L->rcol.as<int>(), L->rval.as<double>(), n, X, W);
-        constexpr int RPB = 4 * (N >= 16 ? 1 : 16 / N);
-        hipLaunchKernelGGL(gemv_kernel<N>, dim3((n + RPB - 1) / RPB), dim3(256), 0, L->stream, L->Ll.as<float>(),
-                           L->ldl, n, X, omega, W);
+        if (N <= 16) {
+            hipLaunchKernelGGL(gemv_sym_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
+                               L->Ll.as<float>(), L->ldl, n, X, chunk, L->gpart.as<double>());
+            hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
+                               L->gpart.as<double>(), nsplit, n, omega, W);
+        } else {
+            constexpr int RPB = 4 * (N >= 16 ? 1 : 16 / N);
+            hipLaunchKernelGGL(gemv_kernel<N>, dim3((n + RPB - 1) / RPB), dim3(256), 0, L->stream, L->Ll.as<float>(),
+                               L->ldl, n, X, omega, W);
+        }
     });
     PQ_HIP(hipGetLastError());
     ++L->matvecs;
 }
 
+void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P);
+
+void apply_bj(pq_loops *L, const double *X, double *Y, int P);
+
 void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P) {
     const int n = L->nf, NR = 2 * P;
+    if (kind == 2) {
+        // deflated two-level: c = Ac^-1 Phi^T x; y = Phi c + BJ(x - A Phi c)
+        const int k = L->kc;
+        const int nchunk = (n + CR_ROWS - 1) / CR_ROWS;
+        dispatch_nr(NR, [&](auto nr) {
+            constexpr int N = decltype(nr)::value;
+            hipLaunchKernelGGL(coarse_restrict_kernel3<N>, dim3((k + CR_COLS - 1) / CR_COLS, nchunk), dim3(256), 0,
+                               L->stream, L->Phit.as<float>(), k, X, n, L->cpart.as<double>());
+        });
+        hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * NR + 255) / 256), dim3(256), 0, L->stream,
+                           L->cpart.as<double>(), nchunk, k * NR, L->ctot.as<double>());
+        hipLaunchKernelGGL(coarse_solve_kernel3, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
+                           L->ctot.as<double>(), L->Ainv.as<double>(), k, P, L->cvec.as<double>());
+        hipLaunchKernelGGL(coarse_prolong_kernel2, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->Phi.as<float>(),
+                           L->RPhi.as<float>(), L->LPhi.as<float>(), k, L->cvec.as<double>(), L->omega_c, X, n, P,
+                           L->T1.as<double>(), L->T2.as<double>());
+        PQ_HIP(hipGetLastError());
+        apply_bj(L, L->T1.as<double>(), Y, P);
+        const size_t count = (size_t)n * NR;
+        hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream,
+                           L->T2.as<double>(), count, Y);
+        PQ_HIP(hipGetLastError());
+        return;
+    }
     if (kind == 0) {
         PQ_HIP(hipMemcpyAsync(Y, X, (size_t)n * NR * 8, hipMemcpyDeviceToDevice, L->stream));
         return;
     }
-    dispatch_nr(NR, [&](auto nr) {
+    apply_bj(L, X, Y, P);
+}
+
+void apply_bj(pq_loops *L, const double *X, double *Y, int P) {
+    dispatch_nr(2 * P, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk), dim3(256), 0, L->stream, L->inv.as<float>(), n, X, Y);
+        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk), dim3(256), 0, L->stream, L->inv.as<float>(), L->nf, X, Y);
     });
     PQ_HIP(hipGetLastError());
+}
+
+// Inverse of a complex k x k matrix (row-major, interleaved re/im) by Gauss-Jordan with partial
+// pivoting; a pivot below eps * max|a| is dropped (its row and column left zero): the coarse
+// problem's near-null directions are then not corrected rather than amplified.
+void small_inverse(std::vector<cplx> &a, int k, std::vector<cplx> &inv) {
+    double amax = 0;
+    for (auto &v : a) amax = std::max(amax, std::abs(v));
+    const double eps = 1e-12 * std::max(amax, 1e-300);
+    std::vector<int> perm(k);
+    std::vector<bool> dropped(k, false);
+    for (int c = 0; c < k; ++c) {
+        int best = c;
+        double bv = std::abs(a[(size_t)c * k + c]);
+        for (int i = c + 1; i < k; ++i)
+            if (std::abs(a[(size_t)i * k + c]) > bv) { bv = std::abs(a[(size_t)i * k + c]); best = i; }
+        perm[c] = best;
+        if (best != c)
+            for (int j = 0; j < k; ++j) std::swap(a[(size_t)c * k + j], a[(size_t)best * k + j]);
+        if (bv <= eps) {
+            dropped[c] = true;
+            for (int j = 0; j < k; ++j) a[(size_t)c * k + j] = 0;
+            for (int i = 0; i < k; ++i) a[(size_t)i * k + c] = 0;
+            continue;
+        }
+        const cplx piv = a[(size_t)c * k + c];
+        a[(size_t)c * k + c] = 1;
+        for (int j = 0; j < k; ++j) a[(size_t)c * k + j] /= piv;
+        for (int i = 0; i < k; ++i) {
+            if (i == c) continue;
+            const cplx f = a[(size_t)i * k + c];
+            if (f == cplx(0)) continue;
+            a[(size_t)i * k + c] = 0;
+            for (int j = 0; j < k; ++j) a[(size_t)i * k + j] -= f * a[(size_t)c * k + j];
+        }
+    }
+    for (int c = k - 1; c >= 0; --c)
+        if (perm[c] != c)
+            for (int i = 0; i < k; ++i) std::swap(a[(size_t)i * k + c], a[(size_t)i * k + perm[c]]);
+    inv = a;
+}
+
+void coarse_at(pq_loops *L, double omega) {
+    if (L->omega_c == omega) return;
+    const int k = L->kc;
+    std::vector<cplx> a((size_t)k * k), inv;
+    for (size_t e = 0; e < a.size(); ++e) a[e] = cplx(L->Rc[e], omega * L->Lc[e]);
+    small_inverse(a, k, inv);
+    std::vector<double> flat(2 * inv.size());
+    for (size_t e = 0; e < inv.size(); ++e) { flat[2 * e] = inv[e].real(); flat[2 * e + 1] = inv[e].imag(); }
+    L->Ainv.reserve(flat.size() * 8);
+    PQ_HIP(hipMemcpyAsync(L->Ainv.p, flat.data(), flat.size() * 8, hipMemcpyHostToDevice, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    L->omega_c = omega;
 }
 
 // h[i][p] for i < nv: conj(V_i) . W, synchronously copied to host.
@@ -477,7 +874,6 @@ void update(pq_loops *L, const double *V, size_t vstride, int nv, const double *
     PQ_HIP(hipGetLastError());
 }
 
-using cplx = std::complex<double>;
 
 }  // namespace
 
@@ -517,6 +913,11 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
             hipLaunchKernelGGL(proj_right_kernel, dim3(n_loops), dim3(256), 0, L->stream, U.as<float>(), ldu,
                                n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, L->Ll.as<float>(),
                                L->ldl);
+            {
+                const long long cnt = (long long)n_loops * n_loops;
+                hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
+                                   L->Ll.as<float>(), L->ldl, n_loops);
+            }
             PQ_HIP(hipGetLastError());
             const int rnnz = r_ptr[n_loops];
             L->rptr.reserve((n_loops + 1) * 4); L->rcol.reserve(std::max(rnnz, 1) * 4); L->rval.reserve(std::max(rnnz, 1) * 8);
@@ -540,8 +941,11 @@ int pq_loops_destroy(pq_loops *L) {
         DeviceScope scope(L->device);
         if (L->stream) (void)hipStreamSynchronize(L->stream);
         for (DevBuf *b : {&L->Ll, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->singular, &L->V, &L->W, &L->Z,
-                          &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal})
+                          &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
+                          &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit})
             b->release();
+        L->Hd.release();
+        if (L->Hh) (void)hipHostFree(L->Hh);
         if (L->stream) (void)hipStreamDestroy(L->stream);
         delete L;
     });
@@ -562,7 +966,8 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
     return guard([&] {
         PQ_REQUIRE(L && port_cols && Zp, "pq_loops_solve_ports: null argument");
         PQ_REQUIRE(P >= 1 && tol > 0 && tol < 1 && restart >= 1 && maxiter >= 1, "pq_loops_solve_ports: invalid options");
-        PQ_REQUIRE(precond == 0 || precond == 1, "pq_loops_solve_ports: preconditioner 0 (none) or 1 (block Jacobi)");
+        PQ_REQUIRE(precond >= 0 && precond <= 2, "pq_loops_solve_ports: preconditioner 0 (none), 1 (block Jacobi) or 2 (two-level)");
+        PQ_REQUIRE(precond != 2 || L->kc > 0, "pq_loops_solve_ports: two-level preconditioner without a coarse space");
         for (int p = 0; p < P; ++p)
             PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl, "pq_loops_solve_ports: port loop out of range");
         DeviceScope scope(L->device);
@@ -575,6 +980,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
         L->V.reserve((m + 1) * vsz * 8);
         L->W.reserve(vsz * 8); L->Z.reserve(vsz * 8); L->X.reserve(vsz * 8); L->B.reserve(vsz * 8); L->R.reserve(vsz * 8);
         L->partial.reserve((size_t)RED_BLOCKS * (m + 2) * P * 2 * 8);
+        L->Hd.reserve(8);
         L->h.reserve((size_t)(m + 2) * P * 2 * 8);
         L->scal.reserve(P * 8);
         double *V = L->V.as<double>(), *W = L->W.as<double>(), *Z = L->Z.as<double>(), *X = L->X.as<double>();
@@ -583,7 +989,15 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
                                L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, B);
             PQ_HIP(hipGetLastError());
-            if (precond == 1 && L->omega_bj != omega) {
+            if (precond == 2) {
+                coarse_at(L, omega);
+                const int nchunk = (n + CR_ROWS - 1) / CR_ROWS;
+                L->cpart.reserve((size_t)nchunk * L->kc * NR * 8);
+                L->cvec.reserve((size_t)L->kc * P * 2 * 8);
+                L->ctot.reserve((size_t)L->kc * NR * 8);
+                L->T1.reserve(vsz * 8); L->T2.reserve(vsz * 8);
+            }
+            if (precond >= 1 && L->omega_bj != omega) {
                 L->nblk = (n + BS - 1) / BS;
                 L->inv.reserve((size_t)L->nblk * BS * BS * 2 * 4);
                 PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
@@ -601,12 +1015,52 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             std::vector<double> bnorm;
             norms(L, B, P, bnorm);
             for (auto &b : bnorm) b = std::max(b, 1e-300);
-            // R = B (x0 = 0)
-            PQ_HIP(hipMemcpyAsync(R, B, vsz * 8, hipMemcpyDeviceToDevice, L->stream));
+            PQ_HIP(hipMemcpyAsync(R, B, vsz * 8, hipMemcpyDeviceToDevice, L->stream));   // x0 = 0
+            // Per Arnoldi step j the device writes, at Hd + j * hstride: the two Gram-Schmidt
+            // passes (m+1 coefficients each) and the new vector's squared norm; one asynchronous copy
+            // per step brings them to pinned host memory. The host works one step behind, so the
+            // device never waits for the Givens rotations.
+            const size_t hstride = (size_t)(2 * (m + 1) + 1) * P * 2;
+            L->Hd.reserve((size_t)m * hstride * 8);
+            if (L->Hh_bytes < (size_t)m * hstride * 8) {
+                if (L->Hh) (void)hipHostFree(L->Hh);
+                L->Hh = nullptr;
+                PQ_HIP(hipHostMalloc((void **)&L->Hh, (size_t)m * hstride * 8, hipHostMallocDefault));
+                L->Hh_bytes = (size_t)m * hstride * 8;
+            }
+            std::vector<hipEvent_t> ev(m);
+            for (auto &e : ev) PQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            double *Hd = L->Hd.as<double>();
+            const int nblk_red = std::min(RED_BLOCKS, (n + 255) / 256);
+            auto dots_to = [&](const double *Vb, size_t vs, int nv, const double *Wv, double *hout) {
+                hipLaunchKernelGGL(dots_partial_kernel, dim3(nblk_red, nv), dim3(256), 0, L->stream, Vb, vs, nv, Wv, n, P,
+                                   L->partial.as<double>());
+                hipLaunchKernelGGL(dots_reduce_kernel, dim3((nv * P * 2 + 255) / 256), dim3(256), 0, L->stream,
+                                   L->partial.as<double>(), nblk_red, nv, P, hout);
+            };
+            auto update_dev = [&](const double *Vb, size_t vs, int nv, const double *hdev, double sign, double *Wv) {
+                const size_t count = (size_t)n * P;
+                hipLaunchKernelGGL(vec_update_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, Vb,
+                                   vs, nv, hdev, sign, n, P, Wv);
+            };
+            auto enqueue_step = [&](int j) {
+                double *h1 = Hd + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2, *hn = h2 + (size_t)(m + 1) * P * 2;
+                apply_precond(L, precond, V + j * vsz, Z, P);
+                apply_operator(L, omega, Z, W, P);
+                dots_to(V, vsz, j + 1, W, h1);
+                update_dev(V, vsz, j + 1, h1, 1.0, W);
+                dots_to(V, vsz, j + 1, W, h2);
+                update_dev(V, vsz, j + 1, h2, 1.0, W);
+                dots_to(W, 0, 1, W, hn);
+                hipLaunchKernelGGL(vec_scale_norm_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, W,
+                                   hn, n, P, V + (j + 1) * vsz);
+                PQ_HIP(hipGetLastError());
+                PQ_HIP(hipMemcpyAsync(L->Hh + j * hstride, h1, hstride * 8, hipMemcpyDeviceToHost, L->stream));
+                PQ_HIP(hipEventRecord(ev[j], L->stream));
+            };
             int its = 0, restarts = 0;
             std::vector<double> res(P, 1.0);
             std::vector<cplx> H((size_t)(m + 1) * m * P), g((size_t)(m + 1) * P), cs((size_t)m * P), sn((size_t)m * P);
-            std::vector<double> hh((size_t)(m + 2) * P * 2), hh2((size_t)(m + 2) * P * 2);
             while (true) {
                 std::vector<double> beta;
                 norms(L, R, P, beta);
@@ -618,28 +1072,20 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 scale_cols(L, R, inv_beta, P, V);
                 std::fill(g.begin(), g.end(), cplx(0));
                 for (int p = 0; p < P; ++p) g[p] = beta[p];
-                int j = 0;
-                std::vector<bool> conv(P, false);
-                for (; j < m && its < maxiter; ++j, ++its) {
-                    apply_precond(L, precond, V + j * vsz, Z, P);
-                    apply_operator(L, omega, Z, W, P);   // W = Rl Z first, then += j w Ll Z
-                    // classical Gram-Schmidt, twice
-                    dots(L, V, vsz, j + 1, W, P, hh.data());
-                    update(L, V, vsz, j + 1, hh.data(), 1.0, P, W);
-                    dots(L, V, vsz, j + 1, W, P, hh2.data());
-                    update(L, V, vsz, j + 1, hh2.data(), 1.0, P, W);
-                    std::vector<double> hn;
-                    norms(L, W, P, hn);
-                    std::vector<double> inv(P);
-                    for (int p = 0; p < P; ++p) inv[p] = hn[p] > 0 ? 1.0 / hn[p] : 0.0;
-                    scale_cols(L, W, inv, P, V + (j + 1) * vsz);
+                const int steps = std::min(m, maxiter - its);
+                int used = 0;                      // Arnoldi steps whose rotations are applied
+                bool converged = false;
+                auto process = [&](int j) {
+                    PQ_HIP(hipEventSynchronize(ev[j]));
+                    const double *h1 = L->Hh + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2;
+                    const double *hn = h2 + (size_t)(m + 1) * P * 2;
                     bool all = true;
                     for (int p = 0; p < P; ++p) {
                         auto Hc = [&](int i) -> cplx & { return H[((size_t)i * m + j) * P + p]; };
                         for (int i = 0; i <= j; ++i)
-                            Hc(i) = cplx(hh[(i * P + p) * 2] + hh2[(i * P + p) * 2],
-                                         hh[(i * P + p) * 2 + 1] + hh2[(i * P + p) * 2 + 1]);
-                        Hc(j + 1) = hn[p];
+                            Hc(i) = cplx(h1[(i * P + p) * 2] + h2[(i * P + p) * 2],
+                                         h1[(i * P + p) * 2 + 1] + h2[(i * P + p) * 2 + 1]);
+                        Hc(j + 1) = std::sqrt(std::max(hn[2 * p], 0.0));
                         for (int i = 0; i < j; ++i) {
                             const cplx a = Hc(i), b = Hc(i + 1);
                             Hc(i) = std::conj(cs[i * P + p]) * a + std::conj(sn[i * P + p]) * b;
@@ -657,34 +1103,43 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                         res[p] = std::abs(g[(j + 1) * P + p]) / bnorm[p];
                         if (res[p] > tol) all = false;
                     }
-                    if (all) { ++j; ++its; break; }
+                    used = j + 1;
+                    return all;
+                };
+                for (int j = 0; j < steps; ++j) {
+                    enqueue_step(j);
+                    if (j > 0 && process(j - 1)) { converged = true; break; }
                 }
+                if (!converged && used < steps) process(steps - 1);
+                const int jn = used;
+                its += jn;
                 // y from the triangular systems; X += M^-1 (V y)
-                std::vector<double> y((size_t)j * P * 2);
+                std::vector<double> y((size_t)jn * P * 2);
                 for (int p = 0; p < P; ++p) {
-                    std::vector<cplx> yy(j);
-                    for (int i = j - 1; i >= 0; --i) {
-                        cplx s = g[i * P + p];
-                        for (int k = i + 1; k < j; ++k) s -= H[((size_t)i * m + k) * P + p] * yy[k];
+                    std::vector<cplx> yy(jn);
+                    for (int i = jn - 1; i >= 0; --i) {
+                        cplx sacc = g[i * P + p];
+                        for (int k = i + 1; k < jn; ++k) sacc -= H[((size_t)i * m + k) * P + p] * yy[k];
                         const cplx d = H[((size_t)i * m + i) * P + p];
-                        yy[i] = std::abs(d) > 0 ? s / d : cplx(0);
+                        yy[i] = std::abs(d) > 0 ? sacc / d : cplx(0);
                     }
-                    for (int i = 0; i < j; ++i) { y[(i * P + p) * 2] = yy[i].real(); y[(i * P + p) * 2 + 1] = yy[i].imag(); }
+                    for (int i = 0; i < jn; ++i) { y[(i * P + p) * 2] = yy[i].real(); y[(i * P + p) * 2 + 1] = yy[i].imag(); }
                 }
                 PQ_HIP(hipMemsetAsync(W, 0, vsz * 8, L->stream));
-                if (j > 0) update(L, V, vsz, j, y.data(), -1.0, P, W);           // W = V y
+                if (jn > 0) update(L, V, vsz, jn, y.data(), -1.0, P, W);          // W = V y
                 apply_precond(L, precond, W, Z, P);
                 {
                     std::vector<double> one((size_t)P * 2, 0.0);
                     for (int p = 0; p < P; ++p) one[2 * p] = 1.0;
-                    update(L, Z, 0, 1, one.data(), -1.0, P, X);                     // X += Z
+                    update(L, Z, 0, 1, one.data(), -1.0, P, X);                    // X += Z
                 }
-                // true residual R = B - A X
-                apply_operator(L, omega, X, R, P);
+                apply_operator(L, omega, X, R, P);                                  // true residual
                 hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B, vsz, R);
                 PQ_HIP(hipGetLastError());
                 ++restarts;
             }
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            for (auto &e : ev) (void)hipEventDestroy(e);
             L->iterations = its; L->restarts = restarts;
             double worst = 0;
             for (double r : res) worst = std::max(worst, r);
@@ -733,6 +1188,77 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
         }
         if (X_host && n > 0) PQ_HIP(hipMemcpy(X_host, X, vsz * 8, hipMemcpyDeviceToHost));
         L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    });
+}
+
+// The coarse space of the two-level preconditioner: Phi (n_free, k) FP64 host, row-major.
+// Rl Phi and Ll Phi and the Galerkin matrices Phi^T Rl Phi, Phi^T Ll Phi are formed on the device
+// once; each frequency then only inverts the k x k coarse matrix.
+int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi) {
+    return guard([&] {
+        PQ_REQUIRE(L && (k == 0 || Phi), "pq_loops_set_coarse: null argument");
+        PQ_REQUIRE(k >= 0 && k <= 4096, "pq_loops_set_coarse: 0 <= k <= 4096");
+        DeviceScope scope(L->device);
+        L->kc = k;
+        L->omega_c = -1;
+        if (k == 0) return;
+        const int n = L->nf;
+        L->Phi.reserve((size_t)n * k * 4); L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
+        DevBuf Phi64;
+        Phi64.reserve((size_t)n * k * 8);
+        PQ_HIP(hipMemcpyAsync(Phi64.p, Phi, (size_t)n * k * 8, hipMemcpyHostToDevice, L->stream));
+        // A Phi with omega = 1 on chunks of 16 columns: real part = Rl Phi, imaginary part = Ll Phi
+        constexpr int CH = 16;
+        DevBuf X, W;
+        X.reserve((size_t)n * 2 * CH * 8); W.reserve((size_t)n * 2 * CH * 8);
+        std::vector<double> hx((size_t)n * 2 * CH), hphi((size_t)n * k);
+        std::copy(Phi, Phi + (size_t)n * k, hphi.begin());
+        std::vector<double> hw((size_t)n * 2 * CH), hr((size_t)n * k), hl((size_t)n * k);
+        for (int c0 = 0; c0 < k; c0 += CH) {
+            const int cn = std::min(CH, k - c0);
+            std::fill(hx.begin(), hx.end(), 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < cn; ++c) hx[(size_t)i * 2 * CH + c] = hphi[(size_t)i * k + c0 + c];
+            PQ_HIP(hipMemcpyAsync(X.p, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, L->stream));
+            apply_operator(L, 1.0, X.as<double>(), W.as<double>(), CH);
+            PQ_HIP(hipMemcpyAsync(hw.data(), W.p, hw.size() * 8, hipMemcpyDeviceToHost, L->stream));
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < cn; ++c) {
+                    hr[(size_t)i * k + c0 + c] = hw[(size_t)i * 2 * CH + c];
+                    hl[(size_t)i * k + c0 + c] = hw[(size_t)i * 2 * CH + CH + c];
+                }
+        }
+        DevBuf RPhi64, LPhi64;
+        RPhi64.reserve(hr.size() * 8); LPhi64.reserve(hl.size() * 8);
+        PQ_HIP(hipMemcpyAsync(RPhi64.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, L->stream));
+        PQ_HIP(hipMemcpyAsync(LPhi64.p, hl.data(), hl.size() * 8, hipMemcpyHostToDevice, L->stream));
+        {   // FP32 copies for the per-iteration apply
+            std::vector<float> f32((size_t)n * k);
+            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hphi[e];
+            PQ_HIP(hipMemcpy(L->Phi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
+            L->Phit.reserve((size_t)n * k * 4);
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < k; ++c) f32[(size_t)c * n + i] = (float)hphi[(size_t)i * k + c];
+            PQ_HIP(hipMemcpy(L->Phit.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
+            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hr[e];
+            PQ_HIP(hipMemcpy(L->RPhi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
+            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hl[e];
+            PQ_HIP(hipMemcpy(L->LPhi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
+        }
+        DevBuf G;
+        G.reserve((size_t)k * k * 8);
+        L->Rc.assign((size_t)k * k, 0.0);
+        L->Lc.assign((size_t)k * k, 0.0);
+        hipLaunchKernelGGL(gram_kernel, dim3(k * k), dim3(256), 0, L->stream, Phi64.as<double>(), k, k,
+                           RPhi64.as<double>(), k, k, n, G.as<double>());
+        PQ_HIP(hipMemcpyAsync(L->Rc.data(), G.p, (size_t)k * k * 8, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        hipLaunchKernelGGL(gram_kernel, dim3(k * k), dim3(256), 0, L->stream, Phi64.as<double>(), k, k,
+                           LPhi64.as<double>(), k, k, n, G.as<double>());
+        PQ_HIP(hipMemcpyAsync(L->Lc.data(), G.p, (size_t)k * k * 8, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        PQ_HIP(hipGetLastError());
     });
 }
 

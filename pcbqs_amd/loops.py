@@ -92,6 +92,7 @@ class LoopBasis:
     port_cols: np.ndarray         # column of each port's loop
     tree: np.ndarray              # (nb,) bool
     positions: np.ndarray         # (nloops, 2) loop centroids
+    cotree: np.ndarray            # (nloops,) the non-tree branch that closes each loop (coefficient +1)
 
 
 def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_terminals=True):
@@ -195,7 +196,7 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
     order = np.r_[order_free, src_cols]
     C = C[:, order].tocsc()
     port_branch = cot[src_cols]
-    return LoopBasis(C, None, len(free_cols), np.arange(len(free_cols), nl), tree, pos[order]), port_branch
+    return LoopBasis(C, None, len(free_cols), np.arange(len(free_cols), nl), tree, pos[order], cot[order]), port_branch
 
 
 def _attach(parent, upb, a, b, k, adj):

@@ -18,7 +18,7 @@ from .backend import Mi355xBackend, padded
 from .geometry import MU0, RWG, FillGeometry
 from .loops import LoopModel
 
-PRECONDITIONERS = {'none': 0, 'block-jacobi': 1}
+PRECONDITIONERS = {'none': 0, 'block-jacobi': 1, 'two-level': 2}
 
 
 class PortSolver:
@@ -67,6 +67,26 @@ class PortSolver:
         self._handle = handle
         self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
         self.setup_seconds = dict(topology=t1 - t0, device=time.perf_counter() - t1)
+
+    def set_coarse(self, phi):
+        """Coarse space (n_free, k) of the two-level preconditioner (None or k = 0: none)."""
+        phi = np.zeros((self.model.n_free, 0)) if phi is None else np.ascontiguousarray(phi, np.float64)
+        if phi.ndim != 2 or phi.shape[0] != self.model.n_free:
+            raise ValueError('coarse space must have one row per free loop')
+        call('pq_loops_set_coarse', self._handle, phi.shape[1], ptr(phi) if phi.shape[1] else None)
+        self.coarse_k = phi.shape[1]
+
+    def set_stream_coarse(self, grid=8, vias=True, extra=None):
+        """The geometric coarse space: stream-function hats of a grid x grid net per layer, the
+        via-closed loops (vias=True), plus extra branch patterns (nb, k') when given."""
+        from .coarse import loop_restrict, stream_patterns, via_loop_columns
+        patterns = stream_patterns(self.model, grid)
+        if extra is not None and extra.shape[1]:
+            patterns = np.hstack([patterns, extra])
+        cols = [loop_restrict(self.model, patterns)]
+        if vias:
+            cols.append(via_loop_columns(self.model))
+        self.set_coarse(np.hstack(cols))
 
     @property
     def n_ports(self):

@@ -26,13 +26,15 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize('name', ['c1', 'c2-small'])
-@pytest.mark.parametrize('precond', ['block-jacobi', 'none'])
+@pytest.mark.parametrize('precond', ['block-jacobi', 'none', 'two-level'])
 def test_port_impedance_matches_the_fp64_oracle(name, precond):
     from pcbqs_amd.solve import PortSolver
     b = boards.board_c1() if name == 'c1' else boards.board_c2(1, target=3000)
     freqs = [1e3, 1e6, 1e8]
     print(f'ports {name} {precond} nt={b.nt}', flush=True)
     s = PortSolver(b, device=0)
+    if precond == 'two-level':
+        s.set_stream_coarse(grid=6)
     Z, info = s.ports(freqs, precond=precond, tol=1e-8)
     Zo = oracle_ports(b, freqs)
     w = 2 * np.pi * np.array(freqs)[:, None, None]

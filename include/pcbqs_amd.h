@@ -154,22 +154,42 @@ int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
  * (precond 0: none, 1: block-Jacobi on 64-loop blocks of the free-loop order,
  * 2: two-level: the coarse space of pq_loops_set_coarse, then block-Jacobi),
  * then Zp = Z_ss + Z_sf X. X_host (n_free, 2P) (optional) receives the free
- * loop currents, real parts first. PQ_ECONVERGE when the true relative
+ * loop currents, real parts first; X0_host (optional, same layout) is the
+ * initial iterate (zero when null). PQ_ECONVERGE when the true relative
  * residual exceeds max(5 tol, 1e-13). Replaces direct_ports
  * (src/pcbqs/mqs_ports.py:52-69) solved as port_matrix's iterative route
  * (src/pcbqs/mqs_iterative.py:298-367, _lockstep_gmres :109-170) with the
  * acceptance rule of peec_iterative._gmres (src/pcbqs/peec_iterative.py:100-112). */
 int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *port_cols, int precond, double tol,
-                         int restart, int maxiter, double *Zp, double *X_host, int64_t *iterations,
-                         double *residual);
+                         int restart, int maxiter, double *Zp, double *X_host, const double *X0_host,
+                         int64_t *iterations, double *residual);
+
+/* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
+ * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the
+ * right-hand side of direct_ports (src/pcbqs/mqs_ports.py:52-69). */
+int pq_loops_rhs(pq_loops *loops, double omega, int P, const int32_t *port_cols, double *B_host);
+
+/* The coarse-space solution Phi (Phi^T Z_ff Phi)^-1 Phi^T B (n_free, 2P) FP64
+ * host for the port right-hand sides at omega: the input of the learned
+ * correction that refines it into an initial iterate. No counterpart in the
+ * reference. */
+int pq_loops_coarse_solution(pq_loops *loops, double omega, int P, const int32_t *port_cols, double *Xc_host);
 
 /* The coarse space of the two-level preconditioner (precond 2 of
- * pq_loops_solve_ports): Phi (n_free, k) FP64 host, row-major. Each solve then
- * applies y = Phi c + BJ(x - A Phi c), c = (Phi^T A Phi)^-1 Phi^T x (coarse
- * correction first, block-Jacobi on the remainder). k = 0 removes it. No
- * counterpart in the reference (its iterative path uses a graph-Laplacian
- * preconditioner, src/pcbqs/mqs_precond.py:27-59). */
-int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi);
+ * pq_loops_solve_ports): Phi (n_free, k) FP64 host, row-major (kept in FP32 on
+ * the device). Each solve then applies y = Phi c + BJ(x - A Phi c),
+ * c = (Phi^T A Phi)^-1 Phi^T x (coarse correction first, block-Jacobi on the
+ * remainder). diagonal != 0 declares that Phi diagonalises both Galerkin
+ * matrices Phi^T Rl Phi and Phi^T Ll Phi (a generalised eigenbasis), so each
+ * frequency inverts only their diagonals. k = 0 removes it. No counterpart in
+ * the reference (its iterative path uses a graph-Laplacian preconditioner,
+ * src/pcbqs/mqs_precond.py:27-59). */
+int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi, int diagonal);
+
+/* The Galerkin matrices Phi^T Rl Phi and Phi^T Ll Phi (k x k FP64, row-major)
+ * of the current coarse space, for a host-side change of basis. No counterpart
+ * in the reference. */
+int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
 
 /* Statistics of the last solve: iterations, operator products, relative
  * residual, wall seconds. Not in the reference (solve_info's role,

@@ -58,9 +58,12 @@ _SIGNATURES = {
                         ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),
     'pq_loops_matrix': ([P, P, P], ctypes.c_int),
-    'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P],
+    'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
                              ctypes.c_int),
-    'pq_loops_set_coarse': ([P, ctypes.c_int, P], ctypes.c_int),
+    'pq_loops_set_coarse': ([P, ctypes.c_int, P, ctypes.c_int], ctypes.c_int),
+    'pq_loops_coarse_matrices': ([P, P, P], ctypes.c_int),
+    'pq_loops_rhs': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
 }
 

@@ -60,9 +60,11 @@ __global__ __launch_bounds__(256) void proj_right_kernel(const float *U, long lo
 // ---- operator products --------------------------------------------------------------------
 
 // W = Rl X on rows [0, n) restricted to columns < n (CSR rows of the full loop matrix); NR = 2P.
+// One wave per row, lanes over its nonzeros (all NR columns each), then a butterfly per column.
 template <int NR>
-__global__ __launch_bounds__(256) void spmv_kernel(const int *ptr, const int *col, const double *val, int n,
-                                                  const double *X, double *W) {
+__global__ __launch_bounds__(256) void spmv_kernel(const int *__restrict__ ptr, const int *__restrict__ col,
+                                                  const double *__restrict__ val, int n, const double *__restrict__ X,
+                                                  double *__restrict__ W) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
@@ -77,15 +79,17 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int *ptr, const int *co
         for (int r = 0; r < NR; ++r) acc[r] += v * X[(size_t)c * NR + r];
     }
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        double s = acc[r];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-        if (lane == 0) W[(size_t)row * NR + r] = s;
+    for (int r = 0; r < NR; ++r)
+        for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off);
+    if (lane < NR) {
+        double v = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r == lane) v = acc[r];
+        W[(size_t)row * NR + lane] = v;
     }
 }
 
-// Y = Ll[0:n, 0:n] X (FP32 matrix, FP64 accumulation), then W_re -= w Y_im, W_im += w Y_re.
-// A block holds RPB rows; the X chunk of 256 columns is staged in LDS and shared by them.
 template <int NR>
 __global__ __launch_bounds__(256) void gemv_kernel(const float *Ll, long long ldl, int n, const double *X,
                                                   double omega, double *W) {
@@ -154,35 +158,39 @@ __global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
 
 // Partial products of the symmetric Ll: part[s][row][r] = sum over columns c of chunk s of
 // Ll[c][row] X[c][r]. One thread per row, so each column read is 256 consecutive floats of row c
-// (coalesced); the chunk's X rows are staged in LDS and read as broadcasts.
-constexpr int GEMV_SPLIT = 16, GEMV_ROWS = 256, GEMV_CHUNK = 640;
+// (coalesced), 16 of them in flight per lane; X[c] is the same for the whole wave and comes
+// through scalar loads, which keeps the LDS return path out of the inner loop.
+constexpr int GEMV_SPLIT = 32, GEMV_ROWS = 256, GEMV_CHUNK = 640;
 template <int NR>
-__global__ __launch_bounds__(256) void gemv_sym_kernel(const float *Ll, long long ldl, int n, const double *X,
-                                                      int chunk, double *part) {
-    __shared__ double xs[GEMV_CHUNK * NR];
+__global__ __launch_bounds__(256) void gemv_sym_kernel(const float *__restrict__ Ll, long long ldl, int n,
+                                                      const double *__restrict__ X, int chunk,
+                                                      double *__restrict__ part) {
     const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(n, c0 + chunk);
-    for (int e = threadIdx.x; e < (c1 - c0) * NR; e += 256) xs[e] = X[(size_t)c0 * NR + e];
-    __syncthreads();
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
+    const float *col = Ll + min(row, n - 1);
+    int c = c0;
+    for (; c + 16 <= c1; c += 16) {
+        float m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = col[(size_t)(c + u) * ldl];
+        const double *x = X + (size_t)c * NR;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double mu = m[u];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += mu * x[u * NR + r];
+        }
+    }
+    for (; c < c1; ++c) {
+        const double mv = col[(size_t)c * ldl];
+        const double *x = X + (size_t)c * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
+    }
     if (row < n) {
-        const float *col = Ll + row;
-        int c = c0;
-        for (; c + 4 <= c1; c += 4) {
-            const double m0 = col[(size_t)c * ldl], m1 = col[(size_t)(c + 1) * ldl];
-            const double m2 = col[(size_t)(c + 2) * ldl], m3 = col[(size_t)(c + 3) * ldl];
-            const double *x = xs + (c - c0) * NR;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += m0 * x[r] + m1 * x[NR + r] + m2 * x[2 * NR + r] + m3 * x[3 * NR + r];
-        }
-        for (; c < c1; ++c) {
-            const double m = col[(size_t)c * ldl];
-            const double *x = xs + (c - c0) * NR;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += m * x[r];
-        }
         double *o = part + ((size_t)blockIdx.y * n + row) * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) o[r] = acc[r];
@@ -301,9 +309,10 @@ __global__ __launch_bounds__(256) void bj_setup_kernel(const float *Ll, long lon
     }
 }
 
-// Y_b = Zinv_b X_b per block (complex), NR = 2P.
+// Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P.
 template <int NR>
-__global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, const double *X, double *Y) {
+__global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, const double *X, const double *A,
+                                                      double *Y) {
     constexpr int P = NR / 2;
     __shared__ double xs[BS * NR];
     const int b = blockIdx.x, r0 = b * BS, m = min(BS, n - r0);
@@ -319,6 +328,10 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, 
             yr += zr * xr - zi * xi;
             yi += zr * xi + zi * xr;
         }
+        if (A) {
+            yr += A[(size_t)(r0 + i) * NR + p];
+            yi += A[(size_t)(r0 + i) * NR + P + p];
+        }
         Y[(size_t)(r0 + i) * NR + p] = yr;
         Y[(size_t)(r0 + i) * NR + P + p] = yi;
     }
@@ -328,61 +341,79 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, 
 
 constexpr int RED_BLOCKS = 256;
 
-// partial[blk][i][p] = sum over the block's rows of conj(V_i[n,p]) W[n,p], i < nv (complex).
-__global__ __launch_bounds__(256) void dots_partial_kernel(const double *V, size_t vstride, int nv, const double *W,
-                                                          int n, int P, double *partial) {
-    __shared__ double sr[256], si[256];
-    const int NR = 2 * P;
+// partial[i][blk][p] = sum over the block's rows of conj(V_i[n,p]) W[n,p], i < nv (complex): one
+// row per thread for all P columns, then one LDS tree over the block for the 2P sums together.
+template <int P>
+__global__ __launch_bounds__(256) void dots_partial_kernel(const double *__restrict__ V, size_t vstride, int nv,
+                                                          const double *__restrict__ W, int n,
+                                                          double *__restrict__ partial) {
+    constexpr int NR = 2 * P;
+    __shared__ double s[NR][256];
     const int i = blockIdx.y;
     const double *Vi = V + (size_t)i * vstride;
-    for (int p = 0; p < P; ++p) {
-        double re = 0, im = 0;
-        for (int row = blockIdx.x * 256 + threadIdx.x; row < n; row += gridDim.x * 256) {
+    double re[P], im[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) re[p] = im[p] = 0;
+    for (int row = blockIdx.x * 256 + threadIdx.x; row < n; row += gridDim.x * 256) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
             const double vr = Vi[(size_t)row * NR + p], vi = Vi[(size_t)row * NR + P + p];
             const double wr = W[(size_t)row * NR + p], wi = W[(size_t)row * NR + P + p];
-            re += vr * wr + vi * wi;
-            im += vr * wi - vi * wr;
+            re[p] += vr * wr + vi * wi;
+            im[p] += vr * wi - vi * wr;
         }
-        sr[threadIdx.x] = re; si[threadIdx.x] = im;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) { sr[threadIdx.x] += sr[threadIdx.x + s]; si[threadIdx.x] += si[threadIdx.x + s]; }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double *o = partial + ((size_t)blockIdx.x * nv + i) * P * 2 + 2 * p;
-            o[0] = sr[0]; o[1] = si[0];
-        }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) { s[2 * p][threadIdx.x] = re[p]; s[2 * p + 1][threadIdx.x] = im[p]; }
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + st];
         __syncthreads();
     }
+    if (threadIdx.x < NR) partial[((size_t)i * gridDim.x + blockIdx.x) * NR + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// h[i][p] = sum over blocks of the partials, in block order.
-__global__ void dots_reduce_kernel(const double *partial, int nblk, int nv, int P, double *h) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nv * P * 2) return;
+// h[i][q] = sum over the nblk block partials of vector i (one wave per output; lanes over blocks,
+// then a fixed butterfly).
+__global__ __launch_bounds__(256) void dots_reduce_kernel(const double *__restrict__ partial, int nblk, int nv,
+                                                         int NR, double *__restrict__ h) {
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= nv * NR) return;
+    const int i = e / NR, q = e % NR;
     double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * nv * P * 2 + e];
-    h[e] = s;
+    for (int b = lane; b < nblk; b += 64) s += partial[((size_t)i * nblk + b) * NR + q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) h[e] = s;
 }
 
-// W[n,p] -= sum_i V_i[n,p] h[i][p] (complex); with sign -1 it adds.
+// W[n,p] -= sum_i V_i[n,p] h[i][p] (complex); with sign -1 it adds. Four adjacent lanes share an
+// output (row, p) and take every fourth vector; two butterfly steps combine them in a fixed order.
 __global__ __launch_bounds__(256) void vec_update_kernel(const double *V, size_t vstride, int nv, const double *h,
                                                         double sign, int n, int P, double *W) {
     const int NR = 2 * P;
-    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (size_t)n * P) return;
-    const int row = (int)(e / P), p = (int)(e % P);
+    const size_t e = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2;
+    const int part = threadIdx.x & 3;
+    const bool live = e < (size_t)n * P;
+    const int row = live ? (int)(e / P) : 0, p = live ? (int)(e % P) : 0;
     double ar = 0, ai = 0;
-    for (int i = 0; i < nv; ++i) {
-        const double hr = h[(i * P + p) * 2], hi = h[(i * P + p) * 2 + 1];
-        const double vr = V[(size_t)i * vstride + (size_t)row * NR + p];
-        const double vi = V[(size_t)i * vstride + (size_t)row * NR + P + p];
-        ar += vr * hr - vi * hi;
-        ai += vr * hi + vi * hr;
+    if (live) {
+        for (int i = part; i < nv; i += 4) {
+            const double hr = h[(i * P + p) * 2], hi = h[(i * P + p) * 2 + 1];
+            const double vr = V[(size_t)i * vstride + (size_t)row * NR + p];
+            const double vi = V[(size_t)i * vstride + (size_t)row * NR + P + p];
+            ar += vr * hr - vi * hi;
+            ai += vr * hi + vi * hr;
+        }
     }
-    W[(size_t)row * NR + p] -= sign * ar;
-    W[(size_t)row * NR + P + p] -= sign * ai;
+    ar += __shfl_xor(ar, 1); ai += __shfl_xor(ai, 1);
+    ar += __shfl_xor(ar, 2); ai += __shfl_xor(ai, 2);
+    if (live && part == 0) {
+        W[(size_t)row * NR + p] -= sign * ar;
+        W[(size_t)row * NR + P + p] -= sign * ai;
+    }
 }
 
 // Y[n,p] = X[n,p] * s[p] (real per-column scale).
@@ -446,91 +477,6 @@ __global__ __launch_bounds__(256) void port_reduce_kernel(const float *Ll, long 
 
 // ---- two-level (coarse space) preconditioner -------------------------------------------------
 
-// G[a][b] = sum_i A[i][a] B[i][b] (A (n, ka), B (n, kb), row-major with leading dims), FP64.
-__global__ __launch_bounds__(256) void gram_kernel(const double *A, int lda, int ka, const double *B, int ldb, int kb,
-                                                  int n, double *G) {
-    __shared__ double sred[256];
-    const int a = blockIdx.x / kb, b = blockIdx.x % kb;
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += 256) s += A[(size_t)i * lda + a] * B[(size_t)i * ldb + b];
-    sred[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) sred[threadIdx.x] += sred[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) G[blockIdx.x] = sred[0];
-}
-
-// t[c][p] = sum_i Phi[i][c] v[i][p] (complex v, real Phi): partial sums per row block.
-__global__ __launch_bounds__(256) void coarse_restrict_kernel(const double *Phi, int k, const double *V, int n, int P,
-                                                             double *partial) {
-    __shared__ double sr[256], si[256];
-    const int NR = 2 * P;
-    const int c = blockIdx.y;
-    for (int p = 0; p < P; ++p) {
-        double re = 0, im = 0;
-        for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-            const double f = Phi[(size_t)i * k + c];
-            re += f * V[(size_t)i * NR + p];
-            im += f * V[(size_t)i * NR + P + p];
-        }
-        sr[threadIdx.x] = re; si[threadIdx.x] = im;
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (threadIdx.x < st) { sr[threadIdx.x] += sr[threadIdx.x + st]; si[threadIdx.x] += si[threadIdx.x + st]; }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double *o = partial + (((size_t)blockIdx.x * k + c) * P + p) * 2;
-            o[0] = sr[0]; o[1] = si[0];
-        }
-        __syncthreads();
-    }
-}
-
-// c = Ainv t: t (k, P) complex summed from nblk partials, Ainv (k, k) complex; one thread per (row, p).
-__global__ void coarse_solve_kernel(const double *partial, int nblk, const double *Ainv, int k, int P, double *cvec) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= k * P) return;
-    const int a = e / P, p = e % P;
-    double yr = 0, yi = 0;
-    for (int b = 0; b < k; ++b) {
-        double tr = 0, ti = 0;
-        for (int q = 0; q < nblk; ++q) {
-            tr += partial[(((size_t)q * k + b) * P + p) * 2];
-            ti += partial[(((size_t)q * k + b) * P + p) * 2 + 1];
-        }
-        const double ar = Ainv[2 * ((size_t)a * k + b)], ai = Ainv[2 * ((size_t)a * k + b) + 1];
-        yr += ar * tr - ai * ti;
-        yi += ar * ti + ai * tr;
-    }
-    cvec[2 * e] = yr; cvec[2 * e + 1] = yi;
-}
-
-// R = V - (RPhi + j w LPhi) c   and   Y = Phi c  (complex c (k, P)).
-__global__ __launch_bounds__(256) void coarse_prolong_kernel(const double *Phi, const double *RPhi, const double *LPhi,
-                                                            int k, const double *cvec, double omega, const double *V,
-                                                            int n, int P, double *R, double *Y) {
-    const int NR = 2 * P;
-    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (size_t)n * P) return;
-    const int i = (int)(e / P), p = (int)(e % P);
-    double ar = 0, ai = 0, yr = 0, yi = 0;
-    for (int c = 0; c < k; ++c) {
-        const double cr = cvec[2 * (c * P + p)], ci = cvec[2 * (c * P + p) + 1];
-        const double r = RPhi[(size_t)i * k + c], l = omega * LPhi[(size_t)i * k + c], f = Phi[(size_t)i * k + c];
-        ar += r * cr - l * ci;
-        ai += r * ci + l * cr;
-        yr += f * cr;
-        yi += f * ci;
-    }
-    R[(size_t)i * NR + p] = V[(size_t)i * NR + p] - ar;
-    R[(size_t)i * NR + P + p] = V[(size_t)i * NR + P + p] - ai;
-    Y[(size_t)i * NR + p] = yr;
-    Y[(size_t)i * NR + P + p] = yi;
-}
-
 // Y[n,p] = X[n,p] / sqrt(nrm[p]) with nrm[p] = (|X_p|^2, 0) as dots_reduce leaves it (0 -> 0).
 __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n, int P, double *Y) {
     const int NR = 2 * P;
@@ -541,55 +487,87 @@ __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n,
     Y[e] = s > 0 ? X[e] / sqrt(s) : 0.0;
 }
 
-// Restriction t = Phi^T v (Phi real FP32 (n, k), v complex (n, 2P)): per row block, each thread
-// owns outputs (c, r) = e / NR, e % NR for e = tid, tid + 256, ... and sums over the block's rows.
-constexpr int CR_MAX = 16;  // outputs per thread: k * NR <= 256 * CR_MAX
-__global__ __launch_bounds__(256) void coarse_restrict_kernel2(const float *Phi, int k, const double *V, int n, int NR,
-                                                              int rows_per_block, double *partial) {
-    const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
-    const int total = k * NR;
-    double acc[CR_MAX];
+// Restriction partials: partial[chunk][c][r] = sum over the chunk's rows i of Phi[i][c] V[i][r],
+// Phi (n, k) FP32 row-major, V (n, NR) FP64. A block covers 64 coarse columns (the lanes, so each
+// row segment of Phi is one coalesced 256-byte read, eight in flight per lane) and cr_rows(NR)
+// rows split over its four waves; a row of V is wave-uniform (scalar loads). The four waves' sums
+// are added in wave order.
+constexpr int cr_rows(int NR) { return NR >= 8 ? 4096 / NR : 512; }
+template <int NR>
+__global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__restrict__ Phi, int k,
+                                                             const double *__restrict__ V, int n,
+                                                             double *__restrict__ partial) {
+    constexpr int ROWS = cr_rows(NR), U = 8;
+    __shared__ double red[4 * 64 * NR];
+    const int i0 = blockIdx.x * ROWS, m = min(ROWS, n - i0);
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.y * 64 + lane;
+    double acc[NR];
 #pragma unroll
-    for (int q = 0; q < CR_MAX; ++q) acc[q] = 0;
-    for (int i = i0; i < i1; ++i) {
-        const float *prow = Phi + (size_t)i * k;
-        const double *vrow = V + (size_t)i * NR;
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    const float *col = Phi + (size_t)i0 * k + min(c, k - 1);
+    const double *v = V + (size_t)i0 * NR;
+    int i = w;
+    for (; i + 4 * (U - 1) < m; i += 4 * U) {
+        float f[U];
 #pragma unroll
-        for (int q = 0; q < CR_MAX; ++q) {
-            const int e = threadIdx.x + q * 256;
-            if (e < total) acc[q] += (double)prow[e / NR] * vrow[e % NR];
+        for (int u = 0; u < U; ++u) f[u] = col[(size_t)(i + 4 * u) * k];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double fu = f[u];
+            const double *vr = v + (size_t)(i + 4 * u) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += fu * vr[r];
         }
     }
+    for (; i < m; i += 4) {
+        const double f = col[(size_t)i * k];
+        const double *vr = v + (size_t)i * NR;
 #pragma unroll
-    for (int q = 0; q < CR_MAX; ++q) {
-        const int e = threadIdx.x + q * 256;
-        if (e < total) partial[(size_t)blockIdx.x * total + e] = acc[q];
+        for (int r = 0; r < NR; ++r) acc[r] += f * vr[r];
     }
-}
-
-// Restriction as a tiled product: partial[chunk][c][r] = sum_{i in chunk} Phit[c][i] V[i][r] for
-// 32 coarse columns per block and 512 rows per chunk (V chunk staged in LDS, Phit = Phi^T FP32).
-constexpr int CR_COLS = 32, CR_ROWS = 512;
-template <int NR>
-__global__ __launch_bounds__(256) void coarse_restrict_kernel3(const float *Phit, int k, const double *V, int n,
-                                                              double *partial) {
-    __shared__ double vs[CR_ROWS * NR];
-    const int c0 = blockIdx.x * CR_COLS, i0 = blockIdx.y * CR_ROWS, i1 = min(n, i0 + CR_ROWS);
-    for (int e = threadIdx.x; e < (i1 - i0) * NR; e += 256) vs[e] = V[(size_t)i0 * NR + e];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) red[(w * 64 + lane) * NR + r] = acc[r];
     __syncthreads();
-    for (int e = threadIdx.x; e < CR_COLS * NR; e += 256) {
-        const int cl = e / NR, r = e % NR, c = c0 + cl;
-        if (c >= k) continue;
-        const float *prow = Phit + (size_t)c * n;
-        double acc = 0;
-        for (int i = i0; i < i1; ++i) acc += (double)prow[i] * vs[(i - i0) * NR + r];
-        partial[((size_t)blockIdx.y * k + c) * NR + r] = acc;
+    for (int e = threadIdx.x; e < 64 * NR; e += 256) {
+        const int cl = e / NR, r = e % NR;
+        if (blockIdx.y * 64 + cl >= k) continue;
+        const double s = red[(0 * 64 + cl) * NR + r] + red[(1 * 64 + cl) * NR + r] + red[(2 * 64 + cl) * NR + r] +
+                         red[(3 * 64 + cl) * NR + r];
+        partial[((size_t)blockIdx.x * k + blockIdx.y * 64 + cl) * NR + r] = s;
     }
 }
 
-// c = Ainv t, one wave per output (a, p), lanes over b.
-__global__ __launch_bounds__(256) void coarse_solve_kernel3(const double *t, const double *Ainv, int k, int P,
-                                                           double *cvec) {
+// t[e] = sum over the row-chunk partials, in chunk order (e = c * NR + r).
+__global__ void coarse_sum_kernel(const double *partial, int nblk, int total, double *t) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    double s = 0;
+    for (int q = 0; q < nblk; ++q) s += partial[(size_t)q * total + e];
+    t[e] = s;
+}
+
+// Diagonal coarse matrix: c[a][p] = D[a] t[a][p] with t summed from the chunk partials in chunk
+// order (t (k, NR), real parts in columns 0..P-1; D and c complex interleaved).
+__global__ void coarse_sum_diag_kernel(const double *partial, int nblk, int k, int P, const double *D,
+                                       double *cvec) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= k * P) return;
+    const int NR = 2 * P, a = e / P, p = e % P;
+    double tr = 0, ti = 0;
+    for (int q = 0; q < nblk; ++q) {
+        tr += partial[((size_t)q * k + a) * NR + p];
+        ti += partial[((size_t)q * k + a) * NR + P + p];
+    }
+    const double dr = D[2 * a], di = D[2 * a + 1];
+    cvec[2 * e] = dr * tr - di * ti;
+    cvec[2 * e + 1] = dr * ti + di * tr;
+}
+
+// c = Ainv t (complex; t (k, NR) with real parts in columns 0..P-1), cvec (k, P) complex
+// interleaved. One wave per output (a, p), lanes over b.
+__global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, const double *Ainv, int k, int P,
+                                                          double *cvec) {
     const int NR = 2 * P;
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -606,61 +584,107 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel3(const double *t, con
     if (lane == 0) { cvec[2 * e] = yr; cvec[2 * e + 1] = yi; }
 }
 
-// t[e] = sum over the row-block partials, in block order (e = c * NR + r).
-__global__ void coarse_sum_kernel(const double *partial, int nblk, int total, double *t) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    double s = 0;
-    for (int q = 0; q < nblk; ++q) s += partial[(size_t)q * total + e];
-    t[e] = s;
-}
-
-// c = Ainv t (complex; t (k, NR) real/imaginary columns), cvec (k, P) complex interleaved.
-__global__ void coarse_solve_kernel2(const double *t, const double *Ainv, int k, int P, double *cvec) {
-    const int NR = 2 * P;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= k * P) return;
-    const int a = e / P, p = e % P;
-    double yr = 0, yi = 0;
-    for (int b = 0; b < k; ++b) {
-        const double tr = t[b * NR + p], ti = t[b * NR + P + p];
-        const double ar = Ainv[2 * ((size_t)a * k + b)], ai = Ainv[2 * ((size_t)a * k + b) + 1];
-        yr += ar * tr - ai * ti;
-        yi += ar * ti + ai * tr;
-    }
-    cvec[2 * e] = yr; cvec[2 * e + 1] = yi;
-}
-
-// R = V - (RPhi + j w LPhi) c and Y = Phi c, one wave per row, lanes over the k coarse columns.
-__global__ __launch_bounds__(256) void coarse_prolong_kernel2(const float *Phi, const float *RPhi, const float *LPhi,
-                                                             int k, const double *cvec, double omega,
-                                                             const double *V, int n, int P, double *R, double *Y) {
-    const int NR = 2 * P;
+// R = V - (RPhi + j w LPhi) c and Y = Phi c from the transposed FP32 bases (k, n): a block covers 64
+// rows (the lanes: coalesced reads of each basis column) and splits the k coarse columns over its
+// cp_slices(P) waves; the coefficients c are wave-uniform loads. The slices' sums meet in LDS and
+// are added in slice order.
+constexpr int cp_slices(int P) { return P >= 16 ? 2 : P >= 8 ? 4 : 8; }
+template <int P>
+__global__ __launch_bounds__(512) void coarse_prolong_kernel(const float *__restrict__ Phit,
+                                                            const float *__restrict__ RPhit,
+                                                            const float *__restrict__ LPhit, int k,
+                                                            const double *__restrict__ cvec, double omega,
+                                                            const double *__restrict__ V, int n,
+                                                            double *__restrict__ R, double *__restrict__ Y) {
+    constexpr int NR = 2 * P, S = cp_slices(P), Q = 4 * P;
+    __shared__ double red[S * Q * 64];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    for (int p = 0; p < P; ++p) {
-        double ar = 0, ai = 0, yr = 0, yi = 0;
-        for (int c = lane; c < k; c += 64) {
-            const double cr = cvec[2 * (c * P + p)], ci = cvec[2 * (c * P + p) + 1];
-            const double r = RPhi[(size_t)i * k + c], l = omega * (double)LPhi[(size_t)i * k + c];
-            const double f = Phi[(size_t)i * k + c];
-            ar += r * cr - l * ci;
-            ai += r * ci + l * cr;
-            yr += f * cr;
-            yi += f * ci;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = blockIdx.x * 64 + lane;
+    double s[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s[q] = 0;
+    auto term = [&](int c, double f, double r, double l) {
+        const double *cv = cvec + (size_t)c * 2 * P;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double cr = cv[2 * p], ci = cv[2 * p + 1];
+            s[4 * p] += r * cr - l * ci;
+            s[4 * p + 1] += r * ci + l * cr;
+            s[4 * p + 2] += f * cr;
+            s[4 * p + 3] += f * ci;
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            ar += __shfl_down(ar, off); ai += __shfl_down(ai, off);
-            yr += __shfl_down(yr, off); yi += __shfl_down(yi, off);
+    };
+    const int ii = min(i, n - 1);
+    constexpr int U = 4;
+    int c = w;
+    for (; c + S * (U - 1) < k; c += S * U) {
+        float f[U], r[U], l[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t at = (size_t)(c + S * u) * n + ii;
+            f[u] = Phit[at]; r[u] = RPhit[at]; l[u] = LPhit[at];
         }
-        if (lane == 0) {
-            R[(size_t)i * NR + p] = V[(size_t)i * NR + p] - ar;
-            R[(size_t)i * NR + P + p] = V[(size_t)i * NR + P + p] - ai;
-            Y[(size_t)i * NR + p] = yr;
-            Y[(size_t)i * NR + P + p] = yi;
+#pragma unroll
+        for (int u = 0; u < U; ++u) term(c + S * u, f[u], r[u], omega * (double)l[u]);
+    }
+    for (; c < k; c += S) {
+        const size_t at = (size_t)c * n + ii;
+        term(c, Phit[at], RPhit[at], omega * (double)LPhit[at]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) red[(w * Q + q) * 64 + lane] = s[q];
+    __syncthreads();
+    for (int e = threadIdx.x; e < Q * 64; e += S * 64) {
+        const int q = e / 64, row = e % 64;
+        const int gi = blockIdx.x * 64 + row;
+        if (gi >= n) continue;
+        double acc = 0;
+        for (int ws = 0; ws < S; ++ws) acc += red[(ws * Q + q) * 64 + row];
+        const int p = q >> 2;
+        switch (q & 3) {
+        case 0: R[(size_t)gi * NR + p] = V[(size_t)gi * NR + p] - acc; break;
+        case 1: R[(size_t)gi * NR + P + p] = V[(size_t)gi * NR + P + p] - acc; break;
+        case 2: Y[(size_t)gi * NR + p] = acc; break;
+        default: Y[(size_t)gi * NR + P + p] = acc; break;
         }
     }
+}
+
+// Out (k, n) = In (n, k)^T, FP32.
+__global__ void transpose_kernel(const float *In, int n, int k, float *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * k) return;
+    const size_t i = e / k, c = e % k;
+    Out[c * n + i] = In[e];
+}
+
+// W = sum of the nsplit column-chunk partials of gemv_sym_kernel (a real product, no frequency).
+template <int NR>
+__global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * NR) return;
+    double s = 0;
+    for (int q = 0; q < nsplit; ++q) s += part[(size_t)q * n * NR + e];
+    W[e] = s;
+}
+
+// X[i][c] = Phi[i][c0 + c] for c < cn, 0 for cn <= c < ld (FP32 -> FP64).
+__global__ void gather_cols_kernel(const float *Phi, int n, int k, int c0, int cn, int ld, double *X) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * ld) return;
+    const size_t i = e / ld;
+    const int c = (int)(e % ld);
+    X[e] = c < cn ? (double)Phi[i * k + c0 + c] : 0.0;
+}
+
+// Out[c0 + c][i] = W[i][c] for c < cn (FP64 (n, ld) -> FP32 transposed (k, n)).
+__global__ void scatter_cols_kernel(const double *W, int n, int ld, int c0, int cn, float *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * ld) return;
+    const size_t i = e / ld;
+    const int c = (int)(e % ld);
+    if (c < cn) Out[(size_t)(c0 + c) * n + i] = (float)W[e];
 }
 
 // Y += X
@@ -686,7 +710,8 @@ struct pq_loops {
     double omega_bj = -1;
     // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
     int kc = 0;
-    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart;
+    bool cdiag = false;           // basis diagonalises the Galerkin matrices
+    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart;   // RPhi, LPhi transposed (k, n)
     std::vector<double> Rc, Lc;
     double omega_c = -1;
     DevBuf Hd;                    // Arnoldi coefficients per step (device)
@@ -739,32 +764,38 @@ void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P
 
 void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P);
 
-void apply_bj(pq_loops *L, const double *X, double *Y, int P);
+void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add = nullptr);
+
+// c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
+void coarse_correction(pq_loops *L, const double *X, int P) {
+    const int n = L->nf, NR = 2 * P, k = L->kc;
+    dispatch_nr(NR, [&](auto nr) {
+        constexpr int N = decltype(nr)::value;
+        const int nchunk = (n + cr_rows(N) - 1) / cr_rows(N);
+        hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
+                           L->Phi.as<float>(), k, X, n, L->cpart.as<double>());
+        if (L->cdiag) {
+            hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * P + 255) / 256), dim3(256), 0, L->stream,
+                               L->cpart.as<double>(), nchunk, k, P, L->Ainv.as<double>(), L->cvec.as<double>());
+        } else {
+            hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * N + 255) / 256), dim3(256), 0, L->stream,
+                               L->cpart.as<double>(), nchunk, k * N, L->ctot.as<double>());
+            hipLaunchKernelGGL(coarse_solve_kernel, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
+                               L->ctot.as<double>(), L->Ainv.as<double>(), k, P, L->cvec.as<double>());
+        }
+        hipLaunchKernelGGL(coarse_prolong_kernel<N / 2>, dim3((n + 63) / 64), dim3(64 * cp_slices(N / 2)), 0,
+                           L->stream, L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>(), k,
+                           L->cvec.as<double>(), L->omega_c, X, n, L->T1.as<double>(), L->T2.as<double>());
+    });
+    PQ_HIP(hipGetLastError());
+}
 
 void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P) {
     const int n = L->nf, NR = 2 * P;
     if (kind == 2) {
         // deflated two-level: c = Ac^-1 Phi^T x; y = Phi c + BJ(x - A Phi c)
-        const int k = L->kc;
-        const int nchunk = (n + CR_ROWS - 1) / CR_ROWS;
-        dispatch_nr(NR, [&](auto nr) {
-            constexpr int N = decltype(nr)::value;
-            hipLaunchKernelGGL(coarse_restrict_kernel3<N>, dim3((k + CR_COLS - 1) / CR_COLS, nchunk), dim3(256), 0,
-                               L->stream, L->Phit.as<float>(), k, X, n, L->cpart.as<double>());
-        });
-        hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * NR + 255) / 256), dim3(256), 0, L->stream,
-                           L->cpart.as<double>(), nchunk, k * NR, L->ctot.as<double>());
-        hipLaunchKernelGGL(coarse_solve_kernel3, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
-                           L->ctot.as<double>(), L->Ainv.as<double>(), k, P, L->cvec.as<double>());
-        hipLaunchKernelGGL(coarse_prolong_kernel2, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->Phi.as<float>(),
-                           L->RPhi.as<float>(), L->LPhi.as<float>(), k, L->cvec.as<double>(), L->omega_c, X, n, P,
-                           L->T1.as<double>(), L->T2.as<double>());
-        PQ_HIP(hipGetLastError());
-        apply_bj(L, L->T1.as<double>(), Y, P);
-        const size_t count = (size_t)n * NR;
-        hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream,
-                           L->T2.as<double>(), count, Y);
-        PQ_HIP(hipGetLastError());
+        coarse_correction(L, X, P);
+        apply_bj(L, L->T1.as<double>(), Y, P, L->T2.as<double>());
         return;
     }
     if (kind == 0) {
@@ -774,10 +805,11 @@ void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P) {
     apply_bj(L, X, Y, P);
 }
 
-void apply_bj(pq_loops *L, const double *X, double *Y, int P) {
+void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add) {
     dispatch_nr(2 * P, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk), dim3(256), 0, L->stream, L->inv.as<float>(), L->nf, X, Y);
+        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk), dim3(256), 0, L->stream, L->inv.as<float>(), L->nf, X,
+                           add, Y);
     });
     PQ_HIP(hipGetLastError());
 }
@@ -825,6 +857,24 @@ void small_inverse(std::vector<cplx> &a, int k, std::vector<cplx> &inv) {
 void coarse_at(pq_loops *L, double omega) {
     if (L->omega_c == omega) return;
     const int k = L->kc;
+    if (L->cdiag) {
+        // a basis that diagonalises both Galerkin matrices: Ac^-1 = diag(1 / (Rc_aa + j w Lc_aa))
+        std::vector<double> d(2 * (size_t)k, 0.0);
+        double amax = 0;
+        for (int a = 0; a < k; ++a)
+            amax = std::max(amax, std::abs(cplx(L->Rc[(size_t)a * k + a], omega * L->Lc[(size_t)a * k + a])));
+        for (int a = 0; a < k; ++a) {
+            const cplx z(L->Rc[(size_t)a * k + a], omega * L->Lc[(size_t)a * k + a]);
+            if (std::abs(z) <= 1e-12 * amax) continue;
+            const cplx v = 1.0 / z;
+            d[2 * a] = v.real(); d[2 * a + 1] = v.imag();
+        }
+        L->Ainv.reserve(d.size() * 8);
+        PQ_HIP(hipMemcpyAsync(L->Ainv.p, d.data(), d.size() * 8, hipMemcpyHostToDevice, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        L->omega_c = omega;
+        return;
+    }
     std::vector<cplx> a((size_t)k * k), inv;
     for (size_t e = 0; e < a.size(); ++e) a[e] = cplx(L->Rc[e], omega * L->Lc[e]);
     small_inverse(a, k, inv);
@@ -836,14 +886,45 @@ void coarse_at(pq_loops *L, double omega) {
     L->omega_c = omega;
 }
 
+// The coarse matrix at omega and the coarse work buffers for P columns.
+void prepare_coarse(pq_loops *L, double omega, int P) {
+    const int n = L->nf, NR = 2 * P;
+    coarse_at(L, omega);
+    const int nchunk = (n + cr_rows(NR) - 1) / cr_rows(NR);
+    L->cpart.reserve((size_t)nchunk * L->kc * NR * 8);
+    L->cvec.reserve((size_t)L->kc * P * 2 * 8);
+    L->ctot.reserve((size_t)L->kc * NR * 8);
+    L->T1.reserve((size_t)n * NR * 8); L->T2.reserve((size_t)n * NR * 8);
+}
+
+// The free-loop right-hand sides B = -Z_fs (n_free, 2P) of the port columns into L->B.
+void port_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols) {
+    const int n = L->nf;
+    L->sc.reserve(P * 4);
+    L->B.reserve((size_t)n * 2 * P * 8);
+    PQ_HIP(hipMemcpyAsync(L->sc.p, port_cols, P * 4, hipMemcpyHostToDevice, L->stream));
+    hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+                       L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega,
+                       L->B.as<double>());
+    PQ_HIP(hipGetLastError());
+}
+
+// h[i][p] for i < nv: conj(V_i) . W into device memory (partials, then their fixed-order sum).
+void dots_launch(pq_loops *L, const double *V, size_t vstride, int nv, const double *W, int P, double *h) {
+    const int nblk = std::min(RED_BLOCKS, (L->nf + 255) / 256);
+    dispatch_nr(2 * P, [&](auto nr) {
+        constexpr int N = decltype(nr)::value;
+        hipLaunchKernelGGL(dots_partial_kernel<N / 2>, dim3(nblk, nv), dim3(256), 0, L->stream, V, vstride, nv, W,
+                           L->nf, L->partial.as<double>());
+    });
+    hipLaunchKernelGGL(dots_reduce_kernel, dim3((nv * 2 * P + 3) / 4), dim3(256), 0, L->stream, L->partial.as<double>(),
+                       nblk, nv, 2 * P, h);
+    PQ_HIP(hipGetLastError());
+}
+
 // h[i][p] for i < nv: conj(V_i) . W, synchronously copied to host.
 void dots(pq_loops *L, const double *V, size_t vstride, int nv, const double *W, int P, double *h_host) {
-    const int nblk = std::min(RED_BLOCKS, (L->nf + 255) / 256);
-    hipLaunchKernelGGL(dots_partial_kernel, dim3(nblk, nv), dim3(256), 0, L->stream, V, vstride, nv, W, L->nf, P,
-                       L->partial.as<double>());
-    hipLaunchKernelGGL(dots_reduce_kernel, dim3((nv * P * 2 + 255) / 256), dim3(256), 0, L->stream,
-                       L->partial.as<double>(), nblk, nv, P, L->h.as<double>());
-    PQ_HIP(hipGetLastError());
+    dots_launch(L, V, vstride, nv, W, P, L->h.as<double>());
     if (h_host) {
         PQ_HIP(hipMemcpyAsync(h_host, L->h.p, (size_t)nv * P * 2 * 8, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
@@ -868,7 +949,7 @@ void scale_cols(pq_loops *L, const double *X, const std::vector<double> &s, int 
 void update(pq_loops *L, const double *V, size_t vstride, int nv, const double *h_host, double sign, int P,
             double *W) {
     PQ_HIP(hipMemcpyAsync(L->h.p, h_host, (size_t)nv * P * 2 * 8, hipMemcpyHostToDevice, L->stream));
-    const size_t count = (size_t)L->nf * P;
+    const size_t count = (size_t)L->nf * P * 4;
     hipLaunchKernelGGL(vec_update_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, V, vstride,
                        nv, L->h.as<double>(), sign, L->nf, P, W);
     PQ_HIP(hipGetLastError());
@@ -961,8 +1042,8 @@ int pq_loops_matrix(pq_loops *L, void **Ll_dev, int64_t *ldl) {
 
 // Port impedances at one angular frequency: X = -Z_ff^-1 Z_fs by GMRES, Zp = Z_ss + Z_sf X.
 int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_cols, int precond, double tol,
-                         int restart, int maxiter, double *Zp, double *X_host, int64_t *iterations,
-                         double *residual) {
+                         int restart, int maxiter, double *Zp, double *X_host, const double *X0_host,
+                         int64_t *iterations, double *residual) {
     return guard([&] {
         PQ_REQUIRE(L && port_cols && Zp, "pq_loops_solve_ports: null argument");
         PQ_REQUIRE(P >= 1 && tol > 0 && tol < 1 && restart >= 1 && maxiter >= 1, "pq_loops_solve_ports: invalid options");
@@ -989,14 +1070,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
                                L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, B);
             PQ_HIP(hipGetLastError());
-            if (precond == 2) {
-                coarse_at(L, omega);
-                const int nchunk = (n + CR_ROWS - 1) / CR_ROWS;
-                L->cpart.reserve((size_t)nchunk * L->kc * NR * 8);
-                L->cvec.reserve((size_t)L->kc * P * 2 * 8);
-                L->ctot.reserve((size_t)L->kc * NR * 8);
-                L->T1.reserve(vsz * 8); L->T2.reserve(vsz * 8);
-            }
+            if (precond == 2) prepare_coarse(L, omega, P);
             if (precond >= 1 && L->omega_bj != omega) {
                 L->nblk = (n + BS - 1) / BS;
                 L->inv.reserve((size_t)L->nblk * BS * BS * 2 * 4);
@@ -1011,11 +1085,19 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 PQ_REQUIRE(!sing, "block-Jacobi: a singular diagonal block");
                 L->omega_bj = omega;
             }
-            PQ_HIP(hipMemsetAsync(X, 0, vsz * 8, L->stream));
             std::vector<double> bnorm;
             norms(L, B, P, bnorm);
             for (auto &b : bnorm) b = std::max(b, 1e-300);
-            PQ_HIP(hipMemcpyAsync(R, B, vsz * 8, hipMemcpyDeviceToDevice, L->stream));   // x0 = 0
+            if (X0_host) {                                                     // R = B - A x0
+                PQ_HIP(hipMemcpyAsync(X, X0_host, vsz * 8, hipMemcpyHostToDevice, L->stream));
+                apply_operator(L, omega, X, R, P);
+                hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B,
+                                   vsz, R);
+                PQ_HIP(hipGetLastError());
+            } else {                                                           // x0 = 0
+                PQ_HIP(hipMemsetAsync(X, 0, vsz * 8, L->stream));
+                PQ_HIP(hipMemcpyAsync(R, B, vsz * 8, hipMemcpyDeviceToDevice, L->stream));
+            }
             // Per Arnoldi step j the device writes, at Hd + j * hstride: the two Gram-Schmidt
             // passes (m+1 coefficients each) and the new vector's squared norm; one asynchronous copy
             // per step brings them to pinned host memory. The host works one step behind, so the
@@ -1031,15 +1113,11 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             std::vector<hipEvent_t> ev(m);
             for (auto &e : ev) PQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             double *Hd = L->Hd.as<double>();
-            const int nblk_red = std::min(RED_BLOCKS, (n + 255) / 256);
             auto dots_to = [&](const double *Vb, size_t vs, int nv, const double *Wv, double *hout) {
-                hipLaunchKernelGGL(dots_partial_kernel, dim3(nblk_red, nv), dim3(256), 0, L->stream, Vb, vs, nv, Wv, n, P,
-                                   L->partial.as<double>());
-                hipLaunchKernelGGL(dots_reduce_kernel, dim3((nv * P * 2 + 255) / 256), dim3(256), 0, L->stream,
-                                   L->partial.as<double>(), nblk_red, nv, P, hout);
+                dots_launch(L, Vb, vs, nv, Wv, P, hout);
             };
             auto update_dev = [&](const double *Vb, size_t vs, int nv, const double *hdev, double sign, double *Wv) {
-                const size_t count = (size_t)n * P;
+                const size_t count = (size_t)n * P * 4;
                 hipLaunchKernelGGL(vec_update_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, Vb,
                                    vs, nv, hdev, sign, n, P, Wv);
             };
@@ -1191,74 +1269,119 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
     });
 }
 
-// The coarse space of the two-level preconditioner: Phi (n_free, k) FP64 host, row-major.
-// Rl Phi and Ll Phi and the Galerkin matrices Phi^T Rl Phi, Phi^T Ll Phi are formed on the device
-// once; each frequency then only inverts the k x k coarse matrix.
-int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi) {
+// The coarse space of the two-level preconditioner: Phi (n_free, k) FP64 host, row-major, rounded
+// to FP32 once (the basis every later product uses). Rl Phi and Ll Phi and the Galerkin matrices
+// Phi^T Rl Phi, Phi^T Ll Phi are formed on the device in chunks of 16 columns; each frequency then
+// only inverts the k x k coarse matrix.
+int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
     return guard([&] {
         PQ_REQUIRE(L && (k == 0 || Phi), "pq_loops_set_coarse: null argument");
         PQ_REQUIRE(k >= 0 && k <= 4096, "pq_loops_set_coarse: 0 <= k <= 4096");
         DeviceScope scope(L->device);
         L->kc = k;
+        L->cdiag = diagonal != 0;
         L->omega_c = -1;
         if (k == 0) return;
         const int n = L->nf;
-        L->Phi.reserve((size_t)n * k * 4); L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
-        DevBuf Phi64;
-        Phi64.reserve((size_t)n * k * 8);
-        PQ_HIP(hipMemcpyAsync(Phi64.p, Phi, (size_t)n * k * 8, hipMemcpyHostToDevice, L->stream));
-        // A Phi with omega = 1 on chunks of 16 columns: real part = Rl Phi, imaginary part = Ll Phi
-        constexpr int CH = 16;
-        DevBuf X, W;
-        X.reserve((size_t)n * 2 * CH * 8); W.reserve((size_t)n * 2 * CH * 8);
-        std::vector<double> hx((size_t)n * 2 * CH), hphi((size_t)n * k);
-        std::copy(Phi, Phi + (size_t)n * k, hphi.begin());
-        std::vector<double> hw((size_t)n * 2 * CH), hr((size_t)n * k), hl((size_t)n * k);
-        for (int c0 = 0; c0 < k; c0 += CH) {
-            const int cn = std::min(CH, k - c0);
-            std::fill(hx.begin(), hx.end(), 0.0);
-            for (int i = 0; i < n; ++i)
-                for (int c = 0; c < cn; ++c) hx[(size_t)i * 2 * CH + c] = hphi[(size_t)i * k + c0 + c];
-            PQ_HIP(hipMemcpyAsync(X.p, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, L->stream));
-            apply_operator(L, 1.0, X.as<double>(), W.as<double>(), CH);
-            PQ_HIP(hipMemcpyAsync(hw.data(), W.p, hw.size() * 8, hipMemcpyDeviceToHost, L->stream));
-            PQ_HIP(hipStreamSynchronize(L->stream));
-            for (int i = 0; i < n; ++i)
-                for (int c = 0; c < cn; ++c) {
-                    hr[(size_t)i * k + c0 + c] = hw[(size_t)i * 2 * CH + c];
-                    hl[(size_t)i * k + c0 + c] = hw[(size_t)i * 2 * CH + CH + c];
-                }
-        }
-        DevBuf RPhi64, LPhi64;
-        RPhi64.reserve(hr.size() * 8); LPhi64.reserve(hl.size() * 8);
-        PQ_HIP(hipMemcpyAsync(RPhi64.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, L->stream));
-        PQ_HIP(hipMemcpyAsync(LPhi64.p, hl.data(), hl.size() * 8, hipMemcpyHostToDevice, L->stream));
-        {   // FP32 copies for the per-iteration apply
+        L->Phi.reserve((size_t)n * k * 4); L->Phit.reserve((size_t)n * k * 4);
+        L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
+        {
             std::vector<float> f32((size_t)n * k);
-            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hphi[e];
-            PQ_HIP(hipMemcpy(L->Phi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
-            L->Phit.reserve((size_t)n * k * 4);
-            for (int i = 0; i < n; ++i)
-                for (int c = 0; c < k; ++c) f32[(size_t)c * n + i] = (float)hphi[(size_t)i * k + c];
-            PQ_HIP(hipMemcpy(L->Phit.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
-            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hr[e];
-            PQ_HIP(hipMemcpy(L->RPhi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
-            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)hl[e];
-            PQ_HIP(hipMemcpy(L->LPhi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice));
+            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)Phi[e];
+            PQ_HIP(hipMemcpyAsync(L->Phi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice, L->stream));
+            PQ_HIP(hipStreamSynchronize(L->stream));
         }
-        DevBuf G;
-        G.reserve((size_t)k * k * 8);
+        hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)(((size_t)n * k + 255) / 256)), dim3(256), 0, L->stream,
+                           L->Phi.as<float>(), n, k, L->Phit.as<float>());
+        constexpr int CH = 16;
+        const int chunk = std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT));
+        const int nsplit = (n + chunk - 1) / chunk;
+        const int nchunk = (n + cr_rows(CH) - 1) / cr_rows(CH);
+        DevBuf X, WR, WL, part, red, t;
+        X.reserve((size_t)n * CH * 8); WR.reserve((size_t)n * CH * 8); WL.reserve((size_t)n * CH * 8);
+        part.reserve((size_t)nsplit * n * CH * 8);
+        red.reserve((size_t)nchunk * k * CH * 8);
+        t.reserve((size_t)k * CH * 8);
         L->Rc.assign((size_t)k * k, 0.0);
         L->Lc.assign((size_t)k * k, 0.0);
-        hipLaunchKernelGGL(gram_kernel, dim3(k * k), dim3(256), 0, L->stream, Phi64.as<double>(), k, k,
-                           RPhi64.as<double>(), k, k, n, G.as<double>());
-        PQ_HIP(hipMemcpyAsync(L->Rc.data(), G.p, (size_t)k * k * 8, hipMemcpyDeviceToHost, L->stream));
+        std::vector<double> ht((size_t)k * CH);
+        const unsigned vb = (unsigned)(((size_t)n * CH + 255) / 256);
+        for (int c0 = 0; c0 < k; c0 += CH) {
+            const int cn = std::min(CH, k - c0);
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(vb), dim3(256), 0, L->stream, L->Phi.as<float>(), n, k, c0, cn,
+                               CH, X.as<double>());
+            hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                               L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
+            hipLaunchKernelGGL(gemv_sym_kernel<CH>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
+                               L->stream, L->Ll.as<float>(), L->ldl, n, X.as<double>(), chunk, part.as<double>());
+            hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
+                               WL.as<double>());
+            for (int which = 0; which < 2; ++which) {
+                const double *W = which ? WL.as<double>() : WR.as<double>();
+                hipLaunchKernelGGL(scatter_cols_kernel, dim3(vb), dim3(256), 0, L->stream, W, n, CH, c0, cn,
+                                   (which ? L->LPhi : L->RPhi).as<float>());
+                hipLaunchKernelGGL(coarse_restrict_kernel<CH>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
+                                   L->Phi.as<float>(), k, W, n, red.as<double>());
+                hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * CH + 255) / 256), dim3(256), 0, L->stream,
+                                   red.as<double>(), nchunk, k * CH, t.as<double>());
+                PQ_HIP(hipGetLastError());
+                PQ_HIP(hipMemcpyAsync(ht.data(), t.p, ht.size() * 8, hipMemcpyDeviceToHost, L->stream));
+                PQ_HIP(hipStreamSynchronize(L->stream));
+                std::vector<double> &G = which ? L->Lc : L->Rc;
+                for (int a = 0; a < k; ++a)
+                    for (int r = 0; r < cn; ++r) G[(size_t)a * k + c0 + r] = ht[(size_t)a * CH + r];
+            }
+        }
+        // the Galerkin matrices of symmetric operators: average away the rounding asymmetry
+        for (std::vector<double> *G : {&L->Rc, &L->Lc})
+            for (int a = 0; a < k; ++a)
+                for (int b = a + 1; b < k; ++b) {
+                    const double v = 0.5 * ((*G)[(size_t)a * k + b] + (*G)[(size_t)b * k + a]);
+                    (*G)[(size_t)a * k + b] = (*G)[(size_t)b * k + a] = v;
+                }
+    });
+}
+
+int pq_loops_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols, double *B_host) {
+    return guard([&] {
+        PQ_REQUIRE(L && port_cols && B_host, "pq_loops_rhs: null argument");
+        PQ_REQUIRE(P >= 1, "pq_loops_rhs: P >= 1");
+        for (int p = 0; p < P; ++p)
+            PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl, "pq_loops_rhs: port loop out of range");
+        DeviceScope scope(L->device);
+        if (L->nf == 0) return;
+        port_rhs(L, omega, P, port_cols);
+        PQ_HIP(hipMemcpyAsync(B_host, L->B.p, (size_t)L->nf * 2 * P * 8, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
-        hipLaunchKernelGGL(gram_kernel, dim3(k * k), dim3(256), 0, L->stream, Phi64.as<double>(), k, k,
-                           LPhi64.as<double>(), k, k, n, G.as<double>());
-        PQ_HIP(hipMemcpyAsync(L->Lc.data(), G.p, (size_t)k * k * 8, hipMemcpyDeviceToHost, L->stream));
+    });
+}
+
+int pq_loops_coarse_solution(pq_loops *L, double omega, int P, const int32_t *port_cols, double *Xc_host) {
+    return guard([&] {
+        PQ_REQUIRE(L && port_cols && Xc_host, "pq_loops_coarse_solution: null argument");
+        PQ_REQUIRE(P >= 1, "pq_loops_coarse_solution: P >= 1");
+        PQ_REQUIRE(L->kc > 0, "pq_loops_coarse_solution: no coarse space set");
+        for (int p = 0; p < P; ++p)
+            PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl,
+                       "pq_loops_coarse_solution: port loop out of range");
+        dispatch_nr(2 * P, [](auto) {});
+        DeviceScope scope(L->device);
+        const int n = L->nf;
+        if (n == 0) return;
+        port_rhs(L, omega, P, port_cols);
+        prepare_coarse(L, omega, P);
+        coarse_correction(L, L->B.as<double>(), P);
+        PQ_HIP(hipMemcpyAsync(Xc_host, L->T2.p, (size_t)n * 2 * P * 8, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
-        PQ_HIP(hipGetLastError());
+    });
+}
+
+int pq_loops_coarse_matrices(pq_loops *L, double *Rc, double *Lc) {
+    return guard([&] {
+        PQ_REQUIRE(L && Rc && Lc, "pq_loops_coarse_matrices: null argument");
+        PQ_REQUIRE(L->kc > 0, "pq_loops_coarse_matrices: no coarse space set");
+        std::copy(L->Rc.begin(), L->Rc.end(), Rc);
+        std::copy(L->Lc.begin(), L->Lc.end(), Lc);
     });
 }
 

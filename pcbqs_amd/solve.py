@@ -68,13 +68,29 @@ class PortSolver:
         self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
         self.setup_seconds = dict(topology=t1 - t0, device=time.perf_counter() - t1)
 
-    def set_coarse(self, phi):
-        """Coarse space (n_free, k) of the two-level preconditioner (None or k = 0: none)."""
+    def set_coarse(self, phi, diagonalize=True):
+        """Coarse space (n_free, k) of the two-level preconditioner (None or k = 0: none).
+
+        With diagonalize, the span is re-expressed in the generalised eigenbasis of the Galerkin
+        pair (Phi^T Rl Phi, Phi^T Ll Phi): Psi = Phi V with V^T Rc V = I and V^T Lc V = diag, so
+        every frequency's coarse solve is a diagonal scaling. Directions with a Galerkin
+        resistance below 1e-10 of the largest are dropped (they carry no loop current)."""
         phi = np.zeros((self.model.n_free, 0)) if phi is None else np.ascontiguousarray(phi, np.float64)
         if phi.ndim != 2 or phi.shape[0] != self.model.n_free:
             raise ValueError('coarse space must have one row per free loop')
-        call('pq_loops_set_coarse', self._handle, phi.shape[1], ptr(phi) if phi.shape[1] else None)
-        self.coarse_k = phi.shape[1]
+        k = phi.shape[1]
+        call('pq_loops_set_coarse', self._handle, k, ptr(phi) if k else None, 0)
+        if k and diagonalize:
+            Rc, Lc = np.empty((k, k)), np.empty((k, k))
+            call('pq_loops_coarse_matrices', self._handle, ptr(Rc), ptr(Lc))
+            s, U = np.linalg.eigh(0.5 * (Rc + Rc.T))
+            keep = s > 1e-10 * s.max()
+            W = U[:, keep] / np.sqrt(s[keep])
+            lam, Q = np.linalg.eigh(W.T @ (0.5 * (Lc + Lc.T)) @ W)
+            phi = np.ascontiguousarray(phi @ (W @ Q))
+            k = phi.shape[1]
+            call('pq_loops_set_coarse', self._handle, k, ptr(phi), 1)
+        self.coarse_k = k
 
     def set_stream_coarse(self, grid=8, vias=True, extra=None):
         """The geometric coarse space: stream-function hats of a grid x grid net per layer, the
@@ -92,8 +108,34 @@ class PortSolver:
     def n_ports(self):
         return len(self.port_cols)
 
-    def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False):
-        """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds)."""
+    def _split(self, x):
+        """(n_free, P) complex -> (n_free, 2P) FP64 (real parts first), the library's layout."""
+        return np.ascontiguousarray(np.hstack([x.real, x.imag]), np.float64)
+
+    def _join(self, y):
+        P = y.shape[1] // 2
+        return y[:, :P] + 1j * y[:, P:]
+
+    def rhs(self, frequency):
+        """Free-loop right-hand sides -Z_fs (n_free, P) complex at one frequency."""
+        P = self.n_ports
+        B = np.zeros((self.model.n_free, 2 * P))
+        call('pq_loops_rhs', self._handle, 2 * np.pi * float(frequency), P, ptr(self.port_cols), ptr(B))
+        return self._join(B)
+
+    def coarse_solution(self, frequency):
+        """The coarse-space solution Phi (Phi^T Z Phi)^-1 Phi^T b (n_free, P) complex of the port
+        right-hand sides (needs a coarse space)."""
+        P = self.n_ports
+        X = np.zeros((self.model.n_free, 2 * P))
+        call('pq_loops_coarse_solution', self._handle, 2 * np.pi * float(frequency), P, ptr(self.port_cols), ptr(X))
+        return self._join(X)
+
+    def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False, x0=None):
+        """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds).
+
+        x0: optional initial iterates, (F, n_free, P) complex or a callable frequency -> (n_free, P)
+        (the learned correction's prediction); GMRES starts from zero otherwise."""
         P = self.n_ports
         f = np.atleast_1d(np.asarray(freqs, float))
         if f.ndim != 1 or not np.isfinite(f).all() or (f < 0).any():
@@ -106,9 +148,15 @@ class PortSolver:
             zp = np.zeros((P, P, 2))
             its, res = ctypes.c_int64(), ctypes.c_double()
             xh = X[i] if currents else None
+            start = None
+            if x0 is not None:
+                guess = x0(fi) if callable(x0) else x0[i]
+                if guess.shape != (self.model.n_free, P):
+                    raise ValueError('initial iterate must be (n_free, P)')
+                start = self._split(guess)
             call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, P, ptr(self.port_cols), kind, float(tol),
-                 int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None, ctypes.byref(its),
-                 ctypes.byref(res))
+                 int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None,
+                 ptr(start) if start is not None else None, ctypes.byref(its), ctypes.byref(res))
             Z[i] = zp[..., 0] + 1j * zp[..., 1]
             seconds, mv = ctypes.c_double(), ctypes.c_int64()
             call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))

@@ -1,12 +1,16 @@
-"""Benchmark: fill Gpair/s (and the GMRES solve) on the 30k-RWG seeded plane pair.
+"""Benchmark: fill Gpair/s and the port GMRES solve on the 30k-RWG seeded plane pair.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed S] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed S] [--no-cpu] [--no-gmres]
 
 A step is one device-resident fill of the board's four moment planes (FP32 far pairs,
 FP64 analytic near pairs) with the inputs already in HBM; value = n(n+1)/2 / seconds / 1e9
 (the reference's pair count, src/pcbqs/static.py:39). The dominant kernel's duration comes
 from HIP events recorded on its launch stream. The CPU baseline is the reference's own
 compiled static_fill (built into oracle/_ref) on the same mesh, on this host's cores.
+
+The GMRES part solves the board's 4 ports at 1 MHz to a relative residual of 1e-8 with plain
+block-Jacobi GMRES and with the learned two-level preconditioner (seconds, iterations, ratio),
+and times the plain path's operator product on the host cores.
 """
 import argparse
 import json
@@ -54,6 +58,77 @@ def cpu_baseline(board):
                 seconds=seconds)
 
 
+def gmres_section(board, freq=1e6, grid=24, repeats=3, cpu=True):
+    """Port solve at one frequency on the same board: plain block-Jacobi GMRES against two-level
+    GMRES whose coarse space holds the grid x grid hats and the learned enrichment. Seconds are
+    the solver's own wall clock per solve (median of the repeats; block-Jacobi's per-frequency
+    set-up is inside it); the per-board set-up of each method is reported beside it."""
+    from pcbqs_amd.learned import MPNN, set_learned_coarse
+    from pcbqs_amd.solve import PortSolver
+    t = time.perf_counter()
+    solver = PortSolver(board, device=0)
+    board_setup = time.perf_counter() - t
+
+    def run(precond):
+        out = [solver.ports([freq], precond=precond, tol=1e-8) for _ in range(repeats)]
+        secs = [o[1][0]['seconds'] for o in out]
+        i = int(np.argsort(secs)[len(secs) // 2])
+        return out[i][0][0], out[i][1][0]
+    Zp, plain = run('block-jacobi')
+    t = time.perf_counter()
+    net = MPNN.load()
+    net_s = set_learned_coarse(solver, net, grid)
+    coarse_setup = time.perf_counter() - t
+    Zl, learned = run('two-level')
+    section = dict(
+        frequency_Hz=freq, ports=solver.n_ports, free_loops=solver.model.n_free, tol=1e-8,
+        plain=dict(preconditioner='block-jacobi (64-loop blocks)', seconds=plain['seconds'],
+                   iterations=plain['iterations'], relative_residual=plain['relative_residual']),
+        learned=dict(preconditioner=f'two-level: {grid}x{grid} hats + via loops + learned enrichment',
+                     seconds=learned['seconds'], iterations=learned['iterations'],
+                     relative_residual=learned['relative_residual'], coarse_columns=solver.coarse_k,
+                     board_setup_s=coarse_setup, network_s=net_s),
+        ratio=learned['seconds'] / plain['seconds'],
+        port_z_rel_diff=float(np.abs(Zl - Zp).max() / np.abs(Zp).max()),
+        board_setup_s=board_setup)
+    if cpu:
+        section['cpu_operator'] = cpu_operator_sample(solver, plain['iterations'])
+    solver.close()
+    return section
+
+
+def cpu_operator_sample(solver, iterations, products=5):
+    """The plain path's operator product (Z_ff on 2P real columns, FP64 numpy on the host cores)
+    timed over a few products and scaled to the plain solve's iteration count: a bounded sample of
+    the CPU GMRES, whose time is ~98% operator products."""
+    import ctypes
+    from pcbqs_amd._lib import call
+    n, P = solver.model.n_free, solver.n_ports
+    dev, ld = ctypes.c_void_p(), ctypes.c_int64()
+    call('pq_loops_matrix', solver._handle, ctypes.byref(dev), ctypes.byref(ld))
+    L32 = np.empty((n, ld.value), np.float32)
+    call('pq_memcpy_d2h', L32.ctypes.data_as(ctypes.c_void_p), dev, L32.nbytes)
+    L = np.ascontiguousarray(L32[:, :n], np.float64)
+    del L32
+    R = solver.model.Rl.tocsr()[:n, :n]
+    X = np.random.default_rng(0).normal(size=(n, 2 * P))
+    omega = 2 * np.pi * 1e6
+    L @ X
+    t = time.perf_counter()
+    for _ in range(products):
+        Y = R @ X
+        Y[:, :P] -= omega * (L @ X[:, P:])
+        Y[:, P:] += omega * (L @ X[:, :P])
+    per = (time.perf_counter() - t) / products
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except OSError:
+        cores = os.cpu_count() or 1
+    return dict(seconds_per_product=per, estimated_solve_s=per * iterations, cores=cores, kind='port',
+                sample=f'{products} FP64 operator products (n={n}, {2 * P} real columns) with numpy, '
+                       f'scaled to the {iterations} iterations of the plain solve')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -61,6 +136,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-gmres', action='store_true')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
                     help='first copy BYTES device-to-device (the scale of a counter run)')
     args = ap.parse_args()
@@ -117,6 +193,8 @@ def main():
                      near_pairs=list(near_pairs), fp64_far_ms=f64, fp64_near_ms=n64,
                      fp64_gpair_s=pairs / ((f64 + n64) * 1e-3) / 1e9),
     )
+    if not args.no_gmres:
+        line['gmres'] = gmres_section(board, cpu=not args.no_cpu)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(board)
     print(json.dumps(line))

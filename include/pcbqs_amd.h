@@ -196,6 +196,36 @@ int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
  * src/pcbqs/mqs_iterative.py:229-230). */
 int pq_loops_last_solve(pq_loops *loops, int64_t *iterations, int64_t *matvecs, double *residual, double *seconds);
 
+/* ---- message-passing network (the learned coarse-space correction) -------
+ * No counterpart in the reference. All pointers are device pointers; stream is
+ * a hipStream_t (NULL: the default stream). Activations are FP32 row-major;
+ * the dense products round both operands to BF16 and accumulate in FP32 on
+ * the matrix cores. Supported (k1, k2, m) shapes: (16, 0, 64), (64, 64, 64),
+ * (64, 0, 128), (64, 0, 16). */
+
+/* C (N, m) = act([A1 | A2] W + bias) (+ res): A1 (N, k1), A2 (N, k2), W
+ * (k1 + k2, m) row-major; relu != 0 applies max(0, .) before the residual res
+ * (optional); Zpre (optional, (N, m)) receives the pre-activation. */
+int pq_mpnn_gemm(int N, const float *A1, int lda1, int k1, const float *A2, int lda2, int k2, const float *W, int m,
+                 const float *bias, const float *res, int ldres, int relu, float *C, int ldc, float *Zpre,
+                 void *stream);
+
+/* Workspace (floats) that pq_mpnn_gemm_tn needs for N rows and a (k, m) weight. */
+int pq_mpnn_workspace(int N, int k, int m, size_t *floats);
+
+/* Weight gradients: dW (k1 + k2, m) = [A1 | A2]^T G and db (m) = column sums
+ * of G (N, m), summed in a fixed order. */
+int pq_mpnn_gemm_tn(int N, const float *A1, int lda1, int k1, const float *A2, int lda2, int k2, const float *G,
+                    int ldg, int m, float *work, size_t work_floats, float *dW, float *db, void *stream);
+
+/* out (N, 64) = base + base2 + (sum over the CSR row i of w[t] in[col[t]]):
+ * the neighbour aggregation (base, base2 optional). */
+int pq_mpnn_aggregate(int N, const int32_t *ptr, const int32_t *col, const float *w, const float *in, int ldi,
+                      const float *base, int ldb, const float *base2, int ldb2, float *out, int ldo, void *stream);
+
+/* out = g where z > 0 else 0 (count elements): the ReLU's backward. */
+int pq_mpnn_relu_mask(size_t count, const float *z, const float *g, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

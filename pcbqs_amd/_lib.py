@@ -62,6 +62,11 @@ _SIGNATURES = {
                              ctypes.c_int),
     'pq_loops_set_coarse': ([P, ctypes.c_int, P, ctypes.c_int], ctypes.c_int),
     'pq_loops_coarse_matrices': ([P, P, P], ctypes.c_int),
+    'pq_mpnn_gemm': ([ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_mpnn_workspace': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+    'pq_mpnn_gemm_tn': ([ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, SIZE, P, P, P], ctypes.c_int),
+    'pq_mpnn_aggregate': ([ctypes.c_int, P, P, P, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P], ctypes.c_int),
+    'pq_mpnn_relu_mask': ([SIZE, P, P, P, P], ctypes.c_int),
     'pq_loops_rhs': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
@@ -80,6 +85,13 @@ def lib():
         if not os.path.exists(LIBRARY):
             raise RuntimeError(f'{LIBRARY} is not built: run __graft_entry__.build() '
                                '(make -C pcbqs_amd/csrc); there is no CPU fallback')
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7). Whichever is
+        # loaded first serves the whole process, so torch goes first: the learned correction's
+        # tensors and this library then share one runtime (and torch gets the one it was built for).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIBRARY)
         for name, (args, res) in _SIGNATURES.items():
             fn = getattr(handle, name)

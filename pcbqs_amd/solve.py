@@ -79,6 +79,7 @@ class PortSolver:
         if phi.ndim != 2 or phi.shape[0] != self.model.n_free:
             raise ValueError('coarse space must have one row per free loop')
         k = phi.shape[1]
+        self.coarse_phi = phi
         call('pq_loops_set_coarse', self._handle, k, ptr(phi) if k else None, 0)
         if k and diagonalize:
             Rc, Lc = np.empty((k, k)), np.empty((k, k))
@@ -92,9 +93,10 @@ class PortSolver:
             call('pq_loops_set_coarse', self._handle, k, ptr(phi), 1)
         self.coarse_k = k
 
-    def set_stream_coarse(self, grid=8, vias=True, extra=None):
+    def set_stream_coarse(self, grid=8, vias=True, extra=None, extra_loops=None):
         """The geometric coarse space: stream-function hats of a grid x grid net per layer, the
-        via-closed loops (vias=True), plus extra branch patterns (nb, k') when given."""
+        via-closed loops (vias=True), plus extra branch patterns (nb, k') and extra loop-space
+        columns (n_free, k'') when given."""
         from .coarse import loop_restrict, stream_patterns, via_loop_columns
         patterns = stream_patterns(self.model, grid)
         if extra is not None and extra.shape[1]:
@@ -102,6 +104,8 @@ class PortSolver:
         cols = [loop_restrict(self.model, patterns)]
         if vias:
             cols.append(via_loop_columns(self.model))
+        if extra_loops is not None and extra_loops.shape[1]:
+            cols.append(extra_loops)
         self.set_coarse(np.hstack(cols))
 
     @property

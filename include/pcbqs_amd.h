@@ -196,6 +196,18 @@ int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
  * src/pcbqs/mqs_iterative.py:229-230). */
 int pq_loops_last_solve(pq_loops *loops, int64_t *iterations, int64_t *matvecs, double *residual, double *seconds);
 
+/* ---- Biot-Savart field --------------------------------------------------------
+ * H (np, 3, K, 2) FP64 host (real, imaginary) at points pts (np, 3) of ns sheet
+ * sources src (ns, 3 + 4K: position, then Jx re, Jx im, Jy re, Jy im per
+ * excitation, each current times its quadrature weight) and nseg straight
+ * filaments seg (nseg, 6 + 2K: p0, p1, then I re, I im per excitation), K in
+ * {1, 2, 4, 8}. precision 64 (parity) or 32 computes on the device; kernel_ms
+ * (optional) receives the kernels' time. Replaces _h_tris and h_segments
+ * (src/pcbqs/biot.py:24-51, 82-105); the caller builds the refined source list
+ * of h_sheets (biot.py:54-79). */
+int pq_biot_field(int device, int precision, int K, int64_t ns, const double *src, int64_t nseg, const double *seg,
+                  int64_t np, const double *pts, double *H, double *kernel_ms);
+
 /* ---- message-passing network (the learned coarse-space correction) -------
  * No counterpart in the reference. All pointers are device pointers; stream is
  * a hipStream_t (NULL: the default stream). Activations are FP32 row-major;

@@ -1,0 +1,129 @@
+"""Biot-Savart field reconstruction on the GPU (kernels in csrc/biot.hip).
+
+Mirrors the reference's interface:
+  rwg_currents(rwg, I)            src/pcbqs/_numerics/currents.py:5-18  J = a r + b per triangle
+  h_sheets(sheets, pts, ...)      src/pcbqs/biot.py:54-79               6-point rule; triangles whose
+                                                                       centroid lies within
+                                                                       near x 2 x size of some point
+                                                                       are split 4x, levels times
+  h_segments(p0, p1, I, pts)      src/pcbqs/biot.py:82-105              finite straight filaments
+  h_field(sheets, pts, segments)  both in one launch sequence
+The host builds the source list (quadrature points with their weighted currents); the device sums
+every (point, source) pair. precision=64 is the parity mode, precision=32 the fast one.
+"""
+import ctypes
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from ._lib import call, ptr
+from .geometry import BAR6, W6, _split4
+
+
+def rwg_currents(rwg, I):
+    """Per-triangle (a (nt[, k]), b (nt, 2[, k])) with J = a r + b from RWG coefficients I (Ne[, k])."""
+    I = np.asarray(I)
+    multi = I.ndim > 1
+    I2 = I.reshape(len(I), -1)
+    nt = len(rwg.tris)
+    a = np.zeros((nt, I2.shape[1]), complex)
+    b = np.zeros((nt, 2, I2.shape[1]), complex)
+    for tri, alpha, v in ((rwg.tp, rwg.l / (2 * rwg.area[rwg.tp]), rwg.vp),
+                          (rwg.tm, -rwg.l / (2 * rwg.area[rwg.tm]), rwg.vm)):
+        ia = I2 * alpha[:, None]
+        np.add.at(a, tri, ia)
+        np.add.at(b, tri, -ia[:, None, :] * v[:, :, None])
+    return (a, b) if multi else (a[:, 0], b[:, :, 0])
+
+
+def _padded(K):
+    for k in (1, 2, 4, 8):
+        if K <= k:
+            return k
+    raise ValueError('at most 8 excitations per call')
+
+
+def sheet_sources(sheets, pts, K, near=1.5, levels=2):
+    """(ns, 3 + 4 K) sources: position, then (Jx re, Jx im, Jy re, Jy im) per excitation, each
+    current times its quadrature weight."""
+    pts = np.asarray(pts, float)
+    tree = cKDTree(pts) if len(pts) else None
+    out = []
+    for nodes, tris, a, b, z in sheets:
+        V = np.asarray(nodes, float)[np.asarray(tris)]
+        a = np.asarray(a).reshape(len(V), -1)
+        b = np.asarray(b).reshape(len(V), 2, -1)
+        e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+        size = np.sqrt(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]) / 2)
+        cen = V.mean(1)
+        if tree is not None:
+            dist, _ = tree.query(np.c_[cen, np.full(len(cen), z)])
+        else:
+            dist = np.full(len(cen), np.inf)
+        nearm = ~(dist > near * 2 * size)
+        groups = [(V[~nearm], a[~nearm], b[~nearm])]
+        Vn, an, bn = V[nearm], a[nearm], b[nearm]
+        for _ in range(levels if len(Vn) else 0):
+            Vn = _split4(Vn)
+            an, bn = np.tile(an, (4, 1)), np.tile(bn, (4, 1, 1))
+        groups.append((Vn, an, bn))
+        for Vg, ag, bg in groups:
+            if not len(Vg):
+                continue
+            ar = 0.5 * np.abs((Vg[:, 1, 0] - Vg[:, 0, 0]) * (Vg[:, 2, 1] - Vg[:, 0, 1]) -
+                              (Vg[:, 1, 1] - Vg[:, 0, 1]) * (Vg[:, 2, 0] - Vg[:, 0, 0]))
+            Q = np.einsum('qk,tkc->tqc', BAR6, Vg)                       # (t, 6, 2)
+            w = ar[:, None] * W6[None]                                    # (t, 6)
+            k = ag.shape[1]
+            Jx = (ag[:, None, :] * Q[..., 0:1] + bg[:, None, 0, :]) * w[..., None]
+            Jy = (ag[:, None, :] * Q[..., 1:2] + bg[:, None, 1, :]) * w[..., None]
+            S = np.zeros((len(Vg) * 6, 3 + 4 * K))
+            S[:, 0], S[:, 1], S[:, 2] = Q[..., 0].ravel(), Q[..., 1].ravel(), z
+            Jx, Jy = Jx.reshape(-1, k), Jy.reshape(-1, k)
+            S[:, 3:3 + 4 * k:4], S[:, 4:4 + 4 * k:4] = Jx.real, Jx.imag
+            S[:, 5:5 + 4 * k:4], S[:, 6:6 + 4 * k:4] = Jy.real, Jy.imag
+            out.append(S)
+    return np.concatenate(out) if out else np.zeros((0, 3 + 4 * K))
+
+
+def segment_sources(p0, p1, I, K):
+    p0, p1 = np.asarray(p0, float).reshape(-1, 3), np.asarray(p1, float).reshape(-1, 3)
+    I = np.asarray(I).reshape(len(p0), -1)
+    S = np.zeros((len(p0), 6 + 2 * K))
+    S[:, 0:3], S[:, 3:6] = p0, p1
+    S[:, 6:6 + 2 * I.shape[1]:2], S[:, 7:7 + 2 * I.shape[1]:2] = I.real, I.imag
+    return S
+
+
+def h_field(sheets, pts, segments=None, near=1.5, levels=2, precision=64, device=0, timing=None):
+    """H (npts, 3[, k]) complex of sheets [(nodes, tris, a, b, z)] and segments (p0, p1, I)."""
+    pts = np.ascontiguousarray(np.asarray(pts, float).reshape(-1, 3))
+    ks = [np.asarray(a).reshape(len(np.asarray(tris)), -1).shape[1] for _, tris, a, _, _ in sheets]
+    multi = any(np.ndim(a) > 1 for _, _, a, _, _ in sheets)
+    if segments is not None:
+        Iseg = np.asarray(segments[2])
+        ks.append(Iseg.reshape(len(Iseg), -1).shape[1] if len(Iseg) else 1)
+        multi = multi or Iseg.ndim > 1
+    k = max(ks) if ks else 1
+    K = _padded(k)
+    src = np.ascontiguousarray(sheet_sources(sheets, pts, K, near, levels))
+    seg = np.ascontiguousarray(segment_sources(*segments, K) if segments is not None and len(segments[0])
+                               else np.zeros((0, 6 + 2 * K)))
+    H = np.zeros((len(pts), 3, K, 2))
+    ms = ctypes.c_double()
+    call('pq_biot_field', device, precision, K, len(src), ptr(src) if len(src) else None, len(seg),
+         ptr(seg) if len(seg) else None, len(pts), ptr(pts), ptr(H), ctypes.byref(ms))
+    if timing is not None:
+        timing.update(kernel_ms=ms.value, sources=len(src), segments=len(seg), points=len(pts))
+    Hc = (H[..., 0] + 1j * H[..., 1])[:, :, :k]
+    return Hc if multi else Hc[:, :, 0]
+
+
+def h_sheets(sheets, pts, near=1.5, levels=2, precision=64, device=0):
+    """The reference's h_sheets on the device."""
+    return h_field(sheets, pts, None, near, levels, precision, device)
+
+
+def h_segments(p0, p1, I, pts, precision=64, device=0):
+    """The reference's h_segments on the device."""
+    return h_field([], pts, (p0, p1, I), precision=precision, device=device)

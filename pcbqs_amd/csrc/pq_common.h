@@ -49,10 +49,15 @@ template <class F> int guard(F &&body) {
     }
 }
 
-// A device allocation owned by a handle.
+// A device allocation, freed when its owner (a handle or a scope) ends. Not copyable.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    ~DevBuf() { release(); }
     void reserve(size_t n) {
         if (n <= bytes) return;
         release();

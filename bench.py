@@ -29,6 +29,16 @@ F_FULL = 209               # flop per far pair, full moments
 B_FULL_FP32 = 32           # bytes stored per unordered pair: 4 planes x 2 entries x 4 B
 
 
+def measured_traffic(seed):
+    """HBM bytes per launch of the far kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over this workload (profiles/far32_tile_c2_30k_traffic.json, seed 0), else None."""
+    path = os.path.join(ROOT, 'profiles', 'far32_tile_c2_30k_traffic.json')
+    if seed != 0 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)['traffic_bytes']
+
+
 def cpu_baseline(board):
     """Time the reference's compiled static_fill (oracle/_ref) on the whole board, or, where it
     is not built, the C restatement (oracle/libcpu_ref.so), on this host's cores."""
@@ -186,8 +196,9 @@ def main():
                              f'{board.info.get("cutouts")} cutouts, {len(board.vseg)} vias, 4 ports; '
                              f'near=3, outer_levels=1', triangles=n, pairs=pairs, analytic_pairs=analytic),
         roofline=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                      frac=achieved / HBM_PEAK, traffic=None,
-                      kernel='far32_kernel (full moments)',
+                      frac=achieved / HBM_PEAK, traffic=measured_traffic(args.seed),
+                      algorithmic_bytes=pairs * B_FULL_FP32,
+                      kernel='far32_tile_kernel (FP32 far pairs, full moments, both reciprocal entries)',
                       valu_frac=pairs * F_FULL / far / FP32_VALU_PEAK),
         kernels=dict(far_ms=float(np.mean(far_ms)), near_ms=float(np.mean(near_ms)),
                      near_pairs=list(near_pairs), fp64_far_ms=f64, fp64_near_ms=n64,

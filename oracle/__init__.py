@@ -9,7 +9,8 @@ package, and only as the checker (never as the thing measured or shipped).
                 reference's own compiled core built into oracle/_ref (when present).
   oracle.mqs    RWG basis, sheet resistance, via R/L, loop basis, dense port solve.
   oracle.biot   Biot-Savart of RWG sheet currents and of straight filaments.
-  oracle.gmres  restarted and lockstep GMRES, and the graph-Laplacian preconditioner.
+  oracle.mpnn   the message-passing network of the learned correction, FP64 with the
+                device's BF16 rounding points.
 
 Parity status of each function is stated in its docstring (what pins it).
 """

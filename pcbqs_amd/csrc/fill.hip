@@ -4,12 +4,16 @@
 // static_fill) and the device row-strip contract of src/pcbqs/static_cupy.py:127-197.
 //
 // Work split:
-//   far kernel   one wave per matrix row, the 64 lanes over 64 consecutive columns, so every
-//                plane store is one coalesced 256 B (FP32) / 512 B (FP64) row segment; the row
-//                triangle's data is wave-uniform (scalar loads), the column triangle's is per
-//                lane. Each entry is computed from its canonical pair (min, max), which makes
-//                reciprocal entries bitwise equal. Pairs the near rule keeps analytic are not
-//                integrated here: they are appended (one atomic per wave) to per-level lists.
+//   far32_tile_kernel  the device-resident FP32 fill: 512-thread blocks over the 64 x 64 tiles of
+//                the upper triangle. Row and column triangles' records sit in LDS; each unordered
+//                pair is integrated once in its canonical order; forward entries are stored from
+//                registers and reverse entries are staged through LDS so that both are coalesced
+//                row segments. Moments are FP32 in each triangle's local frame.
+//   far32_kernel / far64_kernel  row-strip fills (the host fill_symmetric path and the FP64
+//                parity layout): four rows per wave, the 64 lanes over consecutive columns, every
+//                entry from its canonical pair (min, max), so reciprocal entries are bitwise equal.
+//   Pairs the near rule keeps analytic are not integrated by the far kernels: they go to
+//                block-local LDS lists, flushed with one global atomic per block and level.
 //   near kernel  FP64 closed-form source integrals, one thread per (pair, observation point);
 //                each pair's observation sum is then taken sequentially in point order, the
 //                reference's order.
@@ -192,12 +196,6 @@ __device__ __forceinline__ void wave_append(bool want, unsigned long long *list,
     if (want && base + rank < cap) list[base + rank] = entry;
 }
 
-__device__ __forceinline__ void wave_count(bool want, unsigned long long *count) {
-    const unsigned long long mask = __ballot(want);
-    if (!mask) return;
-    if (__lane_id() == __ffsll((long long)mask) - 1) atomicAdd(count, (unsigned long long)__popcll(mask));
-}
-
 // ---- far kernels ---------------------------------------------------------------------------
 
 struct FarArgs {
@@ -287,16 +285,6 @@ __device__ __forceinline__ void block_near_flush(const BlockNear &B, const FarAr
             if (at < A.lists.cap) A.lists.list[which][at] = B.list[which][i];
         }
     }
-}
-
-__device__ __forceinline__ void handle_near(const FarArgs &A, bool valid, int kind, int r, int c) {
-    const bool near_entry = valid && kind >= 0 && (r <= c || !A.mirror);
-    const unsigned long long e = ((unsigned long long)(unsigned)r << 32) | (unsigned)c;
-    const bool l1 = kind > 0;
-    wave_append(near_entry && !l1, A.lists.list[0], A.lists.count + 0, A.lists.cap, e);
-    wave_append(near_entry && l1, A.lists.list[1], A.lists.count + 1, A.lists.cap, e);
-    if (!A.mirror) wave_count(valid && kind >= 0 && r <= c, A.lists.count + 2);
-    else wave_count(near_entry, A.lists.count + 2);
 }
 
 // FP32 local-frame far kernel.

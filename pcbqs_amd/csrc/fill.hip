@@ -105,9 +105,21 @@ __device__ __forceinline__ void inner_integrals(const GeomDev &g, int q, double 
         const double t0 = vx * Nq[2 * e] + vy * Nq[2 * e + 1];
         const double r02 = t0 * t0 + d * d, r0 = sqrt(r02);
         const double rm = sqrt(r02 + sm * sm), rp = sqrt(r02 + sp * sp);
-        const double lg = r0 > 1e-14 * Lq[e] ? asinh(sp / r0) - asinh(sm / r0) : 0.0;
+        // asinh(s / r0) = log(A(s)) - log(r0) with A(s) = s + R, or r0^2 / (R - s) for s < 0 (the
+        // same number without cancellation): the difference of the two asinh is one log
+        double lg = 0.0;
+        if (r0 > 1e-14 * Lq[e]) {
+            const double ap = sp >= 0 ? sp + rp : r02 / (rp - sp);
+            const double am = sm >= 0 ? sm + rm : r02 / (rm - sm);
+            lg = log(ap / am);
+        }
         double term = t0 * lg;
-        if (ad > 0) term -= ad * (atan2(t0 * sp, r02 + ad * rp) - atan2(t0 * sm, r02 + ad * rm));
+        if (ad > 0) {
+            // atan2(y1, x1) - atan2(y2, x2) with x1, x2 > 0: both angles lie in (-pi/2, pi/2), so the
+            // difference lies in (-pi, pi) and is the angle of (x1 x2 + y1 y2, y1 x2 - y2 x1)
+            const double y1 = t0 * sp, x1 = r02 + ad * rp, y2 = t0 * sm, x2 = r02 + ad * rm;
+            term -= ad * atan2(y1 * x2 - y2 * x1, x1 * x2 + y1 * y2);
+        }
         I1 += term;
         if (!SCALAR) {
             const double er = .5 * (r02 * lg + sp * rp - sm * rm);

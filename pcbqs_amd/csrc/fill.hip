@@ -474,9 +474,14 @@ __global__ __launch_bounds__(512) void far32_tile_kernel(GeomDev g, FarArgs A, f
                 rr = ax[0] * Tx[0] + ay[0] * Ty[0] + ax[1] * Tx[1] + ay[1] * Ty[1] + ax[2] * Tx[2] + ay[2] * Ty[2];
             }
             if (store) {
+                // the planes are written once and read by a later kernel: streaming (non-temporal) stores
                 const size_t idx = (size_t)r * A.ld + c;
-                out[idx] = m0;
-                if (!SCALAR) { out[A.plane + idx] = rox; out[2 * A.plane + idx] = roy; out[3 * A.plane + idx] = rr; }
+                __builtin_nontemporal_store(m0, out + idx);
+                if (!SCALAR) {
+                    __builtin_nontemporal_store(rox, out + A.plane + idx);
+                    __builtin_nontemporal_store(roy, out + 2 * A.plane + idx);
+                    __builtin_nontemporal_store(rr, out + 3 * A.plane + idx);
+                }
                 const bool ok = isfinite(m0) && (SCALAR || (isfinite(rox) && isfinite(roy) && isfinite(rsx) &&
                                                             isfinite(rsy) && isfinite(rr)));
                 if (!ok) atomicOr(A.flag, 1u);
@@ -499,11 +504,11 @@ __global__ __launch_bounds__(512) void far32_tile_kernel(GeomDev g, FarArgs A, f
             const float m0 = stage[sidx];
             if (!isnan(m0)) {
                 const size_t idx = (size_t)(C0 + cl) * A.ld + R0 + half * HALF + col;
-                out[idx] = m0;
+                __builtin_nontemporal_store(m0, out + idx);
                 if (!SCALAR) {
-                    out[A.plane + idx] = stage[TILE * PITCH + sidx];
-                    out[2 * A.plane + idx] = stage[2 * TILE * PITCH + sidx];
-                    out[3 * A.plane + idx] = stage[3 * TILE * PITCH + sidx];
+                    __builtin_nontemporal_store(stage[TILE * PITCH + sidx], out + A.plane + idx);
+                    __builtin_nontemporal_store(stage[2 * TILE * PITCH + sidx], out + 2 * A.plane + idx);
+                    __builtin_nontemporal_store(stage[3 * TILE * PITCH + sidx], out + 3 * A.plane + idx);
                 }
             }
         }

@@ -130,6 +130,16 @@ int pq_combine_device(int precision, const void *planes_dev, int64_t nt, int64_t
 
 /* ---- loop-space port solve --------------------------------------------- */
 
+/* Potential coefficients of the charge nodes from the device M0 plane (FP32 or
+ * FP64, (nt, ld)): P[a][b] = factor / (A_a A_b) sum_{p in a, q in b} M0[p][q]
+ * into P_dev (n, ldp) FP32 or FP64, node a's triangles node_tris[node_ptr[a]
+ * .. node_ptr[a+1]) (device arrays), A = node_area (device). factor =
+ * 1 / (4 pi eps0 er). Replaces MQS._set_potential (src/pcbqs/mqs.py:532-556),
+ * P = W^T M0 W / (4 pi eps0 er), without the side-charge panels. */
+int pq_potential_device(int precision, const void *M0_dev, int64_t nt, int64_t ld, int64_t n,
+                        const int32_t *node_ptr, const int32_t *node_tris, const double *node_area, double factor,
+                        int out_precision, void *P_dev, int64_t ldp, void *stream);
+
 typedef struct pq_loops pq_loops;   /* a board's loop operator on one device      */
 
 /* Project the branch inductance onto the loop basis on the device and keep the

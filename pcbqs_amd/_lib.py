@@ -54,6 +54,7 @@ _SIGNATURES = {
     'pq_fill_device': ([P, DBL, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, I64, P, P], ctypes.c_int),
     'pq_fill_last_timing': ([P, P, P, P], ctypes.c_int),
     'pq_combine_device': ([ctypes.c_int, P, I64, I64, P, P, P, P, P, I64, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
+    'pq_potential_device': ([ctypes.c_int, P, I64, I64, I64, P, P, P, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
     'pq_loops_create': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, I64, P, I64, ctypes.c_int, P, P, P, P],
                         ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),

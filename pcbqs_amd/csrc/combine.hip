@@ -84,9 +84,58 @@ __global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx
     }
 }
 
+// P[a][b] = f / (A_a A_b) sum_{p in node a} sum_{q in node b} M0[p][q]: the potential coefficients
+// of the charge nodes (src/pcbqs/mqs.py:532-556, _set_potential: P = W^T M0 W / (4 pi eps0 er) with
+// W[t, node(t)] = 1 / area(node)). Node a's triangles are tris[ptr[a] .. ptr[a+1]). Every entry is
+// summed in its canonical order (min node outer), so P is bitwise symmetric. One wave per row,
+// lanes over 64 consecutive columns (coalesced P stores).
+template <class PT, class OT>
+__global__ __launch_bounds__(256) void potential_kernel(const PT *__restrict__ M0, long long ld,
+                                                       const int *__restrict__ ptr, const int *__restrict__ tris,
+                                                       const double *__restrict__ area, long long n, double f,
+                                                       OT *__restrict__ P, long long ldp) {
+    const int lane = threadIdx.x & 63;
+    const long long a = (long long)blockIdx.y * 4 + (threadIdx.x >> 6);
+    const long long b = (long long)blockIdx.x * 64 + lane;
+    if (a >= n || b >= n) return;
+    const long long lo = a < b ? a : b, hi = a < b ? b : a;
+    double s = 0;
+    for (int i = ptr[lo]; i < ptr[lo + 1]; ++i) {
+        const size_t p = (size_t)tris[i] * ld;
+        for (int j = ptr[hi]; j < ptr[hi + 1]; ++j) s += (double)M0[p + tris[j]];
+    }
+    P[a * ldp + b] = (OT)(f * s / (area[a] * area[b]));
+}
+
 }  // namespace pq
 
 using namespace pq;
+
+extern "C" int pq_potential_device(int precision, const void *M0, int64_t nt, int64_t ld, int64_t n,
+                                   const int32_t *node_ptr, const int32_t *node_tris, const double *node_area,
+                                   double factor, int out_precision, void *P, int64_t ldp, void *stream) {
+    return guard([&] {
+        PQ_REQUIRE(M0 && node_ptr && node_tris && node_area && P, "pq_potential_device: null argument");
+        PQ_REQUIRE(precision == 32 || precision == 64, "pq_potential_device: precision must be 32 or 64");
+        PQ_REQUIRE(out_precision == 32 || out_precision == 64, "pq_potential_device: out_precision 32 or 64");
+        PQ_REQUIRE(nt > 0 && n > 0 && n <= nt, "pq_potential_device: invalid sizes");
+        if (ld == 0) ld = nt;
+        if (ldp == 0) ldp = n;
+        PQ_REQUIRE(ld >= nt && ldp >= n, "pq_potential_device: leading dimension too small");
+        dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4)), block(256);
+        hipStream_t s = (hipStream_t)stream;
+#define PQ_POT(PT, OT)                                                                                     \
+    hipLaunchKernelGGL((potential_kernel<PT, OT>), grid, block, 0, s, (const PT *)M0, (long long)ld, node_ptr, \
+                       node_tris, node_area, (long long)n, factor, (OT *)P, (long long)ldp)
+        if (precision == 32) {
+            if (out_precision == 32) PQ_POT(float, float); else PQ_POT(float, double);
+        } else {
+            if (out_precision == 32) PQ_POT(double, float); else PQ_POT(double, double);
+        }
+#undef PQ_POT
+        PQ_HIP(hipGetLastError());
+    });
+}
 
 extern "C" int pq_combine_device(int precision, const void *planes, int64_t nt, int64_t ld, const double *cent,
                                  const int32_t *tri, const double *alpha, const double *vertex,

@@ -181,13 +181,16 @@ template <class T, bool SCALAR>
 __device__ __forceinline__ void store_entry(T *out, size_t plane, size_t i, int r, int c, const double *m,
                                             unsigned *flag) {
     bool ok = isfinite(m[0]);
-    out[i] = (T)m[0];
+    // planes are written once and read by a later kernel or copy: streaming (non-temporal) stores
+    __builtin_nontemporal_store((T)m[0], out + i);
     if (!SCALAR) {
         double mx, my;
         if (r == c) { mx = (m[1] + m[3]) / 2; my = (m[2] + m[4]) / 2; }
         else if (r < c) { mx = m[1]; my = m[2]; }
         else { mx = m[3]; my = m[4]; }
-        out[plane + i] = (T)mx; out[2 * plane + i] = (T)my; out[3 * plane + i] = (T)m[5];
+        __builtin_nontemporal_store((T)mx, out + plane + i);
+        __builtin_nontemporal_store((T)my, out + 2 * plane + i);
+        __builtin_nontemporal_store((T)m[5], out + 3 * plane + i);
         ok = ok && isfinite(mx) && isfinite(my) && isfinite(m[5]);
     }
     if (!ok) atomicOr(flag, 1u);

@@ -16,6 +16,8 @@ from dataclasses import dataclass
 import numpy as np
 
 MU0 = 4e-7 * np.pi
+C0 = 299792458.0
+EPS0 = 1.0 / (MU0 * C0 ** 2)
 
 _a1, _a2 = 0.445948490915965, 0.091576213509771
 BAR6 = np.array([[1 - 2 * _a1, _a1, _a1], [_a1, 1 - 2 * _a1, _a1], [_a1, _a1, 1 - 2 * _a1],

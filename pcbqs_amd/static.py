@@ -4,6 +4,8 @@ Mirrors, for users switching from the reference (same names, arguments, results,
   FillEngine, register_fill_backend, resolve_workers  src/pcbqs/static_backends.py:57-67,234-336
   static_moments                                      src/pcbqs/static.py:11-95
   edge_inductance                                     src/pcbqs/static.py:103-128
+  potential                                           src/pcbqs/mqs.py:532-570 (_set_potential
+                                                      on the scalar fill, _ensure_potential)
 The only built-in backend is 'mi355x' ('auto' selects it; PCBQS_FILL_BACKEND is honoured);
 other names come from register_fill_backend or the pcbqs.static_backends entry points, as in
 the reference. There is no CPU fallback: without a device, the backend raises RuntimeError.
@@ -14,8 +16,10 @@ import time
 
 import numpy as np
 
-from .backend import Mi355xBackend, combine_device
-from .geometry import FillGeometry
+from . import _lib
+from ._lib import DeviceArray, call, ptr
+from .backend import Mi355xBackend, combine_device, padded
+from .geometry import EPS0, FillGeometry
 
 _FACTORIES = {'mi355x': Mi355xBackend}
 
@@ -194,3 +198,35 @@ def edge_inductance(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None
     L.close()
     planes.close()
     return out, dict(backend='mi355x', workers=1, seconds=time.perf_counter() - started)
+
+
+def potential(nodes, tris, z, node_of, er=1.0, near=3.0, outer_levels=1, precision=64, device=0):
+    """Potential coefficients P (n_nodes, n_nodes) FP64 host of the charge nodes node_of (nt,):
+    P = W^T M0 W / (4 pi eps0 er), W[t, node(t)] = 1 / area(node) (src/pcbqs/mqs.py:532-556; the
+    reference's side-charge panels excluded). M0 is the device's scalar fill (precision 64: the
+    reference layout; 32: FP32 far pairs, FP64 near pairs), which stays in HBM."""
+    nodes, tris = np.asarray(nodes, float), np.asarray(tris)
+    node_of = np.asarray(node_of, np.int64)
+    if node_of.shape != (len(tris),) or node_of.min() < 0:
+        raise ValueError('node_of must give one nonnegative node per triangle')
+    g = FillGeometry.prepare(nodes, tris, z, outer_levels)
+    backend = Mi355xBackend(g, 1, 16384, device)
+    try:
+        planes, _ = backend.fill_device(near, outer_levels, True, precision)
+        n = int(node_of.max()) + 1
+        area = np.bincount(node_of, weights=g.area, minlength=n)
+        if (area <= 0).any():
+            raise ValueError('every charge node needs at least one triangle')
+        order = np.argsort(node_of, kind='stable').astype(np.int32)
+        ptr_ = np.r_[0, np.cumsum(np.bincount(node_of, minlength=n))].astype(np.int32)
+        d_ptr, d_tris, d_area = (DeviceArray.from_host(a, device) for a in (ptr_, order, area))
+        P = DeviceArray((n, n), np.float64, device)
+        call('pq_potential_device', precision, planes.ptr, len(tris), padded(len(tris)), n, d_ptr.ptr, d_tris.ptr,
+             d_area.ptr, 1.0 / (4 * np.pi * EPS0 * er), 64, P.ptr, n, None)
+        call('pq_synchronize', device)
+        out = P.host()
+        for a in (planes, d_ptr, d_tris, d_area, P):
+            a.close()
+        return out
+    finally:
+        backend.close()

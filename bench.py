@@ -101,10 +101,29 @@ def gmres_section(board, freq=1e6, grid=24, repeats=3, cpu=True):
         ratio=learned['seconds'] / plain['seconds'],
         port_z_rel_diff=float(np.abs(Zl - Zp).max() / np.abs(Zp).max()),
         board_setup_s=board_setup)
+    chunk, us = solver.tuning()
+    n = solver.model.n_free
+    section['operator_product'] = dict(
+        kernel='gemv_sym_kernel (omega Ll X, Ll FP32 symmetric, 8 real FP64 columns)', autotuned_chunk=chunk,
+        microseconds=us, algorithmic_bytes=4 * n * n, achieved_GBs=4 * n * n / (us * 1e-6) / 1e9,
+        peak_GBs=HBM_PEAK / 1e9, frac=4 * n * n / (us * 1e-6) / HBM_PEAK if us > 0 else None)
     if cpu:
         section['cpu_operator'] = cpu_operator_sample(solver, plain['iterations'])
     solver.close()
     return section
+
+
+def sweep_section(n_boards, n_freqs=20):
+    """The batched sweep: n_boards seeded uneven boards (1k-8k triangles, 2-4 layers, 4-16 ports) x
+    n_freqs log-spaced frequencies 1 kHz - 100 MHz, two-level GMRES to 1e-8."""
+    from pcbqs_amd import board as boards
+    from pcbqs_amd.sweep import sweep
+    bs = [boards.board_batch(s) for s in range(n_boards)]
+    res, total = sweep(bs, np.logspace(3, 8, n_freqs), precond='two-level', grid=12)
+    total.update(setup_s=sum(r['setup_s'] for r in res), solve_s=sum(r['solve_s'] for r in res),
+                 triangles=sum(r['triangles'] for r in res), mean_iterations=float(np.mean(
+                     [i for r in res for i in r['iterations']])))
+    return total
 
 
 def cpu_operator_sample(solver, iterations, products=5):
@@ -147,6 +166,8 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-gmres', action='store_true')
+    ap.add_argument('--sweep', type=int, default=0, metavar='BOARDS',
+                    help='also run the batched sweep over BOARDS uneven boards x 20 frequencies')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
                     help='first copy BYTES device-to-device (the scale of a counter run)')
     args = ap.parse_args()
@@ -206,6 +227,8 @@ def main():
     )
     if not args.no_gmres:
         line['gmres'] = gmres_section(board, cpu=not args.no_cpu)
+    if args.sweep:
+        line['sweep'] = sweep_section(args.sweep)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(board)
     print(json.dumps(line))

@@ -65,6 +65,11 @@ int pq_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 int pq_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int pq_memset(void *dst_dev, int value, size_t bytes);
 int pq_synchronize(int device);
+/* A non-blocking stream of its own for a caller that runs several pipelines
+ * at once (one per board), its synchronisation and release. */
+int pq_stream_create(int device, void **stream);
+int pq_stream_destroy(void *stream);
+int pq_stream_synchronize(void *stream);
 
 /* ---- fill --------------------------------------------------------------- */
 
@@ -158,7 +163,8 @@ int pq_loops_destroy(pq_loops *loops);
 /* The device loop inductance matrix (FP32, n_loops rows, leading dimension ldl). */
 int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
 
-/* Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
+/* P in 1..8 or 16 (pad other counts by repeating a port column).
+ * Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
  * frequency omega: the free loops eliminated, X = -Z_ff^-1 Z_fs, by restarted
  * lockstep GMRES(restart) with all P columns sharing every operator product
  * (precond 0: none, 1: block-Jacobi on 64-loop blocks of the free-loop order,
@@ -200,6 +206,13 @@ int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi, int diagonal)
  * of the current coarse space, for a host-side change of basis. No counterpart
  * in the reference. */
 int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
+
+/* The operator product's autotuned column-chunk width for NR real columns
+ * (tuned on first use: candidates timed with HIP events on this handle's
+ * matrix) and its measured time per product in microseconds (0 when NR is not
+ * tuned: NR > 16 or fewer than 1024 free loops). No counterpart in the
+ * reference. */
+int pq_loops_tuning(pq_loops *loops, int NR, int *chunk, double *microseconds);
 
 /* Statistics of the last solve: iterations, operator products, relative
  * residual, wall seconds. Not in the reference (solve_info's role,

@@ -47,6 +47,9 @@ _SIGNATURES = {
     'pq_memcpy_d2d': ([P, P, SIZE], ctypes.c_int),
     'pq_memset': ([P, ctypes.c_int, SIZE], ctypes.c_int),
     'pq_synchronize': ([ctypes.c_int], ctypes.c_int),
+    'pq_stream_create': ([ctypes.c_int, P], ctypes.c_int),
+    'pq_stream_destroy': ([P], ctypes.c_int),
+    'pq_stream_synchronize': ([P], ctypes.c_int),
     'pq_fill_create': ([P, ctypes.c_int, P], ctypes.c_int),
     'pq_fill_destroy': ([P], ctypes.c_int),
     'pq_fill_symmetric': ([P, DBL, ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),
@@ -72,6 +75,7 @@ _SIGNATURES = {
     'pq_loops_rhs': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
+    'pq_loops_tuning': ([P, ctypes.c_int, P, P], ctypes.c_int),
 }
 
 

@@ -61,6 +61,26 @@ int pq_memset(void *dst, int value, size_t bytes) {
     });
 }
 
+int pq_stream_create(int device, void **stream) {
+    return guard([&] {
+        PQ_REQUIRE(stream, "pq_stream_create: null output");
+        DeviceScope scope(device);
+        hipStream_t s = nullptr;
+        PQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+
+int pq_stream_destroy(void *stream) {
+    return guard([&] {
+        if (stream) PQ_HIP(hipStreamDestroy((hipStream_t)stream));
+    });
+}
+
+int pq_stream_synchronize(void *stream) {
+    return guard([&] { PQ_HIP(hipStreamSynchronize((hipStream_t)stream)); });
+}
+
 int pq_synchronize(int device) {
     return guard([&] {
         DeviceScope scope(device);

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <map>
 #include <vector>
 
 namespace pq {
@@ -90,60 +91,6 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int *__restrict__ ptr, 
     }
 }
 
-template <int NR>
-__global__ __launch_bounds__(256) void gemv_kernel(const float *Ll, long long ldl, int n, const double *X,
-                                                  double omega, double *W) {
-    constexpr int P = NR / 2;
-    constexpr int RPW = NR >= 16 ? 1 : 16 / NR;   // rows per wave
-    constexpr int RPB = 4 * RPW;
-    __shared__ double xs[256 * NR];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.x * RPB + wave * RPW;
-    double acc[RPW][NR];
-#pragma unroll
-    for (int a = 0; a < RPW; ++a)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[a][r] = 0;
-    for (int c0 = 0; c0 < n; c0 += 256) {
-        const int cn = min(256, n - c0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < cn * NR; e += 256) xs[e] = X[(size_t)c0 * NR + e];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int cl = q * 64 + lane;
-            if (cl >= cn) continue;
-            double xv[NR];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) xv[r] = xs[cl * NR + r];
-#pragma unroll
-            for (int a = 0; a < RPW; ++a) {
-                const int row = row0 + a;
-                const double m = row < n ? (double)Ll[(size_t)row * ldl + c0 + cl] : 0.0;
-#pragma unroll
-                for (int r = 0; r < NR; ++r) acc[a][r] += m * xv[r];
-            }
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < RPW; ++a) {
-        const int row = row0 + a;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            double s = acc[a][r];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-            acc[a][r] = s;
-        }
-        if (lane == 0 && row < n) {
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                W[(size_t)row * NR + p] -= omega * acc[a][P + p];
-                W[(size_t)row * NR + P + p] += omega * acc[a][p];
-            }
-        }
-    }
-}
-
 // Ll := (Ll + Ll^T) / 2 on the upper triangle and its mirror (the loop matrices' symmetric part,
 // as the reference takes it: src/pcbqs/mqs_iterative.py:244-257). Entry (i, j) and (j, i) then
 // agree bitwise, which lets the product read columns of Ll as rows.
@@ -156,26 +103,27 @@ __global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
     Ll[j * ldl + i] = v;
 }
 
-// Partial products of the symmetric Ll: part[s][row][r] = sum over columns c of chunk s of
-// Ll[c][row] X[c][r]. One thread per row, so each column read is 256 consecutive floats of row c
-// (coalesced), 16 of them in flight per lane; X[c] is the same for the whole wave and comes
-// through scalar loads, which keeps the LDS return path out of the inner loop.
+// Panel products: part[s][i][r] = sum over the rows c of chunk s of M[c][i] X[c][r], M (ncols, ld)
+// FP32 row-major, X (ncols, NR) FP64, i < nrows. One thread per i, so each row segment of M is 256
+// consecutive floats (coalesced), 16 of them in flight per lane; X[c] is the same for the whole wave
+// and comes through scalar loads, which keeps the LDS return path out of the inner loop. With the
+// symmetric Ll (rows read as columns) this is Ll X; with a transposed basis (k, n) it is Phi c.
 constexpr int GEMV_SPLIT = 32, GEMV_ROWS = 256, GEMV_CHUNK = 640;
 template <int NR>
-__global__ __launch_bounds__(256) void gemv_sym_kernel(const float *__restrict__ Ll, long long ldl, int n,
-                                                      const double *__restrict__ X, int chunk,
-                                                      double *__restrict__ part) {
+__global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M, long long ld, int nrows, int ncols,
+                                                   const double *__restrict__ X, int chunk,
+                                                   double *__restrict__ part) {
     const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
-    const int c0 = blockIdx.y * chunk, c1 = min(n, c0 + chunk);
+    const int c0 = blockIdx.y * chunk, c1 = min(ncols, c0 + chunk);
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
-    const float *col = Ll + min(row, n - 1);
+    const float *col = M + min(row, nrows - 1);
     int c = c0;
     for (; c + 16 <= c1; c += 16) {
         float m[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = col[(size_t)(c + u) * ldl];
+        for (int u = 0; u < 16; ++u) m[u] = col[(size_t)(c + u) * ld];
         const double *x = X + (size_t)c * NR;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
@@ -185,13 +133,13 @@ __global__ __launch_bounds__(256) void gemv_sym_kernel(const float *__restrict__
         }
     }
     for (; c < c1; ++c) {
-        const double mv = col[(size_t)c * ldl];
+        const double mv = col[(size_t)c * ld];
         const double *x = X + (size_t)c * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
     }
-    if (row < n) {
-        double *o = part + ((size_t)blockIdx.y * n + row) * NR;
+    if (row < nrows) {
+        double *o = part + ((size_t)blockIdx.y * nrows + row) * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) o[r] = acc[r];
     }
@@ -548,7 +496,7 @@ __global__ void coarse_sum_kernel(const double *partial, int nblk, int total, do
 }
 
 // Diagonal coarse matrix: c[a][p] = D[a] t[a][p] with t summed from the chunk partials in chunk
-// order (t (k, NR), real parts in columns 0..P-1; D and c complex interleaved).
+// order (t and c (k, NR), real parts in columns 0..P-1; D complex interleaved).
 __global__ void coarse_sum_diag_kernel(const double *partial, int nblk, int k, int P, const double *D,
                                        double *cvec) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -560,12 +508,12 @@ __global__ void coarse_sum_diag_kernel(const double *partial, int nblk, int k, i
         ti += partial[((size_t)q * k + a) * NR + P + p];
     }
     const double dr = D[2 * a], di = D[2 * a + 1];
-    cvec[2 * e] = dr * tr - di * ti;
-    cvec[2 * e + 1] = dr * ti + di * tr;
+    cvec[(size_t)a * NR + p] = dr * tr - di * ti;
+    cvec[(size_t)a * NR + P + p] = dr * ti + di * tr;
 }
 
-// c = Ainv t (complex; t (k, NR) with real parts in columns 0..P-1), cvec (k, P) complex
-// interleaved. One wave per output (a, p), lanes over b.
+// c = Ainv t (complex; t and c (k, NR) with real parts in columns 0..P-1). One wave per output
+// (a, p), lanes over b.
 __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, const double *Ainv, int k, int P,
                                                           double *cvec) {
     const int NR = 2 * P;
@@ -581,74 +529,33 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
         yi += ar * ti + ai * tr;
     }
     for (int off = 32; off > 0; off >>= 1) { yr += __shfl_down(yr, off); yi += __shfl_down(yi, off); }
-    if (lane == 0) { cvec[2 * e] = yr; cvec[2 * e + 1] = yi; }
+    if (lane == 0) { cvec[(size_t)a * NR + p] = yr; cvec[(size_t)a * NR + P + p] = yi; }
 }
 
-// R = V - (RPhi + j w LPhi) c and Y = Phi c from the transposed FP32 bases (k, n): a block covers 64
-// rows (the lanes: coalesced reads of each basis column) and splits the k coarse columns over its
-// cp_slices(P) waves; the coefficients c are wave-uniform loads. The slices' sums meet in LDS and
-// are added in slice order.
-constexpr int cp_slices(int P) { return P >= 16 ? 2 : P >= 8 ? 4 : 8; }
-template <int P>
-__global__ __launch_bounds__(512) void coarse_prolong_kernel(const float *__restrict__ Phit,
-                                                            const float *__restrict__ RPhit,
-                                                            const float *__restrict__ LPhit, int k,
-                                                            const double *__restrict__ cvec, double omega,
-                                                            const double *__restrict__ V, int n,
-                                                            double *__restrict__ R, double *__restrict__ Y) {
-    constexpr int NR = 2 * P, S = cp_slices(P), Q = 4 * P;
-    __shared__ double red[S * Q * 64];
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = blockIdx.x * 64 + lane;
-    double s[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) s[q] = 0;
-    auto term = [&](int c, double f, double r, double l) {
-        const double *cv = cvec + (size_t)c * 2 * P;
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const double cr = cv[2 * p], ci = cv[2 * p + 1];
-            s[4 * p] += r * cr - l * ci;
-            s[4 * p + 1] += r * ci + l * cr;
-            s[4 * p + 2] += f * cr;
-            s[4 * p + 3] += f * ci;
-        }
-    };
-    const int ii = min(i, n - 1);
-    constexpr int U = 4;
-    int c = w;
-    for (; c + S * (U - 1) < k; c += S * U) {
-        float f[U], r[U], l[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t at = (size_t)(c + S * u) * n + ii;
-            f[u] = Phit[at]; r[u] = RPhit[at]; l[u] = LPhit[at];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) term(c + S * u, f[u], r[u], omega * (double)l[u]);
+// The prolongation's epilogue: with the panel products PY = Phi c, PR = Rl Phi c, PL = Ll Phi c
+// (each nsplit chunk partials of (n, NR), real parts in columns 0..P-1), R = V - (PR + j w PL) and
+// Y = PY, the chunks summed in order.
+template <int NR>
+__global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const double *__restrict__ PR,
+                                        const double *__restrict__ PL, int nsplit, int n, double omega,
+                                        const double *__restrict__ V, double *__restrict__ R,
+                                        double *__restrict__ Y) {
+    constexpr int P = NR / 2;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * P) return;
+    const int row = e / P, p = e % P;
+    double yr = 0, yi = 0, rr = 0, ri = 0, lr = 0, li = 0;
+    for (int q = 0; q < nsplit; ++q) {
+        const size_t o = ((size_t)q * n + row) * NR;
+        yr += PY[o + p]; yi += PY[o + P + p];
+        rr += PR[o + p]; ri += PR[o + P + p];
+        lr += PL[o + p]; li += PL[o + P + p];
     }
-    for (; c < k; c += S) {
-        const size_t at = (size_t)c * n + ii;
-        term(c, Phit[at], RPhit[at], omega * (double)LPhit[at]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) red[(w * Q + q) * 64 + lane] = s[q];
-    __syncthreads();
-    for (int e = threadIdx.x; e < Q * 64; e += S * 64) {
-        const int q = e / 64, row = e % 64;
-        const int gi = blockIdx.x * 64 + row;
-        if (gi >= n) continue;
-        double acc = 0;
-        for (int ws = 0; ws < S; ++ws) acc += red[(ws * Q + q) * 64 + row];
-        const int p = q >> 2;
-        switch (q & 3) {
-        case 0: R[(size_t)gi * NR + p] = V[(size_t)gi * NR + p] - acc; break;
-        case 1: R[(size_t)gi * NR + P + p] = V[(size_t)gi * NR + P + p] - acc; break;
-        case 2: Y[(size_t)gi * NR + p] = acc; break;
-        default: Y[(size_t)gi * NR + P + p] = acc; break;
-        }
-    }
+    const size_t o = (size_t)row * NR;
+    R[o + p] = V[o + p] - rr + omega * li;
+    R[o + P + p] = V[o + P + p] - ri - omega * lr;
+    Y[o + p] = yr;
+    Y[o + P + p] = yi;
 }
 
 // Out (k, n) = In (n, k)^T, FP32.
@@ -659,7 +566,7 @@ __global__ void transpose_kernel(const float *In, int n, int k, float *Out) {
     Out[c * n + i] = In[e];
 }
 
-// W = sum of the nsplit column-chunk partials of gemv_sym_kernel (a real product, no frequency).
+// W = sum of the nsplit column-chunk partials of panel_kernel (a real product, no frequency).
 template <int NR>
 __global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -711,7 +618,7 @@ struct pq_loops {
     // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
     int kc = 0;
     bool cdiag = false;           // basis diagonalises the Galerkin matrices
-    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart;   // RPhi, LPhi transposed (k, n)
+    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart, ppart;   // RPhi, LPhi transposed (k, n)
     std::vector<double> Rc, Lc;
     double omega_c = -1;
     DevBuf Hd;                    // Arnoldi coefficients per step (device)
@@ -721,6 +628,9 @@ struct pq_loops {
     int iterations = 0, restarts = 0;
     double residual = 0, seconds = 0, matvec_ms = 0;
     long long matvecs = 0;
+    // autotuned column-chunk width of the operator product per column count (first use per NR)
+    std::map<int, int> gemv_chunk;
+    std::map<int, float> gemv_us;
 };
 
 namespace {
@@ -731,32 +641,84 @@ template <class F> void dispatch_nr(int NR, F &&f) {
     case 4: f(std::integral_constant<int, 4>()); break;
     case 6: f(std::integral_constant<int, 6>()); break;
     case 8: f(std::integral_constant<int, 8>()); break;
+    case 10: f(std::integral_constant<int, 10>()); break;
+    case 12: f(std::integral_constant<int, 12>()); break;
+    case 14: f(std::integral_constant<int, 14>()); break;
     case 16: f(std::integral_constant<int, 16>()); break;
     case 32: f(std::integral_constant<int, 32>()); break;
-    default: throw Error(PQ_EINVAL, "port count must be 1, 2, 3, 4, 8 or 16");
+    default: throw Error(PQ_EINVAL, "port count must be 1 to 8 or 16");
     }
 }
 
 // W = (Rl + j w Ll) X on the free block.
+int default_chunk(int n) { return std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT)); }
+
+int gemv_chunk(const pq_loops *L, int NR) {
+    auto it = L->gemv_chunk.find(NR);
+    return it != L->gemv_chunk.end() ? it->second : default_chunk(L->nf);
+}
+
+// The autotuner of the operator product: the column-chunk width sets the grid (rows / 256 x
+// n / chunk blocks), the loads in flight and the partial sums the epilogue reads. Each candidate
+// runs once to warm up and three times under HIP events on an input of zeros; the fastest is kept
+// for this handle and column count.
+void tune_gemv(pq_loops *L, int NR) {
+    const int n = L->nf;
+    if (n < 1024 || L->gemv_chunk.count(NR)) return;
+    std::vector<int> cand;
+    for (int c : {128, 192, 256, 320, 448, 640})
+        if ((n + c - 1) / c <= 128 && c < n) cand.push_back(c);
+    if (cand.empty()) return;
+    int maxsplit = 0;
+    for (int c : cand) maxsplit = std::max(maxsplit, (n + c - 1) / c);
+    DevBuf X, part;
+    X.reserve((size_t)n * NR * 8);
+    part.reserve((size_t)maxsplit * n * NR * 8);
+    PQ_HIP(hipMemsetAsync(X.p, 0, (size_t)n * NR * 8, L->stream));
+    hipEvent_t e0, e1;
+    PQ_HIP(hipEventCreate(&e0));
+    PQ_HIP(hipEventCreate(&e1));
+    int best = cand[0];
+    float best_us = 1e30f;
+    for (int c : cand) {
+        const int nsplit = (n + c - 1) / c;
+        auto launch = [&]() {
+            dispatch_nr(NR, [&](auto nr) {
+                constexpr int N = decltype(nr)::value;
+                hipLaunchKernelGGL(panel_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
+                                   L->stream, L->Ll.as<float>(), L->ldl, n, n, X.as<double>(), c, part.as<double>());
+            });
+        };
+        launch();
+        PQ_HIP(hipEventRecord(e0, L->stream));
+        for (int r = 0; r < 3; ++r) launch();
+        PQ_HIP(hipEventRecord(e1, L->stream));
+        PQ_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        PQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1000.f / 3.f;
+        if (us < best_us) { best_us = us; best = c; }
+    }
+    PQ_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    L->gemv_chunk[NR] = best;
+    L->gemv_us[NR] = best_us;
+}
+
 void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P) {
     const int NR = 2 * P, n = L->nf;
-    const int chunk = std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT));
+    const int chunk = gemv_chunk(L, NR);
     const int nsplit = (n + chunk - 1) / chunk;
     L->gpart.reserve((size_t)nsplit * n * NR * 8);
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
         hipLaunchKernelGGL(spmv_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
                            L->rcol.as<int>(), L->rval.as<double>(), n, X, W);
-        if (N <= 16) {
-            hipLaunchKernelGGL(gemv_sym_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
-                               L->Ll.as<float>(), L->ldl, n, X, chunk, L->gpart.as<double>());
-            hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
-                               L->gpart.as<double>(), nsplit, n, omega, W);
-        } else {
-            constexpr int RPB = 4 * (N >= 16 ? 1 : 16 / N);
-            hipLaunchKernelGGL(gemv_kernel<N>, dim3((n + RPB - 1) / RPB), dim3(256), 0, L->stream, L->Ll.as<float>(),
-                               L->ldl, n, X, omega, W);
-        }
+        hipLaunchKernelGGL(panel_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
+                           L->Ll.as<float>(), L->ldl, n, n, X, chunk, L->gpart.as<double>());
+        hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
+                           L->gpart.as<double>(), nsplit, n, omega, W);
     });
     PQ_HIP(hipGetLastError());
     ++L->matvecs;
@@ -765,6 +727,16 @@ void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P
 void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P);
 
 void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add = nullptr);
+
+// Column chunks of the prolongation's panel products: enough blocks to fill the chip (about
+// 2048 with 256-row blocks) and at most 16; on large bases also few enough that the chunk
+// partials (written and read once) stay within a quarter of the bases' own bytes.
+int prolong_splits(int n, int k, int NR) {
+    const int rb = (n + GEMV_ROWS - 1) / GEMV_ROWS;
+    int s = std::min({16, (2048 + rb - 1) / rb, (k + 15) / 16});
+    if ((size_t)12 * k * n >= ((size_t)64 << 20)) s = std::min(s, k / (16 * NR));
+    return std::max(1, s);
+}
 
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
 void coarse_correction(pq_loops *L, const double *X, int P) {
@@ -783,9 +755,18 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
             hipLaunchKernelGGL(coarse_solve_kernel, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
                                L->ctot.as<double>(), L->Ainv.as<double>(), k, P, L->cvec.as<double>());
         }
-        hipLaunchKernelGGL(coarse_prolong_kernel<N / 2>, dim3((n + 63) / 64), dim3(64 * cp_slices(N / 2)), 0,
-                           L->stream, L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>(), k,
-                           L->cvec.as<double>(), L->omega_c, X, n, L->T1.as<double>(), L->T2.as<double>());
+        // prolongation: three panel products over the transposed bases, then the epilogue
+        const int ps = prolong_splits(n, k, N), pc = (k + ps - 1) / ps;
+        const size_t stride = (size_t)ps * n * N;
+        double *part = L->ppart.as<double>();
+        const dim3 grid((n + GEMV_ROWS - 1) / GEMV_ROWS, (k + pc - 1) / pc);
+        const float *bases[3] = {L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
+        for (int b = 0; b < 3; ++b)
+            hipLaunchKernelGGL(panel_kernel<N>, grid, dim3(256), 0, L->stream, bases[b], (long long)n, n, k,
+                               L->cvec.as<double>(), pc, part + b * stride);
+        hipLaunchKernelGGL(prolong_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream, part,
+                           part + stride, part + 2 * stride, (int)grid.y, n, L->omega_c, X, L->T1.as<double>(),
+                           L->T2.as<double>());
     });
     PQ_HIP(hipGetLastError());
 }
@@ -893,6 +874,7 @@ void prepare_coarse(pq_loops *L, double omega, int P) {
     const int nchunk = (n + cr_rows(NR) - 1) / cr_rows(NR);
     L->cpart.reserve((size_t)nchunk * L->kc * NR * 8);
     L->cvec.reserve((size_t)L->kc * P * 2 * 8);
+    L->ppart.reserve((size_t)3 * prolong_splits(n, L->kc, NR) * n * NR * 8);
     L->ctot.reserve((size_t)L->kc * NR * 8);
     L->T1.reserve((size_t)n * NR * 8); L->T2.reserve((size_t)n * NR * 8);
 }
@@ -1023,7 +1005,7 @@ int pq_loops_destroy(pq_loops *L) {
         if (L->stream) (void)hipStreamSynchronize(L->stream);
         for (DevBuf *b : {&L->Ll, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->singular, &L->V, &L->W, &L->Z,
                           &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
-                          &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit})
+                          &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit, &L->ppart})
             b->release();
         L->Hd.release();
         if (L->Hh) (void)hipHostFree(L->Hh);
@@ -1052,6 +1034,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
         for (int p = 0; p < P; ++p)
             PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl, "pq_loops_solve_ports: port loop out of range");
         DeviceScope scope(L->device);
+        tune_gemv(L, 2 * P);
         const auto started = std::chrono::steady_clock::now();
         const int n = L->nf, NR = 2 * P, m = std::min(restart, std::max(n, 1));
         const size_t vsz = (size_t)n * NR;
@@ -1294,7 +1277,8 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)(((size_t)n * k + 255) / 256)), dim3(256), 0, L->stream,
                            L->Phi.as<float>(), n, k, L->Phit.as<float>());
         constexpr int CH = 16;
-        const int chunk = std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT));
+        tune_gemv(L, CH);
+        const int chunk = gemv_chunk(L, CH);
         const int nsplit = (n + chunk - 1) / chunk;
         const int nchunk = (n + cr_rows(CH) - 1) / cr_rows(CH);
         DevBuf X, WR, WL, part, red, t;
@@ -1312,8 +1296,8 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
                                CH, X.as<double>());
             hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
                                L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
-            hipLaunchKernelGGL(gemv_sym_kernel<CH>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
-                               L->stream, L->Ll.as<float>(), L->ldl, n, X.as<double>(), chunk, part.as<double>());
+            hipLaunchKernelGGL(panel_kernel<CH>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
+                               L->stream, L->Ll.as<float>(), L->ldl, n, n, X.as<double>(), chunk, part.as<double>());
             hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
                                WL.as<double>());
             for (int which = 0; which < 2; ++which) {
@@ -1382,6 +1366,17 @@ int pq_loops_coarse_matrices(pq_loops *L, double *Rc, double *Lc) {
         PQ_REQUIRE(L->kc > 0, "pq_loops_coarse_matrices: no coarse space set");
         std::copy(L->Rc.begin(), L->Rc.end(), Rc);
         std::copy(L->Lc.begin(), L->Lc.end(), Lc);
+    });
+}
+
+int pq_loops_tuning(pq_loops *L, int NR, int *chunk, double *microseconds) {
+    return guard([&] {
+        PQ_REQUIRE(L && chunk, "pq_loops_tuning: null argument");
+        DeviceScope scope(L->device);
+        tune_gemv(L, NR);
+        *chunk = gemv_chunk(L, NR);
+        auto it = L->gemv_us.find(NR);
+        if (microseconds) *microseconds = it != L->gemv_us.end() ? it->second : 0.0;
     });
 }
 

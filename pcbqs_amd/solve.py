@@ -24,27 +24,32 @@ PRECONDITIONERS = {'none': 0, 'block-jacobi': 1, 'two-level': 2}
 class PortSolver:
     """One board on one device. `ports(freqs)` returns Zp (F, P, P) complex and per-frequency info."""
 
-    def __init__(self, board, device=0, near=3.0, outer_levels=1, precision=32, ideal_terminals=True):
+    def __init__(self, board, device=0, near=3.0, outer_levels=1, precision=32, ideal_terminals=True, model=None):
         t0 = time.perf_counter()
         self.device = device
-        self.model = LoopModel(board, ideal_terminals=ideal_terminals)
+        self.model = model if model is not None else LoopModel(board, ideal_terminals=ideal_terminals)
         m = self.model
         t1 = time.perf_counter()
         g = FillGeometry.prepare(board.nodes, board.tris, board.z_of, outer_levels)
+        # one non-blocking stream per solver: several boards can run at once from several threads
+        stream = ctypes.c_void_p()
+        call('pq_stream_create', device, ctypes.byref(stream))
+        self._stream = stream
         self.backend = Mi355xBackend(g, 1, 16384, device)
-        planes, self.analytic = self.backend.fill_device(near, outer_levels, False, precision)
+        planes, self.analytic = self.backend.fill_device(near, outer_levels, False, precision, stream=stream)
         rwg = RWG(g.nodes, board.tris)                        # the fill's centred frame, same edges
         ne, nv = m.ne, m.nv
         nbr = ne + nv
         ldb = padded(nbr)
         Lb = DeviceArray((nbr, ldb), np.float32, device).zero()
+        call('pq_stream_synchronize', None)                   # the zero fill ran on the null stream
         tri, alpha, vertex = rwg.combine_arrays()
         scale = 1.0 / rwg.l
         dev = [DeviceArray.from_host(a, device) for a in (tri, alpha, vertex, scale,
                                                           np.ascontiguousarray(g.cent, np.float64))]
         call('pq_combine_device', int(precision), planes.ptr, len(g.tris), planes.shape[-1], dev[4].ptr, dev[0].ptr,
-             dev[1].ptr, dev[2].ptr, dev[3].ptr, ne, MU0 / (4 * np.pi), 32, Lb.ptr, ldb, None)
-        call('pq_synchronize', device)
+             dev[1].ptr, dev[2].ptr, dev[3].ptr, ne, MU0 / (4 * np.pi), 32, Lb.ptr, ldb, stream)
+        call('pq_stream_synchronize', stream)
         for a in dev:
             a.close()
         planes.close()
@@ -62,7 +67,6 @@ class PortSolver:
         handle = ctypes.c_void_p()
         call('pq_loops_create', device, m.n_loops, m.n_free, ptr(colptr), ptr(rows), ptr(sign), len(rows), Lb.ptr,
              ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
-        call('pq_synchronize', device)
         Lb.close()
         self._handle = handle
         self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
@@ -112,6 +116,13 @@ class PortSolver:
     def n_ports(self):
         return len(self.port_cols)
 
+    @property
+    def device_ports(self):
+        """Columns the device solves in lockstep: the port count, or 16 for 9 to 15 ports (the
+        extra columns repeat the last port and are dropped)."""
+        P = self.n_ports
+        return P if P <= 8 or P == 16 else 16
+
     def _split(self, x):
         """(n_free, P) complex -> (n_free, 2P) FP64 (real parts first), the library's layout."""
         return np.ascontiguousarray(np.hstack([x.real, x.imag]), np.float64)
@@ -141,6 +152,9 @@ class PortSolver:
         x0: optional initial iterates, (F, n_free, P) complex or a callable frequency -> (n_free, P)
         (the learned correction's prediction); GMRES starts from zero otherwise."""
         P = self.n_ports
+        Pd = self.device_ports
+        cols = self.port_cols if Pd == P else np.ascontiguousarray(
+            np.r_[self.port_cols, np.full(Pd - P, self.port_cols[-1])], np.int32)
         f = np.atleast_1d(np.asarray(freqs, float))
         if f.ndim != 1 or not np.isfinite(f).all() or (f < 0).any():
             raise ValueError('port frequencies must be finite and nonnegative')
@@ -149,29 +163,44 @@ class PortSolver:
         info = []
         kind = PRECONDITIONERS[precond]
         for i, fi in enumerate(f):
-            zp = np.zeros((P, P, 2))
+            zp = np.zeros((Pd, Pd, 2))
             its, res = ctypes.c_int64(), ctypes.c_double()
-            xh = X[i] if currents else None
+            xh = np.zeros((self.model.n_free, 2 * Pd)) if currents else None
             start = None
             if x0 is not None:
                 guess = x0(fi) if callable(x0) else x0[i]
                 if guess.shape != (self.model.n_free, P):
                     raise ValueError('initial iterate must be (n_free, P)')
+                if Pd != P:
+                    guess = np.hstack([guess, np.repeat(guess[:, -1:], Pd - P, axis=1)])
                 start = self._split(guess)
-            call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, P, ptr(self.port_cols), kind, float(tol),
+            call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, Pd, ptr(cols), kind, float(tol),
                  int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None,
                  ptr(start) if start is not None else None, ctypes.byref(its), ctypes.byref(res))
-            Z[i] = zp[..., 0] + 1j * zp[..., 1]
+            Z[i] = zp[:P, :P, 0] + 1j * zp[:P, :P, 1]
+            if currents:
+                X[i] = np.hstack([xh[:, :P], xh[:, Pd:Pd + P]])
             seconds, mv = ctypes.c_double(), ctypes.c_int64()
             call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))
             info.append(dict(frequency_Hz=float(fi), iterations=int(its.value), matvecs=int(mv.value),
                              relative_residual=float(res.value), seconds=float(seconds.value), preconditioner=precond))
         return (Z, info, X) if currents else (Z, info)
 
+    def tuning(self, columns=None):
+        """(chunk, microseconds) of the autotuned operator product for 2P real columns (P ports by
+        default): the chosen column-chunk width and its time per product."""
+        nr = 2 * self.device_ports if columns is None else int(columns)
+        chunk, us = ctypes.c_int(), ctypes.c_double()
+        call('pq_loops_tuning', self._handle, nr, ctypes.byref(chunk), ctypes.byref(us))
+        return chunk.value, us.value
+
     def close(self):
         if getattr(self, '_handle', None) is not None and _lib._lib is not None:
             _lib._lib.pq_loops_destroy(self._handle)
         self._handle = None
+        if getattr(self, '_stream', None) is not None and _lib._lib is not None:
+            _lib._lib.pq_stream_destroy(self._stream)
+        self._stream = None
         if getattr(self, 'backend', None) is not None:
             self.backend.close()
 

@@ -113,13 +113,13 @@ def gmres_section(board, freq=1e6, grid=24, repeats=3, cpu=True):
     return section
 
 
-def sweep_section(n_boards, n_freqs=20):
+def sweep_section(n_boards, n_freqs=20, concurrency=1):
     """The batched sweep: n_boards seeded uneven boards (1k-8k triangles, 2-4 layers, 4-16 ports) x
     n_freqs log-spaced frequencies 1 kHz - 100 MHz, two-level GMRES to 1e-8."""
     from pcbqs_amd import board as boards
     from pcbqs_amd.sweep import sweep
     bs = [boards.board_batch(s) for s in range(n_boards)]
-    res, total = sweep(bs, np.logspace(3, 8, n_freqs), precond='two-level', grid=12)
+    res, total = sweep(bs, np.logspace(3, 8, n_freqs), precond='two-level', grid=12, concurrency=concurrency)
     total.update(setup_s=sum(r['setup_s'] for r in res), solve_s=sum(r['solve_s'] for r in res),
                  triangles=sum(r['triangles'] for r in res), mean_iterations=float(np.mean(
                      [i for r in res for i in r['iterations']])))
@@ -168,6 +168,7 @@ def main():
     ap.add_argument('--no-gmres', action='store_true')
     ap.add_argument('--sweep', type=int, default=0, metavar='BOARDS',
                     help='also run the batched sweep over BOARDS uneven boards x 20 frequencies')
+    ap.add_argument('--sweep-concurrency', type=int, default=4, help='boards at once in the sweep')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
                     help='first copy BYTES device-to-device (the scale of a counter run)')
     args = ap.parse_args()
@@ -228,7 +229,7 @@ def main():
     if not args.no_gmres:
         line['gmres'] = gmres_section(board, cpu=not args.no_cpu)
     if args.sweep:
-        line['sweep'] = sweep_section(args.sweep)
+        line['sweep'] = sweep_section(args.sweep, concurrency=args.sweep_concurrency)
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(board)
     print(json.dumps(line))

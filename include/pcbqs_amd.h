@@ -207,6 +207,13 @@ int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi, int diagonal)
  * in the reference. */
 int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
 
+/* Capture each Arnoldi step's kernels once as a graph and replay it (enable 1),
+ * launch them one by one (0), or follow PCBQS_AMD_GRAPHS (-1, the default:
+ * on when it is "1"). A capture is invalidated by another thread's
+ * synchronising runtime calls, so several handles driven from several threads
+ * at once should run without graphs. No counterpart in the reference. */
+int pq_loops_use_graphs(pq_loops *loops, int enable);
+
 /* The operator product's autotuned column-chunk width for NR real columns
  * (tuned on first use: candidates timed with HIP events on this handle's
  * matrix) and its measured time per product in microseconds (0 when NR is not

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <map>
 #include <vector>
 
@@ -145,9 +146,11 @@ __global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M,
     }
 }
 
-// W_re -= w sum_s part_im, W_im += w sum_s part_re (chunks summed in order).
+// W_re -= w sum_s part_im, W_im += w sum_s part_re (chunks summed in order); w in device memory, so
+// a captured graph stays valid from one frequency to the next.
 template <int NR>
-__global__ void gemv_epilogue_kernel(const double *part, int nsplit, int n, double omega, double *W) {
+__global__ void gemv_epilogue_kernel(const double *part, int nsplit, int n, const double *omega_dev, double *W) {
+    const double omega = *omega_dev;
     constexpr int P = NR / 2;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * P) return;
@@ -537,7 +540,8 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
 // Y = PY, the chunks summed in order.
 template <int NR>
 __global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const double *__restrict__ PR,
-                                        const double *__restrict__ PL, int nsplit, int n, double omega,
+                                        const double *__restrict__ PL, int nsplit, int n,
+                                        const double *__restrict__ omega_dev,
                                         const double *__restrict__ V, double *__restrict__ R,
                                         double *__restrict__ Y) {
     constexpr int P = NR / 2;
@@ -552,6 +556,7 @@ __global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const dou
         lr += PL[o + p]; li += PL[o + P + p];
     }
     const size_t o = (size_t)row * NR;
+    const double omega = *omega_dev;
     R[o + p] = V[o + p] - rr + omega * li;
     R[o + P + p] = V[o + P + p] - ri - omega * lr;
     Y[o + p] = yr;
@@ -628,6 +633,11 @@ struct pq_loops {
     int iterations = 0, restarts = 0;
     double residual = 0, seconds = 0, matvec_ms = 0;
     long long matvecs = 0;
+    DevBuf omega_dev;             // the solve's angular frequency, read by the kernels
+    int graphs_on = -1;           // capture the Arnoldi steps: -1 = PCBQS_AMD_GRAPHS ("1": on)
+    // Arnoldi steps captured as graphs (one per step index), valid while `graph_key` holds
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<const void *> graph_key;
     // autotuned column-chunk width of the operator product per column count (first use per NR)
     std::map<int, int> gemv_chunk;
     std::map<int, float> gemv_us;
@@ -706,7 +716,7 @@ void tune_gemv(pq_loops *L, int NR) {
     L->gemv_us[NR] = best_us;
 }
 
-void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P) {
+void apply_operator(pq_loops *L, const double *X, double *W, int P) {
     const int NR = 2 * P, n = L->nf;
     const int chunk = gemv_chunk(L, NR);
     const int nsplit = (n + chunk - 1) / chunk;
@@ -718,10 +728,9 @@ void apply_operator(pq_loops *L, double omega, const double *X, double *W, int P
         hipLaunchKernelGGL(panel_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
                            L->Ll.as<float>(), L->ldl, n, n, X, chunk, L->gpart.as<double>());
         hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
-                           L->gpart.as<double>(), nsplit, n, omega, W);
+                           L->gpart.as<double>(), nsplit, n, L->omega_dev.as<double>(), W);
     });
     PQ_HIP(hipGetLastError());
-    ++L->matvecs;
 }
 
 void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P);
@@ -765,7 +774,8 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
             hipLaunchKernelGGL(panel_kernel<N>, grid, dim3(256), 0, L->stream, bases[b], (long long)n, n, k,
                                L->cvec.as<double>(), pc, part + b * stride);
         hipLaunchKernelGGL(prolong_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream, part,
-                           part + stride, part + 2 * stride, (int)grid.y, n, L->omega_c, X, L->T1.as<double>(),
+                           part + stride, part + 2 * stride, (int)grid.y, n, L->omega_dev.as<double>(), X,
+                           L->T1.as<double>(),
                            L->T2.as<double>());
     });
     PQ_HIP(hipGetLastError());
@@ -865,6 +875,41 @@ void coarse_at(pq_loops *L, double omega) {
     PQ_HIP(hipMemcpyAsync(L->Ainv.p, flat.data(), flat.size() * 8, hipMemcpyHostToDevice, L->stream));
     PQ_HIP(hipStreamSynchronize(L->stream));
     L->omega_c = omega;
+}
+
+// The solve's angular frequency into device memory (the stream is idle between solves).
+void set_omega(pq_loops *L, double omega) {
+    L->omega_dev.reserve(8);
+    PQ_HIP(hipMemcpy(L->omega_dev.p, &omega, 8, hipMemcpyHostToDevice));
+}
+
+void drop_graphs(pq_loops *L) {
+    for (auto &g : L->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    L->graphs.clear();
+    L->graph_key.clear();
+}
+
+// Graphs of the Arnoldi steps replay fixed kernel arguments: they stay valid while every buffer
+// address, the column count, the preconditioner, the coarse space and the tuned chunk are those
+// they were captured with.
+void check_graphs(pq_loops *L, int P, int precond, int m) {
+    auto cv = [](long long v) { return reinterpret_cast<const void *>(static_cast<intptr_t>(v)); };
+    std::vector<const void *> key = {cv(P), cv(precond), cv(m), cv(L->nf), cv(L->kc), cv(L->cdiag),
+                                     cv(gemv_chunk(L, 2 * P)), L->V.p, L->W.p, L->Z.p, L->partial.p, L->Hd.p,
+                                     L->inv.p, L->T1.p, L->T2.p, L->cpart.p, L->cvec.p, L->ppart.p, L->gpart.p,
+                                     L->ctot.p, L->Phi.p, L->Phit.p, L->RPhi.p, L->LPhi.p, L->Ainv.p,
+                                     L->omega_dev.p, L->Ll.p, L->rptr.p};
+    if (key != L->graph_key) {
+        drop_graphs(L);
+        L->graph_key = key;
+    }
+    if ((int)L->graphs.size() < m) L->graphs.resize(m, nullptr);
+}
+
+bool graphs_enabled() {
+    const char *v = std::getenv("PCBQS_AMD_GRAPHS");
+    return v && v[0] == '1';
 }
 
 // The coarse matrix at omega and the coarse work buffers for P columns.
@@ -1007,6 +1052,7 @@ int pq_loops_destroy(pq_loops *L) {
                           &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
                           &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit, &L->ppart})
             b->release();
+        drop_graphs(L);
         L->Hd.release();
         if (L->Hh) (void)hipHostFree(L->Hh);
         if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -1035,6 +1081,11 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl, "pq_loops_solve_ports: port loop out of range");
         DeviceScope scope(L->device);
         tune_gemv(L, 2 * P);
+        set_omega(L, omega);
+        {   // the operator product's partial sums, sized before any capture
+            const int chunk = gemv_chunk(L, 2 * P), nf = std::max(L->nf, 1);
+            L->gpart.reserve((size_t)((nf + chunk - 1) / chunk) * nf * 2 * P * 8);
+        }
         const auto started = std::chrono::steady_clock::now();
         const int n = L->nf, NR = 2 * P, m = std::min(restart, std::max(n, 1));
         const size_t vsz = (size_t)n * NR;
@@ -1073,7 +1124,8 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             for (auto &b : bnorm) b = std::max(b, 1e-300);
             if (X0_host) {                                                     // R = B - A x0
                 PQ_HIP(hipMemcpyAsync(X, X0_host, vsz * 8, hipMemcpyHostToDevice, L->stream));
-                apply_operator(L, omega, X, R, P);
+                apply_operator(L, X, R, P);
+                ++L->matvecs;
                 hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B,
                                    vsz, R);
                 PQ_HIP(hipGetLastError());
@@ -1104,10 +1156,10 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 hipLaunchKernelGGL(vec_update_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, Vb,
                                    vs, nv, hdev, sign, n, P, Wv);
             };
-            auto enqueue_step = [&](int j) {
+            auto step_body = [&](int j) {
                 double *h1 = Hd + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2, *hn = h2 + (size_t)(m + 1) * P * 2;
                 apply_precond(L, precond, V + j * vsz, Z, P);
-                apply_operator(L, omega, Z, W, P);
+                apply_operator(L, Z, W, P);
                 dots_to(V, vsz, j + 1, W, h1);
                 update_dev(V, vsz, j + 1, h1, 1.0, W);
                 dots_to(V, vsz, j + 1, W, h2);
@@ -1116,7 +1168,35 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 hipLaunchKernelGGL(vec_scale_norm_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, W,
                                    hn, n, P, V + (j + 1) * vsz);
                 PQ_HIP(hipGetLastError());
-                PQ_HIP(hipMemcpyAsync(L->Hh + j * hstride, h1, hstride * 8, hipMemcpyDeviceToHost, L->stream));
+            };
+            // Each step's kernels are captured once as a graph (step j's shapes and offsets are
+            // fixed) and replayed: one launch per step instead of about fifteen.
+            const bool use_graphs = L->graphs_on < 0 ? graphs_enabled() : L->graphs_on != 0;
+            if (use_graphs) check_graphs(L, P, precond, m);
+            auto enqueue_step = [&](int j) {
+                if (use_graphs) {
+                    if (!L->graphs[j]) {
+                        hipGraph_t graph = nullptr;
+                        PQ_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+                        try {
+                            step_body(j);
+                        } catch (...) {
+                            (void)hipStreamEndCapture(L->stream, &graph);
+                            if (graph) (void)hipGraphDestroy(graph);
+                            throw;
+                        }
+                        PQ_HIP(hipStreamEndCapture(L->stream, &graph));
+                        const hipError_t e = hipGraphInstantiate(&L->graphs[j], graph, nullptr, nullptr, 0);
+                        (void)hipGraphDestroy(graph);
+                        PQ_HIP(e);
+                    }
+                    PQ_HIP(hipGraphLaunch(L->graphs[j], L->stream));
+                } else {
+                    step_body(j);
+                }
+                ++L->matvecs;
+                PQ_HIP(hipMemcpyAsync(L->Hh + j * hstride, Hd + j * hstride, hstride * 8, hipMemcpyDeviceToHost,
+                                      L->stream));
                 PQ_HIP(hipEventRecord(ev[j], L->stream));
             };
             int its = 0, restarts = 0;
@@ -1194,7 +1274,8 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                     for (int p = 0; p < P; ++p) one[2 * p] = 1.0;
                     update(L, Z, 0, 1, one.data(), -1.0, P, X);                    // X += Z
                 }
-                apply_operator(L, omega, X, R, P);                                  // true residual
+                apply_operator(L, X, R, P);                                         // true residual
+                ++L->matvecs;
                 hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B, vsz, R);
                 PQ_HIP(hipGetLastError());
                 ++restarts;
@@ -1354,6 +1435,7 @@ int pq_loops_coarse_solution(pq_loops *L, double omega, int P, const int32_t *po
         if (n == 0) return;
         port_rhs(L, omega, P, port_cols);
         prepare_coarse(L, omega, P);
+        set_omega(L, omega);
         coarse_correction(L, L->B.as<double>(), P);
         PQ_HIP(hipMemcpyAsync(Xc_host, L->T2.p, (size_t)n * 2 * P * 8, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
@@ -1366,6 +1448,17 @@ int pq_loops_coarse_matrices(pq_loops *L, double *Rc, double *Lc) {
         PQ_REQUIRE(L->kc > 0, "pq_loops_coarse_matrices: no coarse space set");
         std::copy(L->Rc.begin(), L->Rc.end(), Rc);
         std::copy(L->Lc.begin(), L->Lc.end(), Lc);
+    });
+}
+
+int pq_loops_use_graphs(pq_loops *L, int enable) {
+    return guard([&] {
+        PQ_REQUIRE(L, "pq_loops_use_graphs: null handle");
+        L->graphs_on = enable < 0 ? -1 : (enable != 0);
+        if (!enable) {
+            DeviceScope scope(L->device);
+            drop_graphs(L);
+        }
     });
 }
 

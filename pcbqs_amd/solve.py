@@ -186,6 +186,11 @@ class PortSolver:
                              relative_residual=float(res.value), seconds=float(seconds.value), preconditioner=precond))
         return (Z, info, X) if currents else (Z, info)
 
+    def use_graphs(self, enable=True):
+        """Replay each Arnoldi step as a captured graph (True), launch kernels one by one (False),
+        or follow PCBQS_AMD_GRAPHS (None)."""
+        call('pq_loops_use_graphs', self._handle, -1 if enable is None else int(bool(enable)))
+
     def tuning(self, columns=None):
         """(chunk, microseconds) of the autotuned operator product for 2P real columns (P ports by
         default): the chosen column-chunk width and its time per product."""

@@ -24,11 +24,12 @@ def _host(board, grid, precond):
     return model, phi
 
 
-def _board(i, board, freqs, precond, grid, tol, device):
+def _board(i, board, freqs, precond, grid, tol, device, graphs):
     from .solve import PortSolver
     t = time.perf_counter()
     model, phi = _host(board, grid, precond)
     solver = PortSolver(board, device=device, model=model)
+    solver.use_graphs(graphs)
     if phi is not None:
         solver.set_coarse(phi)
     setup = time.perf_counter() - t
@@ -42,16 +43,22 @@ def _board(i, board, freqs, precond, grid, tol, device):
                 worst_residual=max(r['relative_residual'] for r in info), Z=Z)
 
 
-def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concurrency=4, log=None):
+def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concurrency=4, graphs=False,
+          log=None):
     """(per-board results in board order, totals) of the port solves of every board at every
     frequency. `concurrency` boards run at once, each on its own stream from its own host thread
-    (the library releases the interpreter during device calls), so one board's host work and
-    kernel-launch latency overlap another's kernels."""
+    (the library releases the interpreter during device calls). graphs=True (concurrency 1 only: a
+    capture does not survive other threads' synchronising calls) replays each Arnoldi step as a
+    captured graph; at these board sizes capturing a board's ~100 step graphs costs more than the
+    launches it saves."""
+    if graphs and concurrency > 1:
+        raise ValueError('captured steps need concurrency 1')
     freqs = [float(f) for f in np.atleast_1d(freqs)]
     t0 = time.perf_counter()
     results = [None] * len(boards)
     with ThreadPoolExecutor(max(1, int(concurrency))) as pool:
-        futures = {pool.submit(_board, i, b, freqs, precond, grid, tol, device): i for i, b in enumerate(boards)}
+        futures = {pool.submit(_board, i, b, freqs, precond, grid, tol, device, graphs): i
+                   for i, b in enumerate(boards)}
         for fut in futures:
             r = fut.result()
             results[r['board']] = r
@@ -63,7 +70,7 @@ def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concu
     columns = sum(r['ports'] * len(freqs) for r in results)
     return results, dict(seconds=seconds, boards=len(results), frequencies=len(freqs), solves=solves,
                          port_columns=columns, port_columns_per_s=columns / seconds if seconds > 0 else 0.0,
-                         precond=precond, grid=grid, concurrency=int(concurrency))
+                         precond=precond, grid=grid, concurrency=int(concurrency), graphs=graphs)
 
 
 def main(argv=None):
@@ -75,10 +82,13 @@ def main(argv=None):
     ap.add_argument('--frequencies', type=int, default=20)
     ap.add_argument('--precond', default='two-level')
     ap.add_argument('--grid', type=int, default=12)
+    ap.add_argument('--concurrency', type=int, default=4)
+    ap.add_argument('--graphs', action='store_true')
     args = ap.parse_args(argv)
     freqs = np.logspace(3, 8, args.frequencies)
     bs = [boards.board_batch(s) for s in range(args.boards)]
-    res, total = sweep(bs, freqs, precond=args.precond, grid=args.grid, log=lambda m: print(m, flush=True))
+    res, total = sweep(bs, freqs, precond=args.precond, grid=args.grid, concurrency=args.concurrency,
+                       graphs=args.graphs, log=lambda m: print(m, flush=True))
     print(json.dumps(total))
 
 

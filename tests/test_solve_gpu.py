@@ -55,3 +55,24 @@ def test_copper_reference_fixture_through_the_device_pipeline():
     Z, info = s.ports([0, 1e5, 2e5], tol=1e-10)
     np.testing.assert_allclose(Z, expected['impedance'], rtol=1e-6)
     s.close()
+
+
+@pytest.mark.parametrize('precond', ['block-jacobi', 'two-level'])
+def test_captured_steps_replay_the_eager_solve_bitwise(precond, monkeypatch):
+    """The Arnoldi steps captured as graphs and replayed (across restarts and frequencies, with the
+    solver's eager work in between) give the eager launches' results bit for bit."""
+    from pcbqs_amd.solve import PortSolver
+    b = boards.board_c2(2, target=3000)
+    freqs = [1e3, 1e6, 1e8, 1e6]
+    out = {}
+    for graphs in ('0', '1'):
+        monkeypatch.setenv('PCBQS_AMD_GRAPHS', graphs)
+        s = PortSolver(b, device=0)
+        s.use_graphs(None)                       # follow the environment
+        if precond == 'two-level':
+            s.set_stream_coarse(grid=6)
+        out[graphs] = s.ports(freqs, precond=precond, tol=1e-9, restart=30)
+        s.close()
+    (Z0, i0), (Z1, i1) = out['0'], out['1']
+    assert [r['iterations'] for r in i0] == [r['iterations'] for r in i1]
+    assert np.array_equal(Z0, Z1)

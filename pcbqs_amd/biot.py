@@ -127,3 +127,44 @@ def h_sheets(sheets, pts, near=1.5, levels=2, precision=64, device=0):
 def h_segments(p0, p1, I, pts, precision=64, device=0):
     """The reference's h_segments on the device."""
     return h_field([], pts, (p0, p1, I), precision=precision, device=device)
+
+
+def main(argv=None):
+    """Time the field of random RWG currents on the seeded plane pair at a grid of points."""
+    import argparse
+    import json
+    import time
+    from . import board as boards
+    from .geometry import RWG
+    ap = argparse.ArgumentParser(description='Biot-Savart timing on the seeded 2-layer plane pair')
+    ap.add_argument('--grid', type=int, default=1000, help='points per side of the observation grid')
+    ap.add_argument('--excitations', type=int, default=8)
+    ap.add_argument('--precision', type=int, default=32)
+    ap.add_argument('--height', type=float, default=2e-3)
+    ap.add_argument('--seed', type=int, default=0)
+    args = ap.parse_args(argv)
+    b = boards.board_c2(args.seed)
+    r = RWG(b.nodes, b.tris)
+    rng = np.random.default_rng(args.seed)
+    K = args.excitations
+    I = rng.normal(size=(r.Ne, K)) + 1j * rng.normal(size=(r.Ne, K))
+    a, bb = rwg_currents(r, I)
+    sheets = [(b.nodes, b.tris[b.layer_of == lay], a[b.layer_of == lay], bb[b.layer_of == lay],
+               float(b.z_of[b.layer_of == lay][0])) for lay in np.unique(b.layer_of)]
+    lo, hi = b.nodes.min(0), b.nodes.max(0)
+    X, Y = np.meshgrid(np.linspace(lo[0], hi[0], args.grid), np.linspace(lo[1], hi[1], args.grid))
+    pts = np.c_[X.ravel(), Y.ravel(), np.full(X.size, args.height)]
+    timing = {}
+    t = time.perf_counter()
+    h_field(sheets, pts, precision=args.precision, timing=timing)
+    total = time.perf_counter() - t
+    inter = timing['sources'] * timing['points']
+    kk = 1 if K <= 1 else 2 if K <= 2 else 4 if K <= 4 else 8
+    print(json.dumps(dict(points=timing['points'], sources=timing['sources'], excitations=K, precision=args.precision,
+                          kernel_ms=timing['kernel_ms'], total_s=total,
+                          interactions_per_s=inter / (timing['kernel_ms'] * 1e-3),
+                          tflops=inter * (20 + 16 * kk) / (timing['kernel_ms'] * 1e-3) / 1e12)))
+
+
+if __name__ == '__main__':
+    main()

@@ -185,6 +185,13 @@ int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *po
  * right-hand side of direct_ports (src/pcbqs/mqs_ports.py:52-69). */
 int pq_loops_rhs(pq_loops *loops, double omega, int P, const int32_t *port_cols, double *B_host);
 
+/* W = Z_ff X = (Rl + j omega Ll) X on the free loops, X and W (n_free, 2P) FP64
+ * host in the layout of pq_loops_rhs (real parts in columns 0..P-1, imaginary
+ * parts in P..2P-1), P 1..8 or 16: the GMRES matvec, with the kernel the
+ * autotuner picked for 2P columns. Replaces the product with A in
+ * _lockstep_gmres (src/pcbqs/mqs_iterative.py:109-170). */
+int pq_loops_operator(pq_loops *loops, double omega, int P, const double *X_host, double *W_host);
+
 /* The coarse-space solution Phi (Phi^T Z_ff Phi)^-1 Phi^T B (n_free, 2P) FP64
  * host for the port right-hand sides at omega: the input of the learned
  * correction that refines it into an initial iterate. No counterpart in the
@@ -214,11 +221,13 @@ int pq_loops_coarse_matrices(pq_loops *loops, double *Rc, double *Lc);
  * at once should run without graphs. No counterpart in the reference. */
 int pq_loops_use_graphs(pq_loops *loops, int enable);
 
-/* The operator product's autotuned column-chunk width for NR real columns
- * (tuned on first use: candidates timed with HIP events on this handle's
- * matrix) and its measured time per product in microseconds (0 when NR is not
- * tuned: NR > 16 or fewer than 1024 free loops). No counterpart in the
- * reference. */
+/* The operator product's autotuned kernel for NR real columns (tuned on first
+ * use: candidates timed with HIP events on this handle's matrix): chunk 0 is
+ * the symmetric kernel that reads the tiles on and above the diagonal of Ll
+ * only (NR <= 16), chunk > 0 the column-chunk width of the whole-matrix panel
+ * kernel; and its measured time per product in microseconds (0 when not tuned:
+ * fewer than 1024 free loops). PCBQS_AMD_OPERATOR=panel or =symmetric pins the
+ * choice. No counterpart in the reference. */
 int pq_loops_tuning(pq_loops *loops, int NR, int *chunk, double *microseconds);
 
 /* Statistics of the last solve: iterations, operator products, relative

@@ -73,6 +73,7 @@ _SIGNATURES = {
     'pq_mpnn_aggregate': ([ctypes.c_int, P, P, P, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P], ctypes.c_int),
     'pq_mpnn_relu_mask': ([SIZE, P, P, P, P], ctypes.c_int),
     'pq_loops_rhs': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_loops_operator': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
     'pq_loops_tuning': ([P, ctypes.c_int, P, P], ctypes.c_int),

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <cstring>
+#include <limits>
 #include <map>
 #include <vector>
 
@@ -104,17 +106,19 @@ __global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
     Ll[j * ldl + i] = v;
 }
 
-// Panel products: part[s][i][r] = sum over the rows c of chunk s of M[c][i] X[c][r], M (ncols, ld)
-// FP32 row-major, X (ncols, NR) FP64, i < nrows. One thread per i, so each row segment of M is 256
-// consecutive floats (coalesced), 16 of them in flight per lane; X[c] is the same for the whole wave
-// and comes through scalar loads, which keeps the LDS return path out of the inner loop. With the
-// symmetric Ll (rows read as columns) this is Ll X; with a transposed basis (k, n) it is Phi c.
+// Panel products: part[i][s][r] = sum over the rows c of chunk s of M[c][i] X[c][r], M (ncols, ld)
+// FP32 row-major, X (ncols, NR) FP64, i < nrows. One thread per i (blockDim.x rows per block), so
+// each row segment of M is consecutive floats (coalesced), 16 of them in flight per lane; X[c] is
+// the same for the whole wave and comes through scalar loads, which keeps the LDS return path out
+// of the inner loop. With the symmetric Ll (rows read as columns) this is Ll X. M is streamed once
+// per product, so its loads are non-temporal: they do not displace the vectors and bases that the
+// rest of the iteration re-reads.
 constexpr int GEMV_SPLIT = 32, GEMV_ROWS = 256, GEMV_CHUNK = 640;
 template <int NR>
 __global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M, long long ld, int nrows, int ncols,
                                                    const double *__restrict__ X, int chunk,
                                                    double *__restrict__ part) {
-    const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(ncols, c0 + chunk);
     double acc[NR];
 #pragma unroll
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M,
     for (; c + 16 <= c1; c += 16) {
         float m[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = col[(size_t)(c + u) * ld];
+        for (int u = 0; u < 16; ++u) m[u] = __builtin_nontemporal_load(col + (size_t)(c + u) * ld);
         const double *x = X + (size_t)c * NR;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
@@ -134,13 +138,13 @@ __global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M,
         }
     }
     for (; c < c1; ++c) {
-        const double mv = col[(size_t)c * ld];
+        const double mv = __builtin_nontemporal_load(col + (size_t)c * ld);
         const double *x = X + (size_t)c * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
     }
     if (row < nrows) {
-        double *o = part + ((size_t)blockIdx.y * nrows + row) * NR;
+        double *o = part + ((size_t)row * gridDim.y + blockIdx.y) * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) o[r] = acc[r];
     }
@@ -157,11 +161,226 @@ __global__ void gemv_epilogue_kernel(const double *part, int nsplit, int n, cons
     const int row = e / P, p = e % P;
     double yr = 0, yi = 0;
     for (int q = 0; q < nsplit; ++q) {
-        yr += part[((size_t)q * n + row) * NR + p];
-        yi += part[((size_t)q * n + row) * NR + P + p];
+        yr += part[((size_t)row * nsplit + q) * NR + p];
+        yi += part[((size_t)row * nsplit + q) * NR + P + p];
     }
     W[(size_t)row * NR + p] -= omega * yi;
     W[(size_t)row * NR + P + p] += omega * yr;
+}
+
+// The operator's epilogue fused with the resistive part: W = Rl X + j w (sum of the nsplit chunk
+// partials of Ll X), for NR a power of two. One wave per row: lanes over the row's CSR entries
+// (port columns >= n skipped) and over the partials, each lane forming its NR-vector contribution;
+// then a recursive-halving butterfly (each step a lane keeps half of its vector and adds its
+// partner's copy of that half; once a single value is left, plain xor steps) leaves the sum of
+// column r in lane r * 64 / NR, in a fixed order.
+template <int NR>
+__global__ __launch_bounds__(256) void operator_epilogue_kernel(const int *__restrict__ ptr,
+                                                               const int *__restrict__ col,
+                                                               const double *__restrict__ val, int n,
+                                                               const double *__restrict__ X,
+                                                               const double *__restrict__ part, int nsplit,
+                                                               const double *__restrict__ omega_dev,
+                                                               double *__restrict__ W) {
+    static_assert((NR & (NR - 1)) == 0 && NR >= 2 && NR <= 32, "NR a power of two");
+    constexpr int P = NR / 2;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    double a[NR], b[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = b[r] = 0;
+    for (int q = lane; q < nsplit; q += 64) {
+        const double *pq = part + ((size_t)row * nsplit + q) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) b[r] += pq[r];
+    }
+    // the first 128 entries with every load issued up front (an absent or port entry gets weight 0
+    // and row 0's x); longer rows continue in the loop below
+    const int t0 = ptr[row], t1 = ptr[row + 1];
+    {
+        const int ta = t0 + lane, tb = ta + 64;
+        const int ca = ta < t1 ? col[ta] : n, cb = tb < t1 ? col[tb] : n;
+        const double va = ca < n ? val[ta] : 0.0, vb = cb < n ? val[tb] : 0.0;
+        const double *xa = X + (size_t)(ca < n ? ca : 0) * NR, *xb = X + (size_t)(cb < n ? cb : 0) * NR;
+        double ya[NR], yb[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            ya[r] = xa[r];
+            yb[r] = xb[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            a[r] += va * ya[r];
+            a[r] += vb * yb[r];
+        }
+    }
+    for (int t = t0 + 128 + lane; t < t1; t += 64) {
+        const int c = col[t];
+        if (c < n) {
+            const double v = val[t];
+            const double *x = X + (size_t)c * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) a[r] += v * x[r];
+        }
+    }
+    const double omega = *omega_dev;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const double re = a[p] - omega * b[P + p], im = a[P + p] + omega * b[p];
+        a[p] = re;
+        a[P + p] = im;
+    }
+    // recursive halving over lane bits 5, 4, ...: after it, lane bits above log2(64 / NR) name the
+    // column
+#pragma unroll
+    for (int off = 32, half = NR / 2; half >= 1; off >>= 1, half >>= 1) {
+        const bool upper = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double keep = upper ? a[i + half] : a[i];
+            const double send = upper ? a[i] : a[i + half];
+            a[i] = keep + __shfl_xor(send, off);
+        }
+    }
+#pragma unroll
+    for (int off = 32 / NR; off >= 1; off >>= 1) a[0] += __shfl_xor(a[0], off);
+    constexpr int S = 64 / NR;
+    if (lane % S == 0) W[(size_t)row * NR + lane / S] = a[0];
+}
+
+// ---- the symmetric operator product ----------------------------------------------------------
+//
+// Ll is bitwise symmetric (symmetrise_kernel) and stored whole, but this product reads only the
+// 256 x 256 tiles on and above the diagonal: half the bytes of panel_kernel. In an off-diagonal
+// tile (I, J), I < J, with entries Ll[r][c] (r in row block I, c in column block J):
+//   direct      y[r] += sum_c Ll[r][c] x[c]   thread per row, x[c] wave-uniform (scalar loads);
+//   transposed  y[c] += sum_r Ll[r][c] x[r]   the tile passes through LDS 64 columns at a time;
+//                                             lane per column, each wave sums 64 rows with x[r]
+//                                             wave-uniform, and the four waves' sums are added
+//                                             in wave order.
+// A diagonal tile does the direct product only. Output row block K receives exactly one partial
+// per block index s: the direct one of tile (K, s) for s >= K and the transposed one of tile
+// (s, K) for s < K, so gemv_epilogue_kernel / gemv_sum_kernel add nb = ceil(n / 256) partials in
+// a fixed order. While one 64-column panel is reduced, the next one is already loading.
+constexpr int SYM_T = 256, SYM_SUB = 64, SYM_PITCH = SYM_SUB + 1;
+
+template <bool FULL>
+__device__ __forceinline__ void sym_load(const float *colp, long long ld, int cs, int n, float (&m)[SYM_SUB]) {
+#pragma unroll
+    for (int u = 0; u < SYM_SUB; ++u) m[u] = colp[(size_t)(FULL ? cs + u : min(cs + u, n - 1)) * ld];
+}
+
+template <int NR, bool FULL>
+__device__ __forceinline__ void sym_direct(const float (&m)[SYM_SUB], const double *__restrict__ X, int cs, int n,
+                                           double (&acc)[NR]) {
+#pragma unroll
+    for (int u = 0; u < SYM_SUB; ++u) {
+        if (!FULL && cs + u >= n) break;
+        const double mu = m[u];
+        const double *x = X + (size_t)(cs + u) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += mu * x[r];
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(256, 2) void sym_panel_kernel(const float *__restrict__ M, long long ld, int n, int nb,
+                                                           const double *__restrict__ X, double *__restrict__ part) {
+    constexpr int TD = SYM_T * SYM_PITCH / 2 + 1, ZD = 4 * SYM_SUB * (NR + 1);
+    __shared__ double lds[TD > ZD ? TD : ZD];
+    float *T = reinterpret_cast<float *>(lds);
+    // blocks [0, nb (nb - 1) / 2) are the off-diagonal tiles, column block by column block; then
+    // the nb diagonal tiles
+    const int ntri = nb * (nb - 1) / 2, t = blockIdx.x;
+    int I, J;
+    if (t < ntri) {
+        J = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+        while (J * (J - 1) / 2 > t) --J;
+        while ((J + 1) * J / 2 <= t) ++J;
+        I = t - J * (J - 1) / 2;
+    } else {
+        I = J = t - ntri;
+    }
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r0 = I * SYM_T, c0 = J * SYM_T, cend = min(n, c0 + SYM_T);
+    const int row = r0 + tid;
+    const float *colp = M + min(row, n - 1);          // Ll[row][c] = colp[c * ld]
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    if (I == J) {
+        int c = c0;
+        for (; c + 16 <= cend; c += 16) {
+            float m[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) m[u] = colp[(size_t)(c + u) * ld];
+            const double *x = X + (size_t)c * NR;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const double mu = m[u];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] += mu * x[u * NR + r];
+            }
+        }
+        for (; c < cend; ++c) {
+            const double mv = colp[(size_t)c * ld];
+            const double *x = X + (size_t)c * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
+        }
+        if (row < n) {
+            double *o = part + ((size_t)row * nb + J) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o[r] = acc[r];
+        }
+        return;
+    }
+    // off-diagonal: row block I < nb - 1 is full (256 rows < n); column block J may be ragged
+    float m[SYM_SUB];
+    if (c0 + SYM_SUB <= n) sym_load<true>(colp, ld, c0, n, m);
+    else sym_load<false>(colp, ld, c0, n, m);
+    const double *xr = X + (size_t)(r0 + w * 64) * NR;   // this wave's rows, for the transposed sums
+    double *Z = lds;
+#pragma unroll 1
+    for (int cs = c0; cs < cend; cs += SYM_SUB) {
+        if (cs + SYM_SUB <= n) sym_direct<NR, true>(m, X, cs, n, acc);
+        else sym_direct<NR, false>(m, X, cs, n, acc);
+#pragma unroll
+        for (int u = 0; u < SYM_SUB; ++u) T[tid * SYM_PITCH + u] = m[u];
+        __syncthreads();
+        const int cn = cs + SYM_SUB;
+        if (cn < cend) {
+            if (cn + SYM_SUB <= n) sym_load<true>(colp, ld, cn, n, m);
+            else sym_load<false>(colp, ld, cn, n, m);
+        }
+        double z[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) z[r] = 0;
+        const float *Tw = T + w * 64 * SYM_PITCH + lane;
+#pragma unroll 8
+        for (int i = 0; i < 64; ++i) {
+            const double tv = Tw[i * SYM_PITCH];
+            const double *x = xr + (size_t)i * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) z[r] += tv * x[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) Z[(w * 64 + lane) * (NR + 1) + r] = z[r];
+        __syncthreads();
+        for (int e = tid; e < 64 * NR; e += 256) {
+            const int cl = e / NR, r = e % NR;
+            if (cs + cl < n)
+                part[((size_t)(cs + cl) * nb + I) * NR + r] =
+                    Z[cl * (NR + 1) + r] + Z[(64 + cl) * (NR + 1) + r] + Z[(128 + cl) * (NR + 1) + r] +
+                    Z[(192 + cl) * (NR + 1) + r];
+        }
+        __syncthreads();
+    }
+    double *o = part + ((size_t)row * nb + J) * NR;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) o[r] = acc[r];
 }
 
 // ---- block-Jacobi --------------------------------------------------------------------------
@@ -327,9 +546,9 @@ __global__ __launch_bounds__(256) void dots_partial_kernel(const double *__restr
 }
 
 // h[i][q] = sum over the nblk block partials of vector i (one wave per output; lanes over blocks,
-// then a fixed butterfly).
+// then a fixed butterfly); also into h_host (pinned, coherent host memory) when given.
 __global__ __launch_bounds__(256) void dots_reduce_kernel(const double *__restrict__ partial, int nblk, int nv,
-                                                         int NR, double *__restrict__ h) {
+                                                         int NR, double *__restrict__ h, double *__restrict__ h_host) {
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= nv * NR) return;
@@ -337,7 +556,10 @@ __global__ __launch_bounds__(256) void dots_reduce_kernel(const double *__restri
     double s = 0;
     for (int b = lane; b < nblk; b += 64) s += partial[((size_t)i * nblk + b) * NR + q];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) h[e] = s;
+    if (lane == 0) {
+        h[e] = s;
+        if (h_host) h_host[e] = s;
+    }
 }
 
 // W[n,p] -= sum_i V_i[n,p] h[i][p] (complex); with sign -1 it adds. Four adjacent lanes share an
@@ -428,6 +650,48 @@ __global__ __launch_bounds__(256) void port_reduce_kernel(const float *Ll, long 
 
 // ---- two-level (coarse space) preconditioner -------------------------------------------------
 
+// The second Gram-Schmidt pass and the normalisation in one. h holds, per port p, the
+// coefficients h_i = conj(V_i) . W for i < nv and then (|W_p|^2, 0), as the dots leave them for
+// the nv + 1 vectors V_0..V_{nv-1}, W. Then W_p := (W_p - sum_i V_i h_i) / sqrt(d_p) with
+// d_p = |W_p|^2 - sum_i |h_i|^2 (Pythagoras: the V_i are orthonormal), and nrm[p] = (d_p, 0), the
+// squared Hessenberg subdiagonal (also written to nrm_host, pinned host memory); d_p <= 0 zeroes
+// the column. Four adjacent lanes share an output
+// as in vec_update_kernel; the sums are combined by the same fixed butterfly in every row.
+__global__ __launch_bounds__(256) void vec_update_norm_kernel(const double *V, size_t vstride, int nv, const double *h,
+                                                             int n, int P, double *W, double *nrm, double *nrm_host) {
+    const int NR = 2 * P;
+    const size_t e = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 2;
+    const int part = threadIdx.x & 3;
+    const bool live = e < (size_t)n * P;
+    const int row = live ? (int)(e / P) : 0, p = live ? (int)(e % P) : 0;
+    double ar = 0, ai = 0, hh = 0;
+    if (live) {
+        for (int i = part; i < nv; i += 4) {
+            const double hr = h[(i * P + p) * 2], hi = h[(i * P + p) * 2 + 1];
+            const double vr = V[(size_t)i * vstride + (size_t)row * NR + p];
+            const double vi = V[(size_t)i * vstride + (size_t)row * NR + P + p];
+            ar += vr * hr - vi * hi;
+            ai += vr * hi + vi * hr;
+            hh += hr * hr + hi * hi;
+        }
+    }
+    ar += __shfl_xor(ar, 1); ai += __shfl_xor(ai, 1); hh += __shfl_xor(hh, 1);
+    ar += __shfl_xor(ar, 2); ai += __shfl_xor(ai, 2); hh += __shfl_xor(hh, 2);
+    if (live && part == 0) {
+        const double d = h[((size_t)nv * P + p) * 2] - hh;
+        const double wr = W[(size_t)row * NR + p] - ar, wi = W[(size_t)row * NR + P + p] - ai;
+        const double s = d > 0 ? sqrt(d) : 0.0;
+        W[(size_t)row * NR + p] = d > 0 ? wr / s : 0.0;
+        W[(size_t)row * NR + P + p] = d > 0 ? wi / s : 0.0;
+        if (row == 0) {
+            nrm[2 * p] = d;
+            nrm[2 * p + 1] = 0.0;
+            nrm_host[2 * p] = d;
+            nrm_host[2 * p + 1] = 0.0;
+        }
+    }
+}
+
 // Y[n,p] = X[n,p] / sqrt(nrm[p]) with nrm[p] = (|X_p|^2, 0) as dots_reduce leaves it (0 -> 0).
 __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n, int P, double *Y) {
     const int NR = 2 * P;
@@ -440,17 +704,17 @@ __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n,
 
 // Restriction partials: partial[chunk][c][r] = sum over the chunk's rows i of Phi[i][c] V[i][r],
 // Phi (n, k) FP32 row-major, V (n, NR) FP64. A block covers 64 coarse columns (the lanes, so each
-// row segment of Phi is one coalesced 256-byte read, eight in flight per lane) and cr_rows(NR)
-// rows split over its four waves; a row of V is wave-uniform (scalar loads). The four waves' sums
-// are added in wave order.
+// row segment of Phi is one coalesced 256-byte read, sixteen in flight per lane) and `rows` rows
+// split over its four waves; a row of V is wave-uniform (scalar loads). The four waves' sums are
+// added in wave order.
 constexpr int cr_rows(int NR) { return NR >= 8 ? 4096 / NR : 512; }
 template <int NR>
 __global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__restrict__ Phi, int k,
-                                                             const double *__restrict__ V, int n,
+                                                             const double *__restrict__ V, int n, int rows,
                                                              double *__restrict__ partial) {
-    constexpr int ROWS = cr_rows(NR), U = 8;
+    constexpr int U = 16;
     __shared__ double red[4 * 64 * NR];
-    const int i0 = blockIdx.x * ROWS, m = min(ROWS, n - i0);
+    const int i0 = blockIdx.x * rows, m = min(rows, n - i0);
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.y * 64 + lane;
     double acc[NR];
@@ -498,21 +762,30 @@ __global__ void coarse_sum_kernel(const double *partial, int nblk, int total, do
     t[e] = s;
 }
 
-// Diagonal coarse matrix: c[a][p] = D[a] t[a][p] with t summed from the chunk partials in chunk
-// order (t and c (k, NR), real parts in columns 0..P-1; D complex interleaved).
-__global__ void coarse_sum_diag_kernel(const double *partial, int nblk, int k, int P, const double *D,
-                                       double *cvec) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// Diagonal coarse matrix: c[a][p] = D[a] t[a][p] with t summed from the chunk partials (t and
+// c (k, NR), real parts in columns 0..P-1; D complex interleaved); one wave per output (a, p),
+// lanes over the chunks, then a fixed butterfly.
+__global__ __launch_bounds__(256) void coarse_sum_diag_kernel(const double *partial, int nblk, int k, int P,
+                                                             const double *D, double *cvec) {
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= k * P) return;
     const int NR = 2 * P, a = e / P, p = e % P;
     double tr = 0, ti = 0;
-    for (int q = 0; q < nblk; ++q) {
+    for (int q = lane; q < nblk; q += 64) {
         tr += partial[((size_t)q * k + a) * NR + p];
         ti += partial[((size_t)q * k + a) * NR + P + p];
     }
-    const double dr = D[2 * a], di = D[2 * a + 1];
-    cvec[(size_t)a * NR + p] = dr * tr - di * ti;
-    cvec[(size_t)a * NR + P + p] = dr * ti + di * tr;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        tr += __shfl_xor(tr, off);
+        ti += __shfl_xor(ti, off);
+    }
+    if (lane == 0) {
+        const double dr = D[2 * a], di = D[2 * a + 1];
+        cvec[(size_t)a * NR + p] = dr * tr - di * ti;
+        cvec[(size_t)a * NR + P + p] = dr * ti + di * tr;
+    }
 }
 
 // c = Ainv t (complex; t and c (k, NR) with real parts in columns 0..P-1). One wave per output
@@ -535,32 +808,89 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
     if (lane == 0) { cvec[(size_t)a * NR + p] = yr; cvec[(size_t)a * NR + P + p] = yi; }
 }
 
-// The prolongation's epilogue: with the panel products PY = Phi c, PR = Rl Phi c, PL = Ll Phi c
-// (each nsplit chunk partials of (n, NR), real parts in columns 0..P-1), R = V - (PR + j w PL) and
-// Y = PY, the chunks summed in order.
+// The prolongation: the three transposed bases (k, n) FP32 Phi^T, (Rl Phi)^T, (Ll Phi)^T read in
+// one pass against the coarse vector c (k, NR), thread per row, a chunk of coarse columns per
+// blockIdx.y, eight columns of each basis (24 loads) in flight; c wave-uniform (scalar loads).
+// Chunk partials: PY = Phi c and PA = (Rl + j w Ll) Phi c (real parts in columns 0..P-1).
 template <int NR>
-__global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const double *__restrict__ PR,
-                                        const double *__restrict__ PL, int nsplit, int n,
-                                        const double *__restrict__ omega_dev,
-                                        const double *__restrict__ V, double *__restrict__ R,
-                                        double *__restrict__ Y) {
-    constexpr int P = NR / 2;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * P) return;
-    const int row = e / P, p = e % P;
-    double yr = 0, yi = 0, rr = 0, ri = 0, lr = 0, li = 0;
-    for (int q = 0; q < nsplit; ++q) {
-        const size_t o = ((size_t)q * n + row) * NR;
-        yr += PY[o + p]; yi += PY[o + P + p];
-        rr += PR[o + p]; ri += PR[o + P + p];
-        lr += PL[o + p]; li += PL[o + P + p];
+__global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ B0, const float *__restrict__ B1,
+                                                     const float *__restrict__ B2, int n, int k,
+                                                     const double *__restrict__ C, int chunk,
+                                                     const double *__restrict__ omega_dev, double *__restrict__ PY,
+                                                     double *__restrict__ PA) {
+    constexpr int P = NR / 2, U = 8;
+    const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
+    const int c0 = blockIdx.y * chunk, c1 = min(k, c0 + chunk);
+    double ay[NR], ar[NR], al[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) ay[r] = ar[r] = al[r] = 0;
+    const size_t rr = (size_t)min(row, n - 1);
+    int c = c0;
+    for (; c + U <= c1; c += U) {
+        float f0[U], f1[U], f2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            f0[u] = B0[(size_t)(c + u) * n + rr];
+            f1[u] = B1[(size_t)(c + u) * n + rr];
+            f2[u] = B2[(size_t)(c + u) * n + rr];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double a = f0[u], b = f1[u], d = f2[u];
+            const double *x = C + (size_t)(c + u) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                ay[r] += a * x[r];
+                ar[r] += b * x[r];
+                al[r] += d * x[r];
+            }
+        }
     }
-    const size_t o = (size_t)row * NR;
-    const double omega = *omega_dev;
-    R[o + p] = V[o + p] - rr + omega * li;
-    R[o + P + p] = V[o + P + p] - ri - omega * lr;
-    Y[o + p] = yr;
-    Y[o + P + p] = yi;
+    for (; c < c1; ++c) {
+        const double a = B0[(size_t)c * n + rr], b = B1[(size_t)c * n + rr], d = B2[(size_t)c * n + rr];
+        const double *x = C + (size_t)c * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            ay[r] += a * x[r];
+            ar[r] += b * x[r];
+            al[r] += d * x[r];
+        }
+    }
+    if (row < n) {
+        const double omega = *omega_dev;
+        double *oy = PY + ((size_t)blockIdx.y * n + row) * NR, *oa = PA + ((size_t)blockIdx.y * n + row) * NR;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            oa[p] = ar[p] - omega * al[P + p];
+            oa[P + p] = ar[P + p] + omega * al[p];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) oy[r] = ay[r];
+    }
+}
+
+// R = V - sum PA, Y = sum PY over the nsplit chunk partials of prolong_kernel, in chunk order.
+__global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const double *__restrict__ PA, int nsplit,
+                                        size_t count, const double *__restrict__ V, double *__restrict__ R,
+                                        double *__restrict__ Y) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    double y = 0, a = 0;
+    int q = 0;
+    for (; q + 4 <= nsplit; q += 4) {
+        const double y0 = PY[(size_t)q * count + e], y1 = PY[(size_t)(q + 1) * count + e];
+        const double y2 = PY[(size_t)(q + 2) * count + e], y3 = PY[(size_t)(q + 3) * count + e];
+        const double a0 = PA[(size_t)q * count + e], a1 = PA[(size_t)(q + 1) * count + e];
+        const double a2 = PA[(size_t)(q + 2) * count + e], a3 = PA[(size_t)(q + 3) * count + e];
+        y += y0; y += y1; y += y2; y += y3;
+        a += a0; a += a1; a += a2; a += a3;
+    }
+    for (; q < nsplit; ++q) {
+        y += PY[(size_t)q * count + e];
+        a += PA[(size_t)q * count + e];
+    }
+    R[e] = V[e] - a;
+    Y[e] = y;
 }
 
 // Out (k, n) = In (n, k)^T, FP32.
@@ -577,7 +907,8 @@ __global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * NR) return;
     double s = 0;
-    for (int q = 0; q < nsplit; ++q) s += part[(size_t)q * n * NR + e];
+    const size_t row = e / NR, r = e % NR;
+    for (int q = 0; q < nsplit; ++q) s += part[(row * nsplit + q) * NR + r];
     W[e] = s;
 }
 
@@ -639,8 +970,10 @@ struct pq_loops {
     std::vector<hipGraphExec_t> graphs;
     std::vector<const void *> graph_key;
     // autotuned column-chunk width of the operator product per column count (first use per NR)
-    std::map<int, int> gemv_chunk;
+    std::map<int, int> gemv_chunk, gemv_rows;
     std::map<int, float> gemv_us;
+    // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks)
+    std::map<int, int> crows, psplit;
 };
 
 namespace {
@@ -663,40 +996,108 @@ template <class F> void dispatch_nr(int NR, F &&f) {
 // W = (Rl + j w Ll) X on the free block.
 int default_chunk(int n) { return std::min(GEMV_CHUNK, std::max(64, (n + GEMV_SPLIT - 1) / GEMV_SPLIT)); }
 
-int gemv_chunk(const pq_loops *L, int NR) {
+// The operator product's launch shape: chunk 0 is the symmetric kernel; chunk > 0 the panel
+// kernel with `rows` rows per block and `chunk` columns per partial.
+struct OpShape {
+    int chunk, rows;
+};
+
+OpShape gemv_shape(const pq_loops *L, int NR) {
     auto it = L->gemv_chunk.find(NR);
-    return it != L->gemv_chunk.end() ? it->second : default_chunk(L->nf);
+    if (it == L->gemv_chunk.end()) return {default_chunk(L->nf), GEMV_ROWS};
+    auto r = L->gemv_rows.find(NR);
+    return {it->second, r != L->gemv_rows.end() ? r->second : GEMV_ROWS};
 }
 
-// The autotuner of the operator product: the column-chunk width sets the grid (rows / 256 x
-// n / chunk blocks), the loads in flight and the partial sums the epilogue reads. Each candidate
-// runs once to warm up and three times under HIP events on an input of zeros; the fastest is kept
+int gemv_chunk(const pq_loops *L, int NR) { return gemv_shape(L, NR).chunk; }
+
+// Number of partial sums the operator product writes per row: the column chunks of panel_kernel,
+// or the 256-row blocks of sym_panel_kernel (chunk 0).
+int operator_splits(int n, int chunk) {
+    return chunk > 0 ? (n + chunk - 1) / chunk : (n + SYM_T - 1) / SYM_T;
+}
+
+// part = the Ll X partials (operator_splits of them) on the stream.
+template <int N> void operator_partials(const pq_loops *L, OpShape sh, const double *X, double *part) {
+    const int n = L->nf;
+    if (sh.chunk > 0) {
+        hipLaunchKernelGGL(panel_kernel<N>, dim3((n + sh.rows - 1) / sh.rows, (n + sh.chunk - 1) / sh.chunk),
+                           dim3(sh.rows), 0, L->stream, L->Ll.as<float>(), L->ldl, n, n, X, sh.chunk, part);
+    } else {
+        if constexpr (N <= 16) {
+            const int nb = (n + SYM_T - 1) / SYM_T;
+            hipLaunchKernelGGL(sym_panel_kernel<N>, dim3(nb * (nb + 1) / 2), dim3(256), 0, L->stream,
+                               L->Ll.as<float>(), L->ldl, n, nb, X, part);
+        } else {
+            throw Error(PQ_EINVAL, "symmetric operator product: at most 16 real columns");
+        }
+    }
+}
+
+// PCBQS_AMD_OPERATOR pins the operator product's kernel: "panel" (1) or "symmetric" (2); unset
+// or anything else (0) lets the autotuner choose.
+int operator_mode() {
+    const char *v = std::getenv("PCBQS_AMD_OPERATOR");
+    if (!v) return 0;
+    if (!std::strcmp(v, "panel")) return 1;
+    if (!std::strcmp(v, "symmetric")) return 2;
+    return 0;
+}
+
+// W = (Rl + j w Ll) X with the Ll X partials in `part` (operator_splits(n, chunk) of them).
+template <int N> void operator_launch(const pq_loops *L, OpShape sh, const double *X, double *W, double *part,
+                                      const double *omega_dev) {
+    const int n = L->nf, nsplit = operator_splits(n, sh.chunk);
+    operator_partials<N>(L, sh, X, part);
+    if constexpr ((N & (N - 1)) == 0) {
+        hipLaunchKernelGGL(operator_epilogue_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                           L->rcol.as<int>(), L->rval.as<double>(), n, X, part, nsplit, omega_dev, W);
+    } else {
+        hipLaunchKernelGGL(spmv_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                           L->rcol.as<int>(), L->rval.as<double>(), n, X, W);
+        hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream, part,
+                           nsplit, n, omega_dev, W);
+    }
+}
+
+// The autotuner of the operator product. Candidates: the symmetric upper-triangle kernel (chunk 0,
+// up to 16 real columns) and panel_kernel over the whole matrix with 128 or 256 rows per block and
+// a column-chunk width; together they set the grid, the loads in flight and the partial sums the
+// epilogue reads. Each candidate runs once to warm up and three times under HIP events on an
+// input of zeros (the whole product: partials, resistive part and epilogue); the fastest is kept
 // for this handle and column count.
 void tune_gemv(pq_loops *L, int NR) {
     const int n = L->nf;
-    if (n < 1024 || L->gemv_chunk.count(NR)) return;
-    std::vector<int> cand;
-    for (int c : {128, 192, 256, 320, 448, 640})
-        if ((n + c - 1) / c <= 128 && c < n) cand.push_back(c);
+    const int mode = operator_mode();
+    if (L->gemv_chunk.count(NR) || n == 0) return;
+    // small systems keep the default shape unless the symmetric kernel is pinned
+    if (n < 1024 && !(mode == 2 && NR <= 16)) return;
+    std::vector<OpShape> cand;
+    if (NR <= 16 && mode != 1) cand.push_back({0, SYM_T});
+    if (mode != 2 || cand.empty())
+        for (int rows : {256, 128})
+            for (int c : {128, 192, 256, 320, 448, 640, 960})
+                if ((n + c - 1) / c <= 128 && c < n) cand.push_back({c, rows});
     if (cand.empty()) return;
     int maxsplit = 0;
-    for (int c : cand) maxsplit = std::max(maxsplit, (n + c - 1) / c);
-    DevBuf X, part;
+    for (auto c : cand) maxsplit = std::max(maxsplit, operator_splits(n, c.chunk));
+    DevBuf X, W, part, om;
     X.reserve((size_t)n * NR * 8);
+    W.reserve((size_t)n * NR * 8);
+    om.reserve(8);
     part.reserve((size_t)maxsplit * n * NR * 8);
     PQ_HIP(hipMemsetAsync(X.p, 0, (size_t)n * NR * 8, L->stream));
+    PQ_HIP(hipMemsetAsync(om.p, 0, 8, L->stream));
     hipEvent_t e0, e1;
     PQ_HIP(hipEventCreate(&e0));
     PQ_HIP(hipEventCreate(&e1));
-    int best = cand[0];
+    OpShape best = cand[0];
     float best_us = 1e30f;
-    for (int c : cand) {
-        const int nsplit = (n + c - 1) / c;
+    for (auto c : cand) {
         auto launch = [&]() {
             dispatch_nr(NR, [&](auto nr) {
                 constexpr int N = decltype(nr)::value;
-                hipLaunchKernelGGL(panel_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
-                                   L->stream, L->Ll.as<float>(), L->ldl, n, n, X.as<double>(), c, part.as<double>());
+                operator_launch<N>(L, c, X.as<double>(), W.as<double>(), part.as<double>(), om.as<double>());
             });
         };
         launch();
@@ -712,23 +1113,18 @@ void tune_gemv(pq_loops *L, int NR) {
     PQ_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    L->gemv_chunk[NR] = best;
+    L->gemv_chunk[NR] = best.chunk;
+    L->gemv_rows[NR] = best.rows;
     L->gemv_us[NR] = best_us;
 }
 
 void apply_operator(pq_loops *L, const double *X, double *W, int P) {
     const int NR = 2 * P, n = L->nf;
-    const int chunk = gemv_chunk(L, NR);
-    const int nsplit = (n + chunk - 1) / chunk;
-    L->gpart.reserve((size_t)nsplit * n * NR * 8);
+    const OpShape sh = gemv_shape(L, NR);
+    L->gpart.reserve((size_t)operator_splits(n, sh.chunk) * n * NR * 8);
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL(spmv_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
-                           L->rcol.as<int>(), L->rval.as<double>(), n, X, W);
-        hipLaunchKernelGGL(panel_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
-                           L->Ll.as<float>(), L->ldl, n, n, X, chunk, L->gpart.as<double>());
-        hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
-                           L->gpart.as<double>(), nsplit, n, L->omega_dev.as<double>(), W);
+        operator_launch<N>(L, sh, X, W, L->gpart.as<double>(), L->omega_dev.as<double>());
     });
     PQ_HIP(hipGetLastError());
 }
@@ -747,16 +1143,114 @@ int prolong_splits(int n, int k, int NR) {
     return std::max(1, s);
 }
 
+int restrict_rows(const pq_loops *L, int NR) {
+    auto it = L->crows.find(NR);
+    return it != L->crows.end() ? it->second : cr_rows(NR);
+}
+
+int prolong_split(const pq_loops *L, int NR) {
+    auto it = L->psplit.find(NR);
+    return it != L->psplit.end() ? it->second : prolong_splits(L->nf, L->kc, NR);
+}
+
+// Launch the restriction Phi^T X into `rows`-row chunk partials; returns the chunk count.
+template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k, const double *X, int rows,
+                                     double *part) {
+    const int n = L->nf, nchunk = (n + rows - 1) / rows;
+    hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream, Phi, k, X, n,
+                       rows, part);
+    return nchunk;
+}
+
+// Launch the prolongation of L->cvec in about `ps` column chunks; returns the chunk count.
+template <int N> int prolong_launch(const pq_loops *L, int ps, const double *omega_dev, double *PY, double *PA) {
+    const int n = L->nf, k = L->kc, pc = (k + ps - 1) / ps, nsplit = (k + pc - 1) / pc;
+    hipLaunchKernelGGL(prolong_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
+                       L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc,
+                       omega_dev, PY, PA);
+    return nsplit;
+}
+
+// The coarse kernels' autotuner (first use per column count and coarse space): rows per block of
+// the restriction with its chunk sum (more blocks against more partial sums) and column chunks of
+// the prolongation, each candidate run once to warm up and three times under HIP events on this
+// handle's bases with inputs of zeros; the fastest is kept. Before each timed run a 512 MiB fill
+// evicts the bases from the caches (the last-level cache holds 256 MB, as much as the bases of a
+// large coarse space), as the operator product does between two applications in a solve.
+void tune_coarse(pq_loops *L, int NR) {
+    const int n = L->nf, k = L->kc;
+    if (k == 0 || n < 1024 || L->crows.count(NR)) return;
+    const std::vector<int> rows_c = {128, 256, 512, 1024};
+    std::vector<int> split_c;
+    for (int ps : {4, 8, 12, 16, 24, 32})
+        if (ps <= (k + 15) / 16) split_c.push_back(ps);
+    if (split_c.empty()) split_c.push_back(1);
+    DevBuf X, part, PY, PA, om, flush, cv, dg;
+    constexpr size_t FLUSH = (size_t)512 << 20;
+    flush.reserve(FLUSH);
+    cv.reserve((size_t)k * NR * 8);
+    dg.reserve((size_t)k * 2 * 8);
+    PQ_HIP(hipMemsetAsync(dg.p, 0, (size_t)k * 2 * 8, L->stream));
+    X.reserve((size_t)n * NR * 8);
+    part.reserve((size_t)((n + 127) / 128) * k * NR * 8);
+    PY.reserve((size_t)split_c.back() * n * NR * 8);
+    PA.reserve((size_t)split_c.back() * n * NR * 8);
+    om.reserve(8);
+    L->cvec.reserve((size_t)k * NR * 8);
+    PQ_HIP(hipMemsetAsync(X.p, 0, (size_t)n * NR * 8, L->stream));
+    PQ_HIP(hipMemsetAsync(L->cvec.p, 0, (size_t)k * NR * 8, L->stream));
+    PQ_HIP(hipMemsetAsync(om.p, 0, 8, L->stream));
+    hipEvent_t e0, e1;
+    PQ_HIP(hipEventCreate(&e0));
+    PQ_HIP(hipEventCreate(&e1));
+    auto timed = [&](auto &&launch) {
+        launch();
+        float total = 0;
+        for (int r = 0; r < 3; ++r) {
+            PQ_HIP(hipMemsetAsync(flush.p, r, FLUSH, L->stream));
+            PQ_HIP(hipEventRecord(e0, L->stream));
+            launch();
+            PQ_HIP(hipEventRecord(e1, L->stream));
+            PQ_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            PQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            total += ms;
+        }
+        return total / 3.f;
+    };
+    int best_rows = rows_c[0], best_split = split_c[0];
+    float best = 1e30f;
+    dispatch_nr(NR, [&](auto nr) {
+        constexpr int N = decltype(nr)::value;
+        for (int rows : rows_c) {
+            const float t = timed([&] {
+                const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X.as<double>(), rows, part.as<double>());
+                hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * (N / 2) + 3) / 4), dim3(256), 0, L->stream,
+                                   part.as<double>(), nchunk, k, N / 2, dg.as<double>(), cv.as<double>());
+            });
+            if (t < best) { best = t; best_rows = rows; }
+        }
+        best = 1e30f;
+        for (int ps : split_c) {
+            const float t = timed([&] { prolong_launch<N>(L, ps, om.as<double>(), PY.as<double>(), PA.as<double>()); });
+            if (t < best) { best = t; best_split = ps; }
+        }
+    });
+    PQ_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    L->crows[NR] = best_rows;
+    L->psplit[NR] = best_split;
+}
+
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
 void coarse_correction(pq_loops *L, const double *X, int P) {
     const int n = L->nf, NR = 2 * P, k = L->kc;
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        const int nchunk = (n + cr_rows(N) - 1) / cr_rows(N);
-        hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
-                           L->Phi.as<float>(), k, X, n, L->cpart.as<double>());
+        const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X, restrict_rows(L, N), L->cpart.as<double>());
         if (L->cdiag) {
-            hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * P + 255) / 256), dim3(256), 0, L->stream,
+            hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
                                L->cpart.as<double>(), nchunk, k, P, L->Ainv.as<double>(), L->cvec.as<double>());
         } else {
             hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * N + 255) / 256), dim3(256), 0, L->stream,
@@ -764,19 +1258,14 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
             hipLaunchKernelGGL(coarse_solve_kernel, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
                                L->ctot.as<double>(), L->Ainv.as<double>(), k, P, L->cvec.as<double>());
         }
-        // prolongation: three panel products over the transposed bases, then the epilogue
-        const int ps = prolong_splits(n, k, N), pc = (k + ps - 1) / ps;
-        const size_t stride = (size_t)ps * n * N;
-        double *part = L->ppart.as<double>();
-        const dim3 grid((n + GEMV_ROWS - 1) / GEMV_ROWS, (k + pc - 1) / pc);
-        const float *bases[3] = {L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
-        for (int b = 0; b < 3; ++b)
-            hipLaunchKernelGGL(panel_kernel<N>, grid, dim3(256), 0, L->stream, bases[b], (long long)n, n, k,
-                               L->cvec.as<double>(), pc, part + b * stride);
-        hipLaunchKernelGGL(prolong_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream, part,
-                           part + stride, part + 2 * stride, (int)grid.y, n, L->omega_dev.as<double>(), X,
-                           L->T1.as<double>(),
-                           L->T2.as<double>());
+        // prolongation: the three bases in one pass, then the chunk sums
+        const size_t count = (size_t)n * N;
+        double *PY = L->ppart.as<double>();
+        const int ps = prolong_split(L, N);
+        double *PA = PY + (size_t)ps * count;
+        const int nsplit = prolong_launch<N>(L, ps, L->omega_dev.as<double>(), PY, PA);
+        hipLaunchKernelGGL(prolong_epilogue_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, PY,
+                           PA, nsplit, count, X, L->T1.as<double>(), L->T2.as<double>());
     });
     PQ_HIP(hipGetLastError());
 }
@@ -896,10 +1385,10 @@ void drop_graphs(pq_loops *L) {
 void check_graphs(pq_loops *L, int P, int precond, int m) {
     auto cv = [](long long v) { return reinterpret_cast<const void *>(static_cast<intptr_t>(v)); };
     std::vector<const void *> key = {cv(P), cv(precond), cv(m), cv(L->nf), cv(L->kc), cv(L->cdiag),
-                                     cv(gemv_chunk(L, 2 * P)), L->V.p, L->W.p, L->Z.p, L->partial.p, L->Hd.p,
+                                     cv(gemv_chunk(L, 2 * P)), cv(gemv_shape(L, 2 * P).rows), L->V.p, L->W.p, L->Z.p, L->partial.p, L->Hd.p,
                                      L->inv.p, L->T1.p, L->T2.p, L->cpart.p, L->cvec.p, L->ppart.p, L->gpart.p,
                                      L->ctot.p, L->Phi.p, L->Phit.p, L->RPhi.p, L->LPhi.p, L->Ainv.p,
-                                     L->omega_dev.p, L->Ll.p, L->rptr.p};
+                                     L->omega_dev.p, L->Ll.p, L->rptr.p, L->Hh};
     if (key != L->graph_key) {
         drop_graphs(L);
         L->graph_key = key;
@@ -916,10 +1405,11 @@ bool graphs_enabled() {
 void prepare_coarse(pq_loops *L, double omega, int P) {
     const int n = L->nf, NR = 2 * P;
     coarse_at(L, omega);
-    const int nchunk = (n + cr_rows(NR) - 1) / cr_rows(NR);
+    tune_coarse(L, NR);
+    const int rows = restrict_rows(L, NR), nchunk = (n + rows - 1) / rows;
     L->cpart.reserve((size_t)nchunk * L->kc * NR * 8);
     L->cvec.reserve((size_t)L->kc * P * 2 * 8);
-    L->ppart.reserve((size_t)3 * prolong_splits(n, L->kc, NR) * n * NR * 8);
+    L->ppart.reserve((size_t)2 * prolong_split(L, NR) * n * NR * 8);
     L->ctot.reserve((size_t)L->kc * NR * 8);
     L->T1.reserve((size_t)n * NR * 8); L->T2.reserve((size_t)n * NR * 8);
 }
@@ -937,7 +1427,8 @@ void port_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols) {
 }
 
 // h[i][p] for i < nv: conj(V_i) . W into device memory (partials, then their fixed-order sum).
-void dots_launch(pq_loops *L, const double *V, size_t vstride, int nv, const double *W, int P, double *h) {
+void dots_launch(pq_loops *L, const double *V, size_t vstride, int nv, const double *W, int P, double *h,
+                 double *h_host = nullptr) {
     const int nblk = std::min(RED_BLOCKS, (L->nf + 255) / 256);
     dispatch_nr(2 * P, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
@@ -945,7 +1436,7 @@ void dots_launch(pq_loops *L, const double *V, size_t vstride, int nv, const dou
                            L->nf, L->partial.as<double>());
     });
     hipLaunchKernelGGL(dots_reduce_kernel, dim3((nv * 2 * P + 3) / 4), dim3(256), 0, L->stream, L->partial.as<double>(),
-                       nblk, nv, 2 * P, h);
+                       nblk, nv, 2 * P, h, h_host);
     PQ_HIP(hipGetLastError());
 }
 
@@ -1083,8 +1574,8 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
         tune_gemv(L, 2 * P);
         set_omega(L, omega);
         {   // the operator product's partial sums, sized before any capture
-            const int chunk = gemv_chunk(L, 2 * P), nf = std::max(L->nf, 1);
-            L->gpart.reserve((size_t)((nf + chunk - 1) / chunk) * nf * 2 * P * 8);
+            const int nf = std::max(L->nf, 1);
+            L->gpart.reserve((size_t)operator_splits(nf, gemv_chunk(L, 2 * P)) * nf * 2 * P * 8);
         }
         const auto started = std::chrono::steady_clock::now();
         const int n = L->nf, NR = 2 * P, m = std::min(restart, std::max(n, 1));
@@ -1134,22 +1625,31 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 PQ_HIP(hipMemcpyAsync(R, B, vsz * 8, hipMemcpyDeviceToDevice, L->stream));
             }
             // Per Arnoldi step j the device writes, at Hd + j * hstride: the two Gram-Schmidt
-            // passes (m+1 coefficients each) and the new vector's squared norm; one asynchronous copy
-            // per step brings them to pinned host memory. The host works one step behind, so the
-            // device never waits for the Givens rotations.
+            // passes (m+1 coefficients each) and the new vector's squared norm; the kernels that
+            // produce them also store them at Hh + j * hstride (pinned, coherent host memory), read
+            // once the step's event has fired. The host works one step behind, so the device never
+            // waits for the Givens rotations.
             const size_t hstride = (size_t)(2 * (m + 1) + 1) * P * 2;
             L->Hd.reserve((size_t)m * hstride * 8);
-            if (L->Hh_bytes < (size_t)m * hstride * 8) {
+            const size_t hbytes = (size_t)m * hstride * 8;
+            if (L->Hh_bytes < hbytes) {
                 if (L->Hh) (void)hipHostFree(L->Hh);
                 L->Hh = nullptr;
-                PQ_HIP(hipHostMalloc((void **)&L->Hh, (size_t)m * hstride * 8, hipHostMallocDefault));
-                L->Hh_bytes = (size_t)m * hstride * 8;
+                PQ_HIP(hipHostMalloc((void **)&L->Hh, hbytes, hipHostMallocCoherent));
+                L->Hh_bytes = hbytes;
             }
-            std::vector<hipEvent_t> ev(m);
-            for (auto &e : ev) PQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            // The entries of step j's host copy that its kernels write: the first pass over j + 1
+            // vectors, the second over j + 2 (its last entry |w|^2), the squared norms.
+            const size_t pass = (size_t)(m + 1) * P * 2;
+            auto step_slots = [&](int j, auto &&f) {
+                volatile double *g = L->Hh + j * hstride;
+                for (size_t e = 0; e < (size_t)(j + 1) * P * 2; ++e) f(g[e]);
+                for (size_t e = 0; e < (size_t)(j + 2) * P * 2; ++e) f(g[pass + e]);
+                for (size_t e = 0; e < (size_t)P * 2; ++e) f(g[2 * pass + e]);
+            };
             double *Hd = L->Hd.as<double>();
-            auto dots_to = [&](const double *Vb, size_t vs, int nv, const double *Wv, double *hout) {
-                dots_launch(L, Vb, vs, nv, Wv, P, hout);
+            auto dots_to = [&](const double *Vb, size_t vs, int nv, const double *Wv, double *hout, double *hhost) {
+                dots_launch(L, Vb, vs, nv, Wv, P, hout, hhost);
             };
             auto update_dev = [&](const double *Vb, size_t vs, int nv, const double *hdev, double sign, double *Wv) {
                 const size_t count = (size_t)n * P * 4;
@@ -1158,15 +1658,18 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             };
             auto step_body = [&](int j) {
                 double *h1 = Hd + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2, *hn = h2 + (size_t)(m + 1) * P * 2;
+                // the host copies, written by the same kernels (pinned, coherent memory)
+                double *g1 = L->Hh + j * hstride, *g2 = g1 + (size_t)(m + 1) * P * 2, *gn = g2 + (size_t)(m + 1) * P * 2;
+                // the new vector is built in place in its slot V_{j+1}: CGS2, the second pass
+                // also giving |w|^2, and the last update normalising it
+                double *Wj = V + (j + 1) * vsz;
                 apply_precond(L, precond, V + j * vsz, Z, P);
-                apply_operator(L, Z, W, P);
-                dots_to(V, vsz, j + 1, W, h1);
-                update_dev(V, vsz, j + 1, h1, 1.0, W);
-                dots_to(V, vsz, j + 1, W, h2);
-                update_dev(V, vsz, j + 1, h2, 1.0, W);
-                dots_to(W, 0, 1, W, hn);
-                hipLaunchKernelGGL(vec_scale_norm_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, W,
-                                   hn, n, P, V + (j + 1) * vsz);
+                apply_operator(L, Z, Wj, P);
+                dots_to(V, vsz, j + 1, Wj, h1, g1);
+                update_dev(V, vsz, j + 1, h1, 1.0, Wj);
+                dots_to(V, vsz, j + 2, Wj, h2, g2);
+                hipLaunchKernelGGL(vec_update_norm_kernel, dim3((unsigned)(((size_t)n * P * 4 + 255) / 256)), dim3(256), 0,
+                                   L->stream, V, vsz, j + 1, h2, n, P, Wj, hn, gn);
                 PQ_HIP(hipGetLastError());
             };
             // Each step's kernels are captured once as a graph (step j's shapes and offsets are
@@ -1174,6 +1677,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             const bool use_graphs = L->graphs_on < 0 ? graphs_enabled() : L->graphs_on != 0;
             if (use_graphs) check_graphs(L, P, precond, m);
             auto enqueue_step = [&](int j) {
+                step_slots(j, [](volatile double &v) { v = std::numeric_limits<double>::quiet_NaN(); });
                 if (use_graphs) {
                     if (!L->graphs[j]) {
                         hipGraph_t graph = nullptr;
@@ -1195,9 +1699,24 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                     step_body(j);
                 }
                 ++L->matvecs;
-                PQ_HIP(hipMemcpyAsync(L->Hh + j * hstride, Hd + j * hstride, hstride * 8, hipMemcpyDeviceToHost,
-                                      L->stream));
-                PQ_HIP(hipEventRecord(ev[j], L->stream));
+            };
+            // Step j's coefficients reach the host as the kernels store them: each entry was set to
+            // NaN before the step was enqueued and every aligned 8-byte store lands whole, so the
+            // host waits until no entry is NaN; no event or fence orders the stores. Once the stream
+            // has gone idle, entries that are still NaN are the computed values (a diverged solve,
+            // caught by the residual test); a failed stream raises its error.
+            auto wait_step = [&](int j) {
+                bool idle = false;
+                for (unsigned long spins = 1;; ++spins) {
+                    bool ready = true;
+                    step_slots(j, [&](volatile double &v) { ready = ready && !std::isnan((double)v); });
+                    if (ready || idle) return;
+                    if ((spins & 255) != 0) continue;
+                    const hipError_t e = hipStreamQuery(L->stream);
+                    if (e == hipErrorNotReady) continue;
+                    PQ_HIP(e);
+                    idle = true;
+                }
             };
             int its = 0, restarts = 0;
             std::vector<double> res(P, 1.0);
@@ -1217,7 +1736,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 int used = 0;                      // Arnoldi steps whose rotations are applied
                 bool converged = false;
                 auto process = [&](int j) {
-                    PQ_HIP(hipEventSynchronize(ev[j]));
+                    wait_step(j);
                     const double *h1 = L->Hh + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2;
                     const double *hn = h2 + (size_t)(m + 1) * P * 2;
                     bool all = true;
@@ -1281,7 +1800,6 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 ++restarts;
             }
             PQ_HIP(hipStreamSynchronize(L->stream));
-            for (auto &e : ev) (void)hipEventDestroy(e);
             L->iterations = its; L->restarts = restarts;
             double worst = 0;
             for (double r : res) worst = std::max(worst, r);
@@ -1345,6 +1863,8 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         L->kc = k;
         L->cdiag = diagonal != 0;
         L->omega_c = -1;
+        L->crows.clear();
+        L->psplit.clear();
         if (k == 0) return;
         const int n = L->nf;
         L->Phi.reserve((size_t)n * k * 4); L->Phit.reserve((size_t)n * k * 4);
@@ -1359,8 +1879,8 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
                            L->Phi.as<float>(), n, k, L->Phit.as<float>());
         constexpr int CH = 16;
         tune_gemv(L, CH);
-        const int chunk = gemv_chunk(L, CH);
-        const int nsplit = (n + chunk - 1) / chunk;
+        const OpShape shape = gemv_shape(L, CH);
+        const int nsplit = operator_splits(n, shape.chunk);
         const int nchunk = (n + cr_rows(CH) - 1) / cr_rows(CH);
         DevBuf X, WR, WL, part, red, t;
         X.reserve((size_t)n * CH * 8); WR.reserve((size_t)n * CH * 8); WL.reserve((size_t)n * CH * 8);
@@ -1377,16 +1897,14 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
                                CH, X.as<double>());
             hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
                                L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
-            hipLaunchKernelGGL(panel_kernel<CH>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0,
-                               L->stream, L->Ll.as<float>(), L->ldl, n, n, X.as<double>(), chunk, part.as<double>());
+            operator_partials<CH>(L, shape, X.as<double>(), part.as<double>());
             hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
                                WL.as<double>());
             for (int which = 0; which < 2; ++which) {
                 const double *W = which ? WL.as<double>() : WR.as<double>();
                 hipLaunchKernelGGL(scatter_cols_kernel, dim3(vb), dim3(256), 0, L->stream, W, n, CH, c0, cn,
                                    (which ? L->LPhi : L->RPhi).as<float>());
-                hipLaunchKernelGGL(coarse_restrict_kernel<CH>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
-                                   L->Phi.as<float>(), k, W, n, red.as<double>());
+                restrict_launch<CH>(L, L->Phi.as<float>(), k, W, cr_rows(CH), red.as<double>());
                 hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * CH + 255) / 256), dim3(256), 0, L->stream,
                                    red.as<double>(), nchunk, k * CH, t.as<double>());
                 PQ_HIP(hipGetLastError());
@@ -1417,6 +1935,27 @@ int pq_loops_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols, dou
         if (L->nf == 0) return;
         port_rhs(L, omega, P, port_cols);
         PQ_HIP(hipMemcpyAsync(B_host, L->B.p, (size_t)L->nf * 2 * P * 8, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+    });
+}
+
+int pq_loops_operator(pq_loops *L, double omega, int P, const double *X_host, double *W_host) {
+    return guard([&] {
+        PQ_REQUIRE(L && X_host && W_host, "pq_loops_operator: null argument");
+        PQ_REQUIRE(P >= 1, "pq_loops_operator: P >= 1");
+        dispatch_nr(2 * P, [](auto) {});
+        DeviceScope scope(L->device);
+        const int n = L->nf;
+        if (n == 0) return;
+        tune_gemv(L, 2 * P);
+        set_omega(L, omega);
+        const size_t bytes = (size_t)n * 2 * P * 8;
+        DevBuf X, W;
+        X.reserve(bytes);
+        W.reserve(bytes);
+        PQ_HIP(hipMemcpyAsync(X.p, X_host, bytes, hipMemcpyHostToDevice, L->stream));
+        apply_operator(L, X.as<double>(), W.as<double>(), P);
+        PQ_HIP(hipMemcpyAsync(W_host, W.p, bytes, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
     });
 }

@@ -138,6 +138,14 @@ class PortSolver:
         call('pq_loops_rhs', self._handle, 2 * np.pi * float(frequency), P, ptr(self.port_cols), ptr(B))
         return self._join(B)
 
+    def operator(self, X, frequency):
+        """Z_ff X = (Rl + j w Ll) X on the free loops, X (n_free, P) complex with P <= 8 or 16."""
+        X = np.asarray(X, complex).reshape(self.model.n_free, -1)
+        Xs = self._split(X)
+        W = np.empty_like(Xs)
+        call('pq_loops_operator', self._handle, 2 * np.pi * float(frequency), X.shape[1], ptr(Xs), ptr(W))
+        return self._join(W)
+
     def coarse_solution(self, frequency):
         """The coarse-space solution Phi (Phi^T Z Phi)^-1 Phi^T b (n_free, P) complex of the port
         right-hand sides (needs a coarse space)."""
@@ -193,7 +201,8 @@ class PortSolver:
 
     def tuning(self, columns=None):
         """(chunk, microseconds) of the autotuned operator product for 2P real columns (P ports by
-        default): the chosen column-chunk width and its time per product."""
+        default): 0 for the symmetric upper-triangle kernel, else the whole-matrix kernel's
+        column-chunk width; and its time per product."""
         nr = 2 * self.device_ports if columns is None else int(columns)
         chunk, us = ctypes.c_int(), ctypes.c_double()
         call('pq_loops_tuning', self._handle, nr, ctypes.byref(chunk), ctypes.byref(us))

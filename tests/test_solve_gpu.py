@@ -58,10 +58,31 @@ def test_copper_reference_fixture_through_the_device_pipeline():
 
 
 @pytest.mark.parametrize('precond', ['block-jacobi', 'two-level'])
-def test_captured_steps_replay_the_eager_solve_bitwise(precond, monkeypatch):
+def test_symmetric_operator_solve_matches_the_fp64_oracle(precond, monkeypatch):
+    """The port solve with the operator product pinned to the symmetric upper-triangle kernel."""
+    from pcbqs_amd.solve import PortSolver
+    monkeypatch.setenv('PCBQS_AMD_OPERATOR', 'symmetric')
+    b = boards.board_c2(1, target=3000)
+    freqs = [1e3, 1e6, 1e8]
+    s = PortSolver(b, device=0)
+    if precond == 'two-level':
+        s.set_stream_coarse(grid=6)
+    Z, info = s.ports(freqs, precond=precond, tol=1e-8)
+    assert s.tuning()[0] == 0
+    Zo = oracle_ports(b, freqs)
+    w = 2 * np.pi * np.array(freqs)[:, None, None]
+    assert rel(Z.real, Zo.real) < 1e-6
+    assert rel(Z.imag / w, Zo.imag / w) < 1e-6
+    s.close()
+
+
+@pytest.mark.parametrize('operator', ['auto', 'symmetric'])
+@pytest.mark.parametrize('precond', ['block-jacobi', 'two-level'])
+def test_captured_steps_replay_the_eager_solve_bitwise(precond, operator, monkeypatch):
     """The Arnoldi steps captured as graphs and replayed (across restarts and frequencies, with the
     solver's eager work in between) give the eager launches' results bit for bit."""
     from pcbqs_amd.solve import PortSolver
+    monkeypatch.setenv('PCBQS_AMD_OPERATOR', operator)
     b = boards.board_c2(2, target=3000)
     freqs = [1e3, 1e6, 1e8, 1e6]
     out = {}

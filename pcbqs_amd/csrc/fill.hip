@@ -90,28 +90,42 @@ __device__ __forceinline__ void far64(const double *pa, const double *pb, double
 
 // The closed-form inner integrals (Wilton, Rao et al., IEEE TAP 1984) of 1/R and (r'-rho)/R over
 // source triangle q at observation point (x, y), height offset d (static_kernels.hpp:78-97).
+// Edge e runs from vertex e to vertex e + 1, so its end-point distances are the vertex distances
+// R_e and R_{e+1}: three square roots per point, not six (nor a seventh for r0, whose only use is
+// the guard, taken on r0^2).
 template <bool SCALAR>
 __device__ __forceinline__ void inner_integrals(const GeomDev &g, int q, double x, double y, double d,
                                                 double &I1, double &IRx, double &IRy) {
-    const double ad = fabs(d);
+    const double ad = fabs(d), d2 = d * d;
     I1 = 0; IRx = 0; IRy = 0;
     const double *Vq = g.V + 6 * (size_t)q, *Tq = g.tau + 6 * (size_t)q, *Nq = g.un + 6 * (size_t)q;
     const double *Lq = g.L + 3 * (size_t)q;
+    double vx[3], vy[3], R[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        vx[k] = Vq[2 * k] - x;
+        vy[k] = Vq[2 * k + 1] - y;
+        R[k] = sqrt(vx[k] * vx[k] + vy[k] * vy[k] + d2);
+    }
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        const double vx = Vq[2 * e] - x, vy = Vq[2 * e + 1] - y;
-        const double sm = vx * Tq[2 * e] + vy * Tq[2 * e + 1];
+        const int f = e == 2 ? 0 : e + 1;
+        const double sm = vx[e] * Tq[2 * e] + vy[e] * Tq[2 * e + 1];
         const double sp = sm + Lq[e];
-        const double t0 = vx * Nq[2 * e] + vy * Nq[2 * e + 1];
-        const double r02 = t0 * t0 + d * d, r0 = sqrt(r02);
-        const double rm = sqrt(r02 + sm * sm), rp = sqrt(r02 + sp * sp);
-        // asinh(s / r0) = log(A(s)) - log(r0) with A(s) = s + R, or r0^2 / (R - s) for s < 0 (the
-        // same number without cancellation): the difference of the two asinh is one log
+        const double t0 = vx[e] * Nq[2 * e] + vy[e] * Nq[2 * e + 1];
+        const double r02 = t0 * t0 + d2;
+        const double rm = R[e], rp = R[f];
+        // asinh(s / r0) = log(A(s)) - log(r0) with A(s) = |s| + R for s >= 0 and r0^2 / (|s| + R)
+        // for s < 0 (the same number without cancellation); sp > sm, so the difference of the two
+        // asinh is log(num / den) with one of three (num, den) pairs
         double lg = 0.0;
-        if (r0 > 1e-14 * Lq[e]) {
-            const double ap = sp >= 0 ? sp + rp : r02 / (rp - sp);
-            const double am = sm >= 0 ? sm + rm : r02 / (rm - sm);
-            lg = log(ap / am);
+        const double g0 = 1e-14 * Lq[e];
+        if (r02 > g0 * g0) {
+            const double bp = fabs(sp) + rp, bm = fabs(sm) + rm;
+            const bool pneg = sp < 0, mneg = sm < 0;
+            const double num = pneg ? bm : (mneg ? bp * bm : bp);
+            const double den = pneg ? bp : (mneg ? r02 : bm);
+            lg = log(num / den);
         }
         double term = t0 * lg;
         if (ad > 0) {

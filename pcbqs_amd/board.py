@@ -87,6 +87,15 @@ def _cut(nodes, tris, circles):
     return _compact(nodes, tris[keep])
 
 
+def _keep_rects(nodes, tris, rects):
+    """Keep the triangles whose centroid lies in any rectangle (x0, y0, x1, y1)."""
+    cen = nodes[tris].mean(1)
+    keep = np.zeros(len(tris), bool)
+    for x0, y0, x1, y1 in rects:
+        keep |= (cen[:, 0] >= x0) & (cen[:, 0] <= x1) & (cen[:, 1] >= y0) & (cen[:, 1] <= y1)
+    return _compact(nodes, tris[keep])
+
+
 class _Builder:
     """Stacks layers and resolves terminals, vias and ports into graph nodes."""
 
@@ -244,3 +253,59 @@ def board_batch(seed, triangles=(1000, 8000), layers=(2, 4), ports=(4, 16), gap=
         terms[f'p{k}+'], terms[f'p{k}-'] = (int(lb), box), (int(la), box)
         port_list.append((f'p{k}', f'p{k}+', f'p{k}-'))
     return b.build(terms, vias, port_list, config='batch', seed=seed, layers=nl)
+
+
+def board_c5(seed=0, size=120e-3, n_vias=200, ports=8, gap=0.2e-3, target=67000):
+    """Config 5: a seeded 4-layer PDN, size x size. The outer layers are planes with 8-32 circular
+    cutouts (r 0.5-3 mm); each inner layer is the union of 4-8 rectangular pours (20-60 mm). One grid
+    spacing serves every layer (about `target` triangles in all); n_vias through vias join every
+    layer with copper at their position; `ports` ports between two such layers."""
+    rng = np.random.default_rng(seed)
+    cuts = {lay: [(rng.uniform(0, size), rng.uniform(0, size), rng.uniform(0.5e-3, 3e-3))
+                  for _ in range(int(rng.integers(8, 33)))] for lay in (0, 3)}
+    pours = {}
+    for lay in (1, 2):
+        rects = []
+        for _ in range(int(rng.integers(4, 9))):
+            w, h = rng.uniform(20e-3, 60e-3, 2)
+            x, y = rng.uniform(0, size - w), rng.uniform(0, size - h)
+            rects.append((x, y, x + w, y + h))
+        pours[lay] = rects
+
+    def copper(lay, x, y, margin=0.0):
+        if lay in cuts:
+            return (margin <= x <= size - margin and margin <= y <= size - margin and
+                    all(np.hypot(x - cx, y - cy) > r + margin for cx, cy, r in cuts[lay]))
+        return any(x0 + margin <= x <= x1 - margin and y0 + margin <= y <= y1 - margin
+                   for x0, y0, x1, y1 in pours[lay])
+    probe = rng.uniform(0, size, (4000, 2))
+    cover = sum(np.mean([copper(lay, x, y) for x, y in probe]) for lay in range(4))
+    n = int(round(np.sqrt(target / (2 * cover))))
+    h = size / n
+    b = _Builder()
+    for lay in range(4):
+        nodes, tris = grid_sheet(0, 0, size, size, n, n)
+        if lay in cuts:
+            nodes, tris = _cut(nodes, tris, cuts[lay])
+        else:
+            nodes, tris = _keep_rects(nodes, tris, pours[lay])
+        b.add_layer(lay, nodes, tris, z=lay * gap)
+    vias = []
+    while len(vias) < n_vias:
+        x, y = rng.uniform(2 * h, size - 2 * h, 2)
+        on = tuple(lay for lay in range(4) if copper(lay, x, y, 2 * h))
+        if len(on) >= 2:
+            vias.append((x, y, 0.15e-3, on))
+    terms, port_list = {}, []
+    while len(port_list) < ports:
+        x, y = rng.uniform(3 * h, size - 3 * h, 2)
+        on = [lay for lay in range(4) if copper(lay, x, y, 3 * h)]
+        if len(on) < 2:
+            continue
+        la, lb = sorted(rng.choice(on, 2, replace=False))
+        k = len(port_list)
+        box = (x - 0.75 * h, y - 0.75 * h, x + 0.75 * h, y + 0.75 * h)
+        terms[f'p{k}+'], terms[f'p{k}-'] = (int(lb), box), (int(la), box)
+        port_list.append((f'p{k}', f'p{k}+', f'p{k}-'))
+    return b.build(terms, vias, port_list, config='c5', seed=seed, cell=h, gap=gap, layers=4,
+                   pours={k: len(v) for k, v in pours.items()})

@@ -703,13 +703,14 @@ __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n,
 }
 
 // Restriction partials: partial[chunk][c][r] = sum over the chunk's rows i of Phi[i][c] V[i][r],
-// Phi (n, k) FP32 row-major, V (n, NR) FP64. A block covers 64 coarse columns (the lanes, so each
+// Phi (n, k) FP32 row-major with row stride kp (a multiple of 64 floats, so that every 64-column
+// segment is two whole cache lines), V (n, NR) FP64. A block covers 64 coarse columns (the lanes, so each
 // row segment of Phi is one coalesced 256-byte read, sixteen in flight per lane) and `rows` rows
 // split over its four waves; a row of V is wave-uniform (scalar loads). The four waves' sums are
 // added in wave order.
 constexpr int cr_rows(int NR) { return NR >= 8 ? 4096 / NR : 512; }
 template <int NR>
-__global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__restrict__ Phi, int k,
+__global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__restrict__ Phi, int k, int kp,
                                                              const double *__restrict__ V, int n, int rows,
                                                              double *__restrict__ partial) {
     constexpr int U = 16;
@@ -720,13 +721,13 @@ __global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__res
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
-    const float *col = Phi + (size_t)i0 * k + min(c, k - 1);
+    const float *col = Phi + (size_t)i0 * kp + min(c, k - 1);
     const double *v = V + (size_t)i0 * NR;
     int i = w;
     for (; i + 4 * (U - 1) < m; i += 4 * U) {
         float f[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) f[u] = col[(size_t)(i + 4 * u) * k];
+        for (int u = 0; u < U; ++u) f[u] = col[(size_t)(i + 4 * u) * kp];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const double fu = f[u];
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__res
         }
     }
     for (; i < m; i += 4) {
-        const double f = col[(size_t)i * k];
+        const double f = col[(size_t)i * kp];
         const double *vr = v + (size_t)i * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] += f * vr[r];
@@ -810,15 +811,15 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
 
 // The prolongation: the three transposed bases (k, n) FP32 Phi^T, (Rl Phi)^T, (Ll Phi)^T read in
 // one pass against the coarse vector c (k, NR), thread per row, a chunk of coarse columns per
-// blockIdx.y, eight columns of each basis (24 loads) in flight; c wave-uniform (scalar loads).
+// blockIdx.y, U columns of each basis (3 U loads) in flight; c wave-uniform (scalar loads).
 // Chunk partials: PY = Phi c and PA = (Rl + j w Ll) Phi c (real parts in columns 0..P-1).
-template <int NR>
+template <int NR, int U>
 __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ B0, const float *__restrict__ B1,
                                                      const float *__restrict__ B2, int n, int k,
                                                      const double *__restrict__ C, int chunk,
                                                      const double *__restrict__ omega_dev, double *__restrict__ PY,
                                                      double *__restrict__ PA) {
-    constexpr int P = NR / 2, U = 8;
+    constexpr int P = NR / 2;
     const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(k, c0 + chunk);
     double ay[NR], ar[NR], al[NR];
@@ -893,12 +894,12 @@ __global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const dou
     Y[e] = y;
 }
 
-// Out (k, n) = In (n, k)^T, FP32.
-__global__ void transpose_kernel(const float *In, int n, int k, float *Out) {
+// Out (k, n) = In (n, k)^T, FP32; In has row stride ld >= k.
+__global__ void transpose_kernel(const float *In, int n, int k, int ld, float *Out) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * k) return;
     const size_t i = e / k, c = e % k;
-    Out[c * n + i] = In[e];
+    Out[c * n + i] = In[i * ld + c];
 }
 
 // W = sum of the nsplit column-chunk partials of panel_kernel (a real product, no frequency).
@@ -912,13 +913,13 @@ __global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W
     W[e] = s;
 }
 
-// X[i][c] = Phi[i][c0 + c] for c < cn, 0 for cn <= c < ld (FP32 -> FP64).
-__global__ void gather_cols_kernel(const float *Phi, int n, int k, int c0, int cn, int ld, double *X) {
+// X[i][c] = Phi[i][c0 + c] for c < cn, 0 for cn <= c < ld (FP32 -> FP64); Phi has row stride kp.
+__global__ void gather_cols_kernel(const float *Phi, int n, int kp, int c0, int cn, int ld, double *X) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * ld) return;
     const size_t i = e / ld;
     const int c = (int)(e % ld);
-    X[e] = c < cn ? (double)Phi[i * k + c0 + c] : 0.0;
+    X[e] = c < cn ? (double)Phi[i * kp + c0 + c] : 0.0;
 }
 
 // Out[c0 + c][i] = W[i][c] for c < cn (FP64 (n, ld) -> FP32 transposed (k, n)).
@@ -952,7 +953,7 @@ struct pq_loops {
     int nblk = 0;
     double omega_bj = -1;
     // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
-    int kc = 0;
+    int kc = 0, kp = 0;           // coarse columns; row stride of Phi
     bool cdiag = false;           // basis diagonalises the Galerkin matrices
     DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart, ppart;   // RPhi, LPhi transposed (k, n)
     std::vector<double> Rc, Lc;
@@ -972,8 +973,9 @@ struct pq_loops {
     // autotuned column-chunk width of the operator product per column count (first use per NR)
     std::map<int, int> gemv_chunk, gemv_rows;
     std::map<int, float> gemv_us;
-    // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks)
-    std::map<int, int> crows, psplit;
+    // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks
+    // and loads in flight)
+    std::map<int, int> crows, psplit, pdepth;
 };
 
 namespace {
@@ -1153,21 +1155,32 @@ int prolong_split(const pq_loops *L, int NR) {
     return it != L->psplit.end() ? it->second : prolong_splits(L->nf, L->kc, NR);
 }
 
+int prolong_depth(const pq_loops *L, int NR) {
+    auto it = L->pdepth.find(NR);
+    return it != L->pdepth.end() ? it->second : 8;
+}
+
 // Launch the restriction Phi^T X into `rows`-row chunk partials; returns the chunk count.
 template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k, const double *X, int rows,
                                      double *part) {
     const int n = L->nf, nchunk = (n + rows - 1) / rows;
-    hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream, Phi, k, X, n,
-                       rows, part);
+    hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream, Phi, k, L->kp,
+                       X, n, rows, part);
     return nchunk;
 }
 
-// Launch the prolongation of L->cvec in about `ps` column chunks; returns the chunk count.
-template <int N> int prolong_launch(const pq_loops *L, int ps, const double *omega_dev, double *PY, double *PA) {
+// Launch the prolongation of L->cvec in about `ps` column chunks with `depth` (8 or 16) columns of
+// each basis in flight; returns the chunk count.
+template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, const double *omega_dev, double *PY,
+                                    double *PA) {
     const int n = L->nf, k = L->kc, pc = (k + ps - 1) / ps, nsplit = (k + pc - 1) / pc;
-    hipLaunchKernelGGL(prolong_kernel<N>, dim3((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit), dim3(256), 0, L->stream,
-                       L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc,
-                       omega_dev, PY, PA);
+    const dim3 grid((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit);
+    if (depth == 16)
+        hipLaunchKernelGGL((prolong_kernel<N, 16>), grid, dim3(256), 0, L->stream, L->Phit.as<float>(),
+                           L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
+    else
+        hipLaunchKernelGGL((prolong_kernel<N, 8>), grid, dim3(256), 0, L->stream, L->Phit.as<float>(),
+                           L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
     return nsplit;
 }
 
@@ -1218,7 +1231,7 @@ void tune_coarse(pq_loops *L, int NR) {
         }
         return total / 3.f;
     };
-    int best_rows = rows_c[0], best_split = split_c[0];
+    int best_rows = rows_c[0], best_split = split_c[0], best_depth = 8;
     float best = 1e30f;
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
@@ -1231,16 +1244,20 @@ void tune_coarse(pq_loops *L, int NR) {
             if (t < best) { best = t; best_rows = rows; }
         }
         best = 1e30f;
-        for (int ps : split_c) {
-            const float t = timed([&] { prolong_launch<N>(L, ps, om.as<double>(), PY.as<double>(), PA.as<double>()); });
-            if (t < best) { best = t; best_split = ps; }
-        }
+        for (int depth : {8, 16})
+            for (int ps : split_c) {
+                const float t = timed([&] {
+                    prolong_launch<N>(L, ps, depth, om.as<double>(), PY.as<double>(), PA.as<double>());
+                });
+                if (t < best) { best = t; best_split = ps; best_depth = depth; }
+            }
     });
     PQ_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     L->crows[NR] = best_rows;
     L->psplit[NR] = best_split;
+    L->pdepth[NR] = best_depth;
 }
 
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
@@ -1263,7 +1280,7 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
         double *PY = L->ppart.as<double>();
         const int ps = prolong_split(L, N);
         double *PA = PY + (size_t)ps * count;
-        const int nsplit = prolong_launch<N>(L, ps, L->omega_dev.as<double>(), PY, PA);
+        const int nsplit = prolong_launch<N>(L, ps, prolong_depth(L, N), L->omega_dev.as<double>(), PY, PA);
         hipLaunchKernelGGL(prolong_epilogue_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, PY,
                            PA, nsplit, count, X, L->T1.as<double>(), L->T2.as<double>());
     });
@@ -1865,18 +1882,22 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         L->omega_c = -1;
         L->crows.clear();
         L->psplit.clear();
+        L->pdepth.clear();
         if (k == 0) return;
         const int n = L->nf;
-        L->Phi.reserve((size_t)n * k * 4); L->Phit.reserve((size_t)n * k * 4);
+        L->kp = (k + 63) / 64 * 64;
+        const int kp = L->kp;
+        L->Phi.reserve((size_t)n * kp * 4); L->Phit.reserve((size_t)n * k * 4);
         L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
         {
-            std::vector<float> f32((size_t)n * k);
-            for (size_t e = 0; e < f32.size(); ++e) f32[e] = (float)Phi[e];
+            std::vector<float> f32((size_t)n * kp, 0.0f);
+            for (size_t i = 0; i < (size_t)n; ++i)
+                for (int c = 0; c < k; ++c) f32[i * kp + c] = (float)Phi[i * k + c];
             PQ_HIP(hipMemcpyAsync(L->Phi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice, L->stream));
             PQ_HIP(hipStreamSynchronize(L->stream));
         }
         hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)(((size_t)n * k + 255) / 256)), dim3(256), 0, L->stream,
-                           L->Phi.as<float>(), n, k, L->Phit.as<float>());
+                           L->Phi.as<float>(), n, k, kp, L->Phit.as<float>());
         constexpr int CH = 16;
         tune_gemv(L, CH);
         const OpShape shape = gemv_shape(L, CH);
@@ -1893,7 +1914,7 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         const unsigned vb = (unsigned)(((size_t)n * CH + 255) / 256);
         for (int c0 = 0; c0 < k; c0 += CH) {
             const int cn = std::min(CH, k - c0);
-            hipLaunchKernelGGL(gather_cols_kernel, dim3(vb), dim3(256), 0, L->stream, L->Phi.as<float>(), n, k, c0, cn,
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(vb), dim3(256), 0, L->stream, L->Phi.as<float>(), n, kp, c0, cn,
                                CH, X.as<double>());
             hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
                                L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());

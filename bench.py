@@ -203,6 +203,12 @@ def main():
         near_ms.append(nm)
     seconds = (time.perf_counter() - t0) / args.steps
     far = float(np.mean(far_ms)) * 1e-3
+    # the store ceiling of the same output pattern (the four planes' 64 x 64 tiles, no arithmetic)
+    import ctypes
+    from pcbqs_amd._lib import call
+    probe_ms = ctypes.c_double()
+    call('pq_store_probe', planes.ptr, n, planes.shape[-1], 4, ctypes.byref(probe_ms))
+    store_probe = 16.0 * n * n / (probe_ms.value * 1e-3)
     # FP64 reference-layout fill (the parity mode), once, for the record
     planes.close()
     p64, _ = backend.fill_device(3.0, 1, False, 64)
@@ -222,7 +228,8 @@ def main():
                       frac=achieved / HBM_PEAK, traffic=measured_traffic(args.seed),
                       algorithmic_bytes=pairs * B_FULL_FP32,
                       kernel='far32_tile_kernel (FP32 far pairs, full moments, both reciprocal entries)',
-                      valu_frac=pairs * F_FULL / far / FP32_VALU_PEAK),
+                      valu_frac=pairs * F_FULL / far / FP32_VALU_PEAK,
+                      store_probe_GBs=store_probe / 1e9, frac_of_store_probe=achieved / store_probe),
         kernels=dict(far_ms=float(np.mean(far_ms)), near_ms=float(np.mean(near_ms)),
                      near_pairs=list(near_pairs), fp64_far_ms=f64, fp64_near_ms=n64,
                      fp64_gpair_s=pairs / ((f64 + n64) * 1e-3) / 1e9),

@@ -70,6 +70,12 @@ int pq_synchronize(int device);
 int pq_stream_create(int device, void **stream);
 int pq_stream_destroy(void *stream);
 int pq_stream_synchronize(void *stream);
+/* The store ceiling of the fill's output pattern: nplanes (1..4) FP32 planes of
+ * n x n at leading dimension ld in planes_dev, each 64 x 64 tile written as
+ * 256-byte row segments with non-temporal stores (value r - c), as the FP32
+ * fill stores them, without the pair arithmetic; *ms is the mean of three
+ * timed launches. Measurement only: no counterpart in the reference. */
+int pq_store_probe(void *planes_dev, int64_t n, int64_t ld, int nplanes, double *ms);
 
 /* ---- fill --------------------------------------------------------------- */
 

@@ -50,6 +50,7 @@ _SIGNATURES = {
     'pq_stream_create': ([ctypes.c_int, P], ctypes.c_int),
     'pq_stream_destroy': ([P], ctypes.c_int),
     'pq_stream_synchronize': ([P], ctypes.c_int),
+    'pq_store_probe': ([P, I64, I64, ctypes.c_int, P], ctypes.c_int),
     'pq_fill_create': ([P, ctypes.c_int, P], ctypes.c_int),
     'pq_fill_destroy': ([P], ctypes.c_int),
     'pq_fill_symmetric': ([P, DBL, ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),

@@ -8,6 +8,20 @@ void set_error(const char *msg) { last_error = msg ? msg : ""; }
 
 using namespace pq;
 
+// The store pattern of the FP32 fill without its arithmetic: one wave per 64 x 64 tile writes its
+// 64 rows as 64-float (256 B) segments of every plane, non-temporally, value r - c at (r, c). Timed
+// by pq_store_probe as the store ceiling beside the fill's roofline.
+__global__ __launch_bounds__(64) void store_probe_kernel(float *out, long long n, long long ld, size_t plane,
+                                                        int nplanes) {
+    const long long c = (long long)blockIdx.x * 64 + threadIdx.x;
+    const long long r0 = (long long)blockIdx.y * 64, r1 = r0 + 64 < n ? r0 + 64 : n;
+    if (c >= n) return;
+    for (long long r = r0; r < r1; ++r) {
+        const float v = (float)(r - c);
+        for (int p = 0; p < nplanes; ++p) __builtin_nontemporal_store(v, out + p * plane + r * ld + c);
+    }
+}
+
 extern "C" {
 
 const char *pq_last_error(void) { return last_error.c_str(); }
@@ -79,6 +93,33 @@ int pq_stream_destroy(void *stream) {
 
 int pq_stream_synchronize(void *stream) {
     return guard([&] { PQ_HIP(hipStreamSynchronize((hipStream_t)stream)); });
+}
+
+int pq_store_probe(void *planes, int64_t n, int64_t ld, int nplanes, double *ms) {
+    return guard([&] {
+        PQ_REQUIRE(planes && ms, "pq_store_probe: null argument");
+        PQ_REQUIRE(n > 0 && ld >= n && nplanes >= 1 && nplanes <= 4, "pq_store_probe: invalid sizes");
+        PQ_REQUIRE((n + 63) / 64 <= 65535, "pq_store_probe: more than 65535 row tiles");
+        const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
+        const size_t plane = (size_t)n * ld;
+        hipEvent_t e0, e1;
+        PQ_HIP(hipEventCreate(&e0));
+        PQ_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(store_probe_kernel, grid, dim3(64), 0, 0, (float *)planes, (long long)n, (long long)ld,
+                           plane, nplanes);
+        PQ_HIP(hipEventRecord(e0, 0));
+        for (int r = 0; r < 3; ++r)
+            hipLaunchKernelGGL(store_probe_kernel, grid, dim3(64), 0, 0, (float *)planes, (long long)n, (long long)ld,
+                               plane, nplanes);
+        PQ_HIP(hipEventRecord(e1, 0));
+        PQ_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        PQ_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        PQ_HIP(hipGetLastError());
+        *ms = t / 3.0;
+    });
 }
 
 int pq_synchronize(int device) {

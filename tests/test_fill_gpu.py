@@ -144,3 +144,21 @@ def test_edge_inductance_from_host_moments(mesh):
     ref = F.cr_combine(M.RWG(nodes, mesh[1]), rr, mx, my, m0, 1e-7, 1 / r.l)
     assert L == pytest.approx(ref, rel=1e-12, abs=1e-12 * np.abs(ref).max())
     assert np.linalg.eigvalsh(L).min() > 0
+
+
+def test_store_probe_writes_its_pattern():
+    """The store-ceiling probe (bench.py's reference beside the fill's roofline) writes r - c into
+    every (r, c < n) entry of each plane and reports a positive time."""
+    import ctypes
+    from pcbqs_amd._lib import DeviceArray, call
+    n, ld, planes = 200, 256, 3
+    out = DeviceArray((planes, n, ld), np.float32).zero()
+    ms = ctypes.c_double()
+    call('pq_store_probe', out.ptr, n, ld, planes, ctypes.byref(ms))
+    host = out.host()
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing='ij')
+    for p in range(planes):
+        np.testing.assert_array_equal(host[p, :, :n], (r - c).astype(np.float32))
+        assert not host[p, :, n:].any()
+    assert ms.value > 0
+    out.close()

@@ -35,13 +35,20 @@ def boundary_vertices(tris, n_nodes):
 
 
 def stream_patterns(model, grid=8):
-    """(nb, k) branch currents of coarse stream-function hats on every layer (vias carry none).
-    The stream functions vanish on boundary vertices: no current leaves the copper."""
+    """(nb, k) branch currents of coarse stream-function hats on every layer (vias carry none), a
+    scipy CSC matrix. The stream functions vanish on boundary vertices: no current leaves the
+    copper. Columns run over the layers, then i, then j of the hat at grid point (i, j), without the
+    dropped hat (i = j = grid) and without hats that carry no current. A hat's support is the four
+    grid cells around its point, so every node is evaluated for the 4 x 4 nearest hats only (each
+    hat value from the same expression as a dense evaluation, zeros and rounding-level values at
+    cell edges included)."""
+    from scipy import sparse
     b, r = model.board, model.rwg
     s = edge_signs(r)
     u, v = r.ends[:, 0], r.ends[:, 1]
     interior = ~boundary_vertices(b.tris, len(b.nodes))
-    cols = []
+    rows, cols, vals = [], [], []
+    base = 0
     for lay in np.unique(b.layer_of):
         lt = np.flatnonzero(b.layer_of == lay)
         ln = np.unique(b.tris[lt])
@@ -51,26 +58,56 @@ def stream_patterns(model, grid=8):
         hx, hy = max(gx[1] - gx[0], 1e-30), max(gy[1] - gy[0], 1e-30)
         on = np.zeros(len(b.nodes), bool)
         on[ln] = True
-        edges_on = on[u] & on[v]
+        edges = np.flatnonzero(on[u] & on[v])
         on &= interior
-        count = 0
-        for i in range(grid + 1):
-            for j in range(grid + 1):
-                if i == grid and j == grid:
-                    continue                                    # the dropped hat
-                psi = (np.maximum(0, 1 - np.abs(b.nodes[:, 0] - gx[i]) / hx) *
-                       np.maximum(0, 1 - np.abs(b.nodes[:, 1] - gy[j]) / hy) * on)
-                ib = np.zeros(model.nb)
-                ib[:model.ne] = np.where(edges_on, s * (psi[v] - psi[u]), 0.0)
-                if np.abs(ib).max() > 0:
-                    cols.append(ib)
-                    count += 1
-    return np.array(cols).T if cols else np.zeros((model.nb, 0))
+        # psi of the 4 x 4 hats nearest each node, for the nodes of this layer's edges
+        nodes = np.unique(np.r_[u[edges], v[edges]])
+        x, y = b.nodes[nodes, 0], b.nodes[nodes, 1]
+        i0 = np.clip(np.floor((x - x0) / hx).astype(np.int64), 0, grid) - 1
+        j0 = np.clip(np.floor((y - y0) / hy).astype(np.int64), 0, grid) - 1
+        di, dj = np.meshgrid(np.arange(4), np.arange(4), indexing='ij')
+        ii = i0[:, None] + di.ravel()[None]                       # (nodes, 16)
+        jj = j0[:, None] + dj.ravel()[None]
+        ok = (ii >= 0) & (ii <= grid) & (jj >= 0) & (jj <= grid) & ~((ii == grid) & (jj == grid))
+        ic, jc = np.clip(ii, 0, grid), np.clip(jj, 0, grid)
+        psi = (np.maximum(0, 1 - np.abs(x[:, None] - gx[ic]) / hx) *
+               np.maximum(0, 1 - np.abs(y[:, None] - gy[jc]) / hy) * on[nodes][:, None])
+        psi = np.where(ok, psi, 0.0)
+        hat = ic * (grid + 1) + jc                                # hat index in this layer
+        slot = np.full(len(b.nodes), -1)
+        slot[nodes] = np.arange(len(nodes))
+        # s (psi(v) - psi(u)) for every hat near either end: (edges, 32) entries, duplicates summed
+        su, sv = slot[u[edges]], slot[v[edges]]
+        e_rows = np.repeat(edges, 32)
+        e_hats = np.c_[hat[sv], hat[su]].ravel()
+        e_vals = (s[edges][:, None] * np.c_[psi[sv], -psi[su]]).ravel()
+        keep = e_vals != 0
+        rows.append(e_rows[keep])
+        cols.append(base + e_hats[keep])
+        vals.append(e_vals[keep])
+        base += (grid + 1) ** 2
+    if not rows:
+        return sparse.csc_matrix((model.nb, 0))
+    m = sparse.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(model.nb, base)).tocsc()
+    m.sum_duplicates()
+    m.eliminate_zeros()
+    used = np.flatnonzero(np.diff(m.indptr) > 0)
+    return m[:, used]
 
 
 def loop_restrict(model, patterns):
-    """Branch patterns (nb, k) -> free-loop columns (n_free, k), each normalised to unit 2-norm."""
+    """Branch patterns (nb, k), dense or sparse -> free-loop columns (n_free, k), each normalised to
+    unit 2-norm."""
     cot = model.basis.cotree[:model.n_free]
+    if hasattr(patterns, 'tocsr'):
+        sel = patterns.tocsr()[cot].tocsc()
+        norm = np.sqrt(np.asarray(sel.multiply(sel).sum(0)).ravel())
+        keep = np.flatnonzero(norm > 0)
+        sel = sel[:, keep]
+        # scale each stored entry by its column's norm, then densify once
+        sel.data = sel.data / np.repeat(norm[keep], np.diff(sel.indptr))
+        return np.ascontiguousarray(sel.toarray(), np.float64)
     phi = np.ascontiguousarray(patterns[cot], np.float64)
     norm = np.linalg.norm(phi, axis=0)
     keep = norm > 0

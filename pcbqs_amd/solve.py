@@ -104,7 +104,8 @@ class PortSolver:
         from .coarse import loop_restrict, stream_patterns, via_loop_columns
         patterns = stream_patterns(self.model, grid)
         if extra is not None and extra.shape[1]:
-            patterns = np.hstack([patterns, extra])
+            from scipy import sparse
+            patterns = sparse.hstack([patterns, sparse.csc_matrix(extra)]).tocsc()
         cols = [loop_restrict(self.model, patterns)]
         if vias:
             cols.append(via_loop_columns(self.model))

@@ -68,7 +68,7 @@ def cpu_baseline(board):
                 seconds=seconds)
 
 
-def gmres_section(board, freq=1e6, grid=24, repeats=3, cpu=True):
+def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
     """Port solve at one frequency on the same board: plain block-Jacobi GMRES against two-level
     GMRES whose coarse space holds the grid x grid hats and the learned enrichment. Seconds are
     the solver's own wall clock per solve (median of the repeats; block-Jacobi's per-frequency

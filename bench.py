@@ -92,7 +92,7 @@ def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
     Zl, learned = run('two-level')
     section = dict(
         frequency_Hz=freq, ports=solver.n_ports, free_loops=solver.model.n_free, tol=1e-8,
-        plain=dict(preconditioner='block-jacobi (64-loop blocks)', seconds=plain['seconds'],
+        plain=dict(preconditioner='block-jacobi (128-loop blocks)', seconds=plain['seconds'],
                    iterations=plain['iterations'], relative_residual=plain['relative_residual']),
         learned=dict(preconditioner=f'two-level: {grid}x{grid} hats + via loops + learned enrichment',
                      seconds=learned['seconds'], iterations=learned['iterations'],

@@ -387,8 +387,10 @@ __global__ __launch_bounds__(256, 2) void sym_panel_kernel(const float *__restri
 
 // Inverse of each diagonal block Z_bb = Rl_bb + j w Ll_bb (size <= BS) by in-place Gauss-Jordan
 // with partial pivoting in LDS, complex FP32 (a preconditioner); stored as (nblk, BS, BS)
-// complex (re, im interleaved). A padded block is the identity.
-constexpr int BS = 64;
+// complex (re, im interleaved). A padded block is the identity. 128 loops per block (the block's
+// two FP32 planes fill 132 KB of the 160 KB LDS): against 64-loop blocks the plain solve of the
+// 30k-RWG board takes 421 iterations instead of 644, the two-level one 64 instead of 86.
+constexpr int BS = 128;
 
 __global__ __launch_bounds__(256) void bj_setup_kernel(const float *Ll, long long ldl, const int *ptr,
                                                       const int *col, const double *val, int n, double omega,
@@ -479,18 +481,22 @@ __global__ __launch_bounds__(256) void bj_setup_kernel(const float *Ll, long lon
     }
 }
 
-// Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P.
+// Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P. A workgroup
+// takes bj_rows(NR) rows of block blockIdx.x (from row blockIdx.y * bj_rows), one thread per
+// (row, port) output, so a block is spread over several workgroups.
+constexpr int bj_rows(int NR) { return 512 / NR < BS ? 512 / NR : BS; }
 template <int NR>
 __global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, const double *X, const double *A,
                                                       double *Y) {
-    constexpr int P = NR / 2;
+    constexpr int P = NR / 2, R = bj_rows(NR);
     __shared__ double xs[BS * NR];
     const int b = blockIdx.x, r0 = b * BS, m = min(BS, n - r0);
     for (int e = threadIdx.x; e < BS * NR; e += blockDim.x) xs[e] = (e / NR) < m ? X[(size_t)r0 * NR + e] : 0.0;
     __syncthreads();
     const float *Zi = inv + (size_t)b * BS * BS * 2;
-    for (int e = threadIdx.x; e < m * P; e += blockDim.x) {
-        const int i = e / P, p = e % P;
+    const int i0 = blockIdx.y * R, iend = min(m, i0 + R);
+    for (int e = threadIdx.x; e < (iend - i0) * P; e += blockDim.x) {
+        const int i = i0 + e / P, p = e % P;
         double yr = 0, yi = 0;
         for (int j = 0; j < m; ++j) {
             const double zr = (double)Zi[2 * (i * BS + j)], zi = (double)Zi[2 * (i * BS + j) + 1];
@@ -820,7 +826,7 @@ __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ 
                                                      const double *__restrict__ omega_dev, double *__restrict__ PY,
                                                      double *__restrict__ PA) {
     constexpr int P = NR / 2;
-    const int row = blockIdx.x * GEMV_ROWS + threadIdx.x;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(k, c0 + chunk);
     double ay[NR], ar[NR], al[NR];
 #pragma unroll
@@ -975,7 +981,7 @@ struct pq_loops {
     std::map<int, float> gemv_us;
     // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks
     // and loads in flight)
-    std::map<int, int> crows, psplit, pdepth;
+    std::map<int, int> crows, psplit, pdepth, prows;
 };
 
 namespace {
@@ -1160,6 +1166,11 @@ int prolong_depth(const pq_loops *L, int NR) {
     return it != L->pdepth.end() ? it->second : 8;
 }
 
+int prolong_rows(const pq_loops *L, int NR) {
+    auto it = L->prows.find(NR);
+    return it != L->prows.end() ? it->second : GEMV_ROWS;
+}
+
 // Launch the restriction Phi^T X into `rows`-row chunk partials; returns the chunk count.
 template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k, const double *X, int rows,
                                      double *part) {
@@ -1169,17 +1180,17 @@ template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k,
     return nchunk;
 }
 
-// Launch the prolongation of L->cvec in about `ps` column chunks with `depth` (8 or 16) columns of
-// each basis in flight; returns the chunk count.
-template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, const double *omega_dev, double *PY,
-                                    double *PA) {
+// Launch the prolongation of L->cvec in about `ps` column chunks, `rows` (128 or 256) rows per
+// block, `depth` (8 or 16) columns of each basis in flight; returns the chunk count.
+template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, int rows, const double *omega_dev,
+                                    double *PY, double *PA) {
     const int n = L->nf, k = L->kc, pc = (k + ps - 1) / ps, nsplit = (k + pc - 1) / pc;
-    const dim3 grid((n + GEMV_ROWS - 1) / GEMV_ROWS, nsplit);
+    const dim3 grid((n + rows - 1) / rows, nsplit);
     if (depth == 16)
-        hipLaunchKernelGGL((prolong_kernel<N, 16>), grid, dim3(256), 0, L->stream, L->Phit.as<float>(),
+        hipLaunchKernelGGL((prolong_kernel<N, 16>), grid, dim3(rows), 0, L->stream, L->Phit.as<float>(),
                            L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
     else
-        hipLaunchKernelGGL((prolong_kernel<N, 8>), grid, dim3(256), 0, L->stream, L->Phit.as<float>(),
+        hipLaunchKernelGGL((prolong_kernel<N, 8>), grid, dim3(rows), 0, L->stream, L->Phit.as<float>(),
                            L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
     return nsplit;
 }
@@ -1195,7 +1206,7 @@ void tune_coarse(pq_loops *L, int NR) {
     if (k == 0 || n < 1024 || L->crows.count(NR)) return;
     const std::vector<int> rows_c = {128, 256, 512, 1024};
     std::vector<int> split_c;
-    for (int ps : {4, 8, 12, 16, 24, 32})
+    for (int ps : {4, 8, 12, 16, 24, 32, 48})
         if (ps <= (k + 15) / 16) split_c.push_back(ps);
     if (split_c.empty()) split_c.push_back(1);
     DevBuf X, part, PY, PA, om, flush, cv, dg;
@@ -1231,7 +1242,7 @@ void tune_coarse(pq_loops *L, int NR) {
         }
         return total / 3.f;
     };
-    int best_rows = rows_c[0], best_split = split_c[0], best_depth = 8;
+    int best_rows = rows_c[0], best_split = split_c[0], best_depth = 8, best_prows = GEMV_ROWS;
     float best = 1e30f;
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
@@ -1244,13 +1255,14 @@ void tune_coarse(pq_loops *L, int NR) {
             if (t < best) { best = t; best_rows = rows; }
         }
         best = 1e30f;
-        for (int depth : {8, 16})
-            for (int ps : split_c) {
-                const float t = timed([&] {
-                    prolong_launch<N>(L, ps, depth, om.as<double>(), PY.as<double>(), PA.as<double>());
-                });
-                if (t < best) { best = t; best_split = ps; best_depth = depth; }
-            }
+        for (int prows : {256, 128})
+            for (int depth : {8, 16})
+                for (int ps : split_c) {
+                    const float t = timed([&] {
+                        prolong_launch<N>(L, ps, depth, prows, om.as<double>(), PY.as<double>(), PA.as<double>());
+                    });
+                    if (t < best) { best = t; best_split = ps; best_depth = depth; best_prows = prows; }
+                }
     });
     PQ_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
@@ -1258,6 +1270,7 @@ void tune_coarse(pq_loops *L, int NR) {
     L->crows[NR] = best_rows;
     L->psplit[NR] = best_split;
     L->pdepth[NR] = best_depth;
+    L->prows[NR] = best_prows;
 }
 
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
@@ -1280,7 +1293,8 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
         double *PY = L->ppart.as<double>();
         const int ps = prolong_split(L, N);
         double *PA = PY + (size_t)ps * count;
-        const int nsplit = prolong_launch<N>(L, ps, prolong_depth(L, N), L->omega_dev.as<double>(), PY, PA);
+        const int nsplit = prolong_launch<N>(L, ps, prolong_depth(L, N), prolong_rows(L, N), L->omega_dev.as<double>(),
+                                             PY, PA);
         hipLaunchKernelGGL(prolong_epilogue_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, PY,
                            PA, nsplit, count, X, L->T1.as<double>(), L->T2.as<double>());
     });
@@ -1305,8 +1319,8 @@ void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P) {
 void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add) {
     dispatch_nr(2 * P, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk), dim3(256), 0, L->stream, L->inv.as<float>(), L->nf, X,
-                           add, Y);
+        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk, (BS + bj_rows(N) - 1) / bj_rows(N)), dim3(256), 0,
+                           L->stream, L->inv.as<float>(), L->nf, X, add, Y);
     });
     PQ_HIP(hipGetLastError());
 }
@@ -1883,6 +1897,7 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         L->crows.clear();
         L->psplit.clear();
         L->pdepth.clear();
+        L->prows.clear();
         if (k == 0) return;
         const int n = L->nf;
         L->kp = (k + 63) / 64 * 64;

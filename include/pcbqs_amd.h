@@ -173,7 +173,7 @@ int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
  * Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
  * frequency omega: the free loops eliminated, X = -Z_ff^-1 Z_fs, by restarted
  * lockstep GMRES(restart) with all P columns sharing every operator product
- * (precond 0: none, 1: block-Jacobi on 128-loop blocks of the free-loop order,
+ * (precond 0: none, 1: block-Jacobi on 256-loop blocks of the free-loop order,
  * 2: two-level: the coarse space of pq_loops_set_coarse, then block-Jacobi),
  * then Zp = Z_ss + Z_sf X. X_host (n_free, 2P) (optional) receives the free
  * loop currents, real parts first; X0_host (optional, same layout) is the

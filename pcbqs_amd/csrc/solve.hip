@@ -386,79 +386,96 @@ __global__ __launch_bounds__(256, 2) void sym_panel_kernel(const float *__restri
 // ---- block-Jacobi --------------------------------------------------------------------------
 
 // Inverse of each diagonal block Z_bb = Rl_bb + j w Ll_bb (size <= BS) by in-place Gauss-Jordan
-// with partial pivoting in LDS, complex FP32 (a preconditioner); stored as (nblk, BS, BS)
-// complex (re, im interleaved). A padded block is the identity. 128 loops per block (the block's
-// two FP32 planes fill 132 KB of the 160 KB LDS): against 64-loop blocks the plain solve of the
-// 30k-RWG board takes 421 iterations instead of 644, the two-level one 64 instead of 86.
-constexpr int BS = 128;
+// with partial pivoting, complex FP32 (a preconditioner); stored as (nblk, BS, BS) complex
+// (re, im interleaved). A padded block is the identity. 256 loops per block: against 64-loop
+// blocks the plain solve of the 30k-RWG board takes 375 iterations instead of 644, the two-level
+// one 53 instead of 86 (128-loop blocks: 421 and 64; 512-loop blocks: 283 and 45, but eight times
+// the per-frequency set-up). A 256 x 256 complex block (512 KB) does not fit in LDS, so one
+// workgroup per block factors it in place in the output buffer, where it stays in the L2 of its
+// XCD: wave 0 picks the pivot (largest |a_ik|, the first on ties), the pivot row and the column of
+// factors go through LDS, and the rank-1 update is spread over the workgroup's 1024 threads.
+constexpr int BS = 256;
 
-__global__ __launch_bounds__(256) void bj_setup_kernel(const float *Ll, long long ldl, const int *ptr,
-                                                      const int *col, const double *val, int n, double omega,
-                                                      float *inv, int *singular) {
-    __shared__ float ar[BS][BS + 1], ai[BS][BS + 1];
-    __shared__ int perm[BS];
+__global__ __launch_bounds__(1024) void bj_setup_kernel(const float *Ll, long long ldl, const int *ptr,
+                                                       const int *col, const double *val, int n, double omega,
+                                                       float *inv, int *perm_g, int *singular) {
+    __shared__ float fr_[BS], fi_[BS], pr_[BS], pi_[BS];
     __shared__ int piv;
     const int b = blockIdx.x, r0 = b * BS, m = min(BS, n - r0);
-    for (int e = threadIdx.x; e < BS * BS; e += blockDim.x) {
+    float2 *a = reinterpret_cast<float2 *>(inv) + (size_t)b * BS * BS;
+    int *perm = perm_g + (size_t)b * BS;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int e = tid; e < BS * BS; e += nt) {
         const int i = e / BS, j = e % BS;
         float re = 0, im = 0;
         if (i < m && j < m) im = (float)(omega * (double)Ll[(size_t)(r0 + i) * ldl + r0 + j]);
         else if (i == j) re = 1;
-        ar[i][j] = re; ai[i][j] = im;
+        a[e] = make_float2(re, im);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += blockDim.x)
+    for (int i = tid; i < m; i += nt)
         for (int t = ptr[r0 + i]; t < ptr[r0 + i + 1]; ++t) {
             const int c = col[t] - r0;
-            if (c >= 0 && c < m) ar[i][c] += (float)val[t];
+            if (c >= 0 && c < m) a[(size_t)i * BS + c].x += (float)val[t];
         }
     __syncthreads();
     for (int k = 0; k < BS; ++k) {
-        if (threadIdx.x == 0) {
+        if (tid < 64) {
+            float bv = -1;
             int best = k;
-            float bv = ar[k][k] * ar[k][k] + ai[k][k] * ai[k][k];
-            for (int i = k + 1; i < BS; ++i) {
-                const float v = ar[i][k] * ar[i][k] + ai[i][k] * ai[i][k];
-                if (v > bv) { bv = v; best = i; }
+            for (int i = k + tid; i < BS; i += 64) {
+                const float2 v = a[(size_t)i * BS + k];
+                const float q = v.x * v.x + v.y * v.y;
+                if (q > bv) { bv = q; best = i; }
             }
-            piv = best;
-            perm[k] = best;
-            if (!(bv > 0)) atomicOr(singular, 1);
+            for (int off = 32; off > 0; off >>= 1) {
+                const float ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(best, off);
+                if (ov > bv || (ov == bv && oi < best)) { bv = ov; best = oi; }
+            }
+            if (tid == 0) {
+                piv = best;
+                perm[k] = best;
+                if (!(bv > 0)) atomicOr(singular, 1);
+            }
         }
         __syncthreads();
+        // rows k and p exchanged: old row k moves to row p, the pivot row to LDS
         const int p = piv;
-        if (p != k)
-            for (int j = threadIdx.x; j < BS; j += blockDim.x) {
-                const float tr = ar[k][j], ti = ai[k][j];
-                ar[k][j] = ar[p][j]; ai[k][j] = ai[p][j];
-                ar[p][j] = tr; ai[p][j] = ti;
-            }
+        for (int j = tid; j < BS; j += nt) {
+            const float2 rk = a[(size_t)k * BS + j], rp = a[(size_t)p * BS + j];
+            if (p != k) a[(size_t)p * BS + j] = rk;
+            pr_[j] = rp.x; pi_[j] = rp.y;
+        }
         __syncthreads();
-        const float dr = ar[k][k], di = ai[k][k], dd = dr * dr + di * di;
+        const float dr = pr_[k], di = pi_[k], dd = dr * dr + di * di;
         const float ir = dd > 0 ? dr / dd : 0.f, ii = dd > 0 ? -di / dd : 0.f;   // 1 / a_kk
         __syncthreads();
-        if (threadIdx.x == 0) { ar[k][k] = 1; ai[k][k] = 0; }
+        if (tid == 0) { pr_[k] = 1; pi_[k] = 0; }
         __syncthreads();
-        for (int j = threadIdx.x; j < BS; j += blockDim.x) {
-            const float xr = ar[k][j], xi = ai[k][j];
-            ar[k][j] = xr * ir - xi * ii;
-            ai[k][j] = xr * ii + xi * ir;
+        for (int j = tid; j < BS; j += nt) {
+            const float xr = pr_[j], xi = pi_[j];
+            const float yr = xr * ir - xi * ii, yi = xr * ii + xi * ir;
+            pr_[j] = yr; pi_[j] = yi;
+            a[(size_t)k * BS + j] = make_float2(yr, yi);
         }
-        __syncthreads();
         // column k of the other rows: the factors, then a_ik := 0 before the row update
-        __shared__ float fr_[BS], fi_[BS];
-        for (int i = threadIdx.x; i < BS; i += blockDim.x) {
-            fr_[i] = ar[i][k]; fi_[i] = ai[i][k];
-            if (i != k) { ar[i][k] = 0; ai[i][k] = 0; }
+        for (int i = tid; i < BS; i += nt) {
+            if (i == k) continue;
+            const float2 f = a[(size_t)i * BS + k];
+            fr_[i] = f.x; fi_[i] = f.y;
+            a[(size_t)i * BS + k] = make_float2(0.f, 0.f);
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < BS * BS; e += blockDim.x) {
+        for (int e = tid; e < BS * BS; e += nt) {
             const int i = e / BS, j = e % BS;
             if (i == k) continue;
             const float fr = fr_[i], fi = fi_[i];
-            const float xr = ar[k][j], xi = ai[k][j];
-            ar[i][j] -= fr * xr - fi * xi;
-            ai[i][j] -= fr * xi + fi * xr;
+            const float xr = pr_[j], xi = pi_[j];
+            float2 v = a[e];
+            v.x -= fr * xr - fi * xi;
+            v.y -= fr * xi + fi * xr;
+            a[e] = v;
         }
         __syncthreads();
     }
@@ -466,18 +483,12 @@ __global__ __launch_bounds__(256) void bj_setup_kernel(const float *Ll, long lon
     for (int k = BS - 1; k >= 0; --k) {
         const int p = perm[k];
         if (p != k)
-            for (int i = threadIdx.x; i < BS; i += blockDim.x) {
-                const float tr = ar[i][k], ti = ai[i][k];
-                ar[i][k] = ar[i][p]; ai[i][k] = ai[i][p];
-                ar[i][p] = tr; ai[i][p] = ti;
+            for (int i = tid; i < BS; i += nt) {
+                const float2 t = a[(size_t)i * BS + k];
+                a[(size_t)i * BS + k] = a[(size_t)i * BS + p];
+                a[(size_t)i * BS + p] = t;
             }
         __syncthreads();
-    }
-    float *out = inv + (size_t)b * BS * BS * 2;
-    for (int e = threadIdx.x; e < BS * BS; e += blockDim.x) {
-        const int i = e / BS, j = e % BS;
-        out[2 * e] = ar[i][j];
-        out[2 * e + 1] = ai[i][j];
     }
 }
 
@@ -953,7 +964,7 @@ struct pq_loops {
     int device = 0;
     int nl = 0, nf = 0, P = 0;
     long long ldl = 0;
-    DevBuf Ll, rptr, rcol, rval, sc, inv, singular;
+    DevBuf Ll, rptr, rcol, rval, sc, inv, bjperm, singular;
     DevBuf V, W, Z, X, B, R, partial, h, scal;
     hipStream_t stream = nullptr;
     int nblk = 0;
@@ -1570,7 +1581,7 @@ int pq_loops_destroy(pq_loops *L) {
     return guard([&] {
         DeviceScope scope(L->device);
         if (L->stream) (void)hipStreamSynchronize(L->stream);
-        for (DevBuf *b : {&L->Ll, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->singular, &L->V, &L->W, &L->Z,
+        for (DevBuf *b : {&L->Ll, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->bjperm, &L->singular, &L->V, &L->W, &L->Z,
                           &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
                           &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit, &L->ppart})
             b->release();
@@ -1631,9 +1642,10 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 L->nblk = (n + BS - 1) / BS;
                 L->inv.reserve((size_t)L->nblk * BS * BS * 2 * 4);
                 PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
-                hipLaunchKernelGGL(bj_setup_kernel, dim3(L->nblk), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+                L->bjperm.reserve((size_t)L->nblk * BS * 4);
+                hipLaunchKernelGGL(bj_setup_kernel, dim3(L->nblk), dim3(1024), 0, L->stream, L->Ll.as<float>(), L->ldl,
                                    L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, omega,
-                                   L->inv.as<float>(), L->singular.as<int>());
+                                   L->inv.as<float>(), L->bjperm.as<int>(), L->singular.as<int>());
                 PQ_HIP(hipGetLastError());
                 int sing = 0;
                 PQ_HIP(hipMemcpyAsync(&sing, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));

@@ -493,9 +493,10 @@ __global__ __launch_bounds__(1024) void bj_setup_kernel(const float *Ll, long lo
 }
 
 // Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P. A workgroup
-// takes bj_rows(NR) rows of block blockIdx.x (from row blockIdx.y * bj_rows), one thread per
-// (row, port) output, so a block is spread over several workgroups.
-constexpr int bj_rows(int NR) { return 512 / NR < BS ? 512 / NR : BS; }
+// takes bj_rows(NR) rows of block blockIdx.x (from row blockIdx.y * bj_rows); four adjacent lanes
+// share a (row, port) output and take the columns j = s, s + 4, ... (32 contiguous bytes of the
+// inverse's row per four lanes), combined by two fixed shuffles.
+constexpr int bj_rows(int NR) { return 128 / NR < BS ? 128 / NR : BS; }
 template <int NR>
 __global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, const double *X, const double *A,
                                                       double *Y) {
@@ -504,17 +505,25 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, 
     const int b = blockIdx.x, r0 = b * BS, m = min(BS, n - r0);
     for (int e = threadIdx.x; e < BS * NR; e += blockDim.x) xs[e] = (e / NR) < m ? X[(size_t)r0 * NR + e] : 0.0;
     __syncthreads();
-    const float *Zi = inv + (size_t)b * BS * BS * 2;
-    const int i0 = blockIdx.y * R, iend = min(m, i0 + R);
-    for (int e = threadIdx.x; e < (iend - i0) * P; e += blockDim.x) {
-        const int i = i0 + e / P, p = e % P;
-        double yr = 0, yi = 0;
-        for (int j = 0; j < m; ++j) {
-            const double zr = (double)Zi[2 * (i * BS + j)], zi = (double)Zi[2 * (i * BS + j) + 1];
+    const float2 *Zi = reinterpret_cast<const float2 *>(inv) + (size_t)b * BS * BS;
+    const int sp = threadIdx.x & 3, o = threadIdx.x >> 2;        // column split; (row, port) output
+    const int i = blockIdx.y * R + o / P, p = o % P;
+    const bool live = o < R * P && i < m;
+    double yr = 0, yi = 0;
+    if (live) {
+        const float2 *zrow = Zi + (size_t)i * BS;
+#pragma unroll 8
+        for (int j = sp; j < m; j += 4) {
+            const float2 z = zrow[j];
+            const double zr = z.x, zi = z.y;
             const double xr = xs[j * NR + p], xi = xs[j * NR + P + p];
             yr += zr * xr - zi * xi;
             yi += zr * xi + zi * xr;
         }
+    }
+    yr += __shfl_xor(yr, 1); yi += __shfl_xor(yi, 1);
+    yr += __shfl_xor(yr, 2); yi += __shfl_xor(yi, 2);
+    if (live && sp == 0) {
         if (A) {
             yr += A[(size_t)(r0 + i) * NR + p];
             yi += A[(size_t)(r0 + i) * NR + P + p];

@@ -186,6 +186,26 @@ int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *po
                          int restart, int maxiter, double *Zp, double *X_host, const double *X0_host,
                          int64_t *iterations, double *residual);
 
+/* The direct port solve at omega: the same outputs as pq_loops_solve_ports
+ * (Zp, optional X_host, the correction count in *steps and the final largest
+ * relative residual in *residual), with Z_ff = Rl + j omega Ll formed in FP32
+ * on the device and factored as L D L^T without pivoting (both parts are
+ * symmetric positive definite), then refined in FP64 against the operator
+ * product until the relative residual is at most tol (the reference uses
+ * 1e-12), for at most max_steps corrections, stopping once a correction no
+ * longer halves it; a residual then above 1e-10 returns PQ_ECONVERGE. The
+ * factor is kept for the next call at the same omega. P 1..8 or 16. Replaces
+ * the dense magnetic solve MQS._solve_dense (src/pcbqs/mqs.py:743-808) with
+ * lu_factor_rows / lu_solve_rows and MixedLU (src/pcbqs/kernels.py:125-181,
+ * 266-292: FP32 LAPACK LU refined to 1e-12 in at most 12 steps). */
+int pq_loops_solve_direct(pq_loops *loops, double omega, int P, const int32_t *port_cols, double tol,
+                          int max_steps, double *Zp, double *X_host, int64_t *steps, double *residual);
+
+/* Device time in milliseconds of the factorisation (forming Z_ff and its
+ * L D L^T) that the last pq_loops_solve_direct used. No counterpart in the
+ * reference. */
+int pq_loops_factor_time(pq_loops *loops, double *ms);
+
 /* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
  * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the
  * right-hand side of direct_ports (src/pcbqs/mqs_ports.py:52-69). */

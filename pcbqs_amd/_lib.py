@@ -65,6 +65,8 @@ _SIGNATURES = {
     'pq_loops_matrix': ([P, P, P], ctypes.c_int),
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
                              ctypes.c_int),
+    'pq_loops_solve_direct': ([P, DBL, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P], ctypes.c_int),
+    'pq_loops_factor_time': ([P, P], ctypes.c_int),
     'pq_loops_set_coarse': ([P, ctypes.c_int, P, ctypes.c_int], ctypes.c_int),
     'pq_loops_coarse_matrices': ([P, P, P], ctypes.c_int),
     'pq_biot_field': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, I64, P, I64, P, I64, P, P, P], ctypes.c_int),

@@ -1,0 +1,31 @@
+// Dense complex-symmetric factorisation Z = L D L^T of the free-loop impedance on the device, and
+// the solves with it: the direct (factor-and-refine) port solve of pq_loops_solve_direct.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pq {
+
+constexpr int LDL_BLOCK = 64;   // panel width, diagonal block, trailing-update tile
+
+// The factor's storage: planes Fr, Fi (n x ldf FP32, row-major, ldf a multiple of 64; the lower
+// triangle holds L below the diagonal and D on it), the inverses of the unit lower diagonal blocks
+// Lir, Lii (ceil(n / 64) blocks of 64 x 64), the trailing update's right factor Wr, Wi (n x 64).
+struct LdlFactor {
+    float *Fr, *Fi;
+    long long ldf;
+    int n;
+    float *Lir, *Lii, *Wr, *Wi;
+};
+
+// F = Rl + j omega Ll on the first n loops (lower triangle): Ll FP32 (row stride ldl), Rl CSR FP64.
+void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl, const int *rptr, const int *rcol,
+              const double *rval, double omega);
+
+// Factor in place; *bad (device int, cleared by the caller) is set when a pivot is zero or not finite.
+void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
+
+// X = Z^-1 B for P complex columns (n, 2P) FP64 (real parts first), FP32 factor, FP64 arithmetic.
+// Y, Ys: (n, 2P) FP64 work arrays. P is 1..8 or 16.
+void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, double *Y, double *Ys, double *X);
+
+}  // namespace pq

@@ -13,6 +13,7 @@
 // accumulates in FP64. Rl is CSR (FP64 values). Reductions are two-stage with a fixed order, so
 // results do not depend on scheduling.
 #include "pq_common.h"
+#include "ldl.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1002,6 +1003,12 @@ struct pq_loops {
     // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks
     // and loads in flight)
     std::map<int, int> crows, psplit, pdepth, prows;
+    // the direct solve's L D L^T factor (planes of the factor, block inverses, panel products) and
+    // the frequency it was formed at
+    DevBuf Fldl, Xldl, Wldl;
+    long long ldf = 0;
+    double omega_ldl = -1;
+    double factor_ms = 0;
 };
 
 namespace {
@@ -1525,6 +1532,47 @@ void update(pq_loops *L, const double *V, size_t vstride, int nv, const double *
 }
 
 
+// Zp = Z_ss + Z_sf X: the port loops' own block from Rl and the stored Ll rows, and the coupling
+// to the free loops' solution X (n, 2P) on the device (src/pcbqs/mqs_ports.py:52-69).
+void port_impedance(pq_loops *L, double omega, int P, const int32_t *port_cols, const double *X, double *Zp) {
+    const int n = L->nf;
+    DevBuf red;
+    red.reserve((size_t)P * P * 2 * 8);
+    if (n > 0) {
+        hipLaunchKernelGGL(port_reduce_kernel, dim3(P * P), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+                           L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, X,
+                           red.as<double>());
+        PQ_HIP(hipGetLastError());
+    } else {
+        PQ_HIP(hipMemsetAsync(red.p, 0, (size_t)P * P * 2 * 8, L->stream));
+    }
+    std::vector<double> zsf((size_t)P * P * 2);
+    PQ_HIP(hipMemcpyAsync(zsf.data(), red.p, zsf.size() * 8, hipMemcpyDeviceToHost, L->stream));
+    // Z_ss from the stored rows
+    std::vector<float> llss((size_t)P * P);
+    for (int s = 0; s < P; ++s)
+        for (int q = 0; q < P; ++q)
+            PQ_HIP(hipMemcpyAsync(&llss[s * P + q], L->Ll.as<float>() + (size_t)port_cols[s] * L->ldl + port_cols[q], 4,
+                                  hipMemcpyDeviceToHost, L->stream));
+    std::vector<int> rp(L->nl + 1);
+    PQ_HIP(hipMemcpyAsync(rp.data(), L->rptr.p, (L->nl + 1) * 4, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    for (int s = 0; s < P; ++s) {
+        const int row = port_cols[s];
+        const int cnt = rp[row + 1] - rp[row];
+        std::vector<int> cols(cnt);
+        std::vector<double> vals(cnt);
+        PQ_HIP(hipMemcpy(cols.data(), L->rcol.as<int>() + rp[row], cnt * 4, hipMemcpyDeviceToHost));
+        PQ_HIP(hipMemcpy(vals.data(), L->rval.as<double>() + rp[row], cnt * 8, hipMemcpyDeviceToHost));
+        for (int q = 0; q < P; ++q) {
+            double re = 0;
+            for (int t = 0; t < cnt; ++t) if (cols[t] == port_cols[q]) re += vals[t];
+            Zp[2 * (s * P + q)] = re + zsf[2 * (s * P + q)];
+            Zp[2 * (s * P + q) + 1] = omega * (double)llss[s * P + q] + zsf[2 * (s * P + q) + 1];
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1862,44 +1910,123 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 throw Error(PQ_ECONVERGE, "GMRES did not converge: relative residual " + std::to_string(worst) +
                                               " after " + std::to_string(its) + " iterations");
         }
-        // Zp = Z_ss + Z_sf X
-        DevBuf red;
-        red.reserve((size_t)P * P * 2 * 8);
-        if (n > 0) {
-            hipLaunchKernelGGL(port_reduce_kernel, dim3(P * P), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
-                               L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, X,
-                               red.as<double>());
-            PQ_HIP(hipGetLastError());
-        } else {
-            PQ_HIP(hipMemsetAsync(red.p, 0, (size_t)P * P * 2 * 8, L->stream));
-        }
-        std::vector<double> zsf((size_t)P * P * 2);
-        PQ_HIP(hipMemcpyAsync(zsf.data(), red.p, zsf.size() * 8, hipMemcpyDeviceToHost, L->stream));
-        // Z_ss from the stored rows
-        std::vector<float> llss((size_t)P * P);
-        for (int s = 0; s < P; ++s)
-            for (int q = 0; q < P; ++q)
-                PQ_HIP(hipMemcpyAsync(&llss[s * P + q], L->Ll.as<float>() + (size_t)port_cols[s] * L->ldl + port_cols[q], 4,
-                                      hipMemcpyDeviceToHost, L->stream));
-        std::vector<int> rp(L->nl + 1);
-        PQ_HIP(hipMemcpyAsync(rp.data(), L->rptr.p, (L->nl + 1) * 4, hipMemcpyDeviceToHost, L->stream));
-        PQ_HIP(hipStreamSynchronize(L->stream));
-        for (int s = 0; s < P; ++s) {
-            const int row = port_cols[s];
-            const int cnt = rp[row + 1] - rp[row];
-            std::vector<int> cols(cnt);
-            std::vector<double> vals(cnt);
-            PQ_HIP(hipMemcpy(cols.data(), L->rcol.as<int>() + rp[row], cnt * 4, hipMemcpyDeviceToHost));
-            PQ_HIP(hipMemcpy(vals.data(), L->rval.as<double>() + rp[row], cnt * 8, hipMemcpyDeviceToHost));
-            for (int q = 0; q < P; ++q) {
-                double re = 0;
-                for (int t = 0; t < cnt; ++t) if (cols[t] == port_cols[q]) re += vals[t];
-                Zp[2 * (s * P + q)] = re + zsf[2 * (s * P + q)];
-                Zp[2 * (s * P + q) + 1] = omega * (double)llss[s * P + q] + zsf[2 * (s * P + q) + 1];
-            }
-        }
+        port_impedance(L, omega, P, port_cols, X, Zp);
         if (X_host && n > 0) PQ_HIP(hipMemcpy(X_host, X, vsz * 8, hipMemcpyDeviceToHost));
         L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    });
+}
+
+// The direct port solve at one frequency: Z_ff = Rl + j w Ll formed in FP32 on the device and
+// factored as L D L^T without pivoting (both parts of Z_ff are symmetric positive definite), then
+// X = -Z_ff^-1 Z_fs refined in FP64 as src/pcbqs/kernels.py:125-181 (MixedLU): the FP32 solution is
+// corrected by its residual, taken with the FP64 operator product, until the largest column's
+// relative residual is at most tol, for at most max_steps corrections, stopping early once a
+// correction no longer halves it; a result above 1e-10 then raises PQ_ECONVERGE. The factor is
+// kept for the next call at the same frequency.
+int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_cols, double tol, int max_steps,
+                          double *Zp, double *X_host, int64_t *steps, double *residual) {
+    return guard([&] {
+        PQ_REQUIRE(L && port_cols && Zp, "pq_loops_solve_direct: null argument");
+        PQ_REQUIRE((P >= 1 && P <= 8) || P == 16, "pq_loops_solve_direct: 1 to 8 or 16 ports");
+        PQ_REQUIRE(tol > 0 && tol < 1 && max_steps >= 0, "pq_loops_solve_direct: invalid options");
+        for (int p = 0; p < P; ++p)
+            PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl, "pq_loops_solve_direct: port loop out of range");
+        DeviceScope scope(L->device);
+        const auto started = std::chrono::steady_clock::now();
+        const int n = L->nf, NR = 2 * P;
+        const size_t vsz = (size_t)n * NR;
+        L->matvecs = 0;
+        L->sc.reserve(P * 4);
+        PQ_HIP(hipMemcpyAsync(L->sc.p, port_cols, P * 4, hipMemcpyHostToDevice, L->stream));
+        L->V.reserve(2 * vsz * 8);
+        L->W.reserve(vsz * 8); L->Z.reserve(vsz * 8); L->X.reserve(vsz * 8); L->B.reserve(vsz * 8);
+        L->partial.reserve((size_t)RED_BLOCKS * P * 2 * 8);
+        L->h.reserve((size_t)P * 2 * 8);
+        double *X = L->X.as<double>(), *B = L->B.as<double>(), *W = L->W.as<double>(), *D = L->Z.as<double>();
+        double *Y = L->V.as<double>(), *Ys = Y + vsz;
+        int corrections = 0;
+        double worst = 0;
+        if (n > 0) {
+            tune_gemv(L, NR);
+            set_omega(L, omega);
+            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+                               L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, B);
+            PQ_HIP(hipGetLastError());
+            const int nblk = (n + LDL_BLOCK - 1) / LDL_BLOCK;
+            L->ldf = (long long)nblk * LDL_BLOCK;
+            L->Fldl.reserve((size_t)2 * n * L->ldf * 4);
+            L->Xldl.reserve((size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK * 4);
+            L->Wldl.reserve((size_t)2 * n * LDL_BLOCK * 4);
+            LdlFactor f{L->Fldl.as<float>(), L->Fldl.as<float>() + (size_t)n * L->ldf, L->ldf, n,
+                        L->Xldl.as<float>(), L->Xldl.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,
+                        L->Wldl.as<float>(), L->Wldl.as<float>() + (size_t)n * LDL_BLOCK};
+            if (L->omega_ldl != omega) {
+                L->omega_ldl = -1;
+                hipEvent_t e0, e1;
+                PQ_HIP(hipEventCreate(&e0));
+                PQ_HIP(hipEventCreate(&e1));
+                PQ_HIP(hipEventRecord(e0, L->stream));
+                ldl_form(L->stream, f, L->Ll.as<float>(), L->ldl, L->rptr.as<int>(), L->rcol.as<int>(),
+                         L->rval.as<double>(), omega);
+                PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
+                ldl_factor(L->stream, f, L->singular.as<int>());
+                PQ_HIP(hipEventRecord(e1, L->stream));
+                int bad = 0;
+                PQ_HIP(hipMemcpyAsync(&bad, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));
+                PQ_HIP(hipStreamSynchronize(L->stream));
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                PQ_REQUIRE(!bad, "direct solve: a zero or non-finite pivot in the L D L^T factorisation");
+                L->factor_ms = ms;
+                L->omega_ldl = omega;
+            }
+            std::vector<double> bnorm, rn;
+            norms(L, B, P, bnorm);
+            for (auto &b : bnorm) b = std::max(b, 1e-300);
+            ldl_solve(L->stream, f, P, B, Y, Ys, X);
+            const size_t count = vsz;
+            double last = std::numeric_limits<double>::infinity();
+            for (int step = 0;; ++step) {
+                apply_operator(L, X, W, P);                                     // W = B - Z X
+                ++L->matvecs;
+                hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, B,
+                                   count, W);
+                PQ_HIP(hipGetLastError());
+                norms(L, W, P, rn);
+                worst = 0;
+                for (int p = 0; p < P; ++p) worst = std::max(worst, rn[p] / bnorm[p]);
+                if (!(worst > tol) || step == max_steps) break;
+                if (worst > 0.5 * last) break;                                  // no longer converging
+                last = worst;
+                ldl_solve(L->stream, f, P, W, Y, Ys, D);                        // X += Z^-1 (B - Z X)
+                hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, D,
+                                   count, X);
+                PQ_HIP(hipGetLastError());
+                ++corrections;
+            }
+        }
+        L->iterations = corrections;
+        L->restarts = 0;
+        L->residual = worst;
+        if (steps) *steps = corrections;
+        if (residual) *residual = worst;
+        if (worst > std::max(tol, 1e-10))
+            throw Error(PQ_ECONVERGE, "direct solve: the FP32 factor's refinement stalls at relative residual " +
+                                          std::to_string(worst) + " after " + std::to_string(corrections) +
+                                          " corrections");
+        port_impedance(L, omega, P, port_cols, X, Zp);
+        if (X_host && n > 0) PQ_HIP(hipMemcpy(X_host, X, vsz * 8, hipMemcpyDeviceToHost));
+        L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    });
+}
+
+// Device time (ms) of the factorisation (forming Z_ff and its L D L^T) that the last direct solve used.
+int pq_loops_factor_time(pq_loops *L, double *ms) {
+    return guard([&] {
+        PQ_REQUIRE(L && ms, "pq_loops_factor_time: null argument");
+        *ms = L->factor_ms;
     });
 }
 

@@ -19,6 +19,9 @@ from .geometry import MU0, RWG, FillGeometry
 from .loops import LoopModel
 
 PRECONDITIONERS = {'none': 0, 'block-jacobi': 1, 'two-level': 2}
+# The direct solve's refinement: relative residual and correction count of the reference's MixedLU
+# (src/pcbqs/kernels.py:137: TOL, STEPS = 1e-12, 12).
+DIRECT_TOL, DIRECT_STEPS = 1e-12, 12
 
 
 class PortSolver:
@@ -155,11 +158,18 @@ class PortSolver:
         call('pq_loops_coarse_solution', self._handle, 2 * np.pi * float(frequency), P, ptr(self.port_cols), ptr(X))
         return self._join(X)
 
-    def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False, x0=None):
+    def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False, x0=None,
+              solver='iterative'):
         """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds).
 
+        solver: 'iterative' (lockstep GMRES with `precond`, to `tol`) or 'direct' (the free-loop
+        system factored as L D L^T in FP32 on the device and refined in FP64 to a relative residual
+        of 1e-12 in at most 12 corrections, as the reference's MixedLU; precond, tol, restart,
+        maxiter and x0 do not apply, and `iterations` counts the corrections).
         x0: optional initial iterates, (F, n_free, P) complex or a callable frequency -> (n_free, P)
         (the learned correction's prediction); GMRES starts from zero otherwise."""
+        if solver not in ('iterative', 'direct'):
+            raise ValueError("solver must be 'iterative' or 'direct'")
         P = self.n_ports
         Pd = self.device_ports
         cols = self.port_cols if Pd == P else np.ascontiguousarray(
@@ -170,11 +180,24 @@ class PortSolver:
         Z = np.zeros((len(f), P, P), complex)
         X = np.zeros((len(f), self.model.n_free, 2 * P)) if currents else None
         info = []
-        kind = PRECONDITIONERS[precond]
+        kind = PRECONDITIONERS[precond] if solver == 'iterative' else None
         for i, fi in enumerate(f):
             zp = np.zeros((Pd, Pd, 2))
             its, res = ctypes.c_int64(), ctypes.c_double()
             xh = np.zeros((self.model.n_free, 2 * Pd)) if currents else None
+            if solver == 'direct':
+                call('pq_loops_solve_direct', self._handle, 2 * np.pi * fi, Pd, ptr(cols), DIRECT_TOL, DIRECT_STEPS,
+                     ptr(zp), ptr(xh) if currents else None, ctypes.byref(its), ctypes.byref(res))
+                Z[i] = zp[:P, :P, 0] + 1j * zp[:P, :P, 1]
+                if currents:
+                    X[i] = np.hstack([xh[:, :P], xh[:, Pd:Pd + P]])
+                seconds, mv, fms = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+                call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))
+                call('pq_loops_factor_time', self._handle, ctypes.byref(fms))
+                info.append(dict(frequency_Hz=float(fi), iterations=int(its.value), matvecs=int(mv.value),
+                                 relative_residual=float(res.value), seconds=float(seconds.value),
+                                 preconditioner='direct', factor_ms=float(fms.value)))
+                continue
             start = None
             if x0 is not None:
                 guess = x0(fi) if callable(x0) else x0[i]

@@ -45,93 +45,192 @@ __global__ void ldl_form_r_kernel(const int *ptr, const int *col, const double *
     }
 }
 
-// 1 / (dr + j di), scaled so that neither the square nor the quotient leaves FP32's range early.
+// 1 / (dr + j di) = conj(e) / (s |e|^2) with e = d / s, s = max(|dr|, |di|), so that the square
+// stays in FP32's range. Hardware reciprocals (1 ulp): the factor is FP32 and the solve refines
+// its result in FP64, so correctly rounded divisions (a dependent chain of about a dozen
+// instructions each, on the critical path of every elimination step) buy nothing.
 __device__ inline void cinv(float dr, float di, float &qr, float &qi) {
-    const float s = fmaxf(fabsf(dr), fabsf(di));
-    const float er = dr / s, ei = di / s, den = s * (er * er + ei * ei);
-    qr = er / den;
-    qi = -ei / den;
+    const float rs = __builtin_amdgcn_rcpf(fmaxf(fabsf(dr), fabsf(di)));
+    const float er = dr * rs, ei = di * rs;
+    const float q = __builtin_amdgcn_rcpf(er * er + ei * ei) * rs;
+    qr = er * q;
+    qi = -ei * q;
 }
 
 // The b x b diagonal block at (k, k) factored in place (L below the diagonal, D on it), and the
 // inverse X = L^-1 of its unit lower factor written to Lir / Lii (64 x 64, the identity beyond b).
-// Thread t holds row t >> 2 at columns (t & 3) + 4 s, s < 16, of the block and of X in registers.
-// Step j: the pivot and row j of X go through LDS, then column j of L (l_ij = a_ij / d_j), then
-// every row i > j updates a_ic -= l_ij a_cj (j < c <= i, a_cj the column before scaling) and
-// x_ic -= l_ij x_jc (c <= j). A zero or non-finite pivot sets *bad.
+// The block and X live in LDS (rows past b: the identity). Blocked by 16: for each 16 x 16
+// diagonal sub-block S,
+//   (a) S = L_S D_S L_S^T and X_S = L_S^-1 in 16 steps by one wave (lane shuffles, no barriers);
+//   (b) below it, W = A_21 X_S^T and L_21 = W D_S^-1;
+//   (c) the lower triangle below and right of it, A_22 -= L_21 W^T;
+// then X's block rows below the first, X_I = -X_S,I (sum_{M < I} L_IM X_M), one block row at a
+// time. A zero or non-finite pivot sets *bad.
 __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, long long ldf, int k, int b, float *Lir,
                                                       float *Lii, int *bad) {
-    __shared__ float pv[2], lr[LB], li[LB], cr[LB], ci[LB];
-    __shared__ float rr[2][LB], ri[2][LB];
-    const int t = threadIdx.x, i = t >> 2, c0 = t & 3;
-    const bool live = i < b;
-    float *rowr = Fr + (size_t)(k + (live ? i : 0)) * ldf + k, *rowi = Fi + (size_t)(k + (live ? i : 0)) * ldf + k;
-    float ar[16], ai[16], xr[16], xi[16];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int c = c0 + 4 * s;
-        const bool in = live && c <= i;
-        ar[s] = in ? rowr[c] : 0.f;
-        ai[s] = in ? rowi[c] : 0.f;
-        xr[s] = c == i ? 1.f : 0.f;
-        xi[s] = 0.f;
+    __shared__ float Ar[LB][LB + 1], Ai[LB][LB + 1], Xr[LB][LB + 1], Xi[LB][LB + 1];
+    __shared__ float Wr[LB - 16][17], Wi[LB - 16][17];
+    __shared__ float Tr[16][LB + 1], Ti[16][LB + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < LB * LB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        float vr = 0.f, vi = 0.f;
+        if (r < b && c <= r) {
+            vr = Fr[(size_t)(k + r) * ldf + k + c];
+            vi = Fi[(size_t)(k + r) * ldf + k + c];
+        } else if (r >= b && c == r) {
+            vr = 1.f;
+        }
+        Ar[r][c] = vr;
+        Ai[r][c] = vi;
+        Xr[r][c] = r == c ? 1.f : 0.f;
+        Xi[r][c] = 0.f;
     }
-    for (int j = 0; j < b; ++j) {
-        const int sj = j >> 2, buf = j & 1;
-        const bool colj = c0 == (j & 3);
-        if (i == j) {
-            if (colj) {
-                float vr = 0.f, vi = 0.f;
+    __syncthreads();
+    for (int s0 = 0; s0 < LB; s0 += 16) {
+        // (a) in the registers of wave 0: lane l holds row l >> 2 at columns 4 (l & 3) .. + 3 of the
+        // sub-block and of X_S; step j exchanges the pivot, column j and row j of X_S by lane
+        // shuffles, so the 16 steps need no workgroup barrier
+        if (t < 64) {
+            const int r = t >> 2, g = t & 3;
+            float fr[4], fi[4], gr[4], gi[4];
 #pragma unroll
-                for (int s = 0; s < 16; ++s)
-                    if (s == sj) { vr = ar[s]; vi = ai[s]; }
-                pv[0] = vr;
-                pv[1] = vi;
+            for (int q = 0; q < 4; ++q) {
+                fr[q] = Ar[s0 + r][s0 + 4 * g + q];
+                fi[q] = Ai[s0 + r][s0 + 4 * g + q];
+                gr[q] = 4 * g + q == r ? 1.f : 0.f;
+                gi[q] = 0.f;
+            }
+            for (int j = 0; j < 16; ++j) {
+                const int jq = j & 3, jl = j >> 2;
+                float vr = fr[0], vi = fi[0];                 // this lane's entry in column j's slot
+#pragma unroll
+                for (int q = 1; q < 4; ++q)
+                    if (jq == q) { vr = fr[q]; vi = fi[q]; }
+                const float dr = __shfl(vr, 4 * j + jl), di = __shfl(vi, 4 * j + jl);
+                if (t == 0 && !(isfinite(dr) && isfinite(di) && (dr != 0.f || di != 0.f))) *bad = 1;
+                const float cr = __shfl(vr, 4 * r + jl), ci = __shfl(vi, 4 * r + jl);   // A[r][j]
+                float ur[4], ui[4], wr[4], wi[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    ur[q] = __shfl(vr, 4 * (4 * g + q) + jl);                        // A[c][j]
+                    ui[q] = __shfl(vi, 4 * (4 * g + q) + jl);
+                    wr[q] = __shfl(gr[q], 4 * j + g);                                 // X[j][c]
+                    wi[q] = __shfl(gi[q], 4 * j + g);
+                }
+                if (r > j) {
+                    float qr, qi;
+                    cinv(dr, di, qr, qi);
+                    const float lr = cr * qr - ci * qi, li = cr * qi + ci * qr;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = 4 * g + q;
+                        if (c > j && c <= r) {
+                            fr[q] -= lr * ur[q] - li * ui[q];
+                            fi[q] -= lr * ui[q] + li * ur[q];
+                        }
+                        if (c <= j) {
+                            gr[q] -= lr * wr[q] - li * wi[q];
+                            gi[q] -= lr * wi[q] + li * wr[q];
+                        }
+                        if (c == j) { fr[q] = lr; fi[q] = li; }
+                    }
+                }
             }
 #pragma unroll
-            for (int s = 0; s < 16; ++s) { rr[buf][c0 + 4 * s] = xr[s]; ri[buf][c0 + 4 * s] = xi[s]; }
+            for (int q = 0; q < 4; ++q) {
+                Ar[s0 + r][s0 + 4 * g + q] = fr[q];
+                Ai[s0 + r][s0 + 4 * g + q] = fi[q];
+                Xr[s0 + r][s0 + 4 * g + q] = gr[q];
+                Xi[s0 + r][s0 + 4 * g + q] = gi[q];
+            }
         }
         __syncthreads();
-        const float dr = pv[0], di = pv[1];
-        if (t == 0 && !(isfinite(dr) && isfinite(di) && (dr != 0.f || di != 0.f))) *bad = 1;
-        if (live && i > j && colj) {
-            float vr = 0.f, vi = 0.f;
+        const int rows = LB - s0 - 16;
+        if (rows == 0) break;
+        // (b) W[r][c] = sum_u A[s0 + 16 + r][s0 + u] X[s0 + c][s0 + u] (X_S holds zeros above its
+        // diagonal, so every sum runs over all 16 terms, unrolled)
+        for (int e = t; e < rows * 16; e += 256) {
+            const int r = e >> 4, c = e & 15;
+            float wr = 0.f, wi = 0.f;
 #pragma unroll
-            for (int s = 0; s < 16; ++s)
-                if (s == sj) { vr = ar[s]; vi = ai[s]; }
+            for (int u = 0; u < 16; ++u) {
+                const float ar = Ar[s0 + 16 + r][s0 + u], ai = Ai[s0 + 16 + r][s0 + u];
+                const float xr = Xr[s0 + c][s0 + u], xi = Xi[s0 + c][s0 + u];
+                wr += ar * xr - ai * xi;
+                wi += ar * xi + ai * xr;
+            }
+            Wr[r][c] = wr;
+            Wi[r][c] = wi;
+        }
+        __syncthreads();
+        for (int e = t; e < rows * 16; e += 256) {
+            const int r = e >> 4, c = e & 15;
             float qr, qi;
-            cinv(dr, di, qr, qi);
-            const float l_r = vr * qr - vi * qi, l_i = vr * qi + vi * qr;
-#pragma unroll
-            for (int s = 0; s < 16; ++s)
-                if (s == sj) { ar[s] = l_r; ai[s] = l_i; }
-            lr[i] = l_r; li[i] = l_i;
-            cr[i] = vr; ci[i] = vi;
+            cinv(Ar[s0 + c][s0 + c], Ai[s0 + c][s0 + c], qr, qi);
+            const float wr = Wr[r][c], wi = Wi[r][c];
+            Ar[s0 + 16 + r][s0 + c] = wr * qr - wi * qi;
+            Ai[s0 + 16 + r][s0 + c] = wr * qi + wi * qr;
         }
         __syncthreads();
-        if (live && i > j) {
-            const float l_r = lr[i], l_i = li[i];
+        // (c)
+        for (int e = t; e < rows * rows; e += 256) {
+            const int r = e / rows, c = e % rows;
+            if (c > r) continue;
+            float ur = 0.f, ui = 0.f;
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int c = c0 + 4 * s;
-                if (c > j && c <= i) {
-                    ar[s] -= l_r * cr[c] - l_i * ci[c];
-                    ai[s] -= l_r * ci[c] + l_i * cr[c];
-                }
-                if (c <= j) {
-                    const float ur = rr[buf][c], ui = ri[buf][c];
-                    xr[s] -= l_r * ur - l_i * ui;
-                    xi[s] -= l_r * ui + l_i * ur;
-                }
+            for (int u = 0; u < 16; ++u) {
+                const float lr = Ar[s0 + 16 + r][s0 + u], li = Ai[s0 + 16 + r][s0 + u];
+                const float wr = Wr[c][u], wi = Wi[c][u];
+                ur += lr * wr - li * wi;
+                ui += lr * wi + li * wr;
             }
+            Ar[s0 + 16 + r][s0 + 16 + c] -= ur;
+            Ai[s0 + 16 + r][s0 + 16 + c] -= ui;
         }
+        __syncthreads();
     }
+    // X's block rows I = 1, 2, 3: T = sum_{u < 16 I} L[16 I + r][u] X[u][c] (c < 16 I), then
+    // X[16 I + r][c] = -sum_v X[16 I + r][16 I + v] T[v][c] (X is stored with its zeros, so the
+    // sums run over whole 16-blocks, unrolled).
+    for (int I = 16; I < LB; I += 16) {
+        for (int e = t; e < 16 * I; e += 256) {
+            const int r = e / I, c = e % I;
+            float ur = 0.f, ui = 0.f;
+            for (int u0 = 0; u0 < I; u0 += 16)
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int c = c0 + 4 * s;
-        if (live && c <= i) { rowr[c] = ar[s]; rowi[c] = ai[s]; }
-        Lir[i * LB + c] = xr[s];
-        Lii[i * LB + c] = xi[s];
+            for (int u = u0; u < u0 + 16; ++u) {
+                const float lr = Ar[I + r][u], li = Ai[I + r][u];
+                const float xr = Xr[u][c], xi = Xi[u][c];
+                ur += lr * xr - li * xi;
+                ui += lr * xi + li * xr;
+            }
+            Tr[r][c] = ur;
+            Ti[r][c] = ui;
+        }
+        __syncthreads();
+        for (int e = t; e < 16 * I; e += 256) {
+            const int r = e / I, c = e % I;
+            float ur = 0.f, ui = 0.f;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const float xr = Xr[I + r][I + v], xi = Xi[I + r][I + v];
+                ur += xr * Tr[v][c] - xi * Ti[v][c];
+                ui += xr * Ti[v][c] + xi * Tr[v][c];
+            }
+            Xr[I + r][c] = -ur;
+            Xi[I + r][c] = -ui;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < LB * LB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        if (r < b && c <= r) {
+            Fr[(size_t)(k + r) * ldf + k + c] = Ar[r][c];
+            Fi[(size_t)(k + r) * ldf + k + c] = Ai[r][c];
+        }
+        Lir[e] = Xr[r][c];
+        Lii[e] = Xi[r][c];
     }
 }
 
@@ -230,16 +329,25 @@ __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, lo
             }
 }
 
-// A22 -= L21 W^T on the lower triangle below panel k: one 64 x 64 tile (ti, tj), tj <= ti, per
-// workgroup, blockIdx.x = ti (ti + 1) / 2 + tj.
+// A22 -= L21 W^T on the lower triangle below panel k, one 64 x 64 tile (ti, tj), tj <= ti, per
+// workgroup. part 0: every tile (blockIdx.x = ti (ti + 1) / 2 + tj); part 1: the first tile
+// column (ti = blockIdx.x, tj = 0: the next panel's columns); part 2: the rest (tj >= 1, the
+// triangle of part 0 over ti - 1, tj - 1).
 __global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, long long ldf, int n, int k,
-                                                        const float *Wr, const float *Wi) {
+                                                        const float *Wr, const float *Wi, int part) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
     const int id = blockIdx.x;
-    int ti = (int)((sqrtf(8.f * (float)id + 1.f) - 1.f) * 0.5f);
-    while ((ti + 1) * (ti + 2) / 2 <= id) ++ti;
-    while (ti * (ti + 1) / 2 > id) --ti;
-    const int tj = id - ti * (ti + 1) / 2;
+    int ti, tj;
+    if (part == 1) {
+        ti = id;
+        tj = 0;
+    } else {
+        ti = (int)((sqrtf(8.f * (float)id + 1.f) - 1.f) * 0.5f);
+        while ((ti + 1) * (ti + 2) / 2 <= id) ++ti;
+        while (ti * (ti + 1) / 2 > id) --ti;
+        tj = id - ti * (ti + 1) / 2;
+        if (part == 2) { ++ti; ++tj; }
+    }
     const int m0 = k + LB, I0 = m0 + LB * ti, C0 = m0 + LB * tj;
     TileAcc acc;
     tile_abt(Fr + (size_t)I0 * ldf + k, Fi + (size_t)I0 * ldf + k, ldf, min(LB, n - I0), Wr + (size_t)C0 * LB,
@@ -260,26 +368,52 @@ __global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, l
 
 // ---- solves: L y = b, z = D^-1 y, L^T x = z, P complex columns (n, 2P) FP64 ----
 
-// Forward step over block k0 .. k0 + bk: every workgroup forms y_k = X_k b_k (X_k the stored
-// inverse of L_kk), workgroup 0 writes it to Ys, and each workgroup subtracts L[r, block] y_k from
-// its rows r >= k0 + bk of Y (a thread per row).
+// Forward step over block k0 .. k0 + bk. Every workgroup stages the block's stored inverse X_k in
+// LDS and forms y_k = X_k b_k (workgroup 0 writes it to Ys). Then a workgroup takes 64 rows
+// r >= k0 + bk (lane = row) and its four waves the four 16-column quarters of L[r, block]; the
+// waves' sums of L[r, t] y_k[t] meet in LDS and wave 0 subtracts them from Y in wave order.
 template <int P>
 __global__ __launch_bounds__(256) void ldl_forward_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
                                                          long long ldf, const float *__restrict__ Xr,
                                                          const float *__restrict__ Xi, int n, int k0, int bk, double *Y,
                                                          double *Ys) {
     constexpr int NR = 2 * P;
-    __shared__ double bs[LB][NR], ys[LB][NR];
-    for (int e = threadIdx.x; e < LB * NR; e += 256) {
+    __shared__ float xs[2][LB][LB + 1];
+    __shared__ double bs[LB][NR], ys[LB][NR], part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // this thread's quarter of its row of L and (wave 0) the row of Y, loaded first: they do not
+    // depend on y_k, so their latency overlaps the staging and the small product
+    const int r = k0 + bk + blockIdx.x * LB + lane;   // (rows below the block exist only when bk = 64)
+    float lre[16], lim[16];
+    double yr[P], yi[P];
+    if (r < n) {
+        const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)r * ldf + k0 + 16 * wv);
+        const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)r * ldf + k0 + 16 * wv);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) {
+            const float4 a = lr4[t4], b = li4[t4];
+            lre[4 * t4] = a.x; lre[4 * t4 + 1] = a.y; lre[4 * t4 + 2] = a.z; lre[4 * t4 + 3] = a.w;
+            lim[4 * t4] = b.x; lim[4 * t4 + 1] = b.y; lim[4 * t4 + 2] = b.z; lim[4 * t4 + 3] = b.w;
+        }
+        if (wv == 0)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { yr[p] = Y[(size_t)r * NR + p]; yi[p] = Y[(size_t)r * NR + P + p]; }
+    }
+    for (int e = tid; e < LB * LB; e += 256) {
+        xs[0][e >> 6][e & 63] = Xr[e];
+        xs[1][e >> 6][e & 63] = Xi[e];
+    }
+    for (int e = tid; e < LB * NR; e += 256) {
         const int r = e / NR, q = e % NR;
         bs[r][q] = r < bk ? Y[(size_t)(k0 + r) * NR + q] : 0.0;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < LB * P; e += 256) {
+    for (int e = tid; e < LB * P; e += 256) {
         const int i = e / P, p = e % P;
         double sr = 0, si = 0;
-        for (int t = 0; t <= i; ++t) {
-            const double lr = Xr[i * LB + t], li = Xi[i * LB + t];
+#pragma unroll 16
+        for (int t = 0; t < LB; ++t) {                // X_k is stored with its zeros above the diagonal
+            const double lr = xs[0][i][t], li = xs[1][i][t];
             sr += lr * bs[t][p] - li * bs[t][P + p];
             si += lr * bs[t][P + p] + li * bs[t][p];
         }
@@ -291,31 +425,34 @@ __global__ __launch_bounds__(256) void ldl_forward_kernel(const float *__restric
         }
     }
     __syncthreads();
-    for (int r = k0 + bk + blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
-        double ar[P], ai[P];
+    double ar[P], ai[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
-        const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)r * ldf + k0);
-        const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)r * ldf + k0);
-#pragma unroll 2
-        for (int t4 = 0; t4 < LB / 4; ++t4) {
-            const float4 a = lr4[t4], b = li4[t4];
-            const float lre[4] = {a.x, a.y, a.z, a.w}, lim[4] = {b.x, b.y, b.z, b.w};
+    for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+    if (r < n) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = 4 * t4 + u;
-                const double xr = lre[u], xi = lim[u];
+        for (int u = 0; u < 16; ++u) {
+            const int t = 16 * wv + u;
+            const double xr = lre[u], xi = lim[u];
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    ar[p] += xr * ys[t][p] - xi * ys[t][P + p];
-                    ai[p] += xr * ys[t][P + p] + xi * ys[t][p];
-                }
+            for (int p = 0; p < P; ++p) {
+                ar[p] += xr * ys[t][p] - xi * ys[t][P + p];
+                ai[p] += xr * ys[t][P + p] + xi * ys[t][p];
             }
         }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+    __syncthreads();
+    if (wv == 0 && r < n) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            Y[(size_t)r * NR + p] -= ar[p];
-            Y[(size_t)r * NR + P + p] -= ai[p];
+            Y[(size_t)r * NR + p] = yr[p] - ar[p];
+            Y[(size_t)r * NR + P + p] = yi[p] - ai[p];
         }
     }
 }
@@ -334,26 +471,50 @@ __global__ void ldl_scale_kernel(const float *Fr, const float *Fi, long long ldf
     Z[(size_t)r * NR + P + p] = (yi * dr - yr * di) / den;
 }
 
-// Backward step over block k0 .. k0 + bk: x_k = X_k^T z_k (workgroup 0 writes it to Xo), then each
-// workgroup subtracts sum_t L[k0 + t][r] x_k[t] from its rows r < k0 of Z (a thread per row, the
-// block's rows read across threads).
+// Backward step over block k0 .. k0 + bk: every workgroup stages X_k in LDS and forms
+// x_k = X_k^T z_k (workgroup 0 writes it to Xo). Then a workgroup takes 64 rows r < k0 (lane = r)
+// and its four waves the four 16-row quarters of the block's rows: L[k0 + t][r] read across lanes,
+// the waves' sums of L[k0 + t][r] x_k[t] meet in LDS and wave 0 subtracts them from Z.
 template <int P>
 __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
                                                           long long ldf, const float *__restrict__ Xr,
                                                           const float *__restrict__ Xi, int k0, int bk, double *Z,
                                                           double *Xo) {
     constexpr int NR = 2 * P;
-    __shared__ double zs[LB][NR], xs[LB][NR];
-    for (int e = threadIdx.x; e < LB * NR; e += 256) {
+    __shared__ float xsm[2][LB][LB + 1];
+    __shared__ double zs[LB][NR], xs[LB][NR], part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // this thread's 16 entries of column r of the block's rows and (wave 0) the row of Z, loaded
+    // first (they do not depend on x_k)
+    const int r = blockIdx.x * LB + lane, t0 = 16 * wv, t1 = min(bk, t0 + 16);
+    float lre[16], lim[16];
+    double zr[P], zi[P];
+    if (r < k0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const bool in = t0 + u < t1;
+            lre[u] = in ? Fr[(size_t)(k0 + t0 + u) * ldf + r] : 0.f;
+            lim[u] = in ? Fi[(size_t)(k0 + t0 + u) * ldf + r] : 0.f;
+        }
+        if (wv == 0)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { zr[p] = Z[(size_t)r * NR + p]; zi[p] = Z[(size_t)r * NR + P + p]; }
+    }
+    for (int e = tid; e < LB * LB; e += 256) {
+        xsm[0][e >> 6][e & 63] = Xr[e];
+        xsm[1][e >> 6][e & 63] = Xi[e];
+    }
+    for (int e = tid; e < LB * NR; e += 256) {
         const int r = e / NR, q = e % NR;
         zs[r][q] = r < bk ? Z[(size_t)(k0 + r) * NR + q] : 0.0;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < LB * P; e += 256) {
-        const int c = e / P, p = e % P;
+    for (int e = tid; e < LB * P; e += 256) {
+        const int c = e % LB, p = e / LB;
         double sr = 0, si = 0;
-        for (int t = c; t < bk; ++t) {
-            const double lr = Xr[t * LB + c], li = Xi[t * LB + c];
+#pragma unroll 16
+        for (int t = 0; t < LB; ++t) {                // zeros above X_k's diagonal and past bk in zs
+            const double lr = xsm[0][t][c], li = xsm[1][t][c];
             sr += lr * zs[t][p] - li * zs[t][P + p];
             si += lr * zs[t][P + p] + li * zs[t][p];
         }
@@ -365,22 +526,34 @@ __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restri
         }
     }
     __syncthreads();
-    for (int r = blockIdx.x * 256 + threadIdx.x; r < k0; r += gridDim.x * 256) {
-        double ar[P], ai[P];
+    double ar[P], ai[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
-        for (int t = 0; t < bk; ++t) {
-            const double xr = Fr[(size_t)(k0 + t) * ldf + r], xi = Fi[(size_t)(k0 + t) * ldf + r];
+    for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+    if (r < k0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int t = t0 + u;                     // entries past bk are zero, and xs rows past bk too
+            const double xr = lre[u], xi = lim[u];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 ar[p] += xr * xs[t][p] - xi * xs[t][P + p];
                 ai[p] += xr * xs[t][P + p] + xi * xs[t][p];
             }
         }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+    __syncthreads();
+    if (wv == 0 && r < k0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            Z[(size_t)r * NR + p] -= ar[p];
-            Z[(size_t)r * NR + P + p] -= ai[p];
+            Z[(size_t)r * NR + p] = zr[p] - ar[p];
+            Z[(size_t)r * NR + P + p] = zi[p] - ai[p];
         }
     }
 }
@@ -412,20 +585,40 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
     PQ_HIP(hipGetLastError());
 }
 
+// Lookahead over two streams. On st, per panel k: the diagonal block, the panel (W into buffer
+// k mod 2), then (after the previous panel's bulk update) the first tile column of the trailing
+// update, which is all the next panel reads. The bulk (tile columns >= 1) goes to f.aux after that
+// and runs beside the next panel's diagonal block and panel; the next first-column update waits
+// for it, since both write that tile column. A W buffer is rewritten two panels later, after the
+// bulk update that read it has been waited for.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
     const int n = f.n;
-    for (int k = 0; k < n; k += LB) {
+    auto diag = [&](int k) {
         const int b = std::min(LB, n - k);
-        float *xr = f.Lir + (size_t)(k / LB) * LB * LB, *xi = f.Lii + (size_t)(k / LB) * LB * LB;
-        hipLaunchKernelGGL(ldl_diag_kernel, dim3(1), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b, xr, xi, bad);
-        if (k + b < n) {
-            const int tiles = (n - k - LB + LB - 1) / LB;
-            hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k, xr, xi, f.Wr,
-                               f.Wi);
-            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)tiles * (tiles + 1) / 2)), dim3(256), 0, st,
-                               f.Fr, f.Fi, f.ldf, n, k, f.Wr, f.Wi);
+        hipLaunchKernelGGL(ldl_diag_kernel, dim3(1), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b,
+                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, bad);
+    };
+    if (n <= 0) return;
+    diag(0);
+    bool bulk = false;                                   // a bulk update is pending on f.aux
+    for (int k = 0, p = 0; k + LB < n; k += LB, ++p) {
+        const int tiles = (n - k - LB + LB - 1) / LB;
+        float *wr = f.Wr[p & 1], *wi = f.Wi[p & 1];
+        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
+                           f.Lir + (size_t)p * LB * LB, f.Lii + (size_t)p * LB * LB, wr, wi);
+        if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
+        hipLaunchKernelGGL(ldl_update_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k, wr, wi, 1);
+        bulk = tiles > 1;
+        if (bulk) {
+            PQ_HIP(hipEventRecord(f.ready, st));
+            PQ_HIP(hipStreamWaitEvent(f.aux, f.ready, 0));
+            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(tiles - 1) * tiles / 2)), dim3(256), 0,
+                               f.aux, f.Fr, f.Fi, f.ldf, n, k, wr, wi, 2);
+            PQ_HIP(hipEventRecord(f.done, f.aux));
         }
+        diag(k + LB);
     }
+    if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
     PQ_HIP(hipGetLastError());
 }
 
@@ -438,14 +631,14 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
         constexpr int PP = decltype(pc)::value;
         for (int kb = 0; kb < nb; ++kb) {
             const int k0 = kb * LB, bk = std::min(LB, n - k0);
-            const int rows = n - k0 - bk, grid = std::max(1, (rows + 255) / 256);
+            const int rows = n - k0 - bk, grid = std::max(1, (rows + LB - 1) / LB);
             hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
                                f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys);
         }
         hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256)), dim3(256), 0, st, f.Fr,
                            f.Fi, f.ldf, n, PP, Ys, Y);
         for (int kb = nb - 1; kb >= 0; --kb) {
-            const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + 255) / 256);
+            const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + LB - 1) / LB);
             hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
                                f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X);
         }

@@ -9,19 +9,27 @@ constexpr int LDL_BLOCK = 64;   // panel width, diagonal block, trailing-update 
 
 // The factor's storage: planes Fr, Fi (n x ldf FP32, row-major, ldf a multiple of 64; the lower
 // triangle holds L below the diagonal and D on it), the inverses of the unit lower diagonal blocks
-// Lir, Lii (ceil(n / 64) blocks of 64 x 64), the trailing update's right factor Wr, Wi (n x 64).
+// Lir, Lii (ceil(n / 64) blocks of 64 x 64), the trailing update's right factor Wr, Wi (two
+// buffers of n x 64, alternating between panels). The factorisation also uses a second stream and
+// two events of the caller's (the lookahead: the bulk of one panel's trailing update runs beside
+// the next panel's diagonal block and panel).
 struct LdlFactor {
     float *Fr, *Fi;
     long long ldf;
     int n;
-    float *Lir, *Lii, *Wr, *Wi;
+    float *Lir, *Lii;
+    float *Wr[2], *Wi[2];
+    hipStream_t aux;
+    hipEvent_t ready, done;
 };
 
 // F = Rl + j omega Ll on the first n loops (lower triangle): Ll FP32 (row stride ldl), Rl CSR FP64.
 void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl, const int *rptr, const int *rcol,
               const double *rval, double omega);
 
-// Factor in place; *bad (device int, cleared by the caller) is set when a pivot is zero or not finite.
+// Factor in place on stream st (the work is complete in st's order when it returns; f.aux has no
+// pending work left behind); *bad (device int, cleared by the caller) is set when a pivot is zero or
+// not finite.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
 
 // X = Z^-1 B for P complex columns (n, 2P) FP64 (real parts first), FP32 factor, FP64 arithmetic.

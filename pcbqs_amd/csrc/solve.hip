@@ -1006,6 +1006,8 @@ struct pq_loops {
     // the direct solve's L D L^T factor (planes of the factor, block inverses, panel products) and
     // the frequency it was formed at
     DevBuf Fldl, Xldl, Wldl;
+    hipStream_t aux = nullptr;    // the factorisation's second stream and its two events
+    hipEvent_t ev_ready = nullptr, ev_done = nullptr;
     long long ldf = 0;
     double omega_ldl = -1;
     double factor_ms = 0;
@@ -1645,6 +1647,10 @@ int pq_loops_destroy(pq_loops *L) {
         drop_graphs(L);
         L->Hd.release();
         if (L->Hh) (void)hipHostFree(L->Hh);
+        if (L->aux) (void)hipStreamSynchronize(L->aux);
+        if (L->ev_ready) (void)hipEventDestroy(L->ev_ready);
+        if (L->ev_done) (void)hipEventDestroy(L->ev_done);
+        if (L->aux) (void)hipStreamDestroy(L->aux);
         if (L->stream) (void)hipStreamDestroy(L->stream);
         delete L;
     });
@@ -1956,10 +1962,17 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
             L->ldf = (long long)nblk * LDL_BLOCK;
             L->Fldl.reserve((size_t)2 * n * L->ldf * 4);
             L->Xldl.reserve((size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK * 4);
-            L->Wldl.reserve((size_t)2 * n * LDL_BLOCK * 4);
+            L->Wldl.reserve((size_t)4 * n * LDL_BLOCK * 4);
+            if (!L->aux) {
+                PQ_HIP(hipStreamCreateWithFlags(&L->aux, hipStreamNonBlocking));
+                PQ_HIP(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
+                PQ_HIP(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
+            }
+            const size_t wsz = (size_t)n * LDL_BLOCK;
+            float *Wb = L->Wldl.as<float>();
             LdlFactor f{L->Fldl.as<float>(), L->Fldl.as<float>() + (size_t)n * L->ldf, L->ldf, n,
                         L->Xldl.as<float>(), L->Xldl.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,
-                        L->Wldl.as<float>(), L->Wldl.as<float>() + (size_t)n * LDL_BLOCK};
+                        {Wb, Wb + 2 * wsz}, {Wb + wsz, Wb + 3 * wsz}, L->aux, L->ev_ready, L->ev_done};
             if (L->omega_ldl != omega) {
                 L->omega_ldl = -1;
                 hipEvent_t e0, e1;

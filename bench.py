@@ -10,7 +10,9 @@ compiled static_fill (built into oracle/_ref) on the same mesh, on this host's c
 
 The GMRES part solves the board's 4 ports at 1 MHz to a relative residual of 1e-8 with plain
 block-Jacobi GMRES and with the learned two-level preconditioner (seconds, iterations, ratio),
-and times the plain path's operator product on the host cores.
+reports the direct solve at the same frequency beside them (FP32 L D L^T on the device refined to
+1e-12: factorisation time and rate, solve time), and times the plain path's operator product on
+the host cores.
 """
 import argparse
 import json
@@ -86,6 +88,21 @@ def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
         i = int(np.argsort(secs)[len(secs) // 2])
         return out[i][0][0], dict(out[i][1][0], first_solve_s=secs[0])
     Zp, plain = run('block-jacobi')
+    # the direct solve at the same frequency: the first call forms and factors Z_ff (FP32
+    # L D L^T, n^3 / 6 complex multiply-adds = 4 n^3 / 3 flop), the repeats reuse the factor
+    dout = [solver.ports([freq], solver='direct') for _ in range(repeats)]
+    dinfo = [o[1][0] for o in dout]
+    nf = solver.model.n_free
+    factor_ms = dinfo[0]['factor_ms']
+    direct = dict(
+        method='FP32 L D L^T of Rl + j omega Ll (64-wide panels, MFMA trailing updates, no pivoting), '
+               'FP64 refinement to 1e-12',
+        first_solve_s=dinfo[0]['seconds'], seconds=float(np.median([d['seconds'] for d in dinfo[1:]]))
+        if repeats > 1 else dinfo[0]['seconds'], factor_ms=factor_ms, corrections=dinfo[0]['iterations'],
+        relative_residual=dinfo[0]['relative_residual'],
+        factor_tflops=4 * nf ** 3 / 3 / (factor_ms * 1e-3) / 1e12 if factor_ms > 0 else None,
+        fp32_mfma_peak_tflops=157.3,
+        port_z_rel_diff_vs_plain=float(np.abs(dout[0][0][0] - Zp).max() / np.abs(Zp).max()))
     t = time.perf_counter()
     net = MPNN.load()
     net_s = set_learned_coarse(solver, net, grid)
@@ -103,7 +120,7 @@ def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
                      board_setup_s=coarse_setup, network_s=net_s),
         ratio=learned['seconds'] / plain['seconds'],
         port_z_rel_diff=float(np.abs(Zl - Zp).max() / np.abs(Zp).max()),
-        board_setup_s=board_setup)
+        board_setup_s=board_setup, direct=direct)
     chunk, us = solver.tuning()
     n = solver.model.n_free
     section['operator_product'] = dict(

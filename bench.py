@@ -129,7 +129,10 @@ def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
         algorithmic_bytes=4 * n * n, achieved_GBs=4 * n * n / (us * 1e-6) / 1e9,
         peak_GBs=HBM_PEAK / 1e9, frac=4 * n * n / (us * 1e-6) / HBM_PEAK if us > 0 else None)
     if cpu:
-        section['cpu_operator'] = cpu_operator_sample(solver, plain['iterations'])
+        L, R = host_loop_matrices(solver)
+        section['cpu_operator'] = cpu_operator_sample(solver, plain['iterations'], L=L, R=R)
+        section['direct']['cpu'] = cpu_direct_sample(L, R, 2 * np.pi * freq)
+        del L, R
     solver.close()
     return section
 
@@ -147,20 +150,49 @@ def sweep_section(n_boards, n_freqs=20, concurrency=1):
     return total
 
 
-def cpu_operator_sample(solver, iterations, products=5):
-    """The plain path's operator product (Z_ff on 2P real columns, FP64 numpy on the host cores)
-    timed over a few products and scaled to the plain solve's iteration count: a bounded sample of
-    the CPU GMRES, whose time is ~98% operator products."""
+def host_loop_matrices(solver):
+    """The free-loop block of Ll (FP64 copy of the device's FP32 matrix) and Rl (CSR) on the host."""
     import ctypes
     from pcbqs_amd._lib import call
-    n, P = solver.model.n_free, solver.n_ports
+    n = solver.model.n_free
     dev, ld = ctypes.c_void_p(), ctypes.c_int64()
     call('pq_loops_matrix', solver._handle, ctypes.byref(dev), ctypes.byref(ld))
     L32 = np.empty((n, ld.value), np.float32)
     call('pq_memcpy_d2h', L32.ctypes.data_as(ctypes.c_void_p), dev, L32.nbytes)
     L = np.ascontiguousarray(L32[:, :n], np.float64)
     del L32
-    R = solver.model.Rl.tocsr()[:n, :n]
+    return L, solver.model.Rl.tocsr()[:n, :n]
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except OSError:
+        return os.cpu_count() or 1
+
+
+def cpu_direct_sample(L, R, omega):
+    """The reference's dense path on the host: LAPACK's single-precision complex LU (scipy's
+    lu_factor, as MixedLU, src/pcbqs/kernels.py:143-148) of the same Z_ff, timed once."""
+    from scipy.linalg import lu_factor
+    n = L.shape[0]
+    Z = np.asarray(R.todense(), np.complex64)
+    Z += np.complex64(1j * omega) * L.astype(np.float32)
+    t = time.perf_counter()
+    lu_factor(Z.T, overwrite_a=True, check_finite=False)
+    seconds = time.perf_counter() - t
+    return dict(factor_s=seconds, cores=host_cores(), kind='reference method (LAPACK cgetrf through scipy)',
+                sample=f'one FP32 complex LU of the {n} x {n} free-loop block at {omega / 2 / np.pi:g} Hz '
+                       f'(the refinement, a few solves and products, is not timed)')
+
+
+def cpu_operator_sample(solver, iterations, products=5, L=None, R=None):
+    """The plain path's operator product (Z_ff on 2P real columns, FP64 numpy on the host cores)
+    timed over a few products and scaled to the plain solve's iteration count: a bounded sample of
+    the CPU GMRES, whose time is ~98% operator products."""
+    n, P = solver.model.n_free, solver.n_ports
+    if L is None:
+        L, R = host_loop_matrices(solver)
     X = np.random.default_rng(0).normal(size=(n, 2 * P))
     omega = 2 * np.pi * 1e6
     L @ X
@@ -170,11 +202,7 @@ def cpu_operator_sample(solver, iterations, products=5):
         Y[:, :P] -= omega * (L @ X[:, P:])
         Y[:, P:] += omega * (L @ X[:, :P])
     per = (time.perf_counter() - t) / products
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except OSError:
-        cores = os.cpu_count() or 1
-    return dict(seconds_per_product=per, estimated_solve_s=per * iterations, cores=cores, kind='port',
+    return dict(seconds_per_product=per, estimated_solve_s=per * iterations, cores=host_cores(), kind='port',
                 sample=f'{products} FP64 operator products (n={n}, {2 * P} real columns) with numpy, '
                        f'scaled to the {iterations} iterations of the plain solve')
 

@@ -736,10 +736,10 @@ __global__ void vec_scale_norm_kernel(const double *X, const double *nrm, int n,
 // split over its four waves; a row of V is wave-uniform (scalar loads). The four waves' sums are
 // added in wave order.
 constexpr int cr_rows(int NR) { return NR >= 8 ? 4096 / NR : 512; }
-template <int NR>
-__global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__restrict__ Phi, int k, int kp,
+template <int NR, class T>
+__global__ __launch_bounds__(256) void coarse_restrict_kernel(const T *__restrict__ Phi, int k, int kp,
                                                              const double *__restrict__ V, int n, int rows,
-                                                             double *__restrict__ partial) {
+                                                             double scale, double *__restrict__ partial) {
     constexpr int U = 16;
     __shared__ double red[4 * 64 * NR];
     const int i0 = blockIdx.x * rows, m = min(rows, n - i0);
@@ -748,29 +748,29 @@ __global__ __launch_bounds__(256) void coarse_restrict_kernel(const float *__res
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
-    const float *col = Phi + (size_t)i0 * kp + min(c, k - 1);
+    const T *col = Phi + (size_t)i0 * kp + min(c, k - 1);
     const double *v = V + (size_t)i0 * NR;
     int i = w;
     for (; i + 4 * (U - 1) < m; i += 4 * U) {
-        float f[U];
+        T f[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) f[u] = col[(size_t)(i + 4 * u) * kp];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const double fu = f[u];
+            const double fu = (float)f[u];
             const double *vr = v + (size_t)(i + 4 * u) * NR;
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] += fu * vr[r];
         }
     }
     for (; i < m; i += 4) {
-        const double f = col[(size_t)i * kp];
+        const double f = (float)col[(size_t)i * kp];
         const double *vr = v + (size_t)i * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] += f * vr[r];
     }
 #pragma unroll
-    for (int r = 0; r < NR; ++r) red[(w * 64 + lane) * NR + r] = acc[r];
+    for (int r = 0; r < NR; ++r) red[(w * 64 + lane) * NR + r] = acc[r] * scale;
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * NR; e += 256) {
         const int cl = e / NR, r = e % NR;
@@ -840,12 +840,12 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
 // one pass against the coarse vector c (k, NR), thread per row, a chunk of coarse columns per
 // blockIdx.y, U columns of each basis (3 U loads) in flight; c wave-uniform (scalar loads).
 // Chunk partials: PY = Phi c and PA = (Rl + j w Ll) Phi c (real parts in columns 0..P-1).
-template <int NR, int U>
-__global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ B0, const float *__restrict__ B1,
-                                                     const float *__restrict__ B2, int n, int k,
+template <int NR, int U, class T>
+__global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, const T *__restrict__ B1,
+                                                     const T *__restrict__ B2, int n, int k,
                                                      const double *__restrict__ C, int chunk,
-                                                     const double *__restrict__ omega_dev, double *__restrict__ PY,
-                                                     double *__restrict__ PA) {
+                                                     const double *__restrict__ omega_dev, double s0, double s1,
+                                                     double s2, double *__restrict__ PY, double *__restrict__ PA) {
     constexpr int P = NR / 2;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(k, c0 + chunk);
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ 
     const size_t rr = (size_t)min(row, n - 1);
     int c = c0;
     for (; c + U <= c1; c += U) {
-        float f0[U], f1[U], f2[U];
+        T f0[U], f1[U], f2[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             f0[u] = B0[(size_t)(c + u) * n + rr];
@@ -864,7 +864,7 @@ __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ 
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const double a = f0[u], b = f1[u], d = f2[u];
+            const double a = (float)f0[u], b = (float)f1[u], d = (float)f2[u];
             const double *x = C + (size_t)(c + u) * NR;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
@@ -875,7 +875,8 @@ __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ 
         }
     }
     for (; c < c1; ++c) {
-        const double a = B0[(size_t)c * n + rr], b = B1[(size_t)c * n + rr], d = B2[(size_t)c * n + rr];
+        const double a = (float)B0[(size_t)c * n + rr], b = (float)B1[(size_t)c * n + rr];
+        const double d = (float)B2[(size_t)c * n + rr];
         const double *x = C + (size_t)c * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -886,6 +887,8 @@ __global__ __launch_bounds__(256) void prolong_kernel(const float *__restrict__ 
     }
     if (row < n) {
         const double omega = *omega_dev;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) { ay[r] *= s0; ar[r] *= s1; al[r] *= s2; }
         double *oy = PY + ((size_t)blockIdx.y * n + row) * NR, *oa = PA + ((size_t)blockIdx.y * n + row) * NR;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -919,6 +922,24 @@ __global__ void prolong_epilogue_kernel(const double *__restrict__ PY, const dou
     }
     R[e] = V[e] - a;
     Y[e] = y;
+}
+
+// *mx = max |X[e]| over count entries (the bit patterns of non-negative floats order as unsigned
+// integers, so an integer maximum is exact and independent of order).
+__global__ void absmax_kernel(const float *X, size_t count, unsigned *mx) {
+    unsigned m = 0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(X[e])));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(mx, m);
+}
+
+// Out = In / scale rounded to FP16, for scale = max |In| read from *mx (1 where that is 0).
+__global__ void to_half_kernel(const float *In, size_t count, const unsigned *mx, _Float16 *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const float m = __uint_as_float(*mx);
+    Out[e] = (_Float16)(m > 0.f ? In[e] / m : In[e]);
 }
 
 // Out (k, n) = In (n, k)^T, FP32; In has row stride ld >= k.
@@ -983,6 +1004,11 @@ struct pq_loops {
     int kc = 0, kp = 0;           // coarse columns; row stride of Phi
     bool cdiag = false;           // basis diagonalises the Galerkin matrices
     DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart, ppart;   // RPhi, LPhi transposed (k, n)
+    // FP16 copies of the four bases for the solve's restriction and prolongation, each divided by
+    // its largest magnitude (hscale: Phi, Phi^T, (Rl Phi)^T, (Ll Phi)^T), when coarse_half is set
+    DevBuf Phih, Phith, RPhih, LPhih, hmax;
+    double hscale[4] = {1, 1, 1, 1};
+    bool coarse_half = false;
     std::vector<double> Rc, Lc;
     double omega_c = -1;
     DevBuf Hd;                    // Arnoldi coefficients per step (device)
@@ -1201,11 +1227,16 @@ int prolong_rows(const pq_loops *L, int NR) {
 }
 
 // Launch the restriction Phi^T X into `rows`-row chunk partials; returns the chunk count.
+// half: read the FP16 copy of Phi (and scale by its largest magnitude) instead of Phi.
 template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k, const double *X, int rows,
-                                     double *part) {
+                                     double *part, bool half = false) {
     const int n = L->nf, nchunk = (n + rows - 1) / rows;
-    hipLaunchKernelGGL(coarse_restrict_kernel<N>, dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream, Phi, k, L->kp,
-                       X, n, rows, part);
+    if (half)
+        hipLaunchKernelGGL((coarse_restrict_kernel<N, _Float16>), dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
+                           L->Phih.as<_Float16>(), k, L->kp, X, n, rows, L->hscale[0], part);
+    else
+        hipLaunchKernelGGL((coarse_restrict_kernel<N, float>), dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
+                           Phi, k, L->kp, X, n, rows, 1.0, part);
     return nchunk;
 }
 
@@ -1215,12 +1246,24 @@ template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, int ro
                                     double *PY, double *PA) {
     const int n = L->nf, k = L->kc, pc = (k + ps - 1) / ps, nsplit = (k + pc - 1) / pc;
     const dim3 grid((n + rows - 1) / rows, nsplit);
+    const double *sc = L->hscale;
+    if (L->coarse_half) {
+        const _Float16 *b0 = L->Phith.as<_Float16>(), *b1 = L->RPhih.as<_Float16>(), *b2 = L->LPhih.as<_Float16>();
+        if (depth == 16)
+            hipLaunchKernelGGL((prolong_kernel<N, 16, _Float16>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+                               L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
+        else
+            hipLaunchKernelGGL((prolong_kernel<N, 8, _Float16>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+                               L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
+        return nsplit;
+    }
+    const float *b0 = L->Phit.as<float>(), *b1 = L->RPhi.as<float>(), *b2 = L->LPhi.as<float>();
     if (depth == 16)
-        hipLaunchKernelGGL((prolong_kernel<N, 16>), grid, dim3(rows), 0, L->stream, L->Phit.as<float>(),
-                           L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
+        hipLaunchKernelGGL((prolong_kernel<N, 16, float>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+                           L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     else
-        hipLaunchKernelGGL((prolong_kernel<N, 8>), grid, dim3(rows), 0, L->stream, L->Phit.as<float>(),
-                           L->RPhi.as<float>(), L->LPhi.as<float>(), n, k, L->cvec.as<double>(), pc, omega_dev, PY, PA);
+        hipLaunchKernelGGL((prolong_kernel<N, 8, float>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+                           L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     return nsplit;
 }
 
@@ -1277,7 +1320,8 @@ void tune_coarse(pq_loops *L, int NR) {
         constexpr int N = decltype(nr)::value;
         for (int rows : rows_c) {
             const float t = timed([&] {
-                const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X.as<double>(), rows, part.as<double>());
+                const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X.as<double>(), rows, part.as<double>(),
+                                                      L->coarse_half);
                 hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * (N / 2) + 3) / 4), dim3(256), 0, L->stream,
                                    part.as<double>(), nchunk, k, N / 2, dg.as<double>(), cv.as<double>());
             });
@@ -1307,7 +1351,8 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
     const int n = L->nf, NR = 2 * P, k = L->kc;
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X, restrict_rows(L, N), L->cpart.as<double>());
+        const int nchunk = restrict_launch<N>(L, L->Phi.as<float>(), k, X, restrict_rows(L, N), L->cpart.as<double>(),
+                                              L->coarse_half);
         if (L->cdiag) {
             hipLaunchKernelGGL(coarse_sum_diag_kernel, dim3((k * P + 3) / 4), dim3(256), 0, L->stream,
                                L->cpart.as<double>(), nchunk, k, P, L->Ainv.as<double>(), L->cvec.as<double>());
@@ -1448,6 +1493,7 @@ void check_graphs(pq_loops *L, int P, int precond, int m) {
                                      cv(gemv_chunk(L, 2 * P)), cv(gemv_shape(L, 2 * P).rows), L->V.p, L->W.p, L->Z.p, L->partial.p, L->Hd.p,
                                      L->inv.p, L->T1.p, L->T2.p, L->cpart.p, L->cvec.p, L->ppart.p, L->gpart.p,
                                      L->ctot.p, L->Phi.p, L->Phit.p, L->RPhi.p, L->LPhi.p, L->Ainv.p,
+                                     L->Phih.p, L->Phith.p, L->RPhih.p, L->LPhih.p, cv(L->coarse_half),
                                      L->omega_dev.p, L->Ll.p, L->rptr.p, L->Hh};
     if (key != L->graph_key) {
         drop_graphs(L);
@@ -2119,6 +2165,37 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
                     const double v = 0.5 * ((*G)[(size_t)a * k + b] + (*G)[(size_t)b * k + a]);
                     (*G)[(size_t)a * k + b] = (*G)[(size_t)b * k + a] = v;
                 }
+        // FP16 copies of the bases for the solve's restriction and prolongation, each scaled by its
+        // largest magnitude (the Galerkin matrices above used FP32; PCBQS_AMD_COARSE_FP16=0 keeps the
+        // solve on the FP32 bases). The coarse correction is a preconditioner: its rounding moves the
+        // iterates, not the solution, which is accepted on the true FP64 residual.
+        const char *hv = std::getenv("PCBQS_AMD_COARSE_FP16");
+        L->coarse_half = !(hv && hv[0] == '0');
+        if (L->coarse_half) {
+            L->Phih.reserve((size_t)n * kp * 2); L->Phith.reserve((size_t)n * k * 2);
+            L->RPhih.reserve((size_t)n * k * 2); L->LPhih.reserve((size_t)n * k * 2);
+            L->hmax.reserve(4 * 4);
+            PQ_HIP(hipMemsetAsync(L->hmax.p, 0, 16, L->stream));
+            const float *src[4] = {L->Phi.as<float>(), L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
+            _Float16 *dst[4] = {L->Phih.as<_Float16>(), L->Phith.as<_Float16>(), L->RPhih.as<_Float16>(),
+                                L->LPhih.as<_Float16>()};
+            for (int q = 0; q < 4; ++q) {
+                const size_t cnt = q == 0 ? (size_t)n * kp : (size_t)n * k;
+                hipLaunchKernelGGL(absmax_kernel, dim3(1024), dim3(256), 0, L->stream, src[q], cnt,
+                                   L->hmax.as<unsigned>() + q);
+                hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream, src[q],
+                                   cnt, L->hmax.as<unsigned>() + q, dst[q]);
+            }
+            PQ_HIP(hipGetLastError());
+            unsigned mx[4];
+            PQ_HIP(hipMemcpyAsync(mx, L->hmax.p, 16, hipMemcpyDeviceToHost, L->stream));
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            for (int q = 0; q < 4; ++q) {
+                float m;
+                std::memcpy(&m, &mx[q], 4);
+                L->hscale[q] = m > 0 ? m : 1.0;
+            }
+        }
     });
 }
 

@@ -97,3 +97,26 @@ def test_captured_steps_replay_the_eager_solve_bitwise(precond, operator, monkey
     (Z0, i0), (Z1, i1) = out['0'], out['1']
     assert [r['iterations'] for r in i0] == [r['iterations'] for r in i1]
     assert np.array_equal(Z0, Z1)
+
+
+def test_fp16_coarse_bases_give_the_fp32_bases_solution(monkeypatch):
+    """The two-level solve reads FP16 copies of its coarse bases by default; with the FP32 bases
+    (PCBQS_AMD_COARSE_FP16=0) the port Z agree to the solve tolerance and the iteration counts
+    stay within two of each other, and both match the FP64 oracle."""
+    from pcbqs_amd.solve import PortSolver
+    b = boards.board_c2(1, target=3000)
+    freqs = [1e3, 1e6, 1e8]
+    out = {}
+    for half in ('0', '1'):
+        monkeypatch.setenv('PCBQS_AMD_COARSE_FP16', half)
+        s = PortSolver(b, device=0)
+        s.set_stream_coarse(grid=8)
+        out[half] = s.ports(freqs, precond='two-level', tol=1e-10)
+        s.close()
+    (Z0, i0), (Z1, i1) = out['0'], out['1']
+    assert rel(Z1, Z0) < 1e-8
+    assert all(abs(a['iterations'] - c['iterations']) <= 2 for a, c in zip(i0, i1))
+    Zo = oracle_ports(b, freqs)
+    w = 2 * np.pi * np.array(freqs)[:, None, None]
+    assert rel(Z1.real, Zo.real) < 1e-6
+    assert rel(Z1.imag / w, Zo.imag / w) < 1e-6

@@ -115,7 +115,8 @@ def test_fp16_coarse_bases_give_the_fp32_bases_solution(monkeypatch):
         s.close()
     (Z0, i0), (Z1, i1) = out['0'], out['1']
     assert rel(Z1, Z0) < 1e-8
-    assert all(abs(a['iterations'] - c['iterations']) <= 2 for a, c in zip(i0, i1))
+    its = [(a['iterations'], c['iterations']) for a, c in zip(i0, i1)]
+    assert all(abs(a - c) <= 2 for a, c in its), its
     Zo = oracle_ports(b, freqs)
     w = 2 * np.pi * np.array(freqs)[:, None, None]
     assert rel(Z1.real, Zo.real) < 1e-6

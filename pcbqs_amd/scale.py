@@ -5,7 +5,10 @@ plain block-Jacobi and learned two-level GMRES to 1e-8 at one frequency, and the
 learned solve's port currents at about a million points: a 1 mm grid over the board at heights
 0.5-5 mm above its top layer (FP32 field kernel).
 
-    python -m pcbqs_amd.scale [--seed 0] [--frequency 1e6] [--grid 24] [--heights 69] [--no-field]
+    python -m pcbqs_amd.scale [--seed 0] [--frequency 1e6] [--grid 24] [--heights 69] [--no-field] [--no-direct]
+
+Unless --no-direct, the direct solve (FP32 L D L^T of the free-loop block on the device, FP64
+refinement to 1e-12) runs at the same frequency after the two GMRES solves.
 
 Prints one JSON line of timings, sizes and the agreement of the two solves.
 """
@@ -48,6 +51,7 @@ def main(argv=None):
     ap.add_argument('--grid', type=int, default=24, help='hats per side of the two-level coarse space')
     ap.add_argument('--heights', type=int, default=69, help='field heights between 0.5 and 5 mm')
     ap.add_argument('--no-field', action='store_true')
+    ap.add_argument('--no-direct', action='store_true')
     args = ap.parse_args(argv)
     from . import board as boards
     from .biot import h_field
@@ -92,6 +96,14 @@ def main(argv=None):
         out['ratio'] = il[0]['seconds'] / ip[0]['seconds']
         out['port_z_rel_diff'] = float(np.abs(Zl - Zp).max() / np.abs(Zp).max())
     out['operator_product_us'] = solver.tuning()[1]
+    if not args.no_direct:
+        Zd, idd = solver.ports([f], solver='direct')
+        nf = solver.model.n_free
+        out['direct'] = dict(seconds=idd[0]['seconds'], factor_ms=idd[0]['factor_ms'],
+                             factor_tflops=4 * nf ** 3 / 3 / (idd[0]['factor_ms'] * 1e-3) / 1e12,
+                             corrections=idd[0]['iterations'], relative_residual=idd[0]['relative_residual'],
+                             port_z_rel_diff_vs_learned=float(np.abs(Zd - Zl).max() / np.abs(Zl).max()))
+        progress('direct solve', out['direct'])
     if not args.no_field:
         P = len(b.ports)
         sheets, segments = field_sources(solver.model, X[0][:, :P] + 1j * X[0][:, P:], solver.port_cols)

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, lon
 }
 
 // One wave's 32 x 32 part of a 64 x 64 complex tile product P[i][c] = sum_t A[i][t] B[c][t]
-// (t < 64): waves 2 x 2 over the tile, each 2 x 2 MFMA 16x16x4 FP32 blocks for the real and the
+// (t < kdim, a multiple of 32): waves 2 x 2 over the tile, each 2 x 2 MFMA 16x16x4 FP32 blocks for the real and the
 // imaginary part. The K dimension is staged through LDS in two chunks of 32; in MFMA step s of a
 // chunk the four lane groups g = lane >> 4 supply k = 8 g + s to A and B alike, so each lane reads
 // its eight k values of a row as two 16-byte LDS loads. Rows at or past arows / brows read zero.
@@ -245,7 +245,7 @@ struct TileAcc {
 
 __device__ __attribute__((always_inline)) inline void tile_abt(const float *__restrict__ Ar, const float *__restrict__ Ai, long long lda, int arows,
                                 const float *__restrict__ Br, const float *__restrict__ Bi, long long ldb, int brows,
-                                float (*sA)[LB][TP], float (*sB)[LB][TP], TileAcc &acc) {
+                                int kdim, float (*sA)[LB][TP], float (*sB)[LB][TP], TileAcc &acc) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1, g = lane >> 4, l16 = lane & 15;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -255,7 +255,7 @@ __device__ __attribute__((always_inline)) inline void tile_abt(const float *__re
             acc.i[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int kc = 0; kc < LB; kc += 32) {
+    for (int kc = 0; kc < kdim; kc += 32) {
         for (int e = tid; e < LB * 8; e += 256) {
             // rows past the end load the last valid row (arows, brows >= 1) and are zeroed by value
             const int r = e >> 3, q = (e & 7) * 4, ra = min(r, arows - 1), rb = min(r, brows - 1);
@@ -302,7 +302,8 @@ __device__ __attribute__((always_inline)) inline void tile_abt(const float *__re
 // Below the diagonal block of panel k, rows from I0 = k + 64 + 64 blockIdx.x: W = A21 X^T with
 // X = L11^-1 (so W = A21 L11^-T = L21 D), then L21 = W D^-1 in place of A21.
 __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, long long ldf, int n, int k,
-                                                       const float *Lir, const float *Lii, float *Wr, float *Wi) {
+                                                       const float *Lir, const float *Lii, float *Wr, float *Wi,
+                                                       int ldw) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
     __shared__ float qr[LB], qi[LB];
     const int I0 = k + LB + LB * blockIdx.x;
@@ -311,7 +312,8 @@ __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, lo
         cinv(Fr[dg], Fi[dg], qr[threadIdx.x], qi[threadIdx.x]);
     }
     TileAcc acc;
-    tile_abt(Fr + (size_t)I0 * ldf + k, Fi + (size_t)I0 * ldf + k, ldf, min(LB, n - I0), Lir, Lii, LB, LB, sA, sB, acc);
+    tile_abt(Fr + (size_t)I0 * ldf + k, Fi + (size_t)I0 * ldf + k, ldf, min(LB, n - I0), Lir, Lii, LB, LB, LB, sA, sB,
+             acc);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1, g = lane >> 4, l16 = lane & 15;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -322,36 +324,39 @@ __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, lo
                 const int row = I0 + wr * 32 + a * 16 + 4 * g + q, col = wc * 32 + b * 16 + l16;
                 if (row >= n) continue;
                 const float vr = acc.r[a][b][q], vi = acc.i[a][b][q];
-                Wr[(size_t)row * LB + col] = vr;
-                Wi[(size_t)row * LB + col] = vi;
+                Wr[(size_t)row * ldw + col] = vr;
+                Wi[(size_t)row * ldw + col] = vi;
                 Fr[(size_t)row * ldf + k + col] = vr * qr[col] - vi * qi[col];
                 Fi[(size_t)row * ldf + k + col] = vr * qi[col] + vi * qr[col];
             }
 }
 
-// A22 -= L21 W^T on the lower triangle below panel k, one 64 x 64 tile (ti, tj), tj <= ti, per
-// workgroup. part 0: every tile (blockIdx.x = ti (ti + 1) / 2 + tj); part 1: the first tile
-// column (ti = blockIdx.x, tj = 0: the next panel's columns); part 2: the rest (tj >= 1, the
-// triangle of part 0 over ti - 1, tj - 1).
-__global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, long long ldf, int n, int k,
-                                                        const float *Wr, const float *Wi, int part) {
+// A22 -= L21 W^T on the lower triangle from row / column m0, one 64 x 64 tile (ti, tj), tj <= ti,
+// per workgroup: L21 = F[rows][k0 .. k0 + kdim), W (n, ldw) rows m0 .. (kdim 64: one panel, 128:
+// two). part 1: the first tc tile columns (tj < tc: the next panels' columns); part 2: the rest
+// (tj >= tc; blockIdx.x indexes the triangle over ti - tc, tj - tc).
+__global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, long long ldf, int n, int k0, int kdim,
+                                                        int m0, const float *Wr, const float *Wi, int ldw, int part,
+                                                        int tc) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
-    const int id = blockIdx.x;
-    int ti, tj;
+    const int T = (n - m0 + LB - 1) / LB;
+    int id = blockIdx.x, ti, tj;
     if (part == 1) {
-        ti = id;
         tj = 0;
+        while (id >= T - tj) { id -= T - tj; ++tj; }
+        ti = tj + id;
     } else {
         ti = (int)((sqrtf(8.f * (float)id + 1.f) - 1.f) * 0.5f);
         while ((ti + 1) * (ti + 2) / 2 <= id) ++ti;
         while (ti * (ti + 1) / 2 > id) --ti;
         tj = id - ti * (ti + 1) / 2;
-        if (part == 2) { ++ti; ++tj; }
+        ti += tc;
+        tj += tc;
     }
-    const int m0 = k + LB, I0 = m0 + LB * ti, C0 = m0 + LB * tj;
+    const int I0 = m0 + LB * ti, C0 = m0 + LB * tj;
     TileAcc acc;
-    tile_abt(Fr + (size_t)I0 * ldf + k, Fi + (size_t)I0 * ldf + k, ldf, min(LB, n - I0), Wr + (size_t)C0 * LB,
-             Wi + (size_t)C0 * LB, LB, min(LB, n - C0), sA, sB, acc);
+    tile_abt(Fr + (size_t)I0 * ldf + k0, Fi + (size_t)I0 * ldf + k0, ldf, min(LB, n - I0), Wr + (size_t)C0 * ldw,
+             Wi + (size_t)C0 * ldw, ldw, min(LB, n - C0), kdim, sA, sB, acc);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1, g = lane >> 4, l16 = lane & 15;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -585,38 +590,54 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
     PQ_HIP(hipGetLastError());
 }
 
-// Lookahead over two streams. On st, per panel k: the diagonal block, the panel (W into buffer
-// k mod 2), then (after the previous panel's bulk update) the first tile column of the trailing
-// update, which is all the next panel reads. The bulk (tile columns >= 1) goes to f.aux after that
-// and runs beside the next panel's diagonal block and panel; the next first-column update waits
-// for it, since both write that tile column. A W buffer is rewritten two panels later, after the
-// bulk update that read it has been waited for.
+// Panels are paired into 128-wide super-panels K .. K + 127. On st, per super-panel: the first
+// panel (diagonal block, panel into W[:, 0:64]), its update of the second panel's 64 columns only,
+// the second panel (diagonal block, panel into W[:, 64:128]); then one K = 128 update of the trailing
+// matrix from row / column K + 128 (half the read-modify-write traffic of two 64-wide updates). Of
+// that update, the first two tile columns (all the next super-panel reads) run on st; the rest goes
+// to f.aux after them and runs beside the next super-panel's panels, and the next first-columns
+// update waits for it, since both write those tile columns. W alternates between two buffers, so a
+// buffer is rewritten only after the bulk update that read it has been waited for.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
     const int n = f.n;
+    constexpr int LDW = 2 * LB;
     auto diag = [&](int k) {
         const int b = std::min(LB, n - k);
         hipLaunchKernelGGL(ldl_diag_kernel, dim3(1), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b,
                            f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, bad);
     };
+    auto panel = [&](int k, float *wr, float *wi) {
+        const int tiles = (n - k - LB + LB - 1) / LB;
+        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
+                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, wr, wi, LDW);
+    };
     if (n <= 0) return;
     diag(0);
     bool bulk = false;                                   // a bulk update is pending on f.aux
-    for (int k = 0, p = 0; k + LB < n; k += LB, ++p) {
-        const int tiles = (n - k - LB + LB - 1) / LB;
-        float *wr = f.Wr[p & 1], *wi = f.Wi[p & 1];
-        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
-                           f.Lir + (size_t)p * LB * LB, f.Lii + (size_t)p * LB * LB, wr, wi);
+    for (int K = 0, s = 0; K + LB < n; K += 2 * LB, ++s) {
+        float *wr = f.Wr[s & 1], *wi = f.Wi[s & 1];
+        panel(K, wr, wi);
+        {   // the first panel's update of the second panel's columns (rows from K + 64)
+            const int T = (n - K - LB + LB - 1) / LB;
+            hipLaunchKernelGGL(ldl_update_kernel, dim3(T), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, LB, K + LB, wr, wi,
+                               LDW, 1, 1);
+        }
+        diag(K + LB);
+        if (K + 2 * LB >= n) break;
+        panel(K + LB, wr + LB, wi + LB);
+        const int m0 = K + 2 * LB, T = (n - m0 + LB - 1) / LB, tc = std::min(2, T);
         if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
-        hipLaunchKernelGGL(ldl_update_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k, wr, wi, 1);
-        bulk = tiles > 1;
+        hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)(tc * T - tc * (tc - 1) / 2)), dim3(256), 0, st, f.Fr, f.Fi,
+                           f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 1, tc);
+        bulk = T > tc;
         if (bulk) {
             PQ_HIP(hipEventRecord(f.ready, st));
             PQ_HIP(hipStreamWaitEvent(f.aux, f.ready, 0));
-            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(tiles - 1) * tiles / 2)), dim3(256), 0,
-                               f.aux, f.Fr, f.Fi, f.ldf, n, k, wr, wi, 2);
+            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2)), dim3(256), 0,
+                               f.aux, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc);
             PQ_HIP(hipEventRecord(f.done, f.aux));
         }
-        diag(k + LB);
+        diag(m0);
     }
     if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
     PQ_HIP(hipGetLastError());

@@ -10,7 +10,7 @@ constexpr int LDL_BLOCK = 64;   // panel width, diagonal block, trailing-update 
 // The factor's storage: planes Fr, Fi (n x ldf FP32, row-major, ldf a multiple of 64; the lower
 // triangle holds L below the diagonal and D on it), the inverses of the unit lower diagonal blocks
 // Lir, Lii (ceil(n / 64) blocks of 64 x 64), the trailing update's right factor Wr, Wi (two
-// buffers of n x 64, alternating between panels). The factorisation also uses a second stream and
+// buffers of n x 128, alternating between 128-wide super-panels). The factorisation also uses a second stream and
 // two events of the caller's (the lookahead: the bulk of one panel's trailing update runs beside
 // the next panel's diagonal block and panel).
 struct LdlFactor {

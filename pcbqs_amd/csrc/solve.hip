@@ -2008,13 +2008,13 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
             L->ldf = (long long)nblk * LDL_BLOCK;
             L->Fldl.reserve((size_t)2 * n * L->ldf * 4);
             L->Xldl.reserve((size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK * 4);
-            L->Wldl.reserve((size_t)4 * n * LDL_BLOCK * 4);
+            L->Wldl.reserve((size_t)4 * n * 2 * LDL_BLOCK * 4);
             if (!L->aux) {
                 PQ_HIP(hipStreamCreateWithFlags(&L->aux, hipStreamNonBlocking));
                 PQ_HIP(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
                 PQ_HIP(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
             }
-            const size_t wsz = (size_t)n * LDL_BLOCK;
+            const size_t wsz = (size_t)n * 2 * LDL_BLOCK;
             float *Wb = L->Wldl.as<float>();
             LdlFactor f{L->Fldl.as<float>(), L->Fldl.as<float>() + (size_t)n * L->ldf, L->ldf, n,
                         L->Xldl.as<float>(), L->Xldl.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,

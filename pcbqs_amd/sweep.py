@@ -15,26 +15,26 @@ import numpy as np
 from .loops import LoopModel
 
 
-def _host(board, grid, precond):
+def _host(board, grid, precond, solver):
     model = LoopModel(board)
     phi = None
-    if precond == 'two-level':
+    if precond == 'two-level' and solver == 'iterative':
         from .coarse import loop_restrict, stream_patterns, via_loop_columns
         phi = np.hstack([loop_restrict(model, stream_patterns(model, grid)), via_loop_columns(model)])
     return model, phi
 
 
-def _board(i, board, freqs, precond, grid, tol, device, graphs):
+def _board(i, board, freqs, precond, grid, tol, device, graphs, solver_kind):
     from .solve import PortSolver
     t = time.perf_counter()
-    model, phi = _host(board, grid, precond)
+    model, phi = _host(board, grid, precond, solver_kind)
     solver = PortSolver(board, device=device, model=model)
     solver.use_graphs(graphs)
     if phi is not None:
         solver.set_coarse(phi)
     setup = time.perf_counter() - t
     t = time.perf_counter()
-    Z, info = solver.ports(freqs, precond=precond, tol=tol)
+    Z, info = solver.ports(freqs, precond=precond, tol=tol, solver=solver_kind)
     solve = time.perf_counter() - t
     solver.close()
     return dict(board=i, triangles=int(board.nt), layers=int(len(np.unique(board.layer_of))),
@@ -44,9 +44,11 @@ def _board(i, board, freqs, precond, grid, tol, device, graphs):
 
 
 def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concurrency=4, graphs=False,
-          log=None):
+          log=None, solver='iterative'):
     """(per-board results in board order, totals) of the port solves of every board at every
-    frequency. `concurrency` boards run at once, each on its own stream from its own host thread
+    frequency. solver='direct' factors each board's free-loop block per frequency on the device
+    (FP32 L D L^T refined in FP64 to 1e-12, as the reference's dense path) instead of GMRES; precond,
+    grid and tol then do not apply. `concurrency` boards run at once, each on its own stream from its own host thread
     (the library releases the interpreter during device calls). graphs=True (concurrency 1 only: a
     capture does not survive other threads' synchronising calls) replays each Arnoldi step as a
     captured graph; at these board sizes capturing a board's ~100 step graphs costs more than the
@@ -57,7 +59,7 @@ def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concu
     t0 = time.perf_counter()
     results = [None] * len(boards)
     with ThreadPoolExecutor(max(1, int(concurrency))) as pool:
-        futures = {pool.submit(_board, i, b, freqs, precond, grid, tol, device, graphs): i
+        futures = {pool.submit(_board, i, b, freqs, precond, grid, tol, device, graphs, solver): i
                    for i, b in enumerate(boards)}
         for fut in futures:
             r = fut.result()
@@ -70,7 +72,7 @@ def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concu
     columns = sum(r['ports'] * len(freqs) for r in results)
     return results, dict(seconds=seconds, boards=len(results), frequencies=len(freqs), solves=solves,
                          port_columns=columns, port_columns_per_s=columns / seconds if seconds > 0 else 0.0,
-                         precond=precond, grid=grid, concurrency=int(concurrency), graphs=graphs)
+                         precond=precond, grid=grid, concurrency=int(concurrency), graphs=graphs, solver=solver)
 
 
 def main(argv=None):
@@ -84,11 +86,12 @@ def main(argv=None):
     ap.add_argument('--grid', type=int, default=12)
     ap.add_argument('--concurrency', type=int, default=4)
     ap.add_argument('--graphs', action='store_true')
+    ap.add_argument('--solver', default='iterative', choices=('iterative', 'direct'))
     args = ap.parse_args(argv)
     freqs = np.logspace(3, 8, args.frequencies)
     bs = [boards.board_batch(s) for s in range(args.boards)]
     res, total = sweep(bs, freqs, precond=args.precond, grid=args.grid, concurrency=args.concurrency,
-                       graphs=args.graphs, log=lambda m: print(m, flush=True))
+                       graphs=args.graphs, log=lambda m: print(m, flush=True), solver=args.solver)
     print(json.dumps(total))
 
 

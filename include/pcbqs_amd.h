@@ -151,6 +151,20 @@ int pq_potential_device(int precision, const void *M0_dev, int64_t nt, int64_t l
                         const int32_t *node_ptr, const int32_t *node_tris, const double *node_area, double factor,
                         int out_precision, void *P_dev, int64_t ldp, void *stream);
 
+/* X = A^-1 B for a real symmetric positive definite A (n x n FP64 on the device,
+ * row stride lda; its lower triangle is read) and nrhs right-hand sides B
+ * (n x nrhs FP64 host, row-major; X likewise): A factored as L D L^T in FP32 on
+ * the device, the solution refined in FP64 against A until every column's
+ * relative residual is at most tol, at most max_steps corrections, stopping
+ * once a correction no longer halves it (above 1e-10: PQ_ECONVERGE). *steps:
+ * the most corrections any column group took; *residual: the largest final
+ * relative residual; *factor_ms: device time of forming and factoring. A
+ * non-positive-definite A (zero pivot) returns PQ_EINVAL. Replaces
+ * solve_symmetric's role in capacitance_matrix (src/pcbqs/peec.py:95-123):
+ * C = B^T P^-1 B on the potential matrix P. */
+int pq_spd_solve(int device, int n, const double *A_dev, int64_t lda, int nrhs, const double *B_host, double tol,
+                 int max_steps, double *X_host, int64_t *steps, double *residual, double *factor_ms);
+
 typedef struct pq_loops pq_loops;   /* a board's loop operator on one device      */
 
 /* Project the branch inductance onto the loop basis on the device and keep the

@@ -67,6 +67,8 @@ _SIGNATURES = {
                              ctypes.c_int),
     'pq_loops_solve_direct': ([P, DBL, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P], ctypes.c_int),
     'pq_loops_factor_time': ([P, P], ctypes.c_int),
+    'pq_spd_solve': ([ctypes.c_int, ctypes.c_int, P, I64, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P],
+                     ctypes.c_int),
     'pq_loops_set_coarse': ([P, ctypes.c_int, P, ctypes.c_int], ctypes.c_int),
     'pq_loops_coarse_matrices': ([P, P, P], ctypes.c_int),
     'pq_biot_field': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, I64, P, I64, P, I64, P, P, P], ctypes.c_int),

@@ -14,7 +14,11 @@
 //                     the flops).
 // Complex products are four real MFMA products on split real / imaginary planes.
 #include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 #include "pq_common.h"
 #include "ldl.h"
@@ -563,6 +567,74 @@ __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restri
     }
 }
 
+// ---- the dense symmetric positive definite solve (real A FP64 on the device) ----
+
+// F = A in FP32 (lower triangle), imaginary plane zero.
+__global__ __launch_bounds__(256) void spd_form_kernel(const double *__restrict__ A, long long lda, int n,
+                                                      float *__restrict__ Fr, float *__restrict__ Fi, long long ldf) {
+    for (int row = blockIdx.y; row < n; row += gridDim.y)
+        for (int c = blockIdx.x * 256 + threadIdx.x; c <= row; c += gridDim.x * 256) {
+            Fr[(size_t)row * ldf + c] = (float)A[(size_t)row * lda + c];
+            Fi[(size_t)row * ldf + c] = 0.f;
+        }
+}
+
+// Chunk partials of Y = A X for symmetric A (FP64): thread per row, reading column `row` of the
+// chunk's rows of A (coalesced across threads), X rows wave-uniform; part[(chunk, row)][q].
+template <int NR>
+__global__ __launch_bounds__(256) void spd_gemv_kernel(const double *__restrict__ A, long long lda, int n,
+                                                      const double *__restrict__ X, int chunk,
+                                                      double *__restrict__ part) {
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    const int k0 = blockIdx.y * chunk, k1 = min(n, k0 + chunk);
+    const int rr = min(row, n - 1);
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = 0;
+    for (int k = k0; k < k1; ++k) {
+        const double a = A[(size_t)k * lda + rr];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += a * X[(size_t)k * NR + q];
+    }
+    if (row < n)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) part[((size_t)blockIdx.y * n + row) * NR + q] = acc[q];
+}
+
+// R = B - sum of the nsplit chunk partials (in chunk order).
+__global__ void spd_residual_kernel(const double *__restrict__ part, int nsplit, size_t count,
+                                    const double *__restrict__ B, double *__restrict__ R) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    double s = 0;
+    for (int q = 0; q < nsplit; ++q) s += part[(size_t)q * count + e];
+    R[e] = B[e] - s;
+}
+
+// out[q] = sum over rows of V[row][q]^2 (one workgroup per column q, fixed-order tree).
+__global__ __launch_bounds__(256) void colnorm2_kernel(const double *__restrict__ V, int n, int NR,
+                                                      double *__restrict__ out) {
+    __shared__ double s[256];
+    const int q = blockIdx.x;
+    double a = 0;
+    for (int row = threadIdx.x; row < n; row += 256) {
+        const double v = V[(size_t)row * NR + q];
+        a += v * v;
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = s[0];
+}
+
+__global__ void add_kernel(const double *X, size_t count, double *Y) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) Y[e] += X[e];
+}
+
 template <class F> void dispatch_ports(int P, F &&f) {
     switch (P) {
     case 1: f(std::integral_constant<int, 1>{}); break;
@@ -668,3 +740,134 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
 }
 
 }  // namespace pq
+
+using namespace pq;
+
+extern "C" {
+
+// X = A^-1 B for a real symmetric positive definite A (n x n FP64 on the device, row stride lda,
+// its lower triangle read) and nrhs right-hand sides B (n x nrhs FP64 host, row-major): A factored
+// as L D L^T in FP32 on the device (the direct port solve's kernels), the solution refined in FP64
+// against A until every column's relative residual is at most tol, at most max_steps corrections,
+// stopping once a correction no longer halves it (a result above 1e-10 returns PQ_ECONVERGE). No
+// counterpart as an entry point in the reference: it serves capacitance_matrix's solve_symmetric
+// (src/pcbqs/peec.py:95-123, kernels.py solve_symmetric).
+int pq_spd_solve(int device, int n, const double *A_dev, int64_t lda, int nrhs, const double *B_host, double tol,
+                 int max_steps, double *X_host, int64_t *steps, double *residual, double *factor_ms) {
+    return guard([&] {
+        PQ_REQUIRE(A_dev && B_host && X_host, "pq_spd_solve: null argument");
+        PQ_REQUIRE(n >= 1 && lda >= n && nrhs >= 1 && tol > 0 && tol < 1 && max_steps >= 0,
+                   "pq_spd_solve: invalid arguments");
+        DeviceScope scope(device);
+        DevBuf F, Xi, Wb, bad;                       // freed after the streams have drained (below)
+        hipStream_t st = nullptr, aux = nullptr;
+        hipEvent_t ready = nullptr, done = nullptr, e0 = nullptr, e1 = nullptr;
+        struct Cleanup {
+            hipStream_t &st, &aux;
+            hipEvent_t &ready, &done, &e0, &e1;
+            ~Cleanup() {
+                if (st) (void)hipStreamSynchronize(st);
+                if (aux) (void)hipStreamSynchronize(aux);
+                for (hipEvent_t e : {ready, done, e0, e1})
+                    if (e) (void)hipEventDestroy(e);
+                if (aux) (void)hipStreamDestroy(aux);
+                if (st) (void)hipStreamDestroy(st);
+            }
+        } cleanup{st, aux, ready, done, e0, e1};
+        PQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        PQ_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+        PQ_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        PQ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        PQ_HIP(hipEventCreate(&e0));
+        PQ_HIP(hipEventCreate(&e1));
+        const int nblk = (n + LB - 1) / LB;
+        const long long ldf = (long long)nblk * LB;
+        F.reserve((size_t)2 * n * ldf * 4);
+        Xi.reserve((size_t)2 * nblk * LB * LB * 4);
+        const size_t wsz = (size_t)n * 2 * LB;
+        Wb.reserve(4 * wsz * 4);
+        bad.reserve(4);
+        float *wb = Wb.as<float>();
+        LdlFactor f{F.as<float>(), F.as<float>() + (size_t)n * ldf, ldf, n, Xi.as<float>(),
+                    Xi.as<float>() + (size_t)nblk * LB * LB, {wb, wb + 2 * wsz}, {wb + wsz, wb + 3 * wsz}, aux,
+                    ready, done};
+        PQ_HIP(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(spd_form_kernel, dim3((n + 255) / 256, std::min(n, 65535)), dim3(256), 0, st, A_dev, lda, n,
+                           f.Fr, f.Fi, f.ldf);
+        PQ_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+        ldl_factor(st, f, bad.as<int>());
+        PQ_HIP(hipEventRecord(e1, st));
+        int flag = 0;
+        PQ_HIP(hipMemcpyAsync(&flag, bad.p, 4, hipMemcpyDeviceToHost, st));
+        PQ_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (factor_ms) *factor_ms = ms;
+        PQ_REQUIRE(!flag, "pq_spd_solve: a zero or non-finite pivot (A is not positive definite)");
+        // the right-hand sides in groups of up to 16, each as the real parts of complex columns
+        int64_t worst_steps = 0;
+        double worst = 0;
+        const int chunk = 256, nsplit = (n + chunk - 1) / chunk;
+        for (int c0 = 0; c0 < nrhs; c0 += 16) {
+            const int cn = std::min(16, nrhs - c0);
+            const int P = cn <= 8 ? cn : 16, NR = 2 * P;
+            const size_t count = (size_t)n * NR;
+            std::vector<double> hb(count, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int q = 0; q < cn; ++q) hb[(size_t)i * NR + q] = B_host[(size_t)i * nrhs + c0 + q];
+            DevBuf Bd, X, R, D, Y, Ys, part, nrm;
+            for (DevBuf *b : {&Bd, &X, &R, &D, &Y, &Ys}) b->reserve(count * 8);
+            part.reserve((size_t)nsplit * count * 8);
+            nrm.reserve(NR * 8);
+            PQ_HIP(hipMemcpyAsync(Bd.p, hb.data(), count * 8, hipMemcpyHostToDevice, st));
+            std::vector<double> bn(NR), rn(NR);
+            auto colnorms = [&](const double *V, std::vector<double> &out) {
+                hipLaunchKernelGGL(colnorm2_kernel, dim3(NR), dim3(256), 0, st, V, n, NR, nrm.as<double>());
+                PQ_HIP(hipMemcpyAsync(out.data(), nrm.p, NR * 8, hipMemcpyDeviceToHost, st));
+                PQ_HIP(hipStreamSynchronize(st));
+                for (auto &v : out) v = std::sqrt(v);
+            };
+            auto residual_of = [&](const double *Xv) {
+                dispatch_ports(P, [&](auto pc) {
+                    constexpr int N = 2 * decltype(pc)::value;
+                    hipLaunchKernelGGL(spd_gemv_kernel<N>, dim3((n + 255) / 256, nsplit), dim3(256), 0, st, A_dev, lda,
+                                       n, Xv, chunk, part.as<double>());
+                });
+                hipLaunchKernelGGL(spd_residual_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st,
+                                   part.as<double>(), nsplit, count, Bd.as<double>(), R.as<double>());
+                PQ_HIP(hipGetLastError());
+            };
+            colnorms(Bd.as<double>(), bn);
+            ldl_solve(st, f, P, Bd.as<double>(), Y.as<double>(), Ys.as<double>(), X.as<double>());
+            double last = std::numeric_limits<double>::infinity(), res = 0;
+            int corr = 0;
+            for (int step = 0;; ++step) {
+                residual_of(X.as<double>());
+                colnorms(R.as<double>(), rn);
+                res = 0;
+                for (int q = 0; q < cn; ++q) res = std::max(res, rn[q] / std::max(bn[q], 1e-300));
+                if (!(res > tol) || step == max_steps || res > 0.5 * last) break;
+                last = res;
+                ldl_solve(st, f, P, R.as<double>(), Y.as<double>(), Ys.as<double>(), D.as<double>());
+                hipLaunchKernelGGL(add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, D.as<double>(),
+                                   count, X.as<double>());
+                PQ_HIP(hipGetLastError());
+                ++corr;
+            }
+            worst = std::max(worst, res);
+            worst_steps = std::max<int64_t>(worst_steps, corr);
+            std::vector<double> hx(count);
+            PQ_HIP(hipMemcpyAsync(hx.data(), X.p, count * 8, hipMemcpyDeviceToHost, st));
+            PQ_HIP(hipStreamSynchronize(st));
+            for (int i = 0; i < n; ++i)
+                for (int q = 0; q < cn; ++q) X_host[(size_t)i * nrhs + c0 + q] = hx[(size_t)i * NR + q];
+        }
+        if (steps) *steps = worst_steps;
+        if (residual) *residual = worst;
+        if (worst > std::max(tol, 1e-10))
+            throw Error(PQ_ECONVERGE, "pq_spd_solve: the FP32 factor's refinement stalls at relative residual " +
+                                          std::to_string(worst));
+    });
+}
+
+}  // extern "C"

@@ -200,11 +200,12 @@ def edge_inductance(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None
     return out, dict(backend='mi355x', workers=1, seconds=time.perf_counter() - started)
 
 
-def potential(nodes, tris, z, node_of, er=1.0, near=3.0, outer_levels=1, precision=64, device=0):
+def potential(nodes, tris, z, node_of, er=1.0, near=3.0, outer_levels=1, precision=64, device=0, keep=False):
     """Potential coefficients P (n_nodes, n_nodes) FP64 host of the charge nodes node_of (nt,):
     P = W^T M0 W / (4 pi eps0 er), W[t, node(t)] = 1 / area(node) (src/pcbqs/mqs.py:532-556; the
     reference's side-charge panels excluded). M0 is the device's scalar fill (precision 64: the
-    reference layout; 32: FP32 far pairs, FP64 near pairs), which stays in HBM."""
+    reference layout; 32: FP32 far pairs, FP64 near pairs), which stays in HBM. keep=True returns
+    the device array (the caller closes it) instead of the host copy."""
     nodes, tris = np.asarray(nodes, float), np.asarray(tris)
     node_of = np.asarray(node_of, np.int64)
     if node_of.shape != (len(tris),) or node_of.min() < 0:
@@ -224,9 +225,61 @@ def potential(nodes, tris, z, node_of, er=1.0, near=3.0, outer_levels=1, precisi
         call('pq_potential_device', precision, planes.ptr, len(tris), padded(len(tris)), n, d_ptr.ptr, d_tris.ptr,
              d_area.ptr, 1.0 / (4 * np.pi * EPS0 * er), 64, P.ptr, n, None)
         call('pq_synchronize', device)
-        out = P.host()
-        for a in (planes, d_ptr, d_tris, d_area, P):
+        for a in (planes, d_ptr, d_tris, d_area):
             a.close()
+        if keep:
+            return P
+        out = P.host()
+        P.close()
         return out
     finally:
         backend.close()
+
+
+def capacitance_matrix(nodes, tris, z, node_of, labels, er=1.0, near=3.0, outer_levels=1, precision=64, device=0,
+                       tol=1e-12, max_steps=12, info=None):
+    """Maxwell capacitance C (n_conductors, n_conductors) in F of the conductors labels (n_nodes,)
+    over the charge nodes node_of (nt,): C = B^T P^-1 B with B[node, labels[node]] = 1 and P the
+    potential matrix of `potential` (src/pcbqs/peec.py:95-123: potentials relative to infinity, so
+    row sums need not vanish; symmetrised as the reference does). P stays on the device, where it
+    is factored as L D L^T in FP32 and the solve is refined in FP64 to a relative residual of tol.
+    info (a dict), when given, receives corrections, relative_residual and factor_ms."""
+    import ctypes
+    labels = np.asarray(labels, np.int64)
+    n = int(np.asarray(node_of).max()) + 1
+    if labels.shape != (n,) or labels.min() < 0:
+        raise ValueError('labels must give one conductor per charge node')
+    nc = int(labels.max()) + 1
+    B = np.zeros((n, nc))
+    B[np.arange(n), labels] = 1.0
+    P = potential(nodes, tris, z, node_of, er, near, outer_levels, precision, device, keep=True)
+    try:
+        X = np.empty((n, nc))
+        steps, res, fms = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        call('pq_spd_solve', device, n, P.ptr, n, nc, ptr(np.ascontiguousarray(B)), float(tol), int(max_steps),
+             ptr(X), ctypes.byref(steps), ctypes.byref(res), ctypes.byref(fms))
+    finally:
+        P.close()
+    if info is not None:
+        info.update(corrections=int(steps.value), relative_residual=float(res.value), factor_ms=float(fms.value))
+    C = B.T @ X
+    return (C + C.T) / 2
+
+
+def board_capacitance(board, er=1.0, precision=64, device=0, info=None):
+    """(C, labels) for a board: the Maxwell capacitance matrix of its conductors, the connected
+    components of its copper and vias (ports and lumped elements do not join them), labels the
+    conductor of each charge node; src/pcbqs/peec.py:95-123."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    from .loops import LoopModel
+    model = LoopModel(board)
+    node_of = np.asarray(board.node_of, np.int64)
+    n = int(node_of.max()) + 1
+    size = max(n, int(np.max(model.br)) + 1)
+    a, b = np.asarray(model.br, np.int64).T
+    _, comp = connected_components(coo_matrix((np.ones(len(a)), (a, b)), shape=(size, size)), directed=False)
+    labels = np.unique(comp[:n], return_inverse=True)[1]
+    C = capacitance_matrix(board.nodes, board.tris, board.z_of, node_of, labels, er, precision=precision,
+                           device=device, info=info)
+    return C, labels

@@ -3,84 +3,129 @@
 // Semantics: src/pcbqs/core/static_kernels.hpp:254-310 (static_combine) as called by
 // src/pcbqs/static.py:103-128 (edge_inductance):
 //   L[m,n] = f s_m s_n sum_{i,j} a_mi a_nj [Mrr(p,q) - v_nj.Mr(p,q) - v_mi.Mr(q,p) + v_mi.v_nj M0(p,q)]
-// with p = tri[i][m], q = tri[j][n]. Each entry is evaluated from its canonical pair
-// (min(m,n), max(m,n)) so L is bitwise symmetric. One wave per row of L (row data wave-uniform),
-// lanes over 64 consecutive columns (coalesced L stores); plane gathers hit L2 because RWG
-// edges are numbered in triangle order, so neighbouring rows/columns touch neighbouring
-// triangles.
+// with p = tri[i][m], q = tri[j][n]. Each entry is evaluated once, from its canonical pair
+// (min(m,n), max(m,n)), and stored to both positions, so L is bitwise symmetric. RWG edges are
+// numbered in triangle order, so a 64-edge block touches a compact set of triangles and the plane
+// gathers of a tile hit L2.
 //
 // FP32 planes are in each triangle's local frame (first moments about the row/column
 // triangle's centroid c): with u = c_p - v, the pair term becomes
 //   Mrr'(p,q) + u_nj.Mr'(p,q) + u_mi.Mr'(q,p) + (u_mi.u_nj) M0(p,q),
 // every product of the size of the result (no cancellation of |c|^2 terms).
 #include "pq_common.h"
+#include <type_traits>
 
 namespace pq {
 
+// The combination by 64 x 64 tiles of edges, one canonical tile pair (R <= C) per workgroup,
+// both plane orientations read along rows. Entry (m, n), m in R, n in C, splits into
+//   A(m, n) = sum_{i,j} a_mi a_nj [Mrr(p,q) - v_nj.Mr(p,q) + (v_mi.v_nj) M0(p,q)]   (planes at [p][q])
+//   B(n, m) = sum_{i,j} a_mi a_nj [- v_mi.Mr(q,p)]                                   (planes at [q][p])
+// (local frame: + u_nj.Mr(p,q) and + u_mi.Mr(q,p), u = c - v). Pass A runs a wave per row m with
+// lanes over n; pass B a wave per row n with lanes over m, so that in both the row triangle is
+// wave-uniform and the lanes read along a plane row. Both land in LDS; each entry is then formed
+// once, f s_m s_n (A + B), and written to L[m][n] and L[n][m] (bitwise symmetric). The earlier
+// row-per-wave kernel read Mr(q,p) with 64 lanes on 64 rows, 31x its algorithmic bytes.
 template <class PT, bool LOCAL, class OT>
-__global__ __launch_bounds__(256) void combine_kernel(const PT *M0, const PT *Mx, const PT *My, const PT *Mrr,
-                                                     long long ld, const double *cent, const int *tri,
-                                                     const double *alpha, const double *vertex,
-                                                     const double *scale, long long ne, double factor,
-                                                     OT *L, long long ldl) {
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long c = (long long)blockIdx.x * 64 + lane;
-    const bool cvalid = c < ne;
-    const long long cc = cvalid ? c : ne - 1;
-    const long long rbase = (long long)blockIdx.y * 16 + wave * 4;
-    for (int k = 0; k < 4; ++k) {
-        const long long r = rbase + k;
-        if (r >= ne) break;
-        const long long a = r <= cc ? r : cc, b = r <= cc ? cc : r;
-        int ta[2], tb[2];
-        double al[2], bl[2], va[2][2], vb[2][2];
+__global__ __launch_bounds__(256) void combine_tile_kernel(const PT *M0, const PT *Mx, const PT *My, const PT *Mrr,
+                                                          long long ld, const double *cent, const int *tri,
+                                                          const double *alpha, const double *vertex,
+                                                          const double *scale, long long ne, double factor,
+                                                          OT *L, long long ldl) {
+    // FP64 arithmetic on the FP32 plane values: the two halves of a pair's term (and the weighted
+    // sums over an edge's + and - triangles) partly cancel, so they are not rounded to FP32 apart
+    using AT = double;
+    __shared__ AT tA[64][65], tB[64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long id = blockIdx.x;
+    long long ti = (long long)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= id) ++ti;
+    while (ti * (ti + 1) / 2 > id) --ti;
+    const long long tj = id - ti * (ti + 1) / 2;
+    const long long R0 = tj * 64, C0 = ti * 64;       // R0 <= C0
+    const bool diag = R0 == C0;
+    // one edge's data: triangles, alpha, and v (or u = c - v in the local frame)
+    struct Edge {
+        int t[2];
+        double a[2], x[2], y[2];
+    };
+    auto load = [&](long long e) {
+        Edge d;
+        const long long ee = e < ne ? e : ne - 1;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            ta[i] = tri[i * ne + a]; tb[i] = tri[i * ne + b];
-            al[i] = alpha[i * ne + a]; bl[i] = alpha[i * ne + b];
-            va[i][0] = vertex[2 * (i * ne + a)]; va[i][1] = vertex[2 * (i * ne + a) + 1];
-            vb[i][0] = vertex[2 * (i * ne + b)]; vb[i][1] = vertex[2 * (i * ne + b) + 1];
+            d.t[i] = tri[i * ne + ee];
+            d.a[i] = alpha[i * ne + ee];
+            d.x[i] = vertex[2 * (i * ne + ee)];
+            d.y[i] = vertex[2 * (i * ne + ee) + 1];
             if (LOCAL) {
-                va[i][0] = cent[2 * (size_t)ta[i]] - va[i][0]; va[i][1] = cent[2 * (size_t)ta[i] + 1] - va[i][1];
-                vb[i][0] = cent[2 * (size_t)tb[i]] - vb[i][0]; vb[i][1] = cent[2 * (size_t)tb[i] + 1] - vb[i][1];
+                d.x[i] = cent[2 * (size_t)d.t[i]] - d.x[i];
+                d.y[i] = cent[2 * (size_t)d.t[i] + 1] - d.y[i];
             }
         }
-        double sum = 0;
-        if (LOCAL) {
-            float s32 = 0;
+        return d;
+    };
+    // pass A: rows m in R (wave w: 16 rows), lanes n in C
+    {
+        const Edge n = load(C0 + lane);
+        for (int k = 0; k < 16; ++k) {
+            const int rm = 16 * w + k;
+            const Edge m = load(R0 + rm);
+            AT s = 0;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const size_t p = (size_t)ta[i];
-                const float umx = (float)va[i][0], umy = (float)va[i][1];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const size_t q = (size_t)tb[j];
-                    const float unx = (float)vb[j][0], uny = (float)vb[j][1];
-                    const size_t pq = p * ld + q, qp = q * ld + p;
-                    const float t = (float)Mrr[pq] + unx * (float)Mx[pq] + uny * (float)My[pq] +
-                                    umx * (float)Mx[qp] + umy * (float)My[qp] + (umx * unx + umy * uny) * (float)M0[pq];
-                    s32 += (float)(al[i] * bl[j]) * t;
-                }
-            }
-            sum = s32;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const size_t p = (size_t)ta[i];
-                const double vmx = va[i][0], vmy = va[i][1];
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    const size_t q = (size_t)tb[j];
-                    const double vnx = vb[j][0], vny = vb[j][1];
-                    const size_t pq = p * ld + q, qp = q * ld + p;
-                    sum += al[i] * bl[j] * ((double)Mrr[pq] - vnx * (double)Mx[pq] - vny * (double)My[pq] -
-                                            vmx * (double)Mx[qp] - vmy * (double)My[qp] +
-                                            (vmx * vnx + vmy * vny) * (double)M0[pq]);
+                    const size_t pq = (size_t)m.t[i] * ld + n.t[j];
+                    if (LOCAL) {
+                        s += m.a[i] * n.a[j] * ((double)Mrr[pq] + n.x[j] * (double)Mx[pq] + n.y[j] * (double)My[pq] +
+                                                (m.x[i] * n.x[j] + m.y[i] * n.y[j]) * (double)M0[pq]);
+                    } else {
+                        s += m.a[i] * n.a[j] * ((double)Mrr[pq] - n.x[j] * (double)Mx[pq] - n.y[j] * (double)My[pq] +
+                                                (m.x[i] * n.x[j] + m.y[i] * n.y[j]) * (double)M0[pq]);
+                    }
                 }
-            }
+            tA[rm][lane] = s;
         }
-        if (cvalid) L[r * ldl + c] = (OT)(factor * scale[a] * scale[b] * sum);
+    }
+    // pass B: rows n in C (wave w: 16 rows), lanes m in R
+    {
+        const Edge m = load(R0 + lane);
+        for (int k = 0; k < 16; ++k) {
+            const int cn = 16 * w + k;
+            const Edge n = load(C0 + cn);
+            AT s = 0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const size_t qp = (size_t)n.t[j] * ld + m.t[i];
+                    if (LOCAL) {
+                        s += m.a[i] * n.a[j] * (m.x[i] * (double)Mx[qp] + m.y[i] * (double)My[qp]);
+                    } else {
+                        s += m.a[i] * n.a[j] * (-m.x[i] * (double)Mx[qp] - m.y[i] * (double)My[qp]);
+                    }
+                }
+            tB[cn][lane] = s;
+        }
+    }
+    __syncthreads();
+    // L[m][n] (rows m, lanes n) and L[n][m] (rows n, lanes m), each from the canonical (m, n)
+    for (int k = 0; k < 16; ++k) {
+        const int rm = 16 * w + k;
+        const long long mm = R0 + rm, nn = C0 + lane;
+        if (mm < ne && nn < ne && (!diag || lane >= rm)) {
+            const AT v = tA[rm][lane] + tB[lane][rm];
+            L[mm * ldl + nn] = (OT)(factor * scale[mm] * scale[nn] * (double)v);
+        }
+    }
+    for (int k = 0; k < 16; ++k) {
+        const int cn = 16 * w + k;
+        const long long nn = C0 + cn, mm = R0 + lane;
+        if (mm < ne && nn < ne && (!diag || lane < cn)) {
+            const AT v = tA[lane][cn] + tB[cn][lane];
+            L[nn * ldl + mm] = (OT)(factor * scale[mm] * scale[nn] * (double)v);
+        }
     }
 }
 
@@ -154,25 +199,26 @@ extern "C" int pq_combine_device(int precision, const void *planes, int64_t nt, 
         if (ne == 0) return;
         hipStream_t s = (hipStream_t)stream;
         const size_t plane = (size_t)nt * ld;
-        dim3 grid((unsigned)((ne + 63) / 64), (unsigned)((ne + 15) / 16)), block(256);
+        const long long T = (ne + 63) / 64;
+        dim3 grid((unsigned)(T * (T + 1) / 2)), block(256);
         if (precision == 32) {
             const float *P = (const float *)planes;
             if (out_precision == 32)
-                hipLaunchKernelGGL((combine_kernel<float, true, float>), grid, block, 0, s, P, P + plane, P + 2 * plane,
-                                   P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale, (long long)ne, factor,
-                                   (float *)L, (long long)ldl);
+                hipLaunchKernelGGL((combine_tile_kernel<float, true, float>), grid, block, 0, s, P, P + plane,
+                                   P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
+                                   (long long)ne, factor, (float *)L, (long long)ldl);
             else
-                hipLaunchKernelGGL((combine_kernel<float, true, double>), grid, block, 0, s, P, P + plane,
+                hipLaunchKernelGGL((combine_tile_kernel<float, true, double>), grid, block, 0, s, P, P + plane,
                                    P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
                                    (long long)ne, factor, (double *)L, (long long)ldl);
         } else {
             const double *P = (const double *)planes;
             if (out_precision == 32)
-                hipLaunchKernelGGL((combine_kernel<double, false, float>), grid, block, 0, s, P, P + plane,
+                hipLaunchKernelGGL((combine_tile_kernel<double, false, float>), grid, block, 0, s, P, P + plane,
                                    P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
                                    (long long)ne, factor, (float *)L, (long long)ldl);
             else
-                hipLaunchKernelGGL((combine_kernel<double, false, double>), grid, block, 0, s, P, P + plane,
+                hipLaunchKernelGGL((combine_tile_kernel<double, false, double>), grid, block, 0, s, P, P + plane,
                                    P + 2 * plane, P + 3 * plane, (long long)ld, cent, tri, alpha, vertex, scale,
                                    (long long)ne, factor, (double *)L, (long long)ldl);
         }

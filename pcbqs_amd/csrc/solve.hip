@@ -30,23 +30,35 @@ using cplx = std::complex<double>;
 
 // ---- projection: Ll = C^T Lb C ------------------------------------------------------------
 
-// U[k, j] = sum_t s_t Lb[k, row_t] over the branches t of loop j (C in CSC: colptr, rows, signs).
+// U[k, j] = sum_t s_t Lb[k, row_t] over the branches t of loop j (C in CSC: colptr, rows, signs),
+// PROJ_ROWS rows k per workgroup: each loop's branch list is read once for all of them (it was
+// read once per row, the kernel's dominant traffic), and the rows' gathers of a branch are
+// independent loads in flight together.
+constexpr int PROJ_ROWS = 8;
 __global__ __launch_bounds__(256) void proj_left_kernel(const float *Lb, long long ldb, int nbr,
                                                        const int *colptr, const int *rows, const float *sign,
                                                        int nl, float *U, long long ldu) {
-    const int k = blockIdx.x;
-    const float *Lrow = Lb + (size_t)k * ldb;
+    const int k0 = blockIdx.x * PROJ_ROWS;
+    const float *Lr[PROJ_ROWS];
+#pragma unroll
+    for (int i = 0; i < PROJ_ROWS; ++i) Lr[i] = Lb + (size_t)min(k0 + i, nbr - 1) * ldb;
     for (int j = threadIdx.x; j < nl; j += blockDim.x) {
-        double acc = 0;
+        double acc[PROJ_ROWS];
+#pragma unroll
+        for (int i = 0; i < PROJ_ROWS; ++i) acc[i] = 0;
         for (int t = colptr[j]; t < colptr[j + 1]; ++t) {
             const int r = rows[t];
-            if (r < nbr) acc += (double)sign[t] * (double)Lrow[r];
+            if (r >= nbr) continue;
+            const double sg = sign[t];
+#pragma unroll
+            for (int i = 0; i < PROJ_ROWS; ++i) acc[i] += sg * (double)Lr[i][r];
         }
-        U[(size_t)k * ldu + j] = (float)acc;
+#pragma unroll
+        for (int i = 0; i < PROJ_ROWS; ++i)
+            if (k0 + i < nbr) U[(size_t)(k0 + i) * ldu + j] = (float)acc[i];
     }
 }
 
-// Ll[i, j] = sum_t s_t U[row_t, j] over the branches t of loop i.
 __global__ __launch_bounds__(256) void proj_right_kernel(const float *U, long long ldu, int nbr, const int *colptr,
                                                         const int *rows, const float *sign, int nl, float *Ll,
                                                         long long ldl) {
@@ -1654,7 +1666,8 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
             const long long ldu = L->ldl;
             U.reserve((size_t)n_branch_rows * ldu * 4);
             L->Ll.reserve((size_t)n_loops * L->ldl * 4);
-            hipLaunchKernelGGL(proj_left_kernel, dim3(n_branch_rows), dim3(256), 0, L->stream, Lb_dev, (long long)ldb,
+            hipLaunchKernelGGL(proj_left_kernel, dim3((n_branch_rows + PROJ_ROWS - 1) / PROJ_ROWS), dim3(256), 0, L->stream,
+                               Lb_dev, (long long)ldb,
                                n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, U.as<float>(), ldu);
             hipLaunchKernelGGL(proj_right_kernel, dim3(n_loops), dim3(256), 0, L->stream, U.as<float>(), ldu,
                                n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, L->Ll.as<float>(),

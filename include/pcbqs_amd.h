@@ -178,6 +178,34 @@ typedef struct pq_loops pq_loops;   /* a board's loop operator on one device    
 int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr, const int32_t *c_rows,
                     const float *c_sign, int64_t c_nnz, const float *Lb_dev, int64_t ldb, int n_branch_rows,
                     const int32_t *r_ptr, const int32_t *r_col, const double *r_val, pq_loops **out);
+
+/* The spanning tree the fundamental loops are built on. Loop j is its closing
+ * branch (coefficient +1, running end_a -> end_b) plus the tree path back from
+ * end_b to end_a; parent/up_branch/up_sign give each node's branch to its
+ * parent (up_sign +1 where that branch runs from the parent to the node), and
+ * level_nodes lists the non-root nodes by depth 1, 2, ... (level_ptr,
+ * n_levels + 1 offsets). Mirrors spanning_tree's (up, upb, ups, depth)
+ * and the loop walk of MQS._loops (src/pcbqs/mqs.py:48-129, 592-637). */
+typedef struct pq_loop_tree {
+    int32_t n_nodes;
+    const int32_t *parent;       /* (n_nodes) parent node, -1 at a root           */
+    const int32_t *up_branch;    /* (n_nodes) branch to the parent, -1 at a root  */
+    const float *up_sign;        /* (n_nodes) +1 or -1                            */
+    int32_t n_levels;
+    const int32_t *level_ptr;    /* (n_levels + 1)                                */
+    const int32_t *level_nodes;  /* (level_ptr[n_levels])                         */
+    const int32_t *closing;      /* (n_loops) closing branch, loops in device order */
+    const int32_t *end_a;        /* (n_loops) its first end node                  */
+    const int32_t *end_b;        /* (n_loops) its second end node                 */
+} pq_loop_tree;
+
+/* pq_loops_create with C given by its tree: U = Lb C from prefix sums of Lb's
+ * rows down the tree (U[k][j] = Lb[k][closing j] + S_k(end_a j) - S_k(end_b j)),
+ * then Ll = C^T U the same way on U's rows; FP64 sums, FP32 storage, the same
+ * Ll as pq_loops_create. Lb must be symmetric (both triangles stored). */
+int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tree, const float *Lb_dev,
+                         int64_t ldb, int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col,
+                         const double *r_val, pq_loops **out);
 int pq_loops_destroy(pq_loops *loops);
 
 /* The device loop inductance matrix (FP32, n_loops rows, leading dimension ldl). */

@@ -32,6 +32,14 @@ class Geometry(ctypes.Structure):
                 ('radius', ctypes.c_void_p)]
 
 
+class LoopTree(ctypes.Structure):
+    """pq_loop_tree: the spanning tree of a fundamental loop basis (int32/float32 host arrays)."""
+    _fields_ = [('n_nodes', ctypes.c_int32), ('parent', ctypes.c_void_p), ('up_branch', ctypes.c_void_p),
+                ('up_sign', ctypes.c_void_p), ('n_levels', ctypes.c_int32), ('level_ptr', ctypes.c_void_p),
+                ('level_nodes', ctypes.c_void_p), ('closing', ctypes.c_void_p), ('end_a', ctypes.c_void_p),
+                ('end_b', ctypes.c_void_p)]
+
+
 _lib = None
 P = ctypes.c_void_p
 I32, I64, DBL = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -61,6 +69,8 @@ _SIGNATURES = {
     'pq_potential_device': ([ctypes.c_int, P, I64, I64, I64, P, P, P, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
     'pq_loops_create': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, I64, P, I64, ctypes.c_int, P, P, P, P],
                         ctypes.c_int),
+    'pq_loops_create_tree': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, I64, ctypes.c_int, P, P, P, P],
+                             ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),
     'pq_loops_matrix': ([P, P, P], ctypes.c_int),
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],

@@ -110,6 +110,75 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int *__restrict__ ptr, 
 // Ll := (Ll + Ll^T) / 2 on the upper triangle and its mirror (the loop matrices' symmetric part,
 // as the reference takes it: src/pcbqs/mqs_iterative.py:244-257). Entry (i, j) and (j, i) then
 // agree bitwise, which lets the product read columns of Ll as rows.
+// ---- the projection through the spanning tree ----
+// A loop is its closing branch plus the tree path between the branch's ends: C e_j = e_cot(j) +
+// p(a_j) - p(b_j), p(x) the signed branch set of the path from node x to its root (the part
+// above the two paths' meeting node cancels). So with S[x] = sum over that path of s_y Lb[upb y]
+// (prefix sums down the tree, one level at a time), (Lb C)[k][j] = Lb[k][cot j] + S[a_j][k] -
+// S[b_j][k]: two row reads per loop instead of one gather per branch of the loop. The same again
+// on the rows of U = Lb C gives Ll = C^T U. All sums in FP64.
+
+// S[x][k] = S[parent x][k] + s_x Row[upb x][k] for the nodes x of one level (rows past nvalid
+// carry nothing); k along the lanes, Row read along its rows (Lb is symmetric).
+__global__ __launch_bounds__(256) void tree_level_kernel(const float *__restrict__ Row, long long ldr, int nvalid,
+                                                        int width, const int *__restrict__ nodes, int count,
+                                                        const int *__restrict__ parent, const int *__restrict__ upb,
+                                                        const float *__restrict__ ups, double *__restrict__ S,
+                                                        long long lds) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= width) return;
+    for (int e = blockIdx.y; e < count; e += gridDim.y) {
+        const int x = nodes[e], b = upb[x];
+        double v = S[(size_t)parent[x] * lds + k];
+        if (b >= 0 && b < nvalid) v += (double)ups[x] * (double)Row[(size_t)b * ldr + k];
+        S[(size_t)x * lds + k] = v;
+    }
+}
+
+// U[k][j] = Lb[cot j][k] + S[a_j][k] - S[b_j][k] by 64 x 64 tiles through LDS: rows of S and Lb
+// read along k, rows of U written along j.
+__global__ __launch_bounds__(256) void tree_u_kernel(const float *__restrict__ Lb, long long ldb, int nbr,
+                                                    const double *__restrict__ S, long long lds,
+                                                    const int *__restrict__ cot, const int *__restrict__ ea,
+                                                    const int *__restrict__ eb, int nl, float *__restrict__ U,
+                                                    long long ldu) {
+    __shared__ float t[64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k0 = blockIdx.x * 64, j0 = blockIdx.y * 64, k = k0 + lane;
+    for (int q = 0; q < 16; ++q) {
+        const int jl = 16 * w + q, j = j0 + jl;
+        float v = 0.f;
+        if (j < nl && k < nbr) {
+            const int c = cot[j];
+            double d = S[(size_t)ea[j] * lds + k] - S[(size_t)eb[j] * lds + k];
+            if (c >= 0 && c < nbr) d += (double)Lb[(size_t)c * ldb + k];
+            v = (float)d;
+        }
+        t[jl][lane] = v;
+    }
+    __syncthreads();
+    for (int q = 0; q < 16; ++q) {
+        const int kl = 16 * w + q, kk = k0 + kl, j = j0 + lane;
+        if (kk < nbr && j < nl) U[(size_t)kk * ldu + j] = t[lane][kl];
+    }
+}
+
+// Ll[i][j] = U[cot i][j] + S2[a_i][j] - S2[b_i][j] (rows i, lanes along j).
+__global__ __launch_bounds__(256) void tree_ll_kernel(const float *__restrict__ U, long long ldu, int nbr,
+                                                     const double *__restrict__ S2, long long lds,
+                                                     const int *__restrict__ cot, const int *__restrict__ ea,
+                                                     const int *__restrict__ eb, int nl, float *__restrict__ Ll,
+                                                     long long ldl) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= nl) return;
+    for (int i = blockIdx.y; i < nl; i += gridDim.y) {
+        const int c = cot[i];
+        double d = S2[(size_t)ea[i] * lds + j] - S2[(size_t)eb[i] * lds + j];
+        if (c >= 0 && c < nbr) d += (double)U[(size_t)c * ldu + j];
+        Ll[(size_t)i * ldl + j] = (float)d;
+    }
+}
+
 __global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long i = e / n, j = e % n;
@@ -1633,6 +1702,25 @@ void port_impedance(pq_loops *L, double omega, int P, const int32_t *port_cols, 
     }
 }
 
+// The handle's resistance rows (CSR over loops, FP64) and the flag buffer; synchronises.
+void upload_resistance(pq_loops *L, int n_loops, const int32_t *r_ptr, const int32_t *r_col, const double *r_val) {
+    const int rnnz = r_ptr[n_loops];
+    L->rptr.reserve((n_loops + 1) * 4); L->rcol.reserve(std::max(rnnz, 1) * 4); L->rval.reserve(std::max(rnnz, 1) * 8);
+    PQ_HIP(hipMemcpyAsync(L->rptr.p, r_ptr, (n_loops + 1) * 4, hipMemcpyHostToDevice, L->stream));
+    PQ_HIP(hipMemcpyAsync(L->rcol.p, r_col, (size_t)rnnz * 4, hipMemcpyHostToDevice, L->stream));
+    PQ_HIP(hipMemcpyAsync(L->rval.p, r_val, (size_t)rnnz * 8, hipMemcpyHostToDevice, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    L->singular.reserve(4);
+}
+
+void check_resistance(int n_loops, const int32_t *r_ptr, const int32_t *r_col) {
+    PQ_REQUIRE(r_ptr[0] == 0, "pq_loops_create: invalid resistance rows");
+    for (int i = 0; i < n_loops; ++i)
+        PQ_REQUIRE(r_ptr[i + 1] >= r_ptr[i], "pq_loops_create: resistance rows out of order");
+    for (int32_t t = 0; t < r_ptr[n_loops]; ++t)
+        PQ_REQUIRE(r_col[t] >= 0 && r_col[t] < n_loops, "pq_loops_create: resistance column out of range");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1647,11 +1735,7 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
                    "pq_loops_create: invalid sizes");
         PQ_REQUIRE(c_colptr[0] == 0 && c_colptr[n_loops] == c_nnz, "pq_loops_create: inconsistent loop basis");
         for (int64_t t = 0; t < c_nnz; ++t) PQ_REQUIRE(c_rows[t] >= 0, "pq_loops_create: negative branch index");
-        PQ_REQUIRE(r_ptr[0] == 0, "pq_loops_create: invalid resistance rows");
-        for (int i = 0; i < n_loops; ++i)
-            PQ_REQUIRE(r_ptr[i + 1] >= r_ptr[i], "pq_loops_create: resistance rows out of order");
-        for (int32_t t = 0; t < r_ptr[n_loops]; ++t)
-            PQ_REQUIRE(r_col[t] >= 0 && r_col[t] < n_loops, "pq_loops_create: resistance column out of range");
+        check_resistance(n_loops, r_ptr, r_col);
         DeviceScope scope(device);
         auto *L = new pq_loops();
         try {
@@ -1678,15 +1762,106 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
                                    L->Ll.as<float>(), L->ldl, n_loops);
             }
             PQ_HIP(hipGetLastError());
-            const int rnnz = r_ptr[n_loops];
-            L->rptr.reserve((n_loops + 1) * 4); L->rcol.reserve(std::max(rnnz, 1) * 4); L->rval.reserve(std::max(rnnz, 1) * 8);
-            PQ_HIP(hipMemcpyAsync(L->rptr.p, r_ptr, (n_loops + 1) * 4, hipMemcpyHostToDevice, L->stream));
-            PQ_HIP(hipMemcpyAsync(L->rcol.p, r_col, (size_t)rnnz * 4, hipMemcpyHostToDevice, L->stream));
-            PQ_HIP(hipMemcpyAsync(L->rval.p, r_val, (size_t)rnnz * 8, hipMemcpyHostToDevice, L->stream));
-            PQ_HIP(hipStreamSynchronize(L->stream));
-            L->singular.reserve(4);
+            upload_resistance(L, n_loops, r_ptr, r_col, r_val);
         } catch (...) {
             if (L->stream) (void)hipStreamDestroy(L->stream);
+            delete L;
+            throw;
+        }
+        *out = L;
+    });
+}
+
+int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tr, const float *Lb_dev, int64_t ldb,
+                         int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col, const double *r_val,
+                         pq_loops **out) {
+    return guard([&] {
+        PQ_REQUIRE(out && tr && Lb_dev && r_ptr && r_col && r_val && tr->parent && tr->up_branch && tr->up_sign &&
+                       tr->level_ptr && tr->level_nodes && tr->closing && tr->end_a && tr->end_b,
+                   "pq_loops_create_tree: null argument");
+        const int nn = tr->n_nodes, nlev = tr->n_levels;
+        PQ_REQUIRE(n_loops > 0 && n_free >= 0 && n_free <= n_loops && n_branch_rows > 0 && ldb >= n_branch_rows &&
+                       nn > 0 && nlev >= 0,
+                   "pq_loops_create_tree: invalid sizes");
+        // every listed node hangs under a root or a node of an earlier level, listed once
+        std::vector<char> done(nn, 0);
+        for (int x = 0; x < nn; ++x) {
+            PQ_REQUIRE(tr->parent[x] >= -1 && tr->parent[x] < nn, "pq_loops_create_tree: parent out of range");
+            if (tr->parent[x] < 0) done[x] = 1;
+        }
+        PQ_REQUIRE(tr->level_ptr[0] == 0, "pq_loops_create_tree: invalid levels");
+        for (int d = 0; d < nlev; ++d) {
+            PQ_REQUIRE(tr->level_ptr[d + 1] >= tr->level_ptr[d], "pq_loops_create_tree: levels out of order");
+            for (int e = tr->level_ptr[d]; e < tr->level_ptr[d + 1]; ++e) {
+                const int x = tr->level_nodes[e];
+                PQ_REQUIRE(x >= 0 && x < nn && tr->parent[x] >= 0 && done[tr->parent[x]] == 1 && !done[x],
+                           "pq_loops_create_tree: a level lists a node before its parent, twice, or a root");
+            }
+            for (int e = tr->level_ptr[d]; e < tr->level_ptr[d + 1]; ++e) done[tr->level_nodes[e]] = 1;
+        }
+        for (int j = 0; j < n_loops; ++j)
+            PQ_REQUIRE(tr->end_a[j] >= 0 && tr->end_a[j] < nn && tr->end_b[j] >= 0 && tr->end_b[j] < nn,
+                       "pq_loops_create_tree: loop end out of range");
+        check_resistance(n_loops, r_ptr, r_col);
+        DeviceScope scope(device);
+        auto *L = new pq_loops();
+        try {
+            L->device = device; L->nl = n_loops; L->nf = n_free;
+            L->ldl = (n_loops + 63) / 64 * 64;
+            PQ_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+            const int nbr = n_branch_rows, nlv = tr->level_ptr[nlev];
+            DevBuf par, upb, ups, lev, cot, ea, eb, S, U;
+            par.reserve((size_t)nn * 4); upb.reserve((size_t)nn * 4); ups.reserve((size_t)nn * 4);
+            lev.reserve((size_t)std::max(nlv, 1) * 4);
+            cot.reserve((size_t)n_loops * 4); ea.reserve((size_t)n_loops * 4); eb.reserve((size_t)n_loops * 4);
+            PQ_HIP(hipMemcpy(par.p, tr->parent, (size_t)nn * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(upb.p, tr->up_branch, (size_t)nn * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(ups.p, tr->up_sign, (size_t)nn * 4, hipMemcpyHostToDevice));
+            if (nlv) PQ_HIP(hipMemcpy(lev.p, tr->level_nodes, (size_t)nlv * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(cot.p, tr->closing, (size_t)n_loops * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(ea.p, tr->end_a, (size_t)n_loops * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(eb.p, tr->end_b, (size_t)n_loops * 4, hipMemcpyHostToDevice));
+            auto levels = [&](const float *Row, long long ldr, int width, double *Sd, long long lds) {
+                for (int d = 0; d < nlev; ++d) {
+                    const int c = tr->level_ptr[d + 1] - tr->level_ptr[d];
+                    if (!c) continue;
+                    hipLaunchKernelGGL(tree_level_kernel, dim3((width + 255) / 256, std::min(c, 65535)), dim3(256), 0,
+                                       L->stream, Row, ldr, nbr, width, lev.as<int>() + tr->level_ptr[d], c,
+                                       par.as<int>(), upb.as<int>(), ups.as<float>(), Sd, lds);
+                }
+            };
+            // U = Lb C (nbr x n_loops, FP32, row stride ldu)
+            const long long ldu = L->ldl, lds1 = (nbr + 63) / 64 * 64;
+            S.reserve((size_t)nn * lds1 * 8);
+            PQ_HIP(hipMemsetAsync(S.p, 0, (size_t)nn * lds1 * 8, L->stream));
+            levels(Lb_dev, ldb, nbr, S.as<double>(), lds1);
+            U.reserve((size_t)nbr * ldu * 4);
+            hipLaunchKernelGGL(tree_u_kernel, dim3((nbr + 63) / 64, (n_loops + 63) / 64), dim3(256), 0, L->stream, Lb_dev,
+                               (long long)ldb, nbr, S.as<double>(), lds1, cot.as<int>(), ea.as<int>(), eb.as<int>(),
+                               n_loops, U.as<float>(), ldu);
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            S.release();
+            // Ll = C^T U, the same prefix sums down the tree on the rows of U
+            const long long lds2 = L->ldl;
+            S.reserve((size_t)nn * lds2 * 8);
+            PQ_HIP(hipMemsetAsync(S.p, 0, (size_t)nn * lds2 * 8, L->stream));
+            levels(U.as<float>(), ldu, n_loops, S.as<double>(), lds2);
+            L->Ll.reserve((size_t)n_loops * L->ldl * 4);
+            hipLaunchKernelGGL(tree_ll_kernel, dim3((n_loops + 255) / 256, std::min(n_loops, 65535)), dim3(256), 0,
+                               L->stream, U.as<float>(), ldu, nbr, S.as<double>(), lds2, cot.as<int>(), ea.as<int>(),
+                               eb.as<int>(), n_loops, L->Ll.as<float>(), L->ldl);
+            {
+                const long long cnt = (long long)n_loops * n_loops;
+                hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
+                                   L->Ll.as<float>(), L->ldl, n_loops);
+            }
+            PQ_HIP(hipGetLastError());
+            upload_resistance(L, n_loops, r_ptr, r_col, r_val);
+        } catch (...) {
+            if (L->stream) {
+                (void)hipStreamSynchronize(L->stream);
+                (void)hipStreamDestroy(L->stream);
+            }
             delete L;
             throw;
         }

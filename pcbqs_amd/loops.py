@@ -93,6 +93,11 @@ class LoopBasis:
     tree: np.ndarray              # (nb,) bool
     positions: np.ndarray         # (nloops, 2) loop centroids
     cotree: np.ndarray            # (nloops,) the non-tree branch that closes each loop (coefficient +1)
+    parent: np.ndarray = None     # (n_nodes,) tree parent of each node, -1 at a root
+    upb: np.ndarray = None        # (n_nodes,) branch to the parent, -1 at a root
+    ups: np.ndarray = None        # (n_nodes,) +1 where that branch runs from the parent to the node
+    depth: np.ndarray = None      # (n_nodes,) depth below the root
+    cot_ends: np.ndarray = None   # (nloops, 2) the closing branch's ends (a, b): loop = a -> b, tree b -> a
 
 
 def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_terminals=True):
@@ -196,7 +201,8 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
     order = np.r_[order_free, src_cols]
     C = C[:, order].tocsc()
     port_branch = cot[src_cols]
-    return LoopBasis(C, None, len(free_cols), np.arange(len(free_cols), nl), tree, pos[order], cot[order]), port_branch
+    return LoopBasis(C, None, len(free_cols), np.arange(len(free_cols), nl), tree, pos[order], cot[order],
+                     parent, upb, ups, depth, ends[cot[order]]), port_branch
 
 
 def _attach(parent, upb, a, b, k, adj):

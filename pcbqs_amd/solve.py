@@ -8,6 +8,7 @@ X = -Z_ff^-1 Z_fs, here by lockstep GMRES (src/pcbqs/mqs_iterative.py:109-170, 2
 R = Re Zp and L = Im Zp / w (rl_equivalent, src/pcbqs/mqs_ports.py:72-86).
 """
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -24,10 +25,28 @@ PRECONDITIONERS = {'none': 0, 'block-jacobi': 1, 'two-level': 2}
 DIRECT_TOL, DIRECT_STEPS = 1e-12, 12
 
 
+def tree_arrays(basis):
+    """The pq_loop_tree of a LoopBasis (and the arrays it points into, to keep alive with it)."""
+    depth = np.asarray(basis.depth)
+    kids = np.flatnonzero(np.asarray(basis.parent) >= 0)
+    kids = kids[np.argsort(depth[kids], kind='stable')]
+    n_levels = int(depth.max()) if len(depth) else 0
+    level_ptr = np.searchsorted(depth[kids], np.arange(1, n_levels + 2)).astype(np.int32)
+    arrays = [np.ascontiguousarray(a, t) for a, t in
+              ((basis.parent, np.int32), (basis.upb, np.int32), (basis.ups, np.float32), (level_ptr, np.int32),
+               (kids, np.int32), (basis.cotree, np.int32), (basis.cot_ends[:, 0], np.int32),
+               (basis.cot_ends[:, 1], np.int32))]
+    par, upb, ups, lp, ln, cot, ea, eb = arrays
+    tree = _lib.LoopTree(len(par), ptr(par), ptr(upb), ptr(ups), n_levels, ptr(lp), ptr(ln), ptr(cot), ptr(ea),
+                         ptr(eb))
+    return tree, arrays
+
+
 class PortSolver:
     """One board on one device. `ports(freqs)` returns Zp (F, P, P) complex and per-frequency info."""
 
-    def __init__(self, board, device=0, near=3.0, outer_levels=1, precision=32, ideal_terminals=True, model=None):
+    def __init__(self, board, device=0, near=3.0, outer_levels=1, precision=32, ideal_terminals=True, model=None,
+                 projection=None):
         t0 = time.perf_counter()
         self.device = device
         self.model = model if model is not None else LoopModel(board, ideal_terminals=ideal_terminals)
@@ -59,17 +78,25 @@ class PortSolver:
         for k in range(nv):                                   # the via block
             row = np.ascontiguousarray(m.Lv[k], np.float32)
             call('pq_memcpy_h2d', Lb.offset((ne + k) * ldb + ne), ptr(row), row.nbytes)
-        C = m.basis.C.tocsc()
-        colptr = np.ascontiguousarray(C.indptr, np.int32)
-        rows = np.ascontiguousarray(C.indices, np.int32)
-        sign = np.ascontiguousarray(C.data, np.float32)
         Rl = m.Rl.tocsr()
         Rl.sort_indices()
         rptr, rcol, rval = (np.ascontiguousarray(Rl.indptr, np.int32), np.ascontiguousarray(Rl.indices, np.int32),
                             np.ascontiguousarray(Rl.data, np.float64))
         handle = ctypes.c_void_p()
-        call('pq_loops_create', device, m.n_loops, m.n_free, ptr(colptr), ptr(rows), ptr(sign), len(rows), Lb.ptr,
-             ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
+        self.projection = projection or os.environ.get('PCBQS_AMD_PROJECTION', 'tree')
+        if self.projection == 'tree':
+            tree, keep = tree_arrays(m.basis)
+            call('pq_loops_create_tree', device, m.n_loops, m.n_free, ctypes.byref(tree), Lb.ptr, ldb, nbr, ptr(rptr),
+                 ptr(rcol), ptr(rval), ctypes.byref(handle))
+        elif self.projection == 'columns':
+            C = m.basis.C.tocsc()
+            colptr = np.ascontiguousarray(C.indptr, np.int32)
+            rows = np.ascontiguousarray(C.indices, np.int32)
+            sign = np.ascontiguousarray(C.data, np.float32)
+            call('pq_loops_create', device, m.n_loops, m.n_free, ptr(colptr), ptr(rows), ptr(sign), len(rows), Lb.ptr,
+                 ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
+        else:
+            raise ValueError(f"projection must be 'tree' or 'columns', not {self.projection!r}")
         Lb.close()
         self._handle = handle
         self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
@@ -231,6 +258,16 @@ class PortSolver:
         chunk, us = ctypes.c_int(), ctypes.c_double()
         call('pq_loops_tuning', self._handle, nr, ctypes.byref(chunk), ctypes.byref(us))
         return chunk.value, us.value
+
+    def loop_matrix(self, rows=None):
+        """A host copy of the device's loop inductance Ll (FP32, the first `rows` rows, all columns)."""
+        n = self.model.n_loops
+        rows = n if rows is None else int(rows)
+        dev, ld = ctypes.c_void_p(), ctypes.c_int64()
+        call('pq_loops_matrix', self._handle, ctypes.byref(dev), ctypes.byref(ld))
+        out = np.empty((rows, ld.value), np.float32)
+        call('pq_memcpy_d2h', ptr(out), dev, out.nbytes)
+        return out[:, :n]
 
     def close(self):
         if getattr(self, '_handle', None) is not None and _lib._lib is not None:

@@ -1,0 +1,63 @@
+"""The loop basis's spanning tree, as the device projection reads it (pq_loop_tree): prefix sums of a
+symmetric branch matrix's rows down the tree give C^T A C exactly as the sparse loop matrix C does,
+for boards with vias and several ports."""
+import numpy as np
+import pytest
+
+from pcbqs_amd import board as boards
+from pcbqs_amd.loops import LoopModel
+from pcbqs_amd.solve import tree_arrays
+
+
+def tree_projection(A, nbr, arrays, n_nodes):
+    """C^T A C from the tree arrays, by the device kernels' formulas (rows >= nbr carry nothing)."""
+    parent, upb, ups, level_ptr, level_nodes, cot, ea, eb = arrays
+
+    def prefix(R):
+        S = np.zeros((n_nodes, R.shape[1]))
+        for d in range(len(level_ptr) - 1):
+            for x in level_nodes[level_ptr[d]:level_ptr[d + 1]]:
+                S[x] = S[parent[x]]
+                if 0 <= upb[x] < nbr:
+                    S[x] += ups[x] * R[upb[x]]
+        return S
+
+    def close(R, S):
+        out = S[ea] - S[eb]
+        ok = cot < nbr
+        out[ok] += R[cot[ok]]
+        return out
+
+    U = close(A, prefix(A)).T               # U[k][j] = A[cot j][k] + S_k(a_j) - S_k(b_j): A symmetric
+    return close(U, prefix(U))
+
+
+@pytest.mark.parametrize('make', [lambda: boards.board_c1(target=400), lambda: boards.board_batch(3, (300, 500)),
+                                  lambda: boards.board_batch(0, (800, 1200))])
+def test_tree_prefix_sums_give_the_loop_projection(make):
+    b = make()
+    m = LoopModel(b)
+    nbr = m.ne + m.nv
+    _, arrays = tree_arrays(m.basis)
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((nbr, nbr))
+    A = A + A.T
+    C = m.basis.C.tocsc()[:nbr]
+    ref = (C.T @ (C.T @ A).T).T
+    got = tree_projection(A, nbr, arrays, b.n_nodes)
+    assert got.shape == (m.n_loops, m.n_loops)
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_tree_levels_list_every_non_root_after_its_parent():
+    b = boards.board_batch(2, (300, 500))
+    m = LoopModel(b)
+    tree, (parent, upb, ups, level_ptr, level_nodes, cot, ea, eb) = tree_arrays(m.basis)
+    seen = parent < 0
+    assert level_ptr[0] == 0 and level_ptr[-1] == len(level_nodes) == (~seen).sum()
+    for d in range(tree.n_levels):
+        lvl = level_nodes[level_ptr[d]:level_ptr[d + 1]]
+        assert seen[parent[lvl]].all() and not seen[lvl].any()
+        seen[lvl] = True
+    assert seen.all()
+    assert set(np.unique(ups[parent >= 0])) <= {-1.0, 1.0}

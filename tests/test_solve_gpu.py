@@ -121,3 +121,32 @@ def test_fp16_coarse_bases_give_the_fp32_bases_solution(monkeypatch):
     w = 2 * np.pi * np.array(freqs)[:, None, None]
     assert rel(Z1.real, Zo.real) < 1e-6
     assert rel(Z1.imag / w, Zo.imag / w) < 1e-6
+
+
+@pytest.mark.parametrize('name', ['c1', 'c2-small', 'batch-11-ports', 'copper-reference'])
+def test_tree_projection_matches_the_column_projection(name):
+    """Ll = C^T Lb C by prefix sums down the loop basis's tree (the default) against the sparse
+    column product: the same FP32 matrix to within FP32 rounding of the FP64 sums, bitwise
+    symmetric, on boards with vias, many ports and non-ideal terminals."""
+    from pcbqs_amd.loops import LoopModel
+    from pcbqs_amd.solve import PortSolver
+    ideal = True
+    if name == 'copper-reference':
+        with np.load(os.path.join(GOLDEN, 'copper-reference.npz')) as data:
+            q = M.copper_reference_model(data['nodes'], data['tris'])
+        b = Board(q.nodes, q.tris, q.z_of, q.t_of, np.zeros(len(q.tris), int), q.node_of, q.terminals, q.vseg,
+                  q.via_nodes, q.ports)
+        ideal = False
+    else:
+        b = {'c1': boards.board_c1, 'c2-small': lambda: boards.board_c2(1, target=3000),
+             'batch-11-ports': lambda: boards.board_batch(0)}[name]()
+    m = LoopModel(b, ideal_terminals=ideal)
+    print(f'projection {name}: {m.ne + m.nv} branches, {m.n_loops} loops', flush=True)
+    L = {}
+    for how in ('columns', 'tree'):
+        s = PortSolver(b, device=0, ideal_terminals=ideal, model=m, projection=how)
+        L[how] = s.loop_matrix().astype(np.float64)
+        print(how, s.setup_seconds, flush=True)
+        s.close()
+    assert np.array_equal(L['tree'], L['tree'].T)
+    assert rel(L['tree'], L['columns']) < 1e-6

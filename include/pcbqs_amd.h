@@ -303,6 +303,38 @@ int pq_loops_tuning(pq_loops *loops, int NR, int *chunk, double *microseconds);
  * src/pcbqs/mqs_iterative.py:229-230). */
 int pq_loops_last_solve(pq_loops *loops, int64_t *iterations, int64_t *matvecs, double *residual, double *seconds);
 
+/* ---- compressed (H-matrix) operator product ------------------------------ */
+
+typedef struct pq_hmatrix pq_hmatrix;   /* a block-compressed operator on one device */
+
+/* Upload the blocks of a transpose-symmetric compressed operator of order n.
+ * Block q (meta[6q .. 6q+5] = a_off, la, b_off, lb, rank, same) holds rows
+ * idx[a_off .. a_off+la) and columns idx[b_off .. b_off+lb) of A: dense
+ * (rank -1, U la x lb) or low rank (rank 0..256, U la x rank then V rank x lb);
+ * unless `same`, it also stands for its transpose at (b, a). `factors` is the
+ * blocks' U (and V) row-major one after another, FP64 or (is_complex)
+ * interleaved complex FP64, n_factor elements. Replaces the packed layout of
+ * HMatrix._pack / HBlocks (src/pcbqs/hmatrix.py:235-270,
+ * src/pcbqs/core/hmatrix_kernels.hpp:103-160): inconsistent layouts return
+ * PQ_EINVAL. */
+int pq_hmatrix_create(int device, int64_t n, int64_t n_blocks, const int64_t *meta, const int32_t *idx,
+                      int64_t n_idx, const void *factors, int64_t n_factor, int is_complex, pq_hmatrix **out);
+int pq_hmatrix_destroy(pq_hmatrix *h);
+/* info[0..4] = n, contribution slots, factor bytes, dense blocks, low-rank blocks. */
+int pq_hmatrix_info(pq_hmatrix *h, int64_t info[5]);
+
+/* Y = A X, X and Y (n x k) row-major of the operator's element type, host
+ * memory, synchronised; deterministic (every output row sums its blocks'
+ * contributions in block order). Replaces HMatrix._matmat /
+ * HBlocks.apply / hmatrix_product (src/pcbqs/hmatrix.py:223-297,
+ * src/pcbqs/core/hmatrix_kernels.hpp:20-101). */
+int pq_hmatrix_apply(pq_hmatrix *h, int64_t k, const void *x, void *y);
+/* The same on device buffers, asynchronous on `stream` (NULL: the handle's). */
+int pq_hmatrix_apply_device(pq_hmatrix *h, int64_t k, const void *x_dev, void *y_dev, void *stream);
+/* Mean device milliseconds of `reps` products on the vectors of the last
+ * pq_hmatrix_apply (HIP events). Measurement only: no counterpart in the reference. */
+int pq_hmatrix_time(pq_hmatrix *h, int reps, double *ms);
+
 /* ---- Biot-Savart field --------------------------------------------------------
  * H (np, 3, K, 2) FP64 host (real, imaginary) at points pts (np, 3) of ns sheet
  * sources src (ns, 3 + 4K: position, then Jx re, Jx im, Jy re, Jy im per

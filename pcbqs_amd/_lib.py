@@ -72,6 +72,12 @@ _SIGNATURES = {
     'pq_loops_create_tree': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, I64, ctypes.c_int, P, P, P, P],
                              ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),
+    'pq_hmatrix_create': ([ctypes.c_int, I64, I64, P, P, I64, P, I64, ctypes.c_int, P], ctypes.c_int),
+    'pq_hmatrix_destroy': ([P], ctypes.c_int),
+    'pq_hmatrix_info': ([P, P], ctypes.c_int),
+    'pq_hmatrix_apply': ([P, I64, P, P], ctypes.c_int),
+    'pq_hmatrix_apply_device': ([P, I64, P, P, P], ctypes.c_int),
+    'pq_hmatrix_time': ([P, ctypes.c_int, P], ctypes.c_int),
     'pq_loops_matrix': ([P, P, P], ctypes.c_int),
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
                              ctypes.c_int),

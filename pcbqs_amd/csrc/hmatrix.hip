@@ -7,16 +7,17 @@
 // and columns a (A is transpose-symmetric), so it contributes U^T X[a] (dense) or V^T (U^T X[a])
 // to rows b. The sum is deterministic in two phases: every block writes its contribution rows
 // into slots of its own (forward rows, then transposed rows, slots numbered in block order), and
-// every output row then adds its slots in slot order. Results therefore do not depend on the
-// schedule, and agree with the compiled reference to rounding (the same terms, summed in a
-// different order inside a block).
+// every output row then adds its slots in a fixed order (a wave per row: each lane its share of
+// the slots in slot order, then a butterfly over the lanes). Results therefore do not depend on
+// the schedule, and agree with the compiled reference to rounding (the same terms, summed in a
+// different order).
 //
 // On the device: one workgroup per block (dense and low-rank blocks in separate launches, the
 // low-rank kernel holding V X[b] and U^T X[a] in LDS); each of the block's matrix-vector products
 // gives a thread one output row and all columns of the current column group, splitting the inner
 // dimension over thread groups (reduced through LDS in a fixed order) when the block has fewer
-// than 256 output rows. X and Y are row-major (n x k); k is processed in groups of 8 real or 4
-// complex columns, so the slot buffer holds one group.
+// than 256 output rows. X and Y are row-major (n x k); k is processed in groups of up to 8 real or
+// 4 complex columns, so the slot buffer holds one group.
 #include "pq_common.h"
 
 #include <algorithm>
@@ -59,11 +60,14 @@ __device__ void hm_gemv(const T *__restrict__ M, long long ld, int O, int K, Vec
     for (int o0 = 0; o0 < O; o0 += HM_THREADS) {
         const int Oc = min(HM_THREADS, O - o0);
         const int G = HM_THREADS / Oc;                // thread groups splitting K (uniform)
+        // adjacent lanes take adjacent outputs, so that the lanes of a wave share their gathered rows
         const int o = tid % Oc, g = tid / Oc;
+        const bool act = g < G;
         T acc[KC];
 #pragma unroll
         for (int c = 0; c < KC; ++c) hm_zero(acc[c]);
-        if (g < G) {
+        if (act) {
+#pragma unroll 4
             for (int q = g; q < K; q += G) {
                 const T m = ROW ? M[(size_t)(o0 + o) * ld + q] : M[(size_t)q * ld + o0 + o];
                 T v[KC];
@@ -73,9 +77,9 @@ __device__ void hm_gemv(const T *__restrict__ M, long long ld, int O, int K, Vec
             }
         }
         if (G == 1) {
-            if (g == 0) out(o0 + o, acc);
+            if (act) out(o0 + o, acc);
         } else {
-            if (g < G) {
+            if (act) {
 #pragma unroll
                 for (int c = 0; c < KC; ++c) part[g * Oc + o][c] = acc[c];
             }
@@ -111,6 +115,24 @@ template <class T, int KC> struct XRows {
     }
 };
 
+// Gathered rows of X staged in LDS once per block (HM_XS_BYTES), so that the block's products
+// read them from there instead of through an index load and a dependent global load each.
+constexpr int HM_XS_BYTES = 16384;
+template <class T, int KC> constexpr int hm_xs_rows() { return HM_XS_BYTES / (KC * (int)sizeof(T)); }
+
+template <class T, int KC> __device__ bool hm_stage(T (*xs)[KC], int rows, const XRows<T, KC> &X) {
+    if (rows > hm_xs_rows<T, KC>()) return false;  // uniform over the workgroup
+    __syncthreads();                                  // the previous product has finished reading xs
+    for (int q = threadIdx.x; q < rows; q += HM_THREADS) {
+        T v[KC];
+        X(q, v);
+#pragma unroll
+        for (int c = 0; c < KC; ++c) xs[q][c] = v[c];
+    }
+    __syncthreads();
+    return true;
+}
+
 template <class T, int KC> struct SlotRows {
     T *s;
     long long slot0;
@@ -144,12 +166,18 @@ __global__ __launch_bounds__(HM_THREADS) void hm_dense_kernel(const HBlock *__re
                                                              const T *__restrict__ F, const T *__restrict__ x,
                                                              long long k, int c0, int kc, T *__restrict__ s) {
     __shared__ T part[HM_THREADS][KC];
+    __shared__ T xs[hm_xs_rows<T, KC>()][KC];
     const HBlock b = blocks[list[blockIdx.x]];
     const T *u = F + b.u_off;
     const int la = (int)b.la, lb = (int)b.lb;
     XRows<T, KC> xa{x, idx + b.a_off, k, c0, kc}, xb{x, idx + b.b_off, k, c0, kc};
-    hm_gemv<T, KC, true>(u, lb, la, lb, xb, SlotRows<T, KC>{s, b.fwd}, part);
-    if (!b.same) hm_gemv<T, KC, false>(u, lb, lb, la, xa, SlotRows<T, KC>{s, b.tr}, part);
+    const LdsRows<T, KC> xl{xs};
+    if (hm_stage(xs, lb, xb)) hm_gemv<T, KC, true>(u, lb, la, lb, xl, SlotRows<T, KC>{s, b.fwd}, part);
+    else hm_gemv<T, KC, true>(u, lb, la, lb, xb, SlotRows<T, KC>{s, b.fwd}, part);
+    if (!b.same) {
+        if (hm_stage(xs, la, xa)) hm_gemv<T, KC, false>(u, lb, lb, la, xl, SlotRows<T, KC>{s, b.tr}, part);
+        else hm_gemv<T, KC, false>(u, lb, lb, la, xa, SlotRows<T, KC>{s, b.tr}, part);
+    }
 }
 
 // Phase 1, low-rank blocks: t = V X[b], q = U^T X[a] (LDS), then forward rows U t and, unless
@@ -161,37 +189,56 @@ __global__ __launch_bounds__(HM_THREADS) void hm_lowrank_kernel(const HBlock *__
                                                                long long k, int c0, int kc, T *__restrict__ s) {
     __shared__ T part[HM_THREADS][KC];
     __shared__ T tl[HM_RMAX][KC], ql[HM_RMAX][KC];
+    __shared__ T xs[hm_xs_rows<T, KC>()][KC];
     const HBlock b = blocks[list[blockIdx.x]];
     const T *u = F + b.u_off, *v = F + b.v_off;
     const int la = (int)b.la, lb = (int)b.lb, r = (int)b.rank;
     XRows<T, KC> xa{x, idx + b.a_off, k, c0, kc}, xb{x, idx + b.b_off, k, c0, kc};
-    hm_gemv<T, KC, true>(v, lb, r, lb, xb, LdsStore<T, KC>{tl}, part);
-    if (!b.same) hm_gemv<T, KC, false>(u, r, r, la, xa, LdsStore<T, KC>{ql}, part);
+    const LdsRows<T, KC> xl{xs};
+    if (hm_stage(xs, lb, xb)) hm_gemv<T, KC, true>(v, lb, r, lb, xl, LdsStore<T, KC>{tl}, part);
+    else hm_gemv<T, KC, true>(v, lb, r, lb, xb, LdsStore<T, KC>{tl}, part);
+    if (!b.same) {
+        if (hm_stage(xs, la, xa)) hm_gemv<T, KC, false>(u, r, r, la, xl, LdsStore<T, KC>{ql}, part);
+        else hm_gemv<T, KC, false>(u, r, r, la, xa, LdsStore<T, KC>{ql}, part);
+    }
     __syncthreads();
     hm_gemv<T, KC, true>(u, r, la, r, LdsRows<T, KC>{tl}, SlotRows<T, KC>{s, b.fwd}, part);
     if (!b.same) hm_gemv<T, KC, false>(v, lb, lb, r, LdsRows<T, KC>{ql}, SlotRows<T, KC>{s, b.tr}, part);
 }
 
-// Phase 2: Y[row] = sum of the row's slots in slot order.
+__device__ __forceinline__ double hm_shfl_xor(double v, int m) { return __shfl_xor(v, m); }
+__device__ __forceinline__ cd hm_shfl_xor(const cd &v, int m) { return cd{__shfl_xor(v.re, m), __shfl_xor(v.im, m)}; }
+
+// Phase 2: Y[row] = the sum of the row's slots, one wave per row: lane l adds slots l, l + 64, ...
+// in slot order, then a fixed butterfly adds the lanes (deterministic, schedule-independent).
 template <class T, int KC>
 __global__ __launch_bounds__(HM_THREADS) void hm_rows_kernel(const long long *__restrict__ rp,
                                                             const long long *__restrict__ sl, const T *__restrict__ s,
                                                             long long n, long long k, int c0, int kc,
                                                             T *__restrict__ y) {
-    const long long row = (long long)blockIdx.x * HM_THREADS + threadIdx.x;
-    if (row >= n) return;
+    const long long row = (long long)blockIdx.x * (HM_THREADS / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;                          // whole waves leave together
     T acc[KC];
 #pragma unroll
     for (int c = 0; c < KC; ++c) hm_zero(acc[c]);
-    for (long long p = rp[row]; p < rp[row + 1]; ++p) {
+    const long long p1 = rp[row + 1];
+    for (long long p = rp[row] + lane; p < p1; p += 64) {
         const T *src = s + (size_t)sl[p] * KC;
 #pragma unroll
         for (int c = 0; c < KC; ++c) hm_add(acc[c], src[c]);
     }
-    T *dst = y + (size_t)row * k + c0;
 #pragma unroll
-    for (int c = 0; c < KC; ++c)
-        if (c < kc) dst[c] = acc[c];
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) hm_add(acc[c], hm_shfl_xor(acc[c], m));
+    }
+    if (lane == 0) {
+        T *dst = y + (size_t)row * k + c0;
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            if (c < kc) dst[c] = acc[c];
+    }
 }
 
 }  // namespace
@@ -225,16 +272,25 @@ template <class T, int KC> void hm_apply(pq_hmatrix *h, long long k, const void 
                                h->blocks.as<HBlock>(), h->lr_list.as<int>(), h->idx.as<int>(), h->F.as<T>(), xd, k,
                                (int)c0, kc, h->s.as<T>());
         if (h->n)
-            hipLaunchKernelGGL((hm_rows_kernel<T, KC>), dim3((unsigned)((h->n + HM_THREADS - 1) / HM_THREADS)),
+            hipLaunchKernelGGL((hm_rows_kernel<T, KC>), dim3((unsigned)((h->n + 3) / 4)),
                                dim3(HM_THREADS), 0, st, h->rp.as<long long>(), h->sl.as<long long>(), h->s.as<T>(),
                                h->n, k, (int)c0, kc, yd);
         PQ_HIP(hipGetLastError());
     }
 }
 
+// The column group is the smallest of 2, 4, 8 real (1, 2, 4 complex) columns that holds k, or the
+// largest: fewer columns per group, fewer registers and no work on padding columns.
 void apply_device(pq_hmatrix *h, long long k, const void *x, void *y, hipStream_t st) {
-    if (h->complex_) hm_apply<cd, 4>(h, k, x, y, st);
-    else hm_apply<double, 8>(h, k, x, y, st);
+    if (h->complex_) {
+        if (k <= 1) hm_apply<cd, 1>(h, k, x, y, st);
+        else if (k <= 2) hm_apply<cd, 2>(h, k, x, y, st);
+        else hm_apply<cd, 4>(h, k, x, y, st);
+    } else {
+        if (k <= 2) hm_apply<double, 2>(h, k, x, y, st);
+        else if (k <= 4) hm_apply<double, 4>(h, k, x, y, st);
+        else hm_apply<double, 8>(h, k, x, y, st);
+    }
 }
 
 }  // namespace

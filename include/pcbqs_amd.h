@@ -63,6 +63,9 @@ int pq_free(void *ptr_dev);
 int pq_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int pq_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 int pq_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
+/* count elements of elem_bytes each, stride_bytes apart on the device, packed
+ * on the host (e.g. a matrix diagonal: stride = (ld + 1) * element size). */
+int pq_memcpy_d2h_strided(void *dst, const void *src_dev, size_t elem_bytes, size_t stride_bytes, size_t count);
 int pq_memset(void *dst_dev, int value, size_t bytes);
 int pq_synchronize(int device);
 /* A non-blocking stream of its own for a caller that runs several pipelines
@@ -164,6 +167,27 @@ int pq_potential_device(int precision, const void *M0_dev, int64_t nt, int64_t l
  * C = B^T P^-1 B on the potential matrix P. */
 int pq_spd_solve(int device, int n, const double *A_dev, int64_t lda, int nrhs, const double *B_host, double tol,
                  int max_steps, double *X_host, int64_t *steps, double *residual, double *factor_ms);
+
+typedef struct pq_csym pq_csym;     /* a factored complex-symmetric matrix        */
+
+/* Factor the complex-symmetric n x n matrix given by its lower-triangle
+ * entries (rows >= cols, unique; vals interleaved complex FP64) on the device:
+ * FP32 L D L^T without pivoting (the real part positive definite: a graph
+ * Laplacian A D^-1 A^T with Re 1/d > 0), keeping the entries in FP64 for
+ * refinement. *factor_ms: device time of forming and factoring; *bytes: the
+ * factor's storage. A zero pivot returns PQ_EINVAL. Replaces the per-frequency
+ * splu of the graph preconditioner's Laplacian (src/pcbqs/mqs_precond.py:50-51). */
+int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const int32_t *cols, const double *vals,
+                   pq_csym **out, double *factor_ms, int64_t *bytes);
+/* X = A^-1 B, B and X (n x nrhs) interleaved complex FP64 host arrays: FP32
+ * factor solves, FP64 residuals against the entries, corrections until the
+ * relative residual of every column is <= tol (at most max_steps, stopping
+ * when it no longer halves); a final residual above max(tol, 1e-10) returns
+ * PQ_ECONVERGE. Replaces lu.solve of the same preconditioner
+ * (src/pcbqs/mqs_precond.py:57). */
+int pq_csym_solve(pq_csym *h, int nrhs, const double *B, double tol, int max_steps, double *X, int64_t *steps,
+                  double *residual);
+int pq_csym_destroy(pq_csym *h);
 
 typedef struct pq_loops pq_loops;   /* a board's loop operator on one device      */
 

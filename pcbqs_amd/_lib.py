@@ -53,6 +53,7 @@ _SIGNATURES = {
     'pq_memcpy_h2d': ([P, P, SIZE], ctypes.c_int),
     'pq_memcpy_d2h': ([P, P, SIZE], ctypes.c_int),
     'pq_memcpy_d2d': ([P, P, SIZE], ctypes.c_int),
+    'pq_memcpy_d2h_strided': ([P, P, SIZE, SIZE, SIZE], ctypes.c_int),
     'pq_memset': ([P, ctypes.c_int, SIZE], ctypes.c_int),
     'pq_synchronize': ([ctypes.c_int], ctypes.c_int),
     'pq_stream_create': ([ctypes.c_int, P], ctypes.c_int),
@@ -72,6 +73,9 @@ _SIGNATURES = {
     'pq_loops_create_tree': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, I64, ctypes.c_int, P, P, P, P],
                              ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),
+    'pq_csym_create': ([ctypes.c_int, ctypes.c_int, I64, P, P, P, P, P, P], ctypes.c_int),
+    'pq_csym_solve': ([P, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P], ctypes.c_int),
+    'pq_csym_destroy': ([P], ctypes.c_int),
     'pq_hmatrix_create': ([ctypes.c_int, I64, I64, P, P, I64, P, I64, ctypes.c_int, P], ctypes.c_int),
     'pq_hmatrix_destroy': ([P], ctypes.c_int),
     'pq_hmatrix_info': ([P, P], ctypes.c_int),

@@ -650,6 +650,41 @@ template <class F> void dispatch_ports(int P, F &&f) {
     }
 }
 
+// A complex-symmetric matrix given by its lower-triangle entries (unique, row >= column): the
+// FP32 planes of the factor (zeroed beforehand).
+__global__ __launch_bounds__(256) void csym_scatter_kernel(const int *__restrict__ rows, const int *__restrict__ cols,
+                                                          const double *__restrict__ vals, long long nnz,
+                                                          float *__restrict__ Fr, float *__restrict__ Fi,
+                                                          long long ldf) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nnz) return;
+    const size_t o = (size_t)rows[t] * ldf + cols[t];
+    Fr[o] = (float)vals[2 * t];
+    Fi[o] = (float)vals[2 * t + 1];
+}
+
+// R = B - A X for P complex columns (rows [P real parts, P imaginary parts]), A in CSR with both
+// triangles, complex FP64; one thread per row, the row's entries in CSR order.
+__global__ __launch_bounds__(256) void csym_residual_kernel(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                           const double *__restrict__ av, int n, int P,
+                                                           const double *__restrict__ X,
+                                                           const double *__restrict__ B, double *__restrict__ R) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int NR = 2 * P;
+    for (int q = 0; q < P; ++q) {
+        double re = 0, im = 0;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) {
+            const double ar = av[2 * e], ai = av[2 * e + 1];
+            const double xr = X[(size_t)ci[e] * NR + q], xi = X[(size_t)ci[e] * NR + P + q];
+            re += ar * xr - ai * xi;
+            im += ar * xi + ai * xr;
+        }
+        R[(size_t)i * NR + q] = B[(size_t)i * NR + q] - re;
+        R[(size_t)i * NR + P + q] = B[(size_t)i * NR + P + q] - im;
+    }
+}
+
 }  // namespace
 
 void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl, const int *rptr, const int *rcol,
@@ -866,6 +901,210 @@ int pq_spd_solve(int device, int n, const double *A_dev, int64_t lda, int nrhs, 
         if (residual) *residual = worst;
         if (worst > std::max(tol, 1e-10))
             throw Error(PQ_ECONVERGE, "pq_spd_solve: the FP32 factor's refinement stalls at relative residual " +
+                                          std::to_string(worst));
+    });
+}
+
+}  // extern "C"
+
+// ---- a complex-symmetric solver on the device from sparse entries ------------------------------
+
+struct pq_csym {
+    int device = 0, n = 0;
+    hipStream_t st = nullptr, aux = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    DevBuf F, Xi, Wb, rp, ci, av;
+    LdlFactor f{};
+    double factor_ms = 0;
+};
+
+namespace {
+void csym_release(pq_csym *h) {
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    if (h->aux) (void)hipStreamSynchronize(h->aux);
+    for (hipEvent_t e : {h->ready, h->done})
+        if (e) (void)hipEventDestroy(e);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+}  // namespace
+
+extern "C" {
+
+int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const int32_t *cols, const double *vals,
+                   pq_csym **out, double *factor_ms, int64_t *bytes) {
+    return guard([&] {
+        PQ_REQUIRE(out && rows && cols && vals, "pq_csym_create: null argument");
+        PQ_REQUIRE(n >= 1 && nnz >= 0, "pq_csym_create: invalid sizes");
+        std::vector<long long> key((size_t)nnz);
+        std::vector<int> cnt((size_t)n + 1, 0);
+        for (int64_t t = 0; t < nnz; ++t) {
+            PQ_REQUIRE(rows[t] >= 0 && rows[t] < n && cols[t] >= 0 && cols[t] <= rows[t],
+                       "pq_csym_create: entries must lie in the lower triangle of an n x n matrix");
+            PQ_REQUIRE(std::isfinite(vals[2 * t]) && std::isfinite(vals[2 * t + 1]), "pq_csym_create: entry not finite");
+            key[(size_t)t] = (long long)rows[t] * n + cols[t];
+            ++cnt[(size_t)rows[t] + 1];
+            if (rows[t] != cols[t]) ++cnt[(size_t)cols[t] + 1];
+        }
+        std::sort(key.begin(), key.end());
+        PQ_REQUIRE(std::adjacent_find(key.begin(), key.end()) == key.end(), "pq_csym_create: duplicate entry");
+        // both triangles in CSR for the FP64 residual
+        for (int i = 0; i < n; ++i) cnt[(size_t)i + 1] += cnt[(size_t)i];
+        const int full = cnt[(size_t)n];
+        std::vector<int> ci((size_t)std::max(full, 1)), next(cnt.begin(), cnt.end() - 1);
+        std::vector<double> av((size_t)2 * std::max(full, 1));
+        auto put = [&](int r, int c, double re, double im) {
+            const int e = next[(size_t)r]++;
+            ci[(size_t)e] = c;
+            av[(size_t)2 * e] = re;
+            av[(size_t)2 * e + 1] = im;
+        };
+        for (int64_t t = 0; t < nnz; ++t) {
+            put(rows[t], cols[t], vals[2 * t], vals[2 * t + 1]);
+            if (rows[t] != cols[t]) put(cols[t], rows[t], vals[2 * t], vals[2 * t + 1]);
+        }
+        DeviceScope scope(device);
+        auto *h = new pq_csym();
+        try {
+            h->device = device; h->n = n;
+            PQ_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+            PQ_HIP(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+            PQ_HIP(hipEventCreateWithFlags(&h->ready, hipEventDisableTiming));
+            PQ_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+            const int nblk = (n + LB - 1) / LB;
+            const long long ldf = (long long)nblk * LB;
+            h->F.reserve((size_t)2 * n * ldf * 4);
+            h->Xi.reserve((size_t)2 * nblk * LB * LB * 4);
+            const size_t wsz = (size_t)n * 2 * LB;
+            h->Wb.reserve(4 * wsz * 4);
+            float *wb = h->Wb.as<float>();
+            h->f = LdlFactor{h->F.as<float>(), h->F.as<float>() + (size_t)n * ldf, ldf, n, h->Xi.as<float>(),
+                             h->Xi.as<float>() + (size_t)nblk * LB * LB, {wb, wb + 2 * wsz}, {wb + wsz, wb + 3 * wsz},
+                             h->aux, h->ready, h->done};
+            h->rp.reserve(cnt.size() * 4);
+            h->ci.reserve(ci.size() * 4);
+            h->av.reserve(av.size() * 8);
+            PQ_HIP(hipMemcpy(h->rp.p, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(h->ci.p, ci.data(), ci.size() * 4, hipMemcpyHostToDevice));
+            PQ_HIP(hipMemcpy(h->av.p, av.data(), av.size() * 8, hipMemcpyHostToDevice));
+            DevBuf r, c, v, bad;
+            r.reserve((size_t)std::max<int64_t>(nnz, 1) * 4);
+            c.reserve((size_t)std::max<int64_t>(nnz, 1) * 4);
+            v.reserve((size_t)std::max<int64_t>(nnz, 1) * 16);
+            bad.reserve(4);
+            if (nnz) {
+                PQ_HIP(hipMemcpy(r.p, rows, (size_t)nnz * 4, hipMemcpyHostToDevice));
+                PQ_HIP(hipMemcpy(c.p, cols, (size_t)nnz * 4, hipMemcpyHostToDevice));
+                PQ_HIP(hipMemcpy(v.p, vals, (size_t)nnz * 16, hipMemcpyHostToDevice));
+            }
+            hipEvent_t e0, e1;
+            PQ_HIP(hipEventCreate(&e0));
+            PQ_HIP(hipEventCreate(&e1));
+            PQ_HIP(hipEventRecord(e0, h->st));
+            PQ_HIP(hipMemsetAsync(h->F.p, 0, (size_t)2 * n * ldf * 4, h->st));
+            if (nnz)
+                hipLaunchKernelGGL(csym_scatter_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, h->st,
+                                   r.as<int>(), c.as<int>(), v.as<double>(), (long long)nnz, h->f.Fr, h->f.Fi, ldf);
+            PQ_HIP(hipMemsetAsync(bad.p, 0, 4, h->st));
+            ldl_factor(h->st, h->f, bad.as<int>());
+            PQ_HIP(hipEventRecord(e1, h->st));
+            int flag = 0;
+            PQ_HIP(hipMemcpyAsync(&flag, bad.p, 4, hipMemcpyDeviceToHost, h->st));
+            PQ_HIP(hipStreamSynchronize(h->st));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            h->factor_ms = ms;
+            PQ_REQUIRE(!flag, "pq_csym_create: a zero or non-finite pivot");
+        } catch (...) {
+            csym_release(h);
+            throw;
+        }
+        if (factor_ms) *factor_ms = h->factor_ms;
+        if (bytes) *bytes = (int64_t)(h->F.bytes + h->Xi.bytes);
+        *out = h;
+    });
+}
+
+int pq_csym_destroy(pq_csym *h) {
+    if (!h) return PQ_OK;
+    return guard([&] {
+        DeviceScope scope(h->device);
+        csym_release(h);
+    });
+}
+
+int pq_csym_solve(pq_csym *h, int nrhs, const double *B, double tol, int max_steps, double *X, int64_t *steps,
+                  double *residual) {
+    return guard([&] {
+        PQ_REQUIRE(h && B && X && nrhs >= 1 && tol > 0 && tol < 1 && max_steps >= 0, "pq_csym_solve: invalid arguments");
+        DeviceScope scope(h->device);
+        const int n = h->n;
+        hipStream_t st = h->st;
+        int64_t worst_steps = 0;
+        double worst = 0;
+        for (int c0 = 0; c0 < nrhs; c0 += 16) {
+            const int cn = std::min(16, nrhs - c0);
+            const int P = cn <= 8 ? cn : 16, NR = 2 * P;
+            const size_t count = (size_t)n * NR;
+            std::vector<double> hb(count, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int q = 0; q < cn; ++q) {
+                    hb[(size_t)i * NR + q] = B[2 * ((size_t)i * nrhs + c0 + q)];
+                    hb[(size_t)i * NR + P + q] = B[2 * ((size_t)i * nrhs + c0 + q) + 1];
+                }
+            DevBuf Bd, Xd, R, D, Y, Ys, nrm;
+            for (DevBuf *b : {&Bd, &Xd, &R, &D, &Y, &Ys}) b->reserve(count * 8);
+            nrm.reserve(NR * 8);
+            PQ_HIP(hipMemcpyAsync(Bd.p, hb.data(), count * 8, hipMemcpyHostToDevice, st));
+            std::vector<double> bn(NR), rn(NR);
+            auto colnorms = [&](const double *V, std::vector<double> &o) {
+                hipLaunchKernelGGL(colnorm2_kernel, dim3(NR), dim3(256), 0, st, V, n, NR, nrm.as<double>());
+                PQ_HIP(hipMemcpyAsync(o.data(), nrm.p, NR * 8, hipMemcpyDeviceToHost, st));
+                PQ_HIP(hipStreamSynchronize(st));
+            };
+            auto relres = [&]() {
+                double r = 0;
+                for (int q = 0; q < cn; ++q)
+                    r = std::max(r, std::sqrt((rn[q] + rn[P + q]) / std::max(bn[q] + bn[P + q], 1e-300)));
+                return r;
+            };
+            colnorms(Bd.as<double>(), bn);
+            ldl_solve(st, h->f, P, Bd.as<double>(), Y.as<double>(), Ys.as<double>(), Xd.as<double>());
+            double last = std::numeric_limits<double>::infinity(), res = 0;
+            int corr = 0;
+            for (int step = 0;; ++step) {
+                hipLaunchKernelGGL(csym_residual_kernel, dim3((n + 255) / 256), dim3(256), 0, st, h->rp.as<int>(),
+                                   h->ci.as<int>(), h->av.as<double>(), n, P, Xd.as<double>(), Bd.as<double>(),
+                                   R.as<double>());
+                PQ_HIP(hipGetLastError());
+                colnorms(R.as<double>(), rn);
+                res = relres();
+                if (!(res > tol) || step == max_steps || res > 0.5 * last) break;
+                last = res;
+                ldl_solve(st, h->f, P, R.as<double>(), Y.as<double>(), Ys.as<double>(), D.as<double>());
+                hipLaunchKernelGGL(add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, D.as<double>(),
+                                   count, Xd.as<double>());
+                PQ_HIP(hipGetLastError());
+                ++corr;
+            }
+            worst = std::max(worst, res);
+            worst_steps = std::max<int64_t>(worst_steps, corr);
+            std::vector<double> hx(count);
+            PQ_HIP(hipMemcpyAsync(hx.data(), Xd.p, count * 8, hipMemcpyDeviceToHost, st));
+            PQ_HIP(hipStreamSynchronize(st));
+            for (int i = 0; i < n; ++i)
+                for (int q = 0; q < cn; ++q) {
+                    X[2 * ((size_t)i * nrhs + c0 + q)] = hx[(size_t)i * NR + q];
+                    X[2 * ((size_t)i * nrhs + c0 + q) + 1] = hx[(size_t)i * NR + P + q];
+                }
+        }
+        if (steps) *steps = worst_steps;
+        if (residual) *residual = worst;
+        if (worst > std::max(tol, 1e-10))
+            throw Error(PQ_ECONVERGE, "pq_csym_solve: the FP32 factor's refinement stalls at relative residual " +
                                           std::to_string(worst));
     });
 }

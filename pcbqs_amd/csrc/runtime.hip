@@ -63,6 +63,14 @@ int pq_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     });
 }
 
+int pq_memcpy_d2h_strided(void *dst, const void *src, size_t elem_bytes, size_t stride_bytes, size_t count) {
+    return guard([&] {
+        PQ_REQUIRE(stride_bytes >= elem_bytes, "pq_memcpy_d2h_strided: stride below the element size");
+        if (elem_bytes && count)
+            PQ_HIP(hipMemcpy2D(dst, elem_bytes, src, stride_bytes, elem_bytes, count, hipMemcpyDeviceToHost));
+    });
+}
+
 int pq_memcpy_d2d(void *dst, const void *src, size_t bytes) {
     return guard([&] {
         if (bytes) PQ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));

@@ -78,6 +78,11 @@ class PortSolver:
         for k in range(nv):                                   # the via block
             row = np.ascontiguousarray(m.Lv[k], np.float32)
             call('pq_memcpy_h2d', Lb.offset((ne + k) * ldb + ne), ptr(row), row.nbytes)
+        # the branches' self inductances (the graph preconditioner's D), from the device matrix's diagonal
+        self.branch_ldiag = np.zeros(m.nb)
+        diag = np.empty(nbr, np.float32)
+        call('pq_memcpy_d2h_strided', ptr(diag), Lb.ptr, 4, (ldb + 1) * 4, nbr)
+        self.branch_ldiag[:nbr] = diag
         Rl = m.Rl.tocsr()
         Rl.sort_indices()
         rptr, rcol, rval = (np.ascontiguousarray(Rl.indptr, np.int32), np.ascontiguousarray(Rl.indices, np.int32),
@@ -258,6 +263,15 @@ class PortSolver:
         chunk, us = ctypes.c_int(), ctypes.c_double()
         call('pq_loops_tuning', self._handle, nr, ctypes.byref(chunk), ctypes.byref(us))
         return chunk.value, us.value
+
+    def graph_preconditioner(self, frequency):
+        """The graph preconditioner (C_f^T D C_f)^-1 at one frequency, D the branches' self impedances
+        (src/pcbqs/mqs_iterative.py:337-341): (LinearOperator on the free loops, factor bytes)."""
+        from .precond import branch_diagonal, graph, graph_setup
+        p = getattr(self, '_graph_setup', None) or graph_setup(self.model)
+        self._graph_setup = p
+        d = branch_diagonal(self.model.R.diagonal(), self.branch_ldiag, 2 * np.pi * frequency)
+        return graph(p, d, device=self.device)
 
     def loop_matrix(self, rows=None):
         """A host copy of the device's loop inductance Ll (FP32, the first `rows` rows, all columns)."""

@@ -118,10 +118,9 @@ template <class T, int KC> struct XRows {
 // Gathered rows of X staged in LDS once per block (HM_XS_BYTES), so that the block's products
 // read them from there instead of through an index load and a dependent global load each.
 constexpr int HM_XS_BYTES = 16384;
-template <class T, int KC> constexpr int hm_xs_rows() { return HM_XS_BYTES / (KC * (int)sizeof(T)); }
 
-template <class T, int KC> __device__ bool hm_stage(T (*xs)[KC], int rows, const XRows<T, KC> &X) {
-    if (rows > hm_xs_rows<T, KC>()) return false;  // uniform over the workgroup
+template <class T, int KC> __device__ bool hm_stage(T (*xs)[KC], int rows, int cap, const XRows<T, KC> &X) {
+    if (rows > cap) return false;                     // uniform over the workgroup
     __syncthreads();                                  // the previous product has finished reading xs
     for (int q = threadIdx.x; q < rows; q += HM_THREADS) {
         T v[KC];
@@ -159,23 +158,29 @@ template <class T, int KC> struct LdsStore {
     }
 };
 
+// The kernels' LDS, sized per handle at launch (dynamic shared memory): the split-K partials
+// (256 rows), then for low-rank blocks V X[b] and U^T X[a] (the handle's largest rank each), then
+// the staged rows of X (the handle's largest block side, at most HM_XS_BYTES).
+extern __shared__ __align__(16) unsigned char hm_lds[];
+
 // Phase 1, dense blocks: forward rows U X[b]; unless `same`, transposed rows U^T X[a].
 template <class T, int KC>
 __global__ __launch_bounds__(HM_THREADS) void hm_dense_kernel(const HBlock *__restrict__ blocks,
                                                              const int *__restrict__ list, const int *__restrict__ idx,
                                                              const T *__restrict__ F, const T *__restrict__ x,
-                                                             long long k, int c0, int kc, T *__restrict__ s) {
-    __shared__ T part[HM_THREADS][KC];
-    __shared__ T xs[hm_xs_rows<T, KC>()][KC];
+                                                             long long k, int c0, int kc, T *__restrict__ s,
+                                                             int xs_rows) {
+    T (*part)[KC] = reinterpret_cast<T (*)[KC]>(hm_lds);
+    T (*xs)[KC] = part + HM_THREADS;
     const HBlock b = blocks[list[blockIdx.x]];
     const T *u = F + b.u_off;
     const int la = (int)b.la, lb = (int)b.lb;
     XRows<T, KC> xa{x, idx + b.a_off, k, c0, kc}, xb{x, idx + b.b_off, k, c0, kc};
     const LdsRows<T, KC> xl{xs};
-    if (hm_stage(xs, lb, xb)) hm_gemv<T, KC, true>(u, lb, la, lb, xl, SlotRows<T, KC>{s, b.fwd}, part);
+    if (hm_stage(xs, lb, xs_rows, xb)) hm_gemv<T, KC, true>(u, lb, la, lb, xl, SlotRows<T, KC>{s, b.fwd}, part);
     else hm_gemv<T, KC, true>(u, lb, la, lb, xb, SlotRows<T, KC>{s, b.fwd}, part);
     if (!b.same) {
-        if (hm_stage(xs, la, xa)) hm_gemv<T, KC, false>(u, lb, lb, la, xl, SlotRows<T, KC>{s, b.tr}, part);
+        if (hm_stage(xs, la, xs_rows, xa)) hm_gemv<T, KC, false>(u, lb, lb, la, xl, SlotRows<T, KC>{s, b.tr}, part);
         else hm_gemv<T, KC, false>(u, lb, lb, la, xa, SlotRows<T, KC>{s, b.tr}, part);
     }
 }
@@ -186,19 +191,21 @@ template <class T, int KC>
 __global__ __launch_bounds__(HM_THREADS) void hm_lowrank_kernel(const HBlock *__restrict__ blocks,
                                                                const int *__restrict__ list, const int *__restrict__ idx,
                                                                const T *__restrict__ F, const T *__restrict__ x,
-                                                               long long k, int c0, int kc, T *__restrict__ s) {
-    __shared__ T part[HM_THREADS][KC];
-    __shared__ T tl[HM_RMAX][KC], ql[HM_RMAX][KC];
-    __shared__ T xs[hm_xs_rows<T, KC>()][KC];
+                                                               long long k, int c0, int kc, T *__restrict__ s,
+                                                               int rmax, int xs_rows) {
+    T (*part)[KC] = reinterpret_cast<T (*)[KC]>(hm_lds);
+    T (*tl)[KC] = part + HM_THREADS;
+    T (*ql)[KC] = tl + rmax;
+    T (*xs)[KC] = ql + rmax;
     const HBlock b = blocks[list[blockIdx.x]];
     const T *u = F + b.u_off, *v = F + b.v_off;
     const int la = (int)b.la, lb = (int)b.lb, r = (int)b.rank;
     XRows<T, KC> xa{x, idx + b.a_off, k, c0, kc}, xb{x, idx + b.b_off, k, c0, kc};
     const LdsRows<T, KC> xl{xs};
-    if (hm_stage(xs, lb, xb)) hm_gemv<T, KC, true>(v, lb, r, lb, xl, LdsStore<T, KC>{tl}, part);
+    if (hm_stage(xs, lb, xs_rows, xb)) hm_gemv<T, KC, true>(v, lb, r, lb, xl, LdsStore<T, KC>{tl}, part);
     else hm_gemv<T, KC, true>(v, lb, r, lb, xb, LdsStore<T, KC>{tl}, part);
     if (!b.same) {
-        if (hm_stage(xs, la, xa)) hm_gemv<T, KC, false>(u, r, r, la, xl, LdsStore<T, KC>{ql}, part);
+        if (hm_stage(xs, la, xs_rows, xa)) hm_gemv<T, KC, false>(u, r, r, la, xl, LdsStore<T, KC>{ql}, part);
         else hm_gemv<T, KC, false>(u, r, r, la, xa, LdsStore<T, KC>{ql}, part);
     }
     __syncthreads();
@@ -250,6 +257,7 @@ struct pq_hmatrix {
     long long n = 0, nslots = 0, n_factor = 0;
     bool complex_ = false;
     int n_dense = 0, n_lowrank = 0;
+    long long rmax = 0, side_dense = 0, side_lr = 0;  // largest rank; largest block side of each kind
     hipStream_t stream = nullptr;
     pq::DevBuf blocks, idx, rp, sl, F, dense_list, lr_list, s, x, y;
     long long xk = 0;                              // columns the x / y buffers were last sized for
@@ -261,16 +269,20 @@ namespace {
 template <class T, int KC> void hm_apply(pq_hmatrix *h, long long k, const void *x, void *y, hipStream_t st) {
     const T *xd = static_cast<const T *>(x);
     T *yd = static_cast<T *>(y);
+    const int cap = HM_XS_BYTES / (KC * (int)sizeof(T));
+    const int xs_dense = (int)std::min<long long>(h->side_dense, cap), xs_lr = (int)std::min<long long>(h->side_lr, cap);
+    const size_t row = KC * sizeof(T);
+    const size_t lds_dense = (HM_THREADS + xs_dense) * row, lds_lr = (HM_THREADS + 2 * h->rmax + xs_lr) * row;
     for (long long c0 = 0; c0 < k; c0 += KC) {
         const int kc = (int)std::min<long long>(KC, k - c0);
         if (h->n_dense)
-            hipLaunchKernelGGL((hm_dense_kernel<T, KC>), dim3(h->n_dense), dim3(HM_THREADS), 0, st,
+            hipLaunchKernelGGL((hm_dense_kernel<T, KC>), dim3(h->n_dense), dim3(HM_THREADS), lds_dense, st,
                                h->blocks.as<HBlock>(), h->dense_list.as<int>(), h->idx.as<int>(), h->F.as<T>(), xd, k,
-                               (int)c0, kc, h->s.as<T>());
+                               (int)c0, kc, h->s.as<T>(), xs_dense);
         if (h->n_lowrank)
-            hipLaunchKernelGGL((hm_lowrank_kernel<T, KC>), dim3(h->n_lowrank), dim3(HM_THREADS), 0, st,
+            hipLaunchKernelGGL((hm_lowrank_kernel<T, KC>), dim3(h->n_lowrank), dim3(HM_THREADS), lds_lr, st,
                                h->blocks.as<HBlock>(), h->lr_list.as<int>(), h->idx.as<int>(), h->F.as<T>(), xd, k,
-                               (int)c0, kc, h->s.as<T>());
+                               (int)c0, kc, h->s.as<T>(), (int)h->rmax, xs_lr);
         if (h->n)
             hipLaunchKernelGGL((hm_rows_kernel<T, KC>), dim3((unsigned)((h->n + 3) / 4)),
                                dim3(HM_THREADS), 0, st, h->rp.as<long long>(), h->sl.as<long long>(), h->s.as<T>(),
@@ -359,6 +371,14 @@ int pq_hmatrix_create(int device, int64_t n, int64_t n_blocks, const int64_t *me
         try {
             h->device = device; h->n = n; h->nslots = so; h->n_factor = n_factor; h->complex_ = is_complex != 0;
             h->n_dense = (int)dense.size(); h->n_lowrank = (int)lowrank.size();
+            for (const HBlock &b : hb) {
+                const long long side = std::max(b.la, b.lb);
+                if (b.rank < 0) h->side_dense = std::max(h->side_dense, side);
+                else {
+                    h->side_lr = std::max(h->side_lr, side);
+                    h->rmax = std::max(h->rmax, b.rank);
+                }
+            }
             const size_t es = h->complex_ ? 16 : 8;
             PQ_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             auto up = [&](DevBuf &d, const void *src, size_t bytes) {

@@ -15,6 +15,7 @@
 // Complex products are four real MFMA products on split real / imaginary planes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <string>
 #include <type_traits>
@@ -75,6 +76,8 @@ __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, lon
     __shared__ float Ar[LB][LB + 1], Ai[LB][LB + 1], Xr[LB][LB + 1], Xi[LB][LB + 1];
     __shared__ float Wr[LB - 16][17], Wi[LB - 16][17];
     __shared__ float Tr[16][LB + 1], Ti[16][LB + 1];
+    // on the factorisation's critical path beside the bulk trailing update: its waves issue first
+    __builtin_amdgcn_s_setprio(3);
     const int t = threadIdx.x;
     for (int e = t; e < LB * LB; e += 256) {
         const int r = e >> 6, c = e & 63;
@@ -310,6 +313,7 @@ __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, lo
                                                        int ldw) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
     __shared__ float qr[LB], qi[LB];
+    __builtin_amdgcn_s_setprio(3);                    // on the critical path, as the diagonal block
     const int I0 = k + LB + LB * blockIdx.x;
     if (threadIdx.x < LB) {
         const size_t dg = (size_t)(k + threadIdx.x) * ldf + k + threadIdx.x;
@@ -687,6 +691,24 @@ __global__ __launch_bounds__(256) void csym_residual_kernel(const int *__restric
 
 }  // namespace
 
+void ldl_create_aux(hipStream_t *aux) {
+    const char *env = std::getenv("PCBQS_AMD_LDL_CUMASK");
+    int dev = 0, cus = 0;
+    PQ_HIP(hipGetDevice(&dev));
+    PQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if ((env && env[0] == '0') || cus < 64) {
+        PQ_HIP(hipStreamCreateWithFlags(aux, hipStreamNonBlocking));
+        return;
+    }
+    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xffffffffu);
+    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1;
+    for (int i = 0; i < 8; ++i) {
+        const int cu = 32 * i + i;
+        if (cu < cus) mask[(size_t)cu / 32] &= ~(1u << (cu % 32));
+    }
+    PQ_HIP(hipExtStreamCreateWithCUMask(aux, (uint32_t)mask.size(), mask.data()));
+}
+
 void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl, const int *rptr, const int *rcol,
               const double *rval, double omega) {
     const int n = f.n;
@@ -810,7 +832,7 @@ int pq_spd_solve(int device, int n, const double *A_dev, int64_t lda, int nrhs, 
             }
         } cleanup{st, aux, ready, done, e0, e1};
         PQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        PQ_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+        ldl_create_aux(&aux);
         PQ_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
         PQ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         PQ_HIP(hipEventCreate(&e0));
@@ -969,7 +991,7 @@ int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const in
         try {
             h->device = device; h->n = n;
             PQ_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
-            PQ_HIP(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+            ldl_create_aux(&h->aux);
             PQ_HIP(hipEventCreateWithFlags(&h->ready, hipEventDisableTiming));
             PQ_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
             const int nblk = (n + LB - 1) / LB;

@@ -23,6 +23,13 @@ struct LdlFactor {
     hipEvent_t ready, done;
 };
 
+// The factorisation's second stream (the bulk trailing updates). It runs on all CUs but eight, one
+// taken from every 32 (CU 32 i + i, so that both a group-major and an interleaved numbering of CUs
+// over the chiplets leave one free per chiplet): the diagonal blocks and panels on the critical path
+// then find an idle CU instead of sharing one with update workgroups. PCBQS_AMD_LDL_CUMASK=0 gives
+// it every CU.
+void ldl_create_aux(hipStream_t *aux);
+
 // F = Rl + j omega Ll on the first n loops (lower triangle): Ll FP32 (row stride ldl), Rl CSR FP64.
 void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl, const int *rptr, const int *rcol,
               const double *rval, double omega);

@@ -2198,7 +2198,7 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
             L->Xldl.reserve((size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK * 4);
             L->Wldl.reserve((size_t)4 * n * 2 * LDL_BLOCK * 4);
             if (!L->aux) {
-                PQ_HIP(hipStreamCreateWithFlags(&L->aux, hipStreamNonBlocking));
+                ldl_create_aux(&L->aux);
                 PQ_HIP(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
                 PQ_HIP(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
             }

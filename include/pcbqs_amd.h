@@ -371,6 +371,18 @@ int pq_hmatrix_time(pq_hmatrix *h, int reps, double *ms);
 int pq_biot_field(int device, int precision, int K, int64_t ns, const double *src, int64_t nseg, const double *seg,
                   int64_t np, const double *pts, double *H, double *kernel_ms);
 
+/* The near field on a grid: nlayers current densities J (layer, [Jx, Jy],
+ * sx, sy, K) interleaved complex FP64 on a uniform source grid of spacing dx,
+ * dy (host deposit), layer heights layer_z, convolved with the Biot-Savart
+ * kernel of each height difference by a linear fx x fy FFT on the device
+ * (fx >= 3 sx - 2, fy >= 3 sy - 2; hipFFT, FP64), summed over layers; H
+ * (nox, noy, nh, 3, K) interleaved complex = the full convolution's points
+ * (ox0 + step i, oy0 + step j). *kernel_ms: device time. Replaces the FFT
+ * part of h_grid (src/pcbqs/biot.py:108-181). */
+int pq_biot_grid(int device, int nlayers, const double *layer_z, int K, int sx, int sy, const double *J, double dx,
+                 double dy, int nh, const double *heights, int fx, int fy, int ox0, int oy0, int step, int nox,
+                 int noy, double *H, double *kernel_ms);
+
 /* ---- message-passing network (the learned coarse-space correction) -------
  * No counterpart in the reference. All pointers are device pointers; stream is
  * a hipStream_t (NULL: the default stream). Activations are FP32 row-major;

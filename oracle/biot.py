@@ -5,6 +5,14 @@ Restates, in numpy/scipy:
   h_tris         src/pcbqs/biot.py:24-51               (6-point rule per triangle)
   h_sheets       src/pcbqs/biot.py:54-79               (near triangles split 4x, twice)
   h_segments     src/pcbqs/biot.py:82-105              (finite straight filaments)
+  current_deposit src/pcbqs/_numerics/currents.py:28-61 (split, 6-point moments, cloud in cell)
+  h_grid         src/pcbqs/biot.py:108-181             (linear FFT convolution with the kernel)
+
+current_deposit and h_grid are pinned by tests/test_biot_grid_oracle.py: the deposit keeps each
+sheet's total current and current moment exactly (the rule is exact for linear currents, bilinear
+weights keep first moments); the FFT convolution equals the direct convolution sum; and the grid
+field agrees with h_sheets (direct Biot-Savart) to 1 % on the copper fixture's solution, the
+reference's own check (tests/test_mqs.py:122-133).
 
 Pinned by: the copper-reference.npz field (rtol 1e-10) and the magnetic-reference.npz spectral
 field (3 %), tests/test_copper_reference.py.
@@ -107,3 +115,79 @@ def h_segments(p0, p1, I, pts):
             dirn = np.cross(u[k], np.where(ok[:, None], perp / np.where(ok, d, 1)[:, None], 0))
             H += (dirn * g[:, None])[:, :, None] * I2[k][None, None, :]
     return H if multi else H[:, :, 0]
+
+
+def current_deposit(sheet, gx, gy, per_cell=2.0):
+    nodes, tris, a, b, _ = sheet
+    multi = np.ndim(a) > 1
+    a = np.asarray(a).reshape(len(tris), -1)
+    b = np.asarray(b).reshape(len(tris), 2, -1)
+    V = np.asarray(nodes, float)[np.asarray(tris)][:, :, :2]
+    dx, dy = gx[1] - gx[0], gy[1] - gy[0]
+    for _ in range(6):
+        e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+        size = np.sqrt(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]) / 2)
+        big = size > per_cell * min(dx, dy)
+        if not big.any():
+            break
+        V = np.concatenate([V[~big], split4(V[big])])
+        a = np.concatenate([a[~big], np.tile(a[big], (4, 1))])
+        b = np.concatenate([b[~big], np.tile(b[big], (4, 1, 1))])
+    e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+    area = 0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    Q = np.einsum('qk,tkc->tqc', BAR6, V)
+    mom = (a[:, None, None, :] * Q[..., None] + b[:, None, :, :]) * (area[:, None] * W6[None])[..., None, None]
+    fx, fy = (Q[..., 0].ravel() - gx[0]) / dx, (Q[..., 1].ravel() - gy[0]) / dy
+    i0, j0 = np.floor(fx).astype(int), np.floor(fy).astype(int)
+    tx, ty = fx - i0, fy - j0
+    nx, ny = len(gx), len(gy)
+    out = np.zeros((2, nx, ny, a.shape[1]), complex)
+    for di, dj, wgt in ((0, 0, (1 - tx) * (1 - ty)), (1, 0, tx * (1 - ty)), (0, 1, (1 - tx) * ty), (1, 1, tx * ty)):
+        ii, jj = i0 + di, j0 + dj
+        ok = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny)
+        for c in range(2):
+            np.add.at(out[c], (ii[ok], jj[ok]), mom[:, :, c, :].reshape(-1, a.shape[1])[ok] * wgt[ok, None])
+    out /= dx * dy
+    return out if multi else out[..., 0]
+
+
+def grid_setup(sheets, x, y, refine=1):
+    dx, dy = (x[1] - x[0]) / refine, (y[1] - y[0]) / refine
+    pts = np.concatenate([np.asarray(sh[0], float)[np.unique(np.asarray(sh[1]))][:, :2] for sh in sheets])
+    grids, offsets = [], []
+    for dim, axis, step in ((0, x, dx), (1, y, dy)):
+        lo = min(0, int(np.floor((pts[:, dim].min() - axis[0]) / step)) - 2)
+        hi = max(refine * (len(axis) - 1), int(np.ceil((pts[:, dim].max() - axis[0]) / step)) + 2)
+        grids.append(axis[0] + np.arange(lo, hi + 1) * step)
+        offsets.append(-lo)
+    return grids, offsets, (dx, dy)
+
+
+def h_grid(sheets, x, y, heights, segments=None, refine=1, per_cell=2.0):
+    from scipy.fft import fft2, ifft2, next_fast_len
+    x, y, heights = [np.asarray(v, float) for v in (x, y, np.atleast_1d(heights))]
+    multi = any(np.ndim(sh[2]) > 1 for sh in sheets)
+    (gx, gy), offsets, (dx, dy) = grid_setup(sheets, x, y, refine)
+    nx, ny = len(gx), len(gy)
+    shape = (next_fast_len(3 * nx - 2), next_fast_len(3 * ny - 2))
+    rx = np.arange(1 - nx, nx)[:, None] * dx
+    ry = np.arange(1 - ny, ny)[None, :] * dy
+    crop = (slice(nx - 1 + offsets[0], nx - 1 + offsets[0] + refine * len(x), refine),
+            slice(ny - 1 + offsets[1], ny - 1 + offsets[1] + refine * len(y), refine))
+    k = max(np.asarray(sh[2]).reshape(len(sh[1]), -1).shape[1] for sh in sheets)
+    out = np.zeros((len(x), len(y), len(heights), 3, k), complex)
+    for sh in sheets:
+        J = current_deposit(sh, gx, gy, per_cell)
+        J = J.reshape(2, nx, ny, -1)
+        sjx, sjy = fft2(J[0], s=shape, axes=(0, 1)), fft2(J[1], s=shape, axes=(0, 1))
+        for h, height in enumerate(heights):
+            dz = height - float(sh[4])
+            kern = dx * dy / (4 * np.pi * (rx * rx + ry * ry + dz * dz) ** 1.5)
+            kx, ky, kz = [fft2(v * kern, s=shape)[..., None] for v in (rx, ry, dz + 0 * rx)]
+            for c, spec in enumerate((sjy * kz, -sjx * kz, sjx * ky - sjy * kx)):
+                out[:, :, h, c] += ifft2(spec, axes=(0, 1))[crop]
+    if segments is not None and len(segments[0]):
+        X, Y, Z = np.meshgrid(x, y, heights, indexing='ij')
+        pts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+        out += np.asarray(h_segments(*segments, pts)).reshape(len(x), len(y), len(heights), 3, -1)
+    return out if multi else out[..., 0]

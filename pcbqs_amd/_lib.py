@@ -92,6 +92,9 @@ _SIGNATURES = {
     'pq_loops_set_coarse': ([P, ctypes.c_int, P, ctypes.c_int], ctypes.c_int),
     'pq_loops_coarse_matrices': ([P, P, P], ctypes.c_int),
     'pq_biot_field': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, I64, P, I64, P, I64, P, P, P], ctypes.c_int),
+    'pq_biot_grid': ([ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, DBL, DBL,
+                      ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),
     'pq_mpnn_gemm': ([ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, P, P], ctypes.c_int),
     'pq_mpnn_workspace': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
     'pq_mpnn_gemm_tn': ([ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, SIZE, P, P, P], ctypes.c_int),

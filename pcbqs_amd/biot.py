@@ -129,6 +129,102 @@ def h_segments(p0, p1, I, pts, precision=64, device=0):
     return h_field([], pts, (p0, p1, I), precision=precision, device=device)
 
 
+def current_deposit(sheet, gx, gy, K, per_cell=2.0):
+    """A sheet's current as densities Jx, Jy (nx, ny, K) complex (A/m) on the uniform grid gx, gy:
+    each triangle split in four while its size exceeds per_cell cells (at most six times), its
+    current a r + b integrated by the 6-point rule, each point's moment spread onto the four
+    surrounding grid points by bilinear (cloud-in-cell) weights, which keeps the total current and
+    its centre (src/pcbqs/_numerics/currents.py:28-61)."""
+    nodes, tris, a, b, _ = sheet
+    a = np.asarray(a).reshape(len(tris), -1)
+    b = np.asarray(b).reshape(len(tris), 2, -1)
+    V = np.asarray(nodes, float)[np.asarray(tris)][:, :, :2]
+    dx, dy = gx[1] - gx[0], gy[1] - gy[0]
+    for _ in range(6):
+        e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+        size = np.sqrt(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]) / 2)
+        big = size > per_cell * min(dx, dy)
+        if not big.any():
+            break
+        V = np.concatenate([V[~big], _split4(V[big])])
+        a = np.concatenate([a[~big], np.tile(a[big], (4, 1))])
+        b = np.concatenate([b[~big], np.tile(b[big], (4, 1, 1))])
+    e1, e2 = V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]
+    area = 0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    Q = np.einsum('qk,tkc->tqc', BAR6, V)                               # (M, 6, 2)
+    w = (area[:, None] * W6[None]).reshape(-1)
+    k = a.shape[1]
+    mx = ((a[:, None, :] * Q[..., 0:1] + b[:, None, 0, :]).reshape(-1, k)) * w[:, None]
+    my = ((a[:, None, :] * Q[..., 1:2] + b[:, None, 1, :]).reshape(-1, k)) * w[:, None]
+    fx, fy = (Q[..., 0].ravel() - gx[0]) / dx, (Q[..., 1].ravel() - gy[0]) / dy
+    i0, j0 = np.floor(fx).astype(np.int64), np.floor(fy).astype(np.int64)
+    tx, ty = fx - i0, fy - j0
+    nx, ny = len(gx), len(gy)
+    J = np.zeros((2, nx * ny, K), complex)
+    for di, dj, wgt in ((0, 0, (1 - tx) * (1 - ty)), (1, 0, tx * (1 - ty)), (0, 1, (1 - tx) * ty), (1, 1, tx * ty)):
+        ii, jj = i0 + di, j0 + dj
+        ok = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny)
+        cell = ii[ok] * ny + jj[ok]
+        np.add.at(J[0, :, :k], cell, mx[ok] * wgt[ok, None])
+        np.add.at(J[1, :, :k], cell, my[ok] * wgt[ok, None])
+    return J.reshape(2, nx, ny, K) / (dx * dy)
+
+
+def h_grid(sheets, x, y, heights, segments=None, refine=1, per_cell=2.0, device=0, timing=None):
+    """Open-boundary H (len(x), len(y), len(heights), 3[, k]) complex on uniform x, y grids at
+    absolute heights: every sheet's current deposited on a grid of spacing (x step / refine) that
+    covers the observation window and all the copper, convolved with the Biot-Savart kernel by FFT
+    on the device (pq_biot_grid: linear, not periodic, convolution), plus the segments' exact field
+    (src/pcbqs/biot.py:108-181)."""
+    from scipy.fft import next_fast_len
+    x, y, heights = [np.asarray(v, float) for v in (x, y, np.atleast_1d(heights))]
+    for axis in (x, y):
+        if (axis.ndim != 1 or len(axis) < 2 or not np.isfinite(axis).all() or np.diff(axis).min() <= 0 or
+                not np.allclose(np.diff(axis), axis[1] - axis[0], rtol=1e-8, atol=1e-15)):
+            raise ValueError('x and y must be increasing uniform grids (m)')
+    if heights.ndim != 1 or not np.isfinite(heights).all():
+        raise ValueError('heights must be finite absolute z coordinates (m)')
+    zs = np.array([float(sh[4]) for sh in sheets])
+    if len(zs) and np.any(np.isclose(heights[:, None], zs[None], rtol=0, atol=1e-15)):
+        raise ValueError('the FFT field must be taken away from copper planes')
+    if not isinstance(refine, (int, np.integer)) or refine < 1:
+        raise ValueError('refine must be a positive integer')
+    multi = any(np.ndim(sh[2]) > 1 for sh in sheets) or (segments is not None and np.ndim(segments[2]) > 1)
+    ks = [np.asarray(sh[2]).reshape(len(sh[1]), -1).shape[1] for sh in sheets]
+    if segments is not None:
+        Iseg = np.asarray(segments[2])
+        ks.append(Iseg.reshape(len(Iseg), -1).shape[1] if len(Iseg) else 1)
+    k = max(ks) if ks else 1
+    out = np.zeros((len(x), len(y), len(heights), 3, k), complex)
+    if sheets:
+        step = (x[1] - x[0]) / refine, (y[1] - y[0]) / refine
+        allnodes = np.concatenate([np.asarray(sh[0], float)[np.unique(np.asarray(sh[1]))][:, :2] for sh in sheets])
+        grids, offsets = [], []
+        for dim, axis, d in ((0, x, step[0]), (1, y, step[1])):
+            lo = min(0, int(np.floor((allnodes[:, dim].min() - axis[0]) / d)) - 2)
+            hi = max(refine * (len(axis) - 1), int(np.ceil((allnodes[:, dim].max() - axis[0]) / d)) + 2)
+            grids.append(axis[0] + np.arange(lo, hi + 1) * d)
+            offsets.append(-lo)
+        gx, gy = grids
+        sx, sy = len(gx), len(gy)
+        fx, fy = next_fast_len(3 * sx - 2), next_fast_len(3 * sy - 2)
+        J = np.ascontiguousarray(np.stack([current_deposit(sh, gx, gy, k, per_cell) for sh in sheets]))
+        H = np.zeros((len(x), len(y), len(heights), 3, k), complex)
+        ms = ctypes.c_double()
+        call('pq_biot_grid', device, len(sheets), ptr(np.ascontiguousarray(zs)), k, sx, sy, ptr(J), step[0], step[1],
+             len(heights), ptr(np.ascontiguousarray(heights)), fx, fy, sx - 1 + offsets[0], sy - 1 + offsets[1],
+             refine, len(x), len(y), ptr(H), ctypes.byref(ms))
+        out += H
+        if timing is not None:
+            timing.update(kernel_ms=ms.value, source_grid=(sx, sy), fft=(fx, fy))
+    if segments is not None and len(segments[0]):
+        X, Y, Z = np.meshgrid(x, y, heights, indexing='ij')
+        pts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+        Hs = h_field([], pts, segments, device=device)
+        out += np.asarray(Hs).reshape(len(x), len(y), len(heights), 3, -1)
+    return out if multi else out[..., 0]
+
+
 def main(argv=None):
     """Time the field of random RWG currents on the seeded plane pair at a grid of points."""
     import argparse

@@ -13,6 +13,9 @@
 // bound).
 #include "pq_common.h"
 
+#include <hipfft/hipfft.h>
+#include <vector>
+
 #include <chrono>
 
 namespace pq {
@@ -244,6 +247,185 @@ int pq_biot_field(int device, int precision, int K, int64_t ns, const double *sr
             biot_run<double>(K, ns, src, nseg, seg, np, pts, H, kernel_ms);
         else
             biot_run<float>(K, ns, src, nseg, seg, np, pts, H, kernel_ms);
+    });
+}
+
+}  // extern "C"
+
+// ---- the near-field grid: sheet currents on a uniform grid convolved with the Biot-Savart kernel ----
+//
+// Semantics: h_grid (src/pcbqs/biot.py:108-181). Each layer's current density (Jx, Jy on an
+// sx x sy grid of spacing dx, dy, deposited by the host) is convolved, linearly (zero padding to
+// an fx x fy FFT of at least 3 s - 2 points per axis), with the kernel of the layer's height
+// difference dz: G(r) = dx dy / (4 pi |r|^3) over the offsets r = (i dx, j dy, dz), |i| < sx,
+// |j| < sy; H = (Jy * Gz, -Jx * Gz, Jx * Gy - Jy * Gx), summed over layers in the spectrum, then
+// one inverse FFT per component and excitation and the observation window cropped out of the full
+// convolution (every `step`-th point from (ox0, oy0)).
+
+namespace {
+
+__global__ void bg_pad_kernel(const double2 *__restrict__ J, int sx, int sy, int K, int k, int fx, int fy,
+                              double2 *__restrict__ out) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)fx * fy) return;
+    const int i = (int)(e / fy), j = (int)(e % fy);
+    out[e] = (i < sx && j < sy) ? J[((size_t)i * sy + j) * K + k] : make_double2(0.0, 0.0);
+}
+
+__global__ void bg_green_kernel(double dx, double dy, double dz, int sx, int sy, int fx, int fy,
+                                double2 *__restrict__ Gx, double2 *__restrict__ Gy, double2 *__restrict__ Gz) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)fx * fy) return;
+    const int i = (int)(e / fy), j = (int)(e % fy);
+    double gx = 0, gy = 0, gz = 0;
+    if (i < 2 * sx - 1 && j < 2 * sy - 1) {
+        const double rx = (i - (sx - 1)) * dx, ry = (j - (sy - 1)) * dy;
+        const double r2 = rx * rx + ry * ry + dz * dz;
+        const double g = dx * dy / (4.0 * M_PI * r2 * sqrt(r2));
+        gx = rx * g; gy = ry * g; gz = dz * g;
+    }
+    Gx[e] = make_double2(gx, 0.0);
+    Gy[e] = make_double2(gy, 0.0);
+    Gz[e] = make_double2(gz, 0.0);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// A += (Jy Gz, -Jx Gz, Jx Gy - Jy Gx) in the spectrum, one excitation.
+__global__ void bg_accum_kernel(const double2 *__restrict__ SJx, const double2 *__restrict__ SJy,
+                                const double2 *__restrict__ Gx, const double2 *__restrict__ Gy,
+                                const double2 *__restrict__ Gz, long long count, double2 *__restrict__ Ax,
+                                double2 *__restrict__ Ay, double2 *__restrict__ Az) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    const double2 jx = SJx[e], jy = SJy[e], gx = Gx[e], gy = Gy[e], gz = Gz[e];
+    const double2 a = cmul(jy, gz), b = cmul(jx, gz), c = cmul(jx, gy), d = cmul(jy, gx);
+    Ax[e] = make_double2(Ax[e].x + a.x, Ax[e].y + a.y);
+    Ay[e] = make_double2(Ay[e].x - b.x, Ay[e].y - b.y);
+    Az[e] = make_double2(Az[e].x + c.x - d.x, Az[e].y + c.y - d.y);
+}
+
+// H[i][j][h][c][k] = A[ox0 + step i][oy0 + step j] / (fx fy) (the inverse FFT is unnormalised).
+__global__ void bg_crop_kernel(const double2 *__restrict__ A, int fy, double scale, int ox0, int oy0, int step,
+                               int nox, int noy, int nh, int h, int c, int K, int k, double2 *__restrict__ H) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nox * noy) return;
+    const int i = e / noy, j = e % noy;
+    const double2 v = A[(size_t)(ox0 + step * i) * fy + oy0 + step * j];
+    H[((((size_t)i * noy + j) * nh + h) * 3 + c) * K + k] = make_double2(v.x * scale, v.y * scale);
+}
+
+struct FftPlan {
+    hipfftHandle h = 0;
+    bool made = false;
+    ~FftPlan() {
+        if (made) (void)hipfftDestroy(h);
+    }
+};
+
+#define PQ_FFT(call)                                                                                \
+    do {                                                                                            \
+        hipfftResult r_ = (call);                                                                   \
+        if (r_ != HIPFFT_SUCCESS) throw Error(PQ_EDEVICE, std::string(#call) + " failed: " + std::to_string((int)r_)); \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int pq_biot_grid(int device, int nlayers, const double *layer_z, int K, int sx, int sy, const double *J, double dx,
+                 double dy, int nh, const double *heights, int fx, int fy, int ox0, int oy0, int step, int nox,
+                 int noy, double *H, double *kernel_ms) {
+    return guard([&] {
+        PQ_REQUIRE(layer_z && J && heights && H, "pq_biot_grid: null argument");
+        PQ_REQUIRE(nlayers >= 1 && K >= 1 && sx >= 1 && sy >= 1 && nh >= 1 && nox >= 1 && noy >= 1 && step >= 1,
+                   "pq_biot_grid: invalid sizes");
+        PQ_REQUIRE(dx > 0 && dy > 0 && std::isfinite(dx) && std::isfinite(dy), "pq_biot_grid: invalid spacing");
+        PQ_REQUIRE(fx >= 3 * sx - 2 && fy >= 3 * sy - 2, "pq_biot_grid: the FFT is shorter than the linear convolution");
+        PQ_REQUIRE(ox0 >= 0 && oy0 >= 0 && ox0 + (long long)step * (nox - 1) < 3LL * sx - 2 &&
+                       oy0 + (long long)step * (noy - 1) < 3LL * sy - 2,
+                   "pq_biot_grid: the observation window lies outside the convolution");
+        for (int h = 0; h < nh; ++h)
+            for (int l = 0; l < nlayers; ++l)
+                PQ_REQUIRE(std::isfinite(heights[h]) && heights[h] != layer_z[l],
+                           "pq_biot_grid: the field must be taken away from the copper planes");
+        DeviceScope scope(device);
+        hipStream_t st = nullptr;
+        PQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() {
+                if (s) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipStreamDestroy(s);
+                }
+            }
+        } sg{st};
+        const long long F = (long long)fx * fy;
+        const size_t fb = (size_t)F * sizeof(double2);
+        const size_t jcount = (size_t)sx * sy * K;
+        DevBuf Jd, S, G, A, Hd;
+        Jd.reserve(jcount * 2 * sizeof(double2));
+        S.reserve(fb * 2 * K * nlayers);             // the layers' current spectra
+        G.reserve(fb * 3);
+        A.reserve(fb * 3 * K);
+        Hd.reserve((size_t)nox * noy * nh * 3 * K * sizeof(double2));
+        FftPlan plan;
+        PQ_FFT(hipfftPlan2d(&plan.h, fx, fy, HIPFFT_Z2Z));
+        plan.made = true;
+        PQ_FFT(hipfftSetStream(plan.h, st));
+        hipEvent_t e0, e1;
+        PQ_HIP(hipEventCreate(&e0));
+        PQ_HIP(hipEventCreate(&e1));
+        const unsigned gridF = (unsigned)((F + 255) / 256);
+        double2 *Sp = S.as<double2>(), *Gp = G.as<double2>(), *Ap = A.as<double2>();
+        PQ_HIP(hipEventRecord(e0, st));
+        for (int l = 0; l < nlayers; ++l) {
+            PQ_HIP(hipMemcpyAsync(Jd.p, J + (size_t)l * 2 * jcount * 2, jcount * 2 * sizeof(double2),
+                                  hipMemcpyHostToDevice, st));
+            for (int c = 0; c < 2; ++c)
+                for (int k = 0; k < K; ++k) {
+                    double2 *dst = Sp + ((size_t)(l * 2 + c) * K + k) * F;
+                    hipLaunchKernelGGL(bg_pad_kernel, dim3(gridF), dim3(256), 0, st, Jd.as<double2>() + c * jcount,
+                                       sx, sy, K, k, fx, fy, dst);
+                    PQ_FFT(hipfftExecZ2Z(plan.h, reinterpret_cast<hipfftDoubleComplex *>(dst),
+                                         reinterpret_cast<hipfftDoubleComplex *>(dst), HIPFFT_FORWARD));
+                }
+        }
+        for (int h = 0; h < nh; ++h) {
+            PQ_HIP(hipMemsetAsync(Ap, 0, fb * 3 * K, st));
+            for (int l = 0; l < nlayers; ++l) {
+                hipLaunchKernelGGL(bg_green_kernel, dim3(gridF), dim3(256), 0, st, dx, dy, heights[h] - layer_z[l], sx,
+                                   sy, fx, fy, Gp, Gp + F, Gp + 2 * F);
+                for (int c = 0; c < 3; ++c)
+                    PQ_FFT(hipfftExecZ2Z(plan.h, reinterpret_cast<hipfftDoubleComplex *>(Gp + c * F),
+                                         reinterpret_cast<hipfftDoubleComplex *>(Gp + c * F), HIPFFT_FORWARD));
+                for (int k = 0; k < K; ++k)
+                    hipLaunchKernelGGL(bg_accum_kernel, dim3(gridF), dim3(256), 0, st,
+                                       Sp + ((size_t)(l * 2) * K + k) * F, Sp + ((size_t)(l * 2 + 1) * K + k) * F, Gp,
+                                       Gp + F, Gp + 2 * F, F, Ap + (size_t)(0 * K + k) * F, Ap + (size_t)(1 * K + k) * F,
+                                       Ap + (size_t)(2 * K + k) * F);
+            }
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < K; ++k) {
+                    double2 *a = Ap + (size_t)(c * K + k) * F;
+                    PQ_FFT(hipfftExecZ2Z(plan.h, reinterpret_cast<hipfftDoubleComplex *>(a),
+                                         reinterpret_cast<hipfftDoubleComplex *>(a), HIPFFT_BACKWARD));
+                    hipLaunchKernelGGL(bg_crop_kernel, dim3((nox * noy + 255) / 256), dim3(256), 0, st, a, fy,
+                                       1.0 / (double)F, ox0, oy0, step, nox, noy, nh, h, c, K, k, Hd.as<double2>());
+                }
+            PQ_HIP(hipGetLastError());
+        }
+        PQ_HIP(hipEventRecord(e1, st));
+        PQ_HIP(hipMemcpyAsync(H, Hd.p, (size_t)nox * noy * nh * 3 * K * sizeof(double2), hipMemcpyDeviceToHost, st));
+        PQ_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (kernel_ms) *kernel_ms = ms;
     });
 }
 

@@ -696,7 +696,7 @@ void ldl_create_aux(hipStream_t *aux) {
     int dev = 0, cus = 0;
     PQ_HIP(hipGetDevice(&dev));
     PQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if ((env && env[0] == '0') || cus < 64) {
+    if (!(env && env[0] == '1') || cus < 64) {
         PQ_HIP(hipStreamCreateWithFlags(aux, hipStreamNonBlocking));
         return;
     }

@@ -23,11 +23,11 @@ struct LdlFactor {
     hipEvent_t ready, done;
 };
 
-// The factorisation's second stream (the bulk trailing updates). It runs on all CUs but eight, one
-// taken from every 32 (CU 32 i + i, so that both a group-major and an interleaved numbering of CUs
-// over the chiplets leave one free per chiplet): the diagonal blocks and panels on the critical path
-// then find an idle CU instead of sharing one with update workgroups. PCBQS_AMD_LDL_CUMASK=0 gives
-// it every CU.
+// The factorisation's second stream (the bulk trailing updates): a plain non-blocking stream. With
+// PCBQS_AMD_LDL_CUMASK=1 it runs on all CUs but eight, one taken from every 32 (CU 32 i + i, one per
+// chiplet under either CU numbering), leaving them to the diagonal blocks and panels on the critical
+// path; that gained 1-3 % in a process with few streams, but a masked stream needs a hardware queue
+// of its own and cost 45 % in one with several (the queues are then shared), so it is off by default.
 void ldl_create_aux(hipStream_t *aux);
 
 // F = Rl + j omega Ll on the first n loops (lower triangle): Ll FP32 (row stride ldl), Rl CSR FP64.

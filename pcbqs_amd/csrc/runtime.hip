@@ -22,6 +22,11 @@ __global__ __launch_bounds__(64) void store_probe_kernel(float *out, long long n
     }
 }
 
+template <class T> __global__ void gather_strided_kernel(const T *src, size_t stride, size_t count, T *dst) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = src[i * stride];
+}
+
 extern "C" {
 
 const char *pq_last_error(void) { return last_error.c_str(); }
@@ -66,8 +71,22 @@ int pq_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 int pq_memcpy_d2h_strided(void *dst, const void *src, size_t elem_bytes, size_t stride_bytes, size_t count) {
     return guard([&] {
         PQ_REQUIRE(stride_bytes >= elem_bytes, "pq_memcpy_d2h_strided: stride below the element size");
-        if (elem_bytes && count)
-            PQ_HIP(hipMemcpy2D(dst, elem_bytes, src, stride_bytes, elem_bytes, count, hipMemcpyDeviceToHost));
+        PQ_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "pq_memcpy_d2h_strided: elements of 4 or 8 bytes");
+        PQ_REQUIRE(stride_bytes % elem_bytes == 0, "pq_memcpy_d2h_strided: stride not a multiple of the element");
+        if (!count) return;
+        // gathered on the device, then one contiguous copy (a 2-D copy of one-element rows goes row by row)
+        DevBuf tmp;
+        tmp.reserve(count * elem_bytes);
+        const size_t st = stride_bytes / elem_bytes;
+        const unsigned grid = (unsigned)((count + 255) / 256);
+        if (elem_bytes == 4)
+            hipLaunchKernelGGL(gather_strided_kernel<uint32_t>, dim3(grid), dim3(256), 0, 0,
+                               static_cast<const uint32_t *>(src), st, count, tmp.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(gather_strided_kernel<uint64_t>, dim3(grid), dim3(256), 0, 0,
+                               static_cast<const uint64_t *>(src), st, count, tmp.as<uint64_t>());
+        PQ_HIP(hipGetLastError());
+        PQ_HIP(hipMemcpy(dst, tmp.p, count * elem_bytes, hipMemcpyDeviceToHost));
     });
 }
 

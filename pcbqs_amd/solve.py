@@ -23,6 +23,16 @@ PRECONDITIONERS = {'none': 0, 'block-jacobi': 1, 'two-level': 2}
 # The direct solve's refinement: relative residual and correction count of the reference's MixedLU
 # (src/pcbqs/kernels.py:137: TOL, STEPS = 1e-12, 12).
 DIRECT_TOL, DIRECT_STEPS = 1e-12, 12
+# solver='auto' takes the direct solve up to this many free loops, and while a factorisation costs
+# less than this many operator products per port and frequency (src/pcbqs/mqs_iterative.py:30-31).
+DIRECT_LOOPS, PRODUCTS_PER_SOLVE = 8000, 60
+
+
+def port_method(n_free, ports, n_freq):
+    """'direct' or 'iterative' for solver='auto', the rule of the reference's port_method
+    (src/pcbqs/mqs_iterative.py:260-272): direct while the free loops number at most
+    min(8000, 60 x ports x frequencies)."""
+    return 'direct' if n_free <= min(DIRECT_LOOPS, PRODUCTS_PER_SOLVE * ports * n_freq) else 'iterative'
 
 
 def tree_arrays(basis):
@@ -194,14 +204,17 @@ class PortSolver:
               solver='iterative'):
         """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds).
 
-        solver: 'iterative' (lockstep GMRES with `precond`, to `tol`) or 'direct' (the free-loop
+        solver: 'iterative' (lockstep GMRES with `precond`, to `tol`), 'direct' (the free-loop
         system factored as L D L^T in FP32 on the device and refined in FP64 to a relative residual
         of 1e-12 in at most 12 corrections, as the reference's MixedLU; precond, tol, restart,
-        maxiter and x0 do not apply, and `iterations` counts the corrections).
+        maxiter and x0 do not apply, and `iterations` counts the corrections) or 'auto' (the
+        reference's choice between them, `port_method`).
         x0: optional initial iterates, (F, n_free, P) complex or a callable frequency -> (n_free, P)
         (the learned correction's prediction); GMRES starts from zero otherwise."""
-        if solver not in ('iterative', 'direct'):
-            raise ValueError("solver must be 'iterative' or 'direct'")
+        if solver not in ('iterative', 'direct', 'auto'):
+            raise ValueError("solver must be 'iterative', 'direct' or 'auto'")
+        if solver == 'auto':
+            solver = port_method(self.model.n_free, self.n_ports, len(np.atleast_1d(freqs)))
         P = self.n_ports
         Pd = self.device_ports
         cols = self.port_cols if Pd == P else np.ascontiguousarray(

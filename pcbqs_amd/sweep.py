@@ -15,8 +15,8 @@ import numpy as np
 from .loops import LoopModel
 
 
-def _host(board, grid, precond, solver):
-    model = LoopModel(board)
+def _host(board, grid, precond, solver, model=None):
+    model = LoopModel(board) if model is None else model
     phi = None
     if precond == 'two-level' and solver == 'iterative':
         from .coarse import loop_restrict, stream_patterns, via_loop_columns
@@ -25,9 +25,13 @@ def _host(board, grid, precond, solver):
 
 
 def _board(i, board, freqs, precond, grid, tol, device, graphs, solver_kind):
-    from .solve import PortSolver
+    from .solve import PortSolver, port_method
     t = time.perf_counter()
-    model, phi = _host(board, grid, precond, solver_kind)
+    model = None
+    if solver_kind == 'auto':
+        model = LoopModel(board)
+        solver_kind = port_method(model.n_free, len(board.ports), len(freqs))
+    model, phi = _host(board, grid, precond, solver_kind, model)
     solver = PortSolver(board, device=device, model=model)
     solver.use_graphs(graphs)
     if phi is not None:
@@ -38,17 +42,19 @@ def _board(i, board, freqs, precond, grid, tol, device, graphs, solver_kind):
     solve = time.perf_counter() - t
     solver.close()
     return dict(board=i, triangles=int(board.nt), layers=int(len(np.unique(board.layer_of))),
-                ports=int(len(board.ports)), free_loops=int(model.n_free), setup_s=setup, solve_s=solve,
+                ports=int(len(board.ports)), free_loops=int(model.n_free), solver=solver_kind, setup_s=setup,
+                solve_s=solve,
                 iterations=[r['iterations'] for r in info],
                 worst_residual=max(r['relative_residual'] for r in info), Z=Z)
 
 
 def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concurrency=4, graphs=False,
-          log=None, solver='iterative'):
+          log=None, solver='auto'):
     """(per-board results in board order, totals) of the port solves of every board at every
     frequency. solver='direct' factors each board's free-loop block per frequency on the device
     (FP32 L D L^T refined in FP64 to 1e-12, as the reference's dense path) instead of GMRES; precond,
-    grid and tol then do not apply. `concurrency` boards run at once, each on its own stream from its own host thread
+    grid and tol then do not apply. solver='auto' (the default, as the reference's port_method)
+    chooses per board: direct up to min(8000, 60 x ports x frequencies) free loops, else GMRES. `concurrency` boards run at once, each on its own stream from its own host thread
     (the library releases the interpreter during device calls). graphs=True (concurrency 1 only: a
     capture does not survive other threads' synchronising calls) replays each Arnoldi step as a
     captured graph; at these board sizes capturing a board's ~100 step graphs costs more than the
@@ -72,7 +78,8 @@ def sweep(boards, freqs, precond='two-level', grid=12, tol=1e-8, device=0, concu
     columns = sum(r['ports'] * len(freqs) for r in results)
     return results, dict(seconds=seconds, boards=len(results), frequencies=len(freqs), solves=solves,
                          port_columns=columns, port_columns_per_s=columns / seconds if seconds > 0 else 0.0,
-                         precond=precond, grid=grid, concurrency=int(concurrency), graphs=graphs, solver=solver)
+                         precond=precond, grid=grid, concurrency=int(concurrency), graphs=graphs, solver=solver,
+                         boards_direct=sum(r['solver'] == 'direct' for r in results))
 
 
 def main(argv=None):
@@ -86,7 +93,7 @@ def main(argv=None):
     ap.add_argument('--grid', type=int, default=12)
     ap.add_argument('--concurrency', type=int, default=4)
     ap.add_argument('--graphs', action='store_true')
-    ap.add_argument('--solver', default='iterative', choices=('iterative', 'direct'))
+    ap.add_argument('--solver', default='auto', choices=('auto', 'iterative', 'direct'))
     args = ap.parse_args(argv)
     freqs = np.logspace(3, 8, args.frequencies)
     bs = [boards.board_batch(s) for s in range(args.boards)]

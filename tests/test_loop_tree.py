@@ -61,3 +61,13 @@ def test_tree_levels_list_every_non_root_after_its_parent():
         seen[lvl] = True
     assert seen.all()
     assert set(np.unique(ups[parent >= 0])) <= {-1.0, 1.0}
+
+
+def test_automatic_solver_choice_follows_the_reference_rule():
+    """solver='auto': direct while the free loops number at most min(8000, 60 x ports x frequencies)
+    (src/pcbqs/mqs_iterative.py:260-272)."""
+    from pcbqs_amd.solve import port_method
+    assert port_method(2446, 16, 20) == 'direct'
+    assert port_method(8000, 8, 20) == 'direct' and port_method(8001, 8, 20) == 'iterative'
+    assert port_method(240, 4, 1) == 'direct' and port_method(241, 4, 1) == 'iterative'
+    assert port_method(9639, 4, 1) == 'iterative' and port_method(32211, 8, 100) == 'iterative'

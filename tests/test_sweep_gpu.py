@@ -1,6 +1,6 @@
 """The batched sweep over uneven multi-layer boards: every board's port impedances against the FP64
-oracle (1e-6), for 4 to 11 ports (9 to 15 are padded to 16 device columns), with two-level GMRES
-and with the direct solve."""
+oracle (1e-6), for 4 to 11 ports (9 to 15 are padded to 16 device columns), with two-level GMRES,
+with the direct solve, and with the reference's automatic choice between them."""
 import numpy as np
 import pytest
 
@@ -10,7 +10,7 @@ from tests.test_solve_gpu import oracle_ports, rel
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('solver', ['iterative', 'direct'])
+@pytest.mark.parametrize('solver', ['iterative', 'direct', 'auto'])
 def test_sweep_matches_the_oracle_on_uneven_boards(solver):
     from pcbqs_amd.sweep import sweep
     seeds = [3, 2, 0]                         # 4, 6 and 11 ports; 1.6k to 5.5k triangles; 4 layers

@@ -271,6 +271,13 @@ int pq_loops_solve_direct(pq_loops *loops, double omega, int P, const int32_t *p
  * L D L^T) that the last pq_loops_solve_direct used. No counterpart in the
  * reference. */
 int pq_loops_factor_time(pq_loops *loops, double *ms);
+/* Form and factor Z_ff = Rl + j omega Ll at n_omega frequencies together (the
+ * L D L^T kernels take the frequency as a grid dimension: one launch per panel
+ * step for all of them); pq_loops_solve_direct at any of these frequencies then
+ * refines on its factor instead of factoring. *ms: device time of the batch.
+ * Same factors as one pq_loops_solve_direct per frequency (the reference
+ * factors per frequency, MQS._solve_dense, src/pcbqs/mqs.py:743-808). */
+int pq_loops_factor_batch(pq_loops *loops, int n_omega, const double *omegas, double *ms);
 
 /* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
  * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the

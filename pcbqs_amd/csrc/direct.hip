@@ -72,10 +72,12 @@ __device__ inline void cinv(float dr, float di, float &qr, float &qi) {
 // then X's block rows below the first, X_I = -X_S,I (sum_{M < I} L_IM X_M), one block row at a
 // time. A zero or non-finite pivot sets *bad.
 __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, long long ldf, int k, int b, float *Lir,
-                                                      float *Lii, int *bad) {
+                                                      float *Lii, int *bad, long long sF, long long sX) {
     __shared__ float Ar[LB][LB + 1], Ai[LB][LB + 1], Xr[LB][LB + 1], Xi[LB][LB + 1];
     __shared__ float Wr[LB - 16][17], Wi[LB - 16][17];
     __shared__ float Tr[16][LB + 1], Ti[16][LB + 1];
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // matrix blockIdx.z of a batch
+    Lir += blockIdx.z * sX; Lii += blockIdx.z * sX;
     // on the factorisation's critical path beside the bulk trailing update: its waves issue first
     __builtin_amdgcn_s_setprio(3);
     const int t = threadIdx.x;
@@ -310,9 +312,12 @@ __device__ __attribute__((always_inline)) inline void tile_abt(const float *__re
 // X = L11^-1 (so W = A21 L11^-T = L21 D), then L21 = W D^-1 in place of A21.
 __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, long long ldf, int n, int k,
                                                        const float *Lir, const float *Lii, float *Wr, float *Wi,
-                                                       int ldw) {
+                                                       int ldw, long long sF, long long sX, long long sW) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
     __shared__ float qr[LB], qi[LB];
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // matrix blockIdx.z of a batch
+    Lir += blockIdx.z * sX; Lii += blockIdx.z * sX;
+    Wr += blockIdx.z * sW; Wi += blockIdx.z * sW;
     __builtin_amdgcn_s_setprio(3);                    // on the critical path, as the diagonal block
     const int I0 = k + LB + LB * blockIdx.x;
     if (threadIdx.x < LB) {
@@ -345,8 +350,10 @@ __global__ __launch_bounds__(256) void ldl_panel_kernel(float *Fr, float *Fi, lo
 // (tj >= tc; blockIdx.x indexes the triangle over ti - tc, tj - tc).
 __global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, long long ldf, int n, int k0, int kdim,
                                                         int m0, const float *Wr, const float *Wi, int ldw, int part,
-                                                        int tc) {
+                                                        int tc, long long sF, long long sW) {
     __shared__ float sA[2][LB][TP], sB[2][LB][TP];
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // matrix blockIdx.z of a batch
+    Wr += blockIdx.z * sW; Wi += blockIdx.z * sW;
     const int T = (n - m0 + LB - 1) / LB;
     int id = blockIdx.x, ti, tj;
     if (part == 1) {
@@ -730,15 +737,17 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
     const int n = f.n;
     constexpr int LDW = 2 * LB;
+    const unsigned nz = (unsigned)std::max(1, f.batch);
     auto diag = [&](int k) {
         const int b = std::min(LB, n - k);
-        hipLaunchKernelGGL(ldl_diag_kernel, dim3(1), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b,
-                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, bad);
+        hipLaunchKernelGGL(ldl_diag_kernel, dim3(1, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b,
+                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, bad, f.sF, f.sX);
     };
     auto panel = [&](int k, float *wr, float *wi) {
         const int tiles = (n - k - LB + LB - 1) / LB;
-        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
-                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, wr, wi, LDW);
+        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
+                           f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, wr, wi, LDW, f.sF,
+                           f.sX, f.sW);
     };
     if (n <= 0) return;
     diag(0);
@@ -748,22 +757,22 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
         panel(K, wr, wi);
         {   // the first panel's update of the second panel's columns (rows from K + 64)
             const int T = (n - K - LB + LB - 1) / LB;
-            hipLaunchKernelGGL(ldl_update_kernel, dim3(T), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, LB, K + LB, wr, wi,
-                               LDW, 1, 1);
+            hipLaunchKernelGGL(ldl_update_kernel, dim3(T, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, LB, K + LB,
+                               wr, wi, LDW, 1, 1, f.sF, f.sW);
         }
         diag(K + LB);
         if (K + 2 * LB >= n) break;
         panel(K + LB, wr + LB, wi + LB);
         const int m0 = K + 2 * LB, T = (n - m0 + LB - 1) / LB, tc = std::min(2, T);
         if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
-        hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)(tc * T - tc * (tc - 1) / 2)), dim3(256), 0, st, f.Fr, f.Fi,
-                           f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 1, tc);
+        hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)(tc * T - tc * (tc - 1) / 2), 1, nz), dim3(256), 0, st,
+                           f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 1, tc, f.sF, f.sW);
         bulk = T > tc;
         if (bulk) {
             PQ_HIP(hipEventRecord(f.ready, st));
             PQ_HIP(hipStreamWaitEvent(f.aux, f.ready, 0));
-            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2)), dim3(256), 0,
-                               f.aux, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc);
+            hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2), 1, nz), dim3(256),
+                               0, f.aux, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF, f.sW);
             PQ_HIP(hipEventRecord(f.done, f.aux));
         }
         diag(m0);

@@ -21,6 +21,10 @@ struct LdlFactor {
     float *Wr[2], *Wi[2];
     hipStream_t aux;
     hipEvent_t ready, done;
+    // `batch` matrices of the same order factored together (the kernels' blockIdx.z), matrix z's
+    // planes at Fr/Fi + z sF, its block inverses at Lir/Lii + z sX, its W buffers at Wr/Wi + z sW
+    int batch = 1;
+    long long sF = 0, sX = 0, sW = 0;
 };
 
 // The factorisation's second stream (the bulk trailing updates): a plain non-blocking stream. With
@@ -36,7 +40,7 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
 
 // Factor in place on stream st (the work is complete in st's order when it returns; f.aux has no
 // pending work left behind); *bad (device int, cleared by the caller) is set when a pivot is zero or
-// not finite.
+// not finite in any of the f.batch matrices.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
 
 // X = Z^-1 B for P complex columns (n, 2P) FP64 (real parts first), FP32 factor, FP64 arithmetic.

@@ -1118,6 +1118,11 @@ struct pq_loops {
     long long ldf = 0;
     double omega_ldl = -1;
     double factor_ms = 0;
+    // factors of several frequencies formed and factored together (pq_loops_factor_batch): matrix
+    // b at the frequency batch_omega[b], strides as LdlFactor's
+    DevBuf Fbat, Xbat, Wbat;
+    std::vector<double> batch_omega;
+    double batch_ms = 0;
 };
 
 namespace {
@@ -1713,6 +1718,17 @@ void upload_resistance(pq_loops *L, int n_loops, const int32_t *r_ptr, const int
     L->singular.reserve(4);
 }
 
+// The factor of matrix b of the last batch (pq_loops_factor_batch) as a single factor.
+LdlFactor batch_view(pq_loops *L, int b) {
+    const int n = L->nf, nblk = (n + LDL_BLOCK - 1) / LDL_BLOCK;
+    const long long ldf = L->ldf;
+    const size_t sF = (size_t)2 * n * ldf, sX = (size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK,
+                 sW = (size_t)4 * n * 2 * LDL_BLOCK, wsz = (size_t)n * 2 * LDL_BLOCK;
+    float *F = L->Fbat.as<float>() + sF * b, *X = L->Xbat.as<float>() + sX * b, *W = L->Wbat.as<float>() + sW * b;
+    return LdlFactor{F, F + (size_t)n * ldf, ldf, n, X, X + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,
+                     {W, W + 2 * wsz}, {W + wsz, W + 3 * wsz}, L->aux, L->ev_ready, L->ev_done};
+}
+
 void check_resistance(int n_loops, const int32_t *r_ptr, const int32_t *r_col) {
     PQ_REQUIRE(r_ptr[0] == 0, "pq_loops_create: invalid resistance rows");
     for (int i = 0; i < n_loops; ++i)
@@ -2163,6 +2179,61 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
 // relative residual is at most tol, for at most max_steps corrections, stopping early once a
 // correction no longer halves it; a result above 1e-10 then raises PQ_ECONVERGE. The factor is
 // kept for the next call at the same frequency.
+// Form and factor Z_ff at nw frequencies together (one launch per panel step for all of them);
+// later direct solves at these frequencies use these factors. *ms: device time of the batch.
+int pq_loops_factor_batch(pq_loops *L, int nw, const double *omegas, double *ms) {
+    return guard([&] {
+        PQ_REQUIRE(L && omegas && nw >= 1, "pq_loops_factor_batch: invalid arguments");
+        for (int b = 0; b < nw; ++b)
+            PQ_REQUIRE(std::isfinite(omegas[b]) && omegas[b] >= 0, "pq_loops_factor_batch: invalid frequency");
+        DeviceScope scope(L->device);
+        const int n = L->nf;
+        L->batch_omega.clear();
+        L->batch_ms = 0;
+        if (n <= 0) return;
+        const int nblk = (n + LDL_BLOCK - 1) / LDL_BLOCK;
+        const long long ldf = (long long)nblk * LDL_BLOCK;
+        const size_t sF = (size_t)2 * n * ldf, sX = (size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK,
+                     sW = (size_t)4 * n * 2 * LDL_BLOCK;
+        L->ldf = ldf;
+        L->Fbat.reserve(sF * nw * 4);
+        L->Xbat.reserve(sX * nw * 4);
+        L->Wbat.reserve(sW * nw * 4);
+        if (!L->aux) {
+            ldl_create_aux(&L->aux);
+            PQ_HIP(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
+            PQ_HIP(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
+        }
+        L->batch_omega.assign(omegas, omegas + nw);
+        hipEvent_t e0, e1;
+        PQ_HIP(hipEventCreate(&e0));
+        PQ_HIP(hipEventCreate(&e1));
+        PQ_HIP(hipEventRecord(e0, L->stream));
+        for (int b = 0; b < nw; ++b)
+            ldl_form(L->stream, batch_view(L, b), L->Ll.as<float>(), L->ldl, L->rptr.as<int>(), L->rcol.as<int>(),
+                     L->rval.as<double>(), omegas[b]);
+        LdlFactor f = batch_view(L, 0);
+        f.batch = nw;
+        f.sF = (long long)sF; f.sX = (long long)sX; f.sW = (long long)sW;
+        PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
+        ldl_factor(L->stream, f, L->singular.as<int>());
+        PQ_HIP(hipEventRecord(e1, L->stream));
+        int bad = 0;
+        PQ_HIP(hipMemcpyAsync(&bad, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (bad) {
+            L->batch_omega.clear();
+            throw Error(PQ_EINVAL, "direct solve: a zero or non-finite pivot in a batched L D L^T factorisation");
+        }
+        L->batch_ms = t;
+        if (ms) *ms = t;
+    });
+}
+
 int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_cols, double tol, int max_steps,
                           double *Zp, double *X_host, int64_t *steps, double *residual) {
     return guard([&] {
@@ -2207,7 +2278,12 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
             LdlFactor f{L->Fldl.as<float>(), L->Fldl.as<float>() + (size_t)n * L->ldf, L->ldf, n,
                         L->Xldl.as<float>(), L->Xldl.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,
                         {Wb, Wb + 2 * wsz}, {Wb + wsz, Wb + 3 * wsz}, L->aux, L->ev_ready, L->ev_done};
-            if (L->omega_ldl != omega) {
+            // a factor of this frequency from the last batch, if there is one
+            const auto hit = std::find(L->batch_omega.begin(), L->batch_omega.end(), omega);
+            if (hit != L->batch_omega.end()) {
+                f = batch_view(L, (int)(hit - L->batch_omega.begin()));
+                L->factor_ms = L->batch_ms / (double)L->batch_omega.size();
+            } else if (L->omega_ldl != omega) {
                 L->omega_ldl = -1;
                 hipEvent_t e0, e1;
                 PQ_HIP(hipEventCreate(&e0));

@@ -226,10 +226,15 @@ class PortSolver:
         X = np.zeros((len(f), self.model.n_free, 2 * P)) if currents else None
         info = []
         kind = PRECONDITIONERS[precond] if solver == 'iterative' else None
+        batch = self._direct_batch(len(f)) if solver == 'direct' else 1
         for i, fi in enumerate(f):
             zp = np.zeros((Pd, Pd, 2))
             its, res = ctypes.c_int64(), ctypes.c_double()
             xh = np.zeros((self.model.n_free, 2 * Pd)) if currents else None
+            if solver == 'direct' and batch > 1 and i % batch == 0:
+                # factor the next frequencies together; the solves below refine on these factors
+                om = np.ascontiguousarray(2 * np.pi * f[i:i + batch])
+                call('pq_loops_factor_batch', self._handle, len(om), ptr(om), None)
             if solver == 'direct':
                 call('pq_loops_solve_direct', self._handle, 2 * np.pi * fi, Pd, ptr(cols), DIRECT_TOL, DIRECT_STEPS,
                      ptr(zp), ptr(xh) if currents else None, ctypes.byref(its), ctypes.byref(res))
@@ -276,6 +281,16 @@ class PortSolver:
         chunk, us = ctypes.c_int(), ctypes.c_double()
         call('pq_loops_tuning', self._handle, nr, ctypes.byref(chunk), ctypes.byref(us))
         return chunk.value, us.value
+
+    def _direct_batch(self, n_freq):
+        """Frequencies whose direct factorisations run together: all of them up to 32, within about
+        16 GB of factors (PCBQS_AMD_DIRECT_BATCH=0: one at a time)."""
+        if n_freq < 2 or os.environ.get('PCBQS_AMD_DIRECT_BATCH', '1') == '0':
+            return 1
+        n = self.model.n_free
+        nblk = (n + 63) // 64
+        per = 4 * (2 * n * nblk * 64 + 2 * nblk * 64 * 64 + 8 * n * 64)
+        return int(max(1, min(n_freq, 32, (16 << 30) // max(per, 1))))
 
     def graph_preconditioner(self, frequency):
         """The graph preconditioner (C_f^T D C_f)^-1 at one frequency, D the branches' self impedances

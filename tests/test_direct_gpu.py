@@ -154,3 +154,29 @@ def test_copper_reference_fixture_through_the_direct_solve():
     Z, info = s.ports([0, 1e5, 2e5], solver='direct')
     np.testing.assert_allclose(Z, exp['impedance'], rtol=1e-6)
     s.close()
+
+
+def test_batched_factorisations_give_the_single_frequency_factors(monkeypatch):
+    """The frequencies of one call are factored together (pq_loops_factor_batch: the L D L^T kernels
+    take the frequency as a grid dimension); the port impedances equal those of one factorisation
+    per frequency (PCBQS_AMD_DIRECT_BATCH=0) and match the oracle."""
+    from pcbqs_amd import board as boards
+    from pcbqs_amd.solve import PortSolver
+    from tests.test_solve_gpu import oracle_ports, rel
+    b = boards.board_batch(0)
+    freqs = [0.0, 1e3, 1e6, 3e7, 1e8]
+    out = {}
+    for batch in ('0', '1'):
+        monkeypatch.setenv('PCBQS_AMD_DIRECT_BATCH', batch)
+        s = PortSolver(b, device=0)
+        out[batch] = s.ports(freqs, solver='direct')
+        s.close()
+    (Z0, i0), (Z1, i1) = out['0'], out['1']
+    print(f'batched vs single: max diff {np.abs(Z1 - Z0).max() / np.abs(Z0).max():.2e}, '
+          f'factor ms {[round(r["factor_ms"], 2) for r in i1]} vs {[round(r["factor_ms"], 2) for r in i0]}', flush=True)
+    assert np.abs(Z1 - Z0).max() <= 1e-12 * np.abs(Z0).max()
+    assert all(r['relative_residual'] <= 1e-12 for r in i1)
+    Zo = oracle_ports(b, freqs)
+    w = 2 * np.pi * np.maximum(np.array(freqs), 1e-300)[:, None, None]
+    assert rel(Z1.real, Zo.real) < 1e-6
+    assert rel(Z1[1:].imag / w[1:], Zo[1:].imag / w[1:]) < 1e-6

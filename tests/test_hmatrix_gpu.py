@@ -94,3 +94,19 @@ def test_degenerate_layouts():
     assert (D @ x)[[6, 8]].tolist() == [[0.0] * 3] * 2
     assert (D @ np.zeros((9, 0))).shape == (9, 0)
     D.close()
+
+
+def test_from_a_reference_style_operator():
+    """DeviceHMatrix.from_hmatrix takes any object with the reference HMatrix's `blocks`, `shape`
+    and `dtype` (src/pcbqs/hmatrix.py:59-83), and its products keep the operator's result types."""
+    from types import SimpleNamespace
+    rng, pts, A = H.reference_kernel(n=200, seed=8, complex_=True)
+    blocks = compress(lambda a, b: A[np.ix_(a, b)], pts, pts, tol=1e-8, leaf=16, dtype=complex)
+    ref = SimpleNamespace(blocks=blocks, shape=(200, 200), dtype=np.dtype(complex))
+    D = DeviceHMatrix.from_hmatrix(ref)
+    x = rng.normal(size=200)
+    y = D @ x
+    assert y.dtype == np.complex128 and y.shape == (200,)
+    close(y, H.matmat(blocks, 200, x.astype(complex)))
+    np.testing.assert_allclose(D.matvec(x), y, rtol=1e-15, atol=0)
+    D.close()

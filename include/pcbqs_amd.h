@@ -278,6 +278,13 @@ int pq_loops_factor_time(pq_loops *loops, double *ms);
  * Same factors as one pq_loops_solve_direct per frequency (the reference
  * factors per frequency, MQS._solve_dense, src/pcbqs/mqs.py:743-808). */
 int pq_loops_factor_batch(pq_loops *loops, int n_omega, const double *omegas, double *ms);
+/* pq_loops_solve_direct at n_omega frequencies at once (no currents): the
+ * factorisations batched, the refinement in lockstep with every frequency's
+ * solves in the same launches and each frequency's own stopping rule, so the
+ * results are those of n_omega separate calls. Zp (n_omega, P, P) interleaved
+ * complex, steps and residual (n_omega). */
+int pq_loops_solve_direct_batch(pq_loops *loops, int n_omega, const double *omegas, int P, const int32_t *port_cols,
+                                double tol, int max_steps, double *Zp, int64_t *steps, double *residual);
 
 /* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
  * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the

@@ -86,6 +86,7 @@ _SIGNATURES = {
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
                              ctypes.c_int),
     'pq_loops_factor_batch': ([P, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_loops_solve_direct_batch': ([P, ctypes.c_int, P, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P], ctypes.c_int),
     'pq_loops_solve_direct': ([P, DBL, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P], ctypes.c_int),
     'pq_loops_factor_time': ([P, P], ctypes.c_int),
     'pq_spd_solve': ([ctypes.c_int, ctypes.c_int, P, I64, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P],

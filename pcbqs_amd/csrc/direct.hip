@@ -396,7 +396,10 @@ template <int P>
 __global__ __launch_bounds__(256) void ldl_forward_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
                                                          long long ldf, const float *__restrict__ Xr,
                                                          const float *__restrict__ Xi, int n, int k0, int bk, double *Y,
-                                                         double *Ys) {
+                                                         double *Ys, long long sF, long long sX, long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    Y += blockIdx.z * sV; Ys += blockIdx.z * sV;
     constexpr int NR = 2 * P;
     __shared__ float xs[2][LB][LB + 1];
     __shared__ double bs[LB][NR], ys[LB][NR], part[3][LB][NR];
@@ -479,8 +482,10 @@ __global__ __launch_bounds__(256) void ldl_forward_kernel(const float *__restric
 
 // z = D^-1 y (FP64 complex division by the FP32 pivots).
 __global__ void ldl_scale_kernel(const float *Fr, const float *Fi, long long ldf, int n, int P, const double *Ys,
-                                 double *Z) {
+                                 double *Z, long long sF, long long sV) {
     const int NR = 2 * P;
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;
+    Ys += blockIdx.z * sV; Z += blockIdx.z * sV;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * P) return;
     const int r = (int)(e / P), p = (int)(e % P);
@@ -499,7 +504,10 @@ template <int P>
 __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
                                                           long long ldf, const float *__restrict__ Xr,
                                                           const float *__restrict__ Xi, int k0, int bk, double *Z,
-                                                          double *Xo) {
+                                                          double *Xo, long long sF, long long sX, long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
     constexpr int NR = 2 * P;
     __shared__ float xsm[2][LB][LB + 1];
     __shared__ double zs[LB][NR], xs[LB][NR], part[3][LB][NR];
@@ -784,22 +792,25 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
 void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, double *Y, double *Ys, double *X) {
     const int n = f.n, nb = (n + LB - 1) / LB;
     if (n <= 0) return;
-    const size_t bytes = (size_t)n * 2 * P * 8;
+    const unsigned nz = (unsigned)std::max(1, f.batch);
+    const long long sV = (long long)n * 2 * P;           // the systems' vectors, one after another
+    const size_t bytes = (size_t)n * 2 * P * 8 * nz;
     PQ_HIP(hipMemcpyAsync(Y, B, bytes, hipMemcpyDeviceToDevice, st));
     dispatch_ports(P, [&](auto pc) {
         constexpr int PP = decltype(pc)::value;
         for (int kb = 0; kb < nb; ++kb) {
             const int k0 = kb * LB, bk = std::min(LB, n - k0);
             const int rows = n - k0 - bk, grid = std::max(1, (rows + LB - 1) / LB);
-            hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
-                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys);
+            hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys, f.sF, f.sX,
+                               sV);
         }
-        hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256)), dim3(256), 0, st, f.Fr,
-                           f.Fi, f.ldf, n, PP, Ys, Y);
+        hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256), 1, nz), dim3(256), 0, st,
+                           f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV);
         for (int kb = nb - 1; kb >= 0; --kb) {
             const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + LB - 1) / LB);
-            hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
-                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X);
+            hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X, f.sF, f.sX, sV);
         }
     });
     PQ_HIP(hipGetLastError());

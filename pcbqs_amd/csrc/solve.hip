@@ -2345,6 +2345,115 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
     });
 }
 
+// pq_loops_solve_direct at nw frequencies at once: the factorisations batched
+// (pq_loops_factor_batch), then the refinement in lockstep: every frequency's solves with the
+// factor run in the same launches, the residual products one frequency at a time, and each
+// frequency keeps its own stopping rule (a frequency that has stopped takes no more corrections),
+// so every frequency gets exactly the operations of its own pq_loops_solve_direct.
+int pq_loops_solve_direct_batch(pq_loops *L, int nw, const double *omegas, int P, const int32_t *port_cols,
+                                double tol, int max_steps, double *Zp, int64_t *steps, double *residual) {
+    return guard([&] {
+        PQ_REQUIRE(L && omegas && port_cols && Zp && steps && residual && nw >= 1,
+                   "pq_loops_solve_direct_batch: null argument");
+        PQ_REQUIRE((P >= 1 && P <= 8) || P == 16, "pq_loops_solve_direct_batch: 1 to 8 or 16 ports");
+        PQ_REQUIRE(tol > 0 && tol < 1 && max_steps >= 0, "pq_loops_solve_direct_batch: invalid options");
+        for (int p = 0; p < P; ++p)
+            PQ_REQUIRE(port_cols[p] >= L->nf && port_cols[p] < L->nl,
+                       "pq_loops_solve_direct_batch: port loop out of range");
+        const int n = L->nf, NR = 2 * P;
+        DeviceScope scope(L->device);
+        const auto started = std::chrono::steady_clock::now();
+        L->sc.reserve(P * 4);
+        PQ_HIP(hipMemcpyAsync(L->sc.p, port_cols, P * 4, hipMemcpyHostToDevice, L->stream));
+        if (n <= 0) {
+            for (int b = 0; b < nw; ++b) {
+                port_impedance(L, omegas[b], P, port_cols, nullptr, Zp + (size_t)b * P * P * 2);
+                steps[b] = 0;
+                residual[b] = 0;
+            }
+            return;
+        }
+        const int rc = pq_loops_factor_batch(L, nw, omegas, nullptr);
+        if (rc != PQ_OK) throw Error(rc, pq_last_error());
+        const size_t vsz = (size_t)n * NR;
+        DevBuf Bv, Xv, Wv, Dv, Yv, Ysv;
+        for (DevBuf *d : {&Bv, &Xv, &Wv, &Dv, &Yv, &Ysv}) d->reserve(vsz * nw * 8);
+        L->partial.reserve((size_t)RED_BLOCKS * P * 2 * 8);
+        L->h.reserve((size_t)P * 2 * 8);
+        double *B = Bv.as<double>(), *X = Xv.as<double>(), *W = Wv.as<double>(), *D = Dv.as<double>();
+        tune_gemv(L, NR);
+        LdlFactor f = batch_view(L, 0);
+        const int nblk = (n + LDL_BLOCK - 1) / LDL_BLOCK;
+        f.batch = nw;
+        f.sF = (long long)2 * n * L->ldf;
+        f.sX = (long long)2 * nblk * LDL_BLOCK * LDL_BLOCK;
+        f.sW = (long long)4 * n * 2 * LDL_BLOCK;
+        std::vector<std::vector<double>> bnorm(nw);
+        std::vector<double> rn, worst(nw, 0.0), last(nw, std::numeric_limits<double>::infinity());
+        std::vector<int> corr(nw, 0);
+        std::vector<char> active(nw, 1);
+        for (int b = 0; b < nw; ++b) {
+            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(),
+                               L->ldl, L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P,
+                               omegas[b], B + b * vsz);
+            PQ_HIP(hipGetLastError());
+            norms(L, B + b * vsz, P, bnorm[b]);
+            for (auto &v : bnorm[b]) v = std::max(v, 1e-300);
+        }
+        ldl_solve(L->stream, f, P, B, Yv.as<double>(), Ysv.as<double>(), X);
+        int matvecs = 0;
+        for (int step = 0;; ++step) {
+            bool any = false;
+            for (int b = 0; b < nw; ++b) {
+                if (!active[b]) continue;
+                PQ_HIP(hipStreamSynchronize(L->stream));                 // earlier products have read omega
+                set_omega(L, omegas[b]);
+                apply_operator(L, X + b * vsz, W + b * vsz, P);             // W = B - Z X
+                ++matvecs;
+                hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream,
+                                   B + b * vsz, vsz, W + b * vsz);
+                PQ_HIP(hipGetLastError());
+                norms(L, W + b * vsz, P, rn);
+                double w = 0;
+                for (int p = 0; p < P; ++p) w = std::max(w, rn[p] / bnorm[b][p]);
+                worst[b] = w;
+                if (!(w > tol) || step == max_steps || w > 0.5 * last[b]) {
+                    active[b] = 0;
+                    continue;
+                }
+                last[b] = w;
+                any = true;
+            }
+            if (!any) break;
+            ldl_solve(L->stream, f, P, W, Yv.as<double>(), Ysv.as<double>(), D);   // every system at once
+            for (int b = 0; b < nw; ++b)
+                if (active[b]) {
+                    hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream,
+                                       D + b * vsz, vsz, X + b * vsz);
+                    ++corr[b];
+                }
+            PQ_HIP(hipGetLastError());
+        }
+        double wmax = 0;
+        for (int b = 0; b < nw; ++b) {
+            steps[b] = corr[b];
+            residual[b] = worst[b];
+            wmax = std::max(wmax, worst[b]);
+        }
+        if (wmax > std::max(tol, 1e-10))
+            throw Error(PQ_ECONVERGE, "direct solve: the FP32 factor's refinement stalls at relative residual " +
+                                          std::to_string(wmax));
+        for (int b = 0; b < nw; ++b)
+            port_impedance(L, omegas[b], P, port_cols, X + b * vsz, Zp + (size_t)b * P * P * 2);
+        L->iterations = *std::max_element(corr.begin(), corr.end());
+        L->restarts = 0;
+        L->residual = wmax;
+        L->matvecs = matvecs;
+        L->factor_ms = L->batch_ms / nw;
+        L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    });
+}
+
 // Device time (ms) of the factorisation (forming Z_ff and its L D L^T) that the last direct solve used.
 int pq_loops_factor_time(pq_loops *L, double *ms) {
     return guard([&] {

@@ -11,8 +11,8 @@ compiled static_fill (built into oracle/_ref) on the same mesh, on this host's c
 The GMRES part solves the board's 4 ports at 1 MHz to a relative residual of 1e-8 with plain
 block-Jacobi GMRES and with the learned two-level preconditioner (seconds, iterations, ratio),
 reports the direct solve at the same frequency beside them (FP32 L D L^T on the device refined to
-1e-12: factorisation time and rate, solve time), and times the plain path's operator product on
-the host cores.
+1e-12: factorisation time and rate, solve time) and at eight frequencies in one call (factored
+and refined together), and times the plain path's operator product on the host cores.
 """
 import argparse
 import json
@@ -103,6 +103,16 @@ def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
         factor_tflops=4 * nf ** 3 / 3 / (factor_ms * 1e-3) / 1e12 if factor_ms > 0 else None,
         fp32_mfma_peak_tflops=157.3,
         port_z_rel_diff_vs_plain=float(np.abs(dout[0][0][0] - Zp).max() / np.abs(Zp).max()))
+    # eight frequencies (10 kHz .. 100 MHz) in one call: factored and refined together
+    sweep_f = np.logspace(4, 8, 8)
+    t = time.perf_counter()
+    _, binfo = solver.ports(sweep_f, solver='direct')
+    bsec = time.perf_counter() - t
+    bms = binfo[0]['factor_ms']
+    direct['batched'] = dict(frequencies=len(sweep_f), seconds=bsec, seconds_per_frequency=bsec / len(sweep_f),
+                             factor_ms_per_frequency=bms,
+                             factor_tflops=4 * nf ** 3 / 3 / (bms * 1e-3) / 1e12 if bms > 0 else None,
+                             worst_relative_residual=max(r['relative_residual'] for r in binfo))
     t = time.perf_counter()
     net = MPNN.load()
     net_s = set_learned_coarse(solver, net, grid)

@@ -1,188 +1,60 @@
-"""The fill interface of pcbqs, with `mi355x` as its device backend.
+"""Host entry points around the `mi355x` fill backend.
 
-Mirrors, for users switching from the reference (same names, arguments, results, errors):
-  FillEngine, register_fill_backend, resolve_workers  src/pcbqs/static_backends.py:57-67,234-336
-  static_moments                                      src/pcbqs/static.py:11-95
-  edge_inductance                                     src/pcbqs/static.py:103-128
-  potential                                           src/pcbqs/mqs.py:532-570 (_set_potential
-                                                      on the scalar fill, _ensure_potential)
-The only built-in backend is 'mi355x' ('auto' selects it; PCBQS_FILL_BACKEND is honoured);
-other names come from register_fill_backend or the pcbqs.static_backends entry points, as in
-the reference. There is no CPU fallback: without a device, the backend raises RuntimeError.
+  static_moments    the whole-matrix fill (the role of src/pcbqs/static.py:11-95) through the
+                    plugin object's fill_symmetric: host FP64 planes, device tiling
+  edge_inductance   src/pcbqs/static.py:103-128 (static_combine's formula, on the device)
+  potential         src/pcbqs/mqs.py:532-570 (P on the scalar fill)
+  capacitance_matrix, board_capacitance   src/pcbqs/peec.py:95-123
+
+Backend selection by name (`--fill-backend`, PCBQS_FILL_BACKEND, the pcbqs.static_backends
+entry-point group) is the reference's FillEngine's job; this package only provides the plugin
+(`pcbqs_amd.backend:make_backend`, registered in pyproject.toml). There is no CPU fallback:
+without a device the backend raises RuntimeError.
 """
-from importlib import metadata
-import os
 import time
 
 import numpy as np
 
-from . import _lib
 from ._lib import DeviceArray, call, ptr
 from .backend import Mi355xBackend, combine_device, padded
 from .geometry import EPS0, FillGeometry
 
-_FACTORIES = {'mi355x': Mi355xBackend}
+
+def _check_rule(near, outer_levels):
+    if np.ndim(near) != 0 or not near >= 0:
+        raise ValueError('near must be a nonnegative number or infinity')
+    if isinstance(outer_levels, bool) or not isinstance(outer_levels, (int, np.integer)) or \
+            int(outer_levels) not in (0, 1, 2):
+        raise ValueError('outer_levels must be 0, 1 or 2')
+    return float(near), int(outer_levels)
 
 
-def _positive_integer(value, name):
-    if isinstance(value, (bool, np.bool_)) or not isinstance(value, (int, np.integer)) or value < 1:
-        raise ValueError(f'{name} must be a positive integer')
-    return int(value)
-
-
-def resolve_workers(workers=None):
-    """Explicit count, else PCBQS_FILL_THREADS, PCBQS_OMP_THREADS, OMP_NUM_THREADS, else 1."""
-    if workers is None:
-        key = next((k for k in ('PCBQS_FILL_THREADS', 'PCBQS_OMP_THREADS', 'OMP_NUM_THREADS')
-                    if k in os.environ), None)
-        env = os.environ.get(key) if key else None
-        try:
-            workers = int(env.split(',')[0] if key == 'OMP_NUM_THREADS' else env) if env is not None else 1
-        except ValueError as exc:
-            raise ValueError(f'{key} must be a positive integer') from exc
-    return _positive_integer(workers, 'workers')
-
-
-def register_fill_backend(name, factory):
-    if not isinstance(name, str) or not name or name == 'auto' or name in _FACTORIES:
-        raise ValueError(f'invalid or already registered fill backend {name!r}')
-    if not callable(factory):
-        raise TypeError('fill backend factory must be callable')
-    _FACTORIES[name] = factory
-
-
-class FillEngine:
-    def __init__(self, nodes, tris, z, near=3., outer_levels=1, backend='auto', workers=None,
-                 chunk=16384, device=None):
-        if np.ndim(near) != 0 or near < 0 or np.isnan(near):
-            raise ValueError('near must be nonnegative (or infinity)')
-        if not isinstance(outer_levels, (int, np.integer)) or outer_levels not in (0, 1, 2):
-            raise ValueError('outer_levels must be 0, 1 or 2')
-        requested = backend or os.environ.get('PCBQS_FILL_BACKEND', 'auto')
-        if requested == 'auto':
-            requested = os.environ.get('PCBQS_FILL_BACKEND', 'auto')
-        name = 'mi355x' if requested == 'auto' else requested
-        workers = resolve_workers(workers)
-        self.chunk = _positive_integer(chunk, 'chunk')
-        self.near, self.outer_levels = near, outer_levels
-        if name not in _FACTORIES:
-            providers = [ep for ep in metadata.entry_points(group='pcbqs.static_backends') if ep.name == name]
-            if len(providers) != 1:
-                raise ValueError(f'unknown or ambiguous fill backend {name!r}')
-            register_fill_backend(name, providers[0].load())
-        self.geometry = FillGeometry.prepare(nodes, tris, z, outer_levels)
-        self.backend = _FACTORIES[name](self.geometry, workers, self.chunk, device)
-        self.info = dict(backend=name, workers=workers, chunk=self.chunk,
-                         device=getattr(self.backend, 'device_id', device), precision='float64',
-                         triangles=len(self.geometry.tris), outer_levels=int(outer_levels),
-                         near='all' if np.isinf(near) else float(near))
-        if hasattr(self.backend, 'routing'):
-            self.info['routing'] = self.backend.routing
-
-    @property
-    def pair_batch(self):
-        return max(self.chunk, getattr(self.backend, 'min_batch_pairs', 1))
-
-    def kinds(self, p, q, d):
-        g = self.geometry
-        distance = np.sqrt(((g.cent[p] - g.cent[q]) ** 2).sum(-1) + d * d)
-        size = g.radius[p] + g.radius[q]
-        refine = distance <= size
-        if np.isinf(self.near):
-            return np.where(refine, np.int32(self.outer_levels), np.int32(0)).astype(np.int32)
-        close = distance <= self.near * size
-        kind = np.where(close, np.int32(0), np.int32(-1)).astype(np.int32)
-        kind[close & refine] = self.outer_levels
-        return kind
-
-    def evaluate(self, p, q, d, kind=None, scalar_only=False):
-        p, q = np.asarray(p), np.asarray(q)
-        n = len(self.geometry.tris)
-        if (p.ndim != 1 or q.shape != p.shape or p.dtype.kind not in 'iu' or q.dtype.kind not in 'iu' or
-                (len(p) and (p.min() < 0 or q.min() < 0 or p.max() >= n or q.max() >= n))):
-            raise ValueError('pair indices must be matching integer vectors within the mesh')
-        p, q = np.ascontiguousarray(p, np.int32), np.ascontiguousarray(q, np.int32)
-        d = np.asarray(d, float)
-        if d.ndim == 0:
-            if not np.isfinite(d):
-                raise ValueError('pair offsets must be finite')
-            d = np.full(p.shape, float(d))
-        else:
-            d = np.ascontiguousarray(np.broadcast_to(d, p.shape))
-            if not np.isfinite(d).all():
-                raise ValueError('pair offsets must be finite')
-        if kind is None:
-            kind = self.kinds(p, q, d)
-        else:
-            kind = np.asarray(kind)
-            if kind.shape != p.shape or kind.dtype.kind not in 'iu':
-                raise ValueError('pair kinds must select far (-1) or a prepared analytic level')
-            if len(kind):
-                lo, hi = int(kind.min()), int(kind.max())
-                levels = self.geometry.levels
-                if (lo < -1 or hi > max(levels) or
-                        any(level not in levels and (kind == level).any() for level in range(max(0, lo), hi + 1))):
-                    raise ValueError('pair kinds must select far (-1) or a prepared analytic level')
-        kind = np.ascontiguousarray(kind, np.int32)
-        out = self.backend.evaluate(p, q, d, kind, scalar_only)
-        if not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.shape != (len(p), 1 if scalar_only else 6):
-            raise TypeError('fill backend must return a host float64 array of pair moments')
-        if not np.isfinite(out).all():
-            raise FloatingPointError('non-finite triangle-pair moments from fill backend')
-        return out
-
-
-def static_moments(nodes, tris, z, near=3.0, outer_levels=1, chunk=16384, scalar_only=False,
-                   backend='auto', workers=None, device=None, return_info=False):
-    """(local nodes, M0, Mrx, Mry, Mrr) or (local nodes, M0), SI units, as the reference's."""
-    started = time.perf_counter()
-    engine = FillEngine(nodes, tris, z, near, outer_levels, backend, workers, chunk, device)
-    g = engine.geometry
-    n = len(g.tris)
-    matrices = [np.empty((n, n)) for _ in range(1 if scalar_only else 4)]
-    if hasattr(engine.backend, 'fill_symmetric'):
-        analytic = engine.backend.fill_symmetric(matrices, engine.near, engine.outer_levels, scalar_only)
-        scheduler = 'device'
-        count = n * (n + 1) // 2
-    else:
-        # host tiles through evaluate() for registered pair-only backends (static.py:52-91)
-        side = max(1, int(np.sqrt(engine.chunk)))
-        count = analytic = 0
-        for p0 in range(0, n, side):
-            for q0 in range(p0, n, side):
-                p1, q1 = min(n, p0 + side), min(n, q0 + side)
-                p, q = np.meshgrid(np.arange(p0, p1, dtype=np.int32), np.arange(q0, q1, dtype=np.int32),
-                                   indexing='ij')
-                p, q = p.ravel(), q.ravel()
-                if p0 == q0:
-                    upper = p <= q
-                    p, q = p[upper], q[upper]
-                d = g.z[p] - g.z[q]
-                kind = engine.kinds(p, q, d)
-                m = engine.evaluate(p, q, d, kind, scalar_only)
-                count += len(p)
-                analytic += int(np.count_nonzero(kind >= 0))
-                matrices[0][p, q] = matrices[0][q, p] = m[:, 0]
-                if not scalar_only:
-                    mx, my, rr = matrices[1:]
-                    mx[p, q], my[p, q] = m[:, 1], m[:, 2]
-                    mx[q, p], my[q, p] = m[:, 3], m[:, 4]
-                    rr[p, q] = rr[q, p] = m[:, 5]
-                    diagonal = p == q
-                    mx[p[diagonal], q[diagonal]] = (m[diagonal, 1] + m[diagonal, 3]) / 2
-                    my[p[diagonal], q[diagonal]] = (m[diagonal, 2] + m[diagonal, 4]) / 2
-        scheduler = 'host'
-    info = dict(engine.info, scheduler=scheduler, pairs=int(count), analytic_pairs=int(analytic),
-                scalar_only=bool(scalar_only), seconds=time.perf_counter() - started)
-    result = (g.nodes, *matrices)
-    return (result, info) if return_info else result
+def static_moments(nodes, tris, z, near=3.0, outer_levels=1, scalar_only=False, device=None, return_info=False):
+    """(local nodes, M0, Mrx, Mry, Mrr), or (local nodes, M0) when scalar_only: every unordered
+    triangle pair's moments in the reference's layout (SI units, centred frame), filled on the
+    device by the plugin object's fill_symmetric and copied into host FP64 (n, n) matrices.
+    info: pairs n(n+1)/2, analytic pairs, seconds."""
+    near, outer_levels = _check_rule(near, outer_levels)
+    t0 = time.perf_counter()
+    g = FillGeometry.prepare(nodes, tris, z, outer_levels)
+    backend = Mi355xBackend(g, 1, 16384, device)
+    try:
+        n = len(g.tris)
+        planes = [np.empty((n, n)) for _ in range(1 if scalar_only else 4)]
+        analytic = backend.fill_symmetric(planes, near, outer_levels, bool(scalar_only))
+    finally:
+        backend.close()
+    info = dict(backend='mi355x', device=backend.device_id, precision='float64', triangles=n,
+                outer_levels=outer_levels, near='all' if np.isinf(near) else near, scalar_only=bool(scalar_only),
+                pairs=n * (n + 1) // 2, analytic_pairs=int(analytic), seconds=time.perf_counter() - t0)
+    out = (g.nodes, *planes)
+    return (out, info) if return_info else out
 
 
 def edge_inductance(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None, device=0):
     """factor * diag(scale) combine(rwg, Mrr, Mrx, Mry, M0) diag(scale), symmetric: (L, info).
 
     The host moment matrices are uploaded and combined on the device (static_combine's formula)."""
-    from ._lib import DeviceArray
     started = time.perf_counter()
     nt = M0.shape[0]
     planes = DeviceArray((4, nt, nt), np.float64, device)
@@ -190,7 +62,6 @@ def edge_inductance(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None
         m = np.ascontiguousarray(m, np.float64)
         if m.shape != (nt, nt):
             raise ValueError('static_combine: moments must be C-contiguous float64 n x n')
-        from ._lib import call, ptr
         call('pq_memcpy_h2d', planes.offset(k * nt * nt), ptr(m), m.nbytes)
     cent = rwg.nodes[rwg.tris].mean(1)
     L = combine_device(planes, 64, rwg, cent, factor, scale, 64, device)

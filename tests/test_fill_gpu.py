@@ -40,7 +40,7 @@ def test_fill_symmetric_matches_oracle_on_the_island_mesh(mesh, level, scalar_on
     from pcbqs_amd.static import static_moments
     print(f'fill_symmetric islands level={level} scalar={scalar_only} near={near}', flush=True)
     (local, *gpu), info = static_moments(*mesh, near=near, outer_levels=level, scalar_only=scalar_only,
-                                         backend='mi355x', device=0, return_info=True)
+                                         device=0, return_info=True)
     g = F.prepare(*mesh, level)
     ref, pairs, analytic = F.cr_fill(g, near, level, scalar_only)
     assert (info['pairs'], info['analytic_pairs']) == (pairs, analytic)
@@ -56,14 +56,14 @@ def test_fill_symmetric_matches_oracle_on_two_layer_meshes(meshes, name):
     from pcbqs_amd.static import static_moments
     nodes, tris, z = meshes[name]
     print(f'fill_symmetric {name} nt={len(tris)}', flush=True)
-    (local, *gpu), info = static_moments(nodes, tris, z, backend='mi355x', device=0, return_info=True)
+    (local, *gpu), info = static_moments(nodes, tris, z, device=0, return_info=True)
     g = F.prepare(nodes, tris, z, 1)
     ref, pairs, analytic = F.cr_fill(g, 3.0, 1, False)
     assert (info['pairs'], info['analytic_pairs']) == (pairs, analytic)
     for a, b in zip(gpu, ref):
         assert rel(a, b) < 1e-10
     assert np.array_equal(gpu[0], gpu[0].T) and np.array_equal(gpu[3], gpu[3].T)
-    again = static_moments(nodes, tris, z, backend='mi355x', device=0)
+    again = static_moments(nodes, tris, z, device=0)
     assert all(np.array_equal(a, b) for a, b in zip(gpu, again[1:]))
 
 
@@ -95,26 +95,6 @@ def test_device_fill_matches_oracle(meshes, precision):
     backend.close()
 
 
-def test_explicit_pairs_match_oracle(meshes):
-    from pcbqs_amd.static import FillEngine
-    nodes, tris, z = meshes['two_layer']
-    print('pairs two_layer', flush=True)
-    for level in (0, 1, 2):
-        engine = FillEngine(nodes, tris, z, outer_levels=level, backend='mi355x', device=0)
-        g = F.prepare(nodes, tris, z, level)
-        n = len(tris)
-        rng = np.random.default_rng(level)
-        p, q = rng.integers(0, n, 3000).astype(np.int32), rng.integers(0, n, 3000).astype(np.int32)
-        p[:50] = q[:50]
-        d = g.z[p] - g.z[q]
-        for scalar in (False, True):
-            out = engine.evaluate(p, q, d, scalar_only=scalar)
-            ref = F.cr_pairs(g, p, q, d, F.kinds(g, p, q, d, 3.0, level), scalar)
-            assert out == pytest.approx(ref, rel=1e-11, abs=1e-26)
-        with pytest.raises(ValueError, match='prepared analytic level'):
-            engine.evaluate(p, q, d, kind=np.full(len(p), {0: 1, 1: 2, 2: 1}[level], np.int32))
-
-
 @pytest.mark.parametrize('precision', [64, 32])
 def test_device_combine_matches_oracle(meshes, precision):
     from pcbqs_amd.backend import Mi355xBackend, combine_device
@@ -138,7 +118,7 @@ def test_device_combine_matches_oracle(meshes, precision):
 def test_edge_inductance_from_host_moments(mesh):
     from pcbqs_amd.static import edge_inductance, static_moments
     from pcbqs_amd.geometry import RWG
-    nodes, m0, mx, my, rr = static_moments(*mesh, backend='mi355x', device=0)
+    nodes, m0, mx, my, rr = static_moments(*mesh, device=0)
     r = RWG(nodes, mesh[1])
     L, info = edge_inductance(r, rr, mx, my, m0, 1e-7, 1 / r.l)
     ref = F.cr_combine(M.RWG(nodes, mesh[1]), rr, mx, my, m0, 1e-7, 1 / r.l)

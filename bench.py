@@ -161,16 +161,9 @@ def sweep_section(n_boards, n_freqs=20, concurrency=1):
 
 
 def host_loop_matrices(solver):
-    """The free-loop block of Ll (FP64 copy of the device's FP32 matrix) and Rl (CSR) on the host."""
-    import ctypes
-    from pcbqs_amd._lib import call
+    """The free-loop block of the device's FP64 Ll and Rl (CSR) on the host."""
     n = solver.model.n_free
-    dev, ld = ctypes.c_void_p(), ctypes.c_int64()
-    call('pq_loops_matrix', solver._handle, ctypes.byref(dev), ctypes.byref(ld))
-    L32 = np.empty((n, ld.value), np.float32)
-    call('pq_memcpy_d2h', L32.ctypes.data_as(ctypes.c_void_p), dev, L32.nbytes)
-    L = np.ascontiguousarray(L32[:, :n], np.float64)
-    del L32
+    L = np.ascontiguousarray(solver.loop_matrix(n, precision=64)[:, :n])
     return L, solver.model.Rl.tocsr()[:n, :n]
 
 

@@ -192,16 +192,19 @@ int pq_csym_destroy(pq_csym *h);
 typedef struct pq_loops pq_loops;   /* a board's loop operator on one device      */
 
 /* Project the branch inductance onto the loop basis on the device and keep the
- * loop operator: Ll = C^T Lb C (FP32 storage, FP64 accumulation) and Rl (CSR,
- * FP64). C is given in CSC form over loops (c_colptr (n_loops+1), c_rows,
- * c_sign, c_nnz entries; branch rows >= n_branch_rows carry no inductance);
- * Lb_dev is the (n_branch_rows, ldb) FP32 branch inductance (RWG edges then via
- * segments). Loops are ordered free loops first (n_free), then the port loops.
- * Replaces the dense loop matrices of MQS._assemble / _project_inductance
- * (src/pcbqs/mqs.py:639-665). */
+ * loop operator: Ll = C^T Lb C (FP64 accumulation and storage, symmetrised,
+ * plus a copy rounded to FP32 for the Krylov products, the preconditioners
+ * and the FP32 factor) and Rl (CSR, FP64). C is given in CSC form over loops
+ * (c_colptr (n_loops+1), c_rows, c_sign, c_nnz entries; branch rows >=
+ * n_branch_rows carry no inductance); Lb_dev is the (n_branch_rows, ldb)
+ * branch inductance (RWG edges then via segments), FP32 or FP64 as
+ * lb_precision (32 or 64) says. Loops are ordered free loops first (n_free),
+ * then the port loops. Replaces the dense loop matrices of MQS._assemble /
+ * _project_inductance (src/pcbqs/mqs.py:639-665). */
 int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr, const int32_t *c_rows,
-                    const float *c_sign, int64_t c_nnz, const float *Lb_dev, int64_t ldb, int n_branch_rows,
-                    const int32_t *r_ptr, const int32_t *r_col, const double *r_val, pq_loops **out);
+                    const float *c_sign, int64_t c_nnz, const void *Lb_dev, int lb_precision, int64_t ldb,
+                    int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col, const double *r_val,
+                    pq_loops **out);
 
 /* The spanning tree the fundamental loops are built on. Loop j is its closing
  * branch (coefficient +1, running end_a -> end_b) plus the tree path back from
@@ -225,15 +228,16 @@ typedef struct pq_loop_tree {
 
 /* pq_loops_create with C given by its tree: U = Lb C from prefix sums of Lb's
  * rows down the tree (U[k][j] = Lb[k][closing j] + S_k(end_a j) - S_k(end_b j)),
- * then Ll = C^T U the same way on U's rows; FP64 sums, FP32 storage, the same
- * Ll as pq_loops_create. Lb must be symmetric (both triangles stored). */
-int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tree, const float *Lb_dev,
-                         int64_t ldb, int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col,
-                         const double *r_val, pq_loops **out);
+ * then Ll = C^T U the same way on U's rows; FP64 sums and storage (U and Ll),
+ * the same Ll as pq_loops_create. Lb must be symmetric (both triangles stored). */
+int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tree, const void *Lb_dev,
+                         int lb_precision, int64_t ldb, int n_branch_rows, const int32_t *r_ptr,
+                         const int32_t *r_col, const double *r_val, pq_loops **out);
 int pq_loops_destroy(pq_loops *loops);
 
-/* The device loop inductance matrix (FP32, n_loops rows, leading dimension ldl). */
-int pq_loops_matrix(pq_loops *loops, void **Ll_dev, int64_t *ldl);
+/* The device loop inductance matrix (n_loops rows, leading dimension ldl): the
+ * FP64 matrix (precision 64) or its FP32 copy (precision 32). */
+int pq_loops_matrix(pq_loops *loops, int precision, void **Ll_dev, int64_t *ldl);
 
 /* P in 1..8 or 16 (pad other counts by repeating a port column).
  * Port impedances Zp (P,P complex, interleaved re/im, row-major) at angular
@@ -256,14 +260,18 @@ int pq_loops_solve_ports(pq_loops *loops, double omega, int P, const int32_t *po
  * (Zp, optional X_host, the correction count in *steps and the final largest
  * relative residual in *residual), with Z_ff = Rl + j omega Ll formed in FP32
  * on the device and factored as L D L^T without pivoting (both parts are
- * symmetric positive definite), then refined in FP64 against the operator
- * product until the relative residual is at most tol (the reference uses
- * 1e-12), for at most max_steps corrections, stopping once a correction no
- * longer halves it; a residual then above 1e-10 returns PQ_ECONVERGE. The
- * factor is kept for the next call at the same omega. P 1..8 or 16. Replaces
- * the dense magnetic solve MQS._solve_dense (src/pcbqs/mqs.py:743-808) with
- * lu_factor_rows / lu_solve_rows and MixedLU (src/pcbqs/kernels.py:125-181,
- * 266-292: FP32 LAPACK LU refined to 1e-12 in at most 12 steps). */
+ * symmetric positive definite), then refined against the FP64 system (its
+ * residual taken with the FP64 Ll) until the relative residual is at most tol
+ * (the reference uses 1e-12), for at most max_steps corrections, stopping once
+ * a correction no longer halves it. A zero or non-finite FP32 pivot, or a
+ * residual then above max(tol, 1e-10), sends the frequency to an FP64 LU with
+ * partial pivoting of Z_ff (rocSOLVER zgetrf/zgetrs, loaded on first use), as
+ * MixedLU refactors in double; PQ_ECONVERGE only if that too leaves a residual
+ * above max(tol, 1e-10). The factor is kept for the next call at the same
+ * omega. P 1..8 or 16. Replaces the dense magnetic solve MQS._solve_dense
+ * (src/pcbqs/mqs.py:743-808) with lu_factor_rows / lu_solve_rows and MixedLU
+ * (src/pcbqs/kernels.py:125-181, 266-292: FP32 LAPACK LU refined to 1e-12 in at
+ * most 12 steps, an FP64 LU where that stalls). */
 int pq_loops_solve_direct(pq_loops *loops, double omega, int P, const int32_t *port_cols, double tol,
                           int max_steps, double *Zp, double *X_host, int64_t *steps, double *residual);
 
@@ -293,10 +301,11 @@ int pq_loops_rhs(pq_loops *loops, double omega, int P, const int32_t *port_cols,
 
 /* W = Z_ff X = (Rl + j omega Ll) X on the free loops, X and W (n_free, 2P) FP64
  * host in the layout of pq_loops_rhs (real parts in columns 0..P-1, imaginary
- * parts in P..2P-1), P 1..8 or 16: the GMRES matvec, with the kernel the
- * autotuner picked for 2P columns. Replaces the product with A in
+ * parts in P..2P-1), P 1..8 or 16. precision 32: the GMRES matvec with the
+ * FP32 Ll and the kernel the autotuner picked for 2P columns; 64: the FP64
+ * system's product the residuals use. Replaces the product with A in
  * _lockstep_gmres (src/pcbqs/mqs_iterative.py:109-170). */
-int pq_loops_operator(pq_loops *loops, double omega, int P, const double *X_host, double *W_host);
+int pq_loops_operator(pq_loops *loops, double omega, int P, int precision, const double *X_host, double *W_host);
 
 /* The coarse-space solution Phi (Phi^T Z_ff Phi)^-1 Phi^T B (n_free, 2P) FP64
  * host for the port right-hand sides at omega: the input of the learned
@@ -340,6 +349,10 @@ int pq_loops_tuning(pq_loops *loops, int NR, int *chunk, double *microseconds);
  * residual, wall seconds. Not in the reference (solve_info's role,
  * src/pcbqs/mqs_iterative.py:229-230). */
 int pq_loops_last_solve(pq_loops *loops, int64_t *iterations, int64_t *matvecs, double *residual, double *seconds);
+/* The number of frequencies the last direct solve (pq_loops_solve_direct or
+ * pq_loops_solve_direct_batch) solved with the FP64 fallback factor (the role of
+ * MixedLU.double, src/pcbqs/kernels.py:147-154, 179-181). */
+int pq_loops_last_fallbacks(pq_loops *loops, int64_t *count);
 
 /* ---- compressed (H-matrix) operator product ------------------------------ */
 

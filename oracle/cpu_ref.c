@@ -9,6 +9,8 @@
  *   - cr_fill          <- src/pcbqs/core/static_kernels.hpp:184-252 (static_fill)
  *   - cr_pairs         <- src/pcbqs/core/static_kernels.hpp:147-172 (static_pairs)
  *   - cr_combine       <- src/pcbqs/core/static_kernels.hpp:257-310 (static_combine)
+ *   - cr_project       <- src/pcbqs/mqs.py:665-667 (_project_inductance: qform(C, L, C),
+ *                         C^T L C of the sparse loop basis and the dense branch matrix)
  * The arithmetic is evaluated in the same order as the reference so that, built
  * with -ffp-contract=off, results agree with the reference core bit for bit.
  */
@@ -219,6 +221,37 @@ int cr_combine(const double *Mrr, const double *Mrx, const double *Mry, const do
                 out[m * ne + n] = v;
                 out[n * ne + m] = v;
             }
+    }
+    return 0;
+}
+
+/* Ll (nl, nl) = C^T L C for the sparse loop basis C (n rows; CSC: colptr (nl + 1), rows, sign)
+ * and the symmetric dense L (n, n, row stride ld). First Ut = (L C)^T, row j the signed sum of
+ * the rows of L on loop j's branches (L's rows stand for its columns), then Ll[j][i] = the signed
+ * sum of Ut[j] over loop i's branches. Ut (nl, n) is the caller's scratch. */
+int cr_project(const double *L, int64_t ld, int64_t n, const int32_t *colptr, const int32_t *rows,
+               const double *sign, int64_t nl, int workers, double *Ut, double *Ll)
+{
+    for (int64_t j = 0; j < nl; ++j)
+        for (int32_t t = colptr[j]; t < colptr[j + 1]; ++t)
+            if (rows[t] < 0 || rows[t] >= n) return -1;
+#pragma omp parallel for num_threads(workers) schedule(dynamic, 4) if (workers > 1)
+    for (int64_t j = 0; j < nl; ++j) {
+        double *u = Ut + (size_t)j * n;
+        for (int64_t k = 0; k < n; ++k) u[k] = 0.0;
+        for (int32_t t = colptr[j]; t < colptr[j + 1]; ++t) {
+            const double s = sign[t], *row = L + (size_t)rows[t] * ld;
+            for (int64_t k = 0; k < n; ++k) u[k] += s * row[k];
+        }
+    }
+#pragma omp parallel for num_threads(workers) schedule(dynamic, 4) if (workers > 1)
+    for (int64_t j = 0; j < nl; ++j) {
+        const double *u = Ut + (size_t)j * n;
+        for (int64_t i = 0; i < nl; ++i) {
+            double acc = 0.0;
+            for (int32_t t = colptr[i]; t < colptr[i + 1]; ++t) acc += sign[t] * u[rows[t]];
+            Ll[(size_t)j * nl + i] = acc;
+        }
     }
     return 0;
 }

@@ -179,6 +179,7 @@ def cpu_ref():
         lib.cr_pairs.argtypes = [P, P, P, P, P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P]
         lib.cr_combine.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P, P, ctypes.c_int64, ctypes.c_double,
                                    ctypes.c_int, P]
+        lib.cr_project.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, P, P, ctypes.c_int64, ctypes.c_int, P, P]
         lib.cr_max_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -257,6 +258,23 @@ def cr_combine(rwg, Mrr, Mrx, Mry, M0, factor=1.0, scale=None, workers=None):
     if lib.cr_combine(*[_ptr(m) for m in mats], M0.shape[0], _ptr(tri), _ptr(alpha), _ptr(vertex), _ptr(scale),
                       ne, float(factor), int(workers), _ptr(out)):
         raise ValueError('cr_combine: invalid triangle index')
+    return out
+
+
+def cr_project(C, L, workers=None):
+    """C^T L C (dense, nl x nl) for a sparse C (n x nl) and a symmetric dense L (n x n)."""
+    lib = cpu_ref()
+    workers = workers or lib.cr_max_threads()
+    C = C.tocsc()
+    n, nl = C.shape
+    L = np.ascontiguousarray(L, np.float64)
+    if L.shape != (n, n):
+        raise ValueError('cr_project: L must be n x n for a basis of n rows')
+    colptr, rows = np.ascontiguousarray(C.indptr, np.int32), np.ascontiguousarray(C.indices, np.int32)
+    sign = np.ascontiguousarray(C.data, np.float64)
+    Ut, out = np.empty((nl, n)), np.empty((nl, nl))
+    if lib.cr_project(_ptr(L), n, n, _ptr(colptr), _ptr(rows), _ptr(sign), nl, int(workers), _ptr(Ut), _ptr(out)):
+        raise ValueError('cr_project: branch index out of range')
     return out
 
 

@@ -270,7 +270,9 @@ class CopperModel:
         (m0, mx, my, rr), pairs, analytic = F.cr_fill(g, self.near, self.outer_levels, False, workers)
         return g, m0, mx, my, rr
 
-    def assemble(self, moments=None):
+    def assemble(self, moments=None, projection='c'):
+        """The loop matrices; projection 'qform' is the reference's sparse products (mqs.py:665-667),
+        'c' the same C^T Le C by the OpenMP restatement cr_project (what large boards need)."""
         g, m0, mx, my, rr = moments if moments is not None else self.fill()
         r = RWG(g.nodes, self.tris)
         Le = F.cr_combine(r, rr, mx, my, m0, MU0 / (4 * np.pi), 1.0 / r.l)
@@ -285,7 +287,8 @@ class CopperModel:
         C, src, tree = loops(self.n_nodes, ends, is_src, len(self.br), self.ne, self.ideal_terminals)
         Ce, Cv = C[:self.ne], C[self.ne:self.ne + self.nv]
         Rl = np.ascontiguousarray((C.T @ R @ C).toarray())
-        Ll = qform(Ce, Le, Ce) + (qform(Cv, Lv, Cv) if self.nv else 0)
+        Ll = (qform(Ce, Le, Ce) if projection == 'qform' else F.cr_project(Ce, Le)) + \
+            (qform(Cv, Lv, Cv) if self.nv else 0)
         self.mats = dict(Le=Le, Lv=Lv, R=R, C=C, Rl=Rl, Ll=np.ascontiguousarray(Ll), src=src, nb=nb,
                          tree=tree, ends=ends, is_src=is_src)
         return self.mats

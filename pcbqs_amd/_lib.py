@@ -68,9 +68,10 @@ _SIGNATURES = {
     'pq_fill_last_timing': ([P, P, P, P], ctypes.c_int),
     'pq_combine_device': ([ctypes.c_int, P, I64, I64, P, P, P, P, P, I64, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
     'pq_potential_device': ([ctypes.c_int, P, I64, I64, I64, P, P, P, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),
-    'pq_loops_create': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, I64, P, I64, ctypes.c_int, P, P, P, P],
-                        ctypes.c_int),
-    'pq_loops_create_tree': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, I64, ctypes.c_int, P, P, P, P],
+    'pq_loops_create': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, I64, P, ctypes.c_int, I64, ctypes.c_int,
+                         P, P, P, P], ctypes.c_int),
+    'pq_loops_create_tree': ([ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int, I64, ctypes.c_int, P, P,
+                              P, P],
                              ctypes.c_int),
     'pq_loops_destroy': ([P], ctypes.c_int),
     'pq_csym_create': ([ctypes.c_int, ctypes.c_int, I64, P, P, P, P, P, P], ctypes.c_int),
@@ -82,7 +83,8 @@ _SIGNATURES = {
     'pq_hmatrix_apply': ([P, I64, P, P], ctypes.c_int),
     'pq_hmatrix_apply_device': ([P, I64, P, P, P], ctypes.c_int),
     'pq_hmatrix_time': ([P, ctypes.c_int, P], ctypes.c_int),
-    'pq_loops_matrix': ([P, P, P], ctypes.c_int),
+    'pq_loops_matrix': ([P, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_loops_last_fallbacks': ([P, P], ctypes.c_int),
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
                              ctypes.c_int),
     'pq_loops_factor_batch': ([P, ctypes.c_int, P, P], ctypes.c_int),
@@ -103,7 +105,7 @@ _SIGNATURES = {
     'pq_mpnn_aggregate': ([ctypes.c_int, P, P, P, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int, P], ctypes.c_int),
     'pq_mpnn_relu_mask': ([SIZE, P, P, P, P], ctypes.c_int),
     'pq_loops_rhs': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
-    'pq_loops_operator': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
+    'pq_loops_operator': ([P, DBL, ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
     'pq_loops_tuning': ([P, ctypes.c_int, P, P], ctypes.c_int),

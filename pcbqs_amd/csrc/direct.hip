@@ -70,7 +70,7 @@ __device__ inline void cinv(float dr, float di, float &qr, float &qi) {
 //   (b) below it, W = A_21 X_S^T and L_21 = W D_S^-1;
 //   (c) the lower triangle below and right of it, A_22 -= L_21 W^T;
 // then X's block rows below the first, X_I = -X_S,I (sum_{M < I} L_IM X_M), one block row at a
-// time. A zero or non-finite pivot sets *bad.
+// time. A zero or non-finite pivot sets bad[matrix] (one flag per matrix of a batch).
 __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, long long ldf, int k, int b, float *Lir,
                                                       float *Lii, int *bad, long long sF, long long sX) {
     __shared__ float Ar[LB][LB + 1], Ai[LB][LB + 1], Xr[LB][LB + 1], Xi[LB][LB + 1];
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void ldl_diag_kernel(float *Fr, float *Fi, lon
                 for (int q = 1; q < 4; ++q)
                     if (jq == q) { vr = fr[q]; vi = fi[q]; }
                 const float dr = __shfl(vr, 4 * j + jl), di = __shfl(vi, 4 * j + jl);
-                if (t == 0 && !(isfinite(dr) && isfinite(di) && (dr != 0.f || di != 0.f))) *bad = 1;
+                if (t == 0 && !(isfinite(dr) && isfinite(di) && (dr != 0.f || di != 0.f))) bad[blockIdx.z] = 1;
                 const float cr = __shfl(vr, 4 * r + jl), ci = __shfl(vi, 4 * r + jl);   // A[r][j]
                 float ur[4], ui[4], wr[4], wi[4];
 #pragma unroll

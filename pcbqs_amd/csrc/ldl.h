@@ -39,8 +39,8 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
               const double *rval, double omega);
 
 // Factor in place on stream st (the work is complete in st's order when it returns; f.aux has no
-// pending work left behind); *bad (device int, cleared by the caller) is set when a pivot is zero or
-// not finite in any of the f.batch matrices.
+// pending work left behind); bad[z] (f.batch device ints, cleared by the caller) is set when a pivot
+// of matrix z is zero or not finite.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
 
 // X = Z^-1 B for P complex columns (n, 2P) FP64 (real parts first), FP32 factor, FP64 arithmetic.

@@ -9,7 +9,9 @@
 // of src/pcbqs/peec_iterative.py:100-112.
 //
 // Layout: a block vector of P complex columns is an (n, 2P) row-major FP64 array, real parts in
-// columns 0..P-1 and imaginary parts in P..2P-1. Ll is FP32 (n, ldl) row-major; every product
+// columns 0..P-1 and imaginary parts in P..2P-1. Ll is kept twice, (n, ldl) row-major: in FP64
+// (the system: right-hand sides, residuals, port reduction, the FP64 fallback factor) and rounded
+// to FP32 (the Krylov products, the preconditioners and the FP32 factor); every product
 // accumulates in FP64. Rl is CSR (FP64 values). Reductions are two-stage with a fixed order, so
 // results do not depend on scheduling.
 #include "pq_common.h"
@@ -20,8 +22,13 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <limits>
 #include <map>
+#include <mutex>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <type_traits>
 #include <vector>
 
 namespace pq {
@@ -35,11 +42,12 @@ using cplx = std::complex<double>;
 // read once per row, the kernel's dominant traffic), and the rows' gathers of a branch are
 // independent loads in flight together.
 constexpr int PROJ_ROWS = 8;
-__global__ __launch_bounds__(256) void proj_left_kernel(const float *Lb, long long ldb, int nbr,
+template <class TB>
+__global__ __launch_bounds__(256) void proj_left_kernel(const TB *Lb, long long ldb, int nbr,
                                                        const int *colptr, const int *rows, const float *sign,
-                                                       int nl, float *U, long long ldu) {
+                                                       int nl, double *U, long long ldu) {
     const int k0 = blockIdx.x * PROJ_ROWS;
-    const float *Lr[PROJ_ROWS];
+    const TB *Lr[PROJ_ROWS];
 #pragma unroll
     for (int i = 0; i < PROJ_ROWS; ++i) Lr[i] = Lb + (size_t)min(k0 + i, nbr - 1) * ldb;
     for (int j = threadIdx.x; j < nl; j += blockDim.x) {
@@ -55,12 +63,12 @@ __global__ __launch_bounds__(256) void proj_left_kernel(const float *Lb, long lo
         }
 #pragma unroll
         for (int i = 0; i < PROJ_ROWS; ++i)
-            if (k0 + i < nbr) U[(size_t)(k0 + i) * ldu + j] = (float)acc[i];
+            if (k0 + i < nbr) U[(size_t)(k0 + i) * ldu + j] = acc[i];
     }
 }
 
-__global__ __launch_bounds__(256) void proj_right_kernel(const float *U, long long ldu, int nbr, const int *colptr,
-                                                        const int *rows, const float *sign, int nl, float *Ll,
+__global__ __launch_bounds__(256) void proj_right_kernel(const double *U, long long ldu, int nbr, const int *colptr,
+                                                        const int *rows, const float *sign, int nl, double *Ll,
                                                         long long ldl) {
     const int i = blockIdx.x;
     const int t0 = colptr[i], t1 = colptr[i + 1];
@@ -68,9 +76,9 @@ __global__ __launch_bounds__(256) void proj_right_kernel(const float *U, long lo
         double acc = 0;
         for (int t = t0; t < t1; ++t) {
             const int r = rows[t];
-            if (r < nbr) acc += (double)sign[t] * (double)U[(size_t)r * ldu + j];
+            if (r < nbr) acc += (double)sign[t] * U[(size_t)r * ldu + j];
         }
-        Ll[(size_t)i * ldl + j] = (float)acc;
+        Ll[(size_t)i * ldl + j] = acc;
     }
 }
 
@@ -107,20 +115,19 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int *__restrict__ ptr, 
     }
 }
 
-// Ll := (Ll + Ll^T) / 2 on the upper triangle and its mirror (the loop matrices' symmetric part,
-// as the reference takes it: src/pcbqs/mqs_iterative.py:244-257). Entry (i, j) and (j, i) then
-// agree bitwise, which lets the product read columns of Ll as rows.
 // ---- the projection through the spanning tree ----
 // A loop is its closing branch plus the tree path between the branch's ends: C e_j = e_cot(j) +
 // p(a_j) - p(b_j), p(x) the signed branch set of the path from node x to its root (the part
 // above the two paths' meeting node cancels). So with S[x] = sum over that path of s_y Lb[upb y]
 // (prefix sums down the tree, one level at a time), (Lb C)[k][j] = Lb[k][cot j] + S[a_j][k] -
 // S[b_j][k]: two row reads per loop instead of one gather per branch of the loop. The same again
-// on the rows of U = Lb C gives Ll = C^T U. All sums in FP64.
+// on the rows of U = Lb C gives Ll = C^T U. All sums in FP64; U and Ll are stored in FP64 (Lb in
+// FP32 or FP64).
 
 // S[x][k] = S[parent x][k] + s_x Row[upb x][k] for the nodes x of one level (rows past nvalid
 // carry nothing); k along the lanes, Row read along its rows (Lb is symmetric).
-__global__ __launch_bounds__(256) void tree_level_kernel(const float *__restrict__ Row, long long ldr, int nvalid,
+template <class TR>
+__global__ __launch_bounds__(256) void tree_level_kernel(const TR *__restrict__ Row, long long ldr, int nvalid,
                                                         int width, const int *__restrict__ nodes, int count,
                                                         const int *__restrict__ parent, const int *__restrict__ upb,
                                                         const float *__restrict__ ups, double *__restrict__ S,
@@ -137,22 +144,22 @@ __global__ __launch_bounds__(256) void tree_level_kernel(const float *__restrict
 
 // U[k][j] = Lb[cot j][k] + S[a_j][k] - S[b_j][k] by 64 x 64 tiles through LDS: rows of S and Lb
 // read along k, rows of U written along j.
-__global__ __launch_bounds__(256) void tree_u_kernel(const float *__restrict__ Lb, long long ldb, int nbr,
+template <class TB>
+__global__ __launch_bounds__(256) void tree_u_kernel(const TB *__restrict__ Lb, long long ldb, int nbr,
                                                     const double *__restrict__ S, long long lds,
                                                     const int *__restrict__ cot, const int *__restrict__ ea,
-                                                    const int *__restrict__ eb, int nl, float *__restrict__ U,
+                                                    const int *__restrict__ eb, int nl, double *__restrict__ U,
                                                     long long ldu) {
-    __shared__ float t[64][65];
+    __shared__ double t[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int k0 = blockIdx.x * 64, j0 = blockIdx.y * 64, k = k0 + lane;
     for (int q = 0; q < 16; ++q) {
         const int jl = 16 * w + q, j = j0 + jl;
-        float v = 0.f;
+        double v = 0.0;
         if (j < nl && k < nbr) {
             const int c = cot[j];
-            double d = S[(size_t)ea[j] * lds + k] - S[(size_t)eb[j] * lds + k];
-            if (c >= 0 && c < nbr) d += (double)Lb[(size_t)c * ldb + k];
-            v = (float)d;
+            v = S[(size_t)ea[j] * lds + k] - S[(size_t)eb[j] * lds + k];
+            if (c >= 0 && c < nbr) v += (double)Lb[(size_t)c * ldb + k];
         }
         t[jl][lane] = v;
     }
@@ -164,28 +171,34 @@ __global__ __launch_bounds__(256) void tree_u_kernel(const float *__restrict__ L
 }
 
 // Ll[i][j] = U[cot i][j] + S2[a_i][j] - S2[b_i][j] (rows i, lanes along j).
-__global__ __launch_bounds__(256) void tree_ll_kernel(const float *__restrict__ U, long long ldu, int nbr,
+__global__ __launch_bounds__(256) void tree_ll_kernel(const double *__restrict__ U, long long ldu, int nbr,
                                                      const double *__restrict__ S2, long long lds,
                                                      const int *__restrict__ cot, const int *__restrict__ ea,
-                                                     const int *__restrict__ eb, int nl, float *__restrict__ Ll,
+                                                     const int *__restrict__ eb, int nl, double *__restrict__ Ll,
                                                      long long ldl) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= nl) return;
     for (int i = blockIdx.y; i < nl; i += gridDim.y) {
         const int c = cot[i];
         double d = S2[(size_t)ea[i] * lds + j] - S2[(size_t)eb[i] * lds + j];
-        if (c >= 0 && c < nbr) d += (double)U[(size_t)c * ldu + j];
-        Ll[(size_t)i * ldl + j] = (float)d;
+        if (c >= 0 && c < nbr) d += U[(size_t)c * ldu + j];
+        Ll[(size_t)i * ldl + j] = d;
     }
 }
 
-__global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
+// Ll := (Ll + Ll^T) / 2 in FP64 (the loop matrices' symmetric part, as the reference takes it:
+// src/pcbqs/mqs_iterative.py:244-257), and its FP32 copy L32 rounded from it. Entries (i, j) and
+// (j, i) then agree bitwise in both, which lets the products read columns of Ll as rows. One
+// thread per unordered pair (i <= j), so no two threads touch one entry.
+__global__ void symmetrize_kernel(double *Ll, float *L32, long long ldl, int n) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long i = e / n, j = e % n;
-    if (i >= n || j <= i) return;
-    const float v = 0.5f * (Ll[i * ldl + j] + Ll[j * ldl + i]);
+    if (i >= n || j < i) return;
+    const double v = 0.5 * (Ll[i * ldl + j] + Ll[j * ldl + i]);
     Ll[i * ldl + j] = v;
     Ll[j * ldl + i] = v;
+    L32[i * ldl + j] = (float)v;
+    L32[j * ldl + i] = (float)v;
 }
 
 // Panel products: part[i][s][r] = sum over the rows c of chunk s of M[c][i] X[c][r], M (ncols, ld)
@@ -196,8 +209,8 @@ __global__ void symmetrize_kernel(float *Ll, long long ldl, int n) {
 // per product, so its loads are non-temporal: they do not displace the vectors and bases that the
 // rest of the iteration re-reads.
 constexpr int GEMV_SPLIT = 32, GEMV_ROWS = 256, GEMV_CHUNK = 640;
-template <int NR>
-__global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M, long long ld, int nrows, int ncols,
+template <int NR, class T = float>
+__global__ __launch_bounds__(256) void panel_kernel(const T *__restrict__ M, long long ld, int nrows, int ncols,
                                                    const double *__restrict__ X, int chunk,
                                                    double *__restrict__ part) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,10 +218,10 @@ __global__ __launch_bounds__(256) void panel_kernel(const float *__restrict__ M,
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
-    const float *col = M + min(row, nrows - 1);
+    const T *col = M + min(row, nrows - 1);
     int c = c0;
     for (; c + 16 <= c1; c += 16) {
-        float m[16];
+        T m[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) m[u] = __builtin_nontemporal_load(col + (size_t)(c + u) * ld);
         const double *x = X + (size_t)c * NR;
@@ -712,8 +725,8 @@ __global__ void vec_sub_kernel(const double *X, size_t count, double *Y) {
     if (e < count) Y[e] = X[e] - Y[e];
 }
 
-// B[:, p] = -(Rl[0:n, sc_p] + j w Ll[0:n, sc_p]) (the port loops' columns, free rows).
-__global__ void rhs_kernel(const float *Ll, long long ldl, const int *ptr, const int *col, const double *val,
+// B[:, p] = -(Rl[0:n, sc_p] + j w Ll[0:n, sc_p]) (the port loops' columns, free rows; Ll FP64).
+__global__ void rhs_kernel(const double *Ll, long long ldl, const int *ptr, const int *col, const double *val,
                            int n, const int *sc, int P, double omega, double *B) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * P) return;
@@ -722,11 +735,11 @@ __global__ void rhs_kernel(const float *Ll, long long ldl, const int *ptr, const
     for (int t = ptr[row]; t < ptr[row + 1]; ++t)
         if (col[t] == sc[p]) re += val[t];
     B[(size_t)row * 2 * P + p] = -re;
-    B[(size_t)row * 2 * P + P + p] = -omega * (double)Ll[(size_t)row * ldl + sc[p]];
+    B[(size_t)row * 2 * P + P + p] = -omega * Ll[(size_t)row * ldl + sc[p]];
 }
 
-// Zsf X: out[s][q] = sum_i (Rl[sc_s, i] + j w Ll[sc_s, i]) X[i, q] over free i < n (complex).
-__global__ __launch_bounds__(256) void port_reduce_kernel(const float *Ll, long long ldl, const int *ptr,
+// Zsf X: out[s][q] = sum_i (Rl[sc_s, i] + j w Ll[sc_s, i]) X[i, q] over free i < n (complex; Ll FP64).
+__global__ __launch_bounds__(256) void port_reduce_kernel(const double *Ll, long long ldl, const int *ptr,
                                                          const int *col, const double *val, int n, const int *sc,
                                                          int P, double omega, const double *X, double *out) {
     __shared__ double sr[256], si[256];
@@ -735,7 +748,7 @@ __global__ __launch_bounds__(256) void port_reduce_kernel(const float *Ll, long 
     const int row = sc[s];
     double re = 0, im = 0;
     for (int i = threadIdx.x; i < n; i += 256) {
-        const double l = omega * (double)Ll[(size_t)row * ldl + i];
+        const double l = omega * Ll[(size_t)row * ldl + i];
         const double xr = X[(size_t)i * NR + q], xi = X[(size_t)i * NR + P + q];
         re += -l * xi;
         im += l * xr;
@@ -1076,7 +1089,7 @@ struct pq_loops {
     int device = 0;
     int nl = 0, nf = 0, P = 0;
     long long ldl = 0;
-    DevBuf Ll, rptr, rcol, rval, sc, inv, bjperm, singular;
+    DevBuf Ll, Ll64, rptr, rcol, rval, sc, inv, bjperm, singular;   // Ll FP32, Ll64 FP64 (same ldl)
     DevBuf V, W, Z, X, B, R, partial, h, scal;
     hipStream_t stream = nullptr;
     int nblk = 0;
@@ -1123,6 +1136,12 @@ struct pq_loops {
     DevBuf Fbat, Xbat, Wbat;
     std::vector<double> batch_omega;
     double batch_ms = 0;
+    std::vector<int> batch_bad;   // per matrix of the batch: a zero or non-finite FP32 pivot
+    // the FP64 fallback of the direct solve (an LU with partial pivoting of Z_ff in FP64 through
+    // rocSOLVER, loaded on first use): frequencies of the last direct call it solved
+    int fallbacks = 0;
+    void *blas = nullptr;         // rocblas_handle
+    DevBuf gpart64;               // the FP64 product's partials (apart from the captured steps' buffers)
 };
 
 namespace {
@@ -1274,6 +1293,36 @@ void apply_operator(pq_loops *L, const double *X, double *W, int P) {
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
         operator_launch<N>(L, sh, X, W, L->gpart.as<double>(), L->omega_dev.as<double>());
+    });
+    PQ_HIP(hipGetLastError());
+}
+
+// W = (Rl + j w Ll64) X with the FP64 loop matrix: the system's own product, for right-hand
+// sides' residuals (GMRES restarts and acceptance, the direct solve's refinement). The panel
+// kernel over FP64 rows, with the chunk the autotuner chose for the FP32 product (or the default
+// where it chose the symmetric kernel); twice the FP32 product's bytes.
+void apply_operator64(pq_loops *L, const double *X, double *W, int P) {
+    const int NR = 2 * P, n = L->nf;
+    OpShape sh = gemv_shape(L, NR);
+    if (sh.chunk <= 0) sh = {default_chunk(n), GEMV_ROWS};
+    const int nsplit = operator_splits(n, sh.chunk);
+    L->gpart64.reserve((size_t)nsplit * n * NR * 8);
+    double *part = L->gpart64.as<double>();
+    const double *omega_dev = L->omega_dev.as<double>();
+    dispatch_nr(NR, [&](auto nr) {
+        constexpr int N = decltype(nr)::value;
+        hipLaunchKernelGGL((panel_kernel<N, double>), dim3((n + sh.rows - 1) / sh.rows, (n + sh.chunk - 1) / sh.chunk),
+                           dim3(sh.rows), 0, L->stream, L->Ll64.as<double>(), L->ldl, n, n, X, sh.chunk, part);
+        if constexpr ((N & (N - 1)) == 0) {
+            hipLaunchKernelGGL(operator_epilogue_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream,
+                               L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, X, part, nsplit,
+                               omega_dev, W);
+        } else {
+            hipLaunchKernelGGL(spmv_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                               L->rcol.as<int>(), L->rval.as<double>(), n, X, W);
+            hipLaunchKernelGGL(gemv_epilogue_kernel<N>, dim3((n * (N / 2) + 255) / 256), dim3(256), 0, L->stream,
+                               part, nsplit, n, omega_dev, W);
+        }
     });
     PQ_HIP(hipGetLastError());
 }
@@ -1612,7 +1661,7 @@ void port_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols) {
     L->sc.reserve(P * 4);
     L->B.reserve((size_t)n * 2 * P * 8);
     PQ_HIP(hipMemcpyAsync(L->sc.p, port_cols, P * 4, hipMemcpyHostToDevice, L->stream));
-    hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+    hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl,
                        L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega,
                        L->B.as<double>());
     PQ_HIP(hipGetLastError());
@@ -1673,7 +1722,7 @@ void port_impedance(pq_loops *L, double omega, int P, const int32_t *port_cols, 
     DevBuf red;
     red.reserve((size_t)P * P * 2 * 8);
     if (n > 0) {
-        hipLaunchKernelGGL(port_reduce_kernel, dim3(P * P), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+        hipLaunchKernelGGL(port_reduce_kernel, dim3(P * P), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl,
                            L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, X,
                            red.as<double>());
         PQ_HIP(hipGetLastError());
@@ -1683,11 +1732,11 @@ void port_impedance(pq_loops *L, double omega, int P, const int32_t *port_cols, 
     std::vector<double> zsf((size_t)P * P * 2);
     PQ_HIP(hipMemcpyAsync(zsf.data(), red.p, zsf.size() * 8, hipMemcpyDeviceToHost, L->stream));
     // Z_ss from the stored rows
-    std::vector<float> llss((size_t)P * P);
+    std::vector<double> llss((size_t)P * P);
     for (int s = 0; s < P; ++s)
         for (int q = 0; q < P; ++q)
-            PQ_HIP(hipMemcpyAsync(&llss[s * P + q], L->Ll.as<float>() + (size_t)port_cols[s] * L->ldl + port_cols[q], 4,
-                                  hipMemcpyDeviceToHost, L->stream));
+            PQ_HIP(hipMemcpyAsync(&llss[s * P + q], L->Ll64.as<double>() + (size_t)port_cols[s] * L->ldl + port_cols[q],
+                                  8, hipMemcpyDeviceToHost, L->stream));
     std::vector<int> rp(L->nl + 1);
     PQ_HIP(hipMemcpyAsync(rp.data(), L->rptr.p, (L->nl + 1) * 4, hipMemcpyDeviceToHost, L->stream));
     PQ_HIP(hipStreamSynchronize(L->stream));
@@ -1702,7 +1751,7 @@ void port_impedance(pq_loops *L, double omega, int P, const int32_t *port_cols, 
             double re = 0;
             for (int t = 0; t < cnt; ++t) if (cols[t] == port_cols[q]) re += vals[t];
             Zp[2 * (s * P + q)] = re + zsf[2 * (s * P + q)];
-            Zp[2 * (s * P + q) + 1] = omega * (double)llss[s * P + q] + zsf[2 * (s * P + q) + 1];
+            Zp[2 * (s * P + q) + 1] = omega * llss[s * P + q] + zsf[2 * (s * P + q) + 1];
         }
     }
 }
@@ -1737,16 +1786,155 @@ void check_resistance(int n_loops, const int32_t *r_ptr, const int32_t *r_col) {
         PQ_REQUIRE(r_col[t] >= 0 && r_col[t] < n_loops, "pq_loops_create: resistance column out of range");
 }
 
+// ---- the direct solve's FP64 fallback ----------------------------------------------------
+// Where the FP32 factor cannot bring a frequency's residual to the reference's level (a zero or
+// non-finite FP32 pivot, or refinement that stops converging above 1e-10), that frequency is
+// solved again with an FP64 LU with partial pivoting of Z_ff = Rl + j w Ll64, as MixedLU
+// refactors in double (src/pcbqs/kernels.py:168-181). rocSOLVER's zgetrf/zgetrs do the LU; they
+// are loaded on first use, so a process that never falls back never loads them.
+
+// Z (n, n) complex FP64 = (0, w Ll64) over the free block, then Rl's free entries added (Z is
+// symmetric, so its row-major layout is also the column-major one the solver reads).
+__global__ void fallback_form_kernel(const double *Ll, long long ldl, int n, double omega, double2 *Z) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n) return;
+    const long long i = e / n, j = e % n;
+    Z[e] = make_double2(0.0, omega * Ll[i * ldl + j]);
+}
+
+__global__ void fallback_resistance_kernel(const int *ptr, const int *col, const double *val, int n, double2 *Z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int t = ptr[i]; t < ptr[i + 1]; ++t)
+        if (col[t] < n) Z[(size_t)i * n + col[t]].x += val[t];
+}
+
+// (n, 2P) split layout <-> (n, P) complex column-major
+__global__ void fallback_pack_kernel(const double *B, int n, int P, double2 *Bc) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * P) return;
+    const int i = e / P, p = e % P;
+    Bc[(size_t)p * n + i] = make_double2(B[(size_t)i * 2 * P + p], B[(size_t)i * 2 * P + P + p]);
+}
+
+__global__ void fallback_unpack_kernel(const double2 *Bc, int n, int P, double *X) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * P) return;
+    const int i = e / P, p = e % P;
+    const double2 v = Bc[(size_t)p * n + i];
+    X[(size_t)i * 2 * P + p] = v.x;
+    X[(size_t)i * 2 * P + P + p] = v.y;
+}
+
+struct Fp64Lu {
+    decltype(&rocblas_create_handle) create = nullptr;
+    decltype(&rocblas_destroy_handle) destroy = nullptr;
+    decltype(&rocblas_set_stream) set_stream = nullptr;
+    decltype(&rocsolver_zgetrf) getrf = nullptr;
+    decltype(&rocsolver_zgetrs) getrs = nullptr;
+    std::string error;
+};
+
+const Fp64Lu &fp64_lu() {
+    static Fp64Lu lu;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        void *blas = dlopen("librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+        if (!blas) blas = dlopen("librocblas.so", RTLD_NOW | RTLD_LOCAL);
+        void *solver = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+        if (!solver) solver = dlopen("librocsolver.so", RTLD_NOW | RTLD_LOCAL);
+        if (!blas || !solver) {
+            lu.error = std::string("FP64 fallback: cannot load rocBLAS/rocSOLVER: ") + dlerror();
+            return;
+        }
+        lu.create = (decltype(lu.create))dlsym(blas, "rocblas_create_handle");
+        lu.destroy = (decltype(lu.destroy))dlsym(blas, "rocblas_destroy_handle");
+        lu.set_stream = (decltype(lu.set_stream))dlsym(blas, "rocblas_set_stream");
+        lu.getrf = (decltype(lu.getrf))dlsym(solver, "rocsolver_zgetrf");
+        lu.getrs = (decltype(lu.getrs))dlsym(solver, "rocsolver_zgetrs");
+        if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs)
+            lu.error = "FP64 fallback: rocBLAS/rocSOLVER entry points missing";
+    });
+    if (!lu.error.empty()) throw Error(PQ_EDEVICE, lu.error);
+    return lu;
+}
+
+void blas_check(rocblas_status st, const char *what) {
+    if (st != rocblas_status_success)
+        throw Error(PQ_EDEVICE, std::string("FP64 fallback: ") + what + " failed (rocblas status " +
+                                    std::to_string((int)st) + ")");
+}
+
+// X (n, 2P) = Z_ff^-1 B at omega by the FP64 LU; the matrix and pivots live only for the call.
+void fallback_solve(pq_loops *L, double omega, int P, const double *B, double *X) {
+    const Fp64Lu &lu = fp64_lu();
+    const int n = L->nf;
+    if (!L->blas) {
+        rocblas_handle h = nullptr;
+        blas_check(lu.create(&h), "rocblas_create_handle");
+        L->blas = h;
+    }
+    rocblas_handle h = (rocblas_handle)L->blas;
+    blas_check(lu.set_stream(h, L->stream), "rocblas_set_stream");
+    DevBuf Z, piv, info, Bc;
+    Z.reserve((size_t)n * n * 16);
+    piv.reserve((size_t)n * 4);
+    info.reserve(4);
+    Bc.reserve((size_t)n * P * 16);
+    const long long cnt = (long long)n * n;
+    hipLaunchKernelGGL(fallback_form_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
+                       L->Ll64.as<double>(), L->ldl, n, omega, Z.as<double2>());
+    hipLaunchKernelGGL(fallback_resistance_kernel, dim3((n + 255) / 256), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                       L->rcol.as<int>(), L->rval.as<double>(), n, Z.as<double2>());
+    hipLaunchKernelGGL(fallback_pack_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, B, n, P,
+                       Bc.as<double2>());
+    PQ_HIP(hipGetLastError());
+    auto *A = reinterpret_cast<rocblas_double_complex *>(Z.p);
+    blas_check(lu.getrf(h, n, n, A, n, piv.as<rocblas_int>(), info.as<rocblas_int>()), "rocsolver_zgetrf");
+    rocblas_int singular = 0;
+    PQ_HIP(hipMemcpyAsync(&singular, info.p, 4, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    if (singular) throw Error(PQ_ECONVERGE, "direct solve: Z_ff is singular in FP64 (zero pivot " +
+                                                std::to_string(singular) + ")");
+    blas_check(lu.getrs(h, rocblas_operation_none, n, P, A, n, piv.as<rocblas_int>(),
+                        reinterpret_cast<rocblas_double_complex *>(Bc.p), n),
+               "rocsolver_zgetrs");
+    hipLaunchKernelGGL(fallback_unpack_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, Bc.as<double2>(),
+                       n, P, X);
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipStreamSynchronize(L->stream));       // Z, the pivots and Bc are freed on return
+}
+
+// The largest column's relative residual |B - Z X| / |B| (FP64 system) into *worst; W is work.
+double relative_residual64(pq_loops *L, const double *B, const double *X, double *W, int P,
+                           const std::vector<double> &bnorm) {
+    const size_t count = (size_t)L->nf * 2 * P;
+    apply_operator64(L, X, W, P);
+    ++L->matvecs;
+    hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, B, count, W);
+    PQ_HIP(hipGetLastError());
+    std::vector<double> rn;
+    norms(L, W, P, rn);
+    double worst = 0;
+    for (int p = 0; p < P; ++p) {
+        const double r = rn[p] / bnorm[p];
+        worst = std::isnan(r) || std::isnan(worst) ? std::numeric_limits<double>::quiet_NaN() : std::max(worst, r);
+    }
+    return worst;
+}
+
 }  // namespace
 
 extern "C" {
 
 int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr, const int32_t *c_rows,
-                    const float *c_sign, int64_t c_nnz, const float *Lb_dev, int64_t ldb, int n_branch_rows,
-                    const int32_t *r_ptr, const int32_t *r_col, const double *r_val, pq_loops **out) {
+                    const float *c_sign, int64_t c_nnz, const void *Lb_dev, int lb_precision, int64_t ldb,
+                    int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col, const double *r_val,
+                    pq_loops **out) {
     return guard([&] {
         PQ_REQUIRE(out && c_colptr && c_rows && c_sign && Lb_dev && r_ptr && r_col && r_val,
                    "pq_loops_create: null argument");
+        PQ_REQUIRE(lb_precision == 32 || lb_precision == 64, "pq_loops_create: Lb precision must be 32 or 64");
         PQ_REQUIRE(n_loops > 0 && n_free >= 0 && n_free <= n_loops && n_branch_rows > 0 && ldb >= n_branch_rows,
                    "pq_loops_create: invalid sizes");
         PQ_REQUIRE(c_colptr[0] == 0 && c_colptr[n_loops] == c_nnz, "pq_loops_create: inconsistent loop basis");
@@ -1764,18 +1952,25 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
             PQ_HIP(hipMemcpy(cr.p, c_rows, c_nnz * 4, hipMemcpyHostToDevice));
             PQ_HIP(hipMemcpy(cs.p, c_sign, c_nnz * 4, hipMemcpyHostToDevice));
             const long long ldu = L->ldl;
-            U.reserve((size_t)n_branch_rows * ldu * 4);
+            U.reserve((size_t)n_branch_rows * ldu * 8);
             L->Ll.reserve((size_t)n_loops * L->ldl * 4);
-            hipLaunchKernelGGL(proj_left_kernel, dim3((n_branch_rows + PROJ_ROWS - 1) / PROJ_ROWS), dim3(256), 0, L->stream,
-                               Lb_dev, (long long)ldb,
-                               n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, U.as<float>(), ldu);
-            hipLaunchKernelGGL(proj_right_kernel, dim3(n_loops), dim3(256), 0, L->stream, U.as<float>(), ldu,
-                               n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, L->Ll.as<float>(),
+            L->Ll64.reserve((size_t)n_loops * L->ldl * 8);
+            const dim3 lgrid((n_branch_rows + PROJ_ROWS - 1) / PROJ_ROWS);
+            if (lb_precision == 64)
+                hipLaunchKernelGGL(proj_left_kernel<double>, lgrid, dim3(256), 0, L->stream, (const double *)Lb_dev,
+                                   (long long)ldb, n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops,
+                                   U.as<double>(), ldu);
+            else
+                hipLaunchKernelGGL(proj_left_kernel<float>, lgrid, dim3(256), 0, L->stream, (const float *)Lb_dev,
+                                   (long long)ldb, n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops,
+                                   U.as<double>(), ldu);
+            hipLaunchKernelGGL(proj_right_kernel, dim3(n_loops), dim3(256), 0, L->stream, U.as<double>(), ldu,
+                               n_branch_rows, cp.as<int>(), cr.as<int>(), cs.as<float>(), n_loops, L->Ll64.as<double>(),
                                L->ldl);
             {
                 const long long cnt = (long long)n_loops * n_loops;
                 hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
-                                   L->Ll.as<float>(), L->ldl, n_loops);
+                                   L->Ll64.as<double>(), L->Ll.as<float>(), L->ldl, n_loops);
             }
             PQ_HIP(hipGetLastError());
             upload_resistance(L, n_loops, r_ptr, r_col, r_val);
@@ -1788,13 +1983,14 @@ int pq_loops_create(int device, int n_loops, int n_free, const int32_t *c_colptr
     });
 }
 
-int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tr, const float *Lb_dev, int64_t ldb,
-                         int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col, const double *r_val,
-                         pq_loops **out) {
+int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree *tr, const void *Lb_dev,
+                         int lb_precision, int64_t ldb, int n_branch_rows, const int32_t *r_ptr, const int32_t *r_col,
+                         const double *r_val, pq_loops **out) {
     return guard([&] {
         PQ_REQUIRE(out && tr && Lb_dev && r_ptr && r_col && r_val && tr->parent && tr->up_branch && tr->up_sign &&
                        tr->level_ptr && tr->level_nodes && tr->closing && tr->end_a && tr->end_b,
                    "pq_loops_create_tree: null argument");
+        PQ_REQUIRE(lb_precision == 32 || lb_precision == 64, "pq_loops_create_tree: Lb precision must be 32 or 64");
         const int nn = tr->n_nodes, nlev = tr->n_levels;
         PQ_REQUIRE(n_loops > 0 && n_free >= 0 && n_free <= n_loops && n_branch_rows > 0 && ldb >= n_branch_rows &&
                        nn > 0 && nlev >= 0,
@@ -1837,39 +2033,51 @@ int pq_loops_create_tree(int device, int n_loops, int n_free, const pq_loop_tree
             PQ_HIP(hipMemcpy(cot.p, tr->closing, (size_t)n_loops * 4, hipMemcpyHostToDevice));
             PQ_HIP(hipMemcpy(ea.p, tr->end_a, (size_t)n_loops * 4, hipMemcpyHostToDevice));
             PQ_HIP(hipMemcpy(eb.p, tr->end_b, (size_t)n_loops * 4, hipMemcpyHostToDevice));
-            auto levels = [&](const float *Row, long long ldr, int width, double *Sd, long long lds) {
+            auto levels = [&](auto *Row, long long ldr, int width, double *Sd, long long lds) {
+                using TR = std::remove_const_t<std::remove_pointer_t<decltype(Row)>>;
                 for (int d = 0; d < nlev; ++d) {
                     const int c = tr->level_ptr[d + 1] - tr->level_ptr[d];
                     if (!c) continue;
-                    hipLaunchKernelGGL(tree_level_kernel, dim3((width + 255) / 256, std::min(c, 65535)), dim3(256), 0,
-                                       L->stream, Row, ldr, nbr, width, lev.as<int>() + tr->level_ptr[d], c,
+                    hipLaunchKernelGGL(tree_level_kernel<TR>, dim3((width + 255) / 256, std::min(c, 65535)), dim3(256),
+                                       0, L->stream, Row, ldr, nbr, width, lev.as<int>() + tr->level_ptr[d], c,
                                        par.as<int>(), upb.as<int>(), ups.as<float>(), Sd, lds);
                 }
             };
-            // U = Lb C (nbr x n_loops, FP32, row stride ldu)
+            // U = Lb C (nbr x n_loops, FP64, row stride ldu)
             const long long ldu = L->ldl, lds1 = (nbr + 63) / 64 * 64;
             S.reserve((size_t)nn * lds1 * 8);
             PQ_HIP(hipMemsetAsync(S.p, 0, (size_t)nn * lds1 * 8, L->stream));
-            levels(Lb_dev, ldb, nbr, S.as<double>(), lds1);
-            U.reserve((size_t)nbr * ldu * 4);
-            hipLaunchKernelGGL(tree_u_kernel, dim3((nbr + 63) / 64, (n_loops + 63) / 64), dim3(256), 0, L->stream, Lb_dev,
-                               (long long)ldb, nbr, S.as<double>(), lds1, cot.as<int>(), ea.as<int>(), eb.as<int>(),
-                               n_loops, U.as<float>(), ldu);
+            U.reserve((size_t)nbr * ldu * 8);
+            const dim3 ugrid((nbr + 63) / 64, (n_loops + 63) / 64);
+            if (lb_precision == 64) {
+                const double *Lb = (const double *)Lb_dev;
+                levels(Lb, ldb, nbr, S.as<double>(), lds1);
+                hipLaunchKernelGGL(tree_u_kernel<double>, ugrid, dim3(256), 0, L->stream, Lb, (long long)ldb, nbr,
+                                   S.as<double>(), lds1, cot.as<int>(), ea.as<int>(), eb.as<int>(), n_loops,
+                                   U.as<double>(), ldu);
+            } else {
+                const float *Lb = (const float *)Lb_dev;
+                levels(Lb, ldb, nbr, S.as<double>(), lds1);
+                hipLaunchKernelGGL(tree_u_kernel<float>, ugrid, dim3(256), 0, L->stream, Lb, (long long)ldb, nbr,
+                                   S.as<double>(), lds1, cot.as<int>(), ea.as<int>(), eb.as<int>(), n_loops,
+                                   U.as<double>(), ldu);
+            }
             PQ_HIP(hipStreamSynchronize(L->stream));
             S.release();
             // Ll = C^T U, the same prefix sums down the tree on the rows of U
             const long long lds2 = L->ldl;
             S.reserve((size_t)nn * lds2 * 8);
             PQ_HIP(hipMemsetAsync(S.p, 0, (size_t)nn * lds2 * 8, L->stream));
-            levels(U.as<float>(), ldu, n_loops, S.as<double>(), lds2);
+            levels((const double *)U.as<double>(), ldu, n_loops, S.as<double>(), lds2);
             L->Ll.reserve((size_t)n_loops * L->ldl * 4);
+            L->Ll64.reserve((size_t)n_loops * L->ldl * 8);
             hipLaunchKernelGGL(tree_ll_kernel, dim3((n_loops + 255) / 256, std::min(n_loops, 65535)), dim3(256), 0,
-                               L->stream, U.as<float>(), ldu, nbr, S.as<double>(), lds2, cot.as<int>(), ea.as<int>(),
-                               eb.as<int>(), n_loops, L->Ll.as<float>(), L->ldl);
+                               L->stream, U.as<double>(), ldu, nbr, S.as<double>(), lds2, cot.as<int>(), ea.as<int>(),
+                               eb.as<int>(), n_loops, L->Ll64.as<double>(), L->ldl);
             {
                 const long long cnt = (long long)n_loops * n_loops;
                 hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
-                                   L->Ll.as<float>(), L->ldl, n_loops);
+                                   L->Ll64.as<double>(), L->Ll.as<float>(), L->ldl, n_loops);
             }
             PQ_HIP(hipGetLastError());
             upload_resistance(L, n_loops, r_ptr, r_col, r_val);
@@ -1890,7 +2098,7 @@ int pq_loops_destroy(pq_loops *L) {
     return guard([&] {
         DeviceScope scope(L->device);
         if (L->stream) (void)hipStreamSynchronize(L->stream);
-        for (DevBuf *b : {&L->Ll, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->bjperm, &L->singular, &L->V, &L->W, &L->Z,
+        for (DevBuf *b : {&L->Ll, &L->Ll64, &L->gpart64, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->bjperm, &L->singular, &L->V, &L->W, &L->Z,
                           &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
                           &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit, &L->ppart})
             b->release();
@@ -1901,15 +2109,17 @@ int pq_loops_destroy(pq_loops *L) {
         if (L->ev_ready) (void)hipEventDestroy(L->ev_ready);
         if (L->ev_done) (void)hipEventDestroy(L->ev_done);
         if (L->aux) (void)hipStreamDestroy(L->aux);
+        if (L->blas) (void)fp64_lu().destroy((rocblas_handle)L->blas);
         if (L->stream) (void)hipStreamDestroy(L->stream);
         delete L;
     });
 }
 
-int pq_loops_matrix(pq_loops *L, void **Ll_dev, int64_t *ldl) {
+int pq_loops_matrix(pq_loops *L, int precision, void **Ll_dev, int64_t *ldl) {
     return guard([&] {
         PQ_REQUIRE(L, "pq_loops_matrix: null handle");
-        if (Ll_dev) *Ll_dev = L->Ll.p;
+        PQ_REQUIRE(precision == 32 || precision == 64, "pq_loops_matrix: precision must be 32 or 64");
+        if (Ll_dev) *Ll_dev = precision == 64 ? L->Ll64.p : L->Ll.p;
         if (ldl) *ldl = L->ldl;
     });
 }
@@ -1947,7 +2157,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
         double *V = L->V.as<double>(), *W = L->W.as<double>(), *Z = L->Z.as<double>(), *X = L->X.as<double>();
         double *B = L->B.as<double>(), *R = L->R.as<double>();
         if (n > 0) {
-            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl,
                                L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, B);
             PQ_HIP(hipGetLastError());
             if (precond == 2) prepare_coarse(L, omega, P);
@@ -1971,7 +2181,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             for (auto &b : bnorm) b = std::max(b, 1e-300);
             if (X0_host) {                                                     // R = B - A x0
                 PQ_HIP(hipMemcpyAsync(X, X0_host, vsz * 8, hipMemcpyHostToDevice, L->stream));
-                apply_operator(L, X, R, P);
+                apply_operator64(L, X, R, P);
                 ++L->matvecs;
                 hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B,
                                    vsz, R);
@@ -2149,7 +2359,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                     for (int p = 0; p < P; ++p) one[2 * p] = 1.0;
                     update(L, Z, 0, 1, one.data(), -1.0, P, X);                    // X += Z
                 }
-                apply_operator(L, X, R, P);                                         // true residual
+                apply_operator64(L, X, R, P);                             // true residual, FP64 system
                 ++L->matvecs;
                 hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream, B, vsz, R);
                 PQ_HIP(hipGetLastError());
@@ -2215,20 +2425,19 @@ int pq_loops_factor_batch(pq_loops *L, int nw, const double *omegas, double *ms)
         LdlFactor f = batch_view(L, 0);
         f.batch = nw;
         f.sF = (long long)sF; f.sX = (long long)sX; f.sW = (long long)sW;
-        PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
+        // one flag per matrix: a zero or non-finite FP32 pivot sends that frequency to the FP64
+        // fallback in the direct solves that use this batch
+        L->singular.reserve((size_t)nw * 4);
+        PQ_HIP(hipMemsetAsync(L->singular.p, 0, (size_t)nw * 4, L->stream));
         ldl_factor(L->stream, f, L->singular.as<int>());
         PQ_HIP(hipEventRecord(e1, L->stream));
-        int bad = 0;
-        PQ_HIP(hipMemcpyAsync(&bad, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));
+        L->batch_bad.assign(nw, 0);
+        PQ_HIP(hipMemcpyAsync(L->batch_bad.data(), L->singular.p, (size_t)nw * 4, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
         float t = 0;
         (void)hipEventElapsedTime(&t, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        if (bad) {
-            L->batch_omega.clear();
-            throw Error(PQ_EINVAL, "direct solve: a zero or non-finite pivot in a batched L D L^T factorisation");
-        }
         L->batch_ms = t;
         if (ms) *ms = t;
     });
@@ -2260,7 +2469,7 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
         if (n > 0) {
             tune_gemv(L, NR);
             set_omega(L, omega);
-            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(), L->ldl,
+            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl,
                                L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P, omega, B);
             PQ_HIP(hipGetLastError());
             const int nblk = (n + LDL_BLOCK - 1) / LDL_BLOCK;
@@ -2279,9 +2488,12 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
                         L->Xldl.as<float>(), L->Xldl.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK,
                         {Wb, Wb + 2 * wsz}, {Wb + wsz, Wb + 3 * wsz}, L->aux, L->ev_ready, L->ev_done};
             // a factor of this frequency from the last batch, if there is one
+            bool bad_pivot = false;
             const auto hit = std::find(L->batch_omega.begin(), L->batch_omega.end(), omega);
             if (hit != L->batch_omega.end()) {
-                f = batch_view(L, (int)(hit - L->batch_omega.begin()));
+                const int b = (int)(hit - L->batch_omega.begin());
+                f = batch_view(L, b);
+                bad_pivot = L->batch_bad[b] != 0;
                 L->factor_ms = L->batch_ms / (double)L->batch_omega.size();
             } else if (L->omega_ldl != omega) {
                 L->omega_ldl = -1;
@@ -2301,44 +2513,46 @@ int pq_loops_solve_direct(pq_loops *L, double omega, int P, const int32_t *port_
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
-                PQ_REQUIRE(!bad, "direct solve: a zero or non-finite pivot in the L D L^T factorisation");
+                bad_pivot = bad != 0;
                 L->factor_ms = ms;
-                L->omega_ldl = omega;
+                L->omega_ldl = bad_pivot ? -1 : omega;      // a failed factor is not kept
             }
-            std::vector<double> bnorm, rn;
+            std::vector<double> bnorm;
             norms(L, B, P, bnorm);
             for (auto &b : bnorm) b = std::max(b, 1e-300);
-            ldl_solve(L->stream, f, P, B, Y, Ys, X);
             const size_t count = vsz;
-            double last = std::numeric_limits<double>::infinity();
-            for (int step = 0;; ++step) {
-                apply_operator(L, X, W, P);                                     // W = B - Z X
-                ++L->matvecs;
-                hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, B,
-                                   count, W);
-                PQ_HIP(hipGetLastError());
-                norms(L, W, P, rn);
-                worst = 0;
-                for (int p = 0; p < P; ++p) worst = std::max(worst, rn[p] / bnorm[p]);
-                if (!(worst > tol) || step == max_steps) break;
-                if (worst > 0.5 * last) break;                                  // no longer converging
-                last = worst;
-                ldl_solve(L->stream, f, P, W, Y, Ys, D);                        // X += Z^-1 (B - Z X)
-                hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, D,
-                                   count, X);
-                PQ_HIP(hipGetLastError());
-                ++corrections;
+            if (!bad_pivot) {
+                ldl_solve(L->stream, f, P, B, Y, Ys, X);
+                double last = std::numeric_limits<double>::infinity();
+                for (int step = 0;; ++step) {
+                    worst = relative_residual64(L, B, X, W, P, bnorm);        // W = B - Z X (FP64 system)
+                    if (!(worst > tol) || step == max_steps) break;
+                    if (!(worst <= 0.5 * last)) break;                         // no longer converging
+                    last = worst;
+                    ldl_solve(L->stream, f, P, W, Y, Ys, D);                    // X += Z^-1 (B - Z X)
+                    hipLaunchKernelGGL(vec_add_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                                       L->stream, D, count, X);
+                    PQ_HIP(hipGetLastError());
+                    ++corrections;
+                }
             }
+            L->fallbacks = 0;
+            if (bad_pivot || !(worst <= std::max(tol, 1e-10))) {             // FP64 LU, as MixedLU
+                fallback_solve(L, omega, P, B, X);
+                worst = relative_residual64(L, B, X, W, P, bnorm);
+                L->fallbacks = 1;
+            }
+        } else {
+            L->fallbacks = 0;
         }
         L->iterations = corrections;
         L->restarts = 0;
         L->residual = worst;
         if (steps) *steps = corrections;
         if (residual) *residual = worst;
-        if (worst > std::max(tol, 1e-10))
-            throw Error(PQ_ECONVERGE, "direct solve: the FP32 factor's refinement stalls at relative residual " +
-                                          std::to_string(worst) + " after " + std::to_string(corrections) +
-                                          " corrections");
+        if (!(worst <= std::max(tol, 1e-10)))
+            throw Error(PQ_ECONVERGE, "direct solve: relative residual " + std::to_string(worst) +
+                                          " after the FP64 fallback factor");
         port_impedance(L, omega, P, port_cols, X, Zp);
         if (X_host && n > 0) PQ_HIP(hipMemcpy(X_host, X, vsz * 8, hipMemcpyDeviceToHost));
         L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
@@ -2392,8 +2606,10 @@ int pq_loops_solve_direct_batch(pq_loops *L, int nw, const double *omegas, int P
         std::vector<double> rn, worst(nw, 0.0), last(nw, std::numeric_limits<double>::infinity());
         std::vector<int> corr(nw, 0);
         std::vector<char> active(nw, 1);
+        for (int b = 0; b < nw; ++b)
+            if (L->batch_bad[b]) active[b] = 0;          // no usable FP32 factor: the FP64 fallback below
         for (int b = 0; b < nw; ++b) {
-            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll.as<float>(),
+            hipLaunchKernelGGL(rhs_kernel, dim3((n * P + 255) / 256), dim3(256), 0, L->stream, L->Ll64.as<double>(),
                                L->ldl, L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, L->sc.as<int>(), P,
                                omegas[b], B + b * vsz);
             PQ_HIP(hipGetLastError());
@@ -2401,23 +2617,16 @@ int pq_loops_solve_direct_batch(pq_loops *L, int nw, const double *omegas, int P
             for (auto &v : bnorm[b]) v = std::max(v, 1e-300);
         }
         ldl_solve(L->stream, f, P, B, Yv.as<double>(), Ysv.as<double>(), X);
-        int matvecs = 0;
+        L->matvecs = 0;
         for (int step = 0;; ++step) {
             bool any = false;
             for (int b = 0; b < nw; ++b) {
                 if (!active[b]) continue;
                 PQ_HIP(hipStreamSynchronize(L->stream));                 // earlier products have read omega
                 set_omega(L, omegas[b]);
-                apply_operator(L, X + b * vsz, W + b * vsz, P);             // W = B - Z X
-                ++matvecs;
-                hipLaunchKernelGGL(vec_sub_kernel, dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, L->stream,
-                                   B + b * vsz, vsz, W + b * vsz);
-                PQ_HIP(hipGetLastError());
-                norms(L, W + b * vsz, P, rn);
-                double w = 0;
-                for (int p = 0; p < P; ++p) w = std::max(w, rn[p] / bnorm[b][p]);
+                const double w = relative_residual64(L, B + b * vsz, X + b * vsz, W + b * vsz, P, bnorm[b]);
                 worst[b] = w;
-                if (!(w > tol) || step == max_steps || w > 0.5 * last[b]) {
+                if (!(w > tol) || step == max_steps || !(w <= 0.5 * last[b])) {
                     active[b] = 0;
                     continue;
                 }
@@ -2434,21 +2643,31 @@ int pq_loops_solve_direct_batch(pq_loops *L, int nw, const double *omegas, int P
                 }
             PQ_HIP(hipGetLastError());
         }
+        // frequencies the FP32 factor could not bring to the reference's level: the FP64 LU, one
+        // frequency at a time (the others keep their refined solutions)
+        L->fallbacks = 0;
+        for (int b = 0; b < nw; ++b)
+            if (L->batch_bad[b] || !(worst[b] <= std::max(tol, 1e-10))) {
+                PQ_HIP(hipStreamSynchronize(L->stream));
+                set_omega(L, omegas[b]);
+                fallback_solve(L, omegas[b], P, B + b * vsz, X + b * vsz);
+                worst[b] = relative_residual64(L, B + b * vsz, X + b * vsz, W + b * vsz, P, bnorm[b]);
+                ++L->fallbacks;
+            }
         double wmax = 0;
         for (int b = 0; b < nw; ++b) {
             steps[b] = corr[b];
             residual[b] = worst[b];
-            wmax = std::max(wmax, worst[b]);
+            wmax = std::isnan(worst[b]) ? worst[b] : std::max(wmax, worst[b]);
         }
-        if (wmax > std::max(tol, 1e-10))
-            throw Error(PQ_ECONVERGE, "direct solve: the FP32 factor's refinement stalls at relative residual " +
-                                          std::to_string(wmax));
+        if (!(wmax <= std::max(tol, 1e-10)))
+            throw Error(PQ_ECONVERGE, "direct solve: relative residual " + std::to_string(wmax) +
+                                          " after the FP64 fallback factor");
         for (int b = 0; b < nw; ++b)
             port_impedance(L, omegas[b], P, port_cols, X + b * vsz, Zp + (size_t)b * P * P * 2);
         L->iterations = *std::max_element(corr.begin(), corr.end());
         L->restarts = 0;
         L->residual = wmax;
-        L->matvecs = matvecs;
         L->factor_ms = L->batch_ms / nw;
         L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
     });
@@ -2586,10 +2805,11 @@ int pq_loops_rhs(pq_loops *L, double omega, int P, const int32_t *port_cols, dou
     });
 }
 
-int pq_loops_operator(pq_loops *L, double omega, int P, const double *X_host, double *W_host) {
+int pq_loops_operator(pq_loops *L, double omega, int P, int precision, const double *X_host, double *W_host) {
     return guard([&] {
         PQ_REQUIRE(L && X_host && W_host, "pq_loops_operator: null argument");
         PQ_REQUIRE(P >= 1, "pq_loops_operator: P >= 1");
+        PQ_REQUIRE(precision == 32 || precision == 64, "pq_loops_operator: precision must be 32 or 64");
         dispatch_nr(2 * P, [](auto) {});
         DeviceScope scope(L->device);
         const int n = L->nf;
@@ -2601,7 +2821,8 @@ int pq_loops_operator(pq_loops *L, double omega, int P, const double *X_host, do
         X.reserve(bytes);
         W.reserve(bytes);
         PQ_HIP(hipMemcpyAsync(X.p, X_host, bytes, hipMemcpyHostToDevice, L->stream));
-        apply_operator(L, X.as<double>(), W.as<double>(), P);
+        if (precision == 64) apply_operator64(L, X.as<double>(), W.as<double>(), P);
+        else apply_operator(L, X.as<double>(), W.as<double>(), P);
         PQ_HIP(hipMemcpyAsync(W_host, W.p, bytes, hipMemcpyDeviceToHost, L->stream));
         PQ_HIP(hipStreamSynchronize(L->stream));
     });
@@ -2666,6 +2887,13 @@ int pq_loops_last_solve(pq_loops *L, int64_t *iterations, int64_t *matvecs, doub
         if (matvecs) *matvecs = L->matvecs;
         if (residual) *residual = L->residual;
         if (seconds) *seconds = L->seconds;
+    });
+}
+
+int pq_loops_last_fallbacks(pq_loops *L, int64_t *count) {
+    return guard([&] {
+        PQ_REQUIRE(L && count, "pq_loops_last_fallbacks: null argument");
+        *count = L->fallbacks;
     });
 }
 

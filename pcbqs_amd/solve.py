@@ -1,8 +1,12 @@
 """Device-resident port solve of one board: fill -> RWG combine -> loop projection -> GMRES.
 
-The pipeline keeps everything in HBM: the moment planes (FP32 far field, FP64 analytic near
-pairs, local frames), the branch inductance Lb (RWG edges and via segments, FP32), the loop
-operator Ll = C^T Lb C and Rl = C^T R C, and the Krylov basis. Port impedances follow the
+The pipeline keeps everything in HBM: the moment planes, the branch inductance Lb (RWG edges and
+via segments), the loop operator Ll = C^T Lb C and Rl = C^T R C, and the Krylov basis. precision
+32 (the default) is the FP32-bulk pipeline: FP32 far pairs with FP64 analytic near pairs in local
+frames, Lb stored in FP32. precision 64 is the plain FP64 method: FP64 planes in the reference's
+layout and Lb in FP64. Either way Ll is formed and kept in FP64 (the system every residual, right-
+hand side and port reduction uses) with a copy rounded to FP32 for the Krylov products, the
+preconditioners and the FP32 factor. Port impedances follow the
 reference's copper-port reduction (src/pcbqs/mqs_ports.py:35-86): Zp = Z_ss + Z_sf X with
 X = -Z_ff^-1 Z_fs, here by lockstep GMRES (src/pcbqs/mqs_iterative.py:109-170, 298-367);
 R = Re Zp and L = Im Zp / w (rl_equivalent, src/pcbqs/mqs_ports.py:72-86).
@@ -73,25 +77,28 @@ class PortSolver:
         ne, nv = m.ne, m.nv
         nbr = ne + nv
         ldb = padded(nbr)
-        Lb = DeviceArray((nbr, ldb), np.float32, device).zero()
+        lb_type = np.float64 if precision == 64 else np.float32
+        Lb = DeviceArray((nbr, ldb), lb_type, device).zero()
         call('pq_stream_synchronize', None)                   # the zero fill ran on the null stream
         tri, alpha, vertex = rwg.combine_arrays()
         scale = 1.0 / rwg.l
         dev = [DeviceArray.from_host(a, device) for a in (tri, alpha, vertex, scale,
                                                           np.ascontiguousarray(g.cent, np.float64))]
         call('pq_combine_device', int(precision), planes.ptr, len(g.tris), planes.shape[-1], dev[4].ptr, dev[0].ptr,
-             dev[1].ptr, dev[2].ptr, dev[3].ptr, ne, MU0 / (4 * np.pi), 32, Lb.ptr, ldb, stream)
+             dev[1].ptr, dev[2].ptr, dev[3].ptr, ne, MU0 / (4 * np.pi), 64 if precision == 64 else 32, Lb.ptr, ldb,
+             stream)
         call('pq_stream_synchronize', stream)
         for a in dev:
             a.close()
         planes.close()
         for k in range(nv):                                   # the via block
-            row = np.ascontiguousarray(m.Lv[k], np.float32)
+            row = np.ascontiguousarray(m.Lv[k], lb_type)
             call('pq_memcpy_h2d', Lb.offset((ne + k) * ldb + ne), ptr(row), row.nbytes)
         # the branches' self inductances (the graph preconditioner's D), from the device matrix's diagonal
         self.branch_ldiag = np.zeros(m.nb)
-        diag = np.empty(nbr, np.float32)
-        call('pq_memcpy_d2h_strided', ptr(diag), Lb.ptr, 4, (ldb + 1) * 4, nbr)
+        diag = np.empty(nbr, lb_type)
+        size = np.dtype(lb_type).itemsize
+        call('pq_memcpy_d2h_strided', ptr(diag), Lb.ptr, size, (ldb + 1) * size, nbr)
         self.branch_ldiag[:nbr] = diag
         Rl = m.Rl.tocsr()
         Rl.sort_indices()
@@ -101,20 +108,21 @@ class PortSolver:
         self.projection = projection or os.environ.get('PCBQS_AMD_PROJECTION', 'tree')
         if self.projection == 'tree':
             tree, keep = tree_arrays(m.basis)
-            call('pq_loops_create_tree', device, m.n_loops, m.n_free, ctypes.byref(tree), Lb.ptr, ldb, nbr, ptr(rptr),
-                 ptr(rcol), ptr(rval), ctypes.byref(handle))
+            call('pq_loops_create_tree', device, m.n_loops, m.n_free, ctypes.byref(tree), Lb.ptr, 8 * size, ldb, nbr,
+                 ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
         elif self.projection == 'columns':
             C = m.basis.C.tocsc()
             colptr = np.ascontiguousarray(C.indptr, np.int32)
             rows = np.ascontiguousarray(C.indices, np.int32)
             sign = np.ascontiguousarray(C.data, np.float32)
             call('pq_loops_create', device, m.n_loops, m.n_free, ptr(colptr), ptr(rows), ptr(sign), len(rows), Lb.ptr,
-                 ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
+                 8 * size, ldb, nbr, ptr(rptr), ptr(rcol), ptr(rval), ctypes.byref(handle))
         else:
             raise ValueError(f"projection must be 'tree' or 'columns', not {self.projection!r}")
         Lb.close()
         self._handle = handle
         self.port_cols = np.ascontiguousarray(m.port_cols, np.int32)
+        self.precision = precision
         self.setup_seconds = dict(topology=t1 - t0, device=time.perf_counter() - t1)
 
     def set_coarse(self, phi, diagonalize=True):
@@ -184,12 +192,14 @@ class PortSolver:
         call('pq_loops_rhs', self._handle, 2 * np.pi * float(frequency), P, ptr(self.port_cols), ptr(B))
         return self._join(B)
 
-    def operator(self, X, frequency):
-        """Z_ff X = (Rl + j w Ll) X on the free loops, X (n_free, P) complex with P <= 8 or 16."""
+    def operator(self, X, frequency, precision=32):
+        """Z_ff X = (Rl + j w Ll) X on the free loops, X (n_free, P) complex with P <= 8 or 16: with
+        the FP32 copy of Ll (precision 32, the Krylov product) or the FP64 Ll (64, the system)."""
         X = np.asarray(X, complex).reshape(self.model.n_free, -1)
         Xs = self._split(X)
         W = np.empty_like(Xs)
-        call('pq_loops_operator', self._handle, 2 * np.pi * float(frequency), X.shape[1], ptr(Xs), ptr(W))
+        call('pq_loops_operator', self._handle, 2 * np.pi * float(frequency), X.shape[1], int(precision), ptr(Xs),
+             ptr(W))
         return self._join(W)
 
     def coarse_solution(self, frequency):
@@ -238,13 +248,14 @@ class PortSolver:
                 call('pq_loops_solve_direct_batch', self._handle, nw, ptr(om), Pd, ptr(cols), DIRECT_TOL, DIRECT_STEPS,
                      ptr(zp), ptr(st), ptr(rs))
                 seconds = (time.perf_counter() - t) / nw
-                fms = ctypes.c_double()
+                fms, nfb = ctypes.c_double(), ctypes.c_int64()
                 call('pq_loops_factor_time', self._handle, ctypes.byref(fms))
+                call('pq_loops_last_fallbacks', self._handle, ctypes.byref(nfb))
                 for b in range(nw):
                     Z[i0 + b] = zp[b, :P, :P, 0] + 1j * zp[b, :P, :P, 1]
                     info.append(dict(frequency_Hz=float(f[i0 + b]), iterations=int(st[b]), matvecs=int(st[b]) + 1,
                                      relative_residual=float(rs[b]), seconds=seconds, preconditioner='direct',
-                                     factor_ms=float(fms.value)))
+                                     factor_ms=float(fms.value), chunk_fp64_fallbacks=int(nfb.value)))
             return Z, info
         for i, fi in enumerate(f):
             zp = np.zeros((Pd, Pd, 2))
@@ -260,12 +271,13 @@ class PortSolver:
                 Z[i] = zp[:P, :P, 0] + 1j * zp[:P, :P, 1]
                 if currents:
                     X[i] = np.hstack([xh[:, :P], xh[:, Pd:Pd + P]])
-                seconds, mv, fms = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+                seconds, mv, fms, nfb = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
                 call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))
                 call('pq_loops_factor_time', self._handle, ctypes.byref(fms))
+                call('pq_loops_last_fallbacks', self._handle, ctypes.byref(nfb))
                 info.append(dict(frequency_Hz=float(fi), iterations=int(its.value), matvecs=int(mv.value),
                                  relative_residual=float(res.value), seconds=float(seconds.value),
-                                 preconditioner='direct', factor_ms=float(fms.value)))
+                                 preconditioner='direct', factor_ms=float(fms.value), fp64_fallback=bool(nfb.value)))
                 continue
             start = None
             if x0 is not None:
@@ -320,13 +332,14 @@ class PortSolver:
         d = branch_diagonal(self.model.R.diagonal(), self.branch_ldiag, 2 * np.pi * frequency)
         return graph(p, d, device=self.device)
 
-    def loop_matrix(self, rows=None):
-        """A host copy of the device's loop inductance Ll (FP32, the first `rows` rows, all columns)."""
+    def loop_matrix(self, rows=None, precision=32):
+        """A host copy of the device's loop inductance Ll (its FP32 copy, or the FP64 matrix with
+        precision 64; the first `rows` rows, all columns)."""
         n = self.model.n_loops
         rows = n if rows is None else int(rows)
         dev, ld = ctypes.c_void_p(), ctypes.c_int64()
-        call('pq_loops_matrix', self._handle, ctypes.byref(dev), ctypes.byref(ld))
-        out = np.empty((rows, ld.value), np.float32)
+        call('pq_loops_matrix', self._handle, int(precision), ctypes.byref(dev), ctypes.byref(ld))
+        out = np.empty((rows, ld.value), np.float64 if precision == 64 else np.float32)
         call('pq_memcpy_d2h', ptr(out), dev, out.nbytes)
         return out[:, :n]
 

@@ -8,7 +8,9 @@ as L D L^T (64-wide panels, MFMA trailing updates) and refined in FP64 to a rela
   (FP32-stored Ll) matrices;
 - boards: port R and L against the oracle's FP64 dense solve within 1e-6, and against GMRES;
 - the copper fixture at DC (Z_ff = Rl alone) and 100 / 200 kHz;
-- a zero pivot raises instead of returning garbage."""
+- a system whose FP32 factor cannot reach 1e-12 (condition number about 5e7) goes through the FP64
+  fallback factor and matches numpy's FP64 solve, alone and inside a batch whose other frequencies
+  keep their refined solutions; an exactly singular system raises instead of returning garbage."""
 import ctypes
 import os
 
@@ -48,7 +50,7 @@ def spd_loops(n, ports, seed, singular_loop=None):
     rows = np.arange(nl, dtype=np.int32)
     sign = np.ones(nl, np.float32)
     handle = ctypes.c_void_p()
-    call('pq_loops_create', 0, nl, n, ptr(colptr), ptr(rows), ptr(sign), nl, dev.ptr, ldb, nl,
+    call('pq_loops_create', 0, nl, n, ptr(colptr), ptr(rows), ptr(sign), nl, dev.ptr, 32, ldb, nl,
          ptr(np.ascontiguousarray(Rs.indptr, np.int32)), ptr(np.ascontiguousarray(Rs.indices, np.int32)),
          ptr(np.ascontiguousarray(Rs.data, np.float64)), ctypes.byref(handle))
     dev.close()
@@ -108,12 +110,89 @@ def test_factor_is_reused_at_the_same_frequency_and_rebuilt_at_another():
         lib().pq_loops_destroy(handle)
 
 
-def test_zero_pivot_raises():
+def test_singular_system_raises():
+    """A zero row and column: the FP32 factor meets a zero pivot, the FP64 fallback finds the
+    matrix singular, and the solve raises."""
+    from pcbqs_amd._lib import ConvergenceError
     n = 100
     handle, _, _ = spd_loops(n, 2, 5, singular_loop=70)
     try:
-        with pytest.raises(ValueError, match='pivot'):
+        with pytest.raises(ConvergenceError, match='singular'):
             direct(handle, 2 * np.pi * 1e6, n, 2, np.arange(n, n + 2))
+    finally:
+        lib().pq_loops_destroy(handle)
+
+
+def ill_conditioned_loops(n, ports, seed):
+    """C = I; Lb = Q diag(logspace(0, -8)) Q^T 1e-9 (FP32-stored) and Rl = 1e-10 I: at 1 MHz the
+    free block's condition number is about 5e7, beyond what an FP32 factor can refine."""
+    rng = np.random.default_rng(seed)
+    nl = n + ports
+    Q, _ = np.linalg.qr(rng.standard_normal((nl, nl)))
+    Lb = ((Q * np.logspace(0, -8, nl)) @ Q.T * 1e-9)
+    Lb = ((Lb + Lb.T) / 2).astype(np.float32)
+    ldb = (nl + 63) // 64 * 64
+    host = np.zeros((nl, ldb), np.float32)
+    host[:, :nl] = Lb
+    dev = DeviceArray.from_host(host)
+    Rs = sp.identity(nl, format='csr') * 1e-10
+    Rs.sort_indices()
+    colptr = np.arange(nl + 1, dtype=np.int32)
+    rows = np.arange(nl, dtype=np.int32)
+    sign = np.ones(nl, np.float32)
+    handle = ctypes.c_void_p()
+    call('pq_loops_create', 0, nl, n, ptr(colptr), ptr(rows), ptr(sign), nl, dev.ptr, 32, ldb, nl,
+         ptr(np.ascontiguousarray(Rs.indptr, np.int32)), ptr(np.ascontiguousarray(Rs.indices, np.int32)),
+         ptr(np.ascontiguousarray(Rs.data, np.float64)), ctypes.byref(handle))
+    dev.close()
+    return handle, Lb.astype(np.float64), Rs.toarray()
+
+
+def fallbacks(handle):
+    count = ctypes.c_int64()
+    call('pq_loops_last_fallbacks', handle, ctypes.byref(count))
+    return count.value
+
+
+def test_fp64_fallback_factor_solves_what_fp32_cannot():
+    n, P = 300, 4
+    print(f'direct fallback n={n}', flush=True)
+    handle, Ll, R = ill_conditioned_loops(n, P, 3)
+    try:
+        cols = np.arange(n, n + P)
+        omega = 2 * np.pi * 1e6
+        Zff = (R + 1j * omega * Ll)[:n, :n]
+        assert np.linalg.cond(Zff) > 1e7
+        Zp, X, its, res = direct(handle, omega, n, P, cols)
+        assert fallbacks(handle) == 1
+        Ze, Xe = expected(Ll, R, omega, n, cols)
+        assert res <= 1e-12, res
+        assert np.abs(X - Xe).max() <= 1e-6 * np.abs(Xe).max()
+        assert np.abs(Zp - Ze).max() <= 1e-8 * np.abs(Ze).max()
+        # a well-conditioned frequency of the same handle needs no fallback
+        Zp, X, its, res = direct(handle, 0.0, n, P, cols)
+        assert fallbacks(handle) == 0 and res <= 1e-12
+    finally:
+        lib().pq_loops_destroy(handle)
+
+
+def test_batched_direct_solve_falls_back_per_frequency():
+    """In one batched call, only the ill-conditioned frequency takes the FP64 factor; the others
+    keep their FP32-factor solutions, and all match numpy."""
+    n, P = 300, 4
+    handle, Ll, R = ill_conditioned_loops(n, P, 4)
+    try:
+        cols = np.ascontiguousarray(np.arange(n, n + P), np.int32)
+        omegas = np.array([0.0, 2 * np.pi * 1e3, 2 * np.pi * 1e6])
+        zp = np.zeros((3, P, P, 2))
+        st, rs = np.zeros(3, np.int64), np.zeros(3)
+        call('pq_loops_solve_direct_batch', handle, 3, ptr(omegas), P, ptr(cols), 1e-12, 12, ptr(zp), ptr(st), ptr(rs))
+        assert fallbacks(handle) == 1
+        for b, w in enumerate(omegas):
+            Ze, _ = expected(Ll, R, w, n, cols)
+            Z = zp[b, ..., 0] + 1j * zp[b, ..., 1]
+            assert np.abs(Z - Ze).max() <= 1e-8 * np.abs(Ze).max(), b
+            assert rs[b] <= 1e-12, (b, rs[b])
     finally:
         lib().pq_loops_destroy(handle)
 
@@ -174,7 +253,8 @@ def test_batched_factorisations_give_the_single_frequency_factors(monkeypatch):
     (Z0, i0), (Z1, i1) = out['0'], out['1']
     print(f'batched vs single: max diff {np.abs(Z1 - Z0).max() / np.abs(Z0).max():.2e}, '
           f'factor ms {[round(r["factor_ms"], 2) for r in i1]} vs {[round(r["factor_ms"], 2) for r in i0]}', flush=True)
-    assert np.abs(Z1 - Z0).max() <= 1e-12 * np.abs(Z0).max()
+    assert np.array_equal(Z1, Z0)
+    assert [r['iterations'] for r in i1] == [r['iterations'] for r in i0]
     assert all(r['relative_residual'] <= 1e-12 for r in i1)
     Zo = oracle_ports(b, freqs)
     w = 2 * np.pi * np.maximum(np.array(freqs), 1e-300)[:, None, None]

@@ -109,6 +109,18 @@ int pq_fill_symmetric(pq_fill *f, double near, int outer_levels, int scalar_only
 int pq_pairs(pq_fill *f, const int32_t *p, const int32_t *q, const double *d,
              const int32_t *kind, int64_t n, int scalar_only, double *out);
 
+/* pq_pairs without any device allocation: the same arguments and the same
+ * result, bit for bit, with its staging buffers (the pair arrays, the kind
+ * order and the moments) in a fixed device array of the library (about 75 MB
+ * per device, held until the process ends) instead of memory allocated per
+ * call. At most 1048576 pairs per call (PQ_EINVAL beyond). Calls on one device
+ * share that array: calls that overlap on one device race, so callers
+ * serialise them. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+int pq_pairs_noalloc(pq_fill *f, const int32_t *p, const int32_t *q, const double *d,
+                     const int32_t *kind, int64_t n, int scalar_only, double *out);
+
 /* Device-resident fill: planes_dev holds 1 or 4 planes of nt*nt elements,
  * plane k at planes_dev + k*nt*nt, row-major. precision 64: reference layout
  * (moments about the centred origin, FP64 throughout). precision 32: FP32

@@ -64,6 +64,7 @@ _SIGNATURES = {
     'pq_fill_destroy': ([P], ctypes.c_int),
     'pq_fill_symmetric': ([P, DBL, ctypes.c_int, ctypes.c_int, P, P], ctypes.c_int),
     'pq_pairs': ([P, P, P, P, P, I64, ctypes.c_int, P], ctypes.c_int),
+    'pq_pairs_noalloc': ([P, P, P, P, P, I64, ctypes.c_int, P], ctypes.c_int),
     'pq_fill_device': ([P, DBL, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, I64, P, P], ctypes.c_int),
     'pq_fill_last_timing': ([P, P, P, P], ctypes.c_int),
     'pq_combine_device': ([ctypes.c_int, P, I64, I64, P, P, P, P, P, I64, DBL, ctypes.c_int, P, I64, P], ctypes.c_int),

@@ -20,6 +20,7 @@
 // Classification is the reference rule evaluated in FP64 without contraction; an FP32
 // prefilter with a 1e-5 relative margin settles the clearly-far pairs without it.
 #include "pq_common.h"
+#include <mutex>
 #include <cmath>
 #include <vector>
 #include <algorithm>
@@ -637,6 +638,23 @@ __global__ __launch_bounds__(128) void pairs_kernel(GeomDev g, const int *P, con
     for (int k = 0; k < width; ++k) out[i * width + k] = m[k];
 }
 
+// The kind buckets of an explicit pair list on the device (pq_pairs_noalloc): cursor[k + 1] starts
+// at bucket k's first slot (set by the host's counts), and each pair takes the next slot of its
+// bucket. The order within a bucket depends on the atomics' order, but every pair's moments are
+// computed from that pair alone and stored at its own index, so the output does not.
+__global__ void bucket_init_kernel(unsigned *cursor, unsigned s0, unsigned s1, unsigned s2, unsigned s3) {
+    if (threadIdx.x == 0) {
+        cursor[0] = s0; cursor[1] = s1; cursor[2] = s2; cursor[3] = s3;
+    }
+}
+
+__global__ void bucket_scatter_kernel(const int *K, long long n, unsigned *cursor, int *order) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned slot = atomicAdd(&cursor[K[i] + 1], 1u);
+    order[slot] = (int)i;
+}
+
 // FP32 far records: far points relative to the centroid, area, radius.
 __global__ void far32_records(GeomDev g, float4 *rec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -666,7 +684,35 @@ struct pq_fill {
     int cus = 256;
 };
 
+// ---- the allocation-free explicit pair entry (pq_pairs_noalloc) -------------------------------
+// Its staging buffers live in this file-scope device array instead of device memory allocated per
+// call: p, q, kind and the kind order (4 x n int32), d (n FP64), the moments (n x 6 FP64) and four
+// bucket cursors, each part 256-byte aligned. Every device the library runs on holds its own copy
+// until the process ends. Host code must not pass the array's name to a kernel or a copy: its
+// device address comes from hipGetSymbolAddress, taken once per device (at pq_fill_create, with the
+// device current) and kept in scratch_ptr.
+constexpr long long PAIRS_NOALLOC_MAX = 1LL << 20;
+constexpr size_t PAIRS_SCRATCH_BYTES = (size_t)PAIRS_NOALLOC_MAX * (4 * 4 + 8 + 6 * 8) + 8 * 256;
+static __device__ __align__(256) unsigned char scratch[PAIRS_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The device address of `scratch` on the current device `device` (taken once per device).
+void *scratch_setup(int device) {
+    PQ_REQUIRE(device >= 0 && device < SCRATCH_DEVICES, "pq_pairs_noalloc: device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        void *p = nullptr;
+        PQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PQ_REQUIRE(p, "pq_pairs_noalloc: no device address for the scratch array");
+        scratch_ptr[device] = p;
+    }
+    return scratch_ptr[device];
+}
 
 size_t arr_bytes(long long n) { return (size_t)n * sizeof(double); }
 
@@ -751,6 +797,7 @@ int pq_fill_create(const pq_geometry *g, int device, pq_fill **out) {
             throw Error(PQ_EDEVICE, "no HIP device available");
         PQ_REQUIRE(device >= 0 && device < ndev, "pq_fill_create: invalid device ordinal");
         DeviceScope scope(device);
+        scratch_setup(device);                         // once per device, with the device current
         auto *f = new pq_fill();
         try {
             f->device = device;
@@ -988,6 +1035,49 @@ int pq_pairs(pq_fill *f, const int32_t *p, const int32_t *q, const double *d, co
         PQ_HIP(hipMemcpyAsync(out, dout, (size_t)n * width * 8, hipMemcpyDeviceToHost, s));
         PQ_HIP(hipStreamSynchronize(s));
         buf.release();
+    });
+}
+
+int pq_pairs_noalloc(pq_fill *f, const int32_t *p, const int32_t *q, const double *d, const int32_t *kind, int64_t n,
+                     int scalar_only, double *out) {
+    return guard([&] {
+        PQ_REQUIRE(f && (n == 0 || (p && q && d && kind && out)), "pq_pairs_noalloc: null argument");
+        PQ_REQUIRE(n >= 0, "pq_pairs_noalloc: negative pair count");
+        PQ_REQUIRE(n <= PAIRS_NOALLOC_MAX, "pq_pairs_noalloc: at most 1048576 pairs per call");
+        if (n == 0) return;
+        long long bucket[4] = {0, 0, 0, 0};
+        for (int64_t i = 0; i < n; ++i) {
+            PQ_REQUIRE(p[i] >= 0 && p[i] < f->nt && q[i] >= 0 && q[i] < f->nt && kind[i] >= -1 && kind[i] <= 2 &&
+                           std::isfinite(d[i]),
+                       "pq_pairs_noalloc: invalid triangle index, kind or offset");
+            if (kind[i] >= 0)
+                PQ_REQUIRE(f->g.nq[kind[i]] == 6 * (1 << (2 * kind[i])),
+                           "pair kinds must select far (-1) or a prepared analytic level");
+            ++bucket[kind[i] + 1];
+        }
+        DeviceScope scope(f->device);
+        char *base = (char *)scratch_setup(f->device);
+        auto carve = [&](size_t sz) { char *r = base; base += (sz + 255) / 256 * 256; return r; };
+        const int width = scalar_only ? 1 : 6;
+        int *dp = (int *)carve(n * 4), *dq = (int *)carve(n * 4), *dk = (int *)carve(n * 4),
+            *dord = (int *)carve(n * 4);
+        double *dd = (double *)carve(n * 8), *dout = (double *)carve(n * 6 * 8);
+        unsigned *cursor = (unsigned *)carve(4 * sizeof(unsigned));
+        hipStream_t s = f->stream;
+        PQ_HIP(hipMemcpyAsync(dp, p, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dq, q, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dk, kind, n * 4, hipMemcpyHostToDevice, s));
+        PQ_HIP(hipMemcpyAsync(dd, d, n * 8, hipMemcpyHostToDevice, s));
+        // far pairs first, then each analytic level (the buckets pq_pairs sorts on the host)
+        const unsigned s1 = (unsigned)bucket[0], s2 = s1 + (unsigned)bucket[1], s3 = s2 + (unsigned)bucket[2];
+        hipLaunchKernelGGL(bucket_init_kernel, dim3(1), dim3(64), 0, s, cursor, 0u, s1, s2, s3);
+        const dim3 grid((unsigned)((n + 127) / 128));
+        hipLaunchKernelGGL(bucket_scatter_kernel, grid, dim3(128), 0, s, dk, (long long)n, cursor, dord);
+        if (scalar_only) hipLaunchKernelGGL(pairs_kernel<true>, grid, dim3(128), 0, s, f->g, dp, dq, dd, dk, dord, (long long)n, dout);
+        else hipLaunchKernelGGL(pairs_kernel<false>, grid, dim3(128), 0, s, f->g, dp, dq, dd, dk, dord, (long long)n, dout);
+        PQ_HIP(hipGetLastError());
+        PQ_HIP(hipMemcpyAsync(out, dout, (size_t)n * width * 8, hipMemcpyDeviceToHost, s));
+        PQ_HIP(hipStreamSynchronize(s));
     });
 }
 

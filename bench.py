@@ -1,20 +1,27 @@
 """Benchmark: fill Gpair/s and the port GMRES solve on the 30k-RWG seeded plane pair.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed S] [--no-cpu] [--no-gmres]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed S] [--no-cpu] [--no-gmres] [--sweep BOARDS]
 
-A step is one device-resident fill of the board's four moment planes (FP32 far pairs,
-FP64 analytic near pairs) with the inputs already in HBM; value = n(n+1)/2 / seconds / 1e9
-(the reference's pair count, src/pcbqs/static.py:39). The dominant kernel's duration comes
-from HIP events recorded on its launch stream. The CPU baseline is the reference's own
-compiled static_fill (built into oracle/_ref) on the same mesh, on this host's cores.
+A step is one device-resident fill of the board's four moment planes (FP32 far pairs, FP64
+analytic near pairs) with the inputs already in HBM; value = n(n+1)/2 / seconds / 1e9 (the
+reference's pair count, src/pcbqs/static.py:39). The same loop over the plain FP64 fill (the
+reference's precision and layout) is reported beside it (`fp64_fill`). The dominant kernel's
+duration comes from HIP events recorded on its launch stream.
 
-The GMRES part solves the board's 4 ports at 1 MHz to a relative residual of 1e-8 with plain
-block-Jacobi GMRES and with the learned two-level preconditioner (seconds, iterations, ratio),
-reports the direct solve at the same frequency beside them (FP32 L D L^T on the device refined to
-1e-12: factorisation time and rate, solve time) and at eight frequencies in one call (factored
-and refined together), and times the plain path's operator product on the host cores.
+The GMRES part solves the board's 4 ports at 1 MHz to a relative residual of 1e-8 (the FP64
+system's true residual) with plain block-Jacobi GMRES, with two-level GMRES whose coarse space is
+the stream-function hats alone, and with the hats plus the learned enrichment; it reports each
+method's set-up and an end-to-end figure (board set-up + the method's set-up + a 20-frequency
+sweep), and the direct solve beside them.
+
+The CPU leg (skipped with --no-cpu) times the oracle's FP64 fill (oracle/cpu_ref.c, bitwise the
+reference core) on this host's cores, one warm-up and the median of three runs (cpu_baseline), and
+the plain-preconditioned GMRES restated in numpy on the same FP64 system (cpu_gmres); it then uses
+the oracle as the checker: every fill's per-plane error and every port solve's R and L error
+against the oracle's FP64 chain (fill, combine, C^T L C, LU).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -27,13 +34,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # B/s, MI355X
 FP32_VALU_PEAK = 157.3e12  # flop/s, MI355X (vendor)
+FP64_VALU_PEAK = 78.6e12
 F_FULL = 209               # flop per far pair, full moments
-B_FULL_FP32 = 32           # bytes stored per unordered pair: 4 planes x 2 entries x 4 B
+PLANES = ('M0', 'Mrx', 'Mry', 'Mrr')
+FREQ = 1e6
+SWEEP_FREQS = np.logspace(3, 8, 20)
 
 
 def measured_traffic(seed):
     """HBM bytes per launch of the far kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    passes over this workload (profiles/far32_tile_c2_30k_traffic.json, seed 0), else None."""
+    passes over this workload (profiles/far32_tile_c2_30k_traffic.json: seed 0 only), else None."""
     path = os.path.join(ROOT, 'profiles', 'far32_tile_c2_30k_traffic.json')
     if seed != 0 or not os.path.exists(path):
         return None
@@ -41,110 +51,174 @@ def measured_traffic(seed):
         return json.load(f)['traffic_bytes']
 
 
-def cpu_baseline(board):
-    """Time the reference's compiled static_fill (oracle/_ref) on the whole board, or, where it
-    is not built, the C restatement (oracle/libcpu_ref.so), on this host's cores."""
-    from oracle import fill as F
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except OSError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    g = F.prepare(board.nodes, board.tris, board.z_of, 1)
-    n = len(board.tris)
-    mats = [np.zeros((n, n)) for _ in range(4)]              # touched before the clock starts
-    core = F.ref_core()
+def sync():
+    from pcbqs_amd._lib import call
+    call('pq_synchronize', 0)
+
+
+# ---- fills -------------------------------------------------------------------------------------
+
+def timed_fills(backend, precision, steps, warmup):
+    """Step-timed device fills of one precision: (planes, dict of rates and kernel times)."""
+    n = backend.nt
+    pairs = n * (n + 1) // 2
+    planes, analytic = backend.fill_device(3.0, 1, False, precision)
+    for _ in range(warmup):
+        backend.fill_device(3.0, 1, False, precision, out=planes)
+    sync()
+    far_ms, near_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        backend.fill_device(3.0, 1, False, precision, out=planes)
+        f, nm, near_pairs = backend.last_timing()
+        far_ms.append(f)
+        near_ms.append(nm)
+    sync()
+    seconds = (time.perf_counter() - t0) / steps
+    far = float(np.mean(far_ms)) * 1e-3
+    stored = pairs * 8 * (precision // 8)            # 4 planes x 2 reciprocal entries
+    return planes, dict(gpair_s=pairs / seconds / 1e9, ms_per_step=seconds * 1e3, far_ms=far * 1e3,
+                        near_ms=float(np.mean(near_ms)), near_pairs=list(near_pairs), analytic=analytic,
+                        far_store_GBs=stored / far / 1e9, far_hbm_frac=stored / far / HBM_PEAK,
+                        algorithmic_bytes=stored)
+
+
+def store_probe(n, ld):
+    """The store ceiling of the far kernel's output pattern (4 FP32 planes, 64 x 64 tiles, no
+    arithmetic), on a scratch buffer of the planes' shape: bytes/s."""
+    from pcbqs_amd._lib import DeviceArray, call
+    buf = DeviceArray((4, n, ld), np.float32)
+    ms = ctypes.c_double()
+    call('pq_store_probe', buf.ptr, n, ld, 4, ctypes.byref(ms))
+    buf.close()
+    return 16.0 * n * n / (ms.value * 1e-3)
+
+
+def drop_in_fill(backend):
+    """The plugin's fill_symmetric (what `--fill-backend mi355x` callers get): four host FP64 (n, n)
+    matrices, so its rate includes the device-to-host copy of 8 B x 4 x n^2."""
+    n = backend.nt
+    mats = [np.zeros((n, n)) for _ in range(4)]              # pages touched before the clock
     t = time.perf_counter()
-    if core is not None:
-        count, analytic = core.static_fill(F.ref_geometry_dict(g), g.cent, g.radius, g.z, 3.0, 1, False,
-                                           cores, mats)
-        kind = 'reference'
-    else:
-        _, count, analytic = F.cr_fill(g, 3.0, 1, False, cores)
-        kind = 'port'
+    backend.fill_symmetric(mats, 3.0, 1, False)
     seconds = time.perf_counter() - t
     del mats
-    return dict(value=count / seconds / 1e9, unit='Gpair/s', cores=cores, kind=kind,
-                sample=f'the whole fill of the same {n}-triangle board: static_fill, {count:,} unordered '
-                       f'pairs ({analytic:,} analytic), FP64, near=3, outer_levels=1, one run',
-                seconds=seconds)
+    pairs = n * (n + 1) // 2
+    return dict(seconds=seconds, gpair_s_pcie_inclusive=pairs / seconds / 1e9, host_bytes=32 * n * n,
+                note='one call, FP64 planes into caller-owned host memory (device-to-host copy included)')
 
 
-def gmres_section(board, freq=1e6, grid=36, repeats=3, cpu=True):
-    """Port solve at one frequency on the same board: plain block-Jacobi GMRES against two-level
-    GMRES whose coarse space holds the grid x grid hats and the learned enrichment. Seconds are
-    the solver's own wall clock per solve, the median of the repeats at that frequency; the first
-    repeat also builds the frequency's block-Jacobi inverses and coarse matrices and is reported as
-    first_solve_s. The per-board set-up of each method is reported beside it."""
-    from pcbqs_amd.learned import MPNN, set_learned_coarse
+# ---- port solves -------------------------------------------------------------------------------
+
+def median_solve(solver, repeats, **kw):
+    out = [solver.ports([FREQ], **kw) for _ in range(repeats)]
+    secs = [o[1][0]['seconds'] for o in out]
+    i = int(np.argsort(secs)[len(secs) // 2])
+    return out[i][0][0], dict(out[i][1][0], first_solve_s=secs[0])
+
+
+def sweep_seconds(solver, **kw):
+    t = time.perf_counter()
+    _, info = solver.ports(SWEEP_FREQS, **kw)
+    return time.perf_counter() - t, int(sum(i['iterations'] for i in info))
+
+
+def gmres_section(board, grid=36, repeats=3):
+    """Port solves at FREQ on the board: plain block-Jacobi GMRES, two-level GMRES with the hats
+    alone and with the hats plus the learned enrichment, and the direct solve. Seconds per solve are
+    the solver's own clock, the median of the repeats (the first repeat also builds the frequency's
+    block-Jacobi inverses and coarse matrices: first_solve_s). end_to_end: board set-up + the
+    method's set-up + a 20-frequency sweep (1 kHz - 100 MHz). Returns (section, Z per method,
+    solver)."""
+    from pcbqs_amd.learned import MPNN, BranchGraph, enrichment
     from pcbqs_amd.solve import PortSolver
     t = time.perf_counter()
     solver = PortSolver(board, device=0)
     board_setup = time.perf_counter() - t
+    Z = {}
+    Z['plain'], plain = median_solve(solver, repeats, precond='block-jacobi', tol=1e-8)
+    plain_sweep = sweep_seconds(solver, precond='block-jacobi', tol=1e-8)
+    # hats alone
+    t = time.perf_counter()
+    solver.set_stream_coarse(grid)
+    hats_setup = time.perf_counter() - t
+    hats_k = solver.coarse_k
+    Z['hats'], hats = median_solve(solver, repeats, precond='two-level', tol=1e-8)
+    hats_sweep = sweep_seconds(solver, precond='two-level', tol=1e-8)
+    # hats + learned enrichment
+    t = time.perf_counter()
+    net = MPNN.load()
+    tn = time.perf_counter()
+    extra = enrichment(net, BranchGraph(solver.model))
+    net_s = time.perf_counter() - tn
+    solver.set_stream_coarse(grid, extra_loops=extra)
+    learned_setup = time.perf_counter() - t
+    Z['learned'], learned = median_solve(solver, repeats, precond='two-level', tol=1e-8)
+    learned_sweep = sweep_seconds(solver, precond='two-level', tol=1e-8)
 
-    def run(precond):
-        out = [solver.ports([freq], precond=precond, tol=1e-8) for _ in range(repeats)]
-        secs = [o[1][0]['seconds'] for o in out]
-        i = int(np.argsort(secs)[len(secs) // 2])
-        return out[i][0][0], dict(out[i][1][0], first_solve_s=secs[0])
-    Zp, plain = run('block-jacobi')
+    def method(name, info, setup, sw):
+        return dict(preconditioner=name, seconds=info['seconds'], iterations=info['iterations'],
+                    relative_residual=info['relative_residual'], first_solve_s=info['first_solve_s'],
+                    setup_s=setup, end_to_end_s=board_setup + setup + sw[0], sweep_s=sw[0],
+                    sweep_iterations=sw[1])
+    section = dict(
+        frequency_Hz=FREQ, ports=solver.n_ports, free_loops=solver.model.n_free, tol=1e-8,
+        residual='true relative residual of the FP64 system (Ll kept in FP64; FP32 copy for the Krylov products)',
+        board_setup_s=board_setup, sweep_frequencies=len(SWEEP_FREQS),
+        plain=method('block-jacobi (256-loop blocks)', plain, 0.0, plain_sweep),
+        hats=method(f'two-level: {grid}x{grid} hats + via loops', hats, hats_setup, hats_sweep),
+        learned=method(f'two-level: {grid}x{grid} hats + via loops + learned enrichment', learned, learned_setup,
+                       learned_sweep),
+        coarse_columns=dict(hats=hats_k, learned=solver.coarse_k))
+    section['learned']['network_s'] = net_s
+    section['ratio'] = learned['seconds'] / plain['seconds']
+    section['ratio_hats'] = hats['seconds'] / plain['seconds']
+    section['ratio_end_to_end'] = section['learned']['end_to_end_s'] / section['plain']['end_to_end_s']
     # the direct solve at the same frequency: the first call forms and factors Z_ff (FP32
     # L D L^T, n^3 / 6 complex multiply-adds = 4 n^3 / 3 flop), the repeats reuse the factor
-    dout = [solver.ports([freq], solver='direct') for _ in range(repeats)]
+    dout = [solver.ports([FREQ], solver='direct') for _ in range(repeats)]
     dinfo = [o[1][0] for o in dout]
+    Z['direct'] = dout[0][0][0]
     nf = solver.model.n_free
     factor_ms = dinfo[0]['factor_ms']
     direct = dict(
         method='FP32 L D L^T of Rl + j omega Ll (64-wide panels, MFMA trailing updates, no pivoting), '
-               'FP64 refinement to 1e-12',
-        first_solve_s=dinfo[0]['seconds'], seconds=float(np.median([d['seconds'] for d in dinfo[1:]]))
-        if repeats > 1 else dinfo[0]['seconds'], factor_ms=factor_ms, corrections=dinfo[0]['iterations'],
-        relative_residual=dinfo[0]['relative_residual'],
+               'refined against the FP64 system to 1e-12 (FP64 LU fallback where that stalls)',
+        first_solve_s=dinfo[0]['seconds'], seconds=float(np.median([d['seconds'] for d in dinfo[1:]])),
+        factor_ms=factor_ms, corrections=dinfo[0]['iterations'], relative_residual=dinfo[0]['relative_residual'],
+        fp64_fallback=dinfo[0]['fp64_fallback'],
         factor_tflops=4 * nf ** 3 / 3 / (factor_ms * 1e-3) / 1e12 if factor_ms > 0 else None,
-        fp32_mfma_peak_tflops=157.3,
-        port_z_rel_diff_vs_plain=float(np.abs(dout[0][0][0] - Zp).max() / np.abs(Zp).max()))
-    # eight frequencies (10 kHz .. 100 MHz) in one call: factored and refined together
-    sweep_f = np.logspace(4, 8, 8)
+        fp32_mfma_peak_tflops=157.3)
     t = time.perf_counter()
-    _, binfo = solver.ports(sweep_f, solver='direct')
+    _, binfo = solver.ports(SWEEP_FREQS, solver='direct')
     bsec = time.perf_counter() - t
     bms = binfo[0]['factor_ms']
-    direct['batched'] = dict(frequencies=len(sweep_f), seconds=bsec, seconds_per_frequency=bsec / len(sweep_f),
-                             factor_ms_per_frequency=bms,
+    direct['batched'] = dict(frequencies=len(SWEEP_FREQS), seconds=bsec, seconds_per_frequency=bsec / len(SWEEP_FREQS),
+                             end_to_end_s=board_setup + bsec, factor_ms_per_frequency=bms,
                              factor_tflops=4 * nf ** 3 / 3 / (bms * 1e-3) / 1e12 if bms > 0 else None,
                              worst_relative_residual=max(r['relative_residual'] for r in binfo))
-    t = time.perf_counter()
-    net = MPNN.load()
-    net_s = set_learned_coarse(solver, net, grid)
-    coarse_setup = time.perf_counter() - t
-    Zl, learned = run('two-level')
-    section = dict(
-        frequency_Hz=freq, ports=solver.n_ports, free_loops=solver.model.n_free, tol=1e-8,
-        plain=dict(preconditioner='block-jacobi (256-loop blocks)', seconds=plain['seconds'],
-                   iterations=plain['iterations'], relative_residual=plain['relative_residual'],
-                   first_solve_s=plain['first_solve_s']),
-        learned=dict(preconditioner=f'two-level: {grid}x{grid} hats + via loops + learned enrichment',
-                     seconds=learned['seconds'], iterations=learned['iterations'],
-                     first_solve_s=learned['first_solve_s'],
-                     relative_residual=learned['relative_residual'], coarse_columns=solver.coarse_k,
-                     board_setup_s=coarse_setup, network_s=net_s),
-        ratio=learned['seconds'] / plain['seconds'],
-        port_z_rel_diff=float(np.abs(Zl - Zp).max() / np.abs(Zp).max()),
-        board_setup_s=board_setup, direct=direct)
+    section['direct'] = direct
     chunk, us = solver.tuning()
     n = solver.model.n_free
     section['operator_product'] = dict(
-        kernel='operator product (autotuned panel_kernel or sym_panel_kernel + operator_epilogue_kernel: (Rl + j omega Ll) X, Ll FP32, 8 real FP64 columns)', autotuned_chunk=chunk,
+        kernel='operator product (autotuned panel_kernel or sym_panel_kernel + operator_epilogue_kernel: '
+               '(Rl + j omega Ll) X, Ll FP32, 8 real FP64 columns)', autotuned_chunk=chunk,
         microseconds=us, timed='the whole product: Ll X chunk partials, the resistive CSR part and the epilogue',
         algorithmic_bytes=4 * n * n, achieved_GBs=4 * n * n / (us * 1e-6) / 1e9,
         peak_GBs=HBM_PEAK / 1e9, frac=4 * n * n / (us * 1e-6) / HBM_PEAK if us > 0 else None)
-    if cpu:
-        L, R = host_loop_matrices(solver)
-        section['cpu_operator'] = cpu_operator_sample(solver, plain['iterations'], L=L, R=R)
-        section['direct']['cpu'] = cpu_direct_sample(L, R, 2 * np.pi * freq)
-        del L, R
-    solver.close()
-    return section
+    return section, Z, solver
+
+
+def fp64_pipeline_ports(board):
+    """Config 1's plain FP64 method on this board: FP64 planes, FP64 branch inductance and loop
+    matrix, the direct solve at FREQ: (Z, seconds of set-up + solve, info)."""
+    from pcbqs_amd.solve import PortSolver
+    t = time.perf_counter()
+    s = PortSolver(board, device=0, precision=64)
+    Z, info = s.ports([FREQ], solver='direct')
+    seconds = time.perf_counter() - t
+    s.close()
+    return Z[0], seconds, info[0]
 
 
 def sweep_section(n_boards, n_freqs=20, concurrency=1):
@@ -160,54 +234,140 @@ def sweep_section(n_boards, n_freqs=20, concurrency=1):
     return total
 
 
-def host_loop_matrices(solver):
-    """The free-loop block of the device's FP64 Ll and Rl (CSR) on the host."""
+# ---- the CPU leg: baselines on the host cores, then the oracle as the checker -------------------
+
+def host_cpus():
+    """(CPUs in the affinity mask, the cgroup CPU quota or None, the CPU model)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except OSError:
+        affinity = os.cpu_count() or 1
+    quota = None
+    try:
+        with open('/sys/fs/cgroup/cpu.max') as f:
+            q, period = f.read().split()[:2]
+        if q != 'max':
+            quota = max(1, int(round(int(q) / int(period))))
+    except (OSError, ValueError):
+        pass
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return affinity, quota, model
+
+
+def cpu_fill_baseline(board):
+    """The oracle's FP64 fill of the whole board (oracle/cpu_ref.c, the reference core's pair code,
+    64 x 64 dynamic OpenMP tiles) on every CPU of the affinity mask, and also at the cgroup quota's
+    CPU count where that is smaller: one warm-up and the median of three runs each; the faster
+    thread count is the baseline. Returns (baseline dict, the last run's planes)."""
+    from oracle import fill as F
+    affinity, quota, model = host_cpus()
+    g = F.prepare(board.nodes, board.tris, board.z_of, 1)
+    n = len(board.tris)
+    counts = sorted({affinity} | ({quota} if quota and quota < affinity else set()))
+    runs, planes = {}, None
+    for threads in counts:
+        F.cr_fill(g, 3.0, 1, False, threads)                     # warm-up
+        secs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            planes, count, analytic = F.cr_fill(g, 3.0, 1, False, threads)
+            secs.append(time.perf_counter() - t)
+        runs[threads] = float(np.median(secs))
+    best = min(runs, key=runs.get)
+    out = dict(value=count / runs[best] / 1e9, unit='Gpair/s', cores=best, kind='port',
+               sample=f'the whole fill of the same {n}-triangle board: {count:,} unordered pairs ({analytic:,} '
+                      f'analytic), FP64, near=3, outer_levels=1; one warm-up, median of 3 runs',
+               seconds=runs[best], cpu_model=model, affinity_cpus=affinity, cgroup_quota_cpus=quota,
+               median_seconds_by_threads={str(k): v for k, v in runs.items()},
+               binary='oracle/libcpu_ref.so, built from the tracked oracle/cpu_ref.c (bitwise equal to the '
+                      'reference core: tests/test_oracle.py)')
+    core = F.ref_core()
+    if core is not None:                                          # calibration against the reference core
+        t = time.perf_counter()
+        F.ref_fill(core, g, 3.0, 1, False, best)
+        out['reference_core_seconds'] = time.perf_counter() - t
+    return out, planes
+
+
+def plane_errors(dev_planes, ref_planes, local_cent=None, block=2048):
+    """Per-plane relative Frobenius errors of device planes (4, n, ld) against the oracle's (n, n)
+    FP64 planes; FP32 local-frame planes (local_cent given) are first taken to the reference layout."""
+    n = ref_planes[0].shape[0]
+    host = dev_planes.host()[:, :, :n]
+    num, den = np.zeros(4), np.zeros(4)
+    for r0 in range(0, n, block):
+        r1 = min(n, r0 + block)
+        b = [host[k][r0:r1].astype(np.float64) for k in range(4)]
+        if local_cent is not None:
+            cx, cy = local_cent[:, 0], local_cent[:, 1]
+            ax, ay = cx[r0:r1, None], cy[r0:r1, None]
+            mxt = host[1][:, r0:r1].T.astype(np.float64)
+            myt = host[2][:, r0:r1].T.astype(np.float64)
+            b = [b[0], b[1] + ax * b[0], b[2] + ay * b[0],
+                 b[3] + ax * mxt + ay * myt + cx[None, :] * b[1] + cy[None, :] * b[2] +
+                 (ax * cx[None, :] + ay * cy[None, :]) * b[0]]
+        for k in range(4):
+            ref = ref_planes[k][r0:r1]
+            num[k] += np.sum((b[k] - ref) ** 2)
+            den[k] += np.sum(ref ** 2)
+    del host
+    return dict(zip(PLANES, np.sqrt(num / den).tolist()))
+
+
+def oracle_ports(board, ref_planes, freq):
+    """The oracle's FP64 dense chain from its own fill planes: port Z (P, P) at freq."""
+    from oracle import fill as F
+    from oracle import mqs as M
+    q = M.CopperModel(board.nodes, board.tris, board.z_of, board.t_of, board.node_of, board.terminals, board.vseg,
+                      board.via_nodes, board.ports)
+    g = F.prepare(board.nodes, board.tris, board.z_of, 1)
+    q.assemble((g, *ref_planes))
+    return q.port_response([freq])[0][0]
+
+
+def port_errors(Z, Zo, freq):
+    w = 2 * np.pi * freq
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    return dict(port_R=rel(Z.real, Zo.real), port_L=rel(Z.imag / w, Zo.imag / w))
+
+
+def cpu_gmres(solver, iterations_gpu):
+    """The plain-preconditioned solve restated in numpy (oracle/gmres.py: lockstep GMRES(100) of
+    mqs_iterative._lockstep_gmres, block-Jacobi on 256-loop blocks, tol 1e-8, true-residual
+    acceptance of peec_iterative._gmres) on the same FP64 system the device solves, all 4 ports,
+    timed once on the host cores; and the reference's dense method's FP32 LU of the same block."""
+    from oracle import gmres as G
     n = solver.model.n_free
     L = np.ascontiguousarray(solver.loop_matrix(n, precision=64)[:, :n])
-    return L, solver.model.Rl.tocsr()[:n, :n]
-
-
-def host_cores():
-    try:
-        return len(os.sched_getaffinity(0))
-    except OSError:
-        return os.cpu_count() or 1
-
-
-def cpu_direct_sample(L, R, omega):
-    """The reference's dense path on the host: LAPACK's single-precision complex LU (scipy's
-    lu_factor, as MixedLU, src/pcbqs/kernels.py:143-148) of the same Z_ff, timed once."""
+    R = solver.model.Rl.tocsr()[:n, :n]
+    B = solver.rhs(FREQ)
+    omega = 2 * np.pi * FREQ
+    t = time.perf_counter()
+    X, its, res = G.plain_solve(L, R, B, omega, tol=1e-8, restart=100)
+    seconds = time.perf_counter() - t
+    affinity, quota, model = host_cpus()
+    out = dict(seconds=seconds, iterations=its, relative_residual=res, gpu_iterations=iterations_gpu,
+               cores=affinity, cpu_model=model, kind='port',
+               sample=f'the whole plain solve of the {n}-loop system at {FREQ:g} Hz, {B.shape[1]} ports in lockstep, '
+                      f'numpy FP64 (BLAS threads on the host cores)')
     from scipy.linalg import lu_factor
-    n = L.shape[0]
     Z = np.asarray(R.todense(), np.complex64)
     Z += np.complex64(1j * omega) * L.astype(np.float32)
+    del L
     t = time.perf_counter()
     lu_factor(Z.T, overwrite_a=True, check_finite=False)
-    seconds = time.perf_counter() - t
-    return dict(factor_s=seconds, cores=host_cores(), kind='reference method (LAPACK cgetrf through scipy)',
-                sample=f'one FP32 complex LU of the {n} x {n} free-loop block at {omega / 2 / np.pi:g} Hz '
-                       f'(the refinement, a few solves and products, is not timed)')
-
-
-def cpu_operator_sample(solver, iterations, products=5, L=None, R=None):
-    """The plain path's operator product (Z_ff on 2P real columns, FP64 numpy on the host cores)
-    timed over a few products and scaled to the plain solve's iteration count: a bounded sample of
-    the CPU GMRES, whose time is ~98% operator products."""
-    n, P = solver.model.n_free, solver.n_ports
-    if L is None:
-        L, R = host_loop_matrices(solver)
-    X = np.random.default_rng(0).normal(size=(n, 2 * P))
-    omega = 2 * np.pi * 1e6
-    L @ X
-    t = time.perf_counter()
-    for _ in range(products):
-        Y = R @ X
-        Y[:, :P] -= omega * (L @ X[:, P:])
-        Y[:, P:] += omega * (L @ X[:, :P])
-    per = (time.perf_counter() - t) / products
-    return dict(seconds_per_product=per, estimated_solve_s=per * iterations, cores=host_cores(), kind='port',
-                sample=f'{products} FP64 operator products (n={n}, {2 * P} real columns) with numpy, '
-                       f'scaled to the {iterations} iterations of the plain solve')
+    out['reference_method_factor_s'] = time.perf_counter() - t
+    out['reference_method'] = ("MixedLU's FP32 complex LU (LAPACK cgetrf through scipy) of the same block, "
+                               'timed once (its refinement is not timed)')
+    return out
 
 
 def main():
@@ -242,55 +402,59 @@ def main():
     pairs = n * (n + 1) // 2
     g = FillGeometry.prepare(board.nodes, board.tris, board.z_of, 1)
     backend = Mi355xBackend(g, 1, 16384, 0)
-    planes, analytic = backend.fill_device(3.0, 1, False, 32)
-    for _ in range(args.warmup):
-        backend.fill_device(3.0, 1, False, 32, out=planes)
-    far_ms, near_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        backend.fill_device(3.0, 1, False, 32, out=planes)
-        f, nm, near_pairs = backend.last_timing()
-        far_ms.append(f)
-        near_ms.append(nm)
-    seconds = (time.perf_counter() - t0) / args.steps
-    far = float(np.mean(far_ms)) * 1e-3
-    # the store ceiling of the same output pattern (the four planes' 64 x 64 tiles, no arithmetic)
-    import ctypes
-    from pcbqs_amd._lib import call
-    probe_ms = ctypes.c_double()
-    call('pq_store_probe', planes.ptr, n, planes.shape[-1], 4, ctypes.byref(probe_ms))
-    store_probe = 16.0 * n * n / (probe_ms.value * 1e-3)
-    # FP64 reference-layout fill (the parity mode), once, for the record
-    planes.close()
-    p64, _ = backend.fill_device(3.0, 1, False, 64)
-    backend.fill_device(3.0, 1, False, 64, out=p64)
-    f64, n64, _ = backend.last_timing()
-    p64.close()
-    achieved = pairs * B_FULL_FP32 / far
+    planes32, f32 = timed_fills(backend, 32, args.steps, args.warmup)
+    probe = store_probe(n, planes32.shape[-1])
+    planes64, f64 = timed_fills(backend, 64, args.steps, args.warmup)
+    far = f32['far_ms'] * 1e-3
+    achieved = pairs * 32 / far
     line = dict(
         metric='fill Gpair/s; GMRES s to 1e-8 at 30k RWG, learned vs plain precond',
-        value=pairs / seconds / 1e9, unit='Gpair/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=seconds * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
+        value=f32['gpair_s'], unit='Gpair/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=f32['ms_per_step'], higher_is_better=True, scaling='weak', vs_baseline=None,
         dtype='fp32 far / fp64 near', data='synthetic',
         config=dict(workload=f'seeded 2-layer 60x60 mm plane pair (seed {args.seed}), {n} triangles, '
                              f'{board.info.get("cutouts")} cutouts, {len(board.vseg)} vias, 4 ports; '
-                             f'near=3, outer_levels=1', triangles=n, pairs=pairs, analytic_pairs=analytic),
+                             f'near=3, outer_levels=1', triangles=n, pairs=pairs, analytic_pairs=f32['analytic']),
         roofline=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                       frac=achieved / HBM_PEAK, traffic=measured_traffic(args.seed),
-                      algorithmic_bytes=pairs * B_FULL_FP32,
+                      traffic_source='rocprofv3 FETCH_SIZE + WRITE_SIZE of the far kernel on this workload (seed 0), '
+                                     'scaled by a copy kernel of known size: profiles/far32_tile_c2_30k_traffic.json',
+                      algorithmic_bytes=pairs * 32,
                       kernel='far32_tile_kernel (FP32 far pairs, full moments, both reciprocal entries)',
                       valu_frac=pairs * F_FULL / far / FP32_VALU_PEAK,
-                      store_probe_GBs=store_probe / 1e9, frac_of_store_probe=achieved / store_probe),
-        kernels=dict(far_ms=float(np.mean(far_ms)), near_ms=float(np.mean(near_ms)),
-                     near_pairs=list(near_pairs), fp64_far_ms=f64, fp64_near_ms=n64,
-                     fp64_gpair_s=pairs / ((f64 + n64) * 1e-3) / 1e9),
+                      store_probe_GBs=probe / 1e9, frac_of_store_probe=achieved / probe),
+        kernels=dict(far_ms=f32['far_ms'], near_ms=f32['near_ms'], near_pairs=f32['near_pairs']),
+        fp64_fill=dict(f64, dtype='fp64 (the reference precision and layout)',
+                       kernel='far64 kernel + FP64 near kernel; far stores 64 B per unordered pair',
+                       valu_frac_fp64=pairs * F_FULL / (f64['far_ms'] * 1e-3) / FP64_VALU_PEAK),
     )
+    line['drop_in'] = drop_in_fill(backend)
+    backend.close()
+    Z, solver = None, None
     if not args.no_gmres:
-        line['gmres'] = gmres_section(board, cpu=not args.no_cpu)
+        line['gmres'], Z, solver = gmres_section(board)
+        Z['fp64_pipeline'], s64, i64 = fp64_pipeline_ports(board)
+        line['gmres']['fp64_pipeline'] = dict(method='FP64 planes, FP64 Lb and Ll, direct solve', seconds=s64,
+                                              relative_residual=i64['relative_residual'])
     if args.sweep:
         line['sweep'] = sweep_section(args.sweep, concurrency=args.sweep_concurrency)
     if not args.no_cpu:
-        line['cpu_baseline'] = cpu_baseline(board)
+        line['cpu_baseline'], ref_planes = cpu_fill_baseline(board)
+        line['fill_rel_err_vs_fp64'] = plane_errors(planes32, ref_planes, local_cent=g.cent)
+        line['fp64_fill_rel_err_vs_oracle'] = plane_errors(planes64, ref_planes)
+        if Z is not None:
+            Zo = oracle_ports(board, ref_planes, FREQ)
+            errs = {k: port_errors(v, Zo, FREQ) for k, v in Z.items()}
+            line['port_L_rel_err_vs_oracle'] = {k: v['port_L'] for k, v in errs.items()}
+            line['port_R_rel_err_vs_oracle'] = {k: v['port_R'] for k, v in errs.items()}
+        del ref_planes
+        if solver is not None:
+            line['cpu_gmres'] = cpu_gmres(solver, line['gmres']['plain']['iterations'])
+            line['cpu_gmres']['cpu_over_gpu_plain'] = line['cpu_gmres']['seconds'] / line['gmres']['plain']['seconds']
+    planes32.close()
+    planes64.close()
+    if solver is not None:
+        solver.close()
     print(json.dumps(line))
 
 

@@ -336,6 +336,15 @@ int pq_loops_coarse_solution(pq_loops *loops, double omega, int P, const int32_t
  * src/pcbqs/mqs_precond.py:27-59). */
 int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi, int diagonal);
 
+/* pq_loops_set_coarse of the span of Phi (n_free, k) FP64 host, re-expressed on
+ * the device in the generalised eigenbasis of its Galerkin pair: Rc = Phi^T Rl Phi
+ * = U S U^T and (S^-1/2 U^T) Lc (U S^-1/2) = Q Lam Q^T (rocSOLVER dsyevd, FP64),
+ * directions with S below drop * max S removed, basis Psi = Phi U S^-1/2 Q with
+ * Psi^T Rl Psi = I and Psi^T Ll Psi = Lam, so each frequency's coarse solve is a
+ * diagonal scaling. *k_out: the kept dimension. No counterpart in the reference
+ * (the host did the same with two eigh calls). */
+int pq_loops_set_coarse_diagonal(pq_loops *loops, int k, const double *Phi, double drop, int *k_out);
+
 /* The Galerkin matrices Phi^T Rl Phi and Phi^T Ll Phi (k x k FP64, row-major)
  * of the current coarse space, for a host-side change of basis. No counterpart
  * in the reference. */

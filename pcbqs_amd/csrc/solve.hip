@@ -1044,6 +1044,60 @@ __global__ void transpose_kernel(const float *In, int n, int k, int ld, float *O
     Out[c * n + i] = In[i * ld + c];
 }
 
+// ---- the coarse space's device diagonalisation ----
+
+// Phi (n, k) FP64 rounded to FP32: Out (n, kp) with zero padding columns, and Phi overwritten by the
+// rounded values (the basis every later product and Galerkin matrix uses).
+__global__ void round_basis_kernel(double *Phi, int n, int k, int kp, float *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * kp) return;
+    const size_t i = e / kp;
+    const int c = (int)(e % kp);
+    float v = 0.f;
+    if (c < k) {
+        v = (float)Phi[i * k + c];
+        Phi[i * k + c] = (double)v;
+    }
+    Out[e] = v;
+}
+
+// Out[i][c0 + c] = W[i][c] for c < cn (FP64 (n, ld) chunk -> FP64 (n, k)).
+__global__ void scatter_cols64_kernel(const double *W, int n, int ld, int c0, int cn, int k, double *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * ld) return;
+    const size_t i = e / ld;
+    const int c = (int)(e % ld);
+    if (c < cn) Out[i * k + c0 + c] = W[e];
+}
+
+// A := (A + A^T) / 2 for a k x k matrix (one thread per unordered pair).
+__global__ void symmetrize64_kernel(double *A, int k) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = e / k, j = e % k;
+    if (i >= (size_t)k || j <= i) return;
+    const double v = 0.5 * (A[i * k + j] + A[j * k + i]);
+    A[i * k + j] = v;
+    A[j * k + i] = v;
+}
+
+// W (k, kk) row-major: column j is eigenvector first + j of V (an eigensolver's column-major k x k
+// eigenvectors) divided by the square root of its eigenvalue s[first + j].
+__global__ void whiten_kernel(const double *V, const double *s, int k, int first, int kk, double *W) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)k * kk) return;
+    const size_t i = e / kk;
+    const int j = (int)(e % kk);
+    W[e] = V[(size_t)(first + j) * k + i] / sqrt(s[first + j]);
+}
+
+// Out[c][i] = (float) In[i][c]: FP64 (n, k) -> FP32 transposed (k, n).
+__global__ void transpose_round_kernel(const double *In, int n, int k, float *Out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * k) return;
+    const size_t i = e / k, c = e % k;
+    Out[c * n + i] = (float)In[e];
+}
+
 // W = sum of the nsplit column-chunk partials of panel_kernel (a real product, no frequency).
 template <int NR>
 __global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W) {
@@ -1062,6 +1116,15 @@ __global__ void gather_cols_kernel(const float *Phi, int n, int kp, int c0, int 
     const size_t i = e / ld;
     const int c = (int)(e % ld);
     X[e] = c < cn ? (double)Phi[i * kp + c0 + c] : 0.0;
+}
+
+// X[i][c] = Phi[i][c0 + c] for c < cn, else 0 (FP64 (n, k) -> FP64 (n, ld) chunk).
+__global__ void gather_cols64_kernel(const double *Phi, int n, int k, int c0, int cn, int ld, double *X) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * ld) return;
+    const size_t i = e / ld;
+    const int c = (int)(e % ld);
+    X[e] = c < cn ? Phi[i * k + c0 + c] : 0.0;
 }
 
 // Out[c0 + c][i] = W[i][c] for c < cn (FP64 (n, ld) -> FP32 transposed (k, n)).
@@ -1637,6 +1700,20 @@ void check_graphs(pq_loops *L, int P, int precond, int m) {
     if ((int)L->graphs.size() < m) L->graphs.resize(m, nullptr);
 }
 
+// The relative residual estimate above which GMRES steps use the FP64 loop matrix
+// (PCBQS_AMD_RELAX; 1 or more: never).
+double relax_threshold() {
+    const char *v = std::getenv("PCBQS_AMD_RELAX");
+    const double f = v ? std::atof(v) : 0.01;
+    return f > 0 ? f : 0.01;
+}
+
+double inner_tol_factor() {
+    const char *v = std::getenv("PCBQS_AMD_INNER_TOL");
+    const double f = v ? std::atof(v) : 1.0;
+    return f > 0 && f <= 1 ? f : 1.0;
+}
+
 bool graphs_enabled() {
     const char *v = std::getenv("PCBQS_AMD_GRAPHS");
     return v && v[0] == '1';
@@ -1826,17 +1903,21 @@ __global__ void fallback_unpack_kernel(const double2 *Bc, int n, int P, double *
     X[(size_t)i * 2 * P + P + p] = v.y;
 }
 
-struct Fp64Lu {
+// rocBLAS / rocSOLVER entry points this file uses: the FP64 fallback LU, and the coarse space's
+// device diagonalisation (dgemm, dsyevd).
+struct RocLib {
     decltype(&rocblas_create_handle) create = nullptr;
     decltype(&rocblas_destroy_handle) destroy = nullptr;
     decltype(&rocblas_set_stream) set_stream = nullptr;
     decltype(&rocsolver_zgetrf) getrf = nullptr;
     decltype(&rocsolver_zgetrs) getrs = nullptr;
+    decltype(&rocblas_dgemm) dgemm = nullptr;
+    decltype(&rocsolver_dsyevd) syevd = nullptr;
     std::string error;
 };
 
-const Fp64Lu &fp64_lu() {
-    static Fp64Lu lu;
+const RocLib &roc_lib() {
+    static RocLib lu;
     static std::once_flag once;
     std::call_once(once, [] {
         void *blas = dlopen("librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
@@ -1844,7 +1925,7 @@ const Fp64Lu &fp64_lu() {
         void *solver = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
         if (!solver) solver = dlopen("librocsolver.so", RTLD_NOW | RTLD_LOCAL);
         if (!blas || !solver) {
-            lu.error = std::string("FP64 fallback: cannot load rocBLAS/rocSOLVER: ") + dlerror();
+            lu.error = std::string("cannot load rocBLAS/rocSOLVER: ") + dlerror();
             return;
         }
         lu.create = (decltype(lu.create))dlsym(blas, "rocblas_create_handle");
@@ -1852,8 +1933,10 @@ const Fp64Lu &fp64_lu() {
         lu.set_stream = (decltype(lu.set_stream))dlsym(blas, "rocblas_set_stream");
         lu.getrf = (decltype(lu.getrf))dlsym(solver, "rocsolver_zgetrf");
         lu.getrs = (decltype(lu.getrs))dlsym(solver, "rocsolver_zgetrs");
-        if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs)
-            lu.error = "FP64 fallback: rocBLAS/rocSOLVER entry points missing";
+        lu.dgemm = (decltype(lu.dgemm))dlsym(blas, "rocblas_dgemm");
+        lu.syevd = (decltype(lu.syevd))dlsym(solver, "rocsolver_dsyevd");
+        if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs || !lu.dgemm || !lu.syevd)
+            lu.error = "rocBLAS/rocSOLVER entry points missing";
     });
     if (!lu.error.empty()) throw Error(PQ_EDEVICE, lu.error);
     return lu;
@@ -1861,14 +1944,12 @@ const Fp64Lu &fp64_lu() {
 
 void blas_check(rocblas_status st, const char *what) {
     if (st != rocblas_status_success)
-        throw Error(PQ_EDEVICE, std::string("FP64 fallback: ") + what + " failed (rocblas status " +
-                                    std::to_string((int)st) + ")");
+        throw Error(PQ_EDEVICE, std::string(what) + " failed (rocblas status " + std::to_string((int)st) + ")");
 }
 
-// X (n, 2P) = Z_ff^-1 B at omega by the FP64 LU; the matrix and pivots live only for the call.
-void fallback_solve(pq_loops *L, double omega, int P, const double *B, double *X) {
-    const Fp64Lu &lu = fp64_lu();
-    const int n = L->nf;
+// The handle's rocBLAS handle on its stream (created on first use).
+rocblas_handle blas_handle(pq_loops *L) {
+    const RocLib &lu = roc_lib();
     if (!L->blas) {
         rocblas_handle h = nullptr;
         blas_check(lu.create(&h), "rocblas_create_handle");
@@ -1876,6 +1957,26 @@ void fallback_solve(pq_loops *L, double omega, int P, const double *B, double *X
     }
     rocblas_handle h = (rocblas_handle)L->blas;
     blas_check(lu.set_stream(h, L->stream), "rocblas_set_stream");
+    return h;
+}
+
+// Row-major C (m x n) = op(A) op(B) in FP64, K the inner dimension; lda, ldb, ldc the stored row
+// lengths. (rocBLAS is column-major: a row-major matrix is its transpose there, so C^T = op(B)^T
+// op(A)^T is computed.)
+void gemm_rm(pq_loops *L, bool ta, bool tb, int m, int n, int K, const double *A, int lda, const double *B, int ldb,
+             double *C, int ldc) {
+    const double one = 1.0, zero = 0.0;
+    blas_check(roc_lib().dgemm(blas_handle(L), tb ? rocblas_operation_transpose : rocblas_operation_none,
+                               ta ? rocblas_operation_transpose : rocblas_operation_none, n, m, K, &one, B, ldb, A,
+                               lda, &zero, C, ldc),
+               "rocblas_dgemm");
+}
+
+// X (n, 2P) = Z_ff^-1 B at omega by the FP64 LU; the matrix and pivots live only for the call.
+void fallback_solve(pq_loops *L, double omega, int P, const double *B, double *X) {
+    const RocLib &lu = roc_lib();
+    const int n = L->nf;
+    rocblas_handle h = blas_handle(L);
     DevBuf Z, piv, info, Bc;
     Z.reserve((size_t)n * n * 16);
     piv.reserve((size_t)n * 4);
@@ -1921,6 +2022,85 @@ double relative_residual64(pq_loops *L, const double *B, const double *X, double
         worst = std::isnan(r) || std::isnan(worst) ? std::numeric_limits<double>::quiet_NaN() : std::max(worst, r);
     }
     return worst;
+}
+
+// FP16 copies of the bases for the solve's restriction and prolongation, each scaled by its largest
+// magnitude (the Galerkin matrices used the FP32 bases; PCBQS_AMD_COARSE_FP16=0 keeps the solve on
+// the FP32 bases). The coarse correction is a preconditioner: its rounding moves the iterates, not
+// the solution, which is accepted on the true FP64 residual.
+void make_half_bases(pq_loops *L) {
+    const int n = L->nf, k = L->kc, kp = L->kp;
+    const char *hv = std::getenv("PCBQS_AMD_COARSE_FP16");
+    L->coarse_half = !(hv && hv[0] == '0');
+    if (!L->coarse_half) return;
+    L->Phih.reserve((size_t)n * kp * 2); L->Phith.reserve((size_t)n * k * 2);
+    L->RPhih.reserve((size_t)n * k * 2); L->LPhih.reserve((size_t)n * k * 2);
+    L->hmax.reserve(4 * 4);
+    PQ_HIP(hipMemsetAsync(L->hmax.p, 0, 16, L->stream));
+    const float *src[4] = {L->Phi.as<float>(), L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
+    _Float16 *dst[4] = {L->Phih.as<_Float16>(), L->Phith.as<_Float16>(), L->RPhih.as<_Float16>(),
+                        L->LPhih.as<_Float16>()};
+    for (int q = 0; q < 4; ++q) {
+        const size_t cnt = q == 0 ? (size_t)n * kp : (size_t)n * k;
+        hipLaunchKernelGGL(absmax_kernel, dim3(1024), dim3(256), 0, L->stream, src[q], cnt, L->hmax.as<unsigned>() + q);
+        hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream, src[q], cnt,
+                           L->hmax.as<unsigned>() + q, dst[q]);
+    }
+    PQ_HIP(hipGetLastError());
+    unsigned mx[4];
+    PQ_HIP(hipMemcpyAsync(mx, L->hmax.p, 16, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    for (int q = 0; q < 4; ++q) {
+        float m;
+        std::memcpy(&m, &mx[q], 4);
+        L->hscale[q] = m > 0 ? m : 1.0;
+    }
+}
+
+// Rl X and Ll X (FP64 (n, k) row-major each) of the k columns of X (FP64 (n, k)), 16 columns per
+// operator product.
+void basis_products(pq_loops *L, const double *X64, int k, double *RX, double *LX) {
+    const int n = L->nf;
+    constexpr int CH = 16;
+    tune_gemv(L, CH);
+    const OpShape shape = gemv_shape(L, CH);
+    const int nsplit = operator_splits(n, shape.chunk);
+    DevBuf X, WR, WL, part;
+    X.reserve((size_t)n * CH * 8); WR.reserve((size_t)n * CH * 8); WL.reserve((size_t)n * CH * 8);
+    part.reserve((size_t)nsplit * n * CH * 8);
+    const unsigned vb = (unsigned)(((size_t)n * CH + 255) / 256);
+    for (int c0 = 0; c0 < k; c0 += CH) {
+        const int cn = std::min(CH, k - c0);
+        hipLaunchKernelGGL(gather_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, X64, n, k, c0, cn, CH,
+                           X.as<double>());
+        hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                           L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
+        operator_partials<CH>(L, shape, X.as<double>(), part.as<double>());
+        hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
+                           WL.as<double>());
+        hipLaunchKernelGGL(scatter_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, WR.as<double>(), n, CH, c0, cn, k,
+                           RX);
+        hipLaunchKernelGGL(scatter_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, WL.as<double>(), n, CH, c0, cn, k,
+                           LX);
+    }
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipStreamSynchronize(L->stream));             // the chunk buffers are freed on return
+}
+
+// Eigen-decomposition of a symmetric k x k FP64 matrix A in place on the device (rocSOLVER dsyevd):
+// ascending eigenvalues into s (host), eigenvectors as A's columns (column-major).
+void device_eigh(pq_loops *L, double *A, int k, std::vector<double> &s) {
+    DevBuf D, E, info;
+    D.reserve((size_t)k * 8); E.reserve((size_t)k * 8); info.reserve(4);
+    blas_check(roc_lib().syevd(blas_handle(L), rocblas_evect_original, rocblas_fill_upper, k, A, k, D.as<double>(),
+                               E.as<double>(), info.as<rocblas_int>()),
+               "rocsolver_dsyevd");
+    rocblas_int bad = 0;
+    s.resize(k);
+    PQ_HIP(hipMemcpyAsync(&bad, info.p, 4, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipMemcpyAsync(s.data(), D.p, (size_t)k * 8, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    PQ_REQUIRE(bad == 0, "coarse space: the eigensolver did not converge");
 }
 
 }  // namespace
@@ -2109,7 +2289,7 @@ int pq_loops_destroy(pq_loops *L) {
         if (L->ev_ready) (void)hipEventDestroy(L->ev_ready);
         if (L->ev_done) (void)hipEventDestroy(L->ev_done);
         if (L->aux) (void)hipStreamDestroy(L->aux);
-        if (L->blas) (void)fp64_lu().destroy((rocblas_handle)L->blas);
+        if (L->blas) (void)roc_lib().destroy((rocblas_handle)L->blas);
         if (L->stream) (void)hipStreamDestroy(L->stream);
         delete L;
     });
@@ -2222,7 +2402,15 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 hipLaunchKernelGGL(vec_update_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, Vb,
                                    vs, nv, hdev, sign, n, P, Wv);
             };
-            auto step_body = [&](int j) {
+            // Relaxed products (inexact Krylov: Simoncini & Szyld 2003, Bouras & Fraysse 2005): a
+            // product's error enters the solution weighted by that step's coefficient, which is large
+            // while the residual is large and small once it has fallen. So the steps of a cycle use
+            // the FP64 Ll while the last known residual estimate exceeds `relax` (relative), and the
+            // FP32 copy (half the bytes) after; the true residual at the cycle's end then meets the
+            // FP64 system's tolerance without a refinement cycle.
+            const double relax = relax_threshold();
+            double estimate = 1.0;                 // the latest relative residual estimate (max over ports)
+            auto step_body = [&](int j, bool exact) {
                 double *h1 = Hd + j * hstride, *h2 = h1 + (size_t)(m + 1) * P * 2, *hn = h2 + (size_t)(m + 1) * P * 2;
                 // the host copies, written by the same kernels (pinned, coherent memory)
                 double *g1 = L->Hh + j * hstride, *g2 = g1 + (size_t)(m + 1) * P * 2, *gn = g2 + (size_t)(m + 1) * P * 2;
@@ -2230,7 +2418,8 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 // also giving |w|^2, and the last update normalising it
                 double *Wj = V + (j + 1) * vsz;
                 apply_precond(L, precond, V + j * vsz, Z, P);
-                apply_operator(L, Z, Wj, P);
+                if (exact) apply_operator64(L, Z, Wj, P);
+                else apply_operator(L, Z, Wj, P);
                 dots_to(V, vsz, j + 1, Wj, h1, g1);
                 update_dev(V, vsz, j + 1, h1, 1.0, Wj);
                 dots_to(V, vsz, j + 2, Wj, h2, g2);
@@ -2244,12 +2433,15 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             if (use_graphs) check_graphs(L, P, precond, m);
             auto enqueue_step = [&](int j) {
                 step_slots(j, [](volatile double &v) { v = std::numeric_limits<double>::quiet_NaN(); });
-                if (use_graphs) {
+                const bool exact = estimate > relax;
+                if (exact) {
+                    step_body(j, true);            // FP64 steps run eagerly (the graphs hold FP32 steps)
+                } else if (use_graphs) {
                     if (!L->graphs[j]) {
                         hipGraph_t graph = nullptr;
                         PQ_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
                         try {
-                            step_body(j);
+                            step_body(j, false);
                         } catch (...) {
                             (void)hipStreamEndCapture(L->stream, &graph);
                             if (graph) (void)hipGraphDestroy(graph);
@@ -2262,7 +2454,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                     }
                     PQ_HIP(hipGraphLaunch(L->graphs[j], L->stream));
                 } else {
-                    step_body(j);
+                    step_body(j, false);
                 }
                 ++L->matvecs;
             };
@@ -2284,6 +2476,9 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                     idle = true;
                 }
             };
+            // A cycle stops once its residual estimate reaches PCBQS_AMD_INNER_TOL times tol (1 by
+            // default; the FP64 true residual then decides whether another cycle runs).
+            const double inner_tol = tol * inner_tol_factor();
             int its = 0, restarts = 0;
             std::vector<double> res(P, 1.0);
             std::vector<cplx> H((size_t)(m + 1) * m * P), g((size_t)(m + 1) * P), cs((size_t)m * P), sn((size_t)m * P);
@@ -2291,7 +2486,12 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                 std::vector<double> beta;
                 norms(L, R, P, beta);
                 bool done = true;
-                for (int p = 0; p < P; ++p) { res[p] = beta[p] / bnorm[p]; done = done && res[p] <= tol; }
+                estimate = 0;
+                for (int p = 0; p < P; ++p) {
+                    res[p] = beta[p] / bnorm[p];
+                    done = done && res[p] <= tol;
+                    estimate = std::max(estimate, res[p]);
+                }
                 if (done || its >= maxiter) break;
                 std::vector<double> inv_beta(P);
                 for (int p = 0; p < P; ++p) inv_beta[p] = beta[p] > 0 ? 1.0 / beta[p] : 0.0;
@@ -2327,8 +2527,10 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
                         g[(j + 1) * P + p] = -sn[j * P + p] * g[j * P + p];
                         g[j * P + p] = std::conj(cs[j * P + p]) * g[j * P + p];
                         res[p] = std::abs(g[(j + 1) * P + p]) / bnorm[p];
-                        if (res[p] > tol) all = false;
+                        if (res[p] > inner_tol) all = false;
                     }
+                    estimate = 0;
+                    for (int p = 0; p < P; ++p) estimate = std::max(estimate, res[p]);
                     used = j + 1;
                     return all;
                 };
@@ -2757,37 +2959,109 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
                     const double v = 0.5 * ((*G)[(size_t)a * k + b] + (*G)[(size_t)b * k + a]);
                     (*G)[(size_t)a * k + b] = (*G)[(size_t)b * k + a] = v;
                 }
-        // FP16 copies of the bases for the solve's restriction and prolongation, each scaled by its
-        // largest magnitude (the Galerkin matrices above used FP32; PCBQS_AMD_COARSE_FP16=0 keeps the
-        // solve on the FP32 bases). The coarse correction is a preconditioner: its rounding moves the
-        // iterates, not the solution, which is accepted on the true FP64 residual.
-        const char *hv = std::getenv("PCBQS_AMD_COARSE_FP16");
-        L->coarse_half = !(hv && hv[0] == '0');
-        if (L->coarse_half) {
-            L->Phih.reserve((size_t)n * kp * 2); L->Phith.reserve((size_t)n * k * 2);
-            L->RPhih.reserve((size_t)n * k * 2); L->LPhih.reserve((size_t)n * k * 2);
-            L->hmax.reserve(4 * 4);
-            PQ_HIP(hipMemsetAsync(L->hmax.p, 0, 16, L->stream));
-            const float *src[4] = {L->Phi.as<float>(), L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
-            _Float16 *dst[4] = {L->Phih.as<_Float16>(), L->Phith.as<_Float16>(), L->RPhih.as<_Float16>(),
-                                L->LPhih.as<_Float16>()};
-            for (int q = 0; q < 4; ++q) {
-                const size_t cnt = q == 0 ? (size_t)n * kp : (size_t)n * k;
-                hipLaunchKernelGGL(absmax_kernel, dim3(1024), dim3(256), 0, L->stream, src[q], cnt,
-                                   L->hmax.as<unsigned>() + q);
-                hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream, src[q],
-                                   cnt, L->hmax.as<unsigned>() + q, dst[q]);
-            }
-            PQ_HIP(hipGetLastError());
-            unsigned mx[4];
-            PQ_HIP(hipMemcpyAsync(mx, L->hmax.p, 16, hipMemcpyDeviceToHost, L->stream));
+        make_half_bases(L);
+    });
+}
+
+// The coarse space re-expressed, on the device, in the generalised eigenbasis of its Galerkin pair:
+// Phi (n_free, k) FP64 host, rounded to FP32 (the basis the products use); Rc = Phi^T Rl Phi and
+// Lc = Phi^T Ll Phi (FP64 dgemm on the FP64 products Rl Phi, Ll Phi); Rc = U S U^T (dsyevd),
+// directions with S below drop * max S dropped, W = U S^-1/2; W^T Lc W = Q Lam Q^T (dsyevd); the
+// basis Psi = Phi W Q then has Psi^T Rl Psi = I and Psi^T Ll Psi = Lam, so every frequency's coarse
+// solve is a diagonal scaling. Rl Psi and Ll Psi are (Rl Phi) W Q and (Ll Phi) W Q. *k_out receives
+// the kept dimension.
+int pq_loops_set_coarse_diagonal(pq_loops *L, int k, const double *Phi, double drop, int *k_out) {
+    return guard([&] {
+        PQ_REQUIRE(L && Phi && k_out, "pq_loops_set_coarse_diagonal: null argument");
+        PQ_REQUIRE(k >= 1 && k <= 4096, "pq_loops_set_coarse_diagonal: 1 <= k <= 4096");
+        PQ_REQUIRE(drop >= 0 && drop < 1, "pq_loops_set_coarse_diagonal: 0 <= drop < 1");
+        DeviceScope scope(L->device);
+        const int n = L->nf;
+        PQ_REQUIRE(n > 0, "pq_loops_set_coarse_diagonal: no free loops");
+        L->kc = 0;
+        L->omega_c = -1;
+        L->crows.clear(); L->psplit.clear(); L->pdepth.clear(); L->prows.clear();
+        DevBuf P64, RX, LX, Rc, Lc, W, M, T, Y;
+        P64.reserve((size_t)n * k * 8);
+        PQ_HIP(hipMemcpyAsync(P64.p, Phi, (size_t)n * k * 8, hipMemcpyHostToDevice, L->stream));
+        {   // the FP32 basis, and P64 rounded to it
+            DevBuf tmp;
+            const int kp = (k + 63) / 64 * 64;
+            tmp.reserve((size_t)n * kp * 4);
+            hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kp + 255) / 256)), dim3(256), 0,
+                               L->stream, P64.as<double>(), n, k, kp, tmp.as<float>());
+            L->Phi.reserve((size_t)n * kp * 4);
+            PQ_HIP(hipMemcpyAsync(L->Phi.p, tmp.p, (size_t)n * kp * 4, hipMemcpyDeviceToDevice, L->stream));
+            L->kp = kp;
             PQ_HIP(hipStreamSynchronize(L->stream));
-            for (int q = 0; q < 4; ++q) {
-                float m;
-                std::memcpy(&m, &mx[q], 4);
-                L->hscale[q] = m > 0 ? m : 1.0;
-            }
         }
+        RX.reserve((size_t)n * k * 8); LX.reserve((size_t)n * k * 8);
+        basis_products(L, P64.as<double>(), k, RX.as<double>(), LX.as<double>());
+        Rc.reserve((size_t)k * k * 8); Lc.reserve((size_t)k * k * 8);
+        gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, RX.as<double>(), k, Rc.as<double>(), k);
+        gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, LX.as<double>(), k, Lc.as<double>(), k);
+        const unsigned kb = (unsigned)(((size_t)k * k + 255) / 256);
+        hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Rc.as<double>(), k);
+        hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Lc.as<double>(), k);
+        PQ_HIP(hipGetLastError());
+        std::vector<double> sv, lam;
+        device_eigh(L, Rc.as<double>(), k, sv);                        // Rc's columns: U
+        const double smax = sv.empty() ? 0.0 : sv.back();
+        PQ_REQUIRE(smax > 0, "coarse space: the Galerkin resistance is not positive");
+        int first = 0;
+        while (first < k && !(sv[first] > drop * smax)) ++first;
+        const int kk = k - first;
+        W.reserve((size_t)k * kk * 8);
+        {
+            DevBuf sd;
+            sd.reserve((size_t)k * 8);
+            PQ_HIP(hipMemcpyAsync(sd.p, sv.data(), (size_t)k * 8, hipMemcpyHostToDevice, L->stream));
+            hipLaunchKernelGGL(whiten_kernel, dim3((unsigned)(((size_t)k * kk + 255) / 256)), dim3(256), 0, L->stream,
+                               Rc.as<double>(), sd.as<double>(), k, first, kk, W.as<double>());
+            PQ_HIP(hipGetLastError());
+            PQ_HIP(hipStreamSynchronize(L->stream));
+        }
+        T.reserve((size_t)k * kk * 8);
+        M.reserve((size_t)kk * kk * 8);
+        gemm_rm(L, false, false, k, kk, k, Lc.as<double>(), k, W.as<double>(), kk, T.as<double>(), kk);   // Lc W
+        gemm_rm(L, true, false, kk, kk, k, W.as<double>(), kk, T.as<double>(), kk, M.as<double>(), kk);    // W^T Lc W
+        hipLaunchKernelGGL(symmetrize64_kernel, dim3((unsigned)(((size_t)kk * kk + 255) / 256)), dim3(256), 0,
+                           L->stream, M.as<double>(), kk);
+        PQ_HIP(hipGetLastError());
+        device_eigh(L, M.as<double>(), kk, lam);                       // M's columns: Q
+        // T = W Q (M holds Q column-major: as a row-major array it is Q^T)
+        gemm_rm(L, false, true, k, kk, kk, W.as<double>(), kk, M.as<double>(), kk, T.as<double>(), kk);
+        // Psi = Phi T, Rl Psi = (Rl Phi) T, Ll Psi = (Ll Phi) T
+        Y.reserve((size_t)n * kk * 8);
+        const int kpp = (kk + 63) / 64 * 64;
+        L->kc = kk;
+        L->kp = kpp;
+        L->Phi.reserve((size_t)n * kpp * 4); L->Phit.reserve((size_t)n * kk * 4);
+        L->RPhi.reserve((size_t)n * kk * 4); L->LPhi.reserve((size_t)n * kk * 4);
+        const unsigned nb = (unsigned)(((size_t)n * kk + 255) / 256);
+        gemm_rm(L, false, false, n, kk, k, P64.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+        hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kpp + 255) / 256)), dim3(256), 0, L->stream,
+                           Y.as<double>(), n, kk, kpp, L->Phi.as<float>());
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                           L->Phit.as<float>());
+        gemm_rm(L, false, false, n, kk, k, RX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                           L->RPhi.as<float>());
+        gemm_rm(L, false, false, n, kk, k, LX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                           L->LPhi.as<float>());
+        PQ_HIP(hipGetLastError());
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        // the Galerkin pair in the new basis: I and Lam (their off-diagonal parts are rounding)
+        L->Rc.assign((size_t)kk * kk, 0.0);
+        L->Lc.assign((size_t)kk * kk, 0.0);
+        for (int a = 0; a < kk; ++a) {
+            L->Rc[(size_t)a * kk + a] = 1.0;
+            L->Lc[(size_t)a * kk + a] = lam[a];
+        }
+        L->cdiag = true;
+        make_half_bases(L);
+        *k_out = kk;
     });
 }
 

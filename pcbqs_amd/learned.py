@@ -6,8 +6,9 @@ nearest via or port) is one enrichment pattern; on the free loops' cotree branch
 loop-space columns that join the stream-function hats of the two-level preconditioner. The network
 is trained on solver-generated pairs: for converged port solutions x* (two-level GMRES to 1e-10 at
 several frequencies) the loss is the fraction of x* - Pi_hats x* that the learned columns leave
-uncaptured. Block-Jacobi GMRES from zero stays the fallback when a solve fails, and the true residual
-of every solve judges the network. No counterpart in the reference (its iterative path uses a
+uncaptured. Block-Jacobi GMRES from zero is the fallback when a two-level solve fails its acceptance
+test (PortSolver.ports retries it and names the fallback in its info), and the true residual of
+every solve judges the network. No counterpart in the reference (its iterative path uses a
 graph-Laplacian preconditioner, src/pcbqs/mqs_precond.py:27-59).
 
 Graph: one node per branch of the loop model (RWG edges, via segments, port sources). Two RWG edges

@@ -131,12 +131,20 @@ class PortSolver:
         With diagonalize, the span is re-expressed in the generalised eigenbasis of the Galerkin
         pair (Phi^T Rl Phi, Phi^T Ll Phi): Psi = Phi V with V^T Rc V = I and V^T Lc V = diag, so
         every frequency's coarse solve is a diagonal scaling. Directions with a Galerkin
-        resistance below 1e-10 of the largest are dropped (they carry no loop current)."""
+        resistance below 1e-10 of the largest are dropped (they carry no loop current). The
+        change of basis runs on the device (pq_loops_set_coarse_diagonal); diagonalize='host'
+        does it with numpy instead (two eigh calls), False keeps Phi and inverts the k x k coarse
+        matrix per frequency."""
         phi = np.zeros((self.model.n_free, 0)) if phi is None else np.ascontiguousarray(phi, np.float64)
         if phi.ndim != 2 or phi.shape[0] != self.model.n_free:
             raise ValueError('coarse space must have one row per free loop')
         k = phi.shape[1]
         self.coarse_phi = phi
+        if k and diagonalize is True:
+            kept = ctypes.c_int()
+            call('pq_loops_set_coarse_diagonal', self._handle, k, ptr(phi), 1e-10, ctypes.byref(kept))
+            self.coarse_k = kept.value
+            return
         call('pq_loops_set_coarse', self._handle, k, ptr(phi) if k else None, 0)
         if k and diagonalize:
             Rc, Lc = np.empty((k, k)), np.empty((k, k))
@@ -211,7 +219,7 @@ class PortSolver:
         return self._join(X)
 
     def ports(self, freqs, precond='block-jacobi', tol=1e-8, restart=100, maxiter=3000, currents=False, x0=None,
-              solver='iterative'):
+              solver='iterative', fallback_maxiter=3000):
         """Port impedance matrices (F, P, P) complex; info per frequency (iterations, residual, seconds).
 
         solver: 'iterative' (lockstep GMRES with `precond`, to `tol`), 'direct' (the free-loop
@@ -220,7 +228,10 @@ class PortSolver:
         maxiter and x0 do not apply, and `iterations` counts the corrections) or 'auto' (the
         reference's choice between them, `port_method`).
         x0: optional initial iterates, (F, n_free, P) complex or a callable frequency -> (n_free, P)
-        (the learned correction's prediction); GMRES starts from zero otherwise."""
+        (the learned correction's prediction); GMRES starts from zero otherwise.
+        A two-level (learned) solve, or one from an initial iterate, that fails its acceptance test is
+        solved again by block-Jacobi GMRES from zero (at most fallback_maxiter iterations): the plain
+        method is the fallback and the judge; info then names it ('fallback_from')."""
         if solver not in ('iterative', 'direct', 'auto'):
             raise ValueError("solver must be 'iterative', 'direct' or 'auto'")
         if solver == 'auto':
@@ -287,16 +298,27 @@ class PortSolver:
                 if Pd != P:
                     guess = np.hstack([guess, np.repeat(guess[:, -1:], Pd - P, axis=1)])
                 start = self._split(guess)
-            call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, Pd, ptr(cols), kind, float(tol),
-                 int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None,
-                 ptr(start) if start is not None else None, ctypes.byref(its), ctypes.byref(res))
+            used = precond
+            try:
+                call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, Pd, ptr(cols), kind, float(tol),
+                     int(restart), int(maxiter), ptr(zp), ptr(xh) if currents else None,
+                     ptr(start) if start is not None else None, ctypes.byref(its), ctypes.byref(res))
+            except _lib.ConvergenceError:
+                if kind != PRECONDITIONERS['two-level'] and start is None:
+                    raise
+                used = 'block-jacobi'
+                call('pq_loops_solve_ports', self._handle, 2 * np.pi * fi, Pd, ptr(cols),
+                     PRECONDITIONERS['block-jacobi'], float(tol), int(restart), int(fallback_maxiter), ptr(zp),
+                     ptr(xh) if currents else None, None, ctypes.byref(its), ctypes.byref(res))
             Z[i] = zp[:P, :P, 0] + 1j * zp[:P, :P, 1]
             if currents:
                 X[i] = np.hstack([xh[:, :P], xh[:, Pd:Pd + P]])
             seconds, mv = ctypes.c_double(), ctypes.c_int64()
             call('pq_loops_last_solve', self._handle, None, ctypes.byref(mv), None, ctypes.byref(seconds))
             info.append(dict(frequency_Hz=float(fi), iterations=int(its.value), matvecs=int(mv.value),
-                             relative_residual=float(res.value), seconds=float(seconds.value), preconditioner=precond))
+                             relative_residual=float(res.value), seconds=float(seconds.value), preconditioner=used))
+            if used != precond:
+                info[-1]['fallback_from'] = precond if start is None else f'{precond} from an initial iterate'
         return (Z, info, X) if currents else (Z, info)
 
     def use_graphs(self, enable=True):

@@ -150,3 +150,49 @@ def test_tree_projection_matches_the_column_projection(name):
         s.close()
     assert np.array_equal(L['tree'], L['tree'].T)
     assert rel(L['tree'], L['columns']) < 1e-6
+
+
+def test_failed_two_level_solve_falls_back_to_block_jacobi():
+    """A two-level solve that fails its acceptance test (here: at most 2 iterations) is solved again
+    by block-Jacobi GMRES from zero; the result is the plain solve's and info names the fallback."""
+    from pcbqs_amd.solve import PortSolver
+    b = boards.board_c1()
+    freqs = [1e3, 1e6]
+    s = PortSolver(b, device=0)
+    s.set_stream_coarse(grid=6)
+    Zp, plain = s.ports(freqs, precond='block-jacobi', tol=1e-8)
+    Z, info = s.ports(freqs, precond='two-level', tol=1e-8, maxiter=2)
+    assert all(i['preconditioner'] == 'block-jacobi' and i['fallback_from'] == 'two-level' for i in info)
+    assert all(i['relative_residual'] <= 1e-8 for i in info)
+    assert np.array_equal(Z, Zp)
+    from pcbqs_amd._lib import ConvergenceError
+    with pytest.raises(ConvergenceError):
+        s.ports(freqs, precond='block-jacobi', tol=1e-8, maxiter=2)
+    s.close()
+
+
+def test_device_diagonalised_coarse_space_equals_the_host_one():
+    """The coarse space diagonalised on the device (rocSOLVER) and on the host (numpy eigh) span the
+    same space: the coarse solution Phi (Phi^T Z Phi)^-1 Phi^T B does not depend on the basis, so the
+    two agree to the FP32 bases' rounding; the two-level solves give the same port impedances."""
+    from pcbqs_amd.coarse import loop_restrict, stream_patterns, via_loop_columns
+    from pcbqs_amd.solve import PortSolver
+    b = boards.board_c2(1, target=3000)
+    s = PortSolver(b, device=0)
+    m = s.model
+    phi = np.hstack([loop_restrict(m, stream_patterns(m, 10)), via_loop_columns(m)])
+    out = {}
+    for how in ('host', True):
+        s.set_coarse(phi, diagonalize=how)
+        k = s.coarse_k
+        xc = {f: s.coarse_solution(f) for f in (1e3, 1e6, 1e8)}
+        Z, info = s.ports([1e3, 1e6, 1e8], precond='two-level', tol=1e-10)
+        out[how] = (k, xc, Z, [i['iterations'] for i in info])
+    (k0, x0, Z0, it0), (k1, x1, Z1, it1) = out['host'], out[True]
+    print(f'coarse columns host {k0} device {k1}; iterations {it0} vs {it1}', flush=True)
+    assert k0 == k1
+    for f in x0:
+        assert np.abs(x1[f] - x0[f]).max() <= 1e-5 * np.abs(x0[f]).max(), f
+    assert np.abs(Z1 - Z0).max() <= 1e-9 * np.abs(Z0).max()
+    assert all(abs(a - c) <= max(3, 0.1 * a) for a, c in zip(it0, it1))
+    s.close()

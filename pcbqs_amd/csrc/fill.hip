@@ -21,6 +21,7 @@
 // prefilter with a 1e-5 relative margin settles the clearly-far pairs without it.
 #include "pq_common.h"
 #include <mutex>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 #include <algorithm>
@@ -67,7 +68,21 @@ __device__ __noinline__ int exact_kind(double ax, double ay, double az, double a
 
 // ---- per-pair moment functions -----------------------------------------------------------
 
-// FP64 far 3x3 product rule, reference arithmetic order (static_kernels.hpp:57-72).
+// 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps y <- y + y (1 - x y^2) / 2,
+// each of which squares the relative error; the result is within a few ulp of the correctly
+// rounded value, at about half the instructions of a correctly rounded square root and division.
+__device__ __forceinline__ double inv_sqrt64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// FP64 far 3x3 product rule, reference arithmetic order (static_kernels.hpp:57-72); w / sqrt(r^2)
+// taken as w * inv_sqrt64(r^2).
 template <bool SCALAR>
 __device__ __forceinline__ void far64(const double *pa, const double *pb, double w, double d, double *m) {
 #pragma unroll
@@ -79,7 +94,7 @@ __device__ __forceinline__ void far64(const double *pa, const double *pb, double
         for (int j = 0; j < 3; ++j) {
             const double sx = pb[2 * j], sy = pb[2 * j + 1];
             const double dx = x - sx, dy = y - sy;
-            const double gg = w / __builtin_sqrt(dx * dx + dy * dy + d * d);
+            const double gg = w * inv_sqrt64(dx * dx + dy * dy + d * d);
             m[0] += gg;
             if (!SCALAR) {
                 m[1] += gg * x; m[2] += gg * y; m[3] += gg * sx; m[4] += gg * sy;
@@ -535,7 +550,99 @@ __global__ __launch_bounds__(512) void far32_tile_kernel(GeomDev g, FarArgs A, f
     block_near_flush(B, A);
 }
 
-// FP64 reference-layout far kernel.
+// FP64 reference-layout far kernel by upper-triangular 64 x 64 tiles (the whole matrix, both
+// entries of every pair): each unordered far pair r < c is integrated once with far64's arithmetic.
+// Its entry (r, c) is stored along row r (the lanes run over c); the reverse entry (c, r) is staged
+// in LDS, 16 tile rows at a time, and stored along rows c too. Near pairs go to the block's near
+// lists as in the FP32 tile kernel (the near kernels write both of their entries).
+template <bool SCALAR>
+__global__ __launch_bounds__(512) void far64_tile_kernel(GeomDev g, FarArgs A, double *out, int nb) {
+    constexpr int NP = SCALAR ? 1 : 4;
+    constexpr int WAVES = 8, PART = 16, PARTS = TILE / PART, ROWS = PART / WAVES, PITCH = PART + 1;
+    constexpr int RW = 7;                                  // far points (6) and area per triangle
+    __shared__ double stage[NP * TILE * PITCH];
+    __shared__ double rowp[TILE * RW], colp[TILE * RW];
+    __shared__ TriVals rowd[TILE], cold[TILE];
+    __shared__ unsigned long long nlist[2][NEAR_CAP], nbase[2];
+    __shared__ unsigned ncount[5];
+    const BlockNear B{{nlist[0], nlist[1]}, ncount, nbase};
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int br, bc;
+    tile_coords(blockIdx.x, nb, br, bc);
+    const int R0 = br * TILE, C0 = bc * TILE;
+    if (threadIdx.x < 2 * TILE) {
+        const int i = threadIdx.x & (TILE - 1);
+        const bool row = threadIdx.x < TILE;
+        const int t = min((row ? R0 : C0) + i, A.n - 1);
+        double *p = (row ? rowp : colp) + i * RW;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) p[q] = g.P3[6 * (size_t)t + q];
+        p[6] = g.area[t];
+        (row ? rowd : cold)[i] = tri_vals(g, t);
+    }
+    block_near_init(B);
+    __syncthreads();
+    const int c = C0 + lane;
+    const bool cvalid = c < A.n;
+    double pc[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pc[q] = colp[lane * RW + q];
+    const double areac = colp[lane * RW + 6];
+    const TriVals cv = cold[lane];
+    for (int part = 0; part < PARTS; ++part) {
+#pragma unroll
+        for (int k = 0; k < ROWS; ++k) {
+            const int hl = wave * ROWS + k;                // row within the part
+            const int rl = part * PART + hl;
+            const int r = R0 + rl;                         // wave-uniform
+            const bool valid = cvalid && r < A.n && r <= c;
+            const TriVals rv = rowd[rl];
+            const int kind = valid ? classify(rv, cv, A) : -1;
+            block_near(B, A, valid, kind, r, c);
+            const bool store = valid && kind < 0;
+            double pr[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) pr[q] = rowp[rl * RW + q];
+            double m[6];
+            far64<SCALAR>(pr, pc, rowp[rl * RW + 6] * areac / 9, rv.z - cv.z, m);
+            if (store) {
+                store_entry<double, SCALAR>(out, A.plane, (size_t)r * A.ld + c, r, c, m, A.flag);
+                if (!SCALAR && !(isfinite(m[3]) && isfinite(m[4]))) atomicOr(A.flag, 1u);
+            }
+            // reverse entry (c, r): staged at [c - C0][r - R0 - part * PART]
+            const int sidx = lane * PITCH + hl;
+            stage[sidx] = (store && r != c) ? m[0] : __builtin_nan("");
+            if (!SCALAR) {
+                stage[TILE * PITCH + sidx] = m[3];
+                stage[2 * TILE * PITCH + sidx] = m[4];
+                stage[3 * TILE * PITCH + sidx] = m[5];
+            }
+        }
+        __syncthreads();
+        // the staged part: 64 matrix rows C0.. of PART columns, four rows per wave instruction
+        const int sub = lane / PART, col = lane % PART;
+        for (int k = 0; k < TILE / (4 * WAVES); ++k) {
+            const int cl = (wave * (TILE / (4 * WAVES)) + k) * 4 + sub;
+            const int sidx = cl * PITCH + col;
+            const double m0 = stage[sidx];
+            if (!isnan(m0)) {
+                const size_t idx = (size_t)(C0 + cl) * A.ld + R0 + part * PART + col;
+                __builtin_nontemporal_store(m0, out + idx);
+                if (!SCALAR) {
+                    __builtin_nontemporal_store(stage[TILE * PITCH + sidx], out + A.plane + idx);
+                    __builtin_nontemporal_store(stage[2 * TILE * PITCH + sidx], out + 2 * A.plane + idx);
+                    __builtin_nontemporal_store(stage[3 * TILE * PITCH + sidx], out + 3 * A.plane + idx);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    block_near_flush(B, A);
+}
+
+// FP64 reference-layout far kernel over row strips (the strips of fill_symmetric's host planes):
+// every entry of the strip from its canonical pair.
 template <bool SCALAR>
 __global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double *out) {
     __shared__ unsigned long long nlist[2][NEAR_CAP], nbase[2];
@@ -726,6 +833,11 @@ void launch_far(pq_fill *f, int precision, bool scalar, const FarArgs &A, void *
     } else if (precision == 32) {
         if (scalar) hipLaunchKernelGGL(far32_kernel<true>, grid, dim3(256), 0, s, f->g, A, (float *)out);
         else hipLaunchKernelGGL(far32_kernel<false>, grid, dim3(256), 0, s, f->g, A, (float *)out);
+    } else if (A.mirror && A.r0 == 0 && A.rows == A.n && std::getenv("PCBQS_AMD_FAR64_STRIPS") == nullptr) {
+        const int nb = (A.n + TILE - 1) / TILE;
+        dim3 tiles((unsigned)((long long)nb * (nb + 1) / 2));
+        if (scalar) hipLaunchKernelGGL(far64_tile_kernel<true>, tiles, dim3(512), 0, s, f->g, A, (double *)out, nb);
+        else hipLaunchKernelGGL(far64_tile_kernel<false>, tiles, dim3(512), 0, s, f->g, A, (double *)out, nb);
     } else {
         if (scalar) hipLaunchKernelGGL(far64_kernel<true>, grid, dim3(256), 0, s, f->g, A, (double *)out);
         else hipLaunchKernelGGL(far64_kernel<false>, grid, dim3(256), 0, s, f->g, A, (double *)out);

@@ -306,6 +306,25 @@ int pq_loops_factor_batch(pq_loops *loops, int n_omega, const double *omegas, do
 int pq_loops_solve_direct_batch(pq_loops *loops, int n_omega, const double *omegas, int P, const int32_t *port_cols,
                                 double tol, int max_steps, double *Zp, int64_t *steps, double *residual);
 
+/* Direct port solves of `count` boards (loop handles on one device; board b
+ * with P[b] ports, 1..8 or 16, at port_cols[b]) at the same n_omega
+ * frequencies, each group of boards in ONE captured graph launch: boards taken
+ * largest first and grouped while their free-loop counts stay within 3/4 of
+ * the group's largest (each matrix padded with the identity to the group's
+ * size), every board x frequency matrix formed, factored (FP32 L D L^T) and
+ * solved in the same launches, refined against its board's FP64 system for up
+ * to 2 steps with its own stopping rule kept on the device, and reduced to
+ * port impedances. A matrix left above max(tol, 1e-10) is solved again alone
+ * by pq_loops_solve_direct (further corrections, the FP64 fallback). Zp: per
+ * board in order, (n_omega, P[b], P[b]) interleaved complex; steps and residual
+ * (count, n_omega); stats[0] the graph launches, stats[1] the matrices solved
+ * again alone. Replaces the per-board, per-frequency dense solves of the
+ * reference's sweep (src/pcbqs/mqs_iterative.py:298-367 with the direct route
+ * of port_method, :260-272). */
+int pq_loops_solve_direct_multi(int count, pq_loops *const *handles, const int *P, const int32_t *const *port_cols,
+                                int n_omega, const double *omegas, double tol, int max_steps, double *Zp,
+                                int64_t *steps, double *residual, int64_t *stats);
+
 /* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
  * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the
  * right-hand side of direct_ports (src/pcbqs/mqs_ports.py:52-69). */

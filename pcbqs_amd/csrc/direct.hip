@@ -775,6 +775,14 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
         if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
         hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)(tc * T - tc * (tc - 1) / 2), 1, nz), dim3(256), 0, st,
                            f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 1, tc, f.sF, f.sW);
+        if (!f.aux) {                                    // one stream (a captured batch): the bulk in order
+            if (T > tc)
+                hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2), 1, nz),
+                                   dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF,
+                                   f.sW);
+            diag(m0);
+            continue;
+        }
         bulk = T > tc;
         if (bulk) {
             PQ_HIP(hipEventRecord(f.ready, st));

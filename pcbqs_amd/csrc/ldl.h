@@ -39,7 +39,7 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
               const double *rval, double omega);
 
 // Factor in place on stream st (the work is complete in st's order when it returns; f.aux has no
-// pending work left behind); bad[z] (f.batch device ints, cleared by the caller) is set when a pivot
+// pending work left behind; with f.aux null everything runs on st, in order, as graph capture needs); bad[z] (f.batch device ints, cleared by the caller) is set when a pivot
 // of matrix z is zero or not finite.
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
 

@@ -1142,6 +1142,220 @@ __global__ void vec_add_kernel(const double *X, size_t count, double *Y) {
     if (e < count) Y[e] += X[e];
 }
 
+
+// ---- several boards' direct solves in one captured launch ----------------------------------
+__global__ void fill_int_kernel(int *a, int count, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] = v;
+}
+
+// A batch holds nz matrices Z = Rl + j w Ll of boards with at most nB free loops each, every one
+// padded to nB (the identity beyond the board's own n: the factor of blockdiag(Z, I) is Z's factor
+// and the identity), so that the L D L^T kernels of one frequency batch serve uneven boards.
+struct BatchMat {
+    const float *Ll32;        // the board's FP32 loop matrix (ldl), for the FP32 factor
+    const double *Ll64;       // its FP64 loop matrix (ldl): right-hand sides, residuals, ports
+    long long ldl;
+    const int *rptr, *rcol;   // Rl, CSR over the board's loops
+    const double *rval;
+    const int *sc;            // the board's port loop columns (P of them)
+    int n, P;                 // free loops, ports
+    double omega;
+};
+
+// F (planes Fr, Fi, lower triangle) = Rl + j w Ll on the board's n loops, the identity beyond.
+__global__ __launch_bounds__(256) void batch_form_kernel(const BatchMat *mats, int nB, float *F, long long ldf,
+                                                        long long sF) {
+    const BatchMat m = mats[blockIdx.z];
+    float *Fr = F + blockIdx.z * sF, *Fi = Fr + (size_t)nB * ldf;
+    const float w = (float)m.omega;
+    for (int row = blockIdx.y; row < nB; row += gridDim.y)
+        for (int c = blockIdx.x * 256 + threadIdx.x; c <= row; c += gridDim.x * 256) {
+            const bool in = row < m.n;
+            Fr[(size_t)row * ldf + c] = in ? 0.f : (c == row ? 1.f : 0.f);
+            Fi[(size_t)row * ldf + c] = in ? w * m.Ll32[(size_t)row * m.ldl + c] : 0.f;
+        }
+}
+
+__global__ void batch_form_r_kernel(const BatchMat *mats, int nB, float *F, long long ldf, long long sF) {
+    const BatchMat m = mats[blockIdx.z];
+    float *Fr = F + blockIdx.z * sF;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.n) return;
+    for (int t = m.rptr[row]; t < m.rptr[row + 1]; ++t) {
+        const int c = m.rcol[t];
+        if (c <= row) Fr[(size_t)row * ldf + c] += (float)m.rval[t];
+    }
+}
+
+// B[z] (nB, 2P) = -Z_fs of the board's ports (zero rows beyond n, zero columns beyond its P).
+__global__ void batch_rhs_kernel(const BatchMat *mats, int nB, int P, double *B) {
+    const BatchMat m = mats[blockIdx.z];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nB * P) return;
+    const int row = e / P, p = e % P;
+    double re = 0, im = 0;
+    if (row < m.n && p < m.P) {
+        const int sc = m.sc[p];
+        for (int t = m.rptr[row]; t < m.rptr[row + 1]; ++t)
+            if (m.rcol[t] == sc) re += m.rval[t];
+        im = m.omega * m.Ll64[(size_t)row * m.ldl + sc];
+    }
+    double *b = B + (size_t)blockIdx.z * nB * 2 * P;
+    b[(size_t)row * 2 * P + p] = -re;
+    b[(size_t)row * 2 * P + P + p] = -im;
+}
+
+// W = B - (Rl + j w Ll64) X on the board's rows (zero beyond n): a thread per row, Ll64's row read
+// as its column (symmetric), X's rows wave-uniform.
+template <int NR>
+__global__ __launch_bounds__(256) void batch_residual_kernel(const BatchMat *mats, int nB, const double *X,
+                                                            const double *B, double *W) {
+    constexpr int P = NR / 2;
+    const BatchMat m = mats[blockIdx.z];
+    const size_t off = (size_t)blockIdx.z * nB * NR;
+    X += off; B += off; W += off;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nB) return;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    if (row < m.n) {
+        const double *col = m.Ll64 + row;
+        for (int c = 0; c < m.n; ++c) {
+            const double l = col[(size_t)c * m.ldl];
+            const double *x = X + (size_t)c * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += l * x[r];
+        }
+    }
+    double rx[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) rx[r] = 0;
+    if (row < m.n)
+        for (int t = m.rptr[row]; t < m.rptr[row + 1]; ++t) {
+            const int c = m.rcol[t];
+            if (c >= m.n) continue;
+            const double v = m.rval[t];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) rx[r] += v * X[(size_t)c * NR + r];
+        }
+    const double w = m.omega;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        // (Rl + j w L)(xr + j xi) = (Rl xr - w L xi) + j (Rl xi + w L xr)
+        const double zr = rx[p] - w * acc[P + p], zi = rx[P + p] + w * acc[p];
+        const bool in = row < m.n;
+        W[(size_t)row * NR + p] = in ? B[(size_t)row * NR + p] - zr : 0.0;
+        W[(size_t)row * NR + P + p] = in ? B[(size_t)row * NR + P + p] - zi : 0.0;
+    }
+}
+
+// Per matrix: the largest over its ports of |V_p| / bn_p (bn null: the squared norms |V_p|^2 into
+// norm2 instead). One workgroup per matrix, a fixed-order reduction.
+__global__ __launch_bounds__(256) void batch_colnorm_kernel(const BatchMat *mats, int nB, int P, const double *V,
+                                                           const double *bn2, double *norm2, double *res) {
+    __shared__ double part[256];
+    const BatchMat m = mats[blockIdx.x];
+    const int NR = 2 * P;
+    V += (size_t)blockIdx.x * nB * NR;
+    double worst = 0;
+    for (int p = 0; p < m.P; ++p) {
+        double s = 0;
+        for (int row = threadIdx.x; row < m.n; row += 256) {
+            const double a = V[(size_t)row * NR + p], b = V[(size_t)row * NR + P + p];
+            s += a * a + b * b;
+        }
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (norm2) norm2[(size_t)blockIdx.x * P + p] = part[0];
+            else {
+                const double r = sqrt(part[0]) / fmax(sqrt(bn2[(size_t)blockIdx.x * P + p]), 1e-300);
+                worst = (isnan(r) || isnan(worst)) ? r : fmax(worst, r);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && res) res[blockIdx.x] = worst;
+}
+
+// The refinement's stopping rule per matrix (src/pcbqs/kernels.py:169-178): stop once the residual
+// is at most tol, after max_steps corrections, or once a correction no longer halved it; a matrix
+// still active takes the next correction. state: [0] active, [1] corrections; last: the previous
+// residual.
+__global__ void batch_control_kernel(int nz, const double *res, double *last, int *state, int step, double tol,
+                                     int max_steps) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nz || !state[2 * z]) return;
+    const double w = res[z];
+    if (!(w > tol) || step == max_steps || !(w <= 0.5 * last[z])) {
+        state[2 * z] = 0;
+        return;
+    }
+    last[z] = w;
+    state[2 * z + 1] += 1;
+}
+
+__global__ void batch_init_kernel(int nz, double *last, int *state) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nz) return;
+    last[z] = __builtin_inf();
+    state[2 * z] = 1;
+    state[2 * z + 1] = 0;
+}
+
+// X += D for the matrices still active.
+__global__ void batch_add_kernel(int nB, int NR, const int *state, const double *D, double *X) {
+    const size_t per = (size_t)nB * NR;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per || !state[2 * blockIdx.z]) return;
+    X[blockIdx.z * per + e] += D[blockIdx.z * per + e];
+}
+
+// Zp[z] (P x P complex) = Z_ss + Z_sf X, one workgroup per (matrix, s, q).
+__global__ __launch_bounds__(256) void batch_port_kernel(const BatchMat *mats, int nB, int P, const double *X,
+                                                        double *Zp) {
+    __shared__ double sr[256], si[256];
+    const BatchMat m = mats[blockIdx.z];
+    const int s = blockIdx.x / P, q = blockIdx.x % P;
+    const int NR = 2 * P;
+    X += (size_t)blockIdx.z * nB * NR;
+    double re = 0, im = 0;
+    if (s < m.P && q < m.P) {
+        const int row = m.sc[s];
+        for (int i = threadIdx.x; i < m.n; i += 256) {
+            const double l = m.omega * m.Ll64[(size_t)row * m.ldl + i];
+            const double xr = X[(size_t)i * NR + q], xi = X[(size_t)i * NR + P + q];
+            re += -l * xi;
+            im += l * xr;
+        }
+        for (int t = m.rptr[row] + threadIdx.x; t < m.rptr[row + 1]; t += 256) {
+            const int c = m.rcol[t];
+            if (c == m.sc[q]) re += m.rval[t];
+            if (c >= m.n) continue;
+            re += m.rval[t] * X[(size_t)c * NR + q];
+            im += m.rval[t] * X[(size_t)c * NR + P + q];
+        }
+        if (threadIdx.x == 0) im += m.omega * m.Ll64[(size_t)row * m.ldl + m.sc[q]];
+    }
+    sr[threadIdx.x] = re; si[threadIdx.x] = im;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) { sr[threadIdx.x] += sr[threadIdx.x + st]; si[threadIdx.x] += si[threadIdx.x + st]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *o = Zp + ((size_t)blockIdx.z * P * P + blockIdx.x) * 2;
+        o[0] = sr[0];
+        o[1] = si[0];
+    }
+}
+
 }  // namespace pq
 
 using namespace pq;
@@ -2872,6 +3086,203 @@ int pq_loops_solve_direct_batch(pq_loops *L, int nw, const double *omegas, int P
         L->residual = wmax;
         L->factor_ms = L->batch_ms / nw;
         L->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    });
+}
+
+// Direct port solves of several boards at n_omega frequencies each, every batch of boards in one
+// captured graph launch. Boards are taken largest first and grouped while their free-loop counts
+// stay within 3/4 of the group's largest (padding: the identity beyond a board's own loops) and
+// the group's factors fit in about 24 GB (boards of up to 8 ports and of 9 to 16 in separate
+// groups, so that the solves carry no more columns than needed). Per group the graph holds: the nz =
+// boards x frequencies
+// matrices formed in FP32 and factored as L D L^T in the same launches; the right-hand sides;
+// the solve; GRAPH_STEPS refinement steps against each board's FP64 system with every matrix's own
+// stopping rule (kernels.py:169-178) kept on the device; the port reduction. Matrices the graph
+// leaves above max(tol, 1e-10) (or with a zero FP32 pivot) are solved again one by one by
+// pq_loops_solve_direct (more corrections, and the FP64 LU where those stall). Zp: per board in
+// order, (n_omega, P_b, P_b) interleaved complex; steps, residual: (count, n_omega); stats[0] the
+// graph launches, stats[1] the matrices solved again one by one.
+constexpr int GRAPH_STEPS = 2;
+
+int pq_loops_solve_direct_multi(int count, pq_loops *const *handles, const int *Ps, const int32_t *const *port_cols,
+                                int nw, const double *omegas, double tol, int max_steps, double *Zp, int64_t *steps,
+                                double *residual, int64_t *stats) {
+    return guard([&] {
+        PQ_REQUIRE(count >= 1 && handles && Ps && port_cols && nw >= 1 && omegas && Zp && steps && residual,
+                   "pq_loops_solve_direct_multi: null argument");
+        PQ_REQUIRE(tol > 0 && tol < 1 && max_steps >= 0, "pq_loops_solve_direct_multi: invalid options");
+        const int device = handles[0]->device;
+        for (int b = 0; b < count; ++b) {
+            const pq_loops *L = handles[b];
+            PQ_REQUIRE(L && L->device == device, "pq_loops_solve_direct_multi: handles on one device");
+            PQ_REQUIRE((Ps[b] >= 1 && Ps[b] <= 8) || Ps[b] == 16, "pq_loops_solve_direct_multi: 1 to 8 or 16 ports");
+            for (int p = 0; p < Ps[b]; ++p)
+                PQ_REQUIRE(port_cols[b][p] >= L->nf && port_cols[b][p] < L->nl,
+                           "pq_loops_solve_direct_multi: port loop out of range");
+        }
+        for (int f = 0; f < nw; ++f)
+            PQ_REQUIRE(std::isfinite(omegas[f]) && omegas[f] >= 0, "pq_loops_solve_direct_multi: invalid frequency");
+        DeviceScope scope(device);
+        std::vector<size_t> zoff(count + 1, 0);              // each board's offset into Zp
+        for (int b = 0; b < count; ++b) zoff[b + 1] = zoff[b] + (size_t)nw * Ps[b] * Ps[b] * 2;
+        std::vector<int> order(count);
+        for (int b = 0; b < count; ++b) order[b] = b;
+        // boards of up to 8 ports and of 9 to 16 (16 device columns) apart, each set largest first
+        auto wide = [&](int b) { return Ps[b] > 8; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return wide(a) != wide(b) ? wide(a) : handles[a]->nf > handles[b]->nf;
+        });
+        int64_t launches = 0, redone = 0;
+        hipStream_t st = nullptr;
+        PQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+        } sguard{st};
+        const size_t budget = (size_t)24 << 30;
+        // the groups' buffers, grown as needed and reused from group to group
+        DevBuf dm, F, Xi, Wb, B, X, W, D, Y, Ys, bn2, res, last, state, bad, zp, scs;
+        for (size_t g0 = 0; g0 < order.size();) {
+            const int nmax = std::max(handles[order[g0]]->nf, 1);
+            const int nblk = (nmax + LDL_BLOCK - 1) / LDL_BLOCK, nB = nblk * LDL_BLOCK;
+            const long long ldf = nB;
+            const size_t sF = (size_t)2 * nB * ldf, sX = (size_t)2 * nblk * LDL_BLOCK * LDL_BLOCK,
+                         sW = (size_t)4 * nB * 2 * LDL_BLOCK;
+            const size_t per_matrix = (sF + sX + sW) * 4 + (size_t)6 * nB * 32 * 8;
+            size_t g1 = g0 + 1;
+            while (g1 < order.size() && wide(order[g1]) == wide(order[g0]) && 4 * handles[order[g1]]->nf >= 3 * nmax &&
+                   (g1 - g0 + 1) * nw * per_matrix <= budget && (g1 - g0 + 1) * nw <= 65535)
+                ++g1;
+            const int nb_boards = (int)(g1 - g0), nz = nb_boards * nw;
+            int P = 1;
+            for (size_t g = g0; g < g1; ++g) P = std::max(P, Ps[order[g]]);
+            if (P > 8) P = 16;
+            const int NR = 2 * P;
+            // the group's matrices: board-major, frequencies within
+            std::vector<BatchMat> mats(nz);
+            scs.reserve((size_t)nb_boards * 16 * 4);
+            {
+                std::vector<int32_t> sch((size_t)nb_boards * 16, 0);
+                for (int i = 0; i < nb_boards; ++i)
+                    for (int p = 0; p < Ps[order[g0 + i]]; ++p) sch[(size_t)i * 16 + p] = port_cols[order[g0 + i]][p];
+                PQ_HIP(hipMemcpyAsync(scs.p, sch.data(), sch.size() * 4, hipMemcpyHostToDevice, st));
+                PQ_HIP(hipStreamSynchronize(st));
+            }
+            for (int i = 0; i < nb_boards; ++i) {
+                const pq_loops *L = handles[order[g0 + i]];
+                for (int f = 0; f < nw; ++f) {
+                    BatchMat &m = mats[(size_t)i * nw + f];
+                    m.Ll32 = L->Ll.as<float>(); m.Ll64 = L->Ll64.as<double>(); m.ldl = L->ldl;
+                    m.rptr = L->rptr.as<int>(); m.rcol = L->rcol.as<int>(); m.rval = L->rval.as<double>();
+                    m.sc = scs.as<int>() + (size_t)i * 16;
+                    m.n = L->nf; m.P = Ps[order[g0 + i]]; m.omega = omegas[f];
+                }
+            }
+            dm.reserve(sizeof(BatchMat) * nz);
+            PQ_HIP(hipMemcpyAsync(dm.p, mats.data(), sizeof(BatchMat) * nz, hipMemcpyHostToDevice, st));
+            F.reserve(sF * nz * 4); Xi.reserve(sX * nz * 4); Wb.reserve(sW * nz * 4);
+            const size_t vs = (size_t)nB * NR;
+            for (DevBuf *v : {&B, &X, &W, &D, &Y, &Ys}) v->reserve(vs * nz * 8);
+            bn2.reserve((size_t)nz * P * 8); res.reserve((size_t)nz * 8); last.reserve((size_t)nz * 8);
+            state.reserve((size_t)nz * 2 * 4); bad.reserve((size_t)nz * 4); zp.reserve((size_t)nz * P * P * 2 * 8);
+            PQ_HIP(hipStreamSynchronize(st));
+            const size_t wsz = (size_t)nB * 2 * LDL_BLOCK;
+            float *Wf = Wb.as<float>();
+            LdlFactor fct{F.as<float>(), F.as<float>() + (size_t)nB * ldf, ldf, nB, Xi.as<float>(),
+                          Xi.as<float>() + (size_t)nblk * LDL_BLOCK * LDL_BLOCK, {Wf, Wf + 2 * wsz},
+                          {Wf + wsz, Wf + 3 * wsz}, nullptr, nullptr, nullptr};
+            fct.batch = nz;
+            fct.sF = (long long)sF; fct.sX = (long long)sX; fct.sW = (long long)sW;
+            const BatchMat *dmat = dm.as<BatchMat>();
+            const int S = std::min(GRAPH_STEPS, max_steps);
+            auto body = [&]() {
+                const unsigned zb = (unsigned)((nz + 255) / 256);
+                hipLaunchKernelGGL(batch_init_kernel, dim3(zb), dim3(256), 0, st, nz, last.as<double>(), state.as<int>());
+                hipLaunchKernelGGL(fill_int_kernel, dim3(zb), dim3(256), 0, st, bad.as<int>(), nz, 0);
+                hipLaunchKernelGGL(batch_form_kernel, dim3((nB + 255) / 256, std::min(nB, 4096), nz), dim3(256), 0, st,
+                                   dmat, nB, F.as<float>(), ldf, (long long)sF);
+                hipLaunchKernelGGL(batch_form_r_kernel, dim3((nB + 255) / 256, 1, nz), dim3(256), 0, st, dmat, nB,
+                                   F.as<float>(), ldf, (long long)sF);
+                ldl_factor(st, fct, bad.as<int>());
+                hipLaunchKernelGGL(batch_rhs_kernel, dim3((nB * P + 255) / 256, 1, nz), dim3(256), 0, st, dmat, nB, P,
+                                   B.as<double>());
+                hipLaunchKernelGGL(batch_colnorm_kernel, dim3(nz), dim3(256), 0, st, dmat, nB, P, B.as<double>(),
+                                   nullptr, bn2.as<double>(), nullptr);
+                ldl_solve(st, fct, P, B.as<double>(), Y.as<double>(), Ys.as<double>(), X.as<double>());
+                for (int step = 0; step <= S; ++step) {
+                    dispatch_nr(NR, [&](auto nr) {
+                        constexpr int N = decltype(nr)::value;
+                        hipLaunchKernelGGL(batch_residual_kernel<N>, dim3((nB + 255) / 256, 1, nz), dim3(256), 0, st,
+                                           dmat, nB, X.as<double>(), B.as<double>(), W.as<double>());
+                    });
+                    hipLaunchKernelGGL(batch_colnorm_kernel, dim3(nz), dim3(256), 0, st, dmat, nB, P, W.as<double>(),
+                                       bn2.as<double>(), nullptr, res.as<double>());
+                    hipLaunchKernelGGL(batch_control_kernel, dim3(zb), dim3(256), 0, st, nz, res.as<double>(),
+                                       last.as<double>(), state.as<int>(), step, tol, S);
+                    if (step == S) break;
+                    ldl_solve(st, fct, P, W.as<double>(), Y.as<double>(), Ys.as<double>(), D.as<double>());
+                    hipLaunchKernelGGL(batch_add_kernel, dim3((unsigned)((vs + 255) / 256), 1, nz), dim3(256), 0, st,
+                                       nB, NR, state.as<int>(), D.as<double>(), X.as<double>());
+                }
+                hipLaunchKernelGGL(batch_port_kernel, dim3(P * P, 1, nz), dim3(256), 0, st, dmat, nB, P, X.as<double>(),
+                                   zp.as<double>());
+                PQ_HIP(hipGetLastError());
+            };
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            PQ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            PQ_HIP(hipStreamEndCapture(st, &graph));
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            PQ_HIP(ie);
+            const hipError_t le = hipGraphLaunch(exec, st);
+            const hipError_t se = le == hipSuccess ? hipStreamSynchronize(st) : le;
+            (void)hipGraphExecDestroy(exec);
+            PQ_HIP(se);
+            ++launches;
+            std::vector<double> zh((size_t)nz * P * P * 2), rh(nz);
+            std::vector<int> sh((size_t)nz * 2), bh(nz);
+            PQ_HIP(hipMemcpy(zh.data(), zp.p, zh.size() * 8, hipMemcpyDeviceToHost));
+            PQ_HIP(hipMemcpy(rh.data(), res.p, rh.size() * 8, hipMemcpyDeviceToHost));
+            PQ_HIP(hipMemcpy(sh.data(), state.p, sh.size() * 4, hipMemcpyDeviceToHost));
+            PQ_HIP(hipMemcpy(bh.data(), bad.p, bh.size() * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nb_boards; ++i) {
+                const int b = order[g0 + i], Pb = Ps[b];
+                for (int f = 0; f < nw; ++f) {
+                    const int z = i * nw + f;
+                    double *out = Zp + zoff[b] + (size_t)f * Pb * Pb * 2;
+                    if (bh[z] || !(rh[z] <= std::max(tol, 1e-10))) {
+                        // not brought to the reference's level in the graph: one by one, with
+                        // the full refinement and the FP64 fallback
+                        int64_t st1 = 0;
+                        double r1 = 0;
+                        const int rc = pq_loops_solve_direct(handles[b], omegas[f], Pb, port_cols[b], tol, max_steps,
+                                                             out, nullptr, &st1, &r1);
+                        if (rc != PQ_OK) throw Error(rc, pq_last_error());
+                        steps[(size_t)b * nw + f] = st1;
+                        residual[(size_t)b * nw + f] = r1;
+                        ++redone;
+                        continue;
+                    }
+                    for (int s2 = 0; s2 < Pb; ++s2)
+                        for (int q = 0; q < Pb; ++q) {
+                            out[2 * (s2 * Pb + q)] = zh[((size_t)z * P * P + s2 * P + q) * 2];
+                            out[2 * (s2 * Pb + q) + 1] = zh[((size_t)z * P * P + s2 * P + q) * 2 + 1];
+                        }
+                    steps[(size_t)b * nw + f] = sh[2 * z + 1];
+                    residual[(size_t)b * nw + f] = rh[z];
+                }
+            }
+            g0 = g1;
+        }
+        if (stats) { stats[0] = launches; stats[1] = redone; }
     });
 }
 

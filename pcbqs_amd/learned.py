@@ -320,9 +320,10 @@ class BoardData:
     """One board's solver with the hat coarse space, its branch graph, and the converged port
     solutions' residuals after projection onto the hats (the training pairs)."""
 
-    def __init__(self, board, grid=24, device=0, freqs=FREQS, tol=1e-10):
+    def __init__(self, board, grid=24, device=0, freqs=FREQS, tol=1e-10, keep_solver=True):
         from .solve import PortSolver
         torch = _torch()
+        self.rwg = None
         self.solver = PortSolver(board, device=device)
         self.solver.set_stream_coarse(grid)
         self.phi_geo = self.solver.coarse_phi
@@ -339,9 +340,14 @@ class BoardData:
             self.Q = Q
             self.R = R0 - Q @ (Q.T @ R0)                                                 # residual after hats
             self.R0norm = float((R0 ** 2).sum())
+        self.rwg = int(self.solver.model.ne)
+        if not keep_solver:                   # training needs only the graph and the pairs
+            self.solver.close()
+            self.solver = None
 
     def close(self):
-        self.solver.close()
+        if self.solver is not None:
+            self.solver.close()
 
 
 def projection_loss(torch, bd, E):
@@ -429,33 +435,68 @@ def compare(solver, net, freqs, grid=24, tol=1e-8):
     return rows
 
 
+def held_out(net, boards, grid, freq=1e6, tol=1e-8):
+    """Two-level GMRES iterations at freq on held-out boards: (hats alone, hats + the network's
+    enrichment) per board."""
+    from .solve import PortSolver
+    rows = []
+    for b in boards:
+        s = PortSolver(b)
+        s.set_stream_coarse(grid)
+        hats = s.ports([freq], precond='two-level', tol=tol)[1][0]['iterations']
+        set_learned_coarse(s, net, grid)
+        learned = s.ports([freq], precond='two-level', tol=tol)[1][0]['iterations']
+        rows.append(dict(rwg=int(s.model.ne), hats=int(hats), learned=int(learned)))
+        s.close()
+    return rows
+
+
 def main(argv=None):
     import argparse
+    import json
     from . import board as boards
     ap = argparse.ArgumentParser(description='train the learned coarse-space correction on solver pairs')
     ap.add_argument('--boards', type=int, default=12, help='training boards: seeds 1..N of the plane-pair generator')
     ap.add_argument('--target', type=int, default=20000, help='triangles per board')
+    ap.add_argument('--sizes', default=None, metavar='LO:HI',
+                    help='triangles per board drawn uniformly from LO..HI (seeded per board) instead of --target')
     ap.add_argument('--steps', type=int, default=400)
     ap.add_argument('--grid', type=int, default=24)
+    ap.add_argument('--curve', default=None, metavar='N1,N2,...',
+                    help='also train on the first N1, N2, ... boards and report held-out iterations for each')
+    ap.add_argument('--held-out', type=int, default=3, help='held-out boards (seeds 5000..) besides seed 0')
     ap.add_argument('--out', default=WEIGHTS)
     args = ap.parse_args(argv)
     _torch()
+    lo_hi = tuple(int(v) for v in args.sizes.split(':')) if args.sizes else None
+
+    def make(seed):
+        if lo_hi is None:
+            return boards.board_c2(seed, target=args.target)
+        return boards.board_c2(seed, target=int(np.random.default_rng(seed).integers(lo_hi[0], lo_hi[1] + 1)))
     data = []
     t = time.perf_counter()
     for s in range(1, args.boards + 1):
-        data.append(BoardData(boards.board_c2(s, target=args.target), grid=args.grid))
-        print(f'board {s}: {data[-1].graph.nb} branches, hat residual '
+        data.append(BoardData(make(s), grid=args.grid, keep_solver=False))
+        print(f'board {s}: {data[-1].rwg} RWG, {data[-1].graph.nb} branches, hat residual '
               f'{float((data[-1].R ** 2).sum()) / data[-1].R0norm:.3f} ({time.perf_counter() - t:.1f} s)', flush=True)
-    net, hist = train(data, steps=args.steps)
+    held = [boards.board_c2(0)] + [make(5000 + i) for i in range(args.held_out)]
+    counts = sorted({int(c) for c in args.curve.split(',')} | {args.boards}) if args.curve else [args.boards]
+    curve = []
+    for count in counts:
+        t = time.perf_counter()
+        net, hist = train(data[:count], steps=args.steps)
+        train_s = time.perf_counter() - t
+        rows = held_out(net, held, args.grid)
+        curve.append(dict(boards=count, steps=args.steps, train_s=train_s, final_uncaptured=float(np.mean(hist[-50:])),
+                          held_out=rows))
+        print('curve', json.dumps(curve[-1]), flush=True)
+        if count == args.boards:
+            net.save(args.out, boards=args.boards, target=args.target if lo_hi is None else -1, steps=args.steps,
+                     grid=args.grid, final_uncaptured=float(np.mean(hist[-50:])))
     for bd in data:
         bd.close()
-    net.save(args.out, boards=args.boards, target=args.target, steps=args.steps, grid=args.grid,
-             final_uncaptured=float(np.mean(hist[-50:])))
-    held = BoardData(boards.board_c2(0, target=args.target), grid=args.grid)
-    print('held-out seed 0', evaluate(net, held), flush=True)
-    for row in compare(held.solver, net, [1e3, 1e6, 1e8], grid=args.grid):
-        print('held-out seed 0', row, flush=True)
-    held.close()
+    print(json.dumps(dict(learning_curve=curve)), flush=True)
 
 
 if __name__ == '__main__':

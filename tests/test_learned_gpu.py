@@ -43,3 +43,20 @@ def test_trained_network_cuts_iterations_on_a_held_out_board(torch):
     assert learned[0]['relative_residual'] <= 1e-8
     assert np.abs(Zl - Zp).max() / np.abs(Zp).max() < 1e-6
     assert learned[0]['seconds'] < 0.5 * plain[0]['seconds']
+
+
+def test_training_on_boards_spanning_10k_to_40k_rwg(torch):
+    """Config 4's board range: three seeded plane pairs of about 10k, 22k and 40k RWG edges feed one
+    training run (the pairs generated by the solver itself); the projection loss falls."""
+    from pcbqs_amd import board as boards
+    from pcbqs_amd.learned import BoardData, train
+    data = []
+    for seed, target in ((201, 6800), (202, 15000), (203, 26500)):
+        print(f'c4 board seed={seed} target={target}', flush=True)
+        data.append(BoardData(boards.board_c2(seed, target=target), grid=8, freqs=(1e3, 1e6), keep_solver=False))
+    rwg = sorted(bd.rwg for bd in data)
+    print(f'c4 RWG per board: {rwg}', flush=True)
+    assert 10_000 <= rwg[0] and rwg[-1] <= 40_000 and rwg[-1] >= 30_000
+    net, hist = train(data, steps=120, log=None)
+    assert np.isfinite(hist).all()
+    assert np.mean(hist[-20:]) < 0.8 * np.mean(hist[:5]), (hist[:5], hist[-20:])

@@ -325,6 +325,26 @@ int pq_loops_solve_direct_multi(int count, pq_loops *const *handles, const int *
                                 int n_omega, const double *omegas, double tol, int max_steps, double *Zp,
                                 int64_t *steps, double *residual, int64_t *stats);
 
+/* The charge (PEEC) solve's electrostatic block: E = T^T P T / pscale for the
+ * tree branches (end nodes ta, tb, nt of them; T's columns are their incidence
+ * e_a - e_b), P_dev the node potential matrix (n_nodes square FP64, row stride
+ * ldp), into E_dev (nt x nt FP64, row stride lde). Replaces
+ * qform(T, P, T) / pscale of peec.prepare (src/pcbqs/peec.py:178-180). */
+int pq_peec_electro(int device, const double *P_dev, int64_t ldp, int n_nodes, int nt, const int32_t *ta,
+                    const int32_t *tb, double pscale, double *E_dev, int64_t lde);
+
+/* The charge solve at angular frequency omega > 0 on a handle whose n = n_free
+ * unknowns are nl loop currents then n - nl tree coordinates (its Ll the FP64
+ * K^T Lb K, its Rl the sparse K^T R K): Z = Rl + j omega Ll, zscale = max |Z|,
+ * S = Z / zscale on the first nl columns and j omega Z / pscale on the rest,
+ * E_dev (n - nl square, row stride lde) added to the tree block; S X = B / zscale
+ * by an FP64 LU with partial pivoting (rocSOLVER). B, X (n, nrhs) complex host,
+ * interleaved, row-major; *zscale returned. At most 40000 unknowns. Replaces
+ * peec_system with lu_factor_rows / lu_solve_rows of peec.solve
+ * (src/pcbqs/kernels.py:63-78, src/pcbqs/peec.py:236-265). */
+int pq_peec_solve(pq_loops *loops, int nl, const double *E_dev, int64_t lde, double pscale, double omega, int nrhs,
+                  const double *B_host, double *X_host, double *zscale);
+
 /* The free-loop right-hand sides B = -Z_fs (n_free, 2P) FP64 host of the port
  * loop columns at omega (the system pq_loops_solve_ports solves). Replaces the
  * right-hand side of direct_ports (src/pcbqs/mqs_ports.py:52-69). */

@@ -86,6 +86,8 @@ _SIGNATURES = {
     'pq_hmatrix_time': ([P, ctypes.c_int, P], ctypes.c_int),
     'pq_loops_matrix': ([P, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_fallbacks': ([P, P], ctypes.c_int),
+    'pq_peec_electro': ([ctypes.c_int, P, I64, ctypes.c_int, ctypes.c_int, P, P, DBL, P, I64], ctypes.c_int),
+    'pq_peec_solve': ([P, ctypes.c_int, P, I64, DBL, DBL, ctypes.c_int, P, P, P], ctypes.c_int),
     'pq_loops_solve_direct_multi': ([ctypes.c_int, P, P, P, ctypes.c_int, P, DBL, ctypes.c_int, P, P, P, P],
                                     ctypes.c_int),
     'pq_loops_solve_ports': ([P, DBL, ctypes.c_int, P, ctypes.c_int, DBL, ctypes.c_int, ctypes.c_int, P, P, P, P, P],

@@ -19,6 +19,10 @@ mesh is kept as the reference builds it (src/pcbqs/mqs.py:347-425):
   board_c2(seed)  a 60 x 60 mm plane pair with 8-32 circular cutouts (r 0.5-3 mm) and 16-64
                   vias (0.3 mm drill) at uniform positions, ~20k triangles (~30k RWG), 4 ports
   board_batch     uneven multi-layer boards for batched sweeps
+  board_plates    two square plates over each other with a current source between pads on them (the
+                  reference's charge-solve case, tests/test_peec.py:9-18)
+  board_strip     one rectangular sheet with a current source between pads at its two ends (a closed
+                  copper path, tests/test_peec.py:86-101)
 """
 from dataclasses import dataclass, field
 
@@ -309,3 +313,21 @@ def board_c5(seed=0, size=120e-3, n_vias=200, ports=8, gap=0.2e-3, target=67000)
         port_list.append((f'p{k}', f'p{k}+', f'p{k}-'))
     return b.build(terms, vias, port_list, config='c5', seed=seed, cell=h, gap=gap, layers=4,
                    pours={k: len(v) for k, v in pours.items()})
+
+
+def board_plates(h=0.5e-3, size=10e-3, cells=10, whole=False, pad=1e-3):
+    """Two size x size plates at z = 0 (layer B) and z = h (layer F), not joined by copper, with a
+    current source 'I' from a pad on F to one on B (the whole plates when `whole`)."""
+    b = _Builder()
+    for lay, z in ((0, 0.0), (1, h)):
+        b.add_layer(lay, *grid_sheet(0, 0, size, size, cells, cells), z=z)
+    box = (0, 0, size, size) if whole else (0, 0, pad, pad)
+    return b.build({'F': (1, box), 'B': (0, box)}, [], [('I', 'F', 'B')], config='plates', h=h)
+
+
+def board_strip(length=5e-3, width=2e-3, cells=(8, 3), pad=1e-3):
+    """One length x width sheet with a current source 'I' from a pad at x = 0 to one at x = length."""
+    b = _Builder()
+    b.add_layer(0, *grid_sheet(0, 0, length, width, *cells), z=0.0)
+    return b.build({'a': (0, (0, 0, pad, width)), 'b': (0, (length - pad, 0, length, width))}, [], [('I', 'a', 'b')],
+                   config='strip')

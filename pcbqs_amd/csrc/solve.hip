@@ -1356,6 +1356,72 @@ __global__ __launch_bounds__(256) void batch_port_kernel(const BatchMat *mats, i
     }
 }
 
+
+// ---- the charge (PEEC) solve's dense system ---------------------------------------------------
+
+// E[i][j] = (P[a_i][a_j] - P[a_i][b_j] - P[b_i][a_j] + P[b_i][b_j]) / pscale: T^T P T / pscale for
+// the tree branches' incidence columns e_a - e_b.
+__global__ __launch_bounds__(256) void peec_electro_kernel(const double *P, long long ldp, int nt, const int *ta,
+                                                          const int *tb, double inv, double *E, long long lde) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nt * nt) return;
+    const int i = (int)(e / nt), j = (int)(e % nt);
+    const int ai = ta[i], bi = tb[i], aj = ta[j], bj = tb[j];
+    const double v = (P[(size_t)ai * ldp + aj] - P[(size_t)ai * ldp + bj]) - (P[(size_t)bi * ldp + aj] - P[(size_t)bi * ldp + bj]);
+    E[(size_t)i * lde + j] = v * inv;
+}
+
+// S (column-major n x n complex) = Z = Rl + j w Ll over the first n loops of the handle (Z is
+// symmetric: S[j][i] = Z[i][j]); then |Z|'s largest entry into *mx (bit pattern of a nonnegative
+// double, compared as an integer).
+__global__ __launch_bounds__(256) void peec_form_kernel(const double *Ll, long long ldl, int n, double omega, double2 *S) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const size_t i = e / n, j = e % n;
+    S[j * n + i] = make_double2(0.0, omega * Ll[i * ldl + j]);
+}
+
+__global__ void peec_form_r_kernel(const int *ptr, const int *col, const double *val, int n, double2 *S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int t = ptr[i]; t < ptr[i + 1]; ++t)
+        if (col[t] < n) S[(size_t)col[t] * n + i].x += val[t];
+}
+
+__global__ __launch_bounds__(256) void peec_absmax_kernel(const double2 *S, size_t count, unsigned long long *mx) {
+    __shared__ double part[256];
+    double m = 0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        const double2 z = S[e];
+        m = fmax(m, sqrt(z.x * z.x + z.y * z.y));
+    }
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(part[0]));
+}
+
+// The scaled system (src/pcbqs/kernels.py:63-78): columns j < nl divided by zscale, the others
+// multiplied by j w / pscale; E (the tree block's T^T P T / pscale) added where i, j >= nl.
+__global__ __launch_bounds__(256) void peec_scale_kernel(double2 *S, int n, int nl, double zscale, double omega,
+                                                        double pscale, const double *E, long long lde) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const size_t j = e / n, i = e % n;                // column-major
+    double2 z = S[e];
+    if ((int)j < nl) {
+        z.x /= zscale; z.y /= zscale;
+    } else {
+        const double f = omega / pscale;               // (x + j y) j f = -f y + j f x
+        z = make_double2(-f * z.y, f * z.x);
+        if ((int)i >= nl) z.x += E[(i - nl) * lde + (j - nl)];
+    }
+    S[e] = z;
+}
+
 }  // namespace pq
 
 using namespace pq;
@@ -3283,6 +3349,99 @@ int pq_loops_solve_direct_multi(int count, pq_loops *const *handles, const int *
             g0 = g1;
         }
         if (stats) { stats[0] = launches; stats[1] = redone; }
+    });
+}
+
+// T^T P T / pscale of the tree branches (ends ta, tb: host int32, nt of them) into E_dev (nt x nt
+// FP64, row stride lde), P_dev the node potential matrix (n_nodes square, row stride ldp): the charge
+// solve's electrostatic block (src/pcbqs/peec.py:178-180).
+int pq_peec_electro(int device, const double *P_dev, int64_t ldp, int n_nodes, int nt, const int32_t *ta,
+                    const int32_t *tb, double pscale, double *E_dev, int64_t lde) {
+    return guard([&] {
+        PQ_REQUIRE(P_dev && E_dev && (nt == 0 || (ta && tb)), "pq_peec_electro: null argument");
+        PQ_REQUIRE(nt >= 0 && n_nodes > 0 && ldp >= n_nodes && lde >= nt && pscale > 0,
+                   "pq_peec_electro: invalid sizes");
+        for (int i = 0; i < nt; ++i)
+            PQ_REQUIRE(ta[i] >= 0 && ta[i] < n_nodes && tb[i] >= 0 && tb[i] < n_nodes,
+                       "pq_peec_electro: tree branch end out of range");
+        if (nt == 0) return;
+        DeviceScope scope(device);
+        DevBuf a, b;
+        a.reserve((size_t)nt * 4); b.reserve((size_t)nt * 4);
+        PQ_HIP(hipMemcpy(a.p, ta, (size_t)nt * 4, hipMemcpyHostToDevice));
+        PQ_HIP(hipMemcpy(b.p, tb, (size_t)nt * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(peec_electro_kernel, dim3((unsigned)(((size_t)nt * nt + 255) / 256)), dim3(256), 0, nullptr,
+                           P_dev, (long long)ldp, nt, a.as<int>(), b.as<int>(), 1.0 / pscale, E_dev, (long long)lde);
+        PQ_HIP(hipGetLastError());
+        PQ_HIP(hipDeviceSynchronize());
+    });
+}
+
+// The charge solve at one frequency (src/pcbqs/peec.py:236-265 with kernels.py:63-78): on the
+// handle's n = n_free unknowns (nl loop currents, then the n - nl tree coordinates), Z = Rl + j w
+// Ll (FP64), zscale = max |Z|, the system S = Z / zscale on the first nl columns and j w Z / pscale
+// on the others, with E (n - nl square, device) added to the tree block; S X = B / zscale solved by
+// an FP64 LU with partial pivoting (rocSOLVER zgetrf / zgetrs), as the reference's
+// lu_factor_rows / lu_solve_rows. B, X: (n, nrhs) complex host, interleaved, row-major.
+int pq_peec_solve(pq_loops *L, int nl, const double *E_dev, int64_t lde, double pscale, double omega, int nrhs,
+                  const double *B_host, double *X_host, double *zscale_out) {
+    return guard([&] {
+        PQ_REQUIRE(L && B_host && X_host && (E_dev || L->nf == nl), "pq_peec_solve: null argument");
+        const int n = L->nf;
+        PQ_REQUIRE(nl >= 0 && nl <= n && nrhs >= 1 && pscale > 0 && omega > 0 && std::isfinite(omega) &&
+                       lde >= n - nl, "pq_peec_solve: invalid arguments");
+        PQ_REQUIRE(n <= 40000, "pq_peec_solve: at most 40000 unknowns (the dense system)");
+        DeviceScope scope(L->device);
+        if (n == 0) return;
+        const RocLib &lu = roc_lib();
+        rocblas_handle h = blas_handle(L);
+        DevBuf S, piv, info, Bc, mx;
+        S.reserve((size_t)n * n * 16);
+        piv.reserve((size_t)n * 4); info.reserve(4); mx.reserve(8);
+        Bc.reserve((size_t)n * nrhs * 16);
+        const size_t cnt = (size_t)n * n;
+        const unsigned g = (unsigned)((cnt + 255) / 256);
+        hipLaunchKernelGGL(peec_form_kernel, dim3(g), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl, n, omega,
+                           S.as<double2>());
+        hipLaunchKernelGGL(peec_form_r_kernel, dim3((n + 255) / 256), dim3(256), 0, L->stream, L->rptr.as<int>(),
+                           L->rcol.as<int>(), L->rval.as<double>(), n, S.as<double2>());
+        PQ_HIP(hipMemsetAsync(mx.p, 0, 8, L->stream));
+        hipLaunchKernelGGL(peec_absmax_kernel, dim3(1024), dim3(256), 0, L->stream, S.as<double2>(), cnt,
+                           mx.as<unsigned long long>());
+        PQ_HIP(hipGetLastError());
+        double zscale = 0;
+        PQ_HIP(hipMemcpyAsync(&zscale, mx.p, 8, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        zscale = std::max(zscale, std::numeric_limits<double>::min());
+        hipLaunchKernelGGL(peec_scale_kernel, dim3(g), dim3(256), 0, L->stream, S.as<double2>(), n, nl, zscale, omega,
+                           pscale, E_dev, (long long)lde);
+        PQ_HIP(hipGetLastError());
+        {   // B / zscale, column-major
+            std::vector<double> bc((size_t)n * nrhs * 2);
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < nrhs; ++c) {
+                    bc[2 * ((size_t)c * n + i)] = B_host[2 * ((size_t)i * nrhs + c)] / zscale;
+                    bc[2 * ((size_t)c * n + i) + 1] = B_host[2 * ((size_t)i * nrhs + c) + 1] / zscale;
+                }
+            PQ_HIP(hipMemcpyAsync(Bc.p, bc.data(), bc.size() * 8, hipMemcpyHostToDevice, L->stream));
+            auto *A = reinterpret_cast<rocblas_double_complex *>(S.p);
+            blas_check(lu.getrf(h, n, n, A, n, piv.as<rocblas_int>(), info.as<rocblas_int>()), "rocsolver_zgetrf");
+            rocblas_int bad = 0;
+            PQ_HIP(hipMemcpyAsync(&bad, info.p, 4, hipMemcpyDeviceToHost, L->stream));
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            PQ_REQUIRE(bad == 0, "pq_peec_solve: the charge system is singular");
+            blas_check(lu.getrs(h, rocblas_operation_none, n, nrhs, A, n, piv.as<rocblas_int>(),
+                                reinterpret_cast<rocblas_double_complex *>(Bc.p), n),
+                       "rocsolver_zgetrs");
+            PQ_HIP(hipMemcpyAsync(bc.data(), Bc.p, bc.size() * 8, hipMemcpyDeviceToHost, L->stream));
+            PQ_HIP(hipStreamSynchronize(L->stream));
+            for (int i = 0; i < n; ++i)
+                for (int c = 0; c < nrhs; ++c) {
+                    X_host[2 * ((size_t)i * nrhs + c)] = bc[2 * ((size_t)c * n + i)];
+                    X_host[2 * ((size_t)i * nrhs + c) + 1] = bc[2 * ((size_t)c * n + i) + 1];
+                }
+        }
+        if (zscale_out) *zscale_out = zscale;
     });
 }
 

@@ -446,6 +446,47 @@ int pq_hmatrix_apply_device(pq_hmatrix *h, int64_t k, const void *x_dev, void *y
  * pq_hmatrix_apply (HIP events). Measurement only: no counterpart in the reference. */
 int pq_hmatrix_time(pq_hmatrix *h, int reps, double *ms);
 
+/* ---- construction of the compressed RWG inductance on the device --------------
+ * An entry provider over a prepared fill geometry: ne interior edges with
+ * support (ne, 2) plus/minus triangles, vertices (ne, 2, 2) their free
+ * vertices and alpha (ne, 2) = (1/(2 a+), -1/(2 a-)), in the geometry's
+ * (centred) frame; entries are scale * sum of the four support pairs'
+ * canonical moments (near rule `near`, `outer_levels`). Replaces
+ * RwgEntries (src/pcbqs/core/hmatrix_build.hpp:27-43) and the provider
+ * set-up of edge_inductance (src/pcbqs/compressed.py:61-73). The provider
+ * also holds every block built through it until it is destroyed. */
+typedef struct pq_entries pq_entries;
+int pq_rwg_entries_create(pq_fill *f, int64_t ne, const int32_t *support, const double *vertices,
+                          const double *alpha, double scale, double near, int outer_levels, pq_entries **out);
+int pq_entries_destroy(pq_entries *e);
+/* out (nr, nc) row-major FP64 host = entries (rows[i], cols[j]). Replaces
+ * rwg_inductance (src/pcbqs/core/compressed_kernels.hpp:127-144). */
+int pq_entries_block(pq_entries *e, const int32_t *rows, int64_t nr, const int32_t *cols, int64_t nc, double *out);
+/* Checked ACA of nblocks admissible blocks (rows[row_ptr[q]:row_ptr[q+1]] x
+ * cols[col_ptr[q]:col_ptr[q+1]]) to relative tolerance tol, all on the
+ * device. ids[q] = the kept block's id (>= 0) and ranks[q] its rank for a
+ * block that converged, -1 for one that did not (the reference's None);
+ * *entries (optional) = entries evaluated. Factors stay on the device.
+ * Replaces aca_blocks / aca (src/pcbqs/core/hmatrix_build.hpp:143-327). */
+int pq_aca_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const int32_t *rows,
+                  const int64_t *col_ptr, const int32_t *cols, double tol, int64_t *ids, int64_t *ranks,
+                  int64_t *entries);
+/* Dense leaves, same[q] != 0 marking a diagonal block (symmetrised,
+ * (B + B^T) / 2). ids[q] = the kept block's id. Replaces dense_blocks
+ * (hmatrix_build.hpp:329-352) and the symmetrisation of hmatrix.py:204-206. */
+int pq_dense_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const int32_t *rows,
+                    const int64_t *col_ptr, const int32_t *cols, const int32_t *same, int64_t *ids);
+/* Host copies of kept block `id`: dense (m x n) into u, or U (m x r) and V
+ * (r x n), row-major. */
+int pq_entries_factors(pq_entries *e, int64_t id, double *u, double *v);
+/* The product handle of the kept blocks ids[0..nblocks) in that order (the
+ * HMatrix block list, hmatrix.py:80-83), factors gathered on the device. */
+int pq_entries_hmatrix(pq_entries *e, int64_t nblocks, const int64_t *ids, pq_hmatrix **out);
+/* Entries evaluated, ACA blocks run again with the worst-case check room,
+ * factor bytes held, and device milliseconds of the ACA and dense launches
+ * (ms[0], ms[1]). Measurement only: no counterpart in the reference. */
+int pq_entries_stats(pq_entries *e, int64_t *entries, int64_t *reruns, int64_t *bytes, double *ms);
+
 /* ---- Biot-Savart field --------------------------------------------------------
  * H (np, 3, K, 2) FP64 host (real, imaginary) at points pts (np, 3) of ns sheet
  * sources src (ns, 3 + 4K: position, then Jx re, Jx im, Jy re, Jy im per

@@ -84,6 +84,14 @@ _SIGNATURES = {
     'pq_hmatrix_apply': ([P, I64, P, P], ctypes.c_int),
     'pq_hmatrix_apply_device': ([P, I64, P, P, P], ctypes.c_int),
     'pq_hmatrix_time': ([P, ctypes.c_int, P], ctypes.c_int),
+    'pq_rwg_entries_create': ([P, I64, P, P, P, DBL, DBL, ctypes.c_int, P], ctypes.c_int),
+    'pq_entries_destroy': ([P], ctypes.c_int),
+    'pq_entries_block': ([P, P, I64, P, I64, P], ctypes.c_int),
+    'pq_aca_blocks': ([P, I64, P, P, P, P, DBL, P, P, P], ctypes.c_int),
+    'pq_dense_blocks': ([P, I64, P, P, P, P, P, P], ctypes.c_int),
+    'pq_entries_factors': ([P, I64, P, P], ctypes.c_int),
+    'pq_entries_hmatrix': ([P, I64, P, P], ctypes.c_int),
+    'pq_entries_stats': ([P, P, P, P, P], ctypes.c_int),
     'pq_loops_matrix': ([P, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_last_fallbacks': ([P, P], ctypes.c_int),
     'pq_peec_electro': ([ctypes.c_int, P, I64, ctypes.c_int, ctypes.c_int, P, P, DBL, P, I64], ctypes.c_int),

@@ -20,6 +20,7 @@
 // Classification is the reference rule evaluated in FP64 without contraction; an FP32
 // prefilter with a 1e-5 relative margin settles the clearly-far pairs without it.
 #include "pq_common.h"
+#include "pair.h"
 #include <mutex>
 #include <cstdlib>
 #include <cmath>
@@ -32,178 +33,12 @@ constexpr int WAVE = 64;
 constexpr int FAR_ROWS_PER_WAVE = 4;
 constexpr int FAR_ROWS_PER_BLOCK = 4 * FAR_ROWS_PER_WAVE;  // 4 waves per 256-thread block
 
-// Device view of one prepared geometry.
-struct GeomDev {
-    int nt;
-    const double *V, *L, *tau, *un, *area, *P3, *z, *cent, *radius;
-    const double *qp[3];
-    const double *qw[3];
-    int nq[3];
-    // FP32 far records, 2 float4 per triangle: (x0,y0,x1,y1), (x2,y2,area,radius) with the far
-    // points taken relative to the triangle's own centroid.
-    const float4 *far32;
-};
-
 struct NearLists {
     unsigned long long *list[2];   // packed (row << 32) | col entries
     unsigned long long *count;     // [0],[1]: list lengths; [2]: analytic pairs with row <= col
     unsigned long long cap;
 };
 
-// ---- classification ----------------------------------------------------------------------
-
-// The reference rule (static_kernels.hpp:130-136): every product rounded, then summed. It is
-// exactly symmetric in its two triangles (differences are squared, sums commute), so entry (r,c)
-// and (c,r) classify alike without choosing the canonical order.
-__device__ __noinline__ int exact_kind(double ax, double ay, double az, double arad, double bx, double by,
-                                       double bz, double brad, double near, int all_near, int outer) {
-#pragma clang fp contract(off)
-    const double dx = ax - bx, dy = ay - by;
-    const double d = az - bz;
-    const double dist = __builtin_sqrt((dx * dx + dy * dy) + d * d);
-    const double size = arad + brad;
-    if (!all_near && !(dist <= near * size)) return -1;
-    return dist <= size ? outer : 0;
-}
-
-// ---- per-pair moment functions -----------------------------------------------------------
-
-// 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps y <- y + y (1 - x y^2) / 2,
-// each of which squares the relative error; the result is within a few ulp of the correctly
-// rounded value, at about half the instructions of a correctly rounded square root and division.
-__device__ __forceinline__ double inv_sqrt64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const double e = __builtin_fma(-(x * y), y, 1.0);
-        y = __builtin_fma(0.5 * y, e, y);
-    }
-    return y;
-}
-
-// FP64 far 3x3 product rule, reference arithmetic order (static_kernels.hpp:57-72); w / sqrt(r^2)
-// taken as w * inv_sqrt64(r^2).
-template <bool SCALAR>
-__device__ __forceinline__ void far64(const double *pa, const double *pb, double w, double d, double *m) {
-#pragma unroll
-    for (int j = 0; j < 6; ++j) m[j] = 0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const double x = pa[2 * i], y = pa[2 * i + 1];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double sx = pb[2 * j], sy = pb[2 * j + 1];
-            const double dx = x - sx, dy = y - sy;
-            const double gg = w * inv_sqrt64(dx * dx + dy * dy + d * d);
-            m[0] += gg;
-            if (!SCALAR) {
-                m[1] += gg * x; m[2] += gg * y; m[3] += gg * sx; m[4] += gg * sy;
-                m[5] += gg * (x * sx + y * sy);
-            }
-        }
-    }
-}
-
-// The closed-form inner integrals (Wilton, Rao et al., IEEE TAP 1984) of 1/R and (r'-rho)/R over
-// source triangle q at observation point (x, y), height offset d (static_kernels.hpp:78-97).
-// Edge e runs from vertex e to vertex e + 1, so its end-point distances are the vertex distances
-// R_e and R_{e+1}: three square roots per point, not six (nor a seventh for r0, whose only use is
-// the guard, taken on r0^2).
-template <bool SCALAR>
-__device__ __forceinline__ void inner_integrals(const GeomDev &g, int q, double x, double y, double d,
-                                                double &I1, double &IRx, double &IRy) {
-    const double ad = fabs(d), d2 = d * d;
-    I1 = 0; IRx = 0; IRy = 0;
-    const double *Vq = g.V + 6 * (size_t)q, *Tq = g.tau + 6 * (size_t)q, *Nq = g.un + 6 * (size_t)q;
-    const double *Lq = g.L + 3 * (size_t)q;
-    double vx[3], vy[3], R[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        vx[k] = Vq[2 * k] - x;
-        vy[k] = Vq[2 * k + 1] - y;
-        R[k] = sqrt(vx[k] * vx[k] + vy[k] * vy[k] + d2);
-    }
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        const int f = e == 2 ? 0 : e + 1;
-        const double sm = vx[e] * Tq[2 * e] + vy[e] * Tq[2 * e + 1];
-        const double sp = sm + Lq[e];
-        const double t0 = vx[e] * Nq[2 * e] + vy[e] * Nq[2 * e + 1];
-        const double r02 = t0 * t0 + d2;
-        const double rm = R[e], rp = R[f];
-        // asinh(s / r0) = log(A(s)) - log(r0) with A(s) = |s| + R for s >= 0 and r0^2 / (|s| + R)
-        // for s < 0 (the same number without cancellation); sp > sm, so the difference of the two
-        // asinh is log(num / den) with one of three (num, den) pairs
-        double lg = 0.0;
-        const double g0 = 1e-14 * Lq[e];
-        if (r02 > g0 * g0) {
-            const double bp = fabs(sp) + rp, bm = fabs(sm) + rm;
-            const bool pneg = sp < 0, mneg = sm < 0;
-            const double num = pneg ? bm : (mneg ? bp * bm : bp);
-            const double den = pneg ? bp : (mneg ? r02 : bm);
-            lg = log(num / den);
-        }
-        double term = t0 * lg;
-        if (ad > 0) {
-            // atan2(y1, x1) - atan2(y2, x2) with x1, x2 > 0: both angles lie in (-pi/2, pi/2), so the
-            // difference lies in (-pi, pi) and is the angle of (x1 x2 + y1 y2, y1 x2 - y2 x1)
-            const double y1 = t0 * sp, x1 = r02 + ad * rp, y2 = t0 * sm, x2 = r02 + ad * rm;
-            term -= ad * atan2(y1 * x2 - y2 * x1, x1 * x2 + y1 * y2);
-        }
-        I1 += term;
-        if (!SCALAR) {
-            const double er = .5 * (r02 * lg + sp * rp - sm * rm);
-            IRx += er * Nq[2 * e]; IRy += er * Nq[2 * e + 1];
-        }
-    }
-}
-
-// One observation point's contribution to the six moments of canonical pair (a, b).
-// LOCAL: first moments about each triangle's own centroid (FP32 planes' frame).
-template <bool SCALAR, bool LOCAL>
-__device__ __forceinline__ void near_point(const GeomDev &g, int a, int b, int level, int j, double *m) {
-    const int nq = g.nq[level];
-    const double *pt = g.qp[level] + 2 * ((size_t)a * nq + j);
-    const double x = pt[0], y = pt[1];
-    const double d = g.z[a] - g.z[b];
-    double I1, IRx, IRy;
-    inner_integrals<SCALAR>(g, b, x, y, d, I1, IRx, IRy);
-    const double w = g.area[a] * g.qw[level][j];
-    m[0] = w * I1;
-    if (SCALAR) return;
-    double xo = x, yo = y, xs = x, ys = y;
-    if (LOCAL) {
-        xo = x - g.cent[2 * a]; yo = y - g.cent[2 * a + 1];
-        xs = x - g.cent[2 * b]; ys = y - g.cent[2 * b + 1];
-    }
-    const double sx = IRx + xs * I1, sy = IRy + ys * I1;
-    m[1] = w * xo * I1; m[2] = w * yo * I1;
-    m[3] = w * sx; m[4] = w * sy; m[5] = w * (xo * sx + yo * sy);
-}
-
-// Full FP64 moments of one ordered pair (reference static_pair), for explicit pair lists.
-template <bool SCALAR>
-__device__ void pair64(const GeomDev &g, int p, int q, double d, int k, double *m) {
-    if (k < 0) {
-        far64<SCALAR>(g.P3 + 6 * (size_t)p, g.P3 + 6 * (size_t)q, g.area[p] * g.area[q] / 9, d, m);
-        return;
-    }
-    for (int j = 0; j < 6; ++j) m[j] = 0;
-    const int nq = g.nq[k];
-    for (int j = 0; j < nq; ++j) {
-        const double *pt = g.qp[k] + 2 * ((size_t)p * nq + j);
-        const double x = pt[0], y = pt[1];
-        double I1, IRx, IRy;
-        inner_integrals<SCALAR>(g, q, x, y, d, I1, IRx, IRy);
-        const double w = g.area[p] * g.qw[k][j];
-        m[0] += w * I1;
-        if (!SCALAR) {
-            const double sx = IRx + x * I1, sy = IRy + y * I1;
-            m[1] += w * x * I1; m[2] += w * y * I1;
-            m[3] += w * sx; m[4] += w * sy; m[5] += w * (x * sx + y * sy);
-        }
-    }
-}
 
 // Entry (r, c) of the planes from the canonical pair's moments (static_cupy_kernels.py:106-115
 // contract; diagonal: the mean of the two first moments, static_kernels.hpp:230-236).
@@ -777,19 +612,6 @@ __global__ void far32_records(GeomDev g, float4 *rec) {
 // ---- handle ---------------------------------------------------------------------------------
 
 using namespace pq;
-
-struct pq_fill {
-    int device = 0;
-    int nt = 0;
-    GeomDev g{};
-    DevBuf geom, far32, lists[2], counts, flag;
-    unsigned long long cap = 0;
-    hipStream_t stream = nullptr, copy_stream = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    double far_ms = 0, near_ms = 0;
-    long long near_pairs[2] = {0, 0};
-    int cus = 256;
-};
 
 // ---- the allocation-free explicit pair entry (pq_pairs_noalloc) -------------------------------
 // Its staging buffers live in this file-scope device array instead of device memory allocated per

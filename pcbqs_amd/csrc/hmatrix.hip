@@ -19,6 +19,7 @@
 // than 256 output rows. X and Y are row-major (n x k); k is processed in groups of up to 8 real or
 // 4 complex columns, so the slot buffer holds one group.
 #include "pq_common.h"
+#include "hmatrix.h"
 
 #include <algorithm>
 #include <vector>
@@ -310,95 +311,105 @@ void apply_device(pq_hmatrix *h, long long k, const void *x, void *y, hipStream_
 
 using namespace pq;
 
+pq_hmatrix *pq::hmatrix_make(int device, int64_t n, int64_t n_blocks, const int64_t *meta, const int32_t *idx,
+                             int64_t n_idx, int64_t n_factor, bool is_complex,
+                             const std::function<void(void *)> &upload) {
+    PQ_REQUIRE(n >= 0 && n < (1LL << 31) && n_blocks >= 0 && n_blocks < (1LL << 31) && n_idx >= 0 &&
+                   n_factor >= 0,
+               "pq_hmatrix_create: invalid sizes");
+    std::vector<HBlock> hb((size_t)n_blocks);
+    std::vector<int> dense, lowrank;
+    long long so = 0, fo = 0;
+    for (long long q = 0; q < n_blocks; ++q) {
+        const int64_t *m = meta + 6 * q;
+        HBlock &b = hb[(size_t)q];
+        b.a_off = m[0]; b.la = m[1]; b.b_off = m[2]; b.lb = m[3]; b.rank = m[4]; b.same = m[5] != 0;
+        PQ_REQUIRE(b.la >= 0 && b.lb >= 0 && b.a_off >= 0 && b.b_off >= 0 && b.a_off + b.la <= n_idx &&
+                       b.b_off + b.lb <= n_idx && b.la < (1LL << 31) && b.lb < (1LL << 31),
+                   "pq_hmatrix_create: block index range outside the index array");
+        PQ_REQUIRE(b.rank >= -1 && b.rank <= HM_RMAX,
+                   "pq_hmatrix_create: block rank must be -1 (dense) or 0..256");
+        b.fwd = so;
+        so += b.la;
+        b.tr = b.same ? -1 : so;
+        if (!b.same) so += b.lb;
+        b.u_off = fo;
+        if (b.rank < 0) {
+            fo += b.la * b.lb;
+            b.v_off = -1;
+            dense.push_back((int)q);
+        } else {
+            fo += b.la * b.rank;
+            b.v_off = fo;
+            fo += b.rank * b.lb;
+            lowrank.push_back((int)q);
+        }
+    }
+    PQ_REQUIRE(fo == n_factor, "pq_hmatrix_create: factor array length does not match the blocks");
+    for (int64_t t = 0; t < n_idx; ++t)
+        PQ_REQUIRE(idx[t] >= 0 && idx[t] < n, "pq_hmatrix_create: row or column index out of range");
+    // slots of each output row, in slot order (a stable counting sort of the slots' targets)
+    std::vector<long long> rp((size_t)n + 1, 0), sl((size_t)so);
+    for (const HBlock &b : hb) {
+        for (long long i = 0; i < b.la; ++i) ++rp[(size_t)idx[b.a_off + i] + 1];
+        if (!b.same)
+            for (long long j = 0; j < b.lb; ++j) ++rp[(size_t)idx[b.b_off + j] + 1];
+    }
+    for (long long r = 0; r < n; ++r) rp[(size_t)r + 1] += rp[(size_t)r];
+    {
+        std::vector<long long> next(rp.begin(), rp.end() - 1);
+        for (const HBlock &b : hb) {
+            for (long long i = 0; i < b.la; ++i) sl[(size_t)next[(size_t)idx[b.a_off + i]]++] = b.fwd + i;
+            if (!b.same)
+                for (long long j = 0; j < b.lb; ++j) sl[(size_t)next[(size_t)idx[b.b_off + j]]++] = b.tr + j;
+        }
+    }
+    DeviceScope scope(device);
+    auto *h = new pq_hmatrix();
+    try {
+        h->device = device; h->n = n; h->nslots = so; h->n_factor = n_factor; h->complex_ = is_complex;
+        h->n_dense = (int)dense.size(); h->n_lowrank = (int)lowrank.size();
+        for (const HBlock &b : hb) {
+            const long long side = std::max(b.la, b.lb);
+            if (b.rank < 0) h->side_dense = std::max(h->side_dense, side);
+            else {
+                h->side_lr = std::max(h->side_lr, side);
+                h->rmax = std::max(h->rmax, b.rank);
+            }
+        }
+        const size_t es = h->complex_ ? 16 : 8;
+        PQ_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        auto up = [&](DevBuf &d, const void *src, size_t bytes) {
+            d.reserve(std::max<size_t>(bytes, 16));
+            if (bytes) PQ_HIP(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+        };
+        up(h->blocks, hb.data(), hb.size() * sizeof(HBlock));
+        up(h->idx, idx, (size_t)n_idx * 4);
+        up(h->rp, rp.data(), rp.size() * 8);
+        up(h->sl, sl.data(), sl.size() * 8);
+        h->F.reserve(std::max<size_t>((size_t)n_factor * es, 16));
+        if (n_factor) upload(h->F.p);
+        up(h->dense_list, dense.data(), dense.size() * 4);
+        up(h->lr_list, lowrank.data(), lowrank.size() * 4);
+        h->s.reserve(std::max<size_t>((size_t)so * 8 * 8, 16));   // one column group: 8 real or 4 complex
+    } catch (...) {
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        throw;
+    }
+    return h;
+}
+
 extern "C" {
 
 int pq_hmatrix_create(int device, int64_t n, int64_t n_blocks, const int64_t *meta, const int32_t *idx, int64_t n_idx,
                       const void *factors, int64_t n_factor, int is_complex, pq_hmatrix **out) {
     return guard([&] {
         PQ_REQUIRE(out && (n_blocks == 0 || (meta && idx && factors)), "pq_hmatrix_create: null argument");
-        PQ_REQUIRE(n >= 0 && n < (1LL << 31) && n_blocks >= 0 && n_blocks < (1LL << 31) && n_idx >= 0 &&
-                       n_factor >= 0,
-                   "pq_hmatrix_create: invalid sizes");
-        std::vector<HBlock> hb((size_t)n_blocks);
-        std::vector<int> dense, lowrank;
-        long long so = 0, fo = 0;
-        for (long long q = 0; q < n_blocks; ++q) {
-            const int64_t *m = meta + 6 * q;
-            HBlock &b = hb[(size_t)q];
-            b.a_off = m[0]; b.la = m[1]; b.b_off = m[2]; b.lb = m[3]; b.rank = m[4]; b.same = m[5] != 0;
-            PQ_REQUIRE(b.la >= 0 && b.lb >= 0 && b.a_off >= 0 && b.b_off >= 0 && b.a_off + b.la <= n_idx &&
-                           b.b_off + b.lb <= n_idx && b.la < (1LL << 31) && b.lb < (1LL << 31),
-                       "pq_hmatrix_create: block index range outside the index array");
-            PQ_REQUIRE(b.rank >= -1 && b.rank <= HM_RMAX,
-                       "pq_hmatrix_create: block rank must be -1 (dense) or 0..256");
-            b.fwd = so;
-            so += b.la;
-            b.tr = b.same ? -1 : so;
-            if (!b.same) so += b.lb;
-            b.u_off = fo;
-            if (b.rank < 0) {
-                fo += b.la * b.lb;
-                b.v_off = -1;
-                dense.push_back((int)q);
-            } else {
-                fo += b.la * b.rank;
-                b.v_off = fo;
-                fo += b.rank * b.lb;
-                lowrank.push_back((int)q);
-            }
-        }
-        PQ_REQUIRE(fo == n_factor, "pq_hmatrix_create: factor array length does not match the blocks");
-        for (int64_t t = 0; t < n_idx; ++t)
-            PQ_REQUIRE(idx[t] >= 0 && idx[t] < n, "pq_hmatrix_create: row or column index out of range");
-        // slots of each output row, in slot order (a stable counting sort of the slots' targets)
-        std::vector<long long> rp((size_t)n + 1, 0), sl((size_t)so);
-        for (const HBlock &b : hb) {
-            for (long long i = 0; i < b.la; ++i) ++rp[(size_t)idx[b.a_off + i] + 1];
-            if (!b.same)
-                for (long long j = 0; j < b.lb; ++j) ++rp[(size_t)idx[b.b_off + j] + 1];
-        }
-        for (long long r = 0; r < n; ++r) rp[(size_t)r + 1] += rp[(size_t)r];
-        {
-            std::vector<long long> next(rp.begin(), rp.end() - 1);
-            for (const HBlock &b : hb) {
-                for (long long i = 0; i < b.la; ++i) sl[(size_t)next[(size_t)idx[b.a_off + i]]++] = b.fwd + i;
-                if (!b.same)
-                    for (long long j = 0; j < b.lb; ++j) sl[(size_t)next[(size_t)idx[b.b_off + j]]++] = b.tr + j;
-            }
-        }
-        DeviceScope scope(device);
-        auto *h = new pq_hmatrix();
-        try {
-            h->device = device; h->n = n; h->nslots = so; h->n_factor = n_factor; h->complex_ = is_complex != 0;
-            h->n_dense = (int)dense.size(); h->n_lowrank = (int)lowrank.size();
-            for (const HBlock &b : hb) {
-                const long long side = std::max(b.la, b.lb);
-                if (b.rank < 0) h->side_dense = std::max(h->side_dense, side);
-                else {
-                    h->side_lr = std::max(h->side_lr, side);
-                    h->rmax = std::max(h->rmax, b.rank);
-                }
-            }
-            const size_t es = h->complex_ ? 16 : 8;
-            PQ_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-            auto up = [&](DevBuf &d, const void *src, size_t bytes) {
-                d.reserve(std::max<size_t>(bytes, 16));
-                if (bytes) PQ_HIP(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
-            };
-            up(h->blocks, hb.data(), hb.size() * sizeof(HBlock));
-            up(h->idx, idx, (size_t)n_idx * 4);
-            up(h->rp, rp.data(), rp.size() * 8);
-            up(h->sl, sl.data(), sl.size() * 8);
-            up(h->F, factors, (size_t)n_factor * es);
-            up(h->dense_list, dense.data(), dense.size() * 4);
-            up(h->lr_list, lowrank.data(), lowrank.size() * 4);
-            h->s.reserve(std::max<size_t>((size_t)so * 8 * 8, 16));   // one column group: 8 real or 4 complex
-        } catch (...) {
-            if (h->stream) (void)hipStreamDestroy(h->stream);
-            delete h;
-            throw;
-        }
-        *out = h;
+        const size_t es = is_complex ? 16 : 8;
+        *out = hmatrix_make(device, n, n_blocks, meta, idx, n_idx, n_factor, is_complex != 0, [&](void *dst) {
+            PQ_HIP(hipMemcpy(dst, factors, (size_t)n_factor * es, hipMemcpyHostToDevice));
+        });
     });
 }
 

@@ -7,6 +7,13 @@ hmatrix_product (src/pcbqs/hmatrix.py:223-297, src/pcbqs/core/hmatrix_kernels.hp
 device (pq_hmatrix_apply). As in the reference, a real operator applies to complex vectors as
 independent real and imaginary columns, and the adjoint of the transpose-symmetric operator is
 conj(A conj(x)). Low-rank blocks are limited to rank 256 (the reference's ACA stops at 128).
+
+`RwgEntries(nodes, tris, z)` and `build(entries)` construct the compressed RWG inductance itself on
+the device, without a dense matrix: the entry provider of edge_inductance (src/pcbqs/compressed.py:
+59-73), HMatrix's level-by-level native construction (src/pcbqs/hmatrix.py:88-95,179-210) with its
+tree and admissibility on the host, and the checked ACA and dense leaves of aca_blocks /
+dense_blocks (src/pcbqs/core/hmatrix_build.hpp:143-352) as device kernels (pq_aca_blocks,
+pq_dense_blocks) whose factors stay in HBM and become the product handle directly.
 """
 import argparse
 import ctypes
@@ -69,10 +76,10 @@ def _low_rank(block, tol, rng):
 
 def compress(entry, lo, hi, tol=1e-6, leaf=64, eta=1.0, dtype=float, seed=0):
     """Blocks (a, b, U, V, same) of a transpose-symmetric matrix given by entry(rows, cols), on the
-    reference's cluster tree and admissibility rule (src/pcbqs/hmatrix.py:88-95, 163-181): far
+    reference's cluster tree and admissibility rule (src/pcbqs/hmatrix.py:88-95, 152-177): far
     blocks (gap > eta * diameter) low rank when that is economical, diagonal leaves symmetrised,
     everything else subdivided down to dense leaf pairs. The factors come from truncated SVDs to
-    the relative tolerance, where the reference runs checked partial-pivot ACA (hmatrix.py:104-160).
+    the relative tolerance, where the reference runs checked partial-pivot ACA (hmatrix.py:103-150).
     A host-side construction, as the reference's; the product is the device part."""
     lo, hi = np.asarray(lo, float), np.asarray(hi, float)
     rng = np.random.default_rng(seed)
@@ -85,9 +92,7 @@ def compress(entry, lo, hi, tol=1e-6, leaf=64, eta=1.0, dtype=float, seed=0):
     while pending:
         a, b = pending.pop()
         same = a is b
-        gap = np.linalg.norm(np.maximum(0, np.maximum(a.lo - b.hi, b.lo - a.hi)))
-        diameter = max(np.linalg.norm(a.hi - a.lo), np.linalg.norm(b.hi - b.lo))
-        if not same and gap > eta * diameter:
+        if _admissible(a, b, eta):
             f = _low_rank(np.asarray(entry(a.ids, b.ids), dtype), tol, rng)
             if f is not None:
                 blocks.append((a.ids, b.ids, f[0], f[1], False))
@@ -98,15 +103,25 @@ def compress(entry, lo, hi, tol=1e-6, leaf=64, eta=1.0, dtype=float, seed=0):
                 block = (block + block.T) / 2
             blocks.append((a.ids, b.ids, np.ascontiguousarray(block), None, same))
             continue
-        if same:
-            x, y = a.children
-            nxt = [(x, x), (x, y), (y, y)]
-        elif a.children and (not b.children or len(a.ids) >= len(b.ids)):
-            nxt = [(c, b) for c in a.children]
-        else:
-            nxt = [(a, c) for c in b.children]
-        pending.extend(reversed(nxt))
+        pending.extend(reversed(_children(a, b)))
     return blocks
+
+
+def _admissible(a, b, eta):
+    """Far enough apart for a low-rank block: gap > eta * the larger diameter (hmatrix.py:154-156,185-187)."""
+    gap = np.linalg.norm(np.maximum(0, np.maximum(a.lo - b.hi, b.lo - a.hi)))
+    diameter = max(np.linalg.norm(a.hi - a.lo), np.linalg.norm(b.hi - b.lo))
+    return a is not b and gap > eta * diameter
+
+
+def _children(a, b):
+    """The pairs a non-admissible, non-leaf pair splits into (hmatrix.py:172-177)."""
+    if a is b:
+        x, y = a.children
+        return [(x, x), (x, y), (y, y)]
+    if a.children and (not b.children or len(a.ids) >= len(b.ids)):
+        return [(c, b) for c in a.children]
+    return [(a, c) for c in b.children]
 
 
 class DeviceHMatrix(LinearOperator):
@@ -154,6 +169,19 @@ class DeviceHMatrix(LinearOperator):
         self.info = dict(blocks=len(blocks), low_rank_blocks=int(info[4]), dense_blocks=int(info[3]),
                          slots=int(info[1]), bytes=int(info[2]))
         self.storage_bytes = int(info[2])
+
+    @classmethod
+    def _adopt(cls, handle, n, device):
+        """The operator around a product handle built on the device (real FP64)."""
+        self = cls.__new__(cls)
+        LinearOperator.__init__(self, np.dtype(np.float64), (int(n), int(n)))
+        self.complex_, self.device, self._handle = False, device, handle
+        info = (ctypes.c_int64 * 5)()
+        call('pq_hmatrix_info', handle, info)
+        self.info = dict(blocks=int(info[3] + info[4]), low_rank_blocks=int(info[4]), dense_blocks=int(info[3]),
+                         slots=int(info[1]), bytes=int(info[2]))
+        self.storage_bytes = int(info[2])
+        return self
 
     @classmethod
     def from_hmatrix(cls, H, device=0):
@@ -206,6 +234,146 @@ class DeviceHMatrix(LinearOperator):
             self.close()
         except Exception:                                     # noqa: BLE001 (interpreter shutdown)
             pass
+
+
+def _lists(pairs):
+    """(offsets, concatenated int32 ids) of the row clusters and of the column clusters."""
+    rows = [a.ids for a, _ in pairs]
+    cols = [b.ids for _, b in pairs]
+    rp = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int64)
+    cp = np.r_[0, np.cumsum([len(c) for c in cols])].astype(np.int64)
+    return (rp, np.ascontiguousarray(np.concatenate(rows), np.int32),
+            cp, np.ascontiguousarray(np.concatenate(cols), np.int32))
+
+
+class RwgEntries:
+    """The RWG inductance entries of a triangle mesh (nodes, tris, z) on the device: entry (i, j) =
+    mu0 / 4 pi sum over the support triangles of w_i w_j int int (r - v_i).(r' - v_j) / R, every pair
+    integrated in canonical order (compressed.py:59-73, compressed_kernels.hpp:9-39). The geometry is
+    centred as the fill's; `lo`, `hi` are each edge's support box (compressed.py:54-57,132).
+    Also the store of every block built through it (pq_aca_blocks / pq_dense_blocks)."""
+
+    def __init__(self, nodes, tris, z, near=3.0, outer_levels=1, device=0):
+        from .backend import Mi355xBackend
+        from .geometry import MU0, RWG, FillGeometry
+        g = FillGeometry.prepare(nodes, tris, z, outer_levels)
+        self.geometry, self.device = g, device
+        self._handle = None
+        self._backend = Mi355xBackend(g, 1, 16384, device)
+        r = RWG(g.nodes, g.tris)
+        self.n = int(r.Ne)
+        self.support = np.ascontiguousarray(np.stack((r.tp, r.tm), 1), np.int32)
+        self.vertices = np.ascontiguousarray(np.stack((r.vp, r.vm), 1), np.float64)
+        self.alpha = np.ascontiguousarray(np.stack((1 / (2 * r.area[r.tp]), -1 / (2 * r.area[r.tm])), 1))
+        v = g.nodes[g.tris]
+        lo, hi = np.c_[v.min(1), g.z], np.c_[v.max(1), g.z]
+        self.lo, self.hi = lo[self.support].min(1), hi[self.support].max(1)
+        h = ctypes.c_void_p()
+        call('pq_rwg_entries_create', self._backend._handle, self.n, ptr(self.support), ptr(self.vertices),
+             ptr(self.alpha), MU0 / (4 * np.pi), float(near), int(outer_levels), ctypes.byref(h))
+        self._handle = h
+
+    def __call__(self, rows, cols):
+        """The (len(rows), len(cols)) entry block (rwg_inductance, compressed_kernels.hpp:127-144)."""
+        rows = np.ascontiguousarray(rows, np.int32).ravel()
+        cols = np.ascontiguousarray(cols, np.int32).ravel()
+        out = np.empty((len(rows), len(cols)))
+        call('pq_entries_block', self._handle, ptr(rows), len(rows), ptr(cols), len(cols), ptr(out))
+        return out
+
+    def aca(self, pairs, tol):
+        """(kept ids, ranks) of the checked ACA of cluster pairs; -1 where it did not converge."""
+        rp, rows, cp, cols = _lists(pairs)
+        ids, ranks = np.empty(len(pairs), np.int64), np.empty(len(pairs), np.int64)
+        call('pq_aca_blocks', self._handle, len(pairs), ptr(rp), ptr(rows), ptr(cp), ptr(cols), float(tol),
+             ptr(ids), ptr(ranks), None)
+        return ids, ranks
+
+    def dense(self, pairs):
+        """Kept ids of dense blocks (diagonal ones symmetrised)."""
+        rp, rows, cp, cols = _lists(pairs)
+        same = np.array([a is b for a, b in pairs], np.int32)
+        ids = np.empty(len(pairs), np.int64)
+        call('pq_dense_blocks', self._handle, len(pairs), ptr(rp), ptr(rows), ptr(cp), ptr(cols), ptr(same),
+             ptr(ids))
+        return ids
+
+    def factors(self, kept, m, n, rank):
+        """Host copies of a kept block: (dense, None) or (U, V)."""
+        if rank < 0:
+            u = np.empty((m, n))
+            call('pq_entries_factors', self._handle, int(kept), ptr(u), None)
+            return u, None
+        u, v = np.empty((m, rank)), np.empty((rank, n))
+        call('pq_entries_factors', self._handle, int(kept), ptr(u), ptr(v))
+        return u, v
+
+    def stats(self):
+        entries, reruns, nbytes = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        ms = (ctypes.c_double * 2)()
+        call('pq_entries_stats', self._handle, ctypes.byref(entries), ctypes.byref(reruns), ctypes.byref(nbytes), ms)
+        return dict(evaluated_entries=entries.value, aca_reruns=reruns.value, kept_bytes=nbytes.value,
+                    aca_ms=ms[0], dense_ms=ms[1])
+
+    def close(self):
+        from . import _lib
+        if getattr(self, '_handle', None) is not None and _lib._lib is not None:
+            _lib._lib.pq_entries_destroy(self._handle)
+        self._handle = None
+        if getattr(self, '_backend', None) is not None:
+            self._backend.close()
+            self._backend = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:                                     # noqa: BLE001 (interpreter shutdown)
+            pass
+
+
+def build(entries, tol=1e-6, leaf=64, eta=1.0):
+    """The compressed operator of `entries` (an RwgEntries) on the device: per level of the cluster
+    tree, the admissible pairs by ACA, the failed ones subdivided (or dense at the leaves), then the
+    level's dense leaf pairs, diagonal ones symmetrised; blocks in the reference's order
+    (hmatrix.py:179-210). Returns a DeviceHMatrix with `blocks` ((a, b, kept id, rank, same) per
+    block, rank -1 dense) and `info` (counts, device milliseconds, host seconds)."""
+    if not np.isfinite(tol) or not 0 < tol < 1 or not np.isfinite(eta) or eta <= 0:
+        raise ValueError('invalid compression tolerance or admissibility')
+    if not isinstance(leaf, (int, np.integer)) or leaf < 1:
+        raise ValueError('leaf must be a positive integer')
+    t0 = time.perf_counter()
+    layout = []
+    pending = [(_tree(np.arange(entries.n), entries.lo, entries.hi, leaf),) * 2] if entries.n else []
+    while pending:
+        adm, dense, following = [], [], []
+        for a, b in pending:
+            if _admissible(a, b, eta):
+                adm.append((a, b))
+            elif not a.children and not b.children:
+                dense.append((a, b))
+            else:
+                following.extend(_children(a, b))
+        if adm:
+            ids, ranks = entries.aca(adm, tol)
+            for (a, b), k, r in zip(adm, ids, ranks):
+                if k >= 0:
+                    layout.append((a.ids, b.ids, int(k), int(r), False))
+                elif not a.children and not b.children:
+                    dense.append((a, b))
+                else:
+                    following.extend(_children(a, b))
+        if dense:
+            for (a, b), k in zip(dense, entries.dense(dense)):
+                layout.append((a.ids, b.ids, int(k), -1, a is b))
+        pending = following
+    order = np.array([k for _, _, k, _, _ in layout], np.int64)
+    h = ctypes.c_void_p()
+    call('pq_entries_hmatrix', entries._handle, len(order), ptr(order), ctypes.byref(h))
+    H = DeviceHMatrix._adopt(h, entries.n, entries.device)
+    H.blocks = layout
+    H.info.update(entries.stats(), tolerance=tol, leaf=leaf, eta=eta, dense_bytes=entries.n ** 2 * 8,
+                  build_seconds=time.perf_counter() - t0)
+    return H
 
 
 def rwg_inductance(board, device=0):

@@ -1,0 +1,113 @@
+"""GPU parity of the compressed RWG inductance built on the device (pcbqs_amd.hmatrix.RwgEntries /
+build: pq_entries_block, pq_aca_blocks, pq_dense_blocks, pq_entries_hmatrix) against the oracle
+(oracle/hmatrix.py, whose entries and ACA are the reference core's bit for bit):
+- entry blocks equal the oracle's to 1e-12 of the largest entry, for near 3 and near infinity;
+- the device build's product is within 50 tol of the dense FP64 inductance and of the oracle's
+  build (the reference's bound between its native and NumPy builds, tests/test_compressed.py:
+  300-319), its blocks cover every entry once, and its block ranks are the oracle's;
+- two builds give bitwise equal products; invalid indices and tolerances raise ValueError."""
+import numpy as np
+import pytest
+
+from oracle import fill as F
+from oracle import hmatrix as H
+from oracle import mqs as M
+from tests.meshes import two_layer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def case():
+    from pcbqs_amd.hmatrix import RwgEntries
+    nodes, tris, z = two_layer(32, 20, seed=3)
+    g = F.prepare(nodes, tris, z, 1)
+    sup, vx, al = H.rwg_arrays(g)
+    E = RwgEntries(nodes, tris, z, device=0)
+    print(f'aca case: {len(tris)} triangles, {E.n} RWG edges', flush=True)
+    yield g, sup, vx, al, E
+    E.close()
+
+
+@pytest.fixture(scope='module')
+def dense_l(case):
+    g = case[0]
+    (M0, Mrx, Mry, Mrr), _, _ = F.cr_fill(g, 3.0, 1, False)
+    r = M.RWG(g.nodes, g.tris)
+    return M.combine(r, Mrr, Mrx, Mry, M0) * (H.MU0 / (4 * np.pi)) / np.outer(r.l, r.l)
+
+
+@pytest.mark.parametrize('near', [3.0, np.inf])
+def test_entry_blocks_match_the_oracle(case, near):
+    from pcbqs_amd.hmatrix import RwgEntries
+    g, sup, vx, al, E = case
+    assert np.array_equal(E.support, sup) and np.allclose(E.vertices, vx, rtol=0, atol=0)
+    if not np.isfinite(near):
+        E = RwgEntries(*two_layer(32, 20, seed=3), near=near, device=0)
+    rng = np.random.default_rng(5)
+    rows = rng.choice(len(sup), 60, replace=False)
+    cols = np.r_[rows[:7], rng.choice(len(sup), 90, replace=False)]
+    want = H.rwg_block(g, sup, vx, al, rows, cols, near=near)
+    got = E(rows, cols)
+    print(f'entries near={near}: max rel {np.abs(got - want).max() / np.abs(want).max():.2e}', flush=True)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    if not np.isfinite(near):
+        E.close()
+
+
+def _layout_coverage(blocks, n):
+    C = np.zeros((n, n), np.int64)
+    for a, b, _, _, same in blocks:
+        C[np.ix_(a, b)] += 1
+        if not same:
+            C[np.ix_(b, a)] += 1
+    return C
+
+
+def test_device_build_matches_dense_and_the_oracle_build(case, dense_l):
+    from pcbqs_amd.hmatrix import build
+    g, sup, vx, al, E = case
+    n, tol = E.n, 1e-6
+    D = build(E, tol=tol, leaf=32)
+    print(f'device build: {D.info}', flush=True)
+    assert D.info['low_rank_blocks'] > 20
+    assert (_layout_coverage(D.blocks, n) == 1).all()
+    lo, hi = H.edge_boxes(g, sup)
+    np.testing.assert_array_equal(lo, E.lo)
+    ref_blocks, _, _ = H.build(lambda r, c: H.rwg_block(g, sup, vx, al, r, c), lo, hi, tol=tol, leaf=32)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(n, 3))
+    y = D @ x
+    yd, yo = dense_l @ x, H.matmat(ref_blocks, n, x)
+    e_dense = np.linalg.norm(y - yd) / np.linalg.norm(yd)
+    e_oracle = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    print(f'device build vs dense {e_dense:.2e}, vs the oracle build {e_oracle:.2e}', flush=True)
+    assert e_dense <= 50 * tol and e_oracle <= 50 * tol
+    # the same blocks at the same ranks as the oracle's construction (pivots can only part on ties)
+    ours = {(tuple(a), tuple(b)): r for a, b, _, r, _ in D.blocks}
+    theirs = {(tuple(a), tuple(b)): (-1 if v is None else u.shape[1]) for a, b, u, v, _ in ref_blocks}
+    same = sum(ours.get(k) == r for k, r in theirs.items())
+    print(f'blocks: device {len(ours)}, oracle {len(theirs)}, same rank {same}', flush=True)
+    assert same >= 0.98 * len(theirs) and abs(len(ours) - len(theirs)) <= 0.02 * len(theirs)
+    # one low-rank block's factors against the oracle's
+    k = next(i for i, (a, b, _, r, _) in enumerate(D.blocks) if r > 3 and ours[(tuple(a), tuple(b))] ==
+             theirs.get((tuple(a), tuple(b))))
+    a, b, kept, r, _ = D.blocks[k]
+    u, v = E.factors(kept, len(a), len(b), r)
+    ru, rv = next((uu, vv) for aa, bb, uu, vv, _ in ref_blocks if np.array_equal(aa, a) and np.array_equal(bb, b))
+    np.testing.assert_allclose(u @ v, ru @ rv, rtol=0, atol=1e-9 * np.abs(ru @ rv).max())
+    D.close()
+
+
+def test_builds_repeat_bitwise_and_bad_input_raises(case):
+    from pcbqs_amd.hmatrix import build
+    g, sup, vx, al, E = case
+    x = np.random.default_rng(1).normal(size=(E.n, 2))
+    D1, D2 = build(E, tol=1e-5, leaf=48), build(E, tol=1e-5, leaf=48)
+    assert np.array_equal(D1 @ x, D2 @ x)
+    D1.close()
+    D2.close()
+    with pytest.raises(ValueError, match='edge index'):
+        E(np.array([E.n]), np.array([0]))
+    with pytest.raises(ValueError, match='tolerance'):
+        build(E, tol=1.5)

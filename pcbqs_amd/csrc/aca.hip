@@ -41,10 +41,15 @@
 namespace pq {
 namespace {
 
-constexpr int ACA_THREADS = 256;
 constexpr int ACA_RMAX = 128;          // the reference's rank cap
 constexpr int ACA_CHECK = 16;          // rows per check set
 constexpr int ACA_CHECK_FIRST = 160;   // check-row room of the first attempt: the first set and nine more
+// Launch shapes: general blocks (every pair classified, the analytic path compiled in) in 256-thread
+// workgroups; blocks shown far throughout in 256 threads, or 1024 from ACA_WIDE rows plus columns on
+// (the few large blocks of the top tree levels, which would otherwise run one wave per SIMD).
+constexpr int ACA_WIDE = 1536;
+enum AcaShape { ACA_GENERAL = 0, ACA_FAR = 1, ACA_FAR_WIDE = 2 };
+
 
 struct EntryArgs {
     GeomDev g;
@@ -56,13 +61,22 @@ struct EntryArgs {
 };
 
 // Moments of the ordered pair (p, q) from its canonical pair (compressed_kernels.hpp:9-21).
+// FAR: the caller has shown that every pair of the block is far under the near rule (the block's
+// triangle centroids are farther apart than near times the largest radii), so the far rule is
+// applied without classifying, and the analytic path, which needs several times the registers, is
+// not compiled in.
+template <bool FAR>
 __device__ __forceinline__ void canonical64(const EntryArgs &E, int p, int q, double *m) {
     const GeomDev &g = E.g;
     const int a = min(p, q), b = max(p, q);
     const double d = g.z[a] - g.z[b];
-    const int k = exact_kind(g.cent[2 * a], g.cent[2 * a + 1], g.z[a], g.radius[a], g.cent[2 * b], g.cent[2 * b + 1],
-                             g.z[b], g.radius[b], E.near, E.all_near, E.outer);
-    pair64<false>(g, a, b, d, k, m);
+    if (FAR) {
+        far64<false>(g.P3 + 6 * (size_t)a, g.P3 + 6 * (size_t)b, g.area[a] * g.area[b] / 9, d, m);
+    } else {
+        const int k = exact_kind(g.cent[2 * a], g.cent[2 * a + 1], g.z[a], g.radius[a], g.cent[2 * b],
+                                 g.cent[2 * b + 1], g.z[b], g.radius[b], E.near, E.all_near, E.outer);
+        pair64<false>(g, a, b, d, k, m);
+    }
     if (p > q) {
         double t = m[1]; m[1] = m[3]; m[3] = t;
         t = m[2]; m[2] = m[4]; m[4] = t;
@@ -74,17 +88,20 @@ __device__ __forceinline__ void canonical64(const EntryArgs &E, int p, int q, do
 }
 
 // Entry (ea, eb) of the RWG inductance (compressed_kernels.hpp:23-39), products rounded.
+template <bool FAR>
 __device__ double rwg_entry64(const EntryArgs &E, int ea, int eb) {
 #pragma clang fp contract(off)
     double total = 0.0;
+#pragma nounroll
     for (int sa = 0; sa < 2; ++sa) {
         const int p = E.sup[2 * ea + sa];
         const double vpx = E.vx[4 * ea + 2 * sa], vpy = E.vx[4 * ea + 2 * sa + 1], wa = E.al[2 * ea + sa];
+#pragma nounroll
         for (int sb = 0; sb < 2; ++sb) {
             const int q = E.sup[2 * eb + sb];
             const double vqx = E.vx[4 * eb + 2 * sb], vqy = E.vx[4 * eb + 2 * sb + 1], wb = E.al[2 * eb + sb];
             double m[6];
-            canonical64(E, p, q, m);
+            canonical64<FAR>(E, p, q, m);
             const double val = m[5] - (vqx * m[1] + vqy * m[2]) - (vpx * m[3] + vpy * m[4]) +
                                (vpx * vqx + vpy * vqy) * m[0];
             total += (wa * wb) * val;
@@ -98,7 +115,7 @@ __global__ __launch_bounds__(256) void entries_kernel(EntryArgs E, const int *ro
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nr * nc) return;
     const int i = (int)(t / nc), j = (int)(t % nc);
-    out[t] = rwg_entry64(E, rows[i], cols[j]);
+    out[t] = rwg_entry64<false>(E, rows[i], cols[j]);
 }
 
 // ---- the ACA -----------------------------------------------------------------------------------
@@ -124,9 +141,10 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long &s) 
 }
 
 // k <= 16 distinct indices of [0, m): a partial Fisher-Yates shuffle of 0..m-1 whose touched
-// positions (at most 2k) are kept in a small map instead of an m-long array.
-__device__ void draw(unsigned long long &state, int m, int k, int *out) {
-    int key[2 * ACA_CHECK], val[2 * ACA_CHECK], used = 0;
+// positions (at most 2k) are kept in a small map in LDS (key, val: 2k entries) instead of an m-long
+// array. One thread draws.
+__device__ void draw(unsigned long long &state, int m, int k, int *out, int *key, int *val) {
+    int used = 0;
     auto get = [&](int p) {
         for (int t = 0; t < used; ++t)
             if (key[t] == p) return val[t];
@@ -147,11 +165,12 @@ __device__ void draw(unsigned long long &state, int m, int k, int *out) {
     }
 }
 
+template <int T>
 __device__ double block_sum(double v, double *red) {
     const int tid = threadIdx.x;
     red[tid] = v;
     __syncthreads();
-    for (int s = ACA_THREADS / 2; s > 0; s >>= 1) {
+    for (int s = T / 2; s > 0; s >>= 1) {
         if (tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
@@ -162,12 +181,13 @@ __device__ double block_sum(double v, double *red) {
 
 // The largest v over the workgroup, the smallest index among equal values (the compiled loops'
 // first maximum); threads without a candidate pass (-1, INT_MAX).
+template <int T>
 __device__ void block_argmax(double &v, int &idx, double *red, int *redi) {
     const int tid = threadIdx.x;
     red[tid] = v;
     redi[tid] = idx;
     __syncthreads();
-    for (int s = ACA_THREADS / 2; s > 0; s >>= 1) {
+    for (int s = T / 2; s > 0; s >>= 1) {
         if (tid < s) {
             const double v2 = red[tid + s];
             const int i2 = redi[tid + s];
@@ -188,38 +208,39 @@ __device__ __forceinline__ double wave_sum(double s) {
     return s;
 }
 
-__global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int *rows, const int *cols,
-                                                          const AcaBlock *blocks, double *ws, int *iws, AcaOut *out,
-                                                          double tol) {
+template <bool FAR, int T>
+__global__ __launch_bounds__(T) void aca_kernel(EntryArgs E, const int *rows, const int *cols, const AcaBlock *blocks,
+                                               double *ws, int *iws, AcaOut *out, double tol) {
 #pragma clang fp contract(off)
-    __shared__ double red[ACA_THREADS];
-    __shared__ int redi[ACA_THREADS];
+    __shared__ double red[T];
+    __shared__ int redi[T];
     __shared__ double s_u[ACA_RMAX];
     __shared__ int s_int[3];                     // check count, draw status, next pivot row
+    __shared__ int s_key[2 * ACA_CHECK], s_val[2 * ACA_CHECK];
     const AcaBlock B = blocks[blockIdx.x];
     const int *a = rows + B.a_off, *b = cols + B.b_off;
     const int m = B.m, n = B.n, cap = B.cap;
     double *U = ws + B.u_off, *V = ws + B.v_off, *R = ws + B.r_off;
     int *chk = iws + B.i_off, *used = chk + B.chk;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    for (int k = tid; k < m; k += ACA_THREADS) used[k] = 0;
+    for (int k = tid; k < m; k += T) used[k] = 0;
     unsigned long long state = 0;                // thread 0's stream
     if (tid == 0) {
         state = 0x5DEECE66DULL ^ ((unsigned long long)a[0] * 0x100000001B3ULL) ^ ((unsigned long long)b[0] << 21) ^
                 ((unsigned long long)m << 42) ^ (unsigned long long)n;
-        draw(state, m, min(m, ACA_CHECK), chk);
+        draw(state, m, min(m, ACA_CHECK), chk, s_key, s_val);
         s_int[0] = min(m, ACA_CHECK);
     }
     __syncthreads();
     int nchk = s_int[0];
     long long entries = (long long)nchk * n;
     double part = 0;
-    for (long long t = tid; t < (long long)nchk * n; t += ACA_THREADS) {
-        const double x = rwg_entry64(E, a[chk[t / n]], b[t % n]);
+    for (long long t = tid; t < (long long)nchk * n; t += T) {
+        const double x = rwg_entry64<FAR>(E, a[chk[t / n]], b[t % n]);
         R[t] = x;
         part += x * x;
     }
-    double ref2 = block_sum(part, red);
+    double ref2 = block_sum<T>(part, red);
     int rank = 0, i = 0, status = 0;
     for (int it = 0; it < cap; ++it) {
         // the pivot row's residual, into V's next row
@@ -229,50 +250,50 @@ __global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int
         const int ea = a[i];
         double best = -1.0;
         int jb = INT_MAX;
-        for (int j = tid; j < n; j += ACA_THREADS) {
+        for (int j = tid; j < n; j += T) {
             double s = 0.0;
             for (int l = 0; l < rank; ++l) s += s_u[l] * V[(long long)l * n + j];
-            const double v = rwg_entry64(E, ea, b[j]) - s;
+            const double v = rwg_entry64<FAR>(E, ea, b[j]) - s;
             vr[j] = v;
             if (fabs(v) > best) { best = fabs(v); jb = j; }
         }
         entries += n;
-        block_argmax(best, jb, red, redi);
+        block_argmax<T>(best, jb, red, redi);
         const double pivot = vr[jb];
         if (fabs(pivot) > DBL_MIN) {
             if (tid < rank) s_u[tid] = V[(long long)tid * n + jb];
             __syncthreads();
             double *uc = U + (long long)rank * m;
             const int eb = b[jb];
-            for (int k = tid; k < m; k += ACA_THREADS) {
+            for (int k = tid; k < m; k += T) {
                 double s = 0.0;
                 for (int l = 0; l < rank; ++l) s += U[(long long)l * m + k] * s_u[l];
-                uc[k] = rwg_entry64(E, a[k], eb) - s;
+                uc[k] = rwg_entry64<FAR>(E, a[k], eb) - s;
             }
             entries += m;
-            for (int j = tid; j < n; j += ACA_THREADS) vr[j] = vr[j] / pivot;
+            for (int j = tid; j < n; j += T) vr[j] = vr[j] / pivot;
             __syncthreads();
-            for (long long t = tid; t < (long long)nchk * n; t += ACA_THREADS)
+            for (long long t = tid; t < (long long)nchk * n; t += T)
                 R[t] -= uc[chk[t / n]] * vr[t % n];
             ++rank;
         }
         if (tid == 0) used[i] = 1;
         part = 0;
-        for (long long t = tid; t < (long long)nchk * n; t += ACA_THREADS) part += R[t] * R[t];
-        const double res2 = block_sum(part, red);
+        for (long long t = tid; t < (long long)nchk * n; t += T) part += R[t] * R[t];
+        const double res2 = block_sum<T>(part, red);
         if (sqrt(res2) <= tol * fmax(sqrt(ref2), DBL_MIN)) {
             // a second independent check set before accepting
             const int k = min(m, ACA_CHECK);
             if (tid == 0) {
                 s_int[1] = nchk + k > B.chk ? -1 : 0;
-                if (s_int[1] == 0) draw(state, m, k, chk + nchk);
+                if (s_int[1] == 0) draw(state, m, k, chk + nchk, s_key, s_val);
             }
             __syncthreads();
             if (s_int[1] < 0) { status = -1; break; }
             double pe = 0, pf = 0;
-            for (long long t = tid; t < (long long)k * n; t += ACA_THREADS) {
+            for (long long t = tid; t < (long long)k * n; t += T) {
                 const int row = chk[nchk + t / n], j = (int)(t % n);
-                const double x = rwg_entry64(E, a[row], b[j]);
+                const double x = rwg_entry64<FAR>(E, a[row], b[j]);
                 double s = 0.0;
                 for (int l = 0; l < rank; ++l) s += U[(long long)l * m + row] * V[(long long)l * n + j];
                 const double e = x - s;
@@ -281,7 +302,7 @@ __global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int
                 pf += x * x;
             }
             entries += (long long)k * n;
-            const double e2 = block_sum(pe, red), f2 = block_sum(pf, red);
+            const double e2 = block_sum<T>(pe, red), f2 = block_sum<T>(pf, red);
             if (sqrt(e2) <= tol * fmax(sqrt(f2), DBL_MIN)) { status = 1; break; }
             nchk += k;
             ref2 += f2;
@@ -289,7 +310,7 @@ __global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int
         // next pivot row: the unused check row of largest residual (first in check order)
         double wbest = -1.0;
         int wc = INT_MAX;
-        for (int c = wave; c < nchk; c += ACA_THREADS / 64) {
+        for (int c = wave; c < nchk; c += T / 64) {
             if (used[chk[c]]) continue;
             double s = 0.0;
             for (int j = lane; j < n; j += 64) s += R[(long long)c * n + j] * R[(long long)c * n + j];
@@ -301,7 +322,7 @@ __global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int
         if (tid == 0) {
             double bb = -1.0;
             int bc = INT_MAX;
-            for (int w = 0; w < ACA_THREADS / 64; ++w)
+            for (int w = 0; w < T / 64; ++w)
                 if (redi[w] != INT_MAX && (bc == INT_MAX || red[w] > bb || (red[w] == bb && redi[w] < bc))) {
                     bb = red[w];
                     bc = redi[w];
@@ -315,12 +336,12 @@ __global__ __launch_bounds__(ACA_THREADS) void aca_kernel(EntryArgs E, const int
             // else the unused row of largest |U| in the last column (the first unused row at rank 0)
             double vb = -1.0;
             int kb = INT_MAX;
-            for (int k = tid; k < m; k += ACA_THREADS) {
+            for (int k = tid; k < m; k += T) {
                 if (used[k]) continue;
                 const double v = rank ? fabs(U[(long long)(rank - 1) * m + k]) : 0.0;
                 if (v > vb) { vb = v; kb = k; }
             }
-            block_argmax(vb, kb, red, redi);
+            block_argmax<T>(vb, kb, red, redi);
             next = kb == INT_MAX ? -1 : kb;
         }
         if (next < 0) break;
@@ -354,7 +375,7 @@ __global__ __launch_bounds__(256) void dense_kernel(EntryArgs E, const int *rows
     const int *a = rows + B.a_off, *b = cols + B.b_off;
     double *o = dst + B.dst;
     const long long total = (long long)B.m * B.n;
-    for (long long t = threadIdx.x; t < total; t += blockDim.x) o[t] = rwg_entry64(E, a[t / B.n], b[t % B.n]);
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) o[t] = rwg_entry64<false>(E, a[t / B.n], b[t % B.n]);
     if (!B.same) return;
     __syncthreads();
     // (block + block^T) / 2, each unordered pair by one thread
@@ -396,7 +417,9 @@ struct pq_entries {
     long long ne = 0;
     pq_fill *f = nullptr;
     EntryArgs args{};
-    DevBuf sup, vx, al;
+    std::vector<int> sup;                        // host copies for the far test: supports,
+    std::vector<double> cent, z, radius;         // centroids, heights and radii
+    DevBuf dsup, vx, al;
     hipStream_t stream = nullptr;
     std::vector<Kept> kept;
     std::vector<DevBuf> chunks;
@@ -426,6 +449,34 @@ void check_lists(const pq_entries *e, int64_t nblocks, const int64_t *row_ptr, c
         PQ_REQUIRE(cols[t] >= 0 && cols[t] < e->ne, "hmatrix blocks: edge index out of range");
 }
 
+// The launch shape of block q: far throughout when the boxes of its row and column triangles'
+// centroids (x, y, z) are farther apart than near times the sum of the largest radii on each side,
+// with a 1e-12 margin over the rounding of the pair rule (dist <= near (r_p + r_q): near).
+int aca_shape(const pq_entries *e, const int32_t *r, long long m, const int32_t *c, long long n) {
+    if (e->args.all_near) return ACA_GENERAL;
+    auto box = [&](const int32_t *ids, long long k, double lo[3], double hi[3], double &rmax) {
+        for (int d = 0; d < 3; ++d) { lo[d] = HUGE_VAL; hi[d] = -HUGE_VAL; }
+        rmax = 0;
+        for (long long i = 0; i < k; ++i)
+            for (int s = 0; s < 2; ++s) {
+                const int t = e->sup[2 * (size_t)ids[i] + s];
+                const double p[3] = {e->cent[2 * (size_t)t], e->cent[2 * (size_t)t + 1], e->z[(size_t)t]};
+                for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], p[d]); hi[d] = std::max(hi[d], p[d]); }
+                rmax = std::max(rmax, e->radius[(size_t)t]);
+            }
+    };
+    double la[3], ha[3], lb[3], hb[3], ra, rb;
+    box(r, m, la, ha, ra);
+    box(c, n, lb, hb, rb);
+    double g2 = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double gap = std::max(0.0, std::max(la[d] - hb[d], lb[d] - ha[d]));
+        g2 += gap * gap;
+    }
+    if (!(std::sqrt(g2) > e->args.near * (ra + rb) * (1 + 1e-12))) return ACA_GENERAL;
+    return m + n >= ACA_WIDE ? ACA_FAR_WIDE : ACA_FAR;
+}
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
@@ -438,8 +489,8 @@ constexpr size_t ACA_WORKSPACE_BYTES = size_t(4) << 30;   // per launch; larger 
 // attempt's). Returns each block's result.
 std::vector<AcaOut> aca_launch(pq_entries *e, const std::vector<int> &sel, const int64_t *row_ptr,
                                const int64_t *col_ptr, const DevBuf &drows, const DevBuf &dcols,
-                               const std::vector<int> &caps, bool worst, double tol, std::vector<AcaBlock> &desc,
-                               DevBuf &ws, DevBuf &iws) {
+                               const std::vector<int> &caps, bool worst, double tol, int shape,
+                               std::vector<AcaBlock> &desc, DevBuf &ws, DevBuf &iws) {
     desc.assign(sel.size(), AcaBlock{});
     long long dw = 0, iw = 0;
     for (size_t t = 0; t < sel.size(); ++t) {
@@ -461,9 +512,14 @@ std::vector<AcaOut> aca_launch(pq_entries *e, const std::vector<int> &sel, const
     DevBuf ddesc, dout;
     upload(ddesc, desc.data(), desc.size(), e->stream);
     dout.reserve(sel.size() * sizeof(AcaOut));
-    hipLaunchKernelGGL(aca_kernel, dim3((unsigned)sel.size()), dim3(ACA_THREADS), 0, e->stream, e->args,
-                       drows.as<int>(), dcols.as<int>(), ddesc.as<AcaBlock>(), ws.as<double>(), iws.as<int>(),
-                       dout.as<AcaOut>(), tol);
+#define PQ_ACA(FAR, T)                                                                                   \
+    hipLaunchKernelGGL((aca_kernel<FAR, T>), dim3((unsigned)sel.size()), dim3(T), 0, e->stream, e->args,  \
+                       drows.as<int>(), dcols.as<int>(), ddesc.as<AcaBlock>(), ws.as<double>(), iws.as<int>(), \
+                       dout.as<AcaOut>(), tol)
+    if (shape == ACA_FAR_WIDE) PQ_ACA(true, 1024);
+    else if (shape == ACA_FAR) PQ_ACA(true, 256);
+    else PQ_ACA(false, 256);
+#undef PQ_ACA
     PQ_HIP(hipGetLastError());
     std::vector<AcaOut> res(sel.size());
     PQ_HIP(hipMemcpyAsync(res.data(), dout.p, res.size() * sizeof(AcaOut), hipMemcpyDeviceToHost, e->stream));
@@ -536,13 +592,21 @@ int pq_rwg_entries_create(pq_fill *f, int64_t ne, const int32_t *support, const 
             e->ne = ne;
             e->f = f;
             PQ_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-            upload(e->sup, support, (size_t)(2 * ne), e->stream);
+            upload(e->dsup, support, (size_t)(2 * ne), e->stream);
+            e->sup.assign(support, support + 2 * ne);
+            const size_t nt = (size_t)f->nt;
+            e->cent.resize(2 * nt);
+            e->z.resize(nt);
+            e->radius.resize(nt);
+            PQ_HIP(hipMemcpy(e->cent.data(), f->g.cent, 2 * nt * 8, hipMemcpyDeviceToHost));
+            PQ_HIP(hipMemcpy(e->z.data(), f->g.z, nt * 8, hipMemcpyDeviceToHost));
+            PQ_HIP(hipMemcpy(e->radius.data(), f->g.radius, nt * 8, hipMemcpyDeviceToHost));
             upload(e->vx, vertices, (size_t)(4 * ne), e->stream);
             upload(e->al, alpha, (size_t)(2 * ne), e->stream);
             PQ_HIP(hipStreamSynchronize(e->stream));
             EntryArgs &A = e->args;
             A.g = f->g;
-            A.sup = e->sup.as<int>();
+            A.sup = e->dsup.as<int>();
             A.vx = e->vx.as<double>();
             A.al = e->al.as<double>();
             A.scale = scale;
@@ -602,14 +666,15 @@ int pq_aca_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const 
         if (!nblocks) return;
         DeviceScope scope(e->device);
         std::vector<int> caps((size_t)nblocks);
-        std::vector<int> todo;
+        std::vector<int> todo[3];
         for (int64_t q = 0; q < nblocks; ++q) {
             ids[q] = -1;
             ranks[q] = -1;
-            const double m = (double)(row_ptr[q + 1] - row_ptr[q]), n = (double)(col_ptr[q + 1] - col_ptr[q]);
+            const long long mi = row_ptr[q + 1] - row_ptr[q], ni = col_ptr[q + 1] - col_ptr[q];
+            const double m = (double)mi, n = (double)ni;
             const long long cap = std::min(128LL, (long long)((0.7 * m) * n / (m + n)));
             caps[(size_t)q] = (int)std::max(0LL, cap);
-            if (cap >= 1) todo.push_back((int)q);
+            if (cap >= 1) todo[aca_shape(e, rows + row_ptr[q], mi, cols + col_ptr[q], ni)].push_back((int)q);
         }
         DevBuf drows, dcols, ws, iws;
         upload(drows, rows, (size_t)row_ptr[nblocks], e->stream);
@@ -618,16 +683,18 @@ int pq_aca_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const 
         PQ_HIP(hipEventCreate(&e0));
         PQ_HIP(hipEventCreate(&e1));
         PQ_HIP(hipEventRecord(e0, e->stream));
-        for (int pass = 0; pass < 2 && !todo.empty(); ++pass) {
+        for (int shape = ACA_FAR_WIDE; shape >= ACA_GENERAL; --shape)
+        for (int pass = 0; pass < 2 && !todo[shape].empty(); ++pass) {
             const bool worst = pass == 1;
+            std::vector<int> &work = todo[shape];
             std::vector<int> again;
             size_t start = 0;
-            while (start < todo.size()) {
+            while (start < work.size()) {
                 // as many blocks as fit the workspace bound (always at least one)
                 size_t stop = start;
                 size_t bytes = 0;
-                while (stop < todo.size()) {
-                    const int q = todo[stop];
+                while (stop < work.size()) {
+                    const int q = work[stop];
                     const long long m = row_ptr[q + 1] - row_ptr[q], n = col_ptr[q + 1] - col_ptr[q];
                     const long long c = caps[(size_t)q];
                     const long long chk = worst ? ACA_CHECK * (c + 1) : std::min<long long>(ACA_CHECK_FIRST, ACA_CHECK * (c + 1));
@@ -636,10 +703,10 @@ int pq_aca_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const 
                     bytes += add;
                     ++stop;
                 }
-                std::vector<int> sel(todo.begin() + (long)start, todo.begin() + (long)stop);
+                std::vector<int> sel(work.begin() + (long)start, work.begin() + (long)stop);
                 std::vector<AcaBlock> desc;
-                std::vector<AcaOut> res = aca_launch(e, sel, row_ptr, col_ptr, drows, dcols, caps, worst, tol, desc,
-                                                     ws, iws);
+                std::vector<AcaOut> res = aca_launch(e, sel, row_ptr, col_ptr, drows, dcols, caps, worst, tol, shape,
+                                                     desc, ws, iws);
                 for (size_t t = 0; t < sel.size(); ++t) {
                     count += res[t].entries;
                     if (res[t].status < 0) again.push_back(sel[t]);
@@ -648,9 +715,10 @@ int pq_aca_blocks(pq_entries *e, int64_t nblocks, const int64_t *row_ptr, const 
                 start = stop;
             }
             e->reruns += (long long)again.size();
-            todo.swap(again);
+            work.swap(again);
         }
-        PQ_REQUIRE(todo.empty(), "pq_aca_blocks: check rows beyond the worst-case workspace");
+        for (const auto &left : todo)
+            PQ_REQUIRE(left.empty(), "pq_aca_blocks: check rows beyond the worst-case workspace");
         PQ_HIP(hipEventRecord(e1, e->stream));
         PQ_HIP(hipEventSynchronize(e1));
         e->aca_ms += elapsed(e0, e1);

@@ -236,10 +236,8 @@ class DeviceHMatrix(LinearOperator):
             pass
 
 
-def _lists(pairs):
+def _lists(rows, cols):
     """(offsets, concatenated int32 ids) of the row clusters and of the column clusters."""
-    rows = [a.ids for a, _ in pairs]
-    cols = [b.ids for _, b in pairs]
     rp = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int64)
     cp = np.r_[0, np.cumsum([len(c) for c in cols])].astype(np.int64)
     return (rp, np.ascontiguousarray(np.concatenate(rows), np.int32),
@@ -281,21 +279,21 @@ class RwgEntries:
         call('pq_entries_block', self._handle, ptr(rows), len(rows), ptr(cols), len(cols), ptr(out))
         return out
 
-    def aca(self, pairs, tol):
-        """(kept ids, ranks) of the checked ACA of cluster pairs; -1 where it did not converge."""
-        rp, rows, cp, cols = _lists(pairs)
-        ids, ranks = np.empty(len(pairs), np.int64), np.empty(len(pairs), np.int64)
-        call('pq_aca_blocks', self._handle, len(pairs), ptr(rp), ptr(rows), ptr(cp), ptr(cols), float(tol),
+    def aca(self, rows, cols, tol):
+        """(kept ids, ranks) of the checked ACA of the blocks rows[q] x cols[q] (edge index arrays);
+        -1 where it did not converge."""
+        rp, r, cp, c = _lists(rows, cols)
+        ids, ranks = np.empty(len(rows), np.int64), np.empty(len(rows), np.int64)
+        call('pq_aca_blocks', self._handle, len(rows), ptr(rp), ptr(r), ptr(cp), ptr(c), float(tol),
              ptr(ids), ptr(ranks), None)
         return ids, ranks
 
-    def dense(self, pairs):
-        """Kept ids of dense blocks (diagonal ones symmetrised)."""
-        rp, rows, cp, cols = _lists(pairs)
-        same = np.array([a is b for a, b in pairs], np.int32)
-        ids = np.empty(len(pairs), np.int64)
-        call('pq_dense_blocks', self._handle, len(pairs), ptr(rp), ptr(rows), ptr(cp), ptr(cols), ptr(same),
-             ptr(ids))
+    def dense(self, rows, cols, same):
+        """Kept ids of the dense blocks rows[q] x cols[q] (diagonal ones, same[q], symmetrised)."""
+        rp, r, cp, c = _lists(rows, cols)
+        same = np.ascontiguousarray(same, np.int32)
+        ids = np.empty(len(rows), np.int64)
+        call('pq_dense_blocks', self._handle, len(rows), ptr(rp), ptr(r), ptr(cp), ptr(c), ptr(same), ptr(ids))
         return ids
 
     def factors(self, kept, m, n, rank):
@@ -331,48 +329,104 @@ class RwgEntries:
             pass
 
 
+class _Flat:
+    """The cluster tree as arrays (preorder): boxes, children (-1 at leaves), sizes, index arrays."""
+
+    def __init__(self, root):
+        nodes, stack = [], [root]
+        while stack:
+            node = stack.pop()
+            node.k = len(nodes)
+            nodes.append(node)
+            stack.extend(reversed(node.children))
+        self.nodes = nodes
+        self.lo = np.array([n.lo for n in nodes])
+        self.hi = np.array([n.hi for n in nodes])
+        self.kids = np.array([[c.k for c in n.children] if n.children else [-1, -1] for n in nodes], np.int64)
+        self.size = np.array([len(n.ids) for n in nodes], np.int64)
+        self.ids = [np.ascontiguousarray(n.ids, np.int32) for n in nodes]
+
+    def admissible(self, A, B, eta):
+        """_admissible over arrays of node pairs; pairs within 1e-9 of the threshold are decided by
+        the scalar rule itself, so the decisions are _admissible's."""
+        d = np.maximum(0, np.maximum(self.lo[A] - self.hi[B], self.lo[B] - self.hi[A]))
+        gap = np.sqrt((d * d).sum(1))
+        da, db = self.hi[A] - self.lo[A], self.hi[B] - self.lo[B]
+        bound = eta * np.maximum(np.sqrt((da * da).sum(1)), np.sqrt((db * db).sum(1)))
+        adm = (A != B) & (gap > bound)
+        for k in np.flatnonzero((A != B) & (np.abs(gap - bound) <= 1e-9 * np.maximum(gap, bound))):
+            adm[k] = _admissible(self.nodes[A[k]], self.nodes[B[k]], eta)
+        return adm
+
+    def children(self, A, B):
+        """_children of every pair, concatenated in pair order."""
+        kids, size = self.kids, self.size
+        same = A == B
+        xa, ya, xb, yb = kids[A, 0], kids[A, 1], kids[B, 0], kids[B, 1]
+        split_a = ~same & (xa >= 0) & ((xb < 0) | (size[A] >= size[B]))
+        split_b = ~same & ~split_a
+        ca, cb = np.full((len(A), 3), -1, np.int64), np.full((len(A), 3), -1, np.int64)
+        ca[same], cb[same] = np.c_[xa, xa, ya][same], np.c_[xa, ya, ya][same]
+        ca[split_a, :2], cb[split_a, :2] = np.c_[xa, ya][split_a], B[split_a, None]
+        ca[split_b, :2], cb[split_b, :2] = A[split_b, None], np.c_[xb, yb][split_b]
+        keep = ca >= 0
+        return ca[keep], cb[keep]
+
+
 def build(entries, tol=1e-6, leaf=64, eta=1.0):
     """The compressed operator of `entries` (an RwgEntries) on the device: per level of the cluster
     tree, the admissible pairs by ACA, the failed ones subdivided (or dense at the leaves), then the
     level's dense leaf pairs, diagonal ones symmetrised; blocks in the reference's order
-    (hmatrix.py:179-210). Returns a DeviceHMatrix with `blocks` ((a, b, kept id, rank, same) per
-    block, rank -1 dense) and `info` (counts, device milliseconds, host seconds)."""
+    (hmatrix.py:179-210). The level's classification runs over arrays of node pairs. Returns a
+    DeviceHMatrix with `blocks` ((a, b, kept id, rank, same) per block, rank -1 dense) and `info`
+    (counts, device milliseconds, host seconds)."""
     if not np.isfinite(tol) or not 0 < tol < 1 or not np.isfinite(eta) or eta <= 0:
         raise ValueError('invalid compression tolerance or admissibility')
     if not isinstance(leaf, (int, np.integer)) or leaf < 1:
         raise ValueError('leaf must be a positive integer')
     t0 = time.perf_counter()
-    layout = []
-    pending = [(_tree(np.arange(entries.n), entries.lo, entries.hi, leaf),) * 2] if entries.n else []
-    while pending:
-        adm, dense, following = [], [], []
-        for a, b in pending:
-            if _admissible(a, b, eta):
-                adm.append((a, b))
-            elif not a.children and not b.children:
-                dense.append((a, b))
-            else:
-                following.extend(_children(a, b))
-        if adm:
-            ids, ranks = entries.aca(adm, tol)
-            for (a, b), k, r in zip(adm, ids, ranks):
-                if k >= 0:
-                    layout.append((a.ids, b.ids, int(k), int(r), False))
-                elif not a.children and not b.children:
-                    dense.append((a, b))
-                else:
-                    following.extend(_children(a, b))
-        if dense:
-            for (a, b), k in zip(dense, entries.dense(dense)):
-                layout.append((a.ids, b.ids, int(k), -1, a is b))
-        pending = following
+    layout, levels = [], []
+    if entries.n:
+        T = _Flat(_tree(np.arange(entries.n), entries.lo, entries.hi, leaf))
+        A = B = np.zeros(1, np.int64)
+    else:
+        A = B = np.zeros(0, np.int64)
+    while len(A):
+        adm = T.admissible(A, B, eta)
+        leaves = (T.kids[A, 0] < 0) & (T.kids[B, 0] < 0)
+        dense_a, dense_b = A[~adm & leaves], B[~adm & leaves]
+        next_a, next_b = T.children(A[~adm & ~leaves], B[~adm & ~leaves])
+        level = dict(pairs=len(A), aca_blocks=int(adm.sum()), aca_s=0.0, dense_s=0.0)
+        if adm.any():
+            aa, ab = A[adm], B[adm]
+            level['largest_aca_block'] = int((T.size[aa] * T.size[ab]).max())
+            t1 = time.perf_counter()
+            kept, ranks = entries.aca([T.ids[k] for k in aa], [T.ids[k] for k in ab], tol)
+            level['aca_s'] = time.perf_counter() - t1
+            level['aca_converged'] = int((kept >= 0).sum())
+            ok = kept >= 0
+            layout += [(T.ids[a], T.ids[b], int(k), int(r), False) for a, b, k, r in
+                       zip(aa[ok], ab[ok], kept[ok], ranks[ok])]
+            failed_leaves = ~ok & (T.kids[aa, 0] < 0) & (T.kids[ab, 0] < 0)
+            dense_a, dense_b = np.r_[dense_a, aa[failed_leaves]], np.r_[dense_b, ab[failed_leaves]]
+            fa, fb = T.children(aa[~ok & ~failed_leaves], ab[~ok & ~failed_leaves])
+            next_a, next_b = np.r_[next_a, fa], np.r_[next_b, fb]
+        level['dense_blocks'] = len(dense_a)
+        if len(dense_a):
+            same = dense_a == dense_b
+            t1 = time.perf_counter()
+            kept = entries.dense([T.ids[k] for k in dense_a], [T.ids[k] for k in dense_b], same)
+            level['dense_s'] = time.perf_counter() - t1
+            layout += [(T.ids[a], T.ids[b], int(k), -1, bool(s)) for a, b, k, s in zip(dense_a, dense_b, kept, same)]
+        levels.append(level)
+        A, B = next_a, next_b
     order = np.array([k for _, _, k, _, _ in layout], np.int64)
     h = ctypes.c_void_p()
     call('pq_entries_hmatrix', entries._handle, len(order), ptr(order), ctypes.byref(h))
     H = DeviceHMatrix._adopt(h, entries.n, entries.device)
     H.blocks = layout
     H.info.update(entries.stats(), tolerance=tol, leaf=leaf, eta=eta, dense_bytes=entries.n ** 2 * 8,
-                  build_seconds=time.perf_counter() - t0)
+                  build_seconds=time.perf_counter() - t0, levels=levels)
     return H
 
 
@@ -397,38 +451,60 @@ def rwg_inductance(board, device=0):
 
 
 def main(argv=None):
-    """Time the device product on the RWG inductance of a seeded two-layer board compressed at the
-    reference's default options (tol 1e-6, leaf 64, eta 1)."""
+    """Build the compressed RWG inductance of a seeded board and time its device product, at the
+    reference's default options (tol 1e-6, leaf 64, eta 1). --builder device: the device
+    construction (no dense matrix; --check compares products with the dense FP64 inductance of the
+    device fill); --builder svd: the host builder over the dense inductance."""
     from . import board as boards
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument('--triangles', type=int, default=5800)
+    ap.add_argument('--board', choices=['c2', 'c5', 'small'], default='small')
+    ap.add_argument('--triangles', type=int, default=5800, help='triangles of the small board')
+    ap.add_argument('--builder', choices=['device', 'svd'], default='device')
+    ap.add_argument('--check', action='store_true')
     ap.add_argument('--columns', type=int, nargs='+', default=[2, 8])
     ap.add_argument('--tol', type=float, default=1e-6)
     ap.add_argument('--leaf', type=int, default=64)
     ap.add_argument('--reps', type=int, default=50)
     args = ap.parse_args(argv)
-    b = boards.board_c2(0, size=30e-3, target=args.triangles)
+    if args.board == 'c2':
+        b, name = boards.board_c2(0), 'config-2 board (2 layers, 60 x 60 mm)'
+    elif args.board == 'c5':
+        b, name = boards.board_c5(0), 'config-5 board (4 layers, 120 x 120 mm, 200 vias)'
+    else:
+        b, name = boards.board_c2(0, size=30e-3, target=args.triangles), 'seeded 2-layer board'
+    out = dict(builder=args.builder, tol=args.tol, leaf=args.leaf, products={})
+    L = None
     t0 = time.perf_counter()
-    L, lo, hi = rwg_inductance(b)
-    t1 = time.perf_counter()
-    blocks = compress(lambda a, c: L[np.ix_(a, c)], lo, hi, tol=args.tol, leaf=args.leaf)
-    t2 = time.perf_counter()
-    H = DeviceHMatrix(blocks, len(L))
+    if args.builder == 'device':
+        E = RwgEntries(b.nodes, b.tris, b.z_of)
+        t1 = time.perf_counter()
+        H = build(E, tol=args.tol, leaf=args.leaf)
+        t2 = time.perf_counter()
+        n = E.n
+        out.update(provider_s=t1 - t0, build_s=t2 - t1, **{k: v for k, v in H.info.items() if k not in ('tolerance', 'leaf', 'eta')})
+        E.close()
+        if args.check:
+            L, _, _ = rwg_inductance(b)
+    else:
+        L, lo, hi = rwg_inductance(b)
+        t1 = time.perf_counter()
+        blocks = compress(lambda a, c: L[np.ix_(a, c)], lo, hi, tol=args.tol, leaf=args.leaf)
+        t2 = time.perf_counter()
+        H = DeviceHMatrix(blocks, len(L))
+        n = len(L)
+        out.update(fill_combine_s=t1 - t0, compress_s=t2 - t1, blocks=H.info['blocks'],
+                   low_rank_blocks=H.info['low_rank_blocks'], bytes=H.storage_bytes, dense_bytes=L.nbytes)
+    out['workload'] = f'RWG inductance, {name}: {b.nt} triangles, {n} RWG'
     rng = np.random.default_rng(0)
-    out = dict(workload=f'RWG inductance, seeded 2-layer board: {b.nt} triangles, {len(L)} RWG',
-               tol=args.tol, leaf=args.leaf, blocks=H.info['blocks'], low_rank_blocks=H.info['low_rank_blocks'],
-               factor_bytes=H.storage_bytes, dense_bytes=L.nbytes, fill_combine_s=t1 - t0, compress_s=t2 - t1,
-               products={})
     for k in args.columns:
-        x = rng.standard_normal((len(L), k))
+        x = rng.standard_normal((n, k))
         y = H @ x
-        ref = L @ x
         ms = H.time_product(k, args.reps)
-        # algorithmic bytes: every factor once, X gathered per block row/column, the slots written and read
-        slot_bytes = 2 * H.info['slots'] * 8 * min(k, 8) * ((k + 7) // 8)
-        out['products'][k] = dict(ms=ms, rel_err_vs_dense=float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
-                                  factor_GBs=H.storage_bytes * ((k + 7) // 8) / ms / 1e6,
-                                  slot_bytes=slot_bytes)
+        rec = dict(ms=ms, factor_GBs=H.storage_bytes * ((k + 7) // 8) / ms / 1e6)
+        if L is not None:
+            ref = L @ x
+            rec['rel_err_vs_dense'] = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+        out['products'][k] = rec
     print(json.dumps(out), flush=True)
     H.close()
 

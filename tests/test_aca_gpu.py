@@ -111,3 +111,64 @@ def test_builds_repeat_bitwise_and_bad_input_raises(case):
         E(np.array([E.n]), np.array([0]))
     with pytest.raises(ValueError, match='tolerance'):
         build(E, tol=1.5)
+
+
+@pytest.mark.skipif(F.ref_core() is None, reason='reference core (oracle/_ref) not built')
+def test_config2_build_matches_the_reference_core_build():
+    """The 20,336-triangle config-2 board (30,023 RWG): the device build against the reference core's
+    own native build (RwgEntries, aca_blocks, dense_blocks, driven level by level as hmatrix.py:179-210
+    does), the same blocks at the same ranks and products within 50 tol."""
+    from pcbqs_amd import board as boards
+    from pcbqs_amd.hmatrix import DeviceHMatrix, RwgEntries, build
+    core = F.ref_core()
+    b = boards.board_c2(0)
+    E = RwgEntries(b.nodes, b.tris, b.z_of, device=0)
+    D = build(E, tol=1e-6, leaf=64)
+    print(f"c2 device build: {D.info['build_seconds']:.3f} s, {D.info['blocks']} blocks, "
+          f"{D.info['low_rank_blocks']} low rank, {D.info['bytes'] / 1e6:.1f} MB", flush=True)
+    g = F.prepare(b.nodes, b.tris, b.z_of, 1)
+    sup, vx, al = H.rwg_arrays(g)
+    lo, hi = H.edge_boxes(g, sup)
+    provider = core.RwgEntries(F.ref_geometry_dict(g), g.z, g.cent, g.radius, 3.0, 1, sup,
+                               np.ascontiguousarray(vx), np.ascontiguousarray(al), H.MU0 / (4 * np.pi))
+    workers = max(1, min(16, len(__import__('os').sched_getaffinity(0))))
+    blocks, pending = [], [(H.cluster_tree(lo, hi, 64),) * 2]
+    while pending:
+        adm, dense, following = [], [], []
+        for a, c in pending:
+            if H.admissible(a, c, 1.0):
+                adm.append((a, c))
+            elif not a.children and not c.children:
+                dense.append((a, c))
+            else:
+                following.extend(H.children(a, c))
+        if adm:
+            factors, _, _ = core.aca_blocks(provider, [a.ids.astype(np.int64) for a, _ in adm],
+                                            [c.ids.astype(np.int64) for _, c in adm], 1e-6, workers)
+            for (a, c), f in zip(adm, factors):
+                if f is not None:
+                    blocks.append((a.ids, c.ids, f[0], f[1], False))
+                elif not a.children and not c.children:
+                    dense.append((a, c))
+                else:
+                    following.extend(H.children(a, c))
+        if dense:
+            for (a, c), blk in zip(dense, core.dense_blocks(provider, [a.ids.astype(np.int64) for a, _ in dense],
+                                                            [c.ids.astype(np.int64) for _, c in dense], workers)):
+                if a is c:
+                    blk = (blk + blk.T) / 2
+                blocks.append((a.ids, c.ids, blk, None, a is c))
+        pending = following
+    ours = {(tuple(a), tuple(c)): r for a, c, _, r, _ in D.blocks}
+    theirs = {(tuple(a), tuple(c)): (-1 if v is None else u.shape[1]) for a, c, u, v, _ in blocks}
+    same = sum(ours.get(k) == r for k, r in theirs.items())
+    print(f'c2 blocks: device {len(ours)}, reference core {len(theirs)}, same rank {same}', flush=True)
+    assert same >= 0.98 * len(theirs) and abs(len(ours) - len(theirs)) <= 0.02 * len(theirs)
+    R = DeviceHMatrix(blocks, E.n)
+    x = np.random.default_rng(3).normal(size=(E.n, 2))
+    y, yr = D @ x, R @ x
+    err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f'c2 device build vs the reference core build: {err:.2e}', flush=True)
+    assert err <= 50 * 1e-6
+    for h in (R, D, E):
+        h.close()

@@ -19,6 +19,10 @@ reference core) on this host's cores, one warm-up and the median of three runs (
 the plain-preconditioned GMRES restated in numpy on the same FP64 system (cpu_gmres); it then uses
 the oracle as the checker: every fill's per-plane error and every port solve's R and L error
 against the oracle's FP64 chain (fill, combine, C^T L C, LU).
+
+The compressed part (skipped with --no-compressed) builds the board's RWG inductance as an H-matrix
+on the device (tol 1e-6, leaf 64) and, in the CPU leg, times the reference core's own native
+construction of the same operator on every host core and compares the two operators' products.
 """
 import argparse
 import ctypes
@@ -370,6 +374,74 @@ def cpu_gmres(solver, iterations_gpu):
     return out
 
 
+def compressed_section(board, tol=1e-6, leaf=64, reps=20):
+    """The compressed RWG inductance of the board built on the device (entry blocks, checked ACA,
+    dense leaves; no dense matrix) and its product with 8 columns."""
+    from pcbqs_amd.hmatrix import RwgEntries, build
+    E = RwgEntries(board.nodes, board.tris, board.z_of)
+    build(E, tol=tol, leaf=leaf).close()                          # warm-up (module load, first launches)
+    E.close()
+    t = time.perf_counter()
+    E = RwgEntries(board.nodes, board.tris, board.z_of)
+    provider_s = time.perf_counter() - t
+    D = build(E, tol=tol, leaf=leaf)
+    info = D.info
+    x = np.random.default_rng(0).standard_normal((E.n, 8))
+    y = D @ x
+    out = dict(rwg=E.n, tol=tol, leaf=leaf, provider_s=provider_s, build_s=info['build_seconds'],
+               aca_device_ms=info['aca_ms'], dense_device_ms=info['dense_ms'], blocks=info['blocks'],
+               low_rank_blocks=info['low_rank_blocks'], bytes=info['bytes'], dense_bytes=info['dense_bytes'],
+               evaluated_entries=info['evaluated_entries'], product_ms_8_columns=D.time_product(8, reps),
+               note='device construction: the provider (geometry upload) and the build (tree, ACA, dense leaves, '
+                    'assembly); the product is the H-matrix times 8 real columns, vectors in HBM')
+    return out, D, E, x, y
+
+
+def reference_core_build(board, tol=1e-6, leaf=64):
+    """The reference core's own native construction of the same operator (RwgEntries, aca_blocks,
+    dense_blocks; hmatrix_build.hpp), driven level by level as hmatrix.py:179-210 does, on every
+    host core: (seconds, blocks, threads) or None where oracle/_ref is not built."""
+    from oracle import fill as F
+    from oracle import hmatrix as H
+    core = F.ref_core()
+    if core is None:
+        return None
+    g = F.prepare(board.nodes, board.tris, board.z_of, 1)
+    sup, vx, al = H.rwg_arrays(g)
+    lo, hi = H.edge_boxes(g, sup)
+    affinity, quota, _ = host_cpus()
+    threads = quota if quota and quota < affinity else affinity
+    t = time.perf_counter()
+    provider = core.RwgEntries(F.ref_geometry_dict(g), g.z, g.cent, g.radius, 3.0, 1, sup,
+                               np.ascontiguousarray(vx), np.ascontiguousarray(al), H.MU0 / (4 * np.pi))
+    blocks, pending = [], [(H.cluster_tree(lo, hi, leaf),) * 2]
+    while pending:
+        adm, dense, following = [], [], []
+        for a, c in pending:
+            if H.admissible(a, c, 1.0):
+                adm.append((a, c))
+            elif not a.children and not c.children:
+                dense.append((a, c))
+            else:
+                following.extend(H.children(a, c))
+        if adm:
+            factors, _, _ = core.aca_blocks(provider, [a.ids.astype(np.int64) for a, _ in adm],
+                                            [c.ids.astype(np.int64) for _, c in adm], tol, threads)
+            for (a, c), f in zip(adm, factors):
+                if f is not None:
+                    blocks.append((a.ids, c.ids, f[0], f[1], False))
+                elif not a.children and not c.children:
+                    dense.append((a, c))
+                else:
+                    following.extend(H.children(a, c))
+        if dense:
+            for (a, c), blk in zip(dense, core.dense_blocks(provider, [a.ids.astype(np.int64) for a, _ in dense],
+                                                            [c.ids.astype(np.int64) for _, c in dense], threads)):
+                blocks.append((a.ids, c.ids, (blk + blk.T) / 2 if a is c else blk, None, a is c))
+        pending = following
+    return time.perf_counter() - t, blocks, threads
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -378,6 +450,7 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-gmres', action='store_true')
+    ap.add_argument('--no-compressed', action='store_true', help='skip the compressed-operator build')
     ap.add_argument('--sweep', type=int, default=0, metavar='BOARDS',
                     help='also run the batched sweep over BOARDS uneven boards x 20 frequencies')
     ap.add_argument('--sweep-concurrency', type=int, default=4, help='boards at once in the sweep')
@@ -438,6 +511,9 @@ def main():
                                               relative_residual=i64['relative_residual'])
     if args.sweep:
         line['sweep'] = sweep_section(args.sweep, concurrency=args.sweep_concurrency)
+    hm = None
+    if not args.no_compressed:
+        line['compressed'], *hm = compressed_section(board)
     if not args.no_cpu:
         line['cpu_baseline'], ref_planes = cpu_fill_baseline(board)
         line['fill_rel_err_vs_fp64'] = plane_errors(planes32, ref_planes, local_cent=g.cent)
@@ -448,6 +524,22 @@ def main():
             line['port_L_rel_err_vs_oracle'] = {k: v['port_L'] for k, v in errs.items()}
             line['port_R_rel_err_vs_oracle'] = {k: v['port_R'] for k, v in errs.items()}
         del ref_planes
+        if hm and reference_core_build(board) is not None:        # the first run is the warm-up
+            from pcbqs_amd.hmatrix import DeviceHMatrix
+            runs = [reference_core_build(board) for _ in range(3)]
+            seconds, blocks, threads = sorted(runs, key=lambda r: r[0])[1]
+            D, E, x, y = hm
+            R = DeviceHMatrix(blocks, E.n)
+            yr = R @ x
+            R.close()
+            line['compressed']['cpu_reference_build'] = dict(
+                seconds=seconds, threads=threads, kind='reference', blocks=len(blocks),
+                sample='the whole construction of the same operator by the reference core (oracle/_ref, built from '
+                       'its sources): RwgEntries, aca_blocks and dense_blocks level by level, every host core; '
+                       'one warm-up, the median of three runs')
+            line['compressed']['cpu_over_gpu_build'] = seconds / line['compressed']['build_s']
+            line['compressed']['product_rel_err_vs_reference_build'] = float(
+                np.linalg.norm(y - yr) / np.linalg.norm(yr))
         if solver is not None:
             line['cpu_gmres'] = cpu_gmres(solver, line['gmres']['plain']['iterations'])
             line['cpu_gmres']['cpu_over_gpu_plain'] = line['cpu_gmres']['seconds'] / line['gmres']['plain']['seconds']
@@ -455,6 +547,9 @@ def main():
     planes64.close()
     if solver is not None:
         solver.close()
+    if hm:
+        hm[0].close()
+        hm[1].close()
     print(json.dumps(line))
 
 

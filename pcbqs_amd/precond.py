@@ -69,10 +69,18 @@ def _inverse_impedance(p, d, zpair):
     return sparse.diags(on) @ D
 
 
+# The Laplacian is factored dense (FP32 complex, n^2 x 8 bytes: 12.8 GB at this bound, and O(n^3)
+# work per frequency), where the reference's SuperLU is sparse; larger node counts are refused.
+MAX_LAPLACIAN_NODES = 40000
+
+
 class _DeviceLaplacian:
     """The kept-node Laplacian factored on the device (pq_csym_*)."""
 
     def __init__(self, lap, device):
+        if lap.shape[0] > MAX_LAPLACIAN_NODES:
+            raise ValueError(f'graph preconditioner: {lap.shape[0]:,} kept nodes exceed the dense device '
+                             f'factor\'s limit of {MAX_LAPLACIAN_NODES:,} (use block-Jacobi or two-level)')
         low = sparse.tril(lap, format='coo')
         rows, cols = np.ascontiguousarray(low.row, np.int32), np.ascontiguousarray(low.col, np.int32)
         vals = np.ascontiguousarray(low.data, np.complex128)

@@ -65,3 +65,11 @@ def test_host_setup_equals_the_oracle(make):
     pp, qq = dict(p, pairs=np.stack([cand[:5], cand[5:]])), dict(q, pairs=np.stack([cand[:5], cand[5:]]))
     z = rng.normal(size=5) + 0j
     assert abs(_inverse_impedance(qq, d, z) - O.dinv(pp, d, z)).max() <= 1e-15 * abs(O.dinv(pp, d, z)).max()
+
+
+def test_laplacian_beyond_the_dense_limit_is_refused_before_any_device_call():
+    from scipy import sparse
+    from pcbqs_amd.precond import MAX_LAPLACIAN_NODES, _DeviceLaplacian
+    lap = sparse.identity(MAX_LAPLACIAN_NODES + 1, dtype=complex, format='csr')
+    with pytest.raises(ValueError, match='kept nodes exceed'):
+        _DeviceLaplacian(lap, 0)

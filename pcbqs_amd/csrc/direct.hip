@@ -264,22 +264,38 @@ __device__ __attribute__((always_inline)) inline void tile_abt(const float *__re
             acc.i[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int kc = 0; kc < kdim; kc += 32) {
-        for (int e = tid; e < LB * 8; e += 256) {
-            // rows past the end load the last valid row (arows, brows >= 1) and are zeroed by value
+    // Chunk kc + 32's global loads are issued before chunk kc's MFMAs (registers, then LDS after the
+    // chunk's barrier), so their latency overlaps the products.
+    float4 pa[2][2], pb[2][2];              // [e iteration][plane]
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int e = tid + it * 256;   // LB * 8 = 512 float4 slots per plane
             const int r = e >> 3, q = (e & 7) * 4, ra = min(r, arows - 1), rb = min(r, brows - 1);
-            float4 a0 = *reinterpret_cast<const float4 *>(Ar + (size_t)ra * lda + kc + q);
-            float4 a1 = *reinterpret_cast<const float4 *>(Ai + (size_t)ra * lda + kc + q);
-            float4 b0 = *reinterpret_cast<const float4 *>(Br + (size_t)rb * ldb + kc + q);
-            float4 b1 = *reinterpret_cast<const float4 *>(Bi + (size_t)rb * ldb + kc + q);
-            if (r >= arows) a0 = a1 = zero;
-            if (r >= brows) b0 = b1 = zero;
-            *reinterpret_cast<float4 *>(&sA[0][r][q]) = a0;
-            *reinterpret_cast<float4 *>(&sA[1][r][q]) = a1;
-            *reinterpret_cast<float4 *>(&sB[0][r][q]) = b0;
-            *reinterpret_cast<float4 *>(&sB[1][r][q]) = b1;
+            // rows past the end load the last valid row (arows, brows >= 1) and are zeroed by value
+            pa[it][0] = *reinterpret_cast<const float4 *>(Ar + (size_t)ra * lda + kc + q);
+            pa[it][1] = *reinterpret_cast<const float4 *>(Ai + (size_t)ra * lda + kc + q);
+            pb[it][0] = *reinterpret_cast<const float4 *>(Br + (size_t)rb * ldb + kc + q);
+            pb[it][1] = *reinterpret_cast<const float4 *>(Bi + (size_t)rb * ldb + kc + q);
+            if (r >= arows) pa[it][0] = pa[it][1] = zero;
+            if (r >= brows) pb[it][0] = pb[it][1] = zero;
         }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int e = tid + it * 256, r = e >> 3, q = (e & 7) * 4;
+            *reinterpret_cast<float4 *>(&sA[0][r][q]) = pa[it][0];
+            *reinterpret_cast<float4 *>(&sA[1][r][q]) = pa[it][1];
+            *reinterpret_cast<float4 *>(&sB[0][r][q]) = pb[it][0];
+            *reinterpret_cast<float4 *>(&sB[1][r][q]) = pb[it][1];
+        }
+    };
+    fetch(0);
+    for (int kc = 0; kc < kdim; kc += 32) {
+        stage();
         __syncthreads();
+        if (kc + 32 < kdim) fetch(kc + 32);
         float fa[2][2][8], fb[2][2][8];     // [plane][row or column group][s]
 #pragma unroll
         for (int p = 0; p < 2; ++p)

@@ -1,19 +1,33 @@
 // Kernels of the message-passing network over the branch graph (RWG edges and via segments) that
 // refines the coarse-space solution of the loop solve. Dense layers run on the matrix cores:
-// v_mfma_f32_16x16x16_bf16, operands rounded to BF16 (round to nearest even) as they are staged,
-// products accumulated in FP32. Activations live in HBM as FP32 row-major (N, F).
+// gfx950's v_mfma_f32_16x16x32_bf16 (v_mfma_f32_16x16x16_bf16 where the reduction is 16 long),
+// operands rounded to BF16 (round to nearest even) as they are staged, products accumulated in
+// FP32. Activations live in HBM as FP32 row-major (N, F).
 //
-// Operand layout of the 16x16x16 BF16 MFMA (wave64), lane l = 0..63, q = 0..3:
-//   A (16 x 16, rows i, reduction k): lane l supplies A[l % 16][4 (l / 16) + q]
-//   B (16 x 16, reduction k, cols j): lane l supplies B[4 (l / 16) + q][l % 16]
-//   D (16 x 16):                      lane l holds   D[4 (l / 16) + q][l % 16]
+// Operand layout (wave64), lane l = 0..63, g = l / 16, q = 0..3:
+//   16x16x16: A lane l supplies A[l % 16][4 g + q], B lane l supplies B[4 g + q][l % 16]
+//   16x16x32: A lane l supplies A[l % 16][8 g + e], B lane l supplies B[8 g + e][l % 16], e = 0..7
+//   D (16 x 16), both:                 lane l holds   D[4 g + q][l % 16]
+// A and B take their reduction index from the same (lane group, element) pair, so the product is
+// the matrix product whatever order the unit sums that pair's terms in.
 // No counterpart in the reference.
 #include "pq_common.h"
 
 namespace pq {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// D += A B over a reduction of 32: two 8-byte halves per operand (elements 8 g .. 8 g + 7).
+__device__ __forceinline__ f32x4 mfma32(const bf16x4 &a0, const bf16x4 &a1, const bf16x4 &b0, const bf16x4 &b1,
+                                        const f32x4 &c) {
+    const bf16x8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const bf16x8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c, 0,
+                                                   0, 0);
+}
 
 // FP32 -> BF16 bits, round to nearest even (NaN kept quiet).
 __device__ __forceinline__ unsigned short to_bf16(float x) {
@@ -45,17 +59,37 @@ __global__ __launch_bounds__(256) void mpnn_gemm_kernel(int N, const float *__re
     f32x4 acc[M / 16];
 #pragma unroll
     for (int t = 0; t < M / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < K / 16; ++ks) {
-        const int k0 = ks * 16;
-        const float *src = k0 < K1 ? A1 + (size_t)arow * lda1 + k0 : A2 + (size_t)arow * lda2 + (k0 - K1);
-        const float4 v = *reinterpret_cast<const float4 *>(src + kq);
+    auto load4 = [&](const float *p) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
         bf16x4 a;
         a[0] = (short)to_bf16(v.x); a[1] = (short)to_bf16(v.y); a[2] = (short)to_bf16(v.z); a[3] = (short)to_bf16(v.w);
+        return a;
+    };
+    if constexpr (K % 32 == 0 && K1 % 32 == 0) {
+        const int k8 = 8 * (lane >> 4);
 #pragma unroll
-        for (int t = 0; t < M / 16; ++t) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&wt[(t * 16 + (lane & 15)) * KP + k0 + kq]);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        for (int ks = 0; ks < K / 32; ++ks) {
+            const int k0 = ks * 32;
+            const float *src = k0 < K1 ? A1 + (size_t)arow * lda1 + k0 : A2 + (size_t)arow * lda2 + (k0 - K1);
+            const bf16x4 a0 = load4(src + k8), a1 = load4(src + k8 + 4);
+#pragma unroll
+            for (int t = 0; t < M / 16; ++t) {
+                const unsigned short *w = &wt[(t * 16 + (lane & 15)) * KP + k0 + k8];
+                acc[t] = mfma32(a0, a1, *reinterpret_cast<const bf16x4 *>(w), *reinterpret_cast<const bf16x4 *>(w + 4),
+                                acc[t]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ++ks) {
+            const int k0 = ks * 16;
+            const float *src = k0 < K1 ? A1 + (size_t)arow * lda1 + k0 : A2 + (size_t)arow * lda2 + (k0 - K1);
+            const bf16x4 a = load4(src + kq);
+#pragma unroll
+            for (int t = 0; t < M / 16; ++t) {
+                const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&wt[(t * 16 + (lane & 15)) * KP + k0 + kq]);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
@@ -112,16 +146,18 @@ __global__ __launch_bounds__(256) void mpnn_gemm_tn_kernel(int N, int chunk, con
             gt[j * TN_PITCH + r] = to_bf16(v);
         }
         __syncthreads();
+        const int k8 = 8 * (lane >> 4);
 #pragma unroll
-        for (int ns = 0; ns < TN_ROWS / 16; ++ns) {
+        for (int ns = 0; ns < TN_ROWS / 32; ++ns) {
 #pragma unroll
             for (int t = 0; t < TPW; ++t) {
                 const int tile = wave * TPW + t;
                 if (tile >= T) break;
                 const int kt = tile / TM, mt = tile % TM;
-                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&at[(kt * 16 + (lane & 15)) * TN_PITCH + ns * 16 + kq]);
-                const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&gt[(mt * 16 + (lane & 15)) * TN_PITCH + ns * 16 + kq]);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+                const unsigned short *pa = &at[(kt * 16 + (lane & 15)) * TN_PITCH + ns * 32 + k8];
+                const unsigned short *pg = &gt[(mt * 16 + (lane & 15)) * TN_PITCH + ns * 32 + k8];
+                acc[t] = mfma32(*reinterpret_cast<const bf16x4 *>(pa), *reinterpret_cast<const bf16x4 *>(pa + 4),
+                                *reinterpret_cast<const bf16x4 *>(pg), *reinterpret_cast<const bf16x4 *>(pg + 4), acc[t]);
             }
         }
         __syncthreads();

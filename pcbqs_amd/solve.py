@@ -395,3 +395,32 @@ def rl_equivalent(frequency, Z, tolerance=1e-7):
             raise ValueError('non-passive copper reduction; refine/check topology')
         mats.append((V * np.maximum(e, scale * 1e-12)) @ V.T)
     return mats[0], mats[1]
+
+
+def main(argv=None):
+    """Port impedances of a seeded board: the set-up (fill, combine, projection), then the solve at
+    the given frequencies; one JSON line with the set-up time and each solve's info."""
+    import argparse
+    import json
+    from . import board as boards
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument('--board', choices=['c1', 'c2', 'c5'], default='c2')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--solver', choices=['iterative', 'direct', 'auto'], default='iterative')
+    ap.add_argument('--precond', choices=['block-jacobi', 'two-level'], default='block-jacobi')
+    ap.add_argument('--frequencies', type=float, nargs='+', default=[1e6])
+    ap.add_argument('--tol', type=float, default=1e-8)
+    args = ap.parse_args(argv)
+    b = {'c1': lambda: boards.board_c1(), 'c2': lambda: boards.board_c2(args.seed),
+         'c5': lambda: boards.board_c5(args.seed)}[args.board]()
+    s = PortSolver(b, device=0)
+    print(f'set-up: {b.nt} triangles, {s.model.n_free} free loops, {s.setup_seconds}', flush=True)
+    Z, info = s.ports(args.frequencies, solver=args.solver, precond=args.precond, tol=args.tol)
+    print(json.dumps(dict(board=args.board, triangles=int(b.nt), free_loops=int(s.model.n_free),
+                          setup=s.setup_seconds, solves=[{k: v for k, v in i.items() if np.isscalar(v)}
+                                                         for i in info])), flush=True)
+    s.close()
+
+
+if __name__ == '__main__':
+    main()

@@ -181,11 +181,22 @@ __global__ __launch_bounds__(256) void mpnn_gemm_tn_kernel(int N, int chunk, con
 }
 
 // out[e] = sum over the nblk partials of element e, in block order.
-__global__ void mpnn_reduce_kernel(const float *__restrict__ part, int nblk, int width, float *__restrict__ out) {
+// Sum of the nblk partials (rows of width) in two fixed-order passes: slice y of gridDim.y sums
+// partials y, y + S, y + 2S, ... into slice[y]; then the slices are summed in order. The same result
+// for every schedule.
+__global__ void mpnn_reduce_kernel(const float *__restrict__ part, int nblk, int width, float *__restrict__ slice) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= width) return;
     float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * width + e];
+    for (int b = blockIdx.y; b < nblk; b += gridDim.y) s += part[(size_t)b * width + e];
+    slice[(size_t)blockIdx.y * width + e] = s;
+}
+
+__global__ void mpnn_reduce_slices_kernel(const float *__restrict__ slice, int slices, int width, float *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= width) return;
+    float s = 0.f;
+    for (int y = 0; y < slices; ++y) s += slice[(size_t)y * width + e];
     out[e] = s;
 }
 
@@ -266,12 +277,15 @@ int pq_mpnn_gemm(int N, const float *A1, int lda1, int k1, const float *A2, int 
     });
 }
 
-static int tn_blocks(int N) { return std::max(1, std::min(256, (N + 1023) / 1024)); }
+// Weight-gradient blocks: 128 rows each (two staged steps), at most 1024: a 30k-node graph gets
+// 235 workgroups (1,024-row blocks left 30 of the 256 CUs busy: 281 us per layer).
+static int tn_blocks(int N) { return std::max(1, std::min(1024, (N + 127) / 128)); }
+constexpr int TN_SLICES = 16;                 // first-pass slices of the partials' sum
 
 int pq_mpnn_workspace(int N, int k, int m, size_t *floats) {
     return guard([&] {
         PQ_REQUIRE(N > 0 && k > 0 && m > 0 && floats, "pq_mpnn_workspace: invalid argument");
-        *floats = (size_t)(tn_blocks(N) + 1) * ((size_t)k * m + m);
+        *floats = (size_t)(tn_blocks(N) + 1 + TN_SLICES) * ((size_t)k * m + m);
     });
 }
 
@@ -282,7 +296,7 @@ int pq_mpnn_gemm_tn(int N, const float *A1, int lda1, int k1, const float *A2, i
         const int nblk = tn_blocks(N);
         const int K = k1 + k2;
         const size_t width = (size_t)K * m + m;
-        PQ_REQUIRE(work_floats >= (size_t)(nblk + 1) * width, "pq_mpnn_gemm_tn: workspace too small");
+        PQ_REQUIRE(work_floats >= (size_t)(nblk + 1 + TN_SLICES) * width, "pq_mpnn_gemm_tn: workspace too small");
         int chunk = (N + nblk - 1) / nblk;
         chunk = (chunk + TN_ROWS - 1) / TN_ROWS * TN_ROWS;
         dispatch_shape(k1, k2, m, [&](auto a, auto b, auto c) {
@@ -291,8 +305,12 @@ int pq_mpnn_gemm_tn(int N, const float *A1, int lda1, int k1, const float *A2, i
                                A1, lda1, A2, lda2, G, ldg, work);
         });
         // dW then db: the partial of each block is (K * M weights, M column sums)
-        hipLaunchKernelGGL(mpnn_reduce_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                           work, nblk, (int)width, work + (size_t)nblk * width);
+        const int slices = std::min(TN_SLICES, nblk);
+        float *slice = work + (size_t)(nblk + 1) * width;
+        hipLaunchKernelGGL(mpnn_reduce_kernel, dim3((unsigned)((width + 255) / 256), (unsigned)slices), dim3(256), 0,
+                           (hipStream_t)stream, work, nblk, (int)width, slice);
+        hipLaunchKernelGGL(mpnn_reduce_slices_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, slice, slices, (int)width, work + (size_t)nblk * width);
         PQ_HIP(hipMemcpyAsync(dW, work + (size_t)nblk * width, (size_t)K * m * 4, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
         PQ_HIP(hipMemcpyAsync(db, work + (size_t)nblk * width + (size_t)K * m, (size_t)m * 4, hipMemcpyDeviceToDevice,

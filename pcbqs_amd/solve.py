@@ -410,11 +410,20 @@ def main(argv=None):
     ap.add_argument('--precond', choices=['block-jacobi', 'two-level'], default='block-jacobi')
     ap.add_argument('--frequencies', type=float, nargs='+', default=[1e6])
     ap.add_argument('--tol', type=float, default=1e-8)
+    ap.add_argument('--coarse', choices=['hats', 'learned'], default='learned',
+                    help='two-level coarse space: 36 x 36 hats and via loops, or with the learned enrichment')
     args = ap.parse_args(argv)
     b = {'c1': lambda: boards.board_c1(), 'c2': lambda: boards.board_c2(args.seed),
          'c5': lambda: boards.board_c5(args.seed)}[args.board]()
     s = PortSolver(b, device=0)
     print(f'set-up: {b.nt} triangles, {s.model.n_free} free loops, {s.setup_seconds}', flush=True)
+    if args.precond == 'two-level' and args.solver != 'direct':
+        if args.coarse == 'learned':
+            from .learned import MPNN, set_learned_coarse
+            set_learned_coarse(s, MPNN.load(), 36)
+        else:
+            s.set_stream_coarse(36)
+        print(f'coarse space: {s.coarse_k} columns', flush=True)
     Z, info = s.ports(args.frequencies, solver=args.solver, precond=args.precond, tol=args.tol)
     print(json.dumps(dict(board=args.board, triangles=int(b.nt), free_loops=int(s.model.n_free),
                           setup=s.setup_seconds, solves=[{k: v for k, v in i.items() if np.isscalar(v)}

@@ -376,13 +376,23 @@ int pq_loops_coarse_solution(pq_loops *loops, double omega, int P, const int32_t
 int pq_loops_set_coarse(pq_loops *loops, int k, const double *Phi, int diagonal);
 
 /* pq_loops_set_coarse of the span of Phi (n_free, k) FP64 host, re-expressed on
- * the device in the generalised eigenbasis of its Galerkin pair: Rc = Phi^T Rl Phi
- * = U S U^T and (S^-1/2 U^T) Lc (U S^-1/2) = Q Lam Q^T (rocSOLVER dsyevd, FP64),
- * directions with S below drop * max S removed, basis Psi = Phi U S^-1/2 Q with
- * Psi^T Rl Psi = I and Psi^T Ll Psi = Lam, so each frequency's coarse solve is a
- * diagonal scaling. *k_out: the kept dimension. No counterpart in the reference
- * (the host did the same with two eigh calls). */
+ * the device in the generalised eigenbasis of its Galerkin pair (Rc, Lc) =
+ * (Phi^T Rl Phi, Phi^T Ll Phi), FP64 (Ll the FP64 loop matrix). Where Rc's
+ * Cholesky factor Rc = U^T U exists and shows lambda_min(Rc) > drop *
+ * lambda_max(Rc) (1 / ||U^-1||_F^2 > drop * trace Rc), one eigensolve:
+ * U^-T Lc U^-1 = Q Lam Q^T, Psi = Phi U^-1 Q; otherwise Rc = V S V^T, directions
+ * with S below drop * max S removed, and (S^-1/2 V^T) Lc (V S^-1/2) = Q Lam Q^T,
+ * Psi = Phi V S^-1/2 Q (rocSOLVER dpotrf / dsyevd). Either way Psi^T Rl Psi = I
+ * and Psi^T Ll Psi = Lam, so each frequency's coarse solve is a diagonal
+ * scaling. *k_out: the kept dimension. No counterpart in the reference. */
 int pq_loops_set_coarse_diagonal(pq_loops *loops, int k, const double *Phi, double drop, int *k_out);
+
+/* pq_loops_set_coarse_diagonal with Phi given as k sparse columns (CSC over the
+ * n_free rows: colptr[k + 1], rows and vals[colptr[k]]), expanded on the device:
+ * only the nonzeros cross PCIe. *path_out: 1 when the Cholesky route was taken,
+ * 2 for the eigenvector route. No counterpart in the reference. */
+int pq_loops_set_coarse_csc(pq_loops *loops, int k, const int32_t *colptr, const int32_t *rows,
+                            const double *vals, double drop, int *k_out, int *path_out);
 
 /* The Galerkin matrices Phi^T Rl Phi and Phi^T Ll Phi (k x k FP64, row-major)
  * of the current coarse space, for a host-side change of basis. No counterpart

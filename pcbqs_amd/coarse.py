@@ -8,6 +8,7 @@ branch and no other loop uses that branch). Coarse patterns:
   stream_patterns  piecewise-bilinear hat functions of a G x G grid over each layer (one hat per
                    layer dropped: all hats together sum to a constant, which carries no current)
   loop_restrict    branch patterns -> loop-space columns (the cotree rows of the free loops)
+  unique_columns   repeated loop columns removed (tiny hats can restrict to the same vector)
 The learned preconditioner (pcbqs_amd.learned) adds patterns predicted by a graph network.
 """
 import numpy as np
@@ -96,9 +97,10 @@ def stream_patterns(model, grid=8):
     return m[:, used]
 
 
-def loop_restrict(model, patterns):
+def loop_restrict(model, patterns, dense=True):
     """Branch patterns (nb, k), dense or sparse -> free-loop columns (n_free, k), each normalised to
-    unit 2-norm."""
+    unit 2-norm (zero columns dropped); a scipy CSC matrix when dense=False and the patterns are
+    sparse."""
     cot = model.basis.cotree[:model.n_free]
     if hasattr(patterns, 'tocsr'):
         sel = patterns.tocsr()[cot].tocsc()
@@ -107,6 +109,9 @@ def loop_restrict(model, patterns):
         sel = sel[:, keep]
         # scale each stored entry by its column's norm, then densify once
         sel.data = sel.data / np.repeat(norm[keep], np.diff(sel.indptr))
+        if not dense:
+            sel.sort_indices()
+            return sel
         return np.ascontiguousarray(sel.toarray(), np.float64)
     phi = np.ascontiguousarray(patterns[cot], np.float64)
     norm = np.linalg.norm(phi, axis=0)
@@ -114,11 +119,37 @@ def loop_restrict(model, patterns):
     return np.ascontiguousarray(phi[:, keep] / norm[keep])
 
 
-def via_loop_columns(model):
+def unique_columns(m, tol=1e-12):
+    """The columns of a sparse matrix m of unit columns without those that equal an earlier column up
+    to sign (same nonzero rows, values within tol): such a column adds nothing to the span, and
+    leaving it in would make the coarse space's Galerkin matrices singular. Tiny hats on small copper
+    pieces can restrict to the same loop vector."""
+    from scipy import sparse
+    m = sparse.csc_matrix(m)
+    m.sort_indices()
+    seen, keep = {}, []
+    for c in range(m.shape[1]):
+        s, e = m.indptr[c], m.indptr[c + 1]
+        v = m.data[s:e]
+        if e > s and v[0] < 0:
+            v = -v
+        group = seen.setdefault(m.indices[s:e].tobytes(), [])
+        if any(np.max(np.abs(w - v), initial=0.0) <= tol for w in group):
+            continue
+        group.append(v)
+        keep.append(c)
+    return m if len(keep) == m.shape[1] else m[:, keep]
+
+
+def via_loop_columns(model, dense=True):
     """Unit loop-space columns of the free loops closed by a via segment: the inter-layer
-    circulations (down one via, back up another) that no per-layer stream function carries."""
+    circulations (down one via, back up another) that no per-layer stream function carries. A scipy
+    CSC matrix when dense=False."""
     cot = model.basis.cotree[:model.n_free]
     sel = np.flatnonzero((cot >= model.ne) & (cot < model.ne + model.nv))
+    if not dense:
+        from scipy import sparse
+        return sparse.csc_matrix((np.ones(len(sel)), (sel, np.arange(len(sel)))), shape=(model.n_free, len(sel)))
     cols = np.zeros((model.n_free, len(sel)))
     cols[sel, np.arange(len(sel))] = 1.0
     return cols

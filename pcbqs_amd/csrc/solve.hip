@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void operator_epilogue_kernel(const int *__res
 //                                             in wave order.
 // A diagonal tile does the direct product only. Output row block K receives exactly one partial
 // per block index s: the direct one of tile (K, s) for s >= K and the transposed one of tile
-// (s, K) for s < K, so gemv_epilogue_kernel / gemv_sum_kernel add nb = ceil(n / 256) partials in
+// (s, K) for s < K, so the epilogue adds nb = ceil(n / 256) partials in
 // a fixed order. While one 64-column panel is reduced, the next one is already loading.
 constexpr int SYM_T = 256, SYM_SUB = 64, SYM_PITCH = SYM_SUB + 1;
 
@@ -1061,15 +1061,6 @@ __global__ void round_basis_kernel(double *Phi, int n, int k, int kp, float *Out
     Out[e] = v;
 }
 
-// Out[i][c0 + c] = W[i][c] for c < cn (FP64 (n, ld) chunk -> FP64 (n, k)).
-__global__ void scatter_cols64_kernel(const double *W, int n, int ld, int c0, int cn, int k, double *Out) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * ld) return;
-    const size_t i = e / ld;
-    const int c = (int)(e % ld);
-    if (c < cn) Out[i * k + c0 + c] = W[e];
-}
-
 // A := (A + A^T) / 2 for a k x k matrix (one thread per unordered pair).
 __global__ void symmetrize64_kernel(double *A, int k) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1098,42 +1089,96 @@ __global__ void transpose_round_kernel(const double *In, int n, int k, float *Ou
     Out[c * n + i] = (float)In[e];
 }
 
-// W = sum of the nsplit column-chunk partials of panel_kernel (a real product, no frequency).
-template <int NR>
-__global__ void gemv_sum_kernel(const double *part, int nsplit, int n, double *W) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * NR) return;
-    double s = 0;
-    const size_t row = e / NR, r = e % NR;
-    for (int q = 0; q < nsplit; ++q) s += part[(row * nsplit + q) * NR + r];
-    W[e] = s;
+// Phi (n, k) FP64 row-major from k CSC columns (zero beforehand): a thread per column.
+__global__ void csc_scatter_kernel(const int *__restrict__ colptr, const int *__restrict__ rows,
+                                   const double *__restrict__ vals, int k, double *__restrict__ Phi) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    for (int t = colptr[c]; t < colptr[c + 1]; ++t) Phi[(size_t)rows[t] * k + c] = vals[t];
 }
 
-// X[i][c] = Phi[i][c0 + c] for c < cn, 0 for cn <= c < ld (FP32 -> FP64); Phi has row stride kp.
-__global__ void gather_cols_kernel(const float *Phi, int n, int kp, int c0, int cn, int ld, double *X) {
+// A := (double)(float)A over count entries: the FP32 basis every later product and Galerkin matrix
+// uses, kept in FP64 storage.
+__global__ void round_inplace_kernel(double *A, size_t count) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * ld) return;
-    const size_t i = e / ld;
-    const int c = (int)(e % ld);
-    X[e] = c < cn ? (double)Phi[i * kp + c0 + c] : 0.0;
+    if (e < count) A[e] = (double)(float)A[e];
 }
 
-// X[i][c] = Phi[i][c0 + c] for c < cn, else 0 (FP64 (n, k) -> FP64 (n, ld) chunk).
-__global__ void gather_cols64_kernel(const double *Phi, int n, int k, int c0, int cn, int ld, double *X) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * ld) return;
-    const size_t i = e / ld;
-    const int c = (int)(e % ld);
-    X[e] = c < cn ? Phi[i * k + c0 + c] : 0.0;
+// Y (n, k) = Rl X on the free block (X, Y FP64 row-major, k <= 4096 columns): a workgroup per row,
+// each thread holding up to 16 of the row's output columns (c = threadIdx.x + 256 q), so that every
+// nonzero of the row issues 16 independent loads of X's row; each output is the nonzeros' sum in
+// CSR order.
+__global__ __launch_bounds__(256) void csr_rows_kernel(const int *__restrict__ ptr, const int *__restrict__ col,
+                                                       const double *__restrict__ val, int n,
+                                                       const double *__restrict__ X, int k,
+                                                       double *__restrict__ Y) {
+    constexpr int Q = 16;
+    const int row = blockIdx.x;
+    double acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0;
+    for (int t = ptr[row]; t < ptr[row + 1]; ++t) {
+        const int j = col[t];
+        if (j >= n) continue;
+        const double v = val[t];
+        const double *xr = X + (size_t)j * k;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int c = threadIdx.x + 256 * q;
+            if (c < k) acc[q] += v * xr[c];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int c = threadIdx.x + 256 * q;
+        if (c < k) Y[(size_t)row * k + c] = acc[q];
+    }
 }
 
-// Out[c0 + c][i] = W[i][c] for c < cn (FP64 (n, ld) -> FP32 transposed (k, n)).
-__global__ void scatter_cols_kernel(const double *W, int n, int ld, int c0, int cn, float *Out) {
+// Partial sums for strided_sum: part[b] = sum over the entries e = b, b + nb, ... (< count) of
+// A[e * stride] or its square, each workgroup's 256 threads combined by a fixed LDS tree.
+__global__ __launch_bounds__(256) void strided_partial_kernel(const double *__restrict__ A, size_t count,
+                                                              size_t stride, int square, double *__restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0;
+    const size_t nthreads = (size_t)gridDim.x * 256;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < count; e += nthreads) {
+        const double v = A[e * stride];
+        acc += square ? v * v : v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[0] = sum over e < count of A[e * stride] (square = 0) or of its square (square = 1): one
+// 1024-thread workgroup, each thread's strided partial then a fixed LDS tree (deterministic).
+__global__ __launch_bounds__(1024) void strided_sum_kernel(const double *__restrict__ A, size_t count, size_t stride,
+                                                           int square, double *__restrict__ out) {
+    __shared__ double part[1024];
+    double acc = 0;
+    for (size_t e = threadIdx.x; e < count; e += 1024) {
+        const double v = A[e * stride];
+        acc += square ? v * v : v;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// A (k x k, column-major) := the identity.
+__global__ void identity_kernel(double *A, int k) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * ld) return;
-    const size_t i = e / ld;
-    const int c = (int)(e % ld);
-    if (c < cn) Out[(size_t)(c0 + c) * n + i] = (float)W[e];
+    if (e >= (size_t)k * k) return;
+    A[e] = (e / k == e % k) ? 1.0 : 0.0;
 }
 
 // Y += X
@@ -2193,6 +2238,8 @@ struct RocLib {
     decltype(&rocsolver_zgetrs) getrs = nullptr;
     decltype(&rocblas_dgemm) dgemm = nullptr;
     decltype(&rocsolver_dsyevd) syevd = nullptr;
+    decltype(&rocsolver_dpotrf) potrf = nullptr;
+    decltype(&rocblas_dtrsm) trsm = nullptr;
     std::string error;
 };
 
@@ -2215,7 +2262,10 @@ const RocLib &roc_lib() {
         lu.getrs = (decltype(lu.getrs))dlsym(solver, "rocsolver_zgetrs");
         lu.dgemm = (decltype(lu.dgemm))dlsym(blas, "rocblas_dgemm");
         lu.syevd = (decltype(lu.syevd))dlsym(solver, "rocsolver_dsyevd");
-        if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs || !lu.dgemm || !lu.syevd)
+        lu.potrf = (decltype(lu.potrf))dlsym(solver, "rocsolver_dpotrf");
+        lu.trsm = (decltype(lu.trsm))dlsym(blas, "rocblas_dtrsm");
+        if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs || !lu.dgemm || !lu.syevd ||
+            !lu.potrf || !lu.trsm)
             lu.error = "rocBLAS/rocSOLVER entry points missing";
     });
     if (!lu.error.empty()) throw Error(PQ_EDEVICE, lu.error);
@@ -2337,34 +2387,31 @@ void make_half_bases(pq_loops *L) {
     }
 }
 
-// Rl X and Ll X (FP64 (n, k) row-major each) of the k columns of X (FP64 (n, k)), 16 columns per
-// operator product.
+// Rl X and Ll X (FP64 (n, k) row-major each) of the k columns of X (FP64 (n, k)): Rl X by CSR rows,
+// Ll X as one FP64 GEMM on the FP64 loop matrix (symmetric, so its rows serve as its columns).
 void basis_products(pq_loops *L, const double *X64, int k, double *RX, double *LX) {
     const int n = L->nf;
-    constexpr int CH = 16;
-    tune_gemv(L, CH);
-    const OpShape shape = gemv_shape(L, CH);
-    const int nsplit = operator_splits(n, shape.chunk);
-    DevBuf X, WR, WL, part;
-    X.reserve((size_t)n * CH * 8); WR.reserve((size_t)n * CH * 8); WL.reserve((size_t)n * CH * 8);
-    part.reserve((size_t)nsplit * n * CH * 8);
-    const unsigned vb = (unsigned)(((size_t)n * CH + 255) / 256);
-    for (int c0 = 0; c0 < k; c0 += CH) {
-        const int cn = std::min(CH, k - c0);
-        hipLaunchKernelGGL(gather_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, X64, n, k, c0, cn, CH,
-                           X.as<double>());
-        hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
-                           L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
-        operator_partials<CH>(L, shape, X.as<double>(), part.as<double>());
-        hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
-                           WL.as<double>());
-        hipLaunchKernelGGL(scatter_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, WR.as<double>(), n, CH, c0, cn, k,
-                           RX);
-        hipLaunchKernelGGL(scatter_cols64_kernel, dim3(vb), dim3(256), 0, L->stream, WL.as<double>(), n, CH, c0, cn, k,
-                           LX);
-    }
+    hipLaunchKernelGGL(csr_rows_kernel, dim3(n), dim3(256), 0, L->stream, L->rptr.as<int>(), L->rcol.as<int>(),
+                       L->rval.as<double>(), n, X64, k, RX);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipStreamSynchronize(L->stream));             // the chunk buffers are freed on return
+    gemm_rm(L, false, false, n, k, n, L->Ll64.as<double>(), (int)L->ldl, X64, k, LX, k);
+}
+
+// A scalar from a device sum (strided_sum_kernel), synchronising the handle's stream.
+double device_sum(pq_loops *L, const double *A, size_t count, size_t stride, bool square) {
+    constexpr int NB = 512;                                // partials: a fixed count, a fixed order
+    DevBuf out, part;
+    out.reserve(8);
+    part.reserve(NB * 8);
+    hipLaunchKernelGGL(strided_partial_kernel, dim3(NB), dim3(256), 0, L->stream, A, count, stride, square ? 1 : 0,
+                       part.as<double>());
+    hipLaunchKernelGGL(strided_sum_kernel, dim3(1), dim3(1024), 0, L->stream, part.as<double>(), (size_t)NB, (size_t)1,
+                       0, out.as<double>());
+    PQ_HIP(hipGetLastError());
+    double v = 0;
+    PQ_HIP(hipMemcpyAsync(&v, out.p, 8, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    return v;
 }
 
 // Eigen-decomposition of a symmetric k x k FP64 matrix A in place on the device (rocSOLVER dsyevd):
@@ -2381,6 +2428,140 @@ void device_eigh(pq_loops *L, double *A, int k, std::vector<double> &s) {
     PQ_HIP(hipMemcpyAsync(s.data(), D.p, (size_t)k * 8, hipMemcpyDeviceToHost, L->stream));
     PQ_HIP(hipStreamSynchronize(L->stream));
     PQ_REQUIRE(bad == 0, "coarse space: the eigensolver did not converge");
+}
+
+
+// The coarse space of the span of P64 (n_free, k) FP64 row-major on the device, re-expressed in the
+// generalised eigenbasis of its Galerkin pair (Rc, Lc) = (P^T Rl P, P^T Ll64 P), so that every
+// frequency's coarse solve is a diagonal scaling. Two routes to the pair's eigenbasis:
+//  1 (Cholesky): Rc = U^T U (rocSOLVER dpotrf), U^-1 by a triangular solve of the identity,
+//    M = U^-T Lc U^-1 = Q Lam Q^T (one dsyevd), T = U^-1 Q; taken when dpotrf succeeds and
+//    1 / ||U^-1||_F^2 > drop * trace(Rc): since trace(Rc^-1) = ||U^-1||_F^2 <= k / lambda_min and
+//    trace(Rc) >= lambda_max, that guarantees lambda_min > drop * lambda_max, i.e. the eigenvector
+//    route below would drop nothing;
+//  2 (eigenvectors): Rc = V S V^T (dsyevd), directions with S below drop * max S removed,
+//    W = V S^-1/2, W^T Lc W = Q Lam Q^T (dsyevd), T = W Q.
+// Either way Psi = P T has Psi^T Rl Psi = I and Psi^T Ll Psi = Lam. P64 is rounded to FP32 first
+// (the basis every product uses). *path_out: the route taken.
+void coarse_diagonalise(pq_loops *L, DevBuf &P64, int k, double drop, int *k_out, int *path_out) {
+    const int n = L->nf;
+    L->kc = 0;
+    L->omega_c = -1;
+    L->crows.clear(); L->psplit.clear(); L->pdepth.clear(); L->prows.clear();
+    DevBuf RX, LX, Rc, Lc, W, M, T, Y;
+    {
+        const size_t cnt = (size_t)n * k;
+        hipLaunchKernelGGL(round_inplace_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream,
+                           P64.as<double>(), cnt);
+        PQ_HIP(hipGetLastError());
+    }
+    RX.reserve((size_t)n * k * 8); LX.reserve((size_t)n * k * 8);
+    basis_products(L, P64.as<double>(), k, RX.as<double>(), LX.as<double>());
+    Rc.reserve((size_t)k * k * 8); Lc.reserve((size_t)k * k * 8);
+    gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, RX.as<double>(), k, Rc.as<double>(), k);
+    gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, LX.as<double>(), k, Lc.as<double>(), k);
+    const unsigned kb = (unsigned)(((size_t)k * k + 255) / 256);
+    hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Rc.as<double>(), k);
+    hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Lc.as<double>(), k);
+    PQ_HIP(hipGetLastError());
+    std::vector<double> lam;
+    int kk = 0;
+    const RocLib &lib = roc_lib();
+    rocblas_handle h = blas_handle(L);
+    {   // route 1: Cholesky of Rc
+        const double trace = device_sum(L, Rc.as<double>(), (size_t)k, (size_t)k + 1, false);
+        DevBuf U, Ui, info;
+        U.reserve((size_t)k * k * 8); Ui.reserve((size_t)k * k * 8); info.reserve(4);
+        PQ_HIP(hipMemcpyAsync(U.p, Rc.p, (size_t)k * k * 8, hipMemcpyDeviceToDevice, L->stream));
+        blas_check(lib.potrf(h, rocblas_fill_upper, k, U.as<double>(), k, info.as<rocblas_int>()), "rocsolver_dpotrf");
+        rocblas_int bad = 0;
+        PQ_HIP(hipMemcpyAsync(&bad, info.p, 4, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        bool ok = bad == 0 && trace > 0;
+        if (ok) {
+            hipLaunchKernelGGL(identity_kernel, dim3(kb), dim3(256), 0, L->stream, Ui.as<double>(), k);
+            PQ_HIP(hipGetLastError());
+            const double one = 1.0;
+            blas_check(lib.trsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                rocblas_diagonal_non_unit, k, k, &one, U.as<double>(), k, Ui.as<double>(), k),
+                       "rocblas_dtrsm");                                   // Ui = U^-1 (column-major)
+            const double inv_trace = device_sum(L, Ui.as<double>(), (size_t)k * k, 1, true);
+            ok = std::isfinite(inv_trace) && inv_trace > 0 && 1.0 / inv_trace > drop * trace;
+        }
+        if (ok) {
+            // Ui column-major read row-major is Ui^T: M = Ui^T Lc Ui, T = Ui Q (row-major k x k)
+            T.reserve((size_t)k * k * 8); M.reserve((size_t)k * k * 8);
+            gemm_rm(L, false, true, k, k, k, Lc.as<double>(), k, Ui.as<double>(), k, T.as<double>(), k);  // Lc Ui
+            gemm_rm(L, false, false, k, k, k, Ui.as<double>(), k, T.as<double>(), k, M.as<double>(), k);  // Ui^T (.)
+            hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, M.as<double>(), k);
+            PQ_HIP(hipGetLastError());
+            device_eigh(L, M.as<double>(), k, lam);                     // M's columns: Q
+            gemm_rm(L, true, true, k, k, k, Ui.as<double>(), k, M.as<double>(), k, T.as<double>(), k);
+            kk = k;
+            *path_out = 1;
+        }
+    }
+    if (!kk) {   // route 2: eigenvectors of Rc
+        std::vector<double> sv;
+        device_eigh(L, Rc.as<double>(), k, sv);                        // Rc's columns: V
+        const double smax = sv.empty() ? 0.0 : sv.back();
+        PQ_REQUIRE(smax > 0, "coarse space: the Galerkin resistance is not positive");
+        int first = 0;
+        while (first < k && !(sv[first] > drop * smax)) ++first;
+        kk = k - first;
+        W.reserve((size_t)k * kk * 8);
+        {
+            DevBuf sd;
+            sd.reserve((size_t)k * 8);
+            PQ_HIP(hipMemcpyAsync(sd.p, sv.data(), (size_t)k * 8, hipMemcpyHostToDevice, L->stream));
+            hipLaunchKernelGGL(whiten_kernel, dim3((unsigned)(((size_t)k * kk + 255) / 256)), dim3(256), 0, L->stream,
+                               Rc.as<double>(), sd.as<double>(), k, first, kk, W.as<double>());
+            PQ_HIP(hipGetLastError());
+            PQ_HIP(hipStreamSynchronize(L->stream));
+        }
+        T.reserve((size_t)k * kk * 8);
+        M.reserve((size_t)kk * kk * 8);
+        gemm_rm(L, false, false, k, kk, k, Lc.as<double>(), k, W.as<double>(), kk, T.as<double>(), kk);   // Lc W
+        gemm_rm(L, true, false, kk, kk, k, W.as<double>(), kk, T.as<double>(), kk, M.as<double>(), kk);    // W^T Lc W
+        hipLaunchKernelGGL(symmetrize64_kernel, dim3((unsigned)(((size_t)kk * kk + 255) / 256)), dim3(256), 0,
+                           L->stream, M.as<double>(), kk);
+        PQ_HIP(hipGetLastError());
+        device_eigh(L, M.as<double>(), kk, lam);                       // M's columns: Q
+        // T = W Q (M holds Q column-major: as a row-major array it is Q^T)
+        gemm_rm(L, false, true, k, kk, kk, W.as<double>(), kk, M.as<double>(), kk, T.as<double>(), kk);
+        *path_out = 2;
+    }
+    // Psi = Phi T, Rl Psi = (Rl Phi) T, Ll Psi = (Ll Phi) T
+    Y.reserve((size_t)n * kk * 8);
+    const int kpp = (kk + 63) / 64 * 64;
+    L->kc = kk;
+    L->kp = kpp;
+    L->Phi.reserve((size_t)n * kpp * 4); L->Phit.reserve((size_t)n * kk * 4);
+    L->RPhi.reserve((size_t)n * kk * 4); L->LPhi.reserve((size_t)n * kk * 4);
+    const unsigned nb = (unsigned)(((size_t)n * kk + 255) / 256);
+    gemm_rm(L, false, false, n, kk, k, P64.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+    hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kpp + 255) / 256)), dim3(256), 0, L->stream,
+                       Y.as<double>(), n, kk, kpp, L->Phi.as<float>());
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                       L->Phit.as<float>());
+    gemm_rm(L, false, false, n, kk, k, RX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                       L->RPhi.as<float>());
+    gemm_rm(L, false, false, n, kk, k, LX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+                       L->LPhi.as<float>());
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    // the Galerkin pair in the new basis: I and Lam (their off-diagonal parts are rounding)
+    L->Rc.assign((size_t)kk * kk, 0.0);
+    L->Lc.assign((size_t)kk * kk, 0.0);
+    for (int a = 0; a < kk; ++a) {
+        L->Rc[(size_t)a * kk + a] = 1.0;
+        L->Lc[(size_t)a * kk + a] = lam[a];
+    }
+    L->cdiag = true;
+    make_half_bases(L);
+    *k_out = kk;
 }
 
 }  // namespace
@@ -3454,9 +3635,9 @@ int pq_loops_factor_time(pq_loops *L, double *ms) {
 }
 
 // The coarse space of the two-level preconditioner: Phi (n_free, k) FP64 host, row-major, rounded
-// to FP32 once (the basis every later product uses). Rl Phi and Ll Phi and the Galerkin matrices
-// Phi^T Rl Phi, Phi^T Ll Phi are formed on the device in chunks of 16 columns; each frequency then
-// only inverts the k x k coarse matrix.
+// to FP32 once (the basis every later product uses). Rl Phi, Ll Phi (FP64 loop matrix) and the
+// Galerkin matrices Phi^T Rl Phi, Phi^T Ll Phi are formed on the device in FP64; each frequency
+// then only inverts the k x k coarse matrix.
 int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
     return guard([&] {
         PQ_REQUIRE(L && (k == 0 || Phi), "pq_loops_set_coarse: null argument");
@@ -3475,71 +3656,41 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         const int kp = L->kp;
         L->Phi.reserve((size_t)n * kp * 4); L->Phit.reserve((size_t)n * k * 4);
         L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
-        {
-            std::vector<float> f32((size_t)n * kp, 0.0f);
-            for (size_t i = 0; i < (size_t)n; ++i)
-                for (int c = 0; c < k; ++c) f32[i * kp + c] = (float)Phi[i * k + c];
-            PQ_HIP(hipMemcpyAsync(L->Phi.p, f32.data(), f32.size() * 4, hipMemcpyHostToDevice, L->stream));
+        DevBuf P64, RX, LX, G;
+        P64.reserve((size_t)n * k * 8); RX.reserve((size_t)n * k * 8); LX.reserve((size_t)n * k * 8);
+        G.reserve((size_t)k * k * 8);
+        PQ_HIP(hipMemcpyAsync(P64.p, Phi, (size_t)n * k * 8, hipMemcpyHostToDevice, L->stream));
+        // the FP32 basis (P64 rounded to it in place), its transpose, then Rl Phi and Ll Phi (FP64
+        // loop matrix) and the Galerkin matrices Phi^T Rl Phi, Phi^T Ll Phi in FP64
+        hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kp + 255) / 256)), dim3(256), 0, L->stream,
+                           P64.as<double>(), n, k, kp, L->Phi.as<float>());
+        const unsigned nb = (unsigned)(((size_t)n * k + 255) / 256);
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, P64.as<double>(), n, k,
+                           L->Phit.as<float>());
+        PQ_HIP(hipGetLastError());
+        basis_products(L, P64.as<double>(), k, RX.as<double>(), LX.as<double>());
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, RX.as<double>(), n, k,
+                           L->RPhi.as<float>());
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, LX.as<double>(), n, k,
+                           L->LPhi.as<float>());
+        PQ_HIP(hipGetLastError());
+        const unsigned kb = (unsigned)(((size_t)k * k + 255) / 256);
+        for (int which = 0; which < 2; ++which) {
+            std::vector<double> &Gh = which ? L->Lc : L->Rc;
+            Gh.assign((size_t)k * k, 0.0);
+            gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, (which ? LX : RX).as<double>(), k, G.as<double>(), k);
+            // the Galerkin matrices of symmetric operators: average away the rounding asymmetry
+            hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, G.as<double>(), k);
+            PQ_HIP(hipGetLastError());
+            PQ_HIP(hipMemcpyAsync(Gh.data(), G.p, (size_t)k * k * 8, hipMemcpyDeviceToHost, L->stream));
             PQ_HIP(hipStreamSynchronize(L->stream));
         }
-        hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)(((size_t)n * k + 255) / 256)), dim3(256), 0, L->stream,
-                           L->Phi.as<float>(), n, k, kp, L->Phit.as<float>());
-        constexpr int CH = 16;
-        tune_gemv(L, CH);
-        const OpShape shape = gemv_shape(L, CH);
-        const int nsplit = operator_splits(n, shape.chunk);
-        const int nchunk = (n + cr_rows(CH) - 1) / cr_rows(CH);
-        DevBuf X, WR, WL, part, red, t;
-        X.reserve((size_t)n * CH * 8); WR.reserve((size_t)n * CH * 8); WL.reserve((size_t)n * CH * 8);
-        part.reserve((size_t)nsplit * n * CH * 8);
-        red.reserve((size_t)nchunk * k * CH * 8);
-        t.reserve((size_t)k * CH * 8);
-        L->Rc.assign((size_t)k * k, 0.0);
-        L->Lc.assign((size_t)k * k, 0.0);
-        std::vector<double> ht((size_t)k * CH);
-        const unsigned vb = (unsigned)(((size_t)n * CH + 255) / 256);
-        for (int c0 = 0; c0 < k; c0 += CH) {
-            const int cn = std::min(CH, k - c0);
-            hipLaunchKernelGGL(gather_cols_kernel, dim3(vb), dim3(256), 0, L->stream, L->Phi.as<float>(), n, kp, c0, cn,
-                               CH, X.as<double>());
-            hipLaunchKernelGGL(spmv_kernel<CH>, dim3((n + 3) / 4), dim3(256), 0, L->stream, L->rptr.as<int>(),
-                               L->rcol.as<int>(), L->rval.as<double>(), n, X.as<double>(), WR.as<double>());
-            operator_partials<CH>(L, shape, X.as<double>(), part.as<double>());
-            hipLaunchKernelGGL(gemv_sum_kernel<CH>, dim3(vb), dim3(256), 0, L->stream, part.as<double>(), nsplit, n,
-                               WL.as<double>());
-            for (int which = 0; which < 2; ++which) {
-                const double *W = which ? WL.as<double>() : WR.as<double>();
-                hipLaunchKernelGGL(scatter_cols_kernel, dim3(vb), dim3(256), 0, L->stream, W, n, CH, c0, cn,
-                                   (which ? L->LPhi : L->RPhi).as<float>());
-                restrict_launch<CH>(L, L->Phi.as<float>(), k, W, cr_rows(CH), red.as<double>());
-                hipLaunchKernelGGL(coarse_sum_kernel, dim3((k * CH + 255) / 256), dim3(256), 0, L->stream,
-                                   red.as<double>(), nchunk, k * CH, t.as<double>());
-                PQ_HIP(hipGetLastError());
-                PQ_HIP(hipMemcpyAsync(ht.data(), t.p, ht.size() * 8, hipMemcpyDeviceToHost, L->stream));
-                PQ_HIP(hipStreamSynchronize(L->stream));
-                std::vector<double> &G = which ? L->Lc : L->Rc;
-                for (int a = 0; a < k; ++a)
-                    for (int r = 0; r < cn; ++r) G[(size_t)a * k + c0 + r] = ht[(size_t)a * CH + r];
-            }
-        }
-        // the Galerkin matrices of symmetric operators: average away the rounding asymmetry
-        for (std::vector<double> *G : {&L->Rc, &L->Lc})
-            for (int a = 0; a < k; ++a)
-                for (int b = a + 1; b < k; ++b) {
-                    const double v = 0.5 * ((*G)[(size_t)a * k + b] + (*G)[(size_t)b * k + a]);
-                    (*G)[(size_t)a * k + b] = (*G)[(size_t)b * k + a] = v;
-                }
         make_half_bases(L);
     });
 }
 
-// The coarse space re-expressed, on the device, in the generalised eigenbasis of its Galerkin pair:
-// Phi (n_free, k) FP64 host, rounded to FP32 (the basis the products use); Rc = Phi^T Rl Phi and
-// Lc = Phi^T Ll Phi (FP64 dgemm on the FP64 products Rl Phi, Ll Phi); Rc = U S U^T (dsyevd),
-// directions with S below drop * max S dropped, W = U S^-1/2; W^T Lc W = Q Lam Q^T (dsyevd); the
-// basis Psi = Phi W Q then has Psi^T Rl Psi = I and Psi^T Ll Psi = Lam, so every frequency's coarse
-// solve is a diagonal scaling. Rl Psi and Ll Psi are (Rl Phi) W Q and (Ll Phi) W Q. *k_out receives
-// the kept dimension.
+// The coarse space of Phi (n_free, k) FP64 host re-expressed on the device in the generalised
+// eigenbasis of its Galerkin pair (coarse_diagonalise). *k_out receives the kept dimension.
 int pq_loops_set_coarse_diagonal(pq_loops *L, int k, const double *Phi, double drop, int *k_out) {
     return guard([&] {
         PQ_REQUIRE(L && Phi && k_out, "pq_loops_set_coarse_diagonal: null argument");
@@ -3548,90 +3699,45 @@ int pq_loops_set_coarse_diagonal(pq_loops *L, int k, const double *Phi, double d
         DeviceScope scope(L->device);
         const int n = L->nf;
         PQ_REQUIRE(n > 0, "pq_loops_set_coarse_diagonal: no free loops");
-        L->kc = 0;
-        L->omega_c = -1;
-        L->crows.clear(); L->psplit.clear(); L->pdepth.clear(); L->prows.clear();
-        DevBuf P64, RX, LX, Rc, Lc, W, M, T, Y;
+        DevBuf P64;
         P64.reserve((size_t)n * k * 8);
         PQ_HIP(hipMemcpyAsync(P64.p, Phi, (size_t)n * k * 8, hipMemcpyHostToDevice, L->stream));
-        {   // the FP32 basis, and P64 rounded to it
-            DevBuf tmp;
-            const int kp = (k + 63) / 64 * 64;
-            tmp.reserve((size_t)n * kp * 4);
-            hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kp + 255) / 256)), dim3(256), 0,
-                               L->stream, P64.as<double>(), n, k, kp, tmp.as<float>());
-            L->Phi.reserve((size_t)n * kp * 4);
-            PQ_HIP(hipMemcpyAsync(L->Phi.p, tmp.p, (size_t)n * kp * 4, hipMemcpyDeviceToDevice, L->stream));
-            L->kp = kp;
-            PQ_HIP(hipStreamSynchronize(L->stream));
+        int path = 0;
+        coarse_diagonalise(L, P64, k, drop, k_out, &path);
+    });
+}
+
+int pq_loops_set_coarse_csc(pq_loops *L, int k, const int32_t *colptr, const int32_t *rows, const double *vals,
+                            double drop, int *k_out, int *path_out) {
+    return guard([&] {
+        PQ_REQUIRE(L && colptr && k_out && path_out, "pq_loops_set_coarse_csc: null argument");
+        PQ_REQUIRE(k >= 1 && k <= 4096, "pq_loops_set_coarse_csc: 1 <= k <= 4096");
+        PQ_REQUIRE(drop >= 0 && drop < 1, "pq_loops_set_coarse_csc: 0 <= drop < 1");
+        const int n = L->nf;
+        PQ_REQUIRE(n > 0, "pq_loops_set_coarse_csc: no free loops");
+        PQ_REQUIRE(colptr[0] == 0, "pq_loops_set_coarse_csc: colptr[0] must be 0");
+        for (int c = 0; c < k; ++c)
+            PQ_REQUIRE(colptr[c + 1] >= colptr[c], "pq_loops_set_coarse_csc: column pointers out of order");
+        const int64_t nnz = colptr[k];
+        PQ_REQUIRE(nnz == 0 || (rows && vals), "pq_loops_set_coarse_csc: null entries");
+        for (int64_t t = 0; t < nnz; ++t)
+            PQ_REQUIRE(rows[t] >= 0 && rows[t] < n, "pq_loops_set_coarse_csc: row index out of range");
+        DeviceScope scope(L->device);
+        DevBuf P64, cp, rw, vl;
+        P64.reserve((size_t)n * k * 8);
+        cp.reserve((size_t)(k + 1) * 4);
+        rw.reserve((size_t)std::max<int64_t>(nnz, 1) * 4);
+        vl.reserve((size_t)std::max<int64_t>(nnz, 1) * 8);
+        PQ_HIP(hipMemsetAsync(P64.p, 0, (size_t)n * k * 8, L->stream));
+        PQ_HIP(hipMemcpyAsync(cp.p, colptr, (size_t)(k + 1) * 4, hipMemcpyHostToDevice, L->stream));
+        if (nnz) {
+            PQ_HIP(hipMemcpyAsync(rw.p, rows, (size_t)nnz * 4, hipMemcpyHostToDevice, L->stream));
+            PQ_HIP(hipMemcpyAsync(vl.p, vals, (size_t)nnz * 8, hipMemcpyHostToDevice, L->stream));
         }
-        RX.reserve((size_t)n * k * 8); LX.reserve((size_t)n * k * 8);
-        basis_products(L, P64.as<double>(), k, RX.as<double>(), LX.as<double>());
-        Rc.reserve((size_t)k * k * 8); Lc.reserve((size_t)k * k * 8);
-        gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, RX.as<double>(), k, Rc.as<double>(), k);
-        gemm_rm(L, true, false, k, k, n, P64.as<double>(), k, LX.as<double>(), k, Lc.as<double>(), k);
-        const unsigned kb = (unsigned)(((size_t)k * k + 255) / 256);
-        hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Rc.as<double>(), k);
-        hipLaunchKernelGGL(symmetrize64_kernel, dim3(kb), dim3(256), 0, L->stream, Lc.as<double>(), k);
+        hipLaunchKernelGGL(csc_scatter_kernel, dim3((k + 255) / 256), dim3(256), 0, L->stream, cp.as<int>(),
+                           rw.as<int>(), vl.as<double>(), k, P64.as<double>());
         PQ_HIP(hipGetLastError());
-        std::vector<double> sv, lam;
-        device_eigh(L, Rc.as<double>(), k, sv);                        // Rc's columns: U
-        const double smax = sv.empty() ? 0.0 : sv.back();
-        PQ_REQUIRE(smax > 0, "coarse space: the Galerkin resistance is not positive");
-        int first = 0;
-        while (first < k && !(sv[first] > drop * smax)) ++first;
-        const int kk = k - first;
-        W.reserve((size_t)k * kk * 8);
-        {
-            DevBuf sd;
-            sd.reserve((size_t)k * 8);
-            PQ_HIP(hipMemcpyAsync(sd.p, sv.data(), (size_t)k * 8, hipMemcpyHostToDevice, L->stream));
-            hipLaunchKernelGGL(whiten_kernel, dim3((unsigned)(((size_t)k * kk + 255) / 256)), dim3(256), 0, L->stream,
-                               Rc.as<double>(), sd.as<double>(), k, first, kk, W.as<double>());
-            PQ_HIP(hipGetLastError());
-            PQ_HIP(hipStreamSynchronize(L->stream));
-        }
-        T.reserve((size_t)k * kk * 8);
-        M.reserve((size_t)kk * kk * 8);
-        gemm_rm(L, false, false, k, kk, k, Lc.as<double>(), k, W.as<double>(), kk, T.as<double>(), kk);   // Lc W
-        gemm_rm(L, true, false, kk, kk, k, W.as<double>(), kk, T.as<double>(), kk, M.as<double>(), kk);    // W^T Lc W
-        hipLaunchKernelGGL(symmetrize64_kernel, dim3((unsigned)(((size_t)kk * kk + 255) / 256)), dim3(256), 0,
-                           L->stream, M.as<double>(), kk);
-        PQ_HIP(hipGetLastError());
-        device_eigh(L, M.as<double>(), kk, lam);                       // M's columns: Q
-        // T = W Q (M holds Q column-major: as a row-major array it is Q^T)
-        gemm_rm(L, false, true, k, kk, kk, W.as<double>(), kk, M.as<double>(), kk, T.as<double>(), kk);
-        // Psi = Phi T, Rl Psi = (Rl Phi) T, Ll Psi = (Ll Phi) T
-        Y.reserve((size_t)n * kk * 8);
-        const int kpp = (kk + 63) / 64 * 64;
-        L->kc = kk;
-        L->kp = kpp;
-        L->Phi.reserve((size_t)n * kpp * 4); L->Phit.reserve((size_t)n * kk * 4);
-        L->RPhi.reserve((size_t)n * kk * 4); L->LPhi.reserve((size_t)n * kk * 4);
-        const unsigned nb = (unsigned)(((size_t)n * kk + 255) / 256);
-        gemm_rm(L, false, false, n, kk, k, P64.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
-        hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kpp + 255) / 256)), dim3(256), 0, L->stream,
-                           Y.as<double>(), n, kk, kpp, L->Phi.as<float>());
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
-                           L->Phit.as<float>());
-        gemm_rm(L, false, false, n, kk, k, RX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
-                           L->RPhi.as<float>());
-        gemm_rm(L, false, false, n, kk, k, LX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
-                           L->LPhi.as<float>());
-        PQ_HIP(hipGetLastError());
-        PQ_HIP(hipStreamSynchronize(L->stream));
-        // the Galerkin pair in the new basis: I and Lam (their off-diagonal parts are rounding)
-        L->Rc.assign((size_t)kk * kk, 0.0);
-        L->Lc.assign((size_t)kk * kk, 0.0);
-        for (int a = 0; a < kk; ++a) {
-            L->Rc[(size_t)a * kk + a] = 1.0;
-            L->Lc[(size_t)a * kk + a] = lam[a];
-        }
-        L->cdiag = true;
-        make_half_bases(L);
-        *k_out = kk;
+        coarse_diagonalise(L, P64, k, drop, k_out, path_out);
     });
 }
 

@@ -326,7 +326,8 @@ class BoardData:
         self.rwg = None
         self.solver = PortSolver(board, device=device)
         self.solver.set_stream_coarse(grid)
-        self.phi_geo = self.solver.coarse_phi
+        phi = self.solver.coarse_phi
+        self.phi_geo = phi.toarray() if sparse.issparse(phi) else phi
         self.graph = BranchGraph(self.solver.model)
         dev = torch.device('cuda', device)
         self.X = torch.as_tensor(self.graph.static, device=dev)
@@ -384,17 +385,26 @@ def train(boards, steps=400, lr=2e-3, device=0, seed=0, log=print):
     return net, history
 
 
-def enrichment(net, graph):
+def enrichment(net, graph, dense=False):
     """The learned columns (n_free, n_cells * K) of a board, each scaled to unit norm (zero
-    columns dropped)."""
+    columns dropped): a scipy CSC matrix (row l holds the network's output at l's cotree branch in
+    the K columns of its cell, the columns of learned_columns), or a dense array with dense=True."""
     torch = net.torch
     X = torch.as_tensor(graph.static, device=net.device)
     A, _ = graph.batched(1, torch, net.device)
     E = net.infer(X, A)
-    Phi = learned_columns(torch, graph, E).double().cpu().numpy()
-    norm = np.linalg.norm(Phi, axis=0)
-    keep = norm > 0
-    return np.ascontiguousarray(Phi[:, keep] / norm[keep])
+    K = K_CHANNELS
+    vals = E[torch.as_tensor(graph.cot, device=E.device), :K].double().cpu().numpy()
+    n = len(graph.cot)
+    cols = (graph.cell[graph.cot][:, None] * K + np.arange(K)[None, :]).ravel()
+    m = sparse.csc_matrix((vals.ravel(), (np.repeat(np.arange(n), K), cols)), shape=(n, graph.n_cells * K))
+    m.eliminate_zeros()
+    norm = np.sqrt(np.asarray(m.multiply(m).sum(0)).ravel())
+    keep = np.flatnonzero(norm > 0)
+    m = m[:, keep]
+    m.data = m.data / np.repeat(norm[keep], np.diff(m.indptr))
+    m.sort_indices()
+    return np.ascontiguousarray(m.toarray()) if dense else m
 
 
 def set_learned_coarse(solver, net, grid=36, graph=None):

@@ -126,19 +126,41 @@ class PortSolver:
         self.setup_seconds = dict(topology=t1 - t0, device=time.perf_counter() - t1)
 
     def set_coarse(self, phi, diagonalize=True):
-        """Coarse space (n_free, k) of the two-level preconditioner (None or k = 0: none).
+        """Coarse space (n_free, k) of the two-level preconditioner (None or k = 0: none), a dense
+        array or a scipy sparse matrix.
 
         With diagonalize, the span is re-expressed in the generalised eigenbasis of the Galerkin
         pair (Phi^T Rl Phi, Phi^T Ll Phi): Psi = Phi V with V^T Rc V = I and V^T Lc V = diag, so
         every frequency's coarse solve is a diagonal scaling. Directions with a Galerkin
         resistance below 1e-10 of the largest are dropped (they carry no loop current). The
-        change of basis runs on the device (pq_loops_set_coarse_diagonal); diagonalize='host'
-        does it with numpy instead (two eigh calls), False keeps Phi and inverts the k x k coarse
-        matrix per frequency."""
-        phi = np.zeros((self.model.n_free, 0)) if phi is None else np.ascontiguousarray(phi, np.float64)
+        change of basis runs on the device (pq_loops_set_coarse_csc for sparse columns, which
+        cross PCIe as their nonzeros only, else pq_loops_set_coarse_diagonal; coarse_path tells
+        whether the Cholesky route or the eigenvector route was taken); diagonalize='host' does it
+        with numpy instead (two eigh calls), False keeps Phi and inverts the k x k coarse matrix
+        per frequency."""
+        from scipy import sparse
+        if phi is None:
+            phi = np.zeros((self.model.n_free, 0))
         if phi.ndim != 2 or phi.shape[0] != self.model.n_free:
             raise ValueError('coarse space must have one row per free loop')
         k = phi.shape[1]
+        self.coarse_path = None
+        if sparse.issparse(phi) and k and diagonalize is True:
+            m = sparse.csc_matrix(phi, dtype=np.float64)
+            m.sort_indices()
+            self.coarse_phi = m
+            colptr = np.ascontiguousarray(m.indptr, np.int32)
+            rows = np.ascontiguousarray(m.indices, np.int32)
+            vals = np.ascontiguousarray(m.data, np.float64)
+            kept, path = ctypes.c_int(), ctypes.c_int()
+            call('pq_loops_set_coarse_csc', self._handle, k, ptr(colptr), ptr(rows), ptr(vals), 1e-10,
+                 ctypes.byref(kept), ctypes.byref(path))
+            self.coarse_k = kept.value
+            self.coarse_path = {1: 'cholesky', 2: 'eigenvectors'}[path.value]
+            return
+        if sparse.issparse(phi):
+            phi = phi.toarray()
+        phi = np.ascontiguousarray(phi, np.float64)
         self.coarse_phi = phi
         if k and diagonalize is True:
             kept = ctypes.c_int()
@@ -161,18 +183,19 @@ class PortSolver:
     def set_stream_coarse(self, grid=8, vias=True, extra=None, extra_loops=None):
         """The geometric coarse space: stream-function hats of a grid x grid net per layer, the
         via-closed loops (vias=True), plus extra branch patterns (nb, k') and extra loop-space
-        columns (n_free, k'') when given."""
-        from .coarse import loop_restrict, stream_patterns, via_loop_columns
+        columns (n_free, k'', dense or sparse) when given. The columns stay sparse up to the device
+        and repeats are removed (coarse.unique_columns)."""
+        from scipy import sparse
+        from .coarse import loop_restrict, stream_patterns, unique_columns, via_loop_columns
         patterns = stream_patterns(self.model, grid)
         if extra is not None and extra.shape[1]:
-            from scipy import sparse
             patterns = sparse.hstack([patterns, sparse.csc_matrix(extra)]).tocsc()
-        cols = [loop_restrict(self.model, patterns)]
+        cols = [loop_restrict(self.model, patterns, dense=False)]
         if vias:
-            cols.append(via_loop_columns(self.model))
+            cols.append(via_loop_columns(self.model, dense=False))
         if extra_loops is not None and extra_loops.shape[1]:
-            cols.append(extra_loops)
-        self.set_coarse(np.hstack(cols))
+            cols.append(sparse.csc_matrix(extra_loops))
+        self.set_coarse(unique_columns(sparse.hstack(cols, format='csc')))
 
     @property
     def n_ports(self):

@@ -19,8 +19,10 @@ def _host(board, grid, precond, solver, model=None):
     model = LoopModel(board) if model is None else model
     phi = None
     if precond == 'two-level' and solver == 'iterative':
-        from .coarse import loop_restrict, stream_patterns, via_loop_columns
-        phi = np.hstack([loop_restrict(model, stream_patterns(model, grid)), via_loop_columns(model)])
+        from scipy import sparse
+        from .coarse import loop_restrict, stream_patterns, unique_columns, via_loop_columns
+        phi = unique_columns(sparse.hstack([loop_restrict(model, stream_patterns(model, grid), dense=False),
+                                            via_loop_columns(model, dense=False)], format='csc'))
     return model, phi
 
 

@@ -49,3 +49,33 @@ def test_sparse_hats_equal_the_dense_evaluation(name, grid):
     assert np.array_equal(sp.toarray(), dn)
     # the columns' norms are summed in a different order: equal to rounding
     np.testing.assert_allclose(loop_restrict(m, sp), loop_restrict(m, dn), rtol=1e-13, atol=1e-15)
+
+
+def test_sparse_loop_columns_equal_the_dense_ones():
+    """loop_restrict and via_loop_columns give the same columns as scipy CSC (dense=False)."""
+    from pcbqs_amd.coarse import via_loop_columns
+    b = boards.board_c2(2, target=3000)
+    m = LoopModel(b)
+    sp = stream_patterns(m, 8)
+    assert np.array_equal(loop_restrict(m, sp, dense=False).toarray(), loop_restrict(m, sp))
+    assert np.array_equal(via_loop_columns(m, dense=False).toarray(), via_loop_columns(m))
+
+
+def test_unique_columns_drops_repeats_up_to_sign_and_rounding():
+    """A column equal to an earlier one up to sign and 1e-12 is dropped; a column with the same
+    rows but other values, or other rows, is kept; the order of the kept columns is unchanged."""
+    from scipy import sparse
+    from pcbqs_amd.coarse import unique_columns
+    rng = np.random.default_rng(3)
+    a = np.zeros((20, 6))
+    a[[1, 4, 7], 0] = rng.normal(size=3)
+    a[:, 1] = -a[:, 0] * (1 + 1e-14)              # a repeat up to sign and rounding
+    a[[1, 4, 7], 2] = rng.normal(size=3)          # same rows, other values
+    a[[2, 4, 7], 3] = a[[1, 4, 7], 0]             # same values, other rows
+    a[:, 4] = a[:, 2]                             # an exact repeat
+    a[[0, 19], 5] = 1.0
+    a /= np.linalg.norm(a, axis=0)
+    u = unique_columns(sparse.csc_matrix(a))
+    assert np.array_equal(u.toarray(), a[:, [0, 2, 3, 5]])
+    # a matrix without repeats comes back unchanged
+    assert np.array_equal(unique_columns(sparse.csc_matrix(a[:, [0, 2, 3, 5]])).toarray(), a[:, [0, 2, 3, 5]])

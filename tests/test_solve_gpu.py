@@ -196,3 +196,38 @@ def test_device_diagonalised_coarse_space_equals_the_host_one():
     assert np.abs(Z1 - Z0).max() <= 1e-9 * np.abs(Z0).max()
     assert all(abs(a - c) <= max(3, 0.1 * a) for a, c in zip(it0, it1))
     s.close()
+
+
+def test_sparse_coarse_columns_take_the_cholesky_route_unless_repeated():
+    """Coarse columns sent as sparse columns (pq_loops_set_coarse_csc). Without repeated columns the
+    Galerkin resistance's Cholesky factor passes its guard and one eigensolve diagonalises the pair;
+    with a repeated column the guard fails and the eigenvector route drops the repeat. Either way
+    the coarse solution equals the host-diagonalised one to the FP32 bases' rounding and the
+    two-level solves give the same port impedances."""
+    from scipy import sparse
+    from pcbqs_amd.coarse import loop_restrict, stream_patterns, unique_columns, via_loop_columns
+    from pcbqs_amd.solve import PortSolver
+    b = boards.board_c2(1, target=3000)
+    s = PortSolver(b, device=0)
+    m = s.model
+    phi = unique_columns(sparse.hstack([loop_restrict(m, stream_patterns(m, 10), dense=False),
+                                        via_loop_columns(m, dense=False)], format='csc'))
+    freqs = [1e3, 1e6, 1e8]
+    s.set_coarse(phi.toarray(), diagonalize='host')
+    k0 = s.coarse_k
+    x0 = {f: s.coarse_solution(f) for f in freqs}
+    Z0, info0 = s.ports(freqs, precond='two-level', tol=1e-10)
+    assert k0 == phi.shape[1]
+    repeated = sparse.hstack([phi, -phi[:, [3]]], format='csc')
+    for cols, path in ((phi, 'cholesky'), (repeated, 'eigenvectors')):
+        s.set_coarse(cols)
+        print(f'{cols.shape[1]} columns: {s.coarse_path}, kept {s.coarse_k}', flush=True)
+        assert s.coarse_path == path and s.coarse_k == k0
+        for f in freqs:
+            x1 = s.coarse_solution(f)
+            assert np.abs(x1 - x0[f]).max() <= 1e-5 * np.abs(x0[f]).max(), (path, f)
+        Z1, info1 = s.ports(freqs, precond='two-level', tol=1e-10)
+        assert np.abs(Z1 - Z0).max() <= 1e-9 * np.abs(Z0).max(), path
+        assert all(abs(a['iterations'] - c['iterations']) <= max(3, 0.1 * a['iterations'])
+                   for a, c in zip(info0, info1)), path
+    s.close()

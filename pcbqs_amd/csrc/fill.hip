@@ -14,9 +14,9 @@
 //                entry from its canonical pair (min, max), so reciprocal entries are bitwise equal.
 //   Pairs the near rule keeps analytic are not integrated by the far kernels: they go to
 //                block-local LDS lists, flushed with one global atomic per block and level.
-//   near kernel  FP64 closed-form source integrals, one thread per (pair, observation point);
-//                each pair's observation sum is then taken sequentially in point order, the
-//                reference's order.
+//   near kernel  FP64 closed-form source integrals, one thread per near pair looping over the
+//                pair's observation points, summed sequentially in point order (the reference's
+//                order); both entries stored through the L2 (scattered partial lines merge there).
 // Classification is the reference rule evaluated in FP64 without contraction; an FP32
 // prefilter with a 1e-5 relative margin settles the clearly-far pairs without it.
 #include "pq_common.h"
@@ -528,41 +528,53 @@ __global__ __launch_bounds__(256) void far64_kernel(GeomDev g, FarArgs A, double
 
 // ---- near kernel -----------------------------------------------------------------------------
 
+// Entry (r, c) as store_entry, with ordinary (write-back) stores: the near entries are scattered
+// (a pair's reverse entry lies down a column), and cached stores let the L2 merge the partial lines
+// that nearby pairs write instead of sending every 4- or 8-byte entry to HBM on its own.
+template <class T, bool SCALAR>
+__device__ __forceinline__ void store_entry_cached(T *out, size_t plane, size_t i, int r, int c, const double *m,
+                                                   unsigned *flag) {
+    bool ok = isfinite(m[0]);
+    out[i] = (T)m[0];
+    if (!SCALAR) {
+        double mx, my;
+        if (r == c) { mx = (m[1] + m[3]) / 2; my = (m[2] + m[4]) / 2; }
+        else if (r < c) { mx = m[1]; my = m[2]; }
+        else { mx = m[3]; my = m[4]; }
+        out[plane + i] = (T)mx;
+        out[2 * plane + i] = (T)my;
+        out[3 * plane + i] = (T)m[5];
+        ok = ok && isfinite(mx) && isfinite(my) && isfinite(m[5]);
+    }
+    if (!ok) atomicOr(flag, 1u);
+}
+
+// One thread per near pair of a list: the pair's observation points in order, each point's
+// rounded moments added to the pair's sums in that order (the reference's sequential sum), then
+// both entries stored. No barrier and no exchange between threads: the first layout (a thread
+// per (pair, point), the sums through LDS between two block barriers) spent most of its time
+// waiting on the list and geometry loads of every iteration.
 template <class T, bool SCALAR, bool LOCAL>
 __global__ __launch_bounds__(256) void near_kernel(GeomDev g, const unsigned long long *list,
                                                   const unsigned long long *count, unsigned long long cap,
                                                   int level, int r0, long long ld, int mirror, T *out,
                                                   size_t plane, unsigned *flag) {
-    __shared__ double buf[256 * 6];
-    const int nq = g.nq[level];
-    const int ppb = 256 / nq;
     const unsigned long long total = min(*count, cap);
-    const int lp = threadIdx.x / nq, j = threadIdx.x - lp * nq;
-    for (unsigned long long base = (unsigned long long)blockIdx.x * ppb; base < total;
-         base += (unsigned long long)gridDim.x * ppb) {
-        const unsigned long long idx = base + lp;
-        double m[6] = {0, 0, 0, 0, 0, 0};
-        if (lp < ppb && idx < total) {
-            const unsigned long long e = list[idx];
-            const int r = (int)(e >> 32), c = (int)(e & 0xffffffffu);
-            near_point<SCALAR, LOCAL>(g, min(r, c), max(r, c), level, j, m);
-        }
+    const int nq = g.nq[level];
+    for (unsigned long long idx = (unsigned long long)blockIdx.x * 256 + threadIdx.x; idx < total;
+         idx += (unsigned long long)gridDim.x * 256) {
+        const unsigned long long e = list[idx];
+        const int r = (int)(e >> 32), c = (int)(e & 0xffffffffu);
+        const int a = min(r, c), b = max(r, c);
+        double s[6] = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < nq; ++j) {
+            double m[6] = {0, 0, 0, 0, 0, 0};
+            near_point<SCALAR, LOCAL>(g, a, b, level, j, m);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) buf[threadIdx.x * 6 + k] = m[k];
-        __syncthreads();
-        if (threadIdx.x < ppb && base + threadIdx.x < total) {
-            const unsigned long long e = list[base + threadIdx.x];
-            const int r = (int)(e >> 32), c = (int)(e & 0xffffffffu);
-            double s[6] = {0, 0, 0, 0, 0, 0};
-            for (int jj = 0; jj < nq; ++jj) {
-                const double *v = buf + (threadIdx.x * nq + jj) * 6;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) s[k] += v[k];
-            }
-            store_entry<T, SCALAR>(out, plane, (size_t)(r - r0) * ld + c, r, c, s, flag);
-            if (mirror && r != c) store_entry<T, SCALAR>(out, plane, (size_t)(c - r0) * ld + r, c, r, s, flag);
+            for (int k = 0; k < 6; ++k) s[k] = __dadd_rn(s[k], m[k]);
         }
-        __syncthreads();
+        store_entry_cached<T, SCALAR>(out, plane, (size_t)(r - r0) * ld + c, r, c, s, flag);
+        if (mirror && r != c) store_entry_cached<T, SCALAR>(out, plane, (size_t)(c - r0) * ld + r, c, r, s, flag);
     }
 }
 
@@ -669,9 +681,6 @@ void launch_far(pq_fill *f, int precision, bool scalar, const FarArgs &A, void *
 
 void launch_near(pq_fill *f, int precision, bool scalar, int which, int level, const FarArgs &A, void *out,
                  hipStream_t s) {
-    const int nq = f->g.nq[level];
-    const int ppb = 256 / nq;
-    (void)ppb;
     dim3 grid(f->cus * 8), block(256);
     const unsigned long long *list = f->lists[which].as<unsigned long long>();
     const unsigned long long *count = f->counts.as<unsigned long long>() + which;

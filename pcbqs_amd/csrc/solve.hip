@@ -930,25 +930,31 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
     if (lane == 0) { cvec[(size_t)a * NR + p] = yr; cvec[(size_t)a * NR + P + p] = yi; }
 }
 
-// The prolongation: the three transposed bases (k, n) FP32 Phi^T, (Rl Phi)^T, (Ll Phi)^T read in
-// one pass against the coarse vector c (k, NR), thread per row, a chunk of coarse columns per
-// blockIdx.y, U columns of each basis (3 U loads) in flight; c wave-uniform (scalar loads).
-// Chunk partials: PY = Phi c and PA = (Rl + j w Ll) Phi c (real parts in columns 0..P-1).
+// The prolongation: the three transposed bases (k, n) Phi^T, (Rl Phi)^T, (Ll Phi)^T (FP32, or FP16
+// scaled by s0, s1, s2) read in one pass against the coarse vector c (k, NR). A 256-thread block
+// takes 64 rows (a row per lane) and a chunk of coarse columns (blockIdx.y); its four waves take the
+// chunk's groups of U columns in turn (3 U loads in flight per lane), c wave-uniform (scalar
+// loads), FP64 sums. The waves' sums are combined in LDS in a fixed order, (w0 + w2) + (w1 + w3),
+// so one partial per block: chunk partials PY = Phi c and PA = (Rl + j w Ll) Phi c (real parts in
+// columns 0..P-1). Four waves per row block fill the chip with a quarter of the chunk partials that
+// one wave per row block needs.
 template <int NR, int U, class T>
 __global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, const T *__restrict__ B1,
                                                      const T *__restrict__ B2, int n, int k,
                                                      const double *__restrict__ C, int chunk,
                                                      const double *__restrict__ omega_dev, double s0, double s1,
                                                      double s2, double *__restrict__ PY, double *__restrict__ PA) {
-    constexpr int P = NR / 2;
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int P = NR / 2, V = 3 * NR;
+    __shared__ double red[2 * 64 * V];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = blockIdx.x * 64 + lane;
     const int c0 = blockIdx.y * chunk, c1 = min(k, c0 + chunk);
-    double ay[NR], ar[NR], al[NR];
+    double acc[V];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) ay[r] = ar[r] = al[r] = 0;
+    for (int v = 0; v < V; ++v) acc[v] = 0;
     const size_t rr = (size_t)min(row, n - 1);
-    int c = c0;
-    for (; c + U <= c1; c += U) {
+    int c = c0 + w * U;
+    for (; c + U <= c1; c += 4 * U) {
         T f0[U], f1[U], f2[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -962,35 +968,52 @@ __global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, 
             const double *x = C + (size_t)(c + u) * NR;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                ay[r] += a * x[r];
-                ar[r] += b * x[r];
-                al[r] += d * x[r];
+                acc[r] += a * x[r];
+                acc[NR + r] += b * x[r];
+                acc[2 * NR + r] += d * x[r];
             }
         }
     }
-    for (; c < c1; ++c) {
-        const double a = (float)B0[(size_t)c * n + rr], b = (float)B1[(size_t)c * n + rr];
-        const double d = (float)B2[(size_t)c * n + rr];
-        const double *x = C + (size_t)c * NR;
+    for (int cc = c; cc < c1; ++cc) {          // the chunk's one partial group (this wave's, if any)
+        const double a = (float)B0[(size_t)cc * n + rr], b = (float)B1[(size_t)cc * n + rr];
+        const double d = (float)B2[(size_t)cc * n + rr];
+        const double *x = C + (size_t)cc * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            ay[r] += a * x[r];
-            ar[r] += b * x[r];
-            al[r] += d * x[r];
+            acc[r] += a * x[r];
+            acc[NR + r] += b * x[r];
+            acc[2 * NR + r] += d * x[r];
         }
     }
-    if (row < n) {
-        const double omega = *omega_dev;
+    if (w >= 2) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) { ay[r] *= s0; ar[r] *= s1; al[r] *= s2; }
+        for (int v = 0; v < V; ++v) red[((w - 2) * 64 + lane) * V + v] = acc[v];
+    }
+    __syncthreads();
+    if (w < 2) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += red[(w * 64 + lane) * V + v];
+    }
+    __syncthreads();
+    if (w == 1) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) red[lane * V + v] = acc[v];
+    }
+    __syncthreads();
+    if (w == 0 && row < n) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += red[lane * V + v];
+        const double omega = *omega_dev;
         double *oy = PY + ((size_t)blockIdx.y * n + row) * NR, *oa = PA + ((size_t)blockIdx.y * n + row) * NR;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            oa[p] = ar[p] - omega * al[P + p];
-            oa[P + p] = ar[P + p] + omega * al[p];
+            const double rre = acc[NR + p] * s1, rim = acc[NR + P + p] * s1;
+            const double lre = acc[2 * NR + p] * s2, lim = acc[2 * NR + P + p] * s2;
+            oa[p] = rre - omega * lim;
+            oa[P + p] = rim + omega * lre;
         }
 #pragma unroll
-        for (int r = 0; r < NR; ++r) oy[r] = ay[r];
+        for (int r = 0; r < NR; ++r) oy[r] = acc[r] * s0;
     }
 }
 
@@ -1510,7 +1533,7 @@ struct pq_loops {
     std::map<int, float> gemv_us;
     // autotuned coarse kernels per column count (rows per restriction block, prolongation chunks
     // and loads in flight)
-    std::map<int, int> crows, psplit, pdepth, prows;
+    std::map<int, int> crows, psplit, pdepth;
     // the direct solve's L D L^T factor (planes of the factor, block inverses, panel products) and
     // the frequency it was formed at
     DevBuf Fldl, Xldl, Wldl;
@@ -1719,14 +1742,11 @@ void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P);
 
 void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add = nullptr);
 
-// Column chunks of the prolongation's panel products: enough blocks to fill the chip (about
-// 2048 with 256-row blocks) and at most 16; on large bases also few enough that the chunk
-// partials (written and read once) stay within a quarter of the bases' own bytes.
-int prolong_splits(int n, int k, int NR) {
-    const int rb = (n + GEMV_ROWS - 1) / GEMV_ROWS;
-    int s = std::min({16, (2048 + rb - 1) / rb, (k + 15) / 16});
-    if ((size_t)12 * k * n >= ((size_t)64 << 20)) s = std::min(s, k / (16 * NR));
-    return std::max(1, s);
+// Column chunks of the prolongation (64-row blocks of four waves) before the autotuner has run:
+// enough blocks to fill the chip (about 1024), at most 16, at least 64 columns per chunk.
+int prolong_splits(int n, int k) {
+    const int rb = (n + 63) / 64;
+    return std::max(1, std::min({16, (1024 + rb - 1) / rb, (k + 63) / 64}));
 }
 
 int restrict_rows(const pq_loops *L, int NR) {
@@ -1736,17 +1756,12 @@ int restrict_rows(const pq_loops *L, int NR) {
 
 int prolong_split(const pq_loops *L, int NR) {
     auto it = L->psplit.find(NR);
-    return it != L->psplit.end() ? it->second : prolong_splits(L->nf, L->kc, NR);
+    return it != L->psplit.end() ? it->second : prolong_splits(L->nf, L->kc);
 }
 
 int prolong_depth(const pq_loops *L, int NR) {
     auto it = L->pdepth.find(NR);
     return it != L->pdepth.end() ? it->second : 8;
-}
-
-int prolong_rows(const pq_loops *L, int NR) {
-    auto it = L->prows.find(NR);
-    return it != L->prows.end() ? it->second : GEMV_ROWS;
 }
 
 // Launch the restriction Phi^T X into `rows`-row chunk partials; returns the chunk count.
@@ -1763,29 +1778,29 @@ template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k,
     return nchunk;
 }
 
-// Launch the prolongation of L->cvec in about `ps` column chunks, `rows` (128 or 256) rows per
-// block, `depth` (8 or 16) columns of each basis in flight; returns the chunk count.
-template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, int rows, const double *omega_dev,
-                                    double *PY, double *PA) {
+// Launch the prolongation of L->cvec in about `ps` column chunks, `depth` (8 or 16) columns of each
+// basis in flight per lane; returns the chunk count.
+template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, const double *omega_dev, double *PY,
+                                    double *PA) {
     const int n = L->nf, k = L->kc, pc = (k + ps - 1) / ps, nsplit = (k + pc - 1) / pc;
-    const dim3 grid((n + rows - 1) / rows, nsplit);
+    const dim3 grid((n + 63) / 64, nsplit);
     const double *sc = L->hscale;
     if (L->coarse_half) {
         const _Float16 *b0 = L->Phith.as<_Float16>(), *b1 = L->RPhih.as<_Float16>(), *b2 = L->LPhih.as<_Float16>();
         if (depth == 16)
-            hipLaunchKernelGGL((prolong_kernel<N, 16, _Float16>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+            hipLaunchKernelGGL((prolong_kernel<N, 16, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
                                L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
         else
-            hipLaunchKernelGGL((prolong_kernel<N, 8, _Float16>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+            hipLaunchKernelGGL((prolong_kernel<N, 8, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
                                L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
         return nsplit;
     }
     const float *b0 = L->Phit.as<float>(), *b1 = L->RPhi.as<float>(), *b2 = L->LPhi.as<float>();
     if (depth == 16)
-        hipLaunchKernelGGL((prolong_kernel<N, 16, float>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+        hipLaunchKernelGGL((prolong_kernel<N, 16, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
                            L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     else
-        hipLaunchKernelGGL((prolong_kernel<N, 8, float>), grid, dim3(rows), 0, L->stream, b0, b1, b2, n, k,
+        hipLaunchKernelGGL((prolong_kernel<N, 8, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
                            L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     return nsplit;
 }
@@ -1801,8 +1816,8 @@ void tune_coarse(pq_loops *L, int NR) {
     if (k == 0 || n < 1024 || L->crows.count(NR)) return;
     const std::vector<int> rows_c = {128, 256, 512, 1024};
     std::vector<int> split_c;
-    for (int ps : {4, 8, 12, 16, 24, 32, 48})
-        if (ps <= (k + 15) / 16) split_c.push_back(ps);
+    for (int ps : {1, 2, 3, 4, 6, 8, 12, 16})
+        if (ps == 1 || ps <= (k + 63) / 64) split_c.push_back(ps);
     if (split_c.empty()) split_c.push_back(1);
     DevBuf X, part, PY, PA, om, flush, cv, dg;
     constexpr size_t FLUSH = (size_t)512 << 20;
@@ -1837,7 +1852,7 @@ void tune_coarse(pq_loops *L, int NR) {
         }
         return total / 3.f;
     };
-    int best_rows = rows_c[0], best_split = split_c[0], best_depth = 8, best_prows = GEMV_ROWS;
+    int best_rows = rows_c[0], best_split = split_c[0], best_depth = 8;
     float best = 1e30f;
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
@@ -1851,14 +1866,13 @@ void tune_coarse(pq_loops *L, int NR) {
             if (t < best) { best = t; best_rows = rows; }
         }
         best = 1e30f;
-        for (int prows : {256, 128})
-            for (int depth : {8, 16})
-                for (int ps : split_c) {
-                    const float t = timed([&] {
-                        prolong_launch<N>(L, ps, depth, prows, om.as<double>(), PY.as<double>(), PA.as<double>());
-                    });
-                    if (t < best) { best = t; best_split = ps; best_depth = depth; best_prows = prows; }
-                }
+        for (int depth : {8, 16})
+            for (int ps : split_c) {
+                const float t = timed([&] {
+                    prolong_launch<N>(L, ps, depth, om.as<double>(), PY.as<double>(), PA.as<double>());
+                });
+                if (t < best) { best = t; best_split = ps; best_depth = depth; }
+            }
     });
     PQ_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
@@ -1866,7 +1880,6 @@ void tune_coarse(pq_loops *L, int NR) {
     L->crows[NR] = best_rows;
     L->psplit[NR] = best_split;
     L->pdepth[NR] = best_depth;
-    L->prows[NR] = best_prows;
 }
 
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.
@@ -1890,8 +1903,7 @@ void coarse_correction(pq_loops *L, const double *X, int P) {
         double *PY = L->ppart.as<double>();
         const int ps = prolong_split(L, N);
         double *PA = PY + (size_t)ps * count;
-        const int nsplit = prolong_launch<N>(L, ps, prolong_depth(L, N), prolong_rows(L, N), L->omega_dev.as<double>(),
-                                             PY, PA);
+        const int nsplit = prolong_launch<N>(L, ps, prolong_depth(L, N), L->omega_dev.as<double>(), PY, PA);
         hipLaunchKernelGGL(prolong_epilogue_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream, PY,
                            PA, nsplit, count, X, L->T1.as<double>(), L->T2.as<double>());
     });
@@ -2447,7 +2459,7 @@ void coarse_diagonalise(pq_loops *L, DevBuf &P64, int k, double drop, int *k_out
     const int n = L->nf;
     L->kc = 0;
     L->omega_c = -1;
-    L->crows.clear(); L->psplit.clear(); L->pdepth.clear(); L->prows.clear();
+    L->crows.clear(); L->psplit.clear(); L->pdepth.clear();
     DevBuf RX, LX, Rc, Lc, W, M, T, Y;
     {
         const size_t cnt = (size_t)n * k;
@@ -3649,7 +3661,6 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         L->crows.clear();
         L->psplit.clear();
         L->pdepth.clear();
-        L->prows.clear();
         if (k == 0) return;
         const int n = L->nf;
         L->kp = (k + 63) / 64 * 64;

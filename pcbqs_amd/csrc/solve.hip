@@ -245,6 +245,63 @@ __global__ __launch_bounds__(256) void panel_kernel(const T *__restrict__ M, lon
     }
 }
 
+// panel_kernel with a 256-thread block on 64 rows: its four waves take the chunk's groups of 16
+// columns in turn, and their sums are combined in LDS in a fixed order, (w0 + w2) + (w1 + w3), so
+// the chip fills with a quarter of the chunk partials that one wave per row block needs (fewer
+// partials for the epilogue to read). Same partial layout as panel_kernel.
+template <int NR, class T = float>
+__global__ __launch_bounds__(256) void panel4_kernel(const T *__restrict__ M, long long ld, int nrows, int ncols,
+                                                    const double *__restrict__ X, int chunk,
+                                                    double *__restrict__ part) {
+    __shared__ double red[2 * 64 * NR];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = blockIdx.x * 64 + lane;
+    const int c0 = blockIdx.y * chunk, c1 = min(ncols, c0 + chunk);
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    const T *col = M + min(row, nrows - 1);
+    int c = c0 + 16 * w;
+    for (; c + 16 <= c1; c += 64) {
+        T m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = __builtin_nontemporal_load(col + (size_t)(c + u) * ld);
+        const double *x = X + (size_t)c * NR;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double mu = m[u];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += mu * x[u * NR + r];
+        }
+    }
+    for (int cc = c; cc < c1 && cc < c + 16; ++cc) {     // the chunk's one partial group (this wave's, if any)
+        const double mv = __builtin_nontemporal_load(col + (size_t)cc * ld);
+        const double *x = X + (size_t)cc * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
+    }
+    if (w >= 2) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[((w - 2) * 64 + lane) * NR + r] = acc[r];
+    }
+    __syncthreads();
+    if (w < 2) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += red[(w * 64 + lane) * NR + r];
+    }
+    __syncthreads();
+    if (w == 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[lane * NR + r] = acc[r];
+    }
+    __syncthreads();
+    if (w == 0 && row < nrows) {
+        double *o = part + ((size_t)row * gridDim.y + blockIdx.y) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) o[r] = acc[r] + red[lane * NR + r];
+    }
+}
+
 // W_re -= w sum_s part_im, W_im += w sum_s part_re (chunks summed in order); w in device memory, so
 // a captured graph stays valid from one frequency to the next.
 template <int NR>
@@ -1599,7 +1656,10 @@ int operator_splits(int n, int chunk) {
 // part = the Ll X partials (operator_splits of them) on the stream.
 template <int N> void operator_partials(const pq_loops *L, OpShape sh, const double *X, double *part) {
     const int n = L->nf;
-    if (sh.chunk > 0) {
+    if (sh.chunk > 0 && sh.rows == 64) {
+        hipLaunchKernelGGL(panel4_kernel<N>, dim3((n + 63) / 64, (n + sh.chunk - 1) / sh.chunk), dim3(256), 0,
+                           L->stream, L->Ll.as<float>(), L->ldl, n, n, X, sh.chunk, part);
+    } else if (sh.chunk > 0) {
         hipLaunchKernelGGL(panel_kernel<N>, dim3((n + sh.rows - 1) / sh.rows, (n + sh.chunk - 1) / sh.chunk),
                            dim3(sh.rows), 0, L->stream, L->Ll.as<float>(), L->ldl, n, n, X, sh.chunk, part);
     } else {
@@ -1640,11 +1700,12 @@ template <int N> void operator_launch(const pq_loops *L, OpShape sh, const doubl
 }
 
 // The autotuner of the operator product. Candidates: the symmetric upper-triangle kernel (chunk 0,
-// up to 16 real columns) and panel_kernel over the whole matrix with 128 or 256 rows per block and
-// a column-chunk width; together they set the grid, the loads in flight and the partial sums the
-// epilogue reads. Each candidate runs once to warm up and three times under HIP events on an
-// input of zeros (the whole product: partials, resistive part and epilogue); the fastest is kept
-// for this handle and column count.
+// up to 16 real columns), panel_kernel over the whole matrix with 128 or 256 rows per block and a
+// column-chunk width, and panel4_kernel (rows 64: four waves per 64-row block) with a chunk width;
+// together they set the grid, the loads in flight and the partial sums the epilogue reads. Each
+// candidate runs once to warm up and three times under HIP events on an input of zeros (the whole
+// product: partials, resistive part and epilogue), each time after a cache-evicting fill; the
+// fastest is kept for this handle and column count.
 void tune_gemv(pq_loops *L, int NR) {
     const int n = L->nf;
     const int mode = operator_mode();
@@ -1653,14 +1714,19 @@ void tune_gemv(pq_loops *L, int NR) {
     if (n < 1024 && !(mode == 2 && NR <= 16)) return;
     std::vector<OpShape> cand;
     if (NR <= 16 && mode != 1) cand.push_back({0, SYM_T});
-    if (mode != 2 || cand.empty())
+    if (mode != 2 || cand.empty()) {
         for (int rows : {256, 128})
             for (int c : {128, 192, 256, 320, 448, 640, 960})
                 if ((n + c - 1) / c <= 128 && c < n) cand.push_back({c, rows});
+        for (int c : {256, 320, 448, 640, 960, 1280, 1920})      // four waves per 64-row block
+            if ((n + c - 1) / c <= 128 && c < n) cand.push_back({c, 64});
+    }
     if (cand.empty()) return;
     int maxsplit = 0;
     for (auto c : cand) maxsplit = std::max(maxsplit, operator_splits(n, c.chunk));
-    DevBuf X, W, part, om;
+    DevBuf X, W, part, om, flush;
+    constexpr size_t FLUSH = (size_t)512 << 20;
+    flush.reserve(FLUSH);
     X.reserve((size_t)n * NR * 8);
     W.reserve((size_t)n * NR * 8);
     om.reserve(8);
@@ -1679,14 +1745,22 @@ void tune_gemv(pq_loops *L, int NR) {
                 operator_launch<N>(L, c, X.as<double>(), W.as<double>(), part.as<double>(), om.as<double>());
             });
         };
+        // each timed product starts with Ll out of the caches (a 512 MiB fill in between), as in a
+        // solve, where the Krylov vectors and preconditioner data pass between two products: timed
+        // back to back, part of Ll stays in the 256 MB last-level cache and favours other shapes
         launch();
-        PQ_HIP(hipEventRecord(e0, L->stream));
-        for (int r = 0; r < 3; ++r) launch();
-        PQ_HIP(hipEventRecord(e1, L->stream));
-        PQ_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        PQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const float us = ms * 1000.f / 3.f;
+        float total = 0;
+        for (int r = 0; r < 3; ++r) {
+            PQ_HIP(hipMemsetAsync(flush.p, r, FLUSH, L->stream));
+            PQ_HIP(hipEventRecord(e0, L->stream));
+            launch();
+            PQ_HIP(hipEventRecord(e1, L->stream));
+            PQ_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            PQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            total += ms;
+        }
+        const float us = total * 1000.f / 3.f;
         if (us < best_us) { best_us = us; best = c; }
     }
     PQ_HIP(hipGetLastError());
@@ -1722,8 +1796,12 @@ void apply_operator64(pq_loops *L, const double *X, double *W, int P) {
     const double *omega_dev = L->omega_dev.as<double>();
     dispatch_nr(NR, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL((panel_kernel<N, double>), dim3((n + sh.rows - 1) / sh.rows, (n + sh.chunk - 1) / sh.chunk),
-                           dim3(sh.rows), 0, L->stream, L->Ll64.as<double>(), L->ldl, n, n, X, sh.chunk, part);
+        if (sh.rows == 64)
+            hipLaunchKernelGGL((panel4_kernel<N, double>), dim3((n + 63) / 64, (n + sh.chunk - 1) / sh.chunk),
+                               dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl, n, n, X, sh.chunk, part);
+        else
+            hipLaunchKernelGGL((panel_kernel<N, double>), dim3((n + sh.rows - 1) / sh.rows, (n + sh.chunk - 1) / sh.chunk),
+                               dim3(sh.rows), 0, L->stream, L->Ll64.as<double>(), L->ldl, n, n, X, sh.chunk, part);
         if constexpr ((N & (N - 1)) == 0) {
             hipLaunchKernelGGL(operator_epilogue_kernel<N>, dim3((n + 3) / 4), dim3(256), 0, L->stream,
                                L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, X, part, nsplit,

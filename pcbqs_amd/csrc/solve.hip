@@ -245,63 +245,6 @@ __global__ __launch_bounds__(256) void panel_kernel(const T *__restrict__ M, lon
     }
 }
 
-// panel_kernel with a 256-thread block on 64 rows: its four waves take the chunk's groups of 16
-// columns in turn, and their sums are combined in LDS in a fixed order, (w0 + w2) + (w1 + w3), so
-// the chip fills with a quarter of the chunk partials that one wave per row block needs (fewer
-// partials for the epilogue to read). Same partial layout as panel_kernel.
-template <int NR, class T = float>
-__global__ __launch_bounds__(256) void panel4_kernel(const T *__restrict__ M, long long ld, int nrows, int ncols,
-                                                    const double *__restrict__ X, int chunk,
-                                                    double *__restrict__ part) {
-    __shared__ double red[2 * 64 * NR];
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int row = blockIdx.x * 64 + lane;
-    const int c0 = blockIdx.y * chunk, c1 = min(ncols, c0 + chunk);
-    double acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = 0;
-    const T *col = M + min(row, nrows - 1);
-    int c = c0 + 16 * w;
-    for (; c + 16 <= c1; c += 64) {
-        T m[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = __builtin_nontemporal_load(col + (size_t)(c + u) * ld);
-        const double *x = X + (size_t)c * NR;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const double mu = m[u];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += mu * x[u * NR + r];
-        }
-    }
-    for (int cc = c; cc < c1 && cc < c + 16; ++cc) {     // the chunk's one partial group (this wave's, if any)
-        const double mv = __builtin_nontemporal_load(col + (size_t)cc * ld);
-        const double *x = X + (size_t)cc * NR;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
-    }
-    if (w >= 2) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) red[((w - 2) * 64 + lane) * NR + r] = acc[r];
-    }
-    __syncthreads();
-    if (w < 2) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += red[(w * 64 + lane) * NR + r];
-    }
-    __syncthreads();
-    if (w == 1) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) red[lane * NR + r] = acc[r];
-    }
-    __syncthreads();
-    if (w == 0 && row < nrows) {
-        double *o = part + ((size_t)row * gridDim.y + blockIdx.y) * NR;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) o[r] = acc[r] + red[lane * NR + r];
-    }
-}
-
 // W_re -= w sum_s part_im, W_im += w sum_s part_re (chunks summed in order); w in device memory, so
 // a captured graph stays valid from one frequency to the next.
 template <int NR>
@@ -320,12 +263,75 @@ __global__ void gemv_epilogue_kernel(const double *part, int nsplit, int n, cons
     W[(size_t)row * NR + P + p] += omega * yr;
 }
 
+// One wave's share of row `row` of Rl X (NR real columns): lanes over the row's CSR entries (port
+// columns >= n skipped), each lane's NR-vector of its entries' products in a[]. The first 128
+// entries have every load issued up front (an absent or port entry gets weight 0 and row 0's x);
+// longer rows continue in a loop.
+template <int NR>
+__device__ __forceinline__ void rl_row_lanes(const int *__restrict__ ptr, const int *__restrict__ col,
+                                             const double *__restrict__ val, int n, const double *__restrict__ X,
+                                             int row, int lane, double *a) {
+    const int t0 = ptr[row], t1 = ptr[row + 1];
+    {
+        const int ta = t0 + lane, tb = ta + 64;
+        const int ca = ta < t1 ? col[ta] : n, cb = tb < t1 ? col[tb] : n;
+        const double va = ca < n ? val[ta] : 0.0, vb = cb < n ? val[tb] : 0.0;
+        const double *xa = X + (size_t)(ca < n ? ca : 0) * NR, *xb = X + (size_t)(cb < n ? cb : 0) * NR;
+        double ya[NR], yb[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            ya[r] = xa[r];
+            yb[r] = xb[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            a[r] += va * ya[r];
+            a[r] += vb * yb[r];
+        }
+    }
+    // longer rows (up to ~1,100 entries at 30k RWG: the wave taking one sets the kernel's tail) in
+    // groups of 128, each lane's two entries' loads issued before their products
+    for (int t = t0 + 128 + lane; t < t1; t += 128) {
+        const int tb = t + 64;
+        const int ca = col[t], cb = tb < t1 ? col[tb] : n;
+        const double va = ca < n ? val[t] : 0.0, vb = cb < n ? val[tb] : 0.0;
+        const double *xa = X + (size_t)(ca < n ? ca : 0) * NR, *xb = X + (size_t)(cb < n ? cb : 0) * NR;
+        double ya[NR], yb[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            ya[r] = xa[r];
+            yb[r] = xb[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            a[r] += va * ya[r];
+            a[r] += vb * yb[r];
+        }
+    }
+}
+
+// The wave's sum of the lanes' NR-vectors by recursive halving (each step a lane keeps half of its
+// vector and adds its partner's copy of that half; once a single value is left, plain xor steps):
+// afterwards lane r * 64 / NR holds column r's sum in a[0], in a fixed order.
+template <int NR>
+__device__ __forceinline__ void wave_column_sums(double *a, int lane) {
+#pragma unroll
+    for (int off = 32, half = NR / 2; half >= 1; off >>= 1, half >>= 1) {
+        const bool upper = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double keep = upper ? a[i + half] : a[i];
+            const double send = upper ? a[i] : a[i + half];
+            a[i] = keep + __shfl_xor(send, off);
+        }
+    }
+#pragma unroll
+    for (int off = 32 / NR; off >= 1; off >>= 1) a[0] += __shfl_xor(a[0], off);
+}
+
 // The operator's epilogue fused with the resistive part: W = Rl X + j w (sum of the nsplit chunk
-// partials of Ll X), for NR a power of two. One wave per row: lanes over the row's CSR entries
-// (port columns >= n skipped) and over the partials, each lane forming its NR-vector contribution;
-// then a recursive-halving butterfly (each step a lane keeps half of its vector and adds its
-// partner's copy of that half; once a single value is left, plain xor steps) leaves the sum of
-// column r in lane r * 64 / NR, in a fixed order.
+// partials of Ll X), for NR a power of two. One wave per row: lanes over the row's CSR entries and
+// over the partials, then one recursive-halving sum.
 template <int NR>
 __global__ __launch_bounds__(256) void operator_epilogue_kernel(const int *__restrict__ ptr,
                                                                const int *__restrict__ col,
@@ -347,35 +353,7 @@ __global__ __launch_bounds__(256) void operator_epilogue_kernel(const int *__res
 #pragma unroll
         for (int r = 0; r < NR; ++r) b[r] += pq[r];
     }
-    // the first 128 entries with every load issued up front (an absent or port entry gets weight 0
-    // and row 0's x); longer rows continue in the loop below
-    const int t0 = ptr[row], t1 = ptr[row + 1];
-    {
-        const int ta = t0 + lane, tb = ta + 64;
-        const int ca = ta < t1 ? col[ta] : n, cb = tb < t1 ? col[tb] : n;
-        const double va = ca < n ? val[ta] : 0.0, vb = cb < n ? val[tb] : 0.0;
-        const double *xa = X + (size_t)(ca < n ? ca : 0) * NR, *xb = X + (size_t)(cb < n ? cb : 0) * NR;
-        double ya[NR], yb[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            ya[r] = xa[r];
-            yb[r] = xb[r];
-        }
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            a[r] += va * ya[r];
-            a[r] += vb * yb[r];
-        }
-    }
-    for (int t = t0 + 128 + lane; t < t1; t += 64) {
-        const int c = col[t];
-        if (c < n) {
-            const double v = val[t];
-            const double *x = X + (size_t)c * NR;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) a[r] += v * x[r];
-        }
-    }
+    rl_row_lanes<NR>(ptr, col, val, n, X, row, lane, a);
     const double omega = *omega_dev;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -383,22 +361,75 @@ __global__ __launch_bounds__(256) void operator_epilogue_kernel(const int *__res
         a[p] = re;
         a[P + p] = im;
     }
-    // recursive halving over lane bits 5, 4, ...: after it, lane bits above log2(64 / NR) name the
-    // column
-#pragma unroll
-    for (int off = 32, half = NR / 2; half >= 1; off >>= 1, half >>= 1) {
-        const bool upper = (lane & off) != 0;
-#pragma unroll
-        for (int i = 0; i < half; ++i) {
-            const double keep = upper ? a[i + half] : a[i];
-            const double send = upper ? a[i] : a[i + half];
-            a[i] = keep + __shfl_xor(send, off);
-        }
-    }
-#pragma unroll
-    for (int off = 32 / NR; off >= 1; off >>= 1) a[0] += __shfl_xor(a[0], off);
+    wave_column_sums<NR>(a, lane);
     constexpr int S = 64 / NR;
     if (lane % S == 0) W[(size_t)row * NR + lane / S] = a[0];
+}
+
+// The four-wave panel product of one 64-row block (bx) and column chunk (by) of nsplit: the body of
+// panel4_kernel.
+template <int NR, class T>
+__device__ __forceinline__ void panel4_block(const T *__restrict__ M, long long ld, int nrows, int ncols,
+                                             const double *__restrict__ X, int chunk, int bx, int by, int nsplit,
+                                             double *__restrict__ part, double *red) {
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = bx * 64 + lane;
+    const int c0 = by * chunk, c1 = min(ncols, c0 + chunk);
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    const T *col = M + min(row, nrows - 1);
+    int c = c0 + 16 * w;
+    for (; c + 16 <= c1; c += 64) {
+        T m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = __builtin_nontemporal_load(col + (size_t)(c + u) * ld);
+        const double *x = X + (size_t)c * NR;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double mu = m[u];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += mu * x[u * NR + r];
+        }
+    }
+    for (int cc = c; cc < c1; ++cc) {        // the chunk's one partial group (this wave's, if any)
+        const double mv = __builtin_nontemporal_load(col + (size_t)cc * ld);
+        const double *x = X + (size_t)cc * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += mv * x[r];
+    }
+    if (w >= 2) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[((w - 2) * 64 + lane) * NR + r] = acc[r];
+    }
+    __syncthreads();
+    if (w < 2) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += red[(w * 64 + lane) * NR + r];
+    }
+    __syncthreads();
+    if (w == 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[lane * NR + r] = acc[r];
+    }
+    __syncthreads();
+    if (w == 0 && row < nrows) {
+        double *o = part + ((size_t)row * nsplit + by) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) o[r] = acc[r] + red[lane * NR + r];
+    }
+}
+
+// panel_kernel with a 256-thread block on 64 rows: its four waves take the chunk's groups of 16
+// columns in turn, and their sums are combined in LDS in a fixed order, (w0 + w2) + (w1 + w3), so
+// the chip fills with a quarter of the chunk partials that one wave per row block needs (fewer
+// partials for the epilogue to read). Same partial layout as panel_kernel.
+template <int NR, class T = float>
+__global__ __launch_bounds__(256) void panel4_kernel(const T *__restrict__ M, long long ld, int nrows, int ncols,
+                                                    const double *__restrict__ X, int chunk,
+                                                    double *__restrict__ part) {
+    __shared__ double red[2 * 64 * NR];
+    panel4_block<NR, T>(M, ld, nrows, ncols, X, chunk, blockIdx.x, blockIdx.y, gridDim.y, part, red);
 }
 
 // ---- the symmetric operator product ----------------------------------------------------------
@@ -1783,13 +1814,14 @@ void apply_operator(pq_loops *L, const double *X, double *W, int P) {
 }
 
 // W = (Rl + j w Ll64) X with the FP64 loop matrix: the system's own product, for right-hand
-// sides' residuals (GMRES restarts and acceptance, the direct solve's refinement). The panel
-// kernel over FP64 rows, with the chunk the autotuner chose for the FP32 product (or the default
-// where it chose the symmetric kernel); twice the FP32 product's bytes.
+// sides' residuals (GMRES restarts and acceptance, the direct solve's refinement). The four-wave
+// panel product over FP64 rows and the epilogue; twice the FP32 product's bytes.
 void apply_operator64(pq_loops *L, const double *X, double *W, int P) {
     const int NR = 2 * P, n = L->nf;
-    OpShape sh = gemv_shape(L, NR);
-    if (sh.chunk <= 0) sh = {default_chunk(n), GEMV_ROWS};
+    // a shape fixed by n alone (four-wave 64-row blocks, about eight column chunks), never the
+    // autotuner's: the FP64 product sets the residuals, right-hand sides and refinements, whose
+    // summation order must not depend on which shape won a timing race
+    const OpShape sh = {std::max(256, (n + 8 * 64 - 1) / (8 * 64) * 64), 64};
     const int nsplit = operator_splits(n, sh.chunk);
     L->gpart64.reserve((size_t)nsplit * n * NR * 8);
     double *part = L->gpart64.as<double>();

@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
     if (lane == 0) { cvec[(size_t)a * NR + p] = yr; cvec[(size_t)a * NR + P + p] = yi; }
 }
 
-// The prolongation: the three transposed bases (k, n) Phi^T, (Rl Phi)^T, (Ll Phi)^T (FP32, or FP16
+// The prolongation: the three transposed bases (k, ldt) Phi^T, (Rl Phi)^T, (Ll Phi)^T (FP32, or FP16
 // scaled by s0, s1, s2) read in one pass against the coarse vector c (k, NR). A 256-thread block
 // takes 64 rows (a row per lane) and a chunk of coarse columns (blockIdx.y); its four waves take the
 // chunk's groups of U columns in turn (3 U loads in flight per lane), c wave-uniform (scalar
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(const double *t, cons
 // one wave per row block needs.
 template <int NR, int U, class T>
 __global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, const T *__restrict__ B1,
-                                                     const T *__restrict__ B2, int n, int k,
+                                                     const T *__restrict__ B2, int n, long long ldt, int k,
                                                      const double *__restrict__ C, int chunk,
                                                      const double *__restrict__ omega_dev, double s0, double s1,
                                                      double s2, double *__restrict__ PY, double *__restrict__ PA) {
@@ -1046,9 +1046,9 @@ __global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, 
         T f0[U], f1[U], f2[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            f0[u] = B0[(size_t)(c + u) * n + rr];
-            f1[u] = B1[(size_t)(c + u) * n + rr];
-            f2[u] = B2[(size_t)(c + u) * n + rr];
+            f0[u] = B0[(size_t)(c + u) * ldt + rr];
+            f1[u] = B1[(size_t)(c + u) * ldt + rr];
+            f2[u] = B2[(size_t)(c + u) * ldt + rr];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1063,8 +1063,8 @@ __global__ __launch_bounds__(256) void prolong_kernel(const T *__restrict__ B0, 
         }
     }
     for (int cc = c; cc < c1; ++cc) {          // the chunk's one partial group (this wave's, if any)
-        const double a = (float)B0[(size_t)cc * n + rr], b = (float)B1[(size_t)cc * n + rr];
-        const double d = (float)B2[(size_t)cc * n + rr];
+        const double a = (float)B0[(size_t)cc * ldt + rr], b = (float)B1[(size_t)cc * ldt + rr];
+        const double d = (float)B2[(size_t)cc * ldt + rr];
         const double *x = C + (size_t)cc * NR;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -1192,12 +1192,13 @@ __global__ void whiten_kernel(const double *V, const double *s, int k, int first
     W[e] = V[(size_t)(first + j) * k + i] / sqrt(s[first + j]);
 }
 
-// Out[c][i] = (float) In[i][c]: FP64 (n, k) -> FP32 transposed (k, n).
-__global__ void transpose_round_kernel(const double *In, int n, int k, float *Out) {
+// Out[c][i] = (float) In[i][c]: FP64 (n, k) -> FP32 transposed (k, ldo), ldo >= n, the padding
+// columns i >= n zero (a multiple of 64 floats keeps every 64-row segment on whole cache lines).
+__global__ void transpose_round_kernel(const double *In, int n, int k, long long ldo, float *Out) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * k) return;
-    const size_t i = e / k, c = e % k;
-    Out[c * n + i] = (float)In[e];
+    if (e >= (size_t)ldo * k) return;
+    const size_t c = e / ldo, i = e % ldo;
+    Out[e] = i < (size_t)n ? (float)In[i * k + c] : 0.f;
 }
 
 // Phi (n, k) FP64 row-major from k CSC columns (zero beforehand): a thread per column.
@@ -1595,8 +1596,9 @@ struct pq_loops {
     double omega_bj = -1;
     // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
     int kc = 0, kp = 0;           // coarse columns; row stride of Phi
+    long long ldt = 0;            // row stride of the transposed bases Phi^T, (Rl Phi)^T, (Ll Phi)^T
     bool cdiag = false;           // basis diagonalises the Galerkin matrices
-    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart, ppart;   // RPhi, LPhi transposed (k, n)
+    DevBuf Phi, Phit, RPhi, LPhi, Ainv, cvec, cpart, ctot, T1, T2, gpart, ppart;   // Phit, RPhi, LPhi transposed (k, ldt)
     // FP16 copies of the four bases for the solve's restriction and prolongation, each divided by
     // its largest magnitude (hscale: Phi, Phi^T, (Rl Phi)^T, (Ll Phi)^T), when coarse_half is set
     DevBuf Phih, Phith, RPhih, LPhih, hmax;
@@ -1898,19 +1900,19 @@ template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, const 
     if (L->coarse_half) {
         const _Float16 *b0 = L->Phith.as<_Float16>(), *b1 = L->RPhih.as<_Float16>(), *b2 = L->LPhih.as<_Float16>();
         if (depth == 16)
-            hipLaunchKernelGGL((prolong_kernel<N, 16, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
+            hipLaunchKernelGGL((prolong_kernel<N, 16, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, L->ldt, k,
                                L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
         else
-            hipLaunchKernelGGL((prolong_kernel<N, 8, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
+            hipLaunchKernelGGL((prolong_kernel<N, 8, _Float16>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, L->ldt, k,
                                L->cvec.as<double>(), pc, omega_dev, sc[1], sc[2], sc[3], PY, PA);
         return nsplit;
     }
     const float *b0 = L->Phit.as<float>(), *b1 = L->RPhi.as<float>(), *b2 = L->LPhi.as<float>();
     if (depth == 16)
-        hipLaunchKernelGGL((prolong_kernel<N, 16, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
+        hipLaunchKernelGGL((prolong_kernel<N, 16, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, L->ldt, k,
                            L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     else
-        hipLaunchKernelGGL((prolong_kernel<N, 8, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, k,
+        hipLaunchKernelGGL((prolong_kernel<N, 8, float>), grid, dim3(256), 0, L->stream, b0, b1, b2, n, L->ldt, k,
                            L->cvec.as<double>(), pc, omega_dev, 1.0, 1.0, 1.0, PY, PA);
     return nsplit;
 }
@@ -2485,15 +2487,15 @@ void make_half_bases(pq_loops *L) {
     const char *hv = std::getenv("PCBQS_AMD_COARSE_FP16");
     L->coarse_half = !(hv && hv[0] == '0');
     if (!L->coarse_half) return;
-    L->Phih.reserve((size_t)n * kp * 2); L->Phith.reserve((size_t)n * k * 2);
-    L->RPhih.reserve((size_t)n * k * 2); L->LPhih.reserve((size_t)n * k * 2);
+    L->Phih.reserve((size_t)n * kp * 2); L->Phith.reserve((size_t)L->ldt * k * 2);
+    L->RPhih.reserve((size_t)L->ldt * k * 2); L->LPhih.reserve((size_t)L->ldt * k * 2);
     L->hmax.reserve(4 * 4);
     PQ_HIP(hipMemsetAsync(L->hmax.p, 0, 16, L->stream));
     const float *src[4] = {L->Phi.as<float>(), L->Phit.as<float>(), L->RPhi.as<float>(), L->LPhi.as<float>()};
     _Float16 *dst[4] = {L->Phih.as<_Float16>(), L->Phith.as<_Float16>(), L->RPhih.as<_Float16>(),
                         L->LPhih.as<_Float16>()};
     for (int q = 0; q < 4; ++q) {
-        const size_t cnt = q == 0 ? (size_t)n * kp : (size_t)n * k;
+        const size_t cnt = q == 0 ? (size_t)n * kp : (size_t)L->ldt * k;
         hipLaunchKernelGGL(absmax_kernel, dim3(1024), dim3(256), 0, L->stream, src[q], cnt, L->hmax.as<unsigned>() + q);
         hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, L->stream, src[q], cnt,
                            L->hmax.as<unsigned>() + q, dst[q]);
@@ -2658,19 +2660,20 @@ void coarse_diagonalise(pq_loops *L, DevBuf &P64, int k, double drop, int *k_out
     const int kpp = (kk + 63) / 64 * 64;
     L->kc = kk;
     L->kp = kpp;
-    L->Phi.reserve((size_t)n * kpp * 4); L->Phit.reserve((size_t)n * kk * 4);
-    L->RPhi.reserve((size_t)n * kk * 4); L->LPhi.reserve((size_t)n * kk * 4);
-    const unsigned nb = (unsigned)(((size_t)n * kk + 255) / 256);
+    L->ldt = (n + 63) / 64 * 64;
+    L->Phi.reserve((size_t)n * kpp * 4); L->Phit.reserve((size_t)L->ldt * kk * 4);
+    L->RPhi.reserve((size_t)L->ldt * kk * 4); L->LPhi.reserve((size_t)L->ldt * kk * 4);
+    const unsigned nb = (unsigned)(((size_t)L->ldt * kk + 255) / 256);
     gemm_rm(L, false, false, n, kk, k, P64.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
     hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kpp + 255) / 256)), dim3(256), 0, L->stream,
                        Y.as<double>(), n, kk, kpp, L->Phi.as<float>());
-    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk, L->ldt,
                        L->Phit.as<float>());
     gemm_rm(L, false, false, n, kk, k, RX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
-    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk, L->ldt,
                        L->RPhi.as<float>());
     gemm_rm(L, false, false, n, kk, k, LX.as<double>(), k, T.as<double>(), kk, Y.as<double>(), kk);
-    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk,
+    hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, Y.as<double>(), n, kk, L->ldt,
                        L->LPhi.as<float>());
     PQ_HIP(hipGetLastError());
     PQ_HIP(hipStreamSynchronize(L->stream));
@@ -3775,8 +3778,9 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         const int n = L->nf;
         L->kp = (k + 63) / 64 * 64;
         const int kp = L->kp;
-        L->Phi.reserve((size_t)n * kp * 4); L->Phit.reserve((size_t)n * k * 4);
-        L->RPhi.reserve((size_t)n * k * 4); L->LPhi.reserve((size_t)n * k * 4);
+        L->ldt = (n + 63) / 64 * 64;
+        L->Phi.reserve((size_t)n * kp * 4); L->Phit.reserve((size_t)L->ldt * k * 4);
+        L->RPhi.reserve((size_t)L->ldt * k * 4); L->LPhi.reserve((size_t)L->ldt * k * 4);
         DevBuf P64, RX, LX, G;
         P64.reserve((size_t)n * k * 8); RX.reserve((size_t)n * k * 8); LX.reserve((size_t)n * k * 8);
         G.reserve((size_t)k * k * 8);
@@ -3785,14 +3789,14 @@ int pq_loops_set_coarse(pq_loops *L, int k, const double *Phi, int diagonal) {
         // loop matrix) and the Galerkin matrices Phi^T Rl Phi, Phi^T Ll Phi in FP64
         hipLaunchKernelGGL(round_basis_kernel, dim3((unsigned)(((size_t)n * kp + 255) / 256)), dim3(256), 0, L->stream,
                            P64.as<double>(), n, k, kp, L->Phi.as<float>());
-        const unsigned nb = (unsigned)(((size_t)n * k + 255) / 256);
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, P64.as<double>(), n, k,
+        const unsigned nb = (unsigned)(((size_t)L->ldt * k + 255) / 256);
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, P64.as<double>(), n, k, L->ldt,
                            L->Phit.as<float>());
         PQ_HIP(hipGetLastError());
         basis_products(L, P64.as<double>(), k, RX.as<double>(), LX.as<double>());
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, RX.as<double>(), n, k,
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, RX.as<double>(), n, k, L->ldt,
                            L->RPhi.as<float>());
-        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, LX.as<double>(), n, k,
+        hipLaunchKernelGGL(transpose_round_kernel, dim3(nb), dim3(256), 0, L->stream, LX.as<double>(), n, k, L->ldt,
                            L->LPhi.as<float>());
         PQ_HIP(hipGetLastError());
         const unsigned kb = (unsigned)(((size_t)k * k + 255) / 256);

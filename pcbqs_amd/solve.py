@@ -180,7 +180,7 @@ class PortSolver:
             call('pq_loops_set_coarse', self._handle, k, ptr(phi), 1)
         self.coarse_k = k
 
-    def set_stream_coarse(self, grid=8, vias=True, extra=None, extra_loops=None):
+    def set_stream_coarse(self, grid=8, vias=True, extra=None, extra_loops=None, diagonalize=True):
         """The geometric coarse space: stream-function hats of a grid x grid net per layer, the
         via-closed loops (vias=True), plus extra branch patterns (nb, k') and extra loop-space
         columns (n_free, k'', dense or sparse) when given. The columns stay sparse up to the device
@@ -195,7 +195,7 @@ class PortSolver:
             cols.append(via_loop_columns(self.model, dense=False))
         if extra_loops is not None and extra_loops.shape[1]:
             cols.append(sparse.csc_matrix(extra_loops))
-        self.set_coarse(unique_columns(sparse.hstack(cols, format='csc')))
+        self.set_coarse(unique_columns(sparse.hstack(cols, format='csc')), diagonalize=diagonalize)
 
     @property
     def n_ports(self):
@@ -435,17 +435,22 @@ def main(argv=None):
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--coarse', choices=['hats', 'learned'], default='learned',
                     help='two-level coarse space: 36 x 36 hats and via loops, or with the learned enrichment')
+    ap.add_argument('--coarse-diagonalize', choices=['device', 'host'], default='device',
+                    help="the coarse Galerkin pair's eigenbasis on the device (rocSOLVER) or with numpy on the host "
+                         "(for runs under rocprofv3 --pmc, where rocSOLVER's dsyevd ended the process with a host "
+                         "segmentation fault)")
     args = ap.parse_args(argv)
     b = {'c1': lambda: boards.board_c1(), 'c2': lambda: boards.board_c2(args.seed),
          'c5': lambda: boards.board_c5(args.seed)}[args.board]()
     s = PortSolver(b, device=0)
     print(f'set-up: {b.nt} triangles, {s.model.n_free} free loops, {s.setup_seconds}', flush=True)
     if args.precond == 'two-level' and args.solver != 'direct':
+        how = True if args.coarse_diagonalize == 'device' else 'host'
         if args.coarse == 'learned':
-            from .learned import MPNN, set_learned_coarse
-            set_learned_coarse(s, MPNN.load(), 36)
+            from .learned import MPNN, BranchGraph, enrichment
+            s.set_stream_coarse(36, extra_loops=enrichment(MPNN.load(), BranchGraph(s.model)), diagonalize=how)
         else:
-            s.set_stream_coarse(36)
+            s.set_stream_coarse(36, diagonalize=how)
         print(f'coarse space: {s.coarse_k} columns', flush=True)
     Z, info = s.ports(args.frequencies, solver=args.solver, precond=args.precond, tol=args.tol)
     print(json.dumps(dict(board=args.board, triangles=int(b.nt), free_loops=int(s.model.n_free),

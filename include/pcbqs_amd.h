@@ -394,6 +394,16 @@ int pq_loops_set_coarse_diagonal(pq_loops *loops, int k, const double *Phi, doub
 int pq_loops_set_coarse_csc(pq_loops *loops, int k, const int32_t *colptr, const int32_t *rows,
                             const double *vals, double drop, int *k_out, int *path_out);
 
+/* The block-Jacobi preconditioner's per-handle set-up (otherwise done by the
+ * first solve that uses it): the generalised eigenvectors of every 256-loop
+ * diagonal block's pencil (Ll_bb, Rl_bb), FP64 (rocSOLVER dsygvd, batched),
+ * kept in FP32, so that each frequency's block inverses are
+ * V diag(1 / (1 + j w lam)) V^T, one batched complex GEMM. *modal_out: 1 when
+ * the modes are ready, -1 when a block's Rl_bb is not positive definite (then
+ * every frequency inverts its blocks by Gauss-Jordan with partial pivoting),
+ * 0 when disabled (PCBQS_AMD_BJ_MODAL=0). No counterpart in the reference. */
+int pq_loops_prepare_block_jacobi(pq_loops *loops, int *modal_out);
+
 /* The Galerkin matrices Phi^T Rl Phi and Phi^T Ll Phi (k x k FP64, row-major)
  * of the current coarse space, for a host-side change of basis. No counterpart
  * in the reference. */

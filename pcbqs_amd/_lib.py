@@ -122,6 +122,7 @@ _SIGNATURES = {
     'pq_loops_coarse_solution': ([P, DBL, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_set_coarse_diagonal': ([P, ctypes.c_int, P, DBL, P], ctypes.c_int),
     'pq_loops_set_coarse_csc': ([P, ctypes.c_int, P, P, P, DBL, P, P], ctypes.c_int),
+    'pq_loops_prepare_block_jacobi': ([P, P], ctypes.c_int),
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
     'pq_loops_tuning': ([P, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_use_graphs': ([P, ctypes.c_int], ctypes.c_int),

@@ -675,6 +675,49 @@ __global__ __launch_bounds__(1024) void bj_setup_kernel(const float *Ll, long lo
     }
 }
 
+// The block-Jacobi inverses from each block's modes. Z_b = R_b + j w L_b with R_b symmetric
+// positive definite and L_b symmetric: with the generalised eigenvectors V_b (L_b V = R_b V Lam,
+// V^T R_b V = I), Z_b^-1 = V_b diag(1 / (1 + j w lam)) V_b^T at every frequency. So one batched
+// eigensolve per handle (rocSOLVER dsygvd) replaces the per-frequency Gauss-Jordan inversion, and a
+// frequency's inverses are one batched complex GEMM (rocBLAS cgemm) into the same layout.
+
+// R_b and L_b (FP64, BS x BS) of every diagonal block, grid (BS rows, nblk): L from the FP64 loop
+// matrix's rows, R from the resistance's CSR rows; beyond a short last block's m rows and columns,
+// R = I and L = 0 (modes with lam = 0: the inverse is the identity there, as the padded block is).
+__global__ __launch_bounds__(256) void bj_pencil_kernel(const double *__restrict__ Ll64, long long ldl,
+                                                       const int *__restrict__ ptr, const int *__restrict__ col,
+                                                       const double *__restrict__ val, int n, double *Lb,
+                                                       double *Rb) {
+    const int i = blockIdx.x, b = blockIdx.y, r0 = b * BS, m = min(BS, n - r0);
+    double *lrow = Lb + ((size_t)b * BS + i) * BS, *rrow = Rb + ((size_t)b * BS + i) * BS;
+    for (int j = threadIdx.x; j < BS; j += blockDim.x) {
+        lrow[j] = (i < m && j < m) ? Ll64[(size_t)(r0 + i) * ldl + r0 + j] : 0.0;
+        rrow[j] = (i >= m && i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (i < m)
+        for (int t = ptr[r0 + i] + threadIdx.x; t < ptr[r0 + i + 1]; t += blockDim.x) {   // distinct columns
+            const int c = col[t] - r0;
+            if (c >= 0 && c < m) rrow[c] = val[t];
+        }
+}
+
+// The eigenvectors (FP64, column-major) as the FP32 complex right operand of the inverses' product.
+__global__ void bj_modes_kernel(const double *V, size_t count, float2 *Vc) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) Vc[e] = make_float2((float)V[e], 0.f);
+}
+
+// A_b = V_b diag(1 / (1 + j w lam_b)) (column-major: entry e = i + a BS of block b scales by mode a).
+__global__ void bj_scale_kernel(const float2 *Vc, const double *lam, size_t count, double omega, float2 *A) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const size_t b = e / ((size_t)BS * BS), a = (e / BS) % BS;
+    const double wl = omega * lam[b * BS + a], den = 1.0 + wl * wl;
+    const float v = Vc[e].x;
+    A[e] = make_float2((float)(v / den), (float)(-v * wl / den));
+}
+
 // Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P. A workgroup
 // takes bj_rows(NR) rows of block blockIdx.x (from row blockIdx.y * bj_rows); four adjacent lanes
 // share a (row, port) output and take the columns j = s, s + 4, ... (32 contiguous bytes of the
@@ -1594,6 +1637,8 @@ struct pq_loops {
     hipStream_t stream = nullptr;
     int nblk = 0;
     double omega_bj = -1;
+    int bj_modal = 0;             // block-Jacobi from the blocks' modes: 0 not tried, 1 ready, -1 unusable
+    DevBuf bjV, bjlam, bjA;       // modes (FP32 complex, column-major per block), eigenvalues, scaled modes
     // coarse space: Phi (nf, k), Rl Phi, Ll Phi (FP64), the real k x k Galerkin matrices on the host
     int kc = 0, kp = 0;           // coarse columns; row stride of Phi
     long long ldt = 0;            // row stride of the transposed bases Phi^T, (Rl Phi)^T, (Ll Phi)^T
@@ -2364,6 +2409,8 @@ struct RocLib {
     decltype(&rocsolver_dsyevd) syevd = nullptr;
     decltype(&rocsolver_dpotrf) potrf = nullptr;
     decltype(&rocblas_dtrsm) trsm = nullptr;
+    decltype(&rocsolver_dsygvd_strided_batched) sygvd_batched = nullptr;
+    decltype(&rocblas_cgemm_strided_batched) cgemm_batched = nullptr;
     std::string error;
 };
 
@@ -2388,8 +2435,10 @@ const RocLib &roc_lib() {
         lu.syevd = (decltype(lu.syevd))dlsym(solver, "rocsolver_dsyevd");
         lu.potrf = (decltype(lu.potrf))dlsym(solver, "rocsolver_dpotrf");
         lu.trsm = (decltype(lu.trsm))dlsym(blas, "rocblas_dtrsm");
+        lu.sygvd_batched = (decltype(lu.sygvd_batched))dlsym(solver, "rocsolver_dsygvd_strided_batched");
+        lu.cgemm_batched = (decltype(lu.cgemm_batched))dlsym(blas, "rocblas_cgemm_strided_batched");
         if (!lu.create || !lu.destroy || !lu.set_stream || !lu.getrf || !lu.getrs || !lu.dgemm || !lu.syevd ||
-            !lu.potrf || !lu.trsm)
+            !lu.potrf || !lu.trsm || !lu.sygvd_batched || !lu.cgemm_batched)
             lu.error = "rocBLAS/rocSOLVER entry points missing";
     });
     if (!lu.error.empty()) throw Error(PQ_EDEVICE, lu.error);
@@ -2689,6 +2738,77 @@ void coarse_diagonalise(pq_loops *L, DevBuf &P64, int k, double drop, int *k_out
     *k_out = kk;
 }
 
+
+// The block-Jacobi inverses at omega into L->inv: from the blocks' modes (computed once per handle:
+// the pencils R_b, L_b in FP64, rocSOLVER's batched dsygvd) by one batched cgemm, or, where a block's
+// R_b is not positive definite, by the per-frequency Gauss-Jordan inversion with partial pivoting.
+void block_jacobi_modes(pq_loops *L) {
+    const int n = L->nf;
+    L->nblk = (n + BS - 1) / BS;
+    const int nb = L->nblk;
+    const size_t per = (size_t)BS * BS, count = per * nb;
+    const char *mv = std::getenv("PCBQS_AMD_BJ_MODAL");
+    if (n > 0 && L->bj_modal == 0 && !(mv && mv[0] == '0')) {
+        const RocLib &lib = roc_lib();
+        DevBuf Lb, Rb, E, info;
+        Lb.reserve(count * 8); Rb.reserve(count * 8); E.reserve((size_t)nb * BS * 8); info.reserve((size_t)nb * 4);
+        L->bjlam.reserve((size_t)nb * BS * 8);
+        hipLaunchKernelGGL(bj_pencil_kernel, dim3(BS, nb), dim3(256), 0, L->stream, L->Ll64.as<double>(), L->ldl,
+                           L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, Lb.as<double>(),
+                           Rb.as<double>());
+        PQ_HIP(hipGetLastError());
+        blas_check(lib.sygvd_batched(blas_handle(L), rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, BS,
+                                     Lb.as<double>(), BS, (rocblas_stride)per, Rb.as<double>(), BS,
+                                     (rocblas_stride)per, L->bjlam.as<double>(), BS, E.as<double>(), BS,
+                                     info.as<rocblas_int>(), nb),
+                   "rocsolver_dsygvd_strided_batched");
+        std::vector<rocblas_int> bad(nb);
+        PQ_HIP(hipMemcpyAsync(bad.data(), info.p, (size_t)nb * 4, hipMemcpyDeviceToHost, L->stream));
+        PQ_HIP(hipStreamSynchronize(L->stream));
+        L->bj_modal = std::all_of(bad.begin(), bad.end(), [](rocblas_int v) { return v == 0; }) ? 1 : -1;
+        if (L->bj_modal == 1) {
+            L->bjV.reserve(count * 8);
+            L->bjA.reserve(count * 8);
+            hipLaunchKernelGGL(bj_modes_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream,
+                               Lb.as<double>(), count, L->bjV.as<float2>());
+            PQ_HIP(hipGetLastError());
+            PQ_HIP(hipStreamSynchronize(L->stream));      // the pencils are freed on return
+        }
+    }
+}
+
+void block_jacobi_setup(pq_loops *L, double omega) {
+    const int n = L->nf;
+    L->nblk = (n + BS - 1) / BS;
+    const int nb = L->nblk;
+    const size_t per = (size_t)BS * BS, count = per * nb;
+    L->inv.reserve(count * 2 * 4);
+    block_jacobi_modes(L);
+    if (L->bj_modal == 1) {
+        hipLaunchKernelGGL(bj_scale_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, L->stream,
+                           L->bjV.as<float2>(), L->bjlam.as<double>(), count, omega, L->bjA.as<float2>());
+        PQ_HIP(hipGetLastError());
+        // inv_b = A_b V_b^T (symmetric, so its column-major and row-major layouts agree)
+        const rocblas_float_complex one(1.f, 0.f), zero(0.f, 0.f);
+        blas_check(roc_lib().cgemm_batched(blas_handle(L), rocblas_operation_none, rocblas_operation_transpose, BS, BS,
+                                           BS, &one, L->bjA.as<rocblas_float_complex>(), BS, (rocblas_stride)per,
+                                           L->bjV.as<rocblas_float_complex>(), BS, (rocblas_stride)per, &zero,
+                                           L->inv.as<rocblas_float_complex>(), BS, (rocblas_stride)per, nb),
+                   "rocblas_cgemm_strided_batched");
+        return;
+    }
+    PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
+    L->bjperm.reserve((size_t)nb * BS * 4);
+    hipLaunchKernelGGL(bj_setup_kernel, dim3(nb), dim3(1024), 0, L->stream, L->Ll.as<float>(), L->ldl,
+                       L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, omega, L->inv.as<float>(),
+                       L->bjperm.as<int>(), L->singular.as<int>());
+    PQ_HIP(hipGetLastError());
+    int sing = 0;
+    PQ_HIP(hipMemcpyAsync(&sing, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));
+    PQ_HIP(hipStreamSynchronize(L->stream));
+    PQ_REQUIRE(!sing, "block-Jacobi: a singular diagonal block");
+}
+
 }  // namespace
 
 extern "C" {
@@ -2864,7 +2984,7 @@ int pq_loops_destroy(pq_loops *L) {
     return guard([&] {
         DeviceScope scope(L->device);
         if (L->stream) (void)hipStreamSynchronize(L->stream);
-        for (DevBuf *b : {&L->Ll, &L->Ll64, &L->gpart64, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->bjperm, &L->singular, &L->V, &L->W, &L->Z,
+        for (DevBuf *b : {&L->Ll, &L->Ll64, &L->gpart64, &L->rptr, &L->rcol, &L->rval, &L->sc, &L->inv, &L->bjperm, &L->singular, &L->bjV, &L->bjlam, &L->bjA, &L->V, &L->W, &L->Z,
                           &L->X, &L->B, &L->R, &L->partial, &L->h, &L->scal, &L->Phi, &L->RPhi, &L->LPhi, &L->Ainv,
                           &L->cvec, &L->cpart, &L->ctot, &L->T1, &L->T2, &L->gpart, &L->Phit, &L->ppart})
             b->release();
@@ -2928,18 +3048,7 @@ int pq_loops_solve_ports(pq_loops *L, double omega, int P, const int32_t *port_c
             PQ_HIP(hipGetLastError());
             if (precond == 2) prepare_coarse(L, omega, P);
             if (precond >= 1 && L->omega_bj != omega) {
-                L->nblk = (n + BS - 1) / BS;
-                L->inv.reserve((size_t)L->nblk * BS * BS * 2 * 4);
-                PQ_HIP(hipMemsetAsync(L->singular.p, 0, 4, L->stream));
-                L->bjperm.reserve((size_t)L->nblk * BS * 4);
-                hipLaunchKernelGGL(bj_setup_kernel, dim3(L->nblk), dim3(1024), 0, L->stream, L->Ll.as<float>(), L->ldl,
-                                   L->rptr.as<int>(), L->rcol.as<int>(), L->rval.as<double>(), n, omega,
-                                   L->inv.as<float>(), L->bjperm.as<int>(), L->singular.as<int>());
-                PQ_HIP(hipGetLastError());
-                int sing = 0;
-                PQ_HIP(hipMemcpyAsync(&sing, L->singular.p, 4, hipMemcpyDeviceToHost, L->stream));
-                PQ_HIP(hipStreamSynchronize(L->stream));
-                PQ_REQUIRE(!sing, "block-Jacobi: a singular diagonal block");
+                block_jacobi_setup(L, omega);
                 L->omega_bj = omega;
             }
             std::vector<double> bnorm;
@@ -3863,6 +3972,15 @@ int pq_loops_set_coarse_csc(pq_loops *L, int k, const int32_t *colptr, const int
                            rw.as<int>(), vl.as<double>(), k, P64.as<double>());
         PQ_HIP(hipGetLastError());
         coarse_diagonalise(L, P64, k, drop, k_out, path_out);
+    });
+}
+
+int pq_loops_prepare_block_jacobi(pq_loops *L, int *modal_out) {
+    return guard([&] {
+        PQ_REQUIRE(L, "pq_loops_prepare_block_jacobi: null handle");
+        DeviceScope scope(L->device);
+        block_jacobi_modes(L);
+        if (modal_out) *modal_out = L->bj_modal;
     });
 }
 

@@ -197,6 +197,15 @@ class PortSolver:
             cols.append(sparse.csc_matrix(extra_loops))
         self.set_coarse(unique_columns(sparse.hstack(cols, format='csc')), diagonalize=diagonalize)
 
+    def prepare_block_jacobi(self):
+        """The block-Jacobi preconditioner's per-board set-up (done by the first iterative solve
+        otherwise): the 256-loop blocks' generalised eigenvectors, from which every frequency's block
+        inverses are one batched GEMM. Returns 'modes', or 'gauss-jordan' where a block's resistance
+        is not positive definite (each frequency then inverts its blocks)."""
+        modal = ctypes.c_int()
+        call('pq_loops_prepare_block_jacobi', self._handle, ctypes.byref(modal))
+        return {1: 'modes', -1: 'gauss-jordan', 0: 'gauss-jordan'}[modal.value]
+
     @property
     def n_ports(self):
         return len(self.port_cols)

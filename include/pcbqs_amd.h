@@ -238,6 +238,57 @@ typedef struct pq_loop_tree {
     const int32_t *end_b;        /* (n_loops) its second end node                 */
 } pq_loop_tree;
 
+/* ---- host topology (no device use) ------------------------------------------------------ */
+
+typedef struct pq_loop_basis pq_loop_basis;   /* a fundamental loop basis on the host */
+typedef struct pq_csr pq_csr;                 /* a host CSR matrix (int64 indices, FP64 values) */
+
+/* The fundamental loop basis of a branch graph (spanning_tree and MQS._loops,
+ * src/pcbqs/mqs.py:48-129, 592-637): branches k = 0 .. n_branches-1 run from
+ * ends[2k] to ends[2k+1]; is_source marks the ports (never in the tree);
+ * branches below n_copper are the copper (RWG edges and via segments: a
+ * breadth-first forest from a central root per island, neighbours in
+ * ascending order, each node pair's lowest-numbered branch), the others join
+ * islands the copper leaves apart. One loop per non-tree branch whose ends the
+ * tree joins (an RWG branch, k < n_rwg, with both ends on one node carries
+ * none when ideal_terminals). Loops are ordered free loops first, along a
+ * Morton curve of their centroids (the mean of their branches' branch_pos,
+ * n_branches x 2), then the port loops. Host work only; thread-safe across
+ * handles. */
+int pq_loop_basis_create(int64_t n_nodes, int64_t n_branches, const int64_t *ends, const uint8_t *is_source,
+                         int64_t n_copper, int64_t n_rwg, const double *branch_pos, int ideal_terminals,
+                         pq_loop_basis **out);
+/* info: [0] loops, [1] nonzeros of C, [2] free loops, [3] port loops. */
+int pq_loop_basis_sizes(const pq_loop_basis *basis, int64_t info[4]);
+/* Copies out (each pointer optional): tree (n_branches, 0/1), parent, upb, ups
+ * (+1 where the parent branch runs from the parent to the node), depth
+ * (n_nodes each, -1 / 0 at a root), cotree (loops: the closing branch), C in
+ * CSC (colptr loops + 1, rows and vals nonzeros; rows ascending, vals +-1),
+ * positions (loops x 2, the centroids) and port_branch (port loops). */
+int pq_loop_basis_copy(const pq_loop_basis *basis, uint8_t *tree, int64_t *parent, int64_t *upb, int64_t *ups,
+                       int64_t *depth, int64_t *cotree, int64_t *colptr, int64_t *rows, double *vals,
+                       double *positions, int64_t *port_branch);
+int pq_loop_basis_destroy(pq_loop_basis *basis);
+
+/* Sheet resistance of ne RWG edges in total edge currents (MQS._resistance,
+ * src/pcbqs/mqs.py:478-515): per triangle t and pair (m, n) of its edges,
+ * alpha_m alpha_n (area_t / 3) sum_q (P_q - v_m).(P_q - v_n) / (sigma thickness_t)
+ * / (l_m l_n), alpha = l / (2 area) on T+ (tp) and -l / (2 area) on T- (tm),
+ * v the free vertices (vp, vm: ne x 2), P_q the 3-point rule points
+ * (points: nt x 3 x 2). Summed over the triangles an edge pair shares. */
+int pq_sheet_resistance(int64_t ne, const int64_t *tp, const int64_t *tm, const double *length, const double *vp,
+                        const double *vm, int64_t nt, const double *area, const double *points,
+                        const double *thickness, double sigma, pq_csr **out);
+/* Rl = C^T R C (MQS._assemble's resistive part, src/pcbqs/mqs.py:639-660) with
+ * R = blockdiag(re, diag(rv), 0): re the sheet resistance of the first re
+ * rows' branches, rv (nv) the via barrels that follow, no resistance on the
+ * remaining (port) branches. Rows and columns ascending, exact zeros dropped. */
+int pq_loop_resistance(const pq_loop_basis *basis, const pq_csr *re, int64_t nv, const double *rv, pq_csr **out);
+/* info: [0] rows, [1] columns, [2] nonzeros. */
+int pq_csr_sizes(const pq_csr *m, int64_t info[3]);
+int pq_csr_copy(const pq_csr *m, int64_t *ptr, int64_t *idx, double *val);
+int pq_csr_destroy(pq_csr *m);
+
 /* pq_loops_create with C given by its tree: U = Lb C from prefix sums of Lb's
  * rows down the tree (U[k][j] = Lb[k][closing j] + S_k(end_a j) - S_k(end_b j)),
  * then Ll = C^T U the same way on U's rows; FP64 sums and storage (U and Ll),

@@ -126,6 +126,15 @@ _SIGNATURES = {
     'pq_loops_last_solve': ([P, P, P, P, P], ctypes.c_int),
     'pq_loops_tuning': ([P, ctypes.c_int, P, P], ctypes.c_int),
     'pq_loops_use_graphs': ([P, ctypes.c_int], ctypes.c_int),
+    'pq_loop_basis_create': ([I64, I64, P, P, I64, I64, P, ctypes.c_int, P], ctypes.c_int),
+    'pq_loop_basis_sizes': ([P, P], ctypes.c_int),
+    'pq_loop_basis_copy': ([P, P, P, P, P, P, P, P, P, P, P, P], ctypes.c_int),
+    'pq_loop_basis_destroy': ([P], ctypes.c_int),
+    'pq_sheet_resistance': ([I64, P, P, P, P, P, I64, P, P, P, DBL, P], ctypes.c_int),
+    'pq_loop_resistance': ([P, P, I64, P, P], ctypes.c_int),
+    'pq_csr_sizes': ([P, P], ctypes.c_int),
+    'pq_csr_copy': ([P, P, P, P], ctypes.c_int),
+    'pq_csr_destroy': ([P], ctypes.c_int),
 }
 
 

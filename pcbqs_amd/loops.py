@@ -10,13 +10,18 @@ What is built follows the reference's MQS model (src/pcbqs/mqs.py):
                (mqs.py:478-515); via barrels' DC resistance (mqs.py:132-137)
   L_vias       parallel-filament partial inductances (Grover; mqs.py:40-45,140-153)
 The loop basis is any basis of the cycle space: port impedances and branch currents do not
-depend on the choice. The tree is built with scipy's compiled BFS and the loop paths by a
-vectorized climb to the lowest common ancestor.
+depend on the choice. LoopModel builds it with numpy and scipy (loop_basis, sheet_resistance and
+scipy's C^T R C). native=True builds the same tree and loops (bitwise) and the resistances within
+rounding in the library's host code instead (pq_loop_basis_create, pq_sheet_resistance,
+pq_loop_resistance), 1.5-2x faster per board and without holding the interpreter; it is opt-in: a
+sweep whose worker threads ran it while other threads made their first device calls ended in a
+host segmentation fault inside it whose cause is not known.
 
 The loops are then ordered for the device: free loops first, sorted along a Morton curve of
 their centroids (so that consecutive blocks of loops are spatially compact: block-Jacobi
 blocks), the port loops last.
 """
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -227,10 +232,65 @@ def morton(p):
     return code
 
 
-class LoopModel:
-    """A meshed board (pcbqs_amd.board.Board) as the loop solve sees it."""
+def _csr_from(handle):
+    """A scipy CSR copy of a pq_csr handle."""
+    from ._lib import call, ptr
+    info = np.zeros(3, np.int64)
+    call('pq_csr_sizes', handle, ptr(info))
+    rows, cols, nnz = (int(v) for v in info)
+    indptr, indices, data = np.empty(rows + 1, np.int64), np.empty(nnz, np.int64), np.empty(nnz)
+    call('pq_csr_copy', handle, ptr(indptr), ptr(indices), ptr(data))
+    return sparse.csr_matrix((data, indices, indptr), shape=(rows, cols))
 
-    def __init__(self, board, ideal_terminals=True):
+
+def native_topology(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_terminals, rwg, t_of, sigma, Rv):
+    """(LoopBasis, port branches, Re, Rl) from the library's host code: the loop basis as loop_basis
+    builds it, the sheet resistance as sheet_resistance, Rl = C^T blockdiag(Re, diag(Rv), 0) C."""
+    from ._lib import call, ptr
+    ends = np.ascontiguousarray(np.asarray(ends, np.int64).reshape(-1, 2))
+    src = np.ascontiguousarray(is_src, np.uint8)
+    bpos = np.ascontiguousarray(branch_pos, np.float64)
+    nb = len(ends)
+    hb, hre, hrl = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        call('pq_loop_basis_create', int(n_nodes), nb, ptr(ends), ptr(src), int(n_copper), int(n_rwg), ptr(bpos),
+             int(bool(ideal_terminals)), ctypes.byref(hb))
+        info = np.zeros(4, np.int64)
+        call('pq_loop_basis_sizes', hb, ptr(info))
+        nl, nnz, n_free, n_port = (int(v) for v in info)
+        tree = np.empty(nb, np.uint8)
+        parent, upb, ups, depth = (np.empty(int(n_nodes), np.int64) for _ in range(4))
+        cot, colptr, rows = np.empty(nl, np.int64), np.empty(nl + 1, np.int64), np.empty(nnz, np.int64)
+        vals, pos, port_branch = np.empty(nnz), np.empty((nl, 2)), np.empty(n_port, np.int64)
+        call('pq_loop_basis_copy', hb, ptr(tree), ptr(parent), ptr(upb), ptr(ups), ptr(depth), ptr(cot), ptr(colptr),
+             ptr(rows), ptr(vals), ptr(pos), ptr(port_branch))
+        # the sheet resistance: the 3-point rule points in numpy (the same products as sheet_resistance)
+        points = np.ascontiguousarray(np.einsum('qk,tkc->tqc', BAR3, rwg.nodes[rwg.tris]))
+        tp, tm = np.ascontiguousarray(rwg.tp, np.int64), np.ascontiguousarray(rwg.tm, np.int64)
+        vp, vm = np.ascontiguousarray(rwg.vp, np.float64), np.ascontiguousarray(rwg.vm, np.float64)
+        length, area = np.ascontiguousarray(rwg.l, np.float64), np.ascontiguousarray(rwg.area, np.float64)
+        thick = np.ascontiguousarray(t_of, np.float64)
+        call('pq_sheet_resistance', int(rwg.Ne), ptr(tp), ptr(tm), ptr(length), ptr(vp), ptr(vm), len(area), ptr(area),
+             ptr(points), ptr(thick), float(sigma), ctypes.byref(hre))
+        Re = _csr_from(hre)
+        rv = np.ascontiguousarray(Rv, np.float64)
+        call('pq_loop_resistance', hb, hre, len(rv), ptr(rv), ctypes.byref(hrl))
+        Rl = _csr_from(hrl)
+    finally:
+        for h, name in ((hrl, 'pq_csr_destroy'), (hre, 'pq_csr_destroy'), (hb, 'pq_loop_basis_destroy')):
+            if h.value:
+                call(name, h)
+    C = sparse.csc_matrix((vals, rows, colptr), shape=(nb, nl))
+    basis = LoopBasis(C, None, n_free, np.arange(n_free, nl), tree.astype(bool), pos, cot, parent, upb, ups, depth,
+                      ends[cot])
+    return basis, port_branch, Re, Rl
+
+
+class LoopModel:
+    """A meshed board (pcbqs_amd.board.Board) as the loop solve sees it. native=True builds the
+    topology and resistances with the library's host code (opt-in, see the module docstring)."""
+
+    def __init__(self, board, ideal_terminals=True, native=False):
         self.board = board
         self.rwg = RWG(board.nodes, board.tris)
         r = self.rwg
@@ -249,17 +309,27 @@ class LoopModel:
             bpos[self.ne + k] = (x, y)
         for k, (_, a, b) in enumerate(board.ports):
             bpos[self.ne + self.nv + k] = bpos[:self.ne].mean(0) if self.ne else 0
-        self.basis, port_branch = loop_basis(board.n_nodes, self.ends, self.is_src, len(self.br), self.ne, bpos,
-                                             ideal_terminals)
-        # the ports' loops in port order
-        self.port_cols = np.array([self.basis.port_cols[list(port_branch).index(self.ne + self.nv + k)]
-                                   for k in range(self.np)])
-        self.Re = sheet_resistance(r, board.t_of, board.sigma)
         self.Rv, self.Lv = via_matrices(board.vseg, board.sigma, board.plating)
-        R = sparse.block_diag((self.Re, sparse.diags(self.Rv), sparse.csr_matrix((self.np, self.np))), format='csr')
-        C = self.basis.C
-        self.Rl = (C.T @ R @ C).tocsr()
-        self.R = R
+        if native:
+            self.basis, port_branch, self.Re, self.Rl = native_topology(
+                board.n_nodes, self.ends, self.is_src, len(self.br), self.ne, bpos, ideal_terminals, r, board.t_of,
+                board.sigma, self.Rv)
+        else:
+            self.basis, port_branch = loop_basis(board.n_nodes, self.ends, self.is_src, len(self.br), self.ne, bpos,
+                                                 ideal_terminals)
+            self.Re = sheet_resistance(r, board.t_of, board.sigma)
+            C = self.basis.C
+            self.Rl = (C.T @ self.R @ C).tocsr()
+        # the ports' loops in port order
+        where = {int(b): k for k, b in enumerate(port_branch)}
+        self.port_cols = np.array([self.basis.port_cols[where[self.ne + self.nv + k]] for k in range(self.np)],
+                                  dtype=np.int64)
+
+    @property
+    def R(self):
+        """The branch resistance blockdiag(Re, diag(Rv), 0) (sparse CSR)."""
+        return sparse.block_diag((self.Re, sparse.diags(self.Rv), sparse.csr_matrix((self.np, self.np))),
+                                 format='csr')
 
     @property
     def n_loops(self):

@@ -71,3 +71,30 @@ def test_automatic_solver_choice_follows_the_reference_rule():
     assert port_method(8000, 8, 20) == 'direct' and port_method(8001, 8, 20) == 'iterative'
     assert port_method(240, 4, 1) == 'direct' and port_method(241, 4, 1) == 'iterative'
     assert port_method(9639, 4, 1) == 'iterative' and port_method(32211, 8, 100) == 'iterative'
+
+
+@pytest.mark.parametrize('make', [lambda: boards.board_c1(target=600), lambda: boards.board_batch(3, (300, 500)),
+                                  lambda: boards.board_batch(5, (1500, 2500))])
+def test_library_topology_equals_the_numpy_construction(make):
+    """pq_loop_basis_create / pq_sheet_resistance / pq_loop_resistance against loop_basis,
+    sheet_resistance and scipy's C^T R C: the same tree and loops bitwise, every loop a cycle of the
+    branch graph, Re bitwise, Rl exactly symmetric and within rounding of the scipy product."""
+    from scipy import sparse
+    b = make()
+    n, p = LoopModel(b, native=True), LoopModel(b)
+    a, q = n.basis, p.basis
+    assert a.n_free == q.n_free and np.array_equal(n.port_cols, p.port_cols)
+    for name in ('tree', 'parent', 'upb', 'ups', 'depth', 'cotree', 'positions', 'cot_ends'):
+        assert np.array_equal(getattr(a, name), getattr(q, name)), name
+    assert (a.C != q.C).nnz == 0 and np.array_equal(a.C.indptr, q.C.indptr) and np.array_equal(a.C.indices, q.C.indices)
+    # every column of C is a cycle: the node-branch incidence annihilates it
+    ends = n.ends
+    nb = len(ends)
+    inc = sparse.csr_matrix((np.r_[np.ones(nb), -np.ones(nb)], (np.r_[ends[:, 0], ends[:, 1]],
+                                                                  np.r_[np.arange(nb), np.arange(nb)])),
+                            shape=(b.n_nodes, nb))
+    assert abs(inc @ a.C).max() == 0
+    assert (n.Re != p.Re).nnz == 0
+    assert (n.Rl != n.Rl.T).nnz == 0
+    ref = (a.C.T @ n.R @ a.C).tocsr()
+    assert abs(n.Rl - ref).max() <= 1e-13 * abs(ref).max()

@@ -776,7 +776,9 @@ int pq_fill_create(const pq_geometry *g, int device, pq_fill **out) {
             G.far32 = f->far32.as<float4>();
             hipLaunchKernelGGL(far32_records, dim3((nt + 255) / 256), dim3(256), 0, 0, G, f->far32.as<float4>());
             PQ_HIP(hipGetLastError());
-            f->counts.reserve(4 * sizeof(unsigned long long));
+            // [0..3] the list counters, [4] the device fill's non-finite flag: one clear, one copy back
+            f->counts.reserve(5 * sizeof(unsigned long long));
+            PQ_HIP(hipHostMalloc((void **)&f->host_stat, 5 * sizeof(unsigned long long), hipHostMallocDefault));
             f->flag.reserve(sizeof(unsigned));
             PQ_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
             PQ_HIP(hipStreamCreateWithFlags(&f->copy_stream, hipStreamNonBlocking));
@@ -801,6 +803,7 @@ int pq_fill_destroy(pq_fill *f) {
         if (f->copy_stream) (void)hipStreamDestroy(f->copy_stream);
         f->geom.release(); f->far32.release(); f->lists[0].release(); f->lists[1].release();
         f->counts.release(); f->flag.release();
+        if (f->host_stat) (void)hipHostFree(f->host_stat);
         delete f;
     });
 }
@@ -820,19 +823,18 @@ int pq_fill_device(pq_fill *f, double near, int outer, int scalar_only, int prec
         reserve_lists(f, std::max<unsigned long long>(1ull << 20, 64ull * n));
         for (int attempt = 0; attempt < 2; ++attempt) {
             FarArgs A = far_args(f, near, outer, 0, (int)n, 1, plane, ld);
-            PQ_HIP(hipMemsetAsync(f->counts.p, 0, 4 * sizeof(unsigned long long), s));
-            PQ_HIP(hipMemsetAsync(f->flag.p, 0, sizeof(unsigned), s));
+            A.flag = (unsigned *)(f->counts.as<unsigned long long>() + 4);
+            PQ_HIP(hipMemsetAsync(f->counts.p, 0, 5 * sizeof(unsigned long long), s));
             PQ_HIP(hipEventRecord(f->ev[0], s));
             launch_far(f, precision, scalar_only, A, planes, s);
             PQ_HIP(hipEventRecord(f->ev[1], s));
             launch_near(f, precision, scalar_only, 0, 0, A, planes, s);
             if (outer > 0) launch_near(f, precision, scalar_only, 1, outer, A, planes, s);
             PQ_HIP(hipEventRecord(f->ev[2], s));
-            unsigned long long counts[4];
-            unsigned flag = 0;
-            PQ_HIP(hipMemcpyAsync(counts, f->counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
-            PQ_HIP(hipMemcpyAsync(&flag, f->flag.p, sizeof(flag), hipMemcpyDeviceToHost, s));
+            PQ_HIP(hipMemcpyAsync(f->host_stat, f->counts.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             PQ_HIP(hipStreamSynchronize(s));
+            const unsigned long long *counts = f->host_stat;
+            const unsigned flag = (unsigned)counts[4];
             const unsigned long long need = std::max(counts[0], counts[1]);
             if (need > f->cap) {             // lists overflowed: grow them and fill again
                 PQ_REQUIRE(attempt == 0, "pq_fill_device: near lists overflowed twice");

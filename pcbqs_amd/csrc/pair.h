@@ -215,6 +215,7 @@ struct pq_fill {
     unsigned long long cap = 0;
     hipStream_t stream = nullptr, copy_stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    unsigned long long *host_stat = nullptr;   // pinned: the device fill's list counts and flag
     double far_ms = 0, near_ms = 0;
     long long near_pairs[2] = {0, 0};
     int cus = 256;

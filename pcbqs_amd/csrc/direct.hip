@@ -371,16 +371,39 @@ __global__ __launch_bounds__(256) void ldl_update_kernel(float *Fr, float *Fi, l
     Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // matrix blockIdx.z of a batch
     Wr += blockIdx.z * sW; Wi += blockIdx.z * sW;
     const int T = (n - m0 + LB - 1) / LB;
-    int id = blockIdx.x, ti, tj;
+    // Workgroups go to the 8 XCDs in turn (workgroup w to XCD w mod 8), each with its own L2: give
+    // each XCD one contiguous run of tiles, so that the tiles it holds at once share their L21 rows
+    // and W blocks in its L2 instead of every XCD fetching every operand.
+    int id, ti, tj;
+    {
+        const unsigned nw = gridDim.x, q = nw / 8, r = nw % 8, x = blockIdx.x % 8, j = blockIdx.x / 8;
+        id = (int)(x * q + min(x, r) + j);
+    }
     if (part == 1) {
         tj = 0;
         while (id >= T - tj) { id -= T - tj; ++tj; }
         ti = tj + id;
     } else {
-        ti = (int)((sqrtf(8.f * (float)id + 1.f) - 1.f) * 0.5f);
-        while ((ti + 1) * (ti + 2) / 2 <= id) ++ti;
-        while (ti * (ti + 1) / 2 > id) --ti;
-        tj = id - ti * (ti + 1) / 2;
+        // The triangle over ti - tc >= tj - tc in bands of G tile rows, a band's tiles column by
+        // column: consecutive tiles share their W block, and a band's G row blocks of L21 stay in
+        // the L2 while the band's columns pass.
+        constexpr int G = 16;
+        const int Tt = T - tc;
+        int b = (int)((sqrtf(8.f * (float)id + 1.f) - 1.f) * 0.5f) / G;
+        while (b > 0 && (b * G) * (b * G + 1) / 2 > id) --b;
+        while ((b + 1) * G < Tt && ((b + 1) * G) * ((b + 1) * G + 1) / 2 <= id) ++b;
+        const int r0 = b * G, g = min(G, Tt - r0);
+        int o = id - r0 * (r0 + 1) / 2;
+        if (o < r0 * g) {
+            ti = r0 + o % g;
+            tj = o / g;
+        } else {
+            o -= r0 * g;
+            int c = 0;
+            while (o >= g - c) { o -= g - c; ++c; }
+            ti = r0 + c + o;
+            tj = r0 + c;
+        }
         ti += tc;
         tj += tc;
     }

@@ -1006,6 +1006,76 @@ __global__ __launch_bounds__(256) void coarse_restrict_kernel(const T *__restric
     }
 }
 
+// The same restriction partials with four consecutive coarse columns per lane (one 8-byte FP16 or
+// 16-byte FP32 load per row instead of a 2- or 4-byte one), a block over 256 columns and `rows` rows
+// split over its four waves as above. Each sum runs over the same rows in the same order and the
+// four waves' sums are added in wave order, so the partials equal coarse_restrict_kernel's
+// bitwise. The waves' sums meet in LDS one right-hand side at a time and leave as contiguous
+// (column, rhs) rows. NR <= 16.
+template <int NR, class T>
+__global__ __launch_bounds__(256) void coarse_restrict4_kernel(const T *__restrict__ Phi, int k, int kp,
+                                                              const double *__restrict__ V, int n, int rows,
+                                                              double scale, double *__restrict__ partial) {
+    static_assert(NR <= 16, "four-column restriction: at most 16 right-hand sides");
+    constexpr int U = 16;
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __shared__ double red[3][256];
+    __shared__ double outs[256 * NR];
+    const int i0 = blockIdx.x * rows, m = min(rows, n - i0);
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cb = blockIdx.y * 256, c0 = cb + 4 * lane;
+    double acc[4][NR];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[j][r] = 0;
+    // kp is a multiple of 64, so a lane whose first column lies inside the padded row has all four
+    const T *col = Phi + (size_t)i0 * kp + (c0 < kp ? c0 : 0);
+    const double *v = V + (size_t)i0 * NR;
+    int i = w;
+    for (; i + 4 * (U - 1) < m; i += 4 * U) {
+        T4 f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[u] = *reinterpret_cast<const T4 *>(col + (size_t)(i + 4 * u) * kp);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double *vr = v + (size_t)(i + 4 * u) * NR;
+            const double f0 = (float)f[u].x, f1 = (float)f[u].y, f2 = (float)f[u].z, f3 = (float)f[u].w;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double x = vr[r];
+                acc[0][r] += f0 * x; acc[1][r] += f1 * x; acc[2][r] += f2 * x; acc[3][r] += f3 * x;
+            }
+        }
+    }
+    for (; i < m; i += 4) {
+        const T4 f = *reinterpret_cast<const T4 *>(col + (size_t)i * kp);
+        const double *vr = v + (size_t)i * NR;
+        const double f0 = (float)f.x, f1 = (float)f.y, f2 = (float)f.z, f3 = (float)f.w;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double x = vr[r];
+            acc[0][r] += f0 * x; acc[1][r] += f1 * x; acc[2][r] += f2 * x; acc[3][r] += f3 * x;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (w > 0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[w - 1][4 * lane + j] = acc[j][r] * scale;
+        __syncthreads();
+        if (w == 0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                outs[(4 * lane + j) * NR + r] =
+                    acc[j][r] * scale + red[0][4 * lane + j] + red[1][4 * lane + j] + red[2][4 * lane + j];
+        __syncthreads();
+    }
+    const int ncol = min(256, k - cb);
+    for (int e = threadIdx.x; e < ncol * NR; e += 256)
+        partial[((size_t)blockIdx.x * k + cb) * NR + e] = outs[e];
+}
+
 // t[e] = sum over the row-chunk partials, in chunk order (e = c * NR + r).
 __global__ void coarse_sum_kernel(const double *partial, int nblk, int total, double *t) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1926,6 +1996,15 @@ int prolong_depth(const pq_loops *L, int NR) {
 template <int N> int restrict_launch(const pq_loops *L, const float *Phi, int k, const double *X, int rows,
                                      double *part, bool half = false) {
     const int n = L->nf, nchunk = (n + rows - 1) / rows;
+    if constexpr (N <= 16) {
+        if (half)
+            hipLaunchKernelGGL((coarse_restrict4_kernel<N, _Float16>), dim3(nchunk, (k + 255) / 256), dim3(256), 0,
+                               L->stream, L->Phih.as<_Float16>(), k, L->kp, X, n, rows, L->hscale[0], part);
+        else
+            hipLaunchKernelGGL((coarse_restrict4_kernel<N, float>), dim3(nchunk, (k + 255) / 256), dim3(256), 0,
+                               L->stream, Phi, k, L->kp, X, n, rows, 1.0, part);
+        return nchunk;
+    }
     if (half)
         hipLaunchKernelGGL((coarse_restrict_kernel<N, _Float16>), dim3(nchunk, (k + 63) / 64), dim3(256), 0, L->stream,
                            L->Phih.as<_Float16>(), k, L->kp, X, n, rows, L->hscale[0], part);

@@ -625,6 +625,294 @@ __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restri
     }
 }
 
+// ---- super-block sweeps: SB blocks of 64 rows per sequential launch ----
+//
+// The per-block sweeps above are a chain of one launch per 64-row block, each paying a dependent
+// launch's fixed cost. Here the sweeps go by super-blocks of SB = 8 blocks: one workgroup per
+// system solves a super-block's eight diagonal blocks in turn (each block's product with its
+// stored inverse, then its update of the super-block's later rows, one barrier apart), and one
+// parallel launch then applies the whole super-block to every row beyond it. That pays for a
+// batch of many small systems (a workgroup each); one large system is faster with the per-block
+// sweeps, whose launches spread each block's columns over many workgroups. Every sum runs in a
+// fixed order, so batched and single systems give the same bits.
+constexpr int SB = 8;
+
+// Forward, super-block rows s0 .. s0 + 64 nbk: for each block j, y_j = X_j b_j (to Ys and LDS),
+// then b_r -= L[r, block j] y_j for the super-block's rows r after block j (a thread per row, the
+// block's 64 columns in order).
+template <int P>
+__global__ __launch_bounds__(256) void ldl_fwd_super_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
+                                                           long long ldf, const float *__restrict__ Xr,
+                                                           const float *__restrict__ Xi, int n, int s0, int nbk,
+                                                           double *Y, double *Ys, long long sF, long long sX,
+                                                           long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    Y += blockIdx.z * sV; Ys += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ float xs[2][LB][LB + 1];
+    __shared__ double bs[LB][NR], ys[LB][NR];
+    const int tid = threadIdx.x;
+    const int rend = min(s0 + LB * nbk, n);
+    for (int j = 0; j < nbk; ++j) {
+        const int k0 = s0 + LB * j, bk = min(LB, n - k0), kb = k0 / LB;
+        const float *xr = Xr + (size_t)kb * LB * LB, *xi = Xi + (size_t)kb * LB * LB;
+        for (int e = tid; e < LB * LB; e += 256) {
+            xs[0][e >> 6][e & 63] = xr[e];
+            xs[1][e >> 6][e & 63] = xi[e];
+        }
+        for (int e = tid; e < LB * NR; e += 256) {
+            const int r = e / NR, q = e % NR;
+            bs[r][q] = r < bk ? Y[(size_t)(k0 + r) * NR + q] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < LB * P; e += 256) {
+            const int i = e / P, p = e % P;
+            double sr = 0, si = 0;
+#pragma unroll 16
+            for (int t = 0; t < LB; ++t) {            // X_j is stored with its zeros above the diagonal
+                const double lr = xs[0][i][t], li = xs[1][i][t];
+                sr += lr * bs[t][p] - li * bs[t][P + p];
+                si += lr * bs[t][P + p] + li * bs[t][p];
+            }
+            ys[i][p] = sr;
+            ys[i][P + p] = si;
+            if (i < bk) {
+                Ys[(size_t)(k0 + i) * NR + p] = sr;
+                Ys[(size_t)(k0 + i) * NR + P + p] = si;
+            }
+        }
+        __syncthreads();
+        for (int r = k0 + bk + tid; r < rend; r += 256) {   // rows after a block exist only when bk = 64
+            double ar[P], ai[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+            const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)r * ldf + k0);
+            const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)r * ldf + k0);
+            // the row's 64 entries in four quarters, each quarter's loads issued together
+#pragma unroll 1
+            for (int h = 0; h < 4; ++h) {
+                float4 a[4], b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { a[q] = lr4[4 * h + q]; b[q] = li4[4 * h + q]; }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float lre[4] = {a[q].x, a[q].y, a[q].z, a[q].w}, lim[4] = {b[q].x, b[q].y, b[q].z, b[q].w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int t = 16 * h + 4 * q + u;
+                        const double xr2 = lre[u], xi2 = lim[u];
+#pragma unroll
+                        for (int p = 0; p < P; ++p) {
+                            ar[p] += xr2 * ys[t][p] - xi2 * ys[t][P + p];
+                            ai[p] += xr2 * ys[t][P + p] + xi2 * ys[t][p];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                Y[(size_t)r * NR + p] -= ar[p];
+                Y[(size_t)r * NR + P + p] -= ai[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Forward, rows r >= r0 (= s0 + 64 nbk) after super-block s0: b_r -= sum over the super-block's
+// columns of L[r, t] y_t. A workgroup takes 64 rows (lane = row); in each of the super-block's
+// blocks its four waves take the four 16-column quarters; the waves' sums meet in LDS and wave 0
+// subtracts them in wave order.
+template <int P>
+__global__ __launch_bounds__(256) void ldl_fwd_bulk_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
+                                                          long long ldf, int n, int s0, int nbk, double *Y,
+                                                          const double *__restrict__ Ys, long long sF, long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;
+    Y += blockIdx.z * sV; Ys += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ double part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = s0 + LB * nbk + blockIdx.x * LB + lane;
+    const int rr = min(r, n - 1);                     // rows past n read the last row, their sums unused
+    double ar[P], ai[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+#pragma unroll 2
+    for (int j = 0; j < nbk; ++j) {
+        const int k0 = s0 + LB * j;                   // every block before the rows below is full
+        float lre[16], lim[16];
+        const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)rr * ldf + k0 + 16 * wv);
+        const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)rr * ldf + k0 + 16 * wv);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) {
+            const float4 a = lr4[t4], b = li4[t4];
+            lre[4 * t4] = a.x; lre[4 * t4 + 1] = a.y; lre[4 * t4 + 2] = a.z; lre[4 * t4 + 3] = a.w;
+            lim[4 * t4] = b.x; lim[4 * t4 + 1] = b.y; lim[4 * t4 + 2] = b.z; lim[4 * t4 + 3] = b.w;
+        }
+        const double *yb = Ys + (size_t)(k0 + 16 * wv) * NR;   // wave-uniform rows of y
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double xr = lre[u], xi = lim[u];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const double yr = yb[u * NR + p], yi = yb[u * NR + P + p];
+                ar[p] += xr * yr - xi * yi;
+                ai[p] += xr * yi + xi * yr;
+            }
+        }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+    __syncthreads();
+    if (wv == 0 && r < n) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            Y[(size_t)r * NR + p] -= ar[p];
+            Y[(size_t)r * NR + P + p] -= ai[p];
+        }
+    }
+}
+
+// Backward, super-block rows s0 .. s0 + 64 nbk, last block first: x_j = X_j^T z_j (to Xo and LDS),
+// then z_c -= sum_t L[k0 + t][c] x_t for the super-block's columns c before block j (a thread per
+// column, the block's rows in order).
+template <int P>
+__global__ __launch_bounds__(256) void ldl_bwd_super_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
+                                                           long long ldf, const float *__restrict__ Xr,
+                                                           const float *__restrict__ Xi, int n, int s0, int nbk,
+                                                           double *Z, double *Xo, long long sF, long long sX,
+                                                           long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ float xsm[2][LB][LB + 1];
+    __shared__ double zs[LB][NR], xs[LB][NR];
+    const int tid = threadIdx.x;
+    for (int j = nbk - 1; j >= 0; --j) {
+        const int k0 = s0 + LB * j, bk = min(LB, n - k0), kb = k0 / LB;
+        const float *xr = Xr + (size_t)kb * LB * LB, *xi = Xi + (size_t)kb * LB * LB;
+        for (int e = tid; e < LB * LB; e += 256) {
+            xsm[0][e >> 6][e & 63] = xr[e];
+            xsm[1][e >> 6][e & 63] = xi[e];
+        }
+        for (int e = tid; e < LB * NR; e += 256) {
+            const int r = e / NR, q = e % NR;
+            zs[r][q] = r < bk ? Z[(size_t)(k0 + r) * NR + q] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < LB * P; e += 256) {
+            const int c = e % LB, p = e / LB;
+            double sr = 0, si = 0;
+#pragma unroll 16
+            for (int t = 0; t < LB; ++t) {            // zeros above X_j's diagonal and past bk in zs
+                const double lr = xsm[0][t][c], li = xsm[1][t][c];
+                sr += lr * zs[t][p] - li * zs[t][P + p];
+                si += lr * zs[t][P + p] + li * zs[t][p];
+            }
+            xs[c][p] = sr;
+            xs[c][P + p] = si;
+            if (c < bk) {
+                Xo[(size_t)(k0 + c) * NR + p] = sr;
+                Xo[(size_t)(k0 + c) * NR + P + p] = si;
+            }
+        }
+        __syncthreads();
+        for (int c = s0 + tid; c < k0; c += 256) {
+            double ar[P], ai[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+            for (int t0 = 0; t0 < bk; t0 += 16) {
+                float lre[16], lim[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const bool in = t0 + u < bk;
+                    lre[u] = in ? Fr[(size_t)(k0 + t0 + u) * ldf + c] : 0.f;
+                    lim[u] = in ? Fi[(size_t)(k0 + t0 + u) * ldf + c] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int t = t0 + u;
+                    const double lr = lre[u], li = lim[u];
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        ar[p] += lr * xs[t][p] - li * xs[t][P + p];
+                        ai[p] += lr * xs[t][P + p] + li * xs[t][p];
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                Z[(size_t)c * NR + p] -= ar[p];
+                Z[(size_t)c * NR + P + p] -= ai[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Backward, columns r < s0 before super-block s0: z_r -= sum over the super-block's rows t of
+// L[t][r] x_t. A workgroup takes 64 columns (lane = r, reads across lanes); in each block its four
+// waves take the four 16-row quarters; the waves' sums meet in LDS and wave 0 subtracts them.
+template <int P>
+__global__ __launch_bounds__(256) void ldl_bwd_bulk_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
+                                                          long long ldf, int n, int s0, int nbk, double *Z,
+                                                          const double *__restrict__ Xo, long long sF, long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;
+    Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ double part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = blockIdx.x * LB + lane;             // r < s0 always: s0 is a multiple of 64
+    double ar[P], ai[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+#pragma unroll 2
+    for (int j = 0; j < nbk; ++j) {
+        const int k0 = s0 + LB * j, bk = min(LB, n - k0), t0 = 16 * wv;
+        float lre[16], lim[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const bool in = t0 + u < bk;
+            lre[u] = in ? Fr[(size_t)(k0 + t0 + u) * ldf + r] : 0.f;
+            lim[u] = in ? Fi[(size_t)(k0 + t0 + u) * ldf + r] : 0.f;
+        }
+        const double *xb = Xo + (size_t)(k0 + t0) * NR;  // wave-uniform rows of x
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            if (t0 + u >= bk) break;                   // wave-uniform
+            const double xr = lre[u], xi = lim[u];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const double vr = xb[u * NR + p], vi = xb[u * NR + P + p];
+                ar[p] += xr * vr - xi * vi;
+                ai[p] += xr * vi + xi * vr;
+            }
+        }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            Z[(size_t)r * NR + p] -= ar[p];
+            Z[(size_t)r * NR + P + p] -= ai[p];
+        }
+    }
+}
+
 // ---- the dense symmetric positive definite solve (real A FP64 on the device) ----
 
 // F = A in FP32 (lower triangle), imaginary plane zero.
@@ -840,24 +1128,53 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
     const int n = f.n, nb = (n + LB - 1) / LB;
     if (n <= 0) return;
     const unsigned nz = (unsigned)std::max(1, f.batch);
+    // Super-block sweeps for systems up to 4,096 unknowns (the many small systems of a multi-board
+    // batch: a sequential workgroup per system, 2 launches per 512 rows), per-block sweeps above (a
+    // large system's per-block launches spread each block's columns over many workgroups, which the
+    // one sequential workgroup cannot match). PCBQS_AMD_LDL_BLOCK_SWEEPS=1: per-block sweeps always.
+    static const bool block_env = std::getenv("PCBQS_AMD_LDL_BLOCK_SWEEPS") != nullptr;
+    const bool block_sweeps = block_env || n > 4096;
     const long long sV = (long long)n * 2 * P;           // the systems' vectors, one after another
     const size_t bytes = (size_t)n * 2 * P * 8 * nz;
     PQ_HIP(hipMemcpyAsync(Y, B, bytes, hipMemcpyDeviceToDevice, st));
     dispatch_ports(P, [&](auto pc) {
         constexpr int PP = decltype(pc)::value;
-        for (int kb = 0; kb < nb; ++kb) {
-            const int k0 = kb * LB, bk = std::min(LB, n - k0);
-            const int rows = n - k0 - bk, grid = std::max(1, (rows + LB - 1) / LB);
-            hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
-                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys, f.sF, f.sX,
-                               sV);
+        if (block_sweeps) {                              // one launch per 64-row block
+            for (int kb = 0; kb < nb; ++kb) {
+                const int k0 = kb * LB, bk = std::min(LB, n - k0);
+                const int rows = n - k0 - bk, grid = std::max(1, (rows + LB - 1) / LB);
+                hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                                   f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys, f.sF,
+                                   f.sX, sV);
+            }
+        } else {
+            for (int kb0 = 0; kb0 < nb; kb0 += SB) {
+                const int s0 = kb0 * LB, nbk = std::min(SB, nb - kb0), below = n - s0 - nbk * LB;
+                hipLaunchKernelGGL(ldl_fwd_super_kernel<PP>, dim3(1, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                                   f.Lir, f.Lii, n, s0, nbk, Y, Ys, f.sF, f.sX, sV);
+                if (below > 0)
+                    hipLaunchKernelGGL(ldl_fwd_bulk_kernel<PP>, dim3((below + LB - 1) / LB, 1, nz), dim3(256), 0, st,
+                                       f.Fr, f.Fi, f.ldf, n, s0, nbk, Y, Ys, f.sF, sV);
+            }
         }
         hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256), 1, nz), dim3(256), 0, st,
                            f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV);
-        for (int kb = nb - 1; kb >= 0; --kb) {
-            const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + LB - 1) / LB);
-            hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
-                               f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X, f.sF, f.sX, sV);
+        if (block_sweeps) {
+            for (int kb = nb - 1; kb >= 0; --kb) {
+                const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + LB - 1) / LB);
+                hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                                   f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X, f.sF,
+                                   f.sX, sV);
+            }
+        } else {
+            for (int kb0 = (nb - 1) / SB * SB; kb0 >= 0; kb0 -= SB) {
+                const int s0 = kb0 * LB, nbk = std::min(SB, nb - kb0);
+                hipLaunchKernelGGL(ldl_bwd_super_kernel<PP>, dim3(1, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
+                                   f.Lir, f.Lii, n, s0, nbk, Y, X, f.sF, f.sX, sV);
+                if (s0 > 0)
+                    hipLaunchKernelGGL(ldl_bwd_bulk_kernel<PP>, dim3(s0 / LB, 1, nz), dim3(256), 0, st, f.Fr, f.Fi,
+                                       f.ldf, n, s0, nbk, Y, X, f.sF, sV);
+            }
         }
     });
     PQ_HIP(hipGetLastError());

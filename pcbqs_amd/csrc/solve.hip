@@ -25,6 +25,7 @@
 #include <dlfcn.h>
 #include <limits>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -1854,12 +1855,28 @@ template <int N> void operator_launch(const pq_loops *L, OpShape sh, const doubl
 // candidate runs once to warm up and three times under HIP events on an input of zeros (the whole
 // product: partials, resistive part and epilogue), each time after a cache-evicting fill; the
 // fastest is kept for this handle and column count.
+// Autotuned launch shapes are shared by every handle of the process with the same device and
+// problem shape: the first handle to need one tunes it (under the lock) and later handles take its
+// choice, so that two solvers of one board in one process sum in the same order and give the same
+// bits (the chunk and row-block choices set the order of the partial sums).
+std::mutex tune_mutex;
+std::map<std::tuple<int, int, int, int>, std::tuple<int, int, float>> gemv_cache;        // (dev, n, NR, mode)
+std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>> coarse_cache;  // (dev, n, k, NR, half)
+
 void tune_gemv(pq_loops *L, int NR) {
     const int n = L->nf;
     const int mode = operator_mode();
     if (L->gemv_chunk.count(NR) || n == 0) return;
     // small systems keep the default shape unless the symmetric kernel is pinned
     if (n < 1024 && !(mode == 2 && NR <= 16)) return;
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    const auto key = std::make_tuple(L->device, n, NR, mode);
+    if (auto hit = gemv_cache.find(key); hit != gemv_cache.end()) {
+        L->gemv_chunk[NR] = std::get<0>(hit->second);
+        L->gemv_rows[NR] = std::get<1>(hit->second);
+        L->gemv_us[NR] = std::get<2>(hit->second);
+        return;
+    }
     std::vector<OpShape> cand;
     if (NR <= 16 && mode != 1) cand.push_back({0, SYM_T});
     if (mode != 2 || cand.empty()) {
@@ -1917,6 +1934,7 @@ void tune_gemv(pq_loops *L, int NR) {
     L->gemv_chunk[NR] = best.chunk;
     L->gemv_rows[NR] = best.rows;
     L->gemv_us[NR] = best_us;
+    gemv_cache[key] = std::make_tuple(best.chunk, best.rows, best_us);
 }
 
 void apply_operator(pq_loops *L, const double *X, double *W, int P) {
@@ -2050,6 +2068,14 @@ template <int N> int prolong_launch(const pq_loops *L, int ps, int depth, const 
 void tune_coarse(pq_loops *L, int NR) {
     const int n = L->nf, k = L->kc;
     if (k == 0 || n < 1024 || L->crows.count(NR)) return;
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    const auto key = std::make_tuple(L->device, n, k, NR, (int)L->coarse_half);
+    if (auto hit = coarse_cache.find(key); hit != coarse_cache.end()) {
+        L->crows[NR] = std::get<0>(hit->second);
+        L->psplit[NR] = std::get<1>(hit->second);
+        L->pdepth[NR] = std::get<2>(hit->second);
+        return;
+    }
     const std::vector<int> rows_c = {128, 256, 512, 1024};
     std::vector<int> split_c;
     for (int ps : {1, 2, 3, 4, 6, 8, 12, 16})
@@ -2116,6 +2142,7 @@ void tune_coarse(pq_loops *L, int NR) {
     L->crows[NR] = best_rows;
     L->psplit[NR] = best_split;
     L->pdepth[NR] = best_depth;
+    coarse_cache[key] = std::make_tuple(best_rows, best_split, best_depth);
 }
 
 // c = Ac^-1 Phi^T X, then T1 = X - A Phi c and T2 = Phi c.

@@ -186,9 +186,13 @@ typedef struct pq_csym pq_csym;     /* a factored complex-symmetric matrix      
  * entries (rows >= cols, unique; vals interleaved complex FP64) on the device:
  * FP32 L D L^T without pivoting (the real part positive definite: a graph
  * Laplacian A D^-1 A^T with Re 1/d > 0), keeping the entries in FP64 for
- * refinement. *factor_ms: device time of forming and factoring; *bytes: the
- * factor's storage. A zero pivot returns PQ_EINVAL. Replaces the per-frequency
- * splu of the graph preconditioner's Laplacian (src/pcbqs/mqs_precond.py:50-51). */
+ * refinement. When every entry lies within a band (row - col <= b with
+ * b + 128 < n: the Laplacian in a bandwidth-reducing order), the factorisation
+ * and the triangular sweeps work inside the band only (O(n b^2) work; the
+ * storage stays n x n). *factor_ms: device time of forming and factoring;
+ * *bytes: the factor's storage. A zero pivot returns PQ_EINVAL. Replaces the
+ * per-frequency splu of the graph preconditioner's Laplacian
+ * (src/pcbqs/mqs_precond.py:50-51). */
 int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const int32_t *cols, const double *vals,
                    pq_csym **out, double *factor_ms, int64_t *bytes);
 /* X = A^-1 B, B and X (n x nrhs) interleaved complex FP64 host arrays: FP32

@@ -543,7 +543,8 @@ template <int P>
 __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restrict__ Fr, const float *__restrict__ Fi,
                                                           long long ldf, const float *__restrict__ Xr,
                                                           const float *__restrict__ Xi, int k0, int bk, double *Z,
-                                                          double *Xo, long long sF, long long sX, long long sV) {
+                                                          double *Xo, long long sF, long long sX, long long sV,
+                                                          int r0) {
     Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
     Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
     Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
@@ -552,8 +553,8 @@ __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restri
     __shared__ double zs[LB][NR], xs[LB][NR], part[3][LB][NR];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // this thread's 16 entries of column r of the block's rows and (wave 0) the row of Z, loaded
-    // first (they do not depend on x_k)
-    const int r = blockIdx.x * LB + lane, t0 = 16 * wv, t1 = min(bk, t0 + 16);
+    // first (they do not depend on x_k); columns from r0 (0, or the band's first below a band)
+    const int r = r0 + blockIdx.x * LB + lane, t0 = 16 * wv, t1 = min(bk, t0 + 16);
     float lre[16], lim[16];
     double zr[P], zi[P];
     if (r < k0) {
@@ -1078,9 +1079,13 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
         hipLaunchKernelGGL(ldl_diag_kernel, dim3(1, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, k, b,
                            f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, bad, f.sF, f.sX);
     };
+    // under a band, rows past the last one that the columns just factored reach are zero in L21 and
+    // W: the kernels take that row as their end
+    auto reach = [&](int last_col) { return f.band > 0 ? std::min(n, last_col + 1 + f.band) : n; };
     auto panel = [&](int k, float *wr, float *wi) {
-        const int tiles = (n - k - LB + LB - 1) / LB;
-        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, k,
+        const int ne = reach(k + LB - 1), tiles = (ne - k - LB + LB - 1) / LB;
+        if (tiles <= 0) return;
+        hipLaunchKernelGGL(ldl_panel_kernel, dim3(tiles, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, ne, k,
                            f.Lir + (size_t)(k / LB) * LB * LB, f.Lii + (size_t)(k / LB) * LB * LB, wr, wi, LDW, f.sF,
                            f.sX, f.sW);
     };
@@ -1091,21 +1096,29 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
         float *wr = f.Wr[s & 1], *wi = f.Wi[s & 1];
         panel(K, wr, wi);
         {   // the first panel's update of the second panel's columns (rows from K + 64)
-            const int T = (n - K - LB + LB - 1) / LB;
-            hipLaunchKernelGGL(ldl_update_kernel, dim3(T, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, LB, K + LB,
-                               wr, wi, LDW, 1, 1, f.sF, f.sW);
+            const int ne = reach(K + LB - 1), T = (ne - K - LB + LB - 1) / LB;
+            if (T > 0)
+                hipLaunchKernelGGL(ldl_update_kernel, dim3(T, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, ne, K, LB,
+                                   K + LB, wr, wi, LDW, 1, 1, f.sF, f.sW);
         }
         diag(K + LB);
         if (K + 2 * LB >= n) break;
         panel(K + LB, wr + LB, wi + LB);
-        const int m0 = K + 2 * LB, T = (n - m0 + LB - 1) / LB, tc = std::min(2, T);
+        const int ne = reach(K + 2 * LB - 1);
+        const int m0 = K + 2 * LB, T = (ne - m0 + LB - 1) / LB, tc = std::min(2, T);
+        if (T <= 0) {                                    // a band ending inside the super-panel
+            if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
+            bulk = false;
+            diag(m0);
+            continue;
+        }
         if (bulk) PQ_HIP(hipStreamWaitEvent(st, f.done, 0));
         hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)(tc * T - tc * (tc - 1) / 2), 1, nz), dim3(256), 0, st,
-                           f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 1, tc, f.sF, f.sW);
+                           f.Fr, f.Fi, f.ldf, ne, K, 2 * LB, m0, wr, wi, LDW, 1, tc, f.sF, f.sW);
         if (!f.aux) {                                    // one stream (a captured batch): the bulk in order
             if (T > tc)
                 hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2), 1, nz),
-                                   dim3(256), 0, st, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF,
+                                   dim3(256), 0, st, f.Fr, f.Fi, f.ldf, ne, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF,
                                    f.sW);
             diag(m0);
             continue;
@@ -1115,7 +1128,7 @@ void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad) {
             PQ_HIP(hipEventRecord(f.ready, st));
             PQ_HIP(hipStreamWaitEvent(f.aux, f.ready, 0));
             hipLaunchKernelGGL(ldl_update_kernel, dim3((unsigned)((size_t)(T - tc) * (T - tc + 1) / 2), 1, nz), dim3(256),
-                               0, f.aux, f.Fr, f.Fi, f.ldf, n, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF, f.sW);
+                               0, f.aux, f.Fr, f.Fi, f.ldf, ne, K, 2 * LB, m0, wr, wi, LDW, 2, tc, f.sF, f.sW);
             PQ_HIP(hipEventRecord(f.done, f.aux));
         }
         diag(m0);
@@ -1133,7 +1146,7 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
     // large system's per-block launches spread each block's columns over many workgroups, which the
     // one sequential workgroup cannot match). PCBQS_AMD_LDL_BLOCK_SWEEPS=1: per-block sweeps always.
     static const bool block_env = std::getenv("PCBQS_AMD_LDL_BLOCK_SWEEPS") != nullptr;
-    const bool block_sweeps = block_env || n > 4096;
+    const bool block_sweeps = block_env || n > 4096 || f.band > 0;
     const long long sV = (long long)n * 2 * P;           // the systems' vectors, one after another
     const size_t bytes = (size_t)n * 2 * P * 8 * nz;
     PQ_HIP(hipMemcpyAsync(Y, B, bytes, hipMemcpyDeviceToDevice, st));
@@ -1142,7 +1155,8 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
         if (block_sweeps) {                              // one launch per 64-row block
             for (int kb = 0; kb < nb; ++kb) {
                 const int k0 = kb * LB, bk = std::min(LB, n - k0);
-                const int rows = n - k0 - bk, grid = std::max(1, (rows + LB - 1) / LB);
+                const int rows = f.band > 0 ? std::min(n - k0 - bk, f.band) : n - k0 - bk;
+                const int grid = std::max(1, (rows + LB - 1) / LB);
                 hipLaunchKernelGGL(ldl_forward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
                                    f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, n, k0, bk, Y, Ys, f.sF,
                                    f.sX, sV);
@@ -1161,10 +1175,13 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
                            f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV);
         if (block_sweeps) {
             for (int kb = nb - 1; kb >= 0; --kb) {
-                const int k0 = kb * LB, bk = std::min(LB, n - k0), grid = std::max(1, (k0 + LB - 1) / LB);
+                const int k0 = kb * LB, bk = std::min(LB, n - k0);
+                // columns r < k0 that the block's rows reach: all, or from k0 - band down a band
+                const int r0 = f.band > 0 ? std::max(0, (k0 - f.band) / LB * LB) : 0;
+                const int grid = std::max(1, (k0 - r0 + LB - 1) / LB);
                 hipLaunchKernelGGL(ldl_backward_kernel<PP>, dim3(grid, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf,
                                    f.Lir + (size_t)kb * LB * LB, f.Lii + (size_t)kb * LB * LB, k0, bk, Y, X, f.sF,
-                                   f.sX, sV);
+                                   f.sX, sV, r0);
             }
         } else {
             for (int kb0 = (nb - 1) / SB * SB; kb0 >= 0; kb0 -= SB) {
@@ -1345,9 +1362,11 @@ int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const in
         PQ_REQUIRE(n >= 1 && nnz >= 0, "pq_csym_create: invalid sizes");
         std::vector<long long> key((size_t)nnz);
         std::vector<int> cnt((size_t)n + 1, 0);
+        int band = 0;                                     // the largest row - column of an entry
         for (int64_t t = 0; t < nnz; ++t) {
             PQ_REQUIRE(rows[t] >= 0 && rows[t] < n && cols[t] >= 0 && cols[t] <= rows[t],
                        "pq_csym_create: entries must lie in the lower triangle of an n x n matrix");
+            band = std::max(band, rows[t] - cols[t]);
             PQ_REQUIRE(std::isfinite(vals[2 * t]) && std::isfinite(vals[2 * t + 1]), "pq_csym_create: entry not finite");
             key[(size_t)t] = (long long)rows[t] * n + cols[t];
             ++cnt[(size_t)rows[t] + 1];
@@ -1388,6 +1407,9 @@ int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const in
             h->f = LdlFactor{h->F.as<float>(), h->F.as<float>() + (size_t)n * ldf, ldf, n, h->Xi.as<float>(),
                              h->Xi.as<float>() + (size_t)nblk * LB * LB, {wb, wb + 2 * wsz}, {wb + wsz, wb + 3 * wsz},
                              h->aux, h->ready, h->done};
+            // a banded matrix (the graph Laplacian in a bandwidth-reducing order) factors and sweeps
+            // within its band; the dense kernels otherwise
+            if (band + 2 * LB < n) h->f.band = std::max(band, 1);
             h->rp.reserve(cnt.size() * 4);
             h->ci.reserve(ci.size() * 4);
             h->av.reserve(av.size() * 8);

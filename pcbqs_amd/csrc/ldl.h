@@ -25,6 +25,9 @@ struct LdlFactor {
     // planes at Fr/Fi + z sF, its block inverses at Lir/Lii + z sX, its W buffers at Wr/Wi + z sW
     int batch = 1;
     long long sF = 0, sX = 0, sW = 0;
+    // band > 0: every entry lies within `band` of the diagonal (|i - j| <= band), so the factor does
+    // too, and the factorisation and the sweeps skip the tiles beyond it (storage stays dense)
+    int band = 0;
 };
 
 // The factorisation's second stream (the bulk trailing updates): a plain non-blocking stream. With

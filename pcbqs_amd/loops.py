@@ -63,16 +63,17 @@ def sheet_resistance(rwg, t_of, sigma):
     tri, edge, alpha, vert = tri[order], edge[order], alpha[order], vert[order]
     starts = np.r_[0, np.flatnonzero(np.diff(tri)) + 1]
     counts = np.diff(np.r_[starts, len(tri)])
+    Dq = Pq[tri] - vert[:, None]                                            # (terms, 3, 2): P_q - v
+    wt = rwg.area[tri] / 3.0 / (sigma * t_of[tri])
+    lt = rwg.l[edge]
     rows, cols, vals = [], [], []
     for i in range(3):
         for j in range(3):
             sel = counts > max(i, j)
             a, b = starts[sel] + i, starts[sel] + j
-            t = tri[a]
-            w = rwg.area[t] / 3.0 / (sigma * t_of[t])
-            d = ((Pq[t] - vert[a][:, None]) * (Pq[t] - vert[b][:, None])).sum((1, 2))
+            d = (Dq[a] * Dq[b]).sum((1, 2))
             rows.append(edge[a]); cols.append(edge[b])
-            vals.append(alpha[a] * alpha[b] * w * d / (rwg.l[edge[a]] * rwg.l[edge[b]]))
+            vals.append(alpha[a] * alpha[b] * wt[a] * d / (lt[a] * lt[b]))
     return sparse.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                              shape=(rwg.Ne, rwg.Ne)).tocsr()
 
@@ -115,8 +116,8 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
     # one branch per node pair carries the tree edge (the lowest-numbered)
     lo, hi = np.minimum(a[ks], b[ks]), np.maximum(a[ks], b[ks])
     key = lo * n_nodes + hi
-    first = np.unique(key, return_index=True)[1]
-    pair_branch = dict(zip(key[first].tolist(), ks[first].tolist()))
+    ukey, first = np.unique(key, return_index=True)          # node pairs ascending, each one's first branch
+    pair_branch = ks[first]
     adj = sparse.coo_matrix((np.ones(len(first)), (lo[first], hi[first])), shape=(n_nodes, n_nodes)).tocsr()
     adj = (adj + adj.T).tocsr()
     ncomp, label = csgraph.connected_components(adj, directed=False)
@@ -134,7 +135,7 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
     child = np.flatnonzero(has)
     pk = np.minimum(child, parent[child]) * n_nodes + np.maximum(child, parent[child])
     upb = np.full(n_nodes, -1)
-    upb[child] = [pair_branch[k] for k in pk.tolist()]
+    upb[child] = pair_branch[np.searchsorted(ukey, pk)]        # every tree edge is a copper node pair
     tree[upb[child]] = True
     # parts (non-source branches beyond the copper) join islands the copper leaves apart
     comp = label.copy()
@@ -153,7 +154,7 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
                 tree[k] = True
                 # re-root b's island under a through branch k
                 _attach(parent, upb, a[k], b[k], k, adj)
-    comp = np.array([find(x) for x in comp])
+    comp = np.array([find(x) for x in range(ncomp)], dtype=np.int64)[label] if n_nodes else comp
     # depth and orientation of the parent branch
     depth = np.full(n_nodes, -1)
     roots = np.flatnonzero(parent < 0)
@@ -176,24 +177,22 @@ def loop_basis(n_nodes, ends, is_src, n_copper, n_rwg, branch_pos, ideal_termina
     nl = len(cot)
     col = np.arange(nl)
     rows, cols, vals = [cot], [col], [np.ones(nl)]
-    u, v = b[cot].copy(), a[cot].copy()
-    active = np.ones(nl, bool)
-    while True:
+    # climb both ends to the common ancestor, the deeper end (both at equal depth) one step at a
+    # time; loops leave the working set once their ends meet
+    u, v = b[cot], a[cot]
+    open_ = u != v
+    idx, u, v = col[open_], u[open_], v[open_]
+    while len(idx):
         du, dv = depth[u], depth[v]
-        go_u = active & (du > dv)
-        go_v = active & (dv > du)
-        both = active & (du == dv) & (u != v)
-        mu = go_u | both
-        mv = go_v | both
-        if not (mu.any() or mv.any()):
-            break
-        iu, iv = np.flatnonzero(mu), np.flatnonzero(mv)
-        rows += [upb[u[iu]], upb[v[iv]]]
-        cols += [col[iu], col[iv]]
-        vals += [-ups[u[iu]].astype(float), ups[v[iv]].astype(float)]
-        u[iu] = parent[u[iu]]
-        v[iv] = parent[v[iv]]
-        active = u != v
+        mu, mv = du >= dv, dv >= du
+        uu, vv = u[mu], v[mv]
+        rows += [upb[uu], upb[vv]]
+        cols += [idx[mu], idx[mv]]
+        vals += [-ups[uu].astype(float), ups[vv].astype(float)]
+        u[mu] = parent[uu]
+        v[mv] = parent[vv]
+        open_ = u != v
+        idx, u, v = idx[open_], u[open_], v[open_]
     rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
     C = sparse.coo_matrix((vals, (rows, cols)), shape=(nb, nl)).tocsc()
     C.sum_duplicates()

@@ -719,44 +719,60 @@ __global__ void bj_scale_kernel(const float2 *Vc, const double *lam, size_t coun
     A[e] = make_float2((float)(v / den), (float)(-v * wl / den));
 }
 
-// Y_b = Zinv_b X_b (+ A_b when an addend A is given) per block (complex), NR = 2P. A workgroup
-// takes bj_rows(NR) rows of block blockIdx.x (from row blockIdx.y * bj_rows); four adjacent lanes
-// share a (row, port) output and take the columns j = s, s + 4, ... (32 contiguous bytes of the
-// inverse's row per four lanes), combined by two fixed shuffles.
-constexpr int bj_rows(int NR) { return 128 / NR < BS ? 128 / NR : BS; }
+// Y = Z_b^-1 X (+ A) on every 256-loop block (Z_b^-1 FP32 complex), sixteen lanes per row: lane s of a row reads the
+// row's columns s, s + 16, ... (a wave's load is four rows' 128 contiguous bytes, no lane repeating
+// another's), accumulates every right-hand side against X staged in LDS by right-hand side (conflict-
+// free across the sixteen columns), and the sixteen partial sums meet by a fixed butterfly. A
+// workgroup takes 16 rows of block blockIdx.x from row 16 blockIdx.y.
 template <int NR>
-__global__ __launch_bounds__(256) void bj_apply_kernel(const float *inv, int n, const double *X, const double *A,
-                                                      double *Y) {
-    constexpr int P = NR / 2, R = bj_rows(NR);
-    __shared__ double xs[BS * NR];
+__global__ __launch_bounds__(256) void bj_apply16_kernel(const float *inv, int n, const double *X, const double *A,
+                                                        double *Y) {
+    constexpr int P = NR / 2;
+    __shared__ double xs[NR * BS];
     const int b = blockIdx.x, r0 = b * BS, m = min(BS, n - r0);
-    for (int e = threadIdx.x; e < BS * NR; e += blockDim.x) xs[e] = (e / NR) < m ? X[(size_t)r0 * NR + e] : 0.0;
+    for (int e = threadIdx.x; e < BS * NR; e += blockDim.x) {
+        const int j = e / NR, q = e % NR;
+        xs[q * BS + j] = j < m ? X[(size_t)r0 * NR + e] : 0.0;
+    }
     __syncthreads();
     const float2 *Zi = reinterpret_cast<const float2 *>(inv) + (size_t)b * BS * BS;
-    const int sp = threadIdx.x & 3, o = threadIdx.x >> 2;        // column split; (row, port) output
-    const int i = blockIdx.y * R + o / P, p = o % P;
-    const bool live = o < R * P && i < m;
-    double yr = 0, yi = 0;
+    const int sp = threadIdx.x & 15, i = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const bool live = i < m;
+    double yr[P], yi[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) yr[p] = yi[p] = 0;
     if (live) {
         const float2 *zrow = Zi + (size_t)i * BS;
-#pragma unroll 8
-        for (int j = sp; j < m; j += 4) {
+#pragma unroll 4
+        for (int j = sp; j < m; j += 16) {
             const float2 z = zrow[j];
             const double zr = z.x, zi = z.y;
-            const double xr = xs[j * NR + p], xi = xs[j * NR + P + p];
-            yr += zr * xr - zi * xi;
-            yi += zr * xi + zi * xr;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const double xr = xs[p * BS + j], xi = xs[(P + p) * BS + j];
+                yr[p] += zr * xr - zi * xi;
+                yi[p] += zr * xi + zi * xr;
+            }
         }
     }
-    yr += __shfl_xor(yr, 1); yi += __shfl_xor(yi, 1);
-    yr += __shfl_xor(yr, 2); yi += __shfl_xor(yi, 2);
-    if (live && sp == 0) {
-        if (A) {
-            yr += A[(size_t)(r0 + i) * NR + p];
-            yi += A[(size_t)(r0 + i) * NR + P + p];
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            yr[p] += __shfl_xor(yr[p], o);
+            yi[p] += __shfl_xor(yi[p], o);
         }
-        Y[(size_t)(r0 + i) * NR + p] = yr;
-        Y[(size_t)(r0 + i) * NR + P + p] = yi;
+    if (live && sp < P) {                             // lane sp writes right-hand side sp
+        double vr = yr[0], vi = yi[0];
+#pragma unroll
+        for (int p = 1; p < P; ++p)
+            if (sp == p) { vr = yr[p]; vi = yi[p]; }
+        if (A) {
+            vr += A[(size_t)(r0 + i) * NR + sp];
+            vi += A[(size_t)(r0 + i) * NR + P + sp];
+        }
+        Y[(size_t)(r0 + i) * NR + sp] = vr;
+        Y[(size_t)(r0 + i) * NR + P + sp] = vi;
     }
 }
 
@@ -2191,8 +2207,8 @@ void apply_precond(pq_loops *L, int kind, const double *X, double *Y, int P) {
 void apply_bj(pq_loops *L, const double *X, double *Y, int P, const double *add) {
     dispatch_nr(2 * P, [&](auto nr) {
         constexpr int N = decltype(nr)::value;
-        hipLaunchKernelGGL(bj_apply_kernel<N>, dim3(L->nblk, (BS + bj_rows(N) - 1) / bj_rows(N)), dim3(256), 0,
-                           L->stream, L->inv.as<float>(), L->nf, X, add, Y);
+        hipLaunchKernelGGL(bj_apply16_kernel<N>, dim3(L->nblk, BS / 16), dim3(256), 0, L->stream,
+                           L->inv.as<float>(), L->nf, X, add, Y);
     });
     PQ_HIP(hipGetLastError());
 }

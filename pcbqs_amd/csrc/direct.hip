@@ -28,6 +28,9 @@ namespace pq {
 namespace {
 
 constexpr int LB = LDL_BLOCK;
+// An entry of a one-launch sweep's solution not yet posted (a signalling NaN, which no arithmetic
+// yields; see the persistent sweeps below).
+constexpr unsigned long long LDL_EMPTY = 0x7FF4C0DE5EA7B10CULL;
 constexpr int TP = 36;                      // LDS row pitch (floats) of a 32-wide K chunk
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -519,15 +522,20 @@ __global__ __launch_bounds__(256) void ldl_forward_kernel(const float *__restric
     }
 }
 
-// z = D^-1 y (FP64 complex division by the FP32 pivots).
+// z = D^-1 y (FP64 complex division by the FP32 pivots); with `mark`, its entries (the same
+// shape) are marked empty for a one-launch backward sweep.
 __global__ void ldl_scale_kernel(const float *Fr, const float *Fi, long long ldf, int n, int P, const double *Ys,
-                                 double *Z, long long sF, long long sV) {
+                                 double *Z, long long sF, long long sV, double *mark) {
     const int NR = 2 * P;
     Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;
     Ys += blockIdx.z * sV; Z += blockIdx.z * sV;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * P) return;
     const int r = (int)(e / P), p = (int)(e % P);
+    if (mark) {
+        mark += blockIdx.z * sV;
+        mark[(size_t)r * NR + p] = mark[(size_t)r * NR + P + p] = __longlong_as_double((long long)LDL_EMPTY);
+    }
     const double dr = Fr[(size_t)r * ldf + r], di = Fi[(size_t)r * ldf + r];
     const double yr = Ys[(size_t)r * NR + p], yi = Ys[(size_t)r * NR + P + p];
     const double den = dr * dr + di * di;
@@ -622,6 +630,248 @@ __global__ __launch_bounds__(256) void ldl_backward_kernel(const float *__restri
         for (int p = 0; p < P; ++p) {
             Z[(size_t)r * NR + p] = zr[p] - ar[p];
             Z[(size_t)r * NR + P + p] = zi[p] - ai[p];
+        }
+    }
+}
+
+// ---- persistent sweeps: one launch per sweep ----
+//
+// The per-block sweeps above pay one dependent launch per 64-row block. Here one launch does a
+// whole sweep. Workgroup i owns row block i in the forward sweep (block nb - 1 - i in the backward
+// one, so that workgroups start in the order their blocks are solved). It stages its block's
+// stored inverse, then takes the earlier blocks' solutions as they appear and subtracts their
+// products with its rows in the order and grouping of the per-block kernels, so that both give the
+// same values. The solution entries are their own flags: the launch before marks them empty (a
+// signalling NaN, which no arithmetic yields), a block posts each entry with an agent-scope store
+// and a reader polls them with agent-scope loads until none is marked, so no fence or separate
+// flag sits on the chain. A workgroup waits only for blocks of lower workgroup index, which are
+// dispatched before it, so every wait ends; each wait is bounded all the same, and entries still
+// marked when it runs out are read as the NaN they hold, which the refinement reports as a
+// non-finite residual instead of hanging.
+constexpr int LDL_SPIN = 1 << 20;                    // polls, each ~1 us, before a wait gives up
+
+__global__ void ldl_mark_empty_kernel(double *V, size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) V[e] = __longlong_as_double((long long)LDL_EMPTY);
+}
+
+__device__ __forceinline__ void ldl_post(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until the first `rows` rows of the 64 x NR entries at V are posted, then stores them to
+// S in LDS (rows past `rows` as zero) and passes a barrier. Every thread polls its share (loads
+// from clamped addresses, none skipped, so they are all in flight together).
+template <int NR>
+__device__ __forceinline__ void ldl_take(const double *V, int rows, double (*S)[NR]) {
+    constexpr int CNT = (LB * NR + 255) / 256;
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(V);
+    unsigned long long v[CNT];
+    for (int i = 0;; ++i) {
+        bool all = true;
+#pragma unroll
+        for (int c = 0; c < CNT; ++c) {
+            const int e = threadIdx.x + 256 * c;
+            const bool in = e < LB * NR && e / NR < rows;
+            const unsigned long long x =
+                __hip_atomic_load(src + (in ? e : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v[c] = in ? x : 0ull;
+            all = all && v[c] != LDL_EMPTY;
+        }
+        if (__syncthreads_and(all) || i >= LDL_SPIN) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+        const int e = threadIdx.x + 256 * c;
+        if (e < LB * NR) S[e / NR][e % NR] = __longlong_as_double((long long)v[c]);
+    }
+    __syncthreads();
+}
+
+// Forward: row block rb's rows start from B; for each earlier block k whose per-block launch
+// reaches them (all, or those within the band), b_r -= L[r, block k] y_k (lane = row, wave = a
+// 16-column quarter, the quarters summed in wave order), then y_rb = X_rb b_rb posted to Ys.
+template <int P>
+__global__ __launch_bounds__(256) void ldl_fwd_persist_kernel(const float *__restrict__ Fr,
+                                                             const float *__restrict__ Fi, long long ldf,
+                                                             const float *__restrict__ Xr,
+                                                             const float *__restrict__ Xi, int n, int band,
+                                                             const double *B, double *Ys, long long sF,
+                                                             long long sX, long long sV) {
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    B += blockIdx.z * sV; Ys += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ float xs[2][LB][LB + 1];
+    __shared__ double ys[LB][NR], part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rb = blockIdx.x, R0 = rb * LB, bk = min(LB, n - R0), r = R0 + lane;
+    double yr[P], yi[P];
+    if (wv == 0 && r < n)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { yr[p] = B[(size_t)r * NR + p]; yi[p] = B[(size_t)r * NR + P + p]; }
+    const float *xr = Xr + (size_t)rb * LB * LB, *xi = Xi + (size_t)rb * LB * LB;
+    for (int e = tid; e < LB * LB; e += 256) {
+        xs[0][e >> 6][e & 63] = xr[e];
+        xs[1][e >> 6][e & 63] = xi[e];
+    }
+    for (int k = 0; k < rb; ++k) {
+        const int k0 = k * LB;                        // a full block: it has rows after it
+        const int rows = band > 0 ? min(n - k0 - LB, band) : n - k0 - LB;
+        if (rb - k - 1 >= max(1, (rows + LB - 1) / LB)) continue;
+        float lre[16], lim[16];                       // loaded before the wait: they do not depend on y_k
+        if (r < n) {
+            const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)r * ldf + k0 + 16 * wv);
+            const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)r * ldf + k0 + 16 * wv);
+#pragma unroll
+            for (int t4 = 0; t4 < 4; ++t4) {
+                const float4 a = lr4[t4], b = li4[t4];
+                lre[4 * t4] = a.x; lre[4 * t4 + 1] = a.y; lre[4 * t4 + 2] = a.z; lre[4 * t4 + 3] = a.w;
+                lim[4 * t4] = b.x; lim[4 * t4 + 1] = b.y; lim[4 * t4 + 2] = b.z; lim[4 * t4 + 3] = b.w;
+            }
+        }
+        ldl_take<NR>(Ys + (size_t)k0 * NR, LB, ys);
+        double ar[P], ai[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+        if (r < n) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int t = 16 * wv + u;
+                const double lr = lre[u], li = lim[u];
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    ar[p] += lr * ys[t][p] - li * ys[t][P + p];
+                    ai[p] += lr * ys[t][P + p] + li * ys[t][p];
+                }
+            }
+        }
+        if (wv > 0)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+        __syncthreads();
+        if (wv == 0 && r < n) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+#pragma unroll
+            for (int p = 0; p < P; ++p) { yr[p] = yr[p] - ar[p]; yi[p] = yi[p] - ai[p]; }
+        }
+    }
+    if (wv == 0)                                      // b_rb, zero past the last row
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            ys[lane][p] = lane < bk ? yr[p] : 0.0;
+            ys[lane][P + p] = lane < bk ? yi[p] : 0.0;
+        }
+    __syncthreads();
+    for (int e = tid; e < LB * P; e += 256) {
+        const int i = e / P, p = e % P;
+        double sr = 0, si = 0;
+#pragma unroll 16
+        for (int t = 0; t < LB; ++t) {
+            const double lr = xs[0][i][t], li = xs[1][i][t];
+            sr += lr * ys[t][p] - li * ys[t][P + p];
+            si += lr * ys[t][P + p] + li * ys[t][p];
+        }
+        if (i < bk) {
+            ldl_post(Ys + (size_t)(R0 + i) * NR + p, sr);
+            ldl_post(Ys + (size_t)(R0 + i) * NR + P + p, si);
+        }
+    }
+}
+
+// Backward: column block cb = nb - 1 - blockIdx.x starts from z; for each later block j whose
+// per-block launch reaches its columns (all, or those within the band), z_c -= L[block j, c]^T x_j
+// (lane = column, wave = a 16-row quarter of block j, summed in wave order), then
+// x_cb = X_cb^T z_cb posted to Xo.
+template <int P>
+__global__ __launch_bounds__(256) void ldl_bwd_persist_kernel(const float *__restrict__ Fr,
+                                                             const float *__restrict__ Fi, long long ldf,
+                                                             const float *__restrict__ Xr,
+                                                             const float *__restrict__ Xi, int n, int band,
+                                                             const double *Z, double *Xo, long long sF,
+                                                             long long sX, long long sV) {
+    const int nb = gridDim.x;
+    Fr += blockIdx.z * sF; Fi += blockIdx.z * sF;     // system blockIdx.z of a batch
+    Xr += blockIdx.z * sX; Xi += blockIdx.z * sX;
+    Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
+    constexpr int NR = 2 * P;
+    __shared__ float xsm[2][LB][LB + 1];
+    __shared__ double xs[LB][NR], part[3][LB][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int cb = nb - 1 - (int)blockIdx.x, C0 = cb * LB, bc = min(LB, n - C0), c = C0 + lane;
+    double zr[P], zi[P];
+    if (wv == 0 && c < n)
+#pragma unroll
+        for (int p = 0; p < P; ++p) { zr[p] = Z[(size_t)c * NR + p]; zi[p] = Z[(size_t)c * NR + P + p]; }
+    const float *xr = Xr + (size_t)cb * LB * LB, *xi = Xi + (size_t)cb * LB * LB;
+    for (int e = tid; e < LB * LB; e += 256) {
+        xsm[0][e >> 6][e & 63] = xr[e];
+        xsm[1][e >> 6][e & 63] = xi[e];
+    }
+    for (int j = nb - 1; j > cb; --j) {
+        const int k0 = j * LB, bk = min(LB, n - k0);
+        const int r0 = band > 0 ? max(0, (k0 - band) / LB * LB) : 0;
+        if (C0 < r0) continue;
+        const int t0 = 16 * wv;
+        float lre[16], lim[16];                       // c < k0: every column of this block is below block j
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {                // rows past bk: loaded from the last row, then zeroed
+            const size_t at = (size_t)(k0 + min(t0 + u, bk - 1)) * ldf + c;
+            const float a = Fr[at], b = Fi[at];
+            lre[u] = t0 + u < bk ? a : 0.f;
+            lim[u] = t0 + u < bk ? b : 0.f;
+        }
+        ldl_take<NR>(Xo + (size_t)k0 * NR, bk, xs);
+        double ar[P], ai[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int t = t0 + u;
+            const double lr = lre[u], li = lim[u];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                ar[p] += lr * xs[t][p] - li * xs[t][P + p];
+                ai[p] += lr * xs[t][P + p] + li * xs[t][p];
+            }
+        }
+        if (wv > 0)
+#pragma unroll
+            for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+        __syncthreads();
+        if (wv == 0 && c < n) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+#pragma unroll
+            for (int p = 0; p < P; ++p) { zr[p] = zr[p] - ar[p]; zi[p] = zi[p] - ai[p]; }
+        }
+    }
+    if (wv == 0)                                      // z_cb, zero past the last row
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xs[lane][p] = lane < bc ? zr[p] : 0.0;
+            xs[lane][P + p] = lane < bc ? zi[p] : 0.0;
+        }
+    __syncthreads();
+    for (int e = tid; e < LB * P; e += 256) {
+        const int cc = e % LB, p = e / LB;
+        double sr = 0, si = 0;
+#pragma unroll 16
+        for (int t = 0; t < LB; ++t) {
+            const double lr = xsm[0][t][cc], li = xsm[1][t][cc];
+            sr += lr * xs[t][p] - li * xs[t][P + p];
+            si += lr * xs[t][P + p] + li * xs[t][p];
+        }
+        if (cc < bc) {
+            ldl_post(Xo + (size_t)(C0 + cc) * NR + p, sr);
+            ldl_post(Xo + (size_t)(C0 + cc) * NR + P + p, si);
         }
     }
 }
@@ -1142,12 +1392,28 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
     if (n <= 0) return;
     const unsigned nz = (unsigned)std::max(1, f.batch);
     // Super-block sweeps for systems up to 4,096 unknowns (the many small systems of a multi-board
-    // batch: a sequential workgroup per system, 2 launches per 512 rows), per-block sweeps above (a
-    // large system's per-block launches spread each block's columns over many workgroups, which the
-    // one sequential workgroup cannot match). PCBQS_AMD_LDL_BLOCK_SWEEPS=1: per-block sweeps always.
-    static const bool block_env = std::getenv("PCBQS_AMD_LDL_BLOCK_SWEEPS") != nullptr;
+    // batch: a sequential workgroup per system, 2 launches per 512 rows); above, or within a band,
+    // one launch per sweep (a large system's blocks spread over many workgroups, which the one
+    // sequential workgroup cannot match). PCBQS_AMD_LDL_BLOCK_SWEEPS set (read on every call):
+    // one launch per 64-row block instead, with the same values.
+    const bool block_env = std::getenv("PCBQS_AMD_LDL_BLOCK_SWEEPS") != nullptr;
     const bool block_sweeps = block_env || n > 4096 || f.band > 0;
     const long long sV = (long long)n * 2 * P;           // the systems' vectors, one after another
+    if (block_sweeps && !block_env) {
+        const size_t count = (size_t)n * 2 * P * nz;
+        hipLaunchKernelGGL(ldl_mark_empty_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, Ys, count);
+        dispatch_ports(P, [&](auto pc) {
+            constexpr int PP = decltype(pc)::value;
+            hipLaunchKernelGGL(ldl_fwd_persist_kernel<PP>, dim3(nb, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, f.Lir,
+                               f.Lii, n, f.band, B, Ys, f.sF, f.sX, sV);
+            hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256), 1, nz), dim3(256), 0,
+                               st, f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV, X);
+            hipLaunchKernelGGL(ldl_bwd_persist_kernel<PP>, dim3(nb, 1, nz), dim3(256), 0, st, f.Fr, f.Fi, f.ldf, f.Lir,
+                               f.Lii, n, f.band, Y, X, f.sF, f.sX, sV);
+        });
+        PQ_HIP(hipGetLastError());
+        return;
+    }
     const size_t bytes = (size_t)n * 2 * P * 8 * nz;
     PQ_HIP(hipMemcpyAsync(Y, B, bytes, hipMemcpyDeviceToDevice, st));
     dispatch_ports(P, [&](auto pc) {
@@ -1172,7 +1438,7 @@ void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, doubl
             }
         }
         hipLaunchKernelGGL(ldl_scale_kernel, dim3((unsigned)(((size_t)n * PP + 255) / 256), 1, nz), dim3(256), 0, st,
-                           f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV);
+                           f.Fr, f.Fi, f.ldf, n, PP, Ys, Y, f.sF, sV, nullptr);
         if (block_sweeps) {
             for (int kb = nb - 1; kb >= 0; --kb) {
                 const int k0 = kb * LB, bk = std::min(LB, n - k0);

@@ -47,8 +47,9 @@ void ldl_form(hipStream_t st, const LdlFactor &f, const float *Ll, long long ldl
 void ldl_factor(hipStream_t st, const LdlFactor &f, int *bad);
 
 // X = Z^-1 B for P complex columns (n, 2P) FP64 (real parts first), FP32 factor, FP64 arithmetic.
-// Y, Ys: (n, 2P) FP64 work arrays. P is 1..8 or 16. With f.batch > 1, B, Y, Ys and X hold the
-// batch's systems one after another (n x 2P each) and all are solved in the same launches.
+// Y, Ys: (n, 2P) FP64 work arrays; B, Y, Ys and X do not overlap. P is 1..8 or 16. With
+// f.batch > 1, B, Y, Ys and X hold the batch's systems one after another (n x 2P each) and all are
+// solved in the same launches.
 void ldl_solve(hipStream_t st, const LdlFactor &f, int P, const double *B, double *Y, double *Ys, double *X);
 
 }  // namespace pq

@@ -10,7 +10,8 @@ as L D L^T (64-wide panels, MFMA trailing updates) and refined in FP64 to a rela
 - the copper fixture at DC (Z_ff = Rl alone) and 100 / 200 kHz;
 - a system whose FP32 factor cannot reach 1e-12 (condition number about 5e7) goes through the FP64
   fallback factor and matches numpy's FP64 solve, alone and inside a batch whose other frequencies
-  keep their refined solutions; an exactly singular system raises instead of returning garbage."""
+  keep their refined solutions; an exactly singular system raises instead of returning garbage;
+- above 4,096 unknowns, the one-launch sweeps (per-block flags) against the per-block launches."""
 import ctypes
 import os
 
@@ -172,6 +173,48 @@ def test_fp64_fallback_factor_solves_what_fp32_cannot():
         # a well-conditioned frequency of the same handle needs no fallback
         Zp, X, its, res = direct(handle, 0.0, n, P, cols)
         assert fallbacks(handle) == 0 and res <= 1e-12
+    finally:
+        lib().pq_loops_destroy(handle)
+
+
+def test_one_launch_sweeps_equal_the_per_block_sweeps(monkeypatch):
+    """Above 4,096 unknowns each triangular sweep is one launch whose workgroups wait on per-block
+    flags; it gives the same values as one launch per 64-row block (PCBQS_AMD_LDL_BLOCK_SWEEPS set),
+    alone (the factor kept, so only the sweeps differ) and batched over frequencies, and the refined
+    solution matches numpy's FP64 solve. 4,500 unknowns: 70 full blocks and a partial one."""
+    n = 4500
+    print(f'one-launch sweeps n={n}', flush=True)
+    handle, Ll, R = spd_loops(n, 16, 21)
+    try:
+        w = 2 * np.pi * 1e6
+        for P in (1, 8, 16):
+            cols = np.arange(n, n + P)
+            monkeypatch.delenv('PCBQS_AMD_LDL_BLOCK_SWEEPS', raising=False)
+            a = direct(handle, w, n, P, cols)
+            monkeypatch.setenv('PCBQS_AMD_LDL_BLOCK_SWEEPS', '1')
+            b = direct(handle, w, n, P, cols)
+            assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0]), P
+            assert a[2] == b[2] and a[3] <= 1e-12, (P, a[2], b[2], a[3])
+            if P == 8:
+                Ze, Xe = expected(Ll, R, w, n, cols)
+                assert np.abs(a[1] - Xe).max() <= 1e-9 * np.abs(Xe).max()
+                assert np.abs(a[0] - Ze).max() <= 1e-10 * np.abs(Ze).max()
+        P = 4
+        cols = np.ascontiguousarray(np.arange(n, n + P), np.int32)
+        omegas = np.array([2 * np.pi * 1e3, 2 * np.pi * 1e7])
+        out = []
+        for env in (None, '1'):
+            if env is None:
+                monkeypatch.delenv('PCBQS_AMD_LDL_BLOCK_SWEEPS', raising=False)
+            else:
+                monkeypatch.setenv('PCBQS_AMD_LDL_BLOCK_SWEEPS', env)
+            zp = np.zeros((2, P, P, 2))
+            st, rs = np.zeros(2, np.int64), np.zeros(2)
+            call('pq_loops_solve_direct_batch', handle, 2, ptr(omegas), P, ptr(cols), 1e-12, 12, ptr(zp), ptr(st),
+                 ptr(rs))
+            assert (rs <= 1e-12).all(), rs
+            out.append(zp)
+        assert np.array_equal(out[0], out[1])
     finally:
         lib().pq_loops_destroy(handle)
 

@@ -660,34 +660,38 @@ __device__ __forceinline__ void ldl_post(double *p, double v) {
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Waits until the first `rows` rows of the 64 x NR entries at V are posted, then stores them to
-// S in LDS (rows past `rows` as zero) and passes a barrier. Every thread polls its share (loads
-// from clamped addresses, none skipped, so they are all in flight together).
+// Waits until the 16 rows of the 64 x NR entries at V that the calling wave uses are posted
+// (rows 16 wv .. 16 wv + 15; rows from `rows` on read as zero); lane l keeps entries l + 64 c of
+// them. Each wave polls on its own (loads from clamped addresses, none skipped, so they are all in
+// flight together), so no workgroup barrier sits in the wait.
 template <int NR>
-__device__ __forceinline__ void ldl_take(const double *V, int rows, double (*S)[NR]) {
-    constexpr int CNT = (LB * NR + 255) / 256;
+__device__ __forceinline__ void ldl_take_rows(const double *V, int rows, unsigned long long (&v)[(16 * NR + 63) / 64]) {
+    constexpr int CW = (16 * NR + 63) / 64;
+    const int lane = threadIdx.x & 63, r0 = 16 * (threadIdx.x >> 6);
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(V);
-    unsigned long long v[CNT];
     for (int i = 0;; ++i) {
         bool all = true;
 #pragma unroll
-        for (int c = 0; c < CNT; ++c) {
-            const int e = threadIdx.x + 256 * c;
-            const bool in = e < LB * NR && e / NR < rows;
+        for (int c = 0; c < CW; ++c) {
+            const int e = lane + 64 * c;
+            const bool in = e < 16 * NR && r0 + e / NR < rows;
             const unsigned long long x =
-                __hip_atomic_load(src + (in ? e : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_load(src + (in ? (size_t)r0 * NR + e : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             v[c] = in ? x : 0ull;
             all = all && v[c] != LDL_EMPTY;
         }
-        if (__syncthreads_and(all) || i >= LDL_SPIN) break;
+        if (__all(all) || i >= LDL_SPIN) break;
         __builtin_amdgcn_s_sleep(1);
     }
-#pragma unroll
-    for (int c = 0; c < CNT; ++c) {
-        const int e = threadIdx.x + 256 * c;
-        if (e < LB * NR) S[e / NR][e % NR] = __longlong_as_double((long long)v[c]);
-    }
-    __syncthreads();
+}
+
+// Entry e (local row * NR + column) of the wave's 16 rows, read from the lane that keeps it.
+template <int NR>
+__device__ __forceinline__ double ldl_entry(const unsigned long long (&v)[(16 * NR + 63) / 64], int e) {
+    const unsigned long long x = v[e / 64];
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, e % 64);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), e % 64);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Forward: row block rb's rows start from B; for each earlier block k whose per-block launch
@@ -705,9 +709,10 @@ __global__ __launch_bounds__(256) void ldl_fwd_persist_kernel(const float *__res
     B += blockIdx.z * sV; Ys += blockIdx.z * sV;
     constexpr int NR = 2 * P;
     __shared__ float xs[2][LB][LB + 1];
-    __shared__ double ys[LB][NR], part[3][LB][NR];
+    __shared__ double ys[LB][NR], part[2][3][LB][NR];  // partials by parity of the block taken
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rb = blockIdx.x, R0 = rb * LB, bk = min(LB, n - R0), r = R0 + lane;
+    int taken = 0;
     double yr[P], yi[P];
     if (wv == 0 && r < n)
 #pragma unroll
@@ -721,7 +726,7 @@ __global__ __launch_bounds__(256) void ldl_fwd_persist_kernel(const float *__res
         const int k0 = k * LB;                        // a full block: it has rows after it
         const int rows = band > 0 ? min(n - k0 - LB, band) : n - k0 - LB;
         if (rb - k - 1 >= max(1, (rows + LB - 1) / LB)) continue;
-        float lre[16], lim[16];                       // loaded before the wait: they do not depend on y_k
+        float lre[16] = {}, lim[16] = {};             // loaded before the wait: they do not depend on y_k
         if (r < n) {
             const float4 *lr4 = reinterpret_cast<const float4 *>(Fr + (size_t)r * ldf + k0 + 16 * wv);
             const float4 *li4 = reinterpret_cast<const float4 *>(Fi + (size_t)r * ldf + k0 + 16 * wv);
@@ -732,34 +737,35 @@ __global__ __launch_bounds__(256) void ldl_fwd_persist_kernel(const float *__res
                 lim[4 * t4] = b.x; lim[4 * t4 + 1] = b.y; lim[4 * t4 + 2] = b.z; lim[4 * t4 + 3] = b.w;
             }
         }
-        ldl_take<NR>(Ys + (size_t)k0 * NR, LB, ys);
+        unsigned long long v[(16 * NR + 63) / 64];
+        ldl_take_rows<NR>(Ys + (size_t)k0 * NR, LB, v);
         double ar[P], ai[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
-        if (r < n) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int t = 16 * wv + u;
-                const double lr = lre[u], li = lim[u];
+        for (int u = 0; u < 16; ++u) {                // y_k[16 wv + u], rows past n with zero L
+            const double lr = lre[u], li = lim[u];
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    ar[p] += lr * ys[t][p] - li * ys[t][P + p];
-                    ai[p] += lr * ys[t][P + p] + li * ys[t][p];
-                }
+            for (int p = 0; p < P; ++p) {
+                const double yre = ldl_entry<NR>(v, u * NR + p), yim = ldl_entry<NR>(v, u * NR + P + p);
+                ar[p] += lr * yre - li * yim;
+                ai[p] += lr * yim + li * yre;
             }
         }
+        double(*pt)[LB][NR] = part[taken & 1];
         if (wv > 0)
 #pragma unroll
-            for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+            for (int p = 0; p < P; ++p) { pt[wv - 1][lane][p] = ar[p]; pt[wv - 1][lane][P + p] = ai[p]; }
         __syncthreads();
         if (wv == 0 && r < n) {
 #pragma unroll
             for (int q = 0; q < 3; ++q)
 #pragma unroll
-                for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+                for (int p = 0; p < P; ++p) { ar[p] += pt[q][lane][p]; ai[p] += pt[q][lane][P + p]; }
 #pragma unroll
             for (int p = 0; p < P; ++p) { yr[p] = yr[p] - ar[p]; yi[p] = yi[p] - ai[p]; }
         }
+        ++taken;
     }
     if (wv == 0)                                      // b_rb, zero past the last row
 #pragma unroll
@@ -801,9 +807,10 @@ __global__ __launch_bounds__(256) void ldl_bwd_persist_kernel(const float *__res
     Z += blockIdx.z * sV; Xo += blockIdx.z * sV;
     constexpr int NR = 2 * P;
     __shared__ float xsm[2][LB][LB + 1];
-    __shared__ double xs[LB][NR], part[3][LB][NR];
+    __shared__ double xs[LB][NR], part[2][3][LB][NR];  // partials by parity of the block taken
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int cb = nb - 1 - (int)blockIdx.x, C0 = cb * LB, bc = min(LB, n - C0), c = C0 + lane;
+    int taken = 0;
     double zr[P], zi[P];
     if (wv == 0 && c < n)
 #pragma unroll
@@ -826,32 +833,35 @@ __global__ __launch_bounds__(256) void ldl_bwd_persist_kernel(const float *__res
             lre[u] = t0 + u < bk ? a : 0.f;
             lim[u] = t0 + u < bk ? b : 0.f;
         }
-        ldl_take<NR>(Xo + (size_t)k0 * NR, bk, xs);
+        unsigned long long v[(16 * NR + 63) / 64];
+        ldl_take_rows<NR>(Xo + (size_t)k0 * NR, bk, v);
         double ar[P], ai[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) ar[p] = ai[p] = 0;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int t = t0 + u;
+        for (int u = 0; u < 16; ++u) {                // x_j[t0 + u], zero past bk
             const double lr = lre[u], li = lim[u];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                ar[p] += lr * xs[t][p] - li * xs[t][P + p];
-                ai[p] += lr * xs[t][P + p] + li * xs[t][p];
+                const double xre = ldl_entry<NR>(v, u * NR + p), xim = ldl_entry<NR>(v, u * NR + P + p);
+                ar[p] += lr * xre - li * xim;
+                ai[p] += lr * xim + li * xre;
             }
         }
+        double(*pt)[LB][NR] = part[taken & 1];
         if (wv > 0)
 #pragma unroll
-            for (int p = 0; p < P; ++p) { part[wv - 1][lane][p] = ar[p]; part[wv - 1][lane][P + p] = ai[p]; }
+            for (int p = 0; p < P; ++p) { pt[wv - 1][lane][p] = ar[p]; pt[wv - 1][lane][P + p] = ai[p]; }
         __syncthreads();
         if (wv == 0 && c < n) {
 #pragma unroll
             for (int q = 0; q < 3; ++q)
 #pragma unroll
-                for (int p = 0; p < P; ++p) { ar[p] += part[q][lane][p]; ai[p] += part[q][lane][P + p]; }
+                for (int p = 0; p < P; ++p) { ar[p] += pt[q][lane][p]; ai[p] += pt[q][lane][P + p]; }
 #pragma unroll
             for (int p = 0; p < P; ++p) { zr[p] = zr[p] - ar[p]; zi[p] = zi[p] - ai[p]; }
         }
+        ++taken;
     }
     if (wv == 0)                                      // z_cb, zero past the last row
 #pragma unroll

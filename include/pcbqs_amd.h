@@ -190,7 +190,9 @@ typedef struct pq_csym pq_csym;     /* a factored complex-symmetric matrix      
  * b + 128 < n: the Laplacian in a bandwidth-reducing order), the factorisation
  * and the triangular sweeps work inside the band only (O(n b^2) work; the
  * storage stays n x n). *factor_ms: device time of forming and factoring;
- * *bytes: the factor's storage. A zero pivot returns PQ_EINVAL. Replaces the
+ * *bytes: the factor's storage. A zero pivot returns PQ_EINVAL; a factor
+ * larger than the device's free memory returns PQ_ENOMEM with the bytes it
+ * needs (the dense storage limits n to about 140k on 288 GB). Replaces the
  * per-frequency splu of the graph preconditioner's Laplacian
  * (src/pcbqs/mqs_precond.py:50-51). */
 int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const int32_t *cols, const double *vals,

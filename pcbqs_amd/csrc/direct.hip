@@ -1675,6 +1675,15 @@ int pq_csym_create(int device, int n, int64_t nnz, const int32_t *rows, const in
             PQ_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
             const int nblk = (n + LB - 1) / LB;
             const long long ldf = (long long)nblk * LB;
+            // the factor is stored dense (n x n complex FP32, even within a band): refuse by name
+            // what the device cannot hold instead of failing inside an allocation
+            size_t free_bytes = 0, total_bytes = 0;
+            PQ_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
+            const size_t need = (size_t)2 * n * ldf * 4 + (size_t)2 * nblk * LB * LB * 4 + (size_t)16 * n * 2 * LB;
+            if (need > free_bytes)
+                throw Error(PQ_ENOMEM, "pq_csym_create: the dense factor of " + std::to_string(n) + " unknowns needs " +
+                                           std::to_string(need >> 20) + " MiB, " + std::to_string(free_bytes >> 20) +
+                                           " MiB free on the device");
             h->F.reserve((size_t)2 * n * ldf * 4);
             h->Xi.reserve((size_t)2 * nblk * LB * LB * 4);
             const size_t wsz = (size_t)n * 2 * LB;
